@@ -1,0 +1,250 @@
+/*
+ * pbm.h — C-ABI of the MI355X (gfx950) parallel page/buffer manager.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns 0 (PBM_OK) or a negative error code; `pbm_last_error()` gives the
+ * message of the last failure on the calling thread.  Device pointers are
+ * plain `void*`/typed pointers into HIP device memory; `stream` is a
+ * `hipStream_t` passed as `void*` (NULL = the legacy default stream).
+ *
+ * The reference keeps one allocator per translation unit in file-static
+ * device globals set by cudaMemcpyToSymbol (Paging.cuh:23-28,
+ * RandomWalkBasic_BM.cuh:28-32).  Here a handle owns all device state, so
+ * several pools can coexist and `pbm_destroy` frees everything.
+ *
+ * Bitmap convention (all strategies): one bit per page in 64-bit words,
+ * bit = 1 means used, page id = word * 64 + bit.  Padding bits past
+ * n_pages in the last word are kept set so they are never handed out.
+ */
+#ifndef PBM_H
+#define PBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define PBM_OK               0
+#define PBM_ERR_INVALID     -1   /* bad argument or handle */
+#define PBM_ERR_HIP         -2   /* a HIP runtime call failed */
+#define PBM_ERR_OOM         -3   /* device allocation failed */
+#define PBM_ERR_UNSUPPORTED -4   /* operation not defined for the strategy */
+#define PBM_ERR_RANGE       -5   /* size beyond a documented limit */
+
+/* ---- allocation policies ---------------------------------------------
+ * The reference selects one policy per translation unit by swapping the
+ * included header (unitTests/unitTest1.cu:4-7); here it is a field of the
+ * pool configuration. */
+typedef enum pbm_strategy {
+    PBM_RW_BM        = 0, /* RandomWalkBasic_BM.cuh:60-102   plain bitmap random walk      */
+    PBM_COLLAB_RW_BM = 1, /* CollabRW_BM.cuh:114-160         wave-collaborative window walk */
+    PBM_CRW_BM       = 2, /* CRW_BM.cuh:75-156               clustered walk with hints      */
+    PBM_RW_MALLOC    = 3, /* RWMalloc.cuh:113-334            collaborative walk + getNPage  */
+    PBM_LINKED_LIST  = 4, /* parallelPage.cu:36-46,300-466   FIFO free list                 */
+    PBM_SINGLE_CLOCK = 5, /* parallelPage.cu:470-535         one global clock arm           */
+    PBM_DIST_CLOCK   = 6  /* parallelPage.cu:537-618         per-thread clock arms          */
+} pbm_strategy;
+
+/* Order in which a collaborative step hands the free bits of its window to
+ * the requesting lanes. */
+typedef enum pbm_order {
+    PBM_ORDER_PACKED    = 0, /* lane-major: all free bits of the first word, then the next */
+    PBM_ORDER_REFERENCE = 1  /* round-robin over words, as CollabRW_BM.cuh:140-153 ranks them */
+} pbm_order;
+
+/* How a batch entry point schedules its requests. */
+typedef enum pbm_mode {
+    PBM_MODE_PLAIN   = 0, /* one thread per request calling the strategy's get/free       */
+    PBM_MODE_BATCHED = 1, /* persistent grid, wave-collective windows, phase reservation  */
+    PBM_MODE_SERIAL  = 2  /* one thread (one wave for collective policies) runs the
+                             requests in index order: the schedule of the serial oracle  */
+} pbm_mode;
+
+/* ---- configuration ----------------------------------------------------
+ * Replaces the (nGB, pageSize) arguments of initMemoryManagement
+ * (RandomWalkBasic_BM.cuh:42) and the compile-time TOTAL_N_PAGES/PAGE_SIZE
+ * of the legacy header (parallelPage.cuh:3-4). */
+typedef struct pbm_config {
+    uint32_t n_pages;      /* pages in the pool, 1 .. 2^30                                */
+    uint32_t page_size;    /* bytes per page (any value >= 8)                             */
+    int32_t  strategy;     /* pbm_strategy                                                */
+    int32_t  backing;      /* 1: allocate n_pages*page_size bytes of page memory;
+                              0: metadata-only pool (page_address returns 0)              */
+    uint64_t seed;         /* seeds request walks, clock arms and hints                   */
+    uint32_t walk_budget;  /* random jumps before a walk falls back to a linear sweep;
+                              0 = default (1<<20)                                         */
+    uint32_t max_threads;  /* per-thread state slots (CRW hints, DistClock arms);
+                              0 = default (1<<20)                                         */
+    int32_t  order;        /* pbm_order for collaborative strategies                      */
+    int32_t  device;       /* HIP device ordinal                                          */
+} pbm_config;
+
+/* Device-side event counters of a pool (read with pbm_read_counters). */
+typedef struct pbm_counters {
+    unsigned long long allocs;      /* successful page grants                         */
+    unsigned long long frees;       /* pages returned                                 */
+    unsigned long long failures;    /* requests that got no page                      */
+    unsigned long long steps;       /* walk steps summed over requests                */
+    unsigned long long probes;      /* bitmap words read by walks                     */
+    unsigned long long atomics;     /* read-modify-write atomics on the page map      */
+    unsigned long long fallbacks;   /* walks that exhausted the budget and swept      */
+    unsigned long long errors;      /* double frees, ids out of range, ring overruns  */
+} pbm_counters;
+
+/* Plain-old-data view of a pool that device code receives by value.
+ * Obtained with pbm_get_ctx; valid until pbm_destroy. */
+typedef struct pbm_ctx {
+    unsigned long long* bitmap;      /* n_words page-map words                          */
+    long long*          free_count;  /* reservation counter: pages not yet granted      */
+    pbm_counters*       counters;    /* device counters                                 */
+    char*               pool;        /* page memory base (NULL without backing)         */
+    int32_t*            hint;        /* CRW: last granted page per slot, -1 = none      */
+    unsigned*           arms;        /* DistClock: per-slot arm                         */
+    unsigned long long* clock_arm;   /* SingleClock: global arm (64-bit, never wraps)   */
+    unsigned long long* ring;        /* LinkedList: n_pages slots (lap<<32 | page id)   */
+    unsigned long long* ring_head;   /* LinkedList: next ticket to dequeue              */
+    unsigned long long* ring_tail;   /* LinkedList: next ticket to enqueue              */
+    long long*          phase;       /* batched calls: [1] wave completion counter      */
+    uint32_t n_pages;
+    uint32_t n_words;
+    uint32_t page_size;
+    uint32_t walk_budget;
+    uint32_t max_threads;
+    int32_t  order;
+    uint64_t seed;
+} pbm_ctx;
+
+/* Launch shape of a batch call. */
+typedef struct pbm_launch {
+    int32_t  mode;          /* pbm_mode                                                  */
+    uint32_t block;         /* threads per workgroup, multiple of 64 (0 = 256)           */
+    uint32_t waves_per_cu;  /* batched: persistent waves per CU (0 = 8)                  */
+    uint32_t walk_budget;   /* 0 = the pool's walk_budget                                */
+    uint32_t cluster_words; /* batched: words per window, 64 or 128 or 256 (0 = 64)      */
+    uint32_t flags;         /* PBM_LAUNCH_* bits                                         */
+} pbm_launch;
+
+#define PBM_LAUNCH_NO_COUNTERS 1u  /* skip the device counter updates                  */
+#define PBM_LAUNCH_ONE_WAVE    2u  /* batched mode on a single wave: the schedule of the
+                                      serial oracle (orc_batched_serial)                  */
+
+typedef struct pbm_handle pbm_handle;
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* Creates a pool: page memory (optional), page map cleared to all-free,
+ * reservation counter = n_pages, strategy state initialised.
+ * Replaces initMemoryManagement (RandomWalkBasic_BM.cuh:42-55,
+ * CollabRW_BM.cuh:43-56, CRW_BM.cuh:46-70, RWMalloc.cuh:42-55),
+ * initPages (Paging.cuh:59-85) and the legacy initPages<Strategy>()
+ * family (parallelPage.cuh:10,20,26,33,40,47,54,62,70). */
+int pbm_create(const pbm_config* cfg, pbm_handle** out);
+
+/* Frees every device allocation of the pool.  The reference has no
+ * counterpart (its pools live until process exit). */
+int pbm_destroy(pbm_handle* h);
+
+/* Marks every page free and restores the strategy's initial state
+ * (hints cleared, arms re-seeded, list refilled 0..n-1).
+ * Replaces resetMemoryManager (RandomWalkBasic_BM.cuh:104-106) and
+ * resetBuffer<Strategy>() (parallelPage.cuh:14,30,37,44,51,59,67,75). */
+int pbm_reset(pbm_handle* h, void* stream);
+
+/* Deterministic pre-occupancy: page i is marked used iff
+ * mix(seed, i) < fill * 2^64 (identical on host and device).  Replaces the
+ * self-filling prefillBuffer (RandomWalkBasic_BM.cuh:148-164), whose
+ * pattern depends on the schedule.  The reservation counter is set to
+ * the number of free pages. */
+int pbm_prefill_bernoulli(pbm_handle* h, double fill, uint64_t seed, void* stream);
+
+/* Reference-style pre-fill: (int)((1-freePercentage)*n_pages) requests
+ * are served by the pool's own allocator and never freed, as
+ * prefillBuffer does (RandomWalkBasic_BM.cuh:148-164). */
+int pbm_prefill_requests(pbm_handle* h, float free_percentage, void* stream);
+
+/* Number of used pages (popcount of the page map, padding excluded).
+ * Replaces popc_kernel + cub::DeviceReduce::Sum (RandomWalkBasic_BM.cuh:109-145). */
+int pbm_count_used(pbm_handle* h, uint64_t* used, void* stream);
+
+/* Free pages / n_pages.  Replaces getFreePagePercentage
+ * (RandomWalkBasic_BM.cuh:130-145). */
+int pbm_free_page_percentage(pbm_handle* h, float* out, void* stream);
+
+/* Copies the page map to / from host memory (`words` must equal n_words).
+ * restore also resets the reservation counter to the free pages of the
+ * restored map.  No reference counterpart (checkpoint/resume). */
+int pbm_snapshot_bitmap(pbm_handle* h, uint64_t* host, size_t words, void* stream);
+int pbm_restore_bitmap(pbm_handle* h, const uint64_t* host, size_t words, void* stream);
+
+/* Reads the reservation counter (pages not yet granted). */
+int pbm_read_free_count(pbm_handle* h, int64_t* out, void* stream);
+
+/* Device view for kernels that include pbm_device.hpp.  Replaces the
+ * cudaMemcpyToSymbol publication of d_PageMapRandomWalk_BM / Bitmap_length_d
+ * (RandomWalkBasic_BM.cuh:48-54) and d_page_groups (Paging.cuh:73-81). */
+int pbm_get_ctx(pbm_handle* h, pbm_ctx* out);
+
+int pbm_read_counters(pbm_handle* h, pbm_counters* out, void* stream);
+int pbm_clear_counters(pbm_handle* h, void* stream);
+
+/* ---- batch entry points (device-resident inputs and outputs) ----------- */
+
+/* Serves n one-page requests with indices req_base .. req_base+n-1.
+ * d_ids[i] = page id or -1; d_steps[i] (optional, may be NULL) = walk
+ * steps of request i.  The request index, not the thread, seeds the walk.
+ * Replaces the drivers' get1page_kernel / getPage_kernel
+ * (unitTests/unitTest1.cu:22-30, unitTest1.5.cu:32-36) calling
+ * getPage (RandomWalkBasic_BM.cuh:60-89, CollabRW_BM.cuh:114-160,
+ * CRW_BM.cuh:75-142, RWMalloc.cuh:113-149, parallelPage.cu:342,491,571). */
+int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
+                  int32_t* d_ids, uint32_t* d_steps,
+                  const pbm_launch* launch, void* stream);
+
+/* Returns the n pages in d_ids (entries < 0 are skipped).
+ * Replaces freePage_kernel (unitTests/unitTest1.5.cu:38-44) calling
+ * freePage (RandomWalkBasic_BM.cuh:98-102, parallelPage.cu:399,511,594). */
+int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
+                   const pbm_launch* launch, void* stream);
+
+/* ---- graph-captured phases --------------------------------------------- */
+
+#define PBM_PHASE_ALLOC 1u
+#define PBM_PHASE_FREE  2u
+
+typedef struct pbm_graph pbm_graph;
+
+/* Captures a HIP graph of one allocation phase (pbm_get_pages with
+ * get_launch), one free phase (pbm_free_pages of d_ids with free_launch),
+ * or both in that order (phases = PBM_PHASE_ALLOC | PBM_PHASE_FREE).  The
+ * arguments are frozen into the graph: every launch replays the same
+ * request indices, so a replay of both phases returns the map to its state
+ * before the replay.  No reference counterpart (its drivers launch kernels
+ * one by one, unitTests/unitTest1.cu:64). */
+int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
+                     uint32_t* d_steps, const pbm_launch* get_launch,
+                     const pbm_launch* free_launch, uint32_t phases, pbm_graph** out);
+int pbm_graph_launch(pbm_graph* g, void* stream);
+int pbm_graph_destroy(pbm_graph* g);
+
+/* ---- measurement ------------------------------------------------------ */
+
+/* Device-scope 64-bit atomicOr throughput on `words` distinct words
+ * (uncontended within a launch, every word hit `rounds` times), the
+ * ceiling the allocators' claim/free atomics run against.  Writes
+ * atomics/s to *ops_per_s.  Measurement helper; no reference counterpart. */
+int pbm_atomic_peak(uint32_t words, uint32_t rounds, int device, double* ops_per_s);
+
+/* Human-readable message of the last error on this thread. */
+const char* pbm_last_error(void);
+
+/* Library version string. */
+const char* pbm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PBM_H */

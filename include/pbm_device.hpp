@@ -1,0 +1,482 @@
+/*
+ * pbm_device.hpp — header-only device API of the page manager (gfx950, wave64).
+ *
+ * Include from a HIP kernel and pass the pool's pbm_ctx (pbm_get_ctx) by
+ * value.  This replaces the reference's __device__ API, which reads
+ * file-static globals (RandomWalkBasic_BM.cuh:14-21, CollabRW_BM.cuh:14-21,
+ * CRW_BM.cuh:14-21, RWMalloc.cuh:14-22, parallelPage.cuh:7-75).
+ *
+ *   pbm::get_page_rwbm(ctx, seed, &steps)      getPage        RandomWalkBasic_BM.cuh:60-89
+ *   pbm::get_page_collab(ctx, seed, &steps)    getPage        CollabRW_BM.cuh:114-160 (wave-collective)
+ *   pbm::free_page(ctx, id)                    freePage       RandomWalkBasic_BM.cuh:98-102
+ *   pbm::page_address(ctx, id)                 pageAddress    Paging.cuh:30-38
+ *   pbm::page_id(ctx, ptr)                     getPageID      Paging.cuh:40-52
+ *
+ * Failure handling differs from the reference on purpose: the reference
+ * walks forever when the pool is full (RandomWalkBasic_BM.cuh:67-85) or
+ * traps (RandomWalkBasic.cuh:66-69).  Here every grant first takes a unit
+ * of the reservation counter ctx.free_count; a request that finds it empty
+ * returns -1 at once, and a reserved walk is bounded by walk_budget random
+ * jumps followed by a linear sweep, so no call spins without end.
+ *
+ * Atomics are device (agent) scope and relaxed, as the reference's
+ * atomicOr/atomicAnd are.  Code that writes page contents in one thread and
+ * frees the page for another thread of the same kernel to reuse must put a
+ * __threadfence() before free_page.
+ */
+#ifndef PBM_DEVICE_HPP
+#define PBM_DEVICE_HPP
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pbm.h"
+
+namespace pbm {
+
+typedef unsigned long long u64;
+constexpr u64 kFull = ~0ull;
+constexpr u64 kGolden = 0x9E3779B97F4A7C15ull;
+constexpr u64 kPrefillSalt = 0x243F6A8885A308D3ull;
+constexpr u64 kWaveSalt = 0x13198A2E03707344ull;
+constexpr u64 kArmSalt = 0xA4093822299F31D0ull;
+
+/* ---------------------------------------------------------------------
+ * Random numbers
+ * ------------------------------------------------------------------- */
+
+/* RNG_LCG (RNG_LCG.cuh:6-12): x' = (1103515245*x + 12345) % 2^31 computed
+ * in 64-bit with C++ truncating modulo, so a negative product gives a
+ * negative result in (-2^31, 0]. */
+__host__ __device__ __forceinline__ int32_t lcg(int32_t x) {
+    long long v = 1103515245LL * (long long)x + 12345LL;
+    return (int32_t)(v % 2147483648LL);
+}
+
+/* splitmix64 output function (Steele, Lea, Flood 2014). */
+__host__ __device__ __forceinline__ u64 mix64(u64 z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Walk seed of request r: the high half of output r+1 of splitmix64 seeded
+ * with `seed`.  Replaces the reference's (tid<<15)+(int16)clock() seeds
+ * (RandomWalkBasic_BM.cuh:62-63), which overflow for tid >= 65536 and are
+ * not reproducible. */
+__host__ __device__ __forceinline__ int32_t request_seed(u64 seed, u64 r) {
+    return (int32_t)(uint32_t)(mix64(seed + (r + 1) * kGolden) >> 32);
+}
+
+/* Pre-fill hash of page i: output i+1 of splitmix64 seeded with
+ * mix64(seed ^ kPrefillSalt). */
+__host__ __device__ __forceinline__ u64 page_hash(u64 seed, u64 i) {
+    return mix64(mix64(seed ^ kPrefillSalt) + (i + 1) * kGolden);
+}
+
+/* ---------------------------------------------------------------------
+ * Wave64 helpers (all safe when only some lanes of the wave are active:
+ * they use ballots and read only lanes known to be active)
+ * ------------------------------------------------------------------- */
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+/* popcount of `mask` restricted to lanes below the calling lane. */
+__device__ __forceinline__ unsigned rank_in(u64 mask) {
+    return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ u64 ballot(bool p) { return __ballot(p ? 1 : 0); }
+
+/* Value of v in lane l (l uniform across the wave, lane l active). */
+__device__ __forceinline__ uint32_t readlane(uint32_t v, unsigned l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ u64 readlane64(u64 v, unsigned l) {
+    return ((u64)readlane((uint32_t)(v >> 32), l) << 32) | readlane((uint32_t)v, l);
+}
+
+/* Value of v in lane src (per-lane src, lane src active). */
+__device__ __forceinline__ uint32_t shfl32(uint32_t v, unsigned src) {
+    return (uint32_t)__shfl((int)v, (int)src, 64);
+}
+__device__ __forceinline__ u64 shfl64(u64 v, unsigned src) {
+    return ((u64)shfl32((uint32_t)(v >> 32), src) << 32) | shfl32((uint32_t)v, src);
+}
+
+/* Exclusive prefix sum and wave total of a 7-bit per-lane value (0..127)
+ * over the active lanes, built from seven ballots. */
+__device__ __forceinline__ unsigned scan7(unsigned v, unsigned& total) {
+    unsigned pre = 0, tot = 0;
+#pragma unroll
+    for (int b = 0; b < 7; ++b) {
+        u64 m = ballot((v >> b) & 1u);
+        pre += rank_in(m) << b;
+        tot += (unsigned)__popcll(m) << b;
+    }
+    total = tot;
+    return pre;
+}
+
+/* Position of the k-th (0-based) set bit of x; requires k < popcount(x). */
+__host__ __device__ __forceinline__ unsigned select_bit(u64 x, unsigned k) {
+    unsigned pos = 0;
+    unsigned c = (unsigned)__builtin_popcount((uint32_t)x);
+    if (k >= c) { k -= c; x >>= 32; pos = 32; }
+    uint32_t y = (uint32_t)x;
+    c = (unsigned)__builtin_popcount(y & 0xFFFFu); if (k >= c) { k -= c; y >>= 16; pos += 16; }
+    c = (unsigned)__builtin_popcount(y & 0xFFu);   if (k >= c) { k -= c; y >>= 8;  pos += 8; }
+    c = (unsigned)__builtin_popcount(y & 0xFu);    if (k >= c) { k -= c; y >>= 4;  pos += 4; }
+    c = (unsigned)__builtin_popcount(y & 0x3u);    if (k >= c) { k -= c; y >>= 2;  pos += 2; }
+    c = y & 1u; if (k >= c) pos += 1;
+    return pos;
+}
+
+/* The k lowest set bits of x (all of x when k >= popcount(x)). */
+__host__ __device__ __forceinline__ u64 lowest_bits(u64 x, unsigned k) {
+    if (k == 0) return 0;
+    if (k >= (unsigned)__builtin_popcountll(x)) return x;
+    return x & ((1ull << select_bit(x, k)) - 1ull);
+}
+
+/* ---------------------------------------------------------------------
+ * Page-map access: relaxed, device scope
+ * ------------------------------------------------------------------- */
+
+__device__ __forceinline__ u64 ld_word(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 atom_or(u64* p, u64 m) {
+    return __hip_atomic_fetch_or(p, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 atom_and(u64* p, u64 m) {
+    return __hip_atomic_fetch_and(p, m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ long long atom_add_ll(long long* p, long long v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 atom_add_u64(u64* p, u64 v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* Per-thread event tally; batch kernels sum it over the wave and add it to
+ * ctx.counters once (flush_stats). */
+struct Stats {
+    uint32_t allocs = 0, frees = 0, failures = 0, steps = 0;
+    uint32_t probes = 0, atomics = 0, fallbacks = 0, errors = 0;
+};
+
+/* Adds the wave's Stats to the pool counters: one atomic per counter per
+ * wave when the whole wave is active, per lane otherwise. */
+__device__ inline void flush_stats(const pbm_ctx& c, const Stats& s) {
+    if (!c.counters) return;
+    uint32_t v[8] = {s.allocs, s.frees, s.failures, s.steps,
+                     s.probes, s.atomics, s.fallbacks, s.errors};
+    if (ballot(true) != kFull) {
+        u64* base = (u64*)c.counters;
+        for (int i = 0; i < 8; ++i)
+            if (v[i]) atom_add_u64(base + i, (u64)v[i]);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint32_t x = v[i];
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) x += (uint32_t)__shfl_xor((int)x, d, 64);
+        v[i] = x;
+    }
+    if (lane_id() == 0) {
+        u64* base = (u64*)c.counters;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (v[i]) atom_add_u64(base + i, (u64)v[i]);
+    }
+}
+
+/* ---------------------------------------------------------------------
+ * Reservation counter
+ * ------------------------------------------------------------------- */
+
+/* Takes one unit of ctx.free_count for every active lane with want=true,
+ * one atomic per wave.  Lanes beyond the available count are refused and
+ * the excess is given back.  Exactly min(requests, available) grants are
+ * made over any sequence of calls that only allocate. */
+__device__ inline bool reserve_one(const pbm_ctx& c, bool want) {
+    u64 m = ballot(want);
+    if (m == 0) return false;
+    unsigned cnt = (unsigned)__popcll(m);
+    unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+    long long old = 0;
+    if (lane_id() == leader) old = atom_add_ll(c.free_count, -(long long)cnt);
+    old = (long long)readlane64((u64)old, leader);
+    long long granted = old < 0 ? 0 : (old > (long long)cnt ? (long long)cnt : old);
+    if (lane_id() == leader && granted < (long long)cnt)
+        atom_add_ll(c.free_count, (long long)cnt - granted);
+    return want && (long long)rank_in(m) < granted;
+}
+
+/* Returns one unit to ctx.free_count for every active lane with give=true,
+ * one atomic per wave. */
+__device__ inline void release_one(const pbm_ctx& c, bool give) {
+    u64 m = ballot(give);
+    if (m == 0) return;
+    unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+    if (lane_id() == leader) atom_add_ll(c.free_count, (long long)__popcll(m));
+}
+
+/* ---------------------------------------------------------------------
+ * Paging (Paging.cuh:30-52): one contiguous pool, O(1) both ways
+ * ------------------------------------------------------------------- */
+
+__device__ __forceinline__ void* page_address(const pbm_ctx& c, int32_t id) {
+    if (!c.pool || id < 0 || (uint32_t)id >= c.n_pages) return nullptr;
+    return (void*)(c.pool + (size_t)(uint32_t)id * c.page_size);
+}
+
+__device__ __forceinline__ int32_t page_id(const pbm_ctx& c, const void* p) {
+    if (!c.pool) return -1;
+    const char* q = (const char*)p;
+    if (q < c.pool) return -1;
+    size_t off = (size_t)(q - c.pool);
+    size_t id = off / c.page_size;
+    return id < c.n_pages ? (int32_t)id : -1;
+}
+
+/* ---------------------------------------------------------------------
+ * RandomWalkBasic_BM (RandomWalkBasic_BM.cuh:60-89) on 64-bit words
+ * ------------------------------------------------------------------- */
+
+/* Claims the lowest free bit of word g, retrying while the word is not
+ * full; returns the page id or -1 when the word is (or becomes) full. */
+__device__ __forceinline__ int32_t claim_in_word(const pbm_ctx& c, uint32_t g, u64 w, Stats& st) {
+    while (w != kFull) {
+        unsigned b = (unsigned)__ffsll((long long)~w) - 1u;
+        u64 bit = 1ull << b;
+        u64 old = atom_or(&c.bitmap[g], bit);
+        st.atomics++;
+        if (!(old & bit)) return (int32_t)(g * 64u + b);
+        w = old | bit;
+    }
+    return -1;
+}
+
+/* Linear sweep from word `from`+1 over at most `passes` rounds of the map.
+ * A reserved walk in a phase without frees finds a page within one pass. */
+__device__ inline int32_t sweep_claim(const pbm_ctx& c, uint32_t from, uint32_t& steps,
+                                      Stats& st, uint32_t passes) {
+    const uint32_t L = c.n_words;
+    uint32_t g = from;
+    for (uint64_t k = 0; k < (uint64_t)passes * L; ++k) {
+        g = (g + 1u == L) ? 0u : g + 1u;
+        steps++;
+        u64 w = ld_word(&c.bitmap[g]);
+        st.probes++;
+        if (w == kFull) continue;
+        int32_t id = claim_in_word(c, g, w, st);
+        if (id >= 0) return id;
+    }
+    return -1;
+}
+
+/* Random walk of one reserved request.  Step = one random jump, as
+ * RandomWalkBasic_BM.cuh:68 counts it.  The page id is g*64+b (the
+ * reference computes g<<(5+b) by operator precedence, :81). */
+__device__ inline int32_t walk_rwbm(const pbm_ctx& c, int32_t& seed, uint32_t budget,
+                                    uint32_t& steps, Stats& st) {
+    const uint32_t L = c.n_words;
+    uint32_t g = 0;
+    for (uint32_t j = 0; j < budget; ++j) {
+        steps++;
+        seed = lcg(seed);
+        g = (uint32_t)seed % L;
+        u64 w = ld_word(&c.bitmap[g]);
+        st.probes++;
+        int32_t id = claim_in_word(c, g, w, st);
+        if (id >= 0) return id;
+    }
+    st.fallbacks++;
+    return sweep_claim(c, g, steps, st, 4u);
+}
+
+/* getPage of RandomWalkBasic_BM: reserve, then walk.  Returns -1 when the
+ * pool has no page for this request. */
+__device__ inline int32_t get_page_rwbm(const pbm_ctx& c, int32_t seed, uint32_t* steps_out,
+                                        Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    uint32_t steps = 0;
+    int32_t id = -1;
+    if (reserve_one(c, true)) {
+        id = walk_rwbm(c, seed, c.walk_budget, steps, st);
+        if (id < 0) { atom_add_ll(c.free_count, 1); st.errors++; }
+    }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* freePage (RandomWalkBasic_BM.cuh:98-102): clears the page's bit and
+ * returns a reservation unit.  Freeing a free page or an id out of range
+ * changes nothing and counts an error. */
+__device__ inline void free_page(const pbm_ctx& c, int32_t id, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    bool ok = false;
+    if (id >= 0 && (uint32_t)id < c.n_pages) {
+        u64 bit = 1ull << ((uint32_t)id & 63u);
+        u64 old = atom_and(&c.bitmap[(uint32_t)id >> 6], ~bit);
+        st.atomics++;
+        ok = (old & bit) != 0;
+    }
+    if (ok) st.frees++; else st.errors++;
+    release_one(c, ok);
+}
+
+/* ---------------------------------------------------------------------
+ * Collaborative window step (CollabRW_BM.cuh:114-160, wave64 rebuild)
+ *
+ * Probing lanes each read one word of the window; `need` lanes ask for a
+ * page.  Free bits are designated to needy lanes in rank order, either
+ * lane-major (PBM_ORDER_PACKED) or round-robin over words
+ * (PBM_ORDER_REFERENCE, the order of the reference's has/need ballot
+ * loop, :140-153).  Each giver claims only its designated bits with one
+ * atomicOr and owns mask & ~old, so nothing is ever released; a needy lane
+ * whose designated bit was taken by a racing wave stays needy.  Replaces
+ * the reference's atomicExch of whole words and the leftover release
+ * atomicExch(&w, ~r) (:155-156), which overwrites concurrent frees.
+ * ------------------------------------------------------------------- */
+
+struct WindowResult {
+    unsigned seen_free;   /* free bits the probing lanes read (uniform)       */
+    unsigned designated;  /* bits designated to needy lanes (uniform)         */
+};
+
+/* One step over the words held by probing lanes: `word` is the lane's word
+ * index and `w` the value it read (kFull for lanes that do not probe).  On
+ * return, a needy lane that got a page has need=false and id set. */
+__device__ inline WindowResult window_claim(const pbm_ctx& c, uint32_t word, u64 w,
+                                            bool& need, int32_t& id, int32_t order, Stats& st) {
+    WindowResult res{0u, 0u};
+    u64 fr = ~w;
+    unsigned cnt = (unsigned)__popcll(fr);
+    u64 needy = ballot(need);
+    unsigned n_need = (unsigned)__popcll(needy);
+    unsigned q = rank_in(needy);
+    unsigned total = 0;
+    unsigned pre = scan7(cnt, total);
+    res.seen_free = total;
+    if (n_need == 0 || total == 0) return res;
+
+    unsigned k = 0;          /* bits this lane gives                     */
+    unsigned giver = 0;      /* lane designated to serve this needy lane */
+    unsigned gbit = 0;       /* rank of that bit among the giver's free bits */
+    bool designated = false;
+    if (order == PBM_ORDER_REFERENCE) {
+        unsigned H = 0;
+        for (unsigned j = 0; j < 64u && H < n_need; ++j) {
+            u64 B = ballot(cnt > j);
+            unsigned h = (unsigned)__popcll(B);
+            if (h == 0) break;
+            unsigned take = (n_need - H < h) ? n_need - H : h;
+            if (cnt > j && rank_in(B) < take) k++;
+            if (need && q >= H && q < H + take) {
+                giver = select_bit(B, q - H);
+                gbit = j;
+                designated = true;
+            }
+            H += take;
+        }
+        res.designated = H;
+    } else {
+        unsigned avail = n_need > pre ? n_need - pre : 0u;
+        k = cnt < avail ? cnt : avail;
+        u64 G = ballot(k > 0);
+        unsigned base = 0;
+        while (G) {
+            unsigned g = (unsigned)__ffsll((long long)G) - 1u;
+            G &= G - 1ull;
+            unsigned kg = readlane(k, g);
+            if (need && q >= base && q < base + kg) {
+                giver = g;
+                gbit = q - base;
+                designated = true;
+            }
+            base += kg;
+        }
+        res.designated = base;
+    }
+
+    u64 mask = lowest_bits(fr, k);
+    u64 owned = 0;
+    if (mask) {
+        u64 old = atom_or(&c.bitmap[word], mask);
+        st.atomics++;
+        owned = mask & ~old;
+    }
+    unsigned src = designated ? giver : lane_id();
+    u64 g_owned = shfl64(owned, src);
+    u64 g_free = shfl64(fr, src);
+    uint32_t g_word = shfl32(word, src);
+    if (designated) {
+        unsigned b = select_bit(g_free, gbit);
+        if ((g_owned >> b) & 1ull) {
+            id = (int32_t)(g_word * 64u + b);
+            need = false;
+        }
+    }
+    return res;
+}
+
+/* Loads the lane's word (when probing) and runs window_claim. */
+__device__ inline WindowResult window_step(const pbm_ctx& c, bool probe, uint32_t word,
+                                           bool& need, int32_t& id, int32_t order, Stats& st) {
+    u64 w = kFull;
+    if (probe) { w = ld_word(&c.bitmap[word]); st.probes++; }
+    return window_claim(c, word, w, need, id, order, st);
+}
+
+/* getPage of CollabRW_BM: wave-collective over the active lanes, which all
+ * probe and all ask for a page.  The window of step s starts at
+ * (uint32)lcg^s(seed) % n_words, lane l probing word start+l (mod n_words);
+ * lanes l >= n_words do not probe, so a small map is never read twice in
+ * one window.  seed is taken from the lowest active lane.  Step = one
+ * window, the same for every lane served in it (CollabRW_BM.cuh:124). */
+__device__ inline int32_t get_page_collab(const pbm_ctx& c, int32_t seed, uint32_t* steps_out,
+                                          Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t L = c.n_words;
+    seed = (int32_t)__builtin_amdgcn_readfirstlane(seed);
+    bool need = reserve_one(c, true);
+    int32_t id = -1;
+    uint32_t steps = 0;
+    const unsigned lane = lane_id();
+    uint32_t budget = c.walk_budget;
+    uint32_t start = 0;
+    uint32_t jumps = 0, sweeps = 0;
+    const uint32_t max_sweeps = 4u * ((L + 63u) / 64u) + 4u;
+    while (ballot(need)) {
+        if (jumps < budget) {
+            seed = lcg(seed);
+            start = (uint32_t)seed % L;
+            jumps++;
+        } else {
+            if (sweeps == 0) st.fallbacks++;
+            start = (uint32_t)(((u64)start + 64u) % L);
+            if (++sweeps > max_sweeps) break;
+        }
+        if (need) steps++;
+        uint32_t word = (uint32_t)(((u64)start + lane) % L);
+        window_step(c, lane < L, word, need, id, c.order, st);
+    }
+    if (need) { atom_add_ll(c.free_count, 1); st.errors++; }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+}  // namespace pbm
+
+#endif  /* PBM_DEVICE_HPP */

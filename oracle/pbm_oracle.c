@@ -1,0 +1,424 @@
+/*
+ * pbm_oracle.c — serial CPU restatement of the reference's page-allocation
+ * policies.  TEST INFRASTRUCTURE ONLY (see pbm_oracle.h): the product
+ * library never links, loads or calls this file.
+ *
+ * Pinning status of each function (how tests/ check it independently):
+ *   orc_lcg               glibc random_r TYPE_0 state update (non-negative
+ *                         seeds) and numpy int64 fmod (negative seeds)
+ *   orc_mix64             published splitmix64 constants; bijectivity via
+ *                         its inverse; avalanche statistics
+ *   orc_prefill_bernoulli popcount within binomial bounds of f*T
+ *   orc_rwbm_serial       hand-computed known answers on tiny pools,
+ *                         invariants, mean steps vs the closed form
+ *   orc_collab_serial     hand-built window known answers (both orders),
+ *                         invariants, brute-force set property
+ *   orc_batched_serial    same as collab plus phase admission known answer
+ *   orc_check_alloc       planted violations (duplicate, not-free, lost bit)
+ */
+#include "pbm_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef uint64_t u64;
+#define FULL (~(u64)0)
+#define GOLDEN 0x9E3779B97F4A7C15ull
+#define PREFILL_SALT 0x243F6A8885A308D3ull
+#define WAVE_SALT 0x13198A2E03707344ull
+
+/* ------------------------------------------------------------ random */
+
+/* RNG_LCG.cuh:6-12 — (1103515245*x + 12345) % (1<<31) in long long with
+ * C truncating modulo (LCG_M expands to 1<<31 without parentheses; the
+ * sign of a C remainder follows the dividend, so the divisor's sign does
+ * not matter). */
+int32_t orc_lcg(int32_t x) {
+    long long v = 1103515245LL * (long long)x + 12345LL;
+    return (int32_t)(v % 2147483648LL);
+}
+
+/* splitmix64 output function (Steele, Lea & Flood, OOPSLA 2014). */
+uint64_t orc_mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Seed of request r: high 32 bits of splitmix64 output r+1 from state
+ * `seed`.  Stands in for (tid<<15)+(int16)clock()
+ * (RandomWalkBasic_BM.cuh:62-63), which is not reproducible. */
+int32_t orc_request_seed(uint64_t seed, uint64_t r) {
+    return (int32_t)(uint32_t)(orc_mix64(seed + (r + 1) * GOLDEN) >> 32);
+}
+
+uint64_t orc_page_hash(uint64_t seed, uint64_t i) {
+    return orc_mix64(orc_mix64(seed ^ PREFILL_SALT) + (i + 1) * GOLDEN);
+}
+
+/* ------------------------------------------------------------ page map */
+
+uint32_t orc_n_words(uint32_t n_pages) { return (n_pages + 63u) / 64u; }
+
+static u64 pad_mask(uint32_t n_pages, uint32_t word) {
+    u64 first = (u64)word * 64u;
+    if (first + 64u <= n_pages) return 0;
+    uint32_t valid = (uint32_t)(n_pages - first);
+    return ~(((u64)1 << valid) - 1u);
+}
+
+/* resetMemoryManager (RandomWalkBasic_BM.cuh:104-106): all free.  Padding
+ * bits of the last word stay set. */
+void orc_clear(uint64_t* bm, uint32_t n_pages) {
+    uint32_t L = orc_n_words(n_pages);
+    for (uint32_t i = 0; i < L; ++i) bm[i] = pad_mask(n_pages, i);
+}
+
+void orc_prefill_bernoulli(uint64_t* bm, uint32_t n_pages, double fill, uint64_t seed) {
+    uint32_t L = orc_n_words(n_pages);
+    int all = fill >= 1.0;
+    u64 thr = (fill > 0.0 && fill < 1.0) ? (u64)ldexp(fill, 64) : 0;
+    for (uint32_t i = 0; i < L; ++i) {
+        u64 w = 0;
+        for (uint32_t b = 0; b < 64; ++b) {
+            u64 p = (u64)i * 64u + b;
+            int used = p >= n_pages ? 1 : (all || orc_page_hash(seed, p) < thr);
+            w |= (u64)used << b;
+        }
+        bm[i] = w;
+    }
+}
+
+/* getFreePagePercentage numerator (RandomWalkBasic_BM.cuh:109-145). */
+int64_t orc_count_free(const uint64_t* bm, uint32_t n_pages) {
+    uint32_t L = orc_n_words(n_pages);
+    int64_t f = 0;
+    for (uint32_t i = 0; i < L; ++i) f += 64 - __builtin_popcountll(bm[i] | pad_mask(n_pages, i));
+    return f;
+}
+
+/* ------------------------------------------------------------ RW_BM */
+
+/* Inner loop of RandomWalkBasic_BM getPage (:72-85): while the word is
+ * not full, set its lowest 0-bit; success when that bit was 0. */
+static int32_t claim_lowest(uint64_t* bm, uint32_t g) {
+    while (bm[g] != FULL) {
+        u64 v = bm[g];
+        int b = __builtin_ctzll(~v);
+        u64 bit = (u64)1 << b;
+        u64 old = bm[g];
+        bm[g] |= bit;
+        if (!(old & bit)) return (int32_t)(g * 64u + (uint32_t)b);
+    }
+    return -1;
+}
+
+/* One reserved walk: `budget` random jumps (RandomWalkBasic_BM.cuh:67-71,
+ * step = one jump, :68), page id g*64+b, then a linear sweep of at most
+ * four passes starting after the last jumped word. */
+static int32_t rwbm_walk(uint64_t* bm, uint32_t L, int32_t* seed, uint32_t budget,
+                         uint32_t* steps) {
+    uint32_t g = 0;
+    for (uint32_t j = 0; j < budget; ++j) {
+        (*steps)++;
+        *seed = orc_lcg(*seed);
+        g = (uint32_t)*seed % L;
+        int32_t id = claim_lowest(bm, g);
+        if (id >= 0) return id;
+    }
+    for (u64 k = 0; k < 4ull * L; ++k) {
+        g = (g + 1u == L) ? 0u : g + 1u;
+        (*steps)++;
+        int32_t id = claim_lowest(bm, g);
+        if (id >= 0) return id;
+    }
+    return -1;
+}
+
+uint32_t orc_rwbm_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                         uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* ids,
+                         uint32_t* steps) {
+    uint32_t L = orc_n_words(n_pages), granted = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t s = 0;
+        int32_t id = -1;
+        if (*free_count > 0) {
+            (*free_count)--;
+            int32_t sd = orc_request_seed(seed, req_base + r);
+            id = rwbm_walk(bm, L, &sd, budget, &s);
+            if (id < 0) (*free_count)++;
+        }
+        if (id >= 0) granted++;
+        ids[r] = id;
+        if (steps) steps[r] = s;
+    }
+    return granted;
+}
+
+/* ------------------------------------------------------------ CollabRW */
+
+/* One collaborative step over a window.  probe[l]: lane l read word
+ * word[l]; need[l]: lane l asks for a page.  Designated bits are set in
+ * the map (serial: nothing races), served lanes get ids and need=0.
+ * Returns the bits designated; *seen = free bits read.
+ *
+ * order 1 follows the reference's has/need loop literally
+ * (CollabRW_BM.cuh:129-153): the s-th needy lane takes the lowest free bit
+ * of the (s+1)-th has-lane (fks, :81-94); every has-lane ranked below the
+ * number of needy lanes drops its lowest bit (:148-149); masks are
+ * recomputed until nobody needs or nobody has; leftovers return to the map
+ * (the atomicExch(&w, ~r) of :155-156 restores exactly them when nothing
+ * races).  order 0 hands out all free bits of lane 0's word first, then
+ * lane 1's, and so on. */
+static uint32_t collab_claim(uint64_t* bm, const int* probe, const uint32_t* word, int* need,
+                             int32_t* ids, int order, uint32_t* seen) {
+    u64 r[64];
+    uint32_t total = 0, given = 0;
+    for (int l = 0; l < 64; ++l) {
+        r[l] = probe[l] ? ~bm[word[l]] : 0;
+        total += (uint32_t)__builtin_popcountll(r[l]);
+    }
+    *seen = total;
+    if (order == 1) {
+        for (;;) {
+            u64 has = 0, needm = 0;
+            for (int l = 0; l < 64; ++l) {
+                if (r[l]) has |= (u64)1 << l;
+                if (need[l]) needm |= (u64)1 << l;
+            }
+            if (!has || !needm) break;
+            int n_need = __builtin_popcountll(needm);
+            int32_t give_id[64];
+            int give_src[64];
+            for (int l = 0; l < 64; ++l) {
+                give_src[l] = -1;
+                if (!need[l]) continue;
+                int s = __builtin_popcountll(needm & (((u64)1 << l) - 1u));
+                /* lane of the (s+1)-th set bit of has, if any */
+                int cnt = 0;
+                for (int j = 0; j < 64; ++j)
+                    if ((has >> j) & 1u) {
+                        if (cnt == s) { give_src[l] = j; break; }
+                        cnt++;
+                    }
+                if (give_src[l] >= 0) {
+                    int j = give_src[l];
+                    give_id[l] = (int32_t)(word[j] * 64u + (uint32_t)__builtin_ctzll(r[j]));
+                }
+            }
+            for (int l = 0; l < 64; ++l)
+                if (need[l] && give_src[l] >= 0) {
+                    ids[l] = give_id[l];
+                    bm[(uint32_t)give_id[l] >> 6] |= (u64)1 << ((uint32_t)give_id[l] & 63u);
+                    need[l] = 0;
+                    given++;
+                }
+            for (int l = 0; l < 64; ++l)
+                if (r[l] && __builtin_popcountll(has & (((u64)1 << l) - 1u)) < n_need)
+                    r[l] &= r[l] - 1u;
+        }
+    } else {
+        int l = 0;
+        for (int q = 0; q < 64; ++q) {
+            if (!need[q]) continue;
+            while (l < 64 && r[l] == 0) l++;
+            if (l == 64) break;
+            int b = __builtin_ctzll(r[l]);
+            r[l] &= r[l] - 1u;
+            ids[q] = (int32_t)(word[l] * 64u + (uint32_t)b);
+            bm[word[l]] |= (u64)1 << b;
+            need[q] = 0;
+            given++;
+        }
+    }
+    return given;
+}
+
+static int any(const int* v) {
+    for (int l = 0; l < 64; ++l)
+        if (v[l]) return 1;
+    return 0;
+}
+
+uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                           uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                           int32_t* ids, uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    const uint32_t max_sweeps = 4u * ((L + 63u) / 64u) + 4u;
+    uint32_t granted_total = 0;
+    for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        int active[64], need[64], probe[64];
+        int32_t cid[64];
+        uint32_t cst[64], word[64];
+        uint32_t n_act = 0;
+        for (int l = 0; l < 64; ++l) {
+            active[l] = ch * 64u + (uint32_t)l < n;
+            n_act += (uint32_t)active[l];
+            cid[l] = -1;
+            cst[l] = 0;
+        }
+        /* wave-aggregated reservation: the lowest `granted` active lanes */
+        int64_t old = *free_count;
+        int64_t g = old < 0 ? 0 : (old > (int64_t)n_act ? (int64_t)n_act : old);
+        *free_count -= g;
+        for (int l = 0; l < 64; ++l) need[l] = active[l] && l < g;
+        int32_t sd = orc_request_seed(seed, req_base + (u64)ch * 64u);
+        uint32_t jumps = 0, sweeps = 0, start = 0;
+        while (any(need)) {
+            if (jumps < budget) {
+                sd = orc_lcg(sd);
+                start = (uint32_t)sd % L;
+                jumps++;
+            } else {
+                start = (uint32_t)(((u64)start + 64u) % L);
+                if (++sweeps > max_sweeps) break;
+            }
+            for (int l = 0; l < 64; ++l) {
+                if (need[l]) cst[l]++;
+                probe[l] = active[l] && (uint32_t)l < L;
+                word[l] = (uint32_t)(((u64)start + (uint32_t)l) % L);
+            }
+            uint32_t seen;
+            collab_claim(bm, probe, word, need, cid, order, &seen);
+        }
+        for (int l = 0; l < 64; ++l) {
+            if (need[l]) (*free_count)++;
+            if (!active[l]) continue;
+            ids[ch * 64u + l] = cid[l];
+            if (steps) steps[ch * 64u + l] = cst[l];
+            if (cid[l] >= 0) granted_total++;
+        }
+    }
+    return granted_total;
+}
+
+uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                            uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                            uint32_t cluster_words, int32_t* ids, uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    const uint32_t J = cluster_words / 64u;
+    const uint32_t span = 64u * J;
+    const uint32_t max_sweeps = 2u * ((L + span - 1u) / span) + 2u;
+    /* phase admission: the first min(n, free) requests by index */
+    int64_t old = *free_count;
+    int64_t granted = old < 0 ? 0 : (old > (int64_t)n ? (int64_t)n : old);
+    *free_count -= granted;
+    int32_t wseed = orc_request_seed(seed ^ WAVE_SALT, req_base + 0);
+    int have = 0;
+    uint32_t start = 0, got = 0;
+    for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        int need[64], valid[64];
+        int32_t cid[64];
+        uint32_t cst[64];
+        for (int l = 0; l < 64; ++l) {
+            uint32_t r = ch * 64u + (uint32_t)l;
+            valid[l] = r < n;
+            need[l] = valid[l] && (int64_t)r < granted;
+            cid[l] = -1;
+            cst[l] = 0;
+        }
+        uint32_t jumps = 0, sweeps = 0, iters = 0;
+        const uint32_t max_iters = budget + max_sweeps + 65536u;
+        while (any(need)) {
+            if (++iters > max_iters) break;
+            if (!have) {
+                if (jumps < budget) {
+                    wseed = orc_lcg(wseed);
+                    start = (uint32_t)wseed % L;
+                    jumps++;
+                } else {
+                    start = (uint32_t)(((u64)start + span) % L);
+                    if (++sweeps > max_sweeps) break;
+                }
+                have = 1;
+            }
+            for (int l = 0; l < 64; ++l)
+                if (need[l]) cst[l]++;
+            uint32_t seen_all = 0, desig_all = 0;
+            int examined_all = 1;
+            for (uint32_t i = 0; i < J; ++i) {
+                if (!any(need)) {
+                    examined_all = 0;
+                    break;
+                }
+                int probe[64];
+                uint32_t word[64];
+                for (int l = 0; l < 64; ++l) {
+                    uint32_t off = 64u * i + (uint32_t)l;
+                    probe[l] = off < L;
+                    word[l] = (uint32_t)(((u64)start + off) % L);
+                }
+                uint32_t seen;
+                desig_all += collab_claim(bm, probe, word, need, cid, order, &seen);
+                seen_all += seen;
+            }
+            if (examined_all && seen_all <= desig_all) have = 0;
+        }
+        for (int l = 0; l < 64; ++l) {
+            if (need[l]) (*free_count)++;
+            if (!valid[l]) continue;
+            ids[ch * 64u + l] = cid[l];
+            if (steps) steps[ch * 64u + l] = cst[l];
+            if (cid[l] >= 0) got++;
+        }
+    }
+    return got;
+}
+
+/* ------------------------------------------------------------ free */
+
+/* freePage (RandomWalkBasic_BM.cuh:98-102): clear the page's bit. */
+uint32_t orc_free(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                  const int32_t* ids) {
+    uint32_t freed = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        int32_t id = ids[i];
+        if (id < 0 || (uint32_t)id >= n_pages) continue;
+        u64 bit = (u64)1 << ((uint32_t)id & 63u);
+        if (bm[(uint32_t)id >> 6] & bit) {
+            bm[(uint32_t)id >> 6] &= ~bit;
+            freed++;
+        }
+    }
+    *free_count += freed;
+    return freed;
+}
+
+/* ------------------------------------------------------------ checks */
+
+int orc_check_alloc(const uint64_t* before, const uint64_t* after, uint32_t n_pages, uint32_t n,
+                    const int32_t* ids, uint32_t* n_granted) {
+    uint32_t L = orc_n_words(n_pages);
+    u64* seen = (u64*)calloc(L ? L : 1, sizeof(u64));
+    int rc = 0;
+    uint32_t g = 0;
+    if (!seen) return 99;
+    for (uint32_t i = 0; i < n && !rc; ++i) {
+        int32_t id = ids[i];
+        if (id < 0) continue;
+        if ((uint32_t)id >= n_pages) { rc = 1; break; }
+        uint32_t w = (uint32_t)id >> 6;
+        u64 bit = (u64)1 << ((uint32_t)id & 63u);
+        if (before[w] & bit) { rc = 2; break; }
+        if (seen[w] & bit) { rc = 3; break; }
+        if (!(after[w] & bit)) { rc = 4; break; }
+        seen[w] |= bit;
+        g++;
+    }
+    for (uint32_t w = 0; w < L && !rc; ++w)
+        if (after[w] != (before[w] | seen[w])) rc = 5;
+    free(seen);
+    if (n_granted) *n_granted = g;
+    return rc;
+}
+
+/* RWBM_TAS (theoretical_models.py:34-41) for words of W bits: the k-th
+ * grant sees T-A+k used pages and needs 1/(1-((T-A+k)/T)^W) probes on
+ * average.  The reference sums k < N-1 and divides by N; this is the mean
+ * over all n grants. */
+double orc_rwbm_expected_steps(double T, double A, double n, int W) {
+    double s = 0;
+    for (double k = 0; k < n; k += 1.0) s += 1.0 / (1.0 - pow((T - A + k) / T, (double)W));
+    return n > 0 ? s / n : 0.0;
+}

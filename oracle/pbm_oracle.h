@@ -1,0 +1,73 @@
+/*
+ * pbm_oracle.h — serial CPU restatement of the reference's page-allocation
+ * policies.  TEST INFRASTRUCTURE ONLY: linked by tests/, by the smoke check
+ * and by the benchmark's CPU-baseline leg as the checker.  The product
+ * library never links, loads or calls it.
+ *
+ * Every walk runs one request at a time in request-index order (the
+ * schedule of one thread, or one wave for the wave-collective policies);
+ * the device library reproduces that schedule in PBM_MODE_SERIAL, where its
+ * results must equal these bit for bit.
+ */
+#ifndef PBM_ORACLE_H
+#define PBM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* random numbers */
+int32_t  orc_lcg(int32_t x);
+uint64_t orc_mix64(uint64_t z);
+int32_t  orc_request_seed(uint64_t seed, uint64_t r);
+uint64_t orc_page_hash(uint64_t seed, uint64_t i);
+
+/* page map */
+uint32_t orc_n_words(uint32_t n_pages);
+void     orc_clear(uint64_t* bm, uint32_t n_pages);
+void     orc_prefill_bernoulli(uint64_t* bm, uint32_t n_pages, double fill, uint64_t seed);
+int64_t  orc_count_free(const uint64_t* bm, uint32_t n_pages);
+
+/* Plain RandomWalkBasic_BM, serial.  free_count is the reservation counter
+ * (in/out).  Returns the number of grants. */
+uint32_t orc_rwbm_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                         uint64_t req_base, uint64_t seed, uint32_t budget,
+                         int32_t* ids, uint32_t* steps);
+
+/* CollabRW_BM device getPage called by one wave over chunks of 64
+ * requests (lanes of the last chunk beyond n inactive). order: 0 packed,
+ * 1 reference round-robin. */
+uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                           uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                           int32_t* ids, uint32_t* steps);
+
+/* Batched persistent allocation with one wave: phase reservation of
+ * min(n, free) for the first requests by index, windows of 64*J words kept
+ * across chunks while they have free bits. */
+uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                            uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                            uint32_t cluster_words, int32_t* ids, uint32_t* steps);
+
+/* freePage for every id >= 0; returns pages actually returned (bits that
+ * were set).  free_count grows by that number. */
+uint32_t orc_free(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                  const int32_t* ids);
+
+/* Checks of one alloc phase against the page map before it (`before`) and
+ * after it (`after`): every granted id unique, in range, free before and
+ * used after, and after == before | granted bits.  Returns 0 when all hold,
+ * else a positive code naming the first violated property. */
+int orc_check_alloc(const uint64_t* before, const uint64_t* after, uint32_t n_pages,
+                    uint32_t n, const int32_t* ids, uint32_t* n_granted);
+
+/* Expected mean probes of n serial single-page grants on a pool of T pages
+ * with A free, words of W bits (theoretical_models.py:34-41 restated). */
+double orc_rwbm_expected_steps(double T, double A, double n, int W);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,155 @@
+"""ctypes binding of liborc.so, the serial CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's CPU-baseline leg, always as the checker or the CPU baseline,
+never as a path of the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liborc.so"
+
+_lib = None
+_P32 = C.POINTER(C.c_int32)
+_PU32 = C.POINTER(C.c_uint32)
+_PU64 = C.POINTER(C.c_uint64)
+_PI64 = C.POINTER(C.c_int64)
+
+_SIGS = {
+    "orc_lcg": (C.c_int32, [C.c_int32]),
+    "orc_mix64": (C.c_uint64, [C.c_uint64]),
+    "orc_request_seed": (C.c_int32, [C.c_uint64, C.c_uint64]),
+    "orc_page_hash": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "orc_n_words": (C.c_uint32, [C.c_uint32]),
+    "orc_clear": (None, [_PU64, C.c_uint32]),
+    "orc_prefill_bernoulli": (None, [_PU64, C.c_uint32, C.c_double, C.c_uint64]),
+    "orc_count_free": (C.c_int64, [_PU64, C.c_uint32]),
+    "orc_rwbm_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                     C.c_uint64, C.c_uint32, _P32, _PU32]),
+    "orc_collab_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                       C.c_uint64, C.c_uint32, C.c_int, _P32, _PU32]),
+    "orc_batched_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                        C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, _P32,
+                                        _PU32]),
+    "orc_free": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _P32]),
+    "orc_check_alloc": (C.c_int, [_PU64, _PU64, C.c_uint32, C.c_uint32, _P32, _PU32]),
+    "orc_rwbm_expected_steps": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int]),
+}
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def n_words(n_pages: int) -> int:
+    return (n_pages + 63) // 64
+
+
+def lcg(x: int) -> int:
+    return lib().orc_lcg(x)
+
+
+def mix64(z: int) -> int:
+    return lib().orc_mix64(z)
+
+
+def request_seed(seed: int, r: int) -> int:
+    return lib().orc_request_seed(seed, r)
+
+
+def page_hash(seed: int, i: int) -> int:
+    return lib().orc_page_hash(seed, i)
+
+
+def cleared(n_pages: int) -> np.ndarray:
+    bm = np.zeros(n_words(n_pages), dtype=np.uint64)
+    lib().orc_clear(_p(bm, _PU64), n_pages)
+    return bm
+
+
+def prefill(n_pages: int, fill: float, seed: int) -> np.ndarray:
+    bm = np.zeros(n_words(n_pages), dtype=np.uint64)
+    lib().orc_prefill_bernoulli(_p(bm, _PU64), n_pages, fill, seed)
+    return bm
+
+
+def count_free(bm: np.ndarray, n_pages: int) -> int:
+    return int(lib().orc_count_free(_p(bm, _PU64), n_pages))
+
+
+class State:
+    """A page map plus its reservation counter, mutated by the serial policies."""
+
+    def __init__(self, bm: np.ndarray, n_pages: int, free_count: int | None = None):
+        self.bm = np.ascontiguousarray(bm, dtype=np.uint64).copy()
+        self.n_pages = n_pages
+        fc = count_free(self.bm, n_pages) if free_count is None else free_count
+        self.free = np.array([fc], dtype=np.int64)
+
+    def _out(self, n):
+        return np.full(n, -1, dtype=np.int32), np.zeros(n, dtype=np.uint32)
+
+    def rwbm(self, n: int, req_base: int, seed: int, budget: int = 1 << 20):
+        ids, steps = self._out(n)
+        lib().orc_rwbm_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                              req_base, seed, budget, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps
+
+    def collab(self, n: int, req_base: int, seed: int, order: int, budget: int = 1 << 20):
+        ids, steps = self._out(n)
+        lib().orc_collab_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                req_base, seed, budget, order, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps
+
+    def batched(self, n: int, req_base: int, seed: int, order: int, cluster_words: int = 64,
+                budget: int = 1 << 20):
+        ids, steps = self._out(n)
+        lib().orc_batched_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                 req_base, seed, budget, order, cluster_words, _p(ids, _P32),
+                                 _p(steps, _PU32))
+        return ids, steps
+
+    def free_ids(self, ids: np.ndarray) -> int:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        return int(lib().orc_free(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64),
+                                  ids.size, _p(ids, _P32)))
+
+
+def check_alloc(before: np.ndarray, after: np.ndarray, n_pages: int, ids: np.ndarray):
+    """(code, granted): code 0 when every invariant holds."""
+    before = np.ascontiguousarray(before, dtype=np.uint64)
+    after = np.ascontiguousarray(after, dtype=np.uint64)
+    ids = np.ascontiguousarray(ids, dtype=np.int32)
+    g = C.c_uint32()
+    rc = lib().orc_check_alloc(_p(before, _PU64), _p(after, _PU64), n_pages, ids.size,
+                               _p(ids, _P32), C.byref(g))
+    return int(rc), int(g.value)
+
+
+def expected_steps(T: float, A: float, n: float, W: int = 64) -> float:
+    return float(lib().orc_rwbm_expected_steps(T, A, n, W))

@@ -1,0 +1,66 @@
+// Internal declarations shared by the library's translation units.
+#ifndef PBM_INTERNAL_HPP
+#define PBM_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+#include <string>
+#include "pbm.h"
+#include "pbm_device.hpp"
+
+struct pbm_handle {
+    pbm_config cfg;
+    pbm_ctx ctx;
+    int device = 0;
+    int n_cu = 0;
+    // device allocations owned by the handle
+    unsigned long long* d_bitmap = nullptr;
+    long long* d_free = nullptr;        // reservation counter
+    pbm_counters* d_counters = nullptr;
+    char* d_pool = nullptr;
+    int32_t* d_hint = nullptr;
+    unsigned* d_arms = nullptr;
+    unsigned long long* d_clock = nullptr;
+    unsigned long long* d_ring = nullptr;
+    unsigned long long* d_ring_ht = nullptr;  // [0] head, [1] tail
+    long long* d_phase = nullptr;
+    unsigned long long* d_scratch = nullptr;  // 8 words for reductions
+};
+
+namespace pbmi {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what, const char* file, int line);
+
+// Makes the handle's device current for the calling thread.
+int use_device(pbm_handle* h);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Re-initialises strategy state after the page map changed wholesale
+// (reset, prefill, restore): hints, arms, clock, free list.
+int reinit_strategy_state(pbm_handle* h, hipStream_t s);
+
+// Recomputes the reservation counter from the page map.
+int recount_free(pbm_handle* h, hipStream_t s);
+
+// LinkedList: refills the FIFO ring with the free pages of the page map in
+// ascending order and resets head/tail tickets.
+int ring_rebuild(pbm_handle* h, hipStream_t s);
+
+}  // namespace pbmi
+
+#define PBM_HIP(expr)                                                            \
+    do {                                                                         \
+        hipError_t e_ = (expr);                                                  \
+        if (e_ != hipSuccess) return pbmi::hip_fail(e_, #expr, __FILE__, __LINE__); \
+    } while (0)
+
+#define PBM_CHECK_HANDLE(h)                                                      \
+    do {                                                                         \
+        if (!(h)) return pbmi::fail(PBM_ERR_INVALID, "null pbm_handle");         \
+        int rc_ = pbmi::use_device(h);                                           \
+        if (rc_) return rc_;                                                     \
+    } while (0)
+
+#endif

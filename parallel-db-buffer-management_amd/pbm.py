@@ -1,0 +1,281 @@
+"""ctypes binding of libpbm.so (the C-ABI declared in include/pbm.h).
+
+Device buffers passed to the batch entry points are plain device pointers;
+callers here use torch tensors for them (torch is only the allocator and the
+stream provider).  Loading fails loudly when libpbm.so has not been built:
+there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libpbm.so"
+
+# pbm_strategy
+RW_BM, COLLAB_RW_BM, CRW_BM, RW_MALLOC, LINKED_LIST, SINGLE_CLOCK, DIST_CLOCK = range(7)
+STRATEGY_NAMES = {
+    RW_BM: "RandomWalkBasic_BM", COLLAB_RW_BM: "CollabRW_BM", CRW_BM: "CRW_BM",
+    RW_MALLOC: "RWMalloc", LINKED_LIST: "LinkedList", SINGLE_CLOCK: "SingleClock",
+    DIST_CLOCK: "DistClock",
+}
+# pbm_order
+ORDER_PACKED, ORDER_REFERENCE = 0, 1
+# pbm_mode
+MODE_PLAIN, MODE_BATCHED, MODE_SERIAL = 0, 1, 2
+LAUNCH_NO_COUNTERS = 1
+LAUNCH_ONE_WAVE = 2
+
+PHASE_ALLOC, PHASE_FREE = 1, 2
+
+ERRORS = {0: "PBM_OK", -1: "PBM_ERR_INVALID", -2: "PBM_ERR_HIP", -3: "PBM_ERR_OOM",
+          -4: "PBM_ERR_UNSUPPORTED", -5: "PBM_ERR_RANGE"}
+
+
+class Config(C.Structure):
+    _fields_ = [("n_pages", C.c_uint32), ("page_size", C.c_uint32), ("strategy", C.c_int32),
+                ("backing", C.c_int32), ("seed", C.c_uint64), ("walk_budget", C.c_uint32),
+                ("max_threads", C.c_uint32), ("order", C.c_int32), ("device", C.c_int32)]
+
+
+class Counters(C.Structure):
+    _fields_ = [(n, C.c_ulonglong) for n in
+                ("allocs", "frees", "failures", "steps", "probes", "atomics", "fallbacks",
+                 "errors")]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class Ctx(C.Structure):
+    _fields_ = [("bitmap", C.c_void_p), ("free_count", C.c_void_p), ("counters", C.c_void_p),
+                ("pool", C.c_void_p), ("hint", C.c_void_p), ("arms", C.c_void_p),
+                ("clock_arm", C.c_void_p), ("ring", C.c_void_p), ("ring_head", C.c_void_p),
+                ("ring_tail", C.c_void_p), ("phase", C.c_void_p),
+                ("n_pages", C.c_uint32), ("n_words", C.c_uint32), ("page_size", C.c_uint32),
+                ("walk_budget", C.c_uint32), ("max_threads", C.c_uint32), ("order", C.c_int32),
+                ("seed", C.c_uint64)]
+
+
+class Launch(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("block", C.c_uint32), ("waves_per_cu", C.c_uint32),
+                ("walk_budget", C.c_uint32), ("cluster_words", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class PbmError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} -> {ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "pbm_create": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "pbm_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_reset": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pbm_prefill_bernoulli": (C.c_int, [C.c_void_p, C.c_double, C.c_uint64, C.c_void_p]),
+    "pbm_prefill_requests": (C.c_int, [C.c_void_p, C.c_float, C.c_void_p]),
+    "pbm_count_used": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
+    "pbm_free_page_percentage": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.c_void_p]),
+    "pbm_snapshot_bitmap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "pbm_restore_bitmap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "pbm_read_free_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+    "pbm_get_ctx": (C.c_int, [C.c_void_p, C.POINTER(Ctx)]),
+    "pbm_read_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters), C.c_void_p]),
+    "pbm_clear_counters": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pbm_get_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                                C.POINTER(Launch), C.c_void_p]),
+    "pbm_free_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
+                                 C.c_void_p]),
+    "pbm_graph_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                                   C.POINTER(Launch), C.POINTER(Launch), C.c_uint32,
+                                   C.POINTER(C.c_void_p)]),
+    "pbm_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pbm_graph_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_atomic_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
+    "pbm_last_error": (C.c_char_p, []),
+    "pbm_version": (C.c_char_p, []),
+}
+
+
+def exported_names() -> list[str]:
+    return sorted(_SIGS)
+
+
+def lib() -> C.CDLL:
+    """The loaded libpbm.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(there is no non-HIP implementation)")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code: int, where: str) -> None:
+    if code != 0:
+        raise PbmError(code, where, lib().pbm_last_error().decode())
+
+
+def launch(mode: int = MODE_PLAIN, block: int = 0, waves_per_cu: int = 0, walk_budget: int = 0,
+           cluster_words: int = 0, flags: int = 0) -> Launch:
+    return Launch(mode, block, waves_per_cu, walk_budget, cluster_words, flags)
+
+
+def _ptr(t) -> C.c_void_p:
+    """Device pointer of a torch tensor (or an int address, or None)."""
+    if t is None:
+        return C.c_void_p(0)
+    if isinstance(t, int):
+        return C.c_void_p(t)
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream(stream) -> C.c_void_p:
+    if stream is None:
+        return C.c_void_p(0)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+class Pool:
+    """One page pool (a pbm_handle)."""
+
+    def __init__(self, n_pages: int, page_size: int = 4096, strategy: int = RW_BM,
+                 backing: bool = False, seed: int = 42, walk_budget: int = 0,
+                 max_threads: int = 0, order: int = ORDER_PACKED, device: int = 0):
+        self.cfg = Config(n_pages, page_size, strategy, int(backing), seed, walk_budget,
+                          max_threads, order, device)
+        h = C.c_void_p()
+        check(lib().pbm_create(C.byref(self.cfg), C.byref(h)), "pbm_create")
+        self.h = h
+        self.n_pages = n_pages
+        self.n_words = (n_pages + 63) // 64
+
+    def close(self) -> None:
+        if self.h:
+            check(lib().pbm_destroy(self.h), "pbm_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state
+    def reset(self, stream=None) -> None:
+        check(lib().pbm_reset(self.h, _stream(stream)), "pbm_reset")
+
+    def prefill_bernoulli(self, fill: float, seed: int, stream=None) -> None:
+        check(lib().pbm_prefill_bernoulli(self.h, fill, seed, _stream(stream)),
+              "pbm_prefill_bernoulli")
+
+    def prefill_requests(self, free_percentage: float, stream=None) -> None:
+        check(lib().pbm_prefill_requests(self.h, free_percentage, _stream(stream)),
+              "pbm_prefill_requests")
+
+    def count_used(self, stream=None) -> int:
+        v = C.c_uint64()
+        check(lib().pbm_count_used(self.h, C.byref(v), _stream(stream)), "pbm_count_used")
+        return int(v.value)
+
+    def free_page_percentage(self, stream=None) -> float:
+        v = C.c_float()
+        check(lib().pbm_free_page_percentage(self.h, C.byref(v), _stream(stream)),
+              "pbm_free_page_percentage")
+        return float(v.value)
+
+    def snapshot(self, stream=None) -> np.ndarray:
+        out = np.empty(self.n_words, dtype=np.uint64)
+        check(lib().pbm_snapshot_bitmap(self.h, out.ctypes.data_as(C.c_void_p), self.n_words,
+                                        _stream(stream)), "pbm_snapshot_bitmap")
+        return out
+
+    def restore(self, words: np.ndarray, stream=None) -> None:
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        check(lib().pbm_restore_bitmap(self.h, w.ctypes.data_as(C.c_void_p), w.size,
+                                       _stream(stream)), "pbm_restore_bitmap")
+
+    def free_count(self, stream=None) -> int:
+        v = C.c_int64()
+        check(lib().pbm_read_free_count(self.h, C.byref(v), _stream(stream)),
+              "pbm_read_free_count")
+        return int(v.value)
+
+    def ctx(self) -> Ctx:
+        c = Ctx()
+        check(lib().pbm_get_ctx(self.h, C.byref(c)), "pbm_get_ctx")
+        return c
+
+    def counters(self, stream=None) -> dict:
+        c = Counters()
+        check(lib().pbm_read_counters(self.h, C.byref(c), _stream(stream)), "pbm_read_counters")
+        return c.as_dict()
+
+    def clear_counters(self, stream=None) -> None:
+        check(lib().pbm_clear_counters(self.h, _stream(stream)), "pbm_clear_counters")
+
+    # ---- batch entry points (device pointers)
+    def get_pages(self, n: int, ids, steps=None, req_base: int = 0, lnch: Launch | None = None,
+                  stream=None) -> None:
+        check(lib().pbm_get_pages(self.h, n, req_base, _ptr(ids), _ptr(steps),
+                                  C.byref(lnch or launch()), _stream(stream)), "pbm_get_pages")
+
+    def free_pages(self, n: int, ids, lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_free_pages(self.h, n, _ptr(ids), C.byref(lnch or launch()),
+                                   _stream(stream)), "pbm_free_pages")
+
+
+class Graph:
+    """A captured allocation and/or free phase (pbm_graph)."""
+
+    def __init__(self, pool: Pool, n: int, ids, steps=None, req_base: int = 0,
+                 get_launch: Launch | None = None, free_launch: Launch | None = None,
+                 phases: int = PHASE_ALLOC | PHASE_FREE):
+        g = C.c_void_p()
+        check(lib().pbm_graph_create(pool.h, n, req_base, _ptr(ids), _ptr(steps),
+                                     C.byref(get_launch or launch()),
+                                     C.byref(free_launch or launch()), phases, C.byref(g)),
+              "pbm_graph_create")
+        self.g = g
+        self.pool = pool  # keeps the pool alive while the graph exists
+
+    def launch(self, stream=None) -> None:
+        check(lib().pbm_graph_launch(self.g, _stream(stream)), "pbm_graph_launch")
+
+    def close(self) -> None:
+        if self.g:
+            check(lib().pbm_graph_destroy(self.g), "pbm_graph_destroy")
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def atomic_peak(words: int = 16384, rounds: int = 64, device: int = 0) -> float:
+    """Device-scope u64 atomicOr rate (ops/s) over `words` spread words."""
+    v = C.c_double()
+    check(lib().pbm_atomic_peak(words, rounds, device, C.byref(v)), "pbm_atomic_peak")
+    return float(v.value)

@@ -1,0 +1,258 @@
+"""Parity of the HIP allocators with the serial oracle, through the C-ABI.
+
+Serial schedules (PBM_MODE_SERIAL, PBM_LAUNCH_ONE_WAVE) must reproduce the
+oracle's page ids and step counts bit for bit.  Concurrent schedules are
+checked on what does not depend on the schedule: every granted page unique
+and free before, the map after the alloc phase equal to the map before plus
+exactly the granted bits, grants = min(requests, free pages), and the map
+after freeing everything equal to the map before.
+"""
+import numpy as np
+import pytest
+
+import pbm
+import pbm_oracle as orc
+from conftest import log_case
+
+pytestmark = pytest.mark.gpu
+
+BIG = 1 << 20
+
+
+def _bufs(torch, n):
+    ids = torch.full((max(n, 1),), -7, dtype=torch.int32, device="cuda")
+    steps = torch.zeros((max(n, 1),), dtype=torch.int32, device="cuda")
+    return ids, steps
+
+
+def _get(torch, pool, n, mode, req_base=0, cw=0, flags=0, budget=0):
+    ids, steps = _bufs(torch, n)
+    pool.get_pages(n, ids, steps, req_base,
+                   pbm.launch(mode=mode, cluster_words=cw, flags=flags, walk_budget=budget))
+    torch.cuda.synchronize()
+    return ids.cpu().numpy()[:n], steps.cpu().numpy().view(np.uint32)[:n]
+
+
+def _free(torch, pool, ids_np, mode):
+    ids = torch.from_numpy(np.ascontiguousarray(ids_np, dtype=np.int32)).cuda()
+    pool.free_pages(len(ids_np), ids, pbm.launch(mode=mode))
+    torch.cuda.synchronize()
+
+
+# ------------------------------------------------------------------ prefill
+
+@pytest.mark.parametrize("T,fill,seed", [(65536, 0.0, 42), (1000, 0.5, 3), (BIG, 0.9, 42),
+                                         (BIG, 0.99, 7), (4096, 1.0, 1)])
+def test_prefill_matches_oracle(gpu, T, fill, seed):
+    log_case(f"prefill T={T} fill={fill} seed={seed}")
+    with pbm.Pool(T) as pool:
+        pool.prefill_bernoulli(fill, seed)
+        got = pool.snapshot()
+        exp = orc.prefill(T, fill, seed)
+        assert np.array_equal(got, exp)
+        assert pool.free_count() == orc.count_free(exp, T)
+        assert pool.count_used() == T - orc.count_free(exp, T)
+
+
+# ------------------------------------------------------------------ serial = oracle
+
+@pytest.mark.parametrize("T,fill,n", [(65536, 0.0, 32768), (1 << 16, 0.9, 8000),
+                                      (1000, 0.3, 900)])
+def test_serial_rwbm_matches_oracle(gpu, T, fill, n):
+    log_case(f"serial rwbm T={T} fill={fill} n={n}")
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=42) as pool:
+        pool.prefill_bernoulli(fill, 42)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_SERIAL, req_base=5)
+        st = orc.State(orc.prefill(T, fill, 42), T)
+        eids, esteps = st.rwbm(n, 5, 42)
+        assert np.array_equal(ids, eids)
+        assert np.array_equal(steps, esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.free_count() == st.free[0]
+
+
+@pytest.mark.parametrize("order", [pbm.ORDER_PACKED, pbm.ORDER_REFERENCE])
+@pytest.mark.parametrize("T,fill,n", [(1 << 16, 0.9, 8000), (1 << 14, 0.99, 500),
+                                      (64 * 40, 0.5, 1500)])
+def test_serial_collab_matches_oracle(gpu, order, T, fill, n):
+    log_case(f"serial collab order={order} T={T} fill={fill} n={n}")
+    with pbm.Pool(T, strategy=pbm.COLLAB_RW_BM, seed=11, order=order) as pool:
+        pool.prefill_bernoulli(fill, 3)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_SERIAL, req_base=64)
+        st = orc.State(orc.prefill(T, fill, 3), T)
+        eids, esteps = st.collab(n, 64, 11, order)
+        assert np.array_equal(ids, eids)
+        assert np.array_equal(steps, esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+
+
+@pytest.mark.parametrize("cw", [64, 128, 256])
+@pytest.mark.parametrize("order", [pbm.ORDER_PACKED, pbm.ORDER_REFERENCE])
+def test_one_wave_batched_matches_oracle(gpu, cw, order):
+    T, fill, n = 1 << 16, 0.9, 8000
+    log_case(f"one-wave batched cw={cw} order={order}")
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=9, order=order) as pool:
+        pool.prefill_bernoulli(fill, 42)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, req_base=3, cw=cw,
+                          flags=pbm.LAUNCH_ONE_WAVE)
+        st = orc.State(orc.prefill(T, fill, 42), T)
+        eids, esteps = st.batched(n, 3, 9, order, cw)
+        assert np.array_equal(ids, eids)
+        assert np.array_equal(steps, esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.free_count() == st.free[0]
+
+
+def test_one_wave_batched_budget_sweep_matches_oracle(gpu):
+    # walk budget 1: after one random window the wave sweeps linearly
+    T, n = 1 << 14, 600
+    log_case("one-wave batched budget=1 sweep")
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=2) as pool:
+        pool.prefill_bernoulli(0.99, 5)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, cw=64, flags=pbm.LAUNCH_ONE_WAVE,
+                          budget=1)
+        st = orc.State(orc.prefill(T, 0.99, 5), T)
+        eids, esteps = st.batched(n, 0, 2, 0, 64, budget=1)
+        assert np.array_equal(ids, eids) and np.array_equal(steps, esteps)
+
+
+# ------------------------------------------------------------------ concurrent invariants
+
+def _check_cycle(torch, pool, T, before, n, get_mode, free_mode, cw=0, expect_prefix=False):
+    A = orc.count_free(before, T)
+    ids, steps = _get(torch, pool, n, get_mode, cw=cw)
+    after = pool.snapshot()
+    rc, g = orc.check_alloc(before, after, T, ids)
+    assert rc == 0, f"invariant {rc} violated"
+    assert g == min(n, A)
+    assert pool.free_count() == A - g
+    if expect_prefix:
+        assert (ids[:g] >= 0).all() and (ids[g:] == -1).all()
+    _free(torch, pool, ids, free_mode)
+    assert np.array_equal(pool.snapshot(), before)
+    assert pool.free_count() == A
+    return ids, steps
+
+
+def test_c1_plain_rwbm_cycle(gpu):
+    # 65,536 pages empty, 32,768 threads each getPage then freePage
+    T, n = 65536, 32768
+    log_case("C1 plain rwbm")
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=42) as pool:
+        pool.clear_counters()
+        ids, steps = _check_cycle(gpu, pool, T, orc.cleared(T), n, pbm.MODE_PLAIN,
+                                  pbm.MODE_PLAIN)
+        c = pool.counters()
+        assert c["allocs"] == n and c["frees"] == n and c["failures"] == 0
+        assert c["errors"] == 0
+        assert c["steps"] == int(steps.sum())
+
+
+@pytest.mark.parametrize("fill", [0.5, 0.9, 0.99])
+@pytest.mark.parametrize("kind", ["plain_rwbm", "plain_collab", "batched64", "batched128",
+                                  "batched256"])
+def test_c2_cycle_invariants(gpu, fill, kind):
+    # 1,048,576 pages, Bernoulli pre-fill, 262,144 requests then frees
+    T, n = BIG, 262144
+    log_case(f"C2 {kind} fill={fill}")
+    strat = pbm.COLLAB_RW_BM if kind == "plain_collab" else pbm.RW_BM
+    with pbm.Pool(T, strategy=strat, seed=42) as pool:
+        pool.prefill_bernoulli(fill, 42)
+        before = orc.prefill(T, fill, 42)
+        if kind.startswith("batched"):
+            _check_cycle(gpu, pool, T, before, n, pbm.MODE_BATCHED, pbm.MODE_BATCHED,
+                         cw=int(kind[7:]), expect_prefix=True)
+        else:
+            _check_cycle(gpu, pool, T, before, n, pbm.MODE_PLAIN, pbm.MODE_PLAIN)
+        c = pool.counters()
+        A = orc.count_free(before, T)
+        assert c["allocs"] == min(n, A) and c["failures"] == n - min(n, A)
+        assert c["frees"] == min(n, A) and c["errors"] == 0
+
+
+def test_repeated_batched_cycles_restore_the_map(gpu):
+    T, n = BIG, 262144
+    log_case("C2 batched x5 cycles")
+    with pbm.Pool(T, seed=1) as pool:
+        pool.prefill_bernoulli(0.9, 42)
+        before = orc.prefill(T, 0.9, 42)
+        for k in range(5):
+            ids, _ = _get(gpu, pool, n, pbm.MODE_BATCHED, req_base=k * n)
+            assert orc.check_alloc(before, pool.snapshot(), T, ids)[0] == 0
+            _free(gpu, pool, ids, pbm.MODE_BATCHED)
+            assert np.array_equal(pool.snapshot(), before)
+
+
+# ------------------------------------------------------------------ edge cases
+
+def test_zero_requests_is_a_no_op(gpu):
+    with pbm.Pool(4096) as pool:
+        pool.get_pages(0, None, None)
+        pool.free_pages(0, None)
+        assert pool.count_used() == 0
+
+
+@pytest.mark.parametrize("mode", [pbm.MODE_PLAIN, pbm.MODE_SERIAL, pbm.MODE_BATCHED])
+def test_one_page_pool(gpu, mode):
+    log_case(f"one-page pool mode={mode}")
+    with pbm.Pool(1, seed=3) as pool:
+        ids, _ = _get(gpu, pool, 5, mode)
+        assert sorted(ids.tolist()) == [-1, -1, -1, -1, 0]
+        assert pool.count_used() == 1
+
+
+@pytest.mark.parametrize("mode", [pbm.MODE_PLAIN, pbm.MODE_BATCHED])
+def test_full_pool_fails_every_request(gpu, mode):
+    with pbm.Pool(65536) as pool:
+        pool.prefill_bernoulli(1.0, 1)
+        pool.clear_counters()
+        ids, steps = _get(gpu, pool, 10000, mode)
+        assert (ids == -1).all() and (steps == 0).all()
+        assert pool.counters()["failures"] == 10000
+
+
+@pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.COLLAB_RW_BM])
+def test_padding_pages_are_never_granted(gpu, strategy):
+    T = 1000  # last word holds 40 padding bits
+    with pbm.Pool(T, strategy=strategy, seed=4) as pool:
+        ids, _ = _get(gpu, pool, 1200, pbm.MODE_PLAIN)
+        got = ids[ids >= 0]
+        assert len(got) == T and len(set(got.tolist())) == T and got.max() < T
+        assert pool.free_count() == 0
+
+
+def test_double_free_and_bad_ids_count_errors(gpu):
+    T = 4096
+    with pbm.Pool(T) as pool:
+        ids, _ = _get(gpu, pool, 100, pbm.MODE_PLAIN)
+        _free(gpu, pool, ids, pbm.MODE_PLAIN)
+        pool.clear_counters()
+        before = pool.snapshot()
+        _free(gpu, pool, ids[:10], pbm.MODE_PLAIN)                  # double free
+        _free(gpu, pool, np.array([T, T + 5, 1 << 30]), pbm.MODE_PLAIN)  # out of range
+        _free(gpu, pool, ids[10:20], pbm.MODE_BATCHED)
+        assert np.array_equal(pool.snapshot(), before)
+        c = pool.counters()
+        assert c["errors"] == 23 and c["frees"] == 0
+        assert pool.free_count() == T
+
+
+def test_reference_style_prefill_requests(gpu):
+    T = 1 << 16
+    with pbm.Pool(T, seed=8) as pool:
+        pool.prefill_requests(0.1)
+        # (int)((1-0.1f)*T) pages requested and kept, as prefillBuffer does
+        used = int(np.float32(1.0 - np.float32(0.1)) * np.float32(T))
+        assert pool.count_used() == used
+        assert abs(pool.free_page_percentage() - (T - used) / T) < 1e-6
+
+
+def test_snapshot_restore_round_trip(gpu):
+    T = 1 << 16
+    with pbm.Pool(T) as pool:
+        bm = orc.prefill(T, 0.7, 99)
+        pool.restore(bm)
+        assert np.array_equal(pool.snapshot(), bm)
+        assert pool.free_count() == orc.count_free(bm, T)
+        pool.reset()
+        assert np.array_equal(pool.snapshot(), orc.cleared(T))

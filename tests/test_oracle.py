@@ -1,0 +1,409 @@
+"""Pins the serial CPU oracle against checks that do not reuse its code:
+library routines (glibc random_r, numpy fmod), the published splitmix64
+sequence, hand-derived known answers on tiny pools, planted violations, and
+the closed-form expected walk length of theoretical_models.py:34-41."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import pbm_oracle as orc
+
+M64 = (1 << 64) - 1
+
+
+# ---------------------------------------------------------------- LCG
+
+def _glibc_type0(seed: int, n: int) -> list[int]:
+    """glibc random_r with an 8-byte state is TYPE_0: x' = (x*1103515245 +
+    12345) & 0x7fffffff — the reference LCG for non-negative x."""
+    libc = C.CDLL("libc.so.6")
+    state = C.create_string_buffer(64)
+    data = C.create_string_buffer(64)
+    assert libc.initstate_r(C.c_uint(seed), state, C.c_size_t(8), data) == 0
+    out, res = [], C.c_int32()
+    for _ in range(n):
+        assert libc.random_r(data, C.byref(res)) == 0
+        out.append(res.value)
+    return out
+
+
+@pytest.mark.parametrize("seed", [42, 1, 12345, 2**31 - 1, 987654321])
+def test_lcg_matches_glibc_random_r(seed):
+    ref = _glibc_type0(seed, 2000)
+    x, mine = seed, []
+    for _ in range(2000):
+        x = orc.lcg(x)
+        mine.append(x)
+    assert mine == ref
+
+
+def test_lcg_negative_inputs_truncate_like_c():
+    rng = np.random.default_rng(5)
+    xs = rng.integers(-(2**31), 0, size=5000, dtype=np.int64)
+    v = np.int64(1103515245) * xs + np.int64(12345)
+    expect = np.fmod(v, np.int64(2**31))  # C truncating remainder
+    got = np.array([orc.lcg(int(x)) for x in xs], dtype=np.int64)
+    assert np.array_equal(got, expect)
+    assert (got <= 0).all()
+
+
+def test_lcg_hand_values():
+    assert orc.lcg(0) == 12345
+    assert orc.lcg(1) == 1103527590
+    assert orc.lcg(-1) == -1103502900
+    # 1103515245 * -2^31 is a multiple of 2^31, leaving 12345 - 2^31
+    assert orc.lcg(-(2**31)) == 12345 - 2**31
+
+
+# ---------------------------------------------------------------- hashes
+
+def test_splitmix64_published_sequence():
+    # splitmix64 seeded with 1234567: first five outputs as published with
+    # Vigna's reference implementation.
+    expect = [6457827717110365317, 3203168211198807973, 9817491932198370423,
+              4593380528125082431, 16408922859458223821]
+    got = [orc.mix64((1234567 + (i + 1) * 0x9E3779B97F4A7C15) & M64) for i in range(5)]
+    assert got == expect
+
+
+def _unmix64(z: int) -> int:
+    def unxorshift(y, s):
+        x = y
+        for _ in range(64 // s + 1):
+            x = y ^ (x >> s)
+        return x & M64
+    z = unxorshift(z, 31)
+    z = (z * pow(0x94D049BB133111EB, -1, 1 << 64)) & M64
+    z = unxorshift(z, 27)
+    z = (z * pow(0xBF58476D1CE4E5B9, -1, 1 << 64)) & M64
+    return unxorshift(z, 30)
+
+
+def test_mix64_is_a_bijection_and_avalanches():
+    rng = np.random.default_rng(1)
+    flips = []
+    for x in rng.integers(0, 2**63, size=300, dtype=np.int64):
+        x = int(x) * 2 + 1
+        y = orc.mix64(x)
+        assert _unmix64(y) == x
+        b = int(rng.integers(0, 64))
+        flips.append(bin(y ^ orc.mix64(x ^ (1 << b))).count("1"))
+    assert 30.0 < float(np.mean(flips)) < 34.0
+
+
+def test_request_seed_is_high_half_of_splitmix_output():
+    for r in (0, 1, 7, 2**20):
+        z = orc.mix64((42 + (r + 1) * 0x9E3779B97F4A7C15) & M64)
+        hi = z >> 32
+        assert orc.request_seed(42, r) == (hi - (1 << 32) if hi >= (1 << 31) else hi)
+
+
+# ---------------------------------------------------------------- pre-fill
+
+@pytest.mark.parametrize("fill", [0.5, 0.9, 0.99])
+def test_prefill_occupancy_within_binomial_bounds(fill):
+    T = 1 << 20
+    bm = orc.prefill(T, fill, 42)
+    used = T - orc.count_free(bm, T)
+    sd = math.sqrt(T * fill * (1 - fill))
+    assert abs(used - fill * T) < 5 * sd
+
+
+def test_prefill_extremes_and_padding():
+    T = 1000  # 16 words, last one has 40 padding bits
+    assert orc.count_free(orc.prefill(T, 0.0, 3), T) == T
+    assert orc.count_free(orc.prefill(T, 1.0, 3), T) == 0
+    bm = orc.cleared(T)
+    assert int(bm[-1]) == (M64 ^ ((1 << 40) - 1))
+    assert orc.count_free(bm, T) == T
+
+
+def test_prefill_bit_is_hash_threshold():
+    T, fill, seed = 4096, 0.3, 9
+    bm = orc.prefill(T, fill, seed)
+    thr = int(math.ldexp(fill, 64))
+    for p in range(0, T, 37):
+        used = (int(bm[p // 64]) >> (p % 64)) & 1
+        assert used == (orc.page_hash(seed, p) < thr)
+
+
+# ---------------------------------------------------------------- RW_BM
+
+def test_rwbm_single_request_empty_pool_takes_lowest_bit_of_random_word():
+    # known answer: on an empty pool the first jump always succeeds, with
+    # the lowest bit of word (uint32)lcg(seed) % L
+    T, seed = 64 * 50, 42
+    L = T // 64
+    st = orc.State(orc.cleared(T), T)
+    ids, steps = st.rwbm(1, 0, seed)
+    s = orc.lcg(orc.request_seed(seed, 0))
+    assert ids[0] == ((s & 0xFFFFFFFF) % L) * 64
+    assert steps[0] == 1
+
+
+def test_rwbm_serial_known_answer_by_hand():
+    # Replays the walk by hand: request r jumps through lcg^k(seed_r); each
+    # jump takes the lowest free bit of its word if any.
+    T, seed, n = 64 * 3, 7, 150
+    L = T // 64
+    bm = [0] * L
+    free = T
+    exp_ids, exp_steps = [], []
+    for r in range(n):
+        if free == 0:
+            exp_ids.append(-1); exp_steps.append(0); continue
+        free -= 1
+        s, k = orc.request_seed(seed, r), 0
+        while True:
+            k += 1
+            s = orc.lcg(s)
+            g = (s & 0xFFFFFFFF) % L
+            if bm[g] != M64:
+                b = ((~bm[g]) & -(~bm[g])).bit_length() - 1
+                bm[g] |= 1 << b
+                exp_ids.append(g * 64 + b); exp_steps.append(k); break
+    st = orc.State(orc.cleared(T), T)
+    ids, steps = st.rwbm(n, 0, seed)
+    assert ids.tolist() == exp_ids
+    assert steps.tolist() == exp_steps
+
+
+def test_rwbm_budget_falls_back_to_sweep():
+    # budget 1 on a pool with one free page: one jump, then a sweep
+    T = 64 * 8
+    bm = orc.prefill(T, 1.0, 1)
+    bm[5] &= ~np.uint64(1 << 17)
+    st = orc.State(bm, T)
+    ids, steps = st.rwbm(3, 0, 11, budget=1)
+    assert ids.tolist() == [5 * 64 + 17, -1, -1]
+    assert steps[0] >= 2 and steps[1] == 0
+
+
+def test_rwbm_invariants_and_failures_exact():
+    T, n = 1 << 14, 6000
+    before = orc.prefill(T, 0.9, 42)
+    A = orc.count_free(before, T)
+    st = orc.State(before, T)
+    ids, _ = st.rwbm(n, 0, 42)
+    rc, g = orc.check_alloc(before, st.bm, T, ids)
+    assert rc == 0
+    assert g == min(n, A)
+    assert (ids[:g] >= 0).all() and (ids[g:] == -1).all()  # serial: first A by index
+    st.free_ids(ids)
+    assert np.array_equal(st.bm, before)
+    assert st.free[0] == A
+
+
+def _conditional_mean_probes(before: np.ndarray, ids: np.ndarray) -> float:
+    """Each jump picks a uniform word, so grant k needs L/NF_k jumps on
+    average, NF_k = words not full before grant k (replayed from the ids)."""
+    bm = [int(w) for w in before]
+    L = len(bm)
+    nf = sum(1 for w in bm if w != M64)
+    tot = 0.0
+    for i in ids:
+        tot += L / nf
+        w, b = int(i) // 64, int(i) % 64
+        bm[w] |= 1 << b
+        if bm[w] == M64:
+            nf -= 1
+    return tot / len(ids)
+
+
+@pytest.mark.parametrize("fill", [0.5, 0.9, 0.99])
+def test_rwbm_mean_steps_match_conditional_expectation(fill):
+    # 15,625 words: (uint32)lcg % L mixes all 32 bits of the LCG state, so
+    # jumps behave as uniform draws
+    T = 1_000_000
+    bm = orc.prefill(T, fill, 42)
+    A = orc.count_free(bm, T)
+    n = min(20000, A // 2)
+    st = orc.State(bm, T)
+    ids, steps = st.rwbm(n, 0, 42)
+    exp = _conditional_mean_probes(bm, ids)
+    assert abs(steps.mean() - exp) / exp < 0.03, (steps.mean(), exp)
+
+
+def test_rwbm_power_of_two_map_walks_a_single_lcg_cycle():
+    # With L = 2^14 words, (uint32)lcg(x) % L keeps the low 14 bits, which
+    # follow x' = a*x + c mod 2^14: one full-period cycle shared by every
+    # request.  Walks then probe linearly along that cycle and cluster, so
+    # they take clearly more jumps than uniform draws would.
+    T = 1 << 20
+    L = T // 64
+    x, seen = 12345, set()
+    for _ in range(L):
+        x = orc.lcg(x)
+        seen.add((x & 0xFFFFFFFF) % L)
+    assert len(seen) == L  # a permutation of all words
+    bm = orc.prefill(T, 0.99, 42)
+    A = orc.count_free(bm, T)
+    st = orc.State(bm, T)
+    ids, steps = st.rwbm(A // 2, 0, 42)
+    assert steps.mean() > 1.15 * _conditional_mean_probes(bm, ids)
+
+
+@pytest.mark.parametrize("fill", [0.5, 0.9])
+def test_rwbm_mean_steps_match_closed_form(fill):
+    # RWBM_TAS assumes free bits stay independent, which holds while few
+    # words have been drained; at 99% fill the walk empties one-free-bit
+    # words faster than the model assumes and the model underestimates.
+    T = 1 << 20
+    bm = orc.prefill(T, fill, 42)
+    A = orc.count_free(bm, T)
+    n = min(20000, A // 2)
+    st = orc.State(bm, T)
+    _, steps = st.rwbm(n, 0, 42)
+    exp = orc.expected_steps(T, A, n, 64)
+    assert abs(steps.mean() - exp) / exp < 0.02, (steps.mean(), exp)
+
+
+def test_expected_steps_closed_form_values():
+    # 1/(1-p^W) at a single grant; the W=32 and W=64 values of the table
+    # for 90% and 99% fill
+    T = 1 << 20
+    for p, w, e in [(0.9, 32, 1.0356), (0.9, 64, 1.0012), (0.99, 32, 3.6361),
+                    (0.99, 64, 2.1079)]:
+        A = T * (1 - p)
+        assert abs(orc.expected_steps(T, A, 1, w) - e) < 1e-3
+
+
+# ---------------------------------------------------------------- CollabRW
+
+def _window_fixture():
+    """Three words with free bits {0,3}, {1}, {2,5,6}, all others used."""
+    T = 64 * 64
+    bm = orc.prefill(T, 1.0, 0)
+    frees = {0: [0, 3], 1: [1], 2: [2, 5, 6]}
+    return T, bm, frees
+
+
+def _force_window(T, seed_wanted_start):
+    """A request seed whose first window starts at word 0."""
+    L = T // 64
+    for s0 in range(1, 1 << 22):
+        if (orc.lcg(s0) & 0xFFFFFFFF) % L == seed_wanted_start:
+            return s0
+    raise AssertionError
+
+
+@pytest.mark.parametrize("order,expect", [
+    # packed: lane 0's free bits first, then lane 1's, ...
+    (0, [0, 3, 64 + 1, 128 + 2, 128 + 5, 128 + 6]),
+    # reference round-robin: lowest bit of each has-lane, then second bits
+    (1, [0, 64 + 1, 128 + 2, 3, 128 + 5, 128 + 6]),
+])
+def test_collab_window_assignment_known_answer(order, expect):
+    T, bm, frees = _window_fixture()
+    for w, bits in frees.items():
+        for b in bits:
+            bm[w] &= ~np.uint64(1 << b)
+    # find a pool seed whose chunk-0 request seed lands the first window on
+    # word 0, by searching seeds of the orc_request_seed derivation
+    L = T // 64
+    seed = None
+    for cand in range(100000):
+        if (orc.lcg(orc.request_seed(cand, 0)) & 0xFFFFFFFF) % L == 0:
+            seed = cand
+            break
+    assert seed is not None
+    st = orc.State(bm, T)
+    ids, steps = st.collab(6, 0, seed, order)
+    assert ids.tolist() == expect
+    assert steps.tolist() == [1] * 6
+
+
+def test_collab_reference_order_matches_round_robin_brute_force():
+    # random windows: the reference loop must equal the round-robin
+    # enumeration (round j hands out the j-th free bit of every word that
+    # has more than j free bits, in lane order)
+    rng = np.random.default_rng(3)
+    T = 64 * 64
+    L = T // 64
+    for trial in range(20):
+        bm = orc.prefill(T, float(rng.uniform(0.7, 0.98)), trial)
+        seed = int(rng.integers(0, 1 << 30))
+        start = (orc.lcg(orc.request_seed(seed, 0)) & 0xFFFFFFFF) % L
+        k = int(rng.integers(1, 65))
+        # only the k active lanes probe (the reference's __activemask())
+        free = [[b for b in range(64) if not (int(bm[(start + l) % L]) >> b) & 1]
+                for l in range(k)]
+        rr = []
+        for j in range(64):
+            for l in range(k):
+                if len(free[l]) > j:
+                    rr.append(((start + l) % L) * 64 + free[l][j])
+        st = orc.State(bm, T)
+        ids, _ = st.collab(k, 0, seed, 1)
+        if len(rr) >= k:
+            assert ids.tolist() == rr[:k]
+
+
+@pytest.mark.parametrize("order", [0, 1])
+def test_collab_invariants(order):
+    T, n = 1 << 14, 5000
+    before = orc.prefill(T, 0.95, 8)
+    A = orc.count_free(before, T)
+    st = orc.State(before, T)
+    ids, steps = st.collab(n, 0, 8, order)
+    rc, g = orc.check_alloc(before, st.bm, T, ids)
+    assert rc == 0 and g == min(n, A)
+    st.free_ids(ids)
+    assert np.array_equal(st.bm, before)
+
+
+# ---------------------------------------------------------------- batched
+
+@pytest.mark.parametrize("cw", [64, 128, 256])
+@pytest.mark.parametrize("order", [0, 1])
+def test_batched_phase_admission_and_invariants(cw, order):
+    T, n = 1 << 15, 9000
+    before = orc.prefill(T, 0.9, 42)
+    A = orc.count_free(before, T)
+    st = orc.State(before, T)
+    ids, steps = st.batched(n, 0, 42, order, cw)
+    rc, g = orc.check_alloc(before, st.bm, T, ids)
+    assert rc == 0 and g == min(n, A)
+    # admission is by index: exactly the first min(n, A) requests succeed
+    assert (ids[:g] >= 0).all() and (ids[g:] == -1).all()
+    assert st.free[0] == A - g
+    st.free_ids(ids)
+    assert np.array_equal(st.bm, before) and st.free[0] == A
+
+
+def test_batched_keeps_window_across_chunks():
+    # an empty pool: chunk 0 takes window w's 64 bits... every 64 requests
+    # fill exactly one word in packed order, so the window is exhausted only
+    # after 64 chunks and ids are consecutive within each word
+    T = 64 * 256
+    st = orc.State(orc.cleared(T), T)
+    ids, steps = st.batched(64 * 3, 0, 5, 0, 64)
+    w0 = ids[0] // 64
+    assert ids[:64].tolist() == [w0 * 64 + b for b in range(64)]
+    assert ids[64] // 64 == (w0 + 1) % (T // 64)
+    assert steps.tolist() == [1] * (64 * 3)
+
+
+# ---------------------------------------------------------------- checker
+
+def test_check_alloc_flags_planted_violations():
+    T = 64 * 4
+    before = orc.prefill(T, 0.5, 2)
+    st = orc.State(before, T)
+    ids, _ = st.rwbm(20, 0, 2)
+    assert orc.check_alloc(before, st.bm, T, ids)[0] == 0
+    dup = ids.copy(); dup[1] = dup[0]
+    assert orc.check_alloc(before, st.bm, T, dup)[0] == 3
+    used = int(np.flatnonzero([(int(before[p // 64]) >> (p % 64)) & 1 for p in range(T)])[0])
+    bad = ids.copy(); bad[0] = used
+    assert orc.check_alloc(before, st.bm, T, bad)[0] == 2
+    lost = st.bm.copy(); lost[ids[0] // 64] &= ~np.uint64(1 << (ids[0] % 64))
+    assert orc.check_alloc(before, lost, T, ids)[0] == 4
+    extra = st.bm.copy()
+    freebits = [p for p in range(T) if not (int(extra[p // 64]) >> (p % 64)) & 1]
+    extra[freebits[0] // 64] |= np.uint64(1 << (freebits[0] % 64))
+    assert orc.check_alloc(before, extra, T, ids)[0] == 5
+    oob = ids.copy(); oob[0] = T
+    assert orc.check_alloc(before, st.bm, T, oob)[0] == 1
