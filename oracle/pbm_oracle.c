@@ -293,19 +293,27 @@ uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, 
     return granted_total;
 }
 
+#define SWEEP_WORDS 16u
+
+/* Batched persistent allocation on one wave (pbm_alloc.hip k_get_batched):
+ * window mode makes up to `budget` random jumps per chunk to windows of
+ * 64*J words, kept across chunks while they show free bits; after the
+ * budget the wave sweeps for good: lane l scans words start+16l .. +15,
+ * offers its first non-full word, and the sweep advances by 1024 words when
+ * no lane saw a second non-full word. */
 uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                             uint64_t req_base, uint64_t seed, uint32_t budget, int order,
                             uint32_t cluster_words, int32_t* ids, uint32_t* steps) {
     const uint32_t L = orc_n_words(n_pages);
     const uint32_t J = cluster_words / 64u;
-    const uint32_t span = 64u * J;
-    const uint32_t max_sweeps = 2u * ((L + span - 1u) / span) + 2u;
+    const uint32_t sweep_span = 64u * SWEEP_WORDS;
+    const uint32_t max_sweeps = 2u * ((L + sweep_span - 1u) / sweep_span) + 2u;
     /* phase admission: the first min(n, free) requests by index */
     int64_t old = *free_count;
     int64_t granted = old < 0 ? 0 : (old > (int64_t)n ? (int64_t)n : old);
     *free_count -= granted;
     int32_t wseed = orc_request_seed(seed ^ WAVE_SALT, req_base + 0);
-    int have = 0;
+    int have = 0, sweep = 0;
     uint32_t start = 0, got = 0;
     for (uint32_t ch = 0; ch * 64u < n; ++ch) {
         int need[64], valid[64];
@@ -328,32 +336,59 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
                     start = (uint32_t)wseed % L;
                     jumps++;
                 } else {
-                    start = (uint32_t)(((u64)start + span) % L);
-                    if (++sweeps > max_sweeps) break;
+                    sweep = 1;
                 }
                 have = 1;
             }
             for (int l = 0; l < 64; ++l)
                 if (need[l]) cst[l]++;
-            uint32_t seen_all = 0, desig_all = 0;
-            int examined_all = 1;
-            for (uint32_t i = 0; i < J; ++i) {
-                if (!any(need)) {
-                    examined_all = 0;
-                    break;
+            if (!sweep) {
+                uint32_t seen_all = 0, desig_all = 0;
+                int examined_all = 1;
+                for (uint32_t i = 0; i < J; ++i) {
+                    if (!any(need)) {
+                        examined_all = 0;
+                        break;
+                    }
+                    int probe[64];
+                    uint32_t word[64];
+                    for (int l = 0; l < 64; ++l) {
+                        uint32_t off = 64u * i + (uint32_t)l;
+                        probe[l] = off < L;
+                        word[l] = (uint32_t)(((u64)start + off) % L);
+                    }
+                    uint32_t seen;
+                    desig_all += collab_claim(bm, probe, word, need, cid, order, &seen);
+                    seen_all += seen;
                 }
-                int probe[64];
+                if (examined_all && seen_all <= desig_all) have = 0;
+            } else {
+                int probe[64], more = 0;
                 uint32_t word[64];
                 for (int l = 0; l < 64; ++l) {
-                    uint32_t off = 64u * i + (uint32_t)l;
-                    probe[l] = off < L;
-                    word[l] = (uint32_t)(((u64)start + off) % L);
+                    probe[l] = 0;
+                    word[l] = 0;
+                    for (uint32_t i = 0; i < SWEEP_WORDS; ++i) {
+                        uint32_t off = (uint32_t)l * SWEEP_WORDS + i;
+                        if (off >= L) break;
+                        uint32_t idx = start + off;
+                        if (idx >= L) idx -= L;
+                        if (bm[idx] == FULL) continue;
+                        if (!probe[l]) {
+                            probe[l] = 1;
+                            word[l] = idx;
+                        } else {
+                            more = 1;
+                        }
+                    }
                 }
                 uint32_t seen;
-                desig_all += collab_claim(bm, probe, word, need, cid, order, &seen);
-                seen_all += seen;
+                collab_claim(bm, probe, word, need, cid, order, &seen);
+                if (any(need) && !more) {
+                    start = (uint32_t)(((u64)start + sweep_span) % L);
+                    if (++sweeps > max_sweeps) break;
+                }
             }
-            if (examined_all && seen_all <= desig_all) have = 0;
         }
         for (int l = 0; l < 64; ++l) {
             if (need[l]) (*free_count)++;

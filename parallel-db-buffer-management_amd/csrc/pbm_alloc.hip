@@ -59,7 +59,15 @@ __global__ __launch_bounds__(1024) void k_get_collab(pbm_ctx c, uint32_t n, uint
     pbm::flush_stats(c, st);
 }
 
-// Persistent batched allocation over windows of 64*J words.
+// Persistent batched allocation.  Each wave serves chunks of 64 requests.
+// Window mode: up to `budget` random jumps per chunk, each to a window of
+// 64*J consecutive words kept across chunks while it still shows free bits.
+// Sweep mode (after the budget, and for the rest of the wave): every lane
+// scans kSweepWords consecutive words and offers its first non-full one, so
+// one step covers 64*kSweepWords words of the map; the wave advances when
+// no lane has a second non-full word in its segment.
+constexpr uint32_t kSweepWords = 16;
+
 template <int J>
 __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                      int32_t* __restrict__ ids,
@@ -70,15 +78,15 @@ __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t L = c.n_words;
-    const uint32_t span = 64u * J;
-    const uint32_t max_sweeps = 2u * ((L + span - 1u) / span) + 2u;
+    const uint32_t sweep_span = 64u * kSweepWords;
+    const uint32_t max_sweeps = 2u * ((L + sweep_span - 1u) / sweep_span) + 2u;
     const uint32_t n_chunks = (n + 63u) / 64u;
     const long long avail =
         __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const long long granted = avail < 0 ? 0 : (avail > (long long)n ? (long long)n : avail);
     Stats st;
     int32_t wseed = request_seed(c.seed ^ kWaveSalt, req_base + wave);
-    bool have = false;
+    bool have = false, sweep = false;
     uint32_t start = 0;
     for (uint32_t ch = wave; ch < n_chunks; ch += n_waves) {
         const uint32_t r = ch * 64u + lane;
@@ -98,38 +106,72 @@ __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint
                     start = (uint32_t)wseed % L;
                     ++jumps;
                 } else {
-                    if (sweeps == 0) st.fallbacks++;
-                    start = (uint32_t)(((u64)start + span) % L);
-                    if (++sweeps > max_sweeps) break;
+                    st.fallbacks++;
+                    sweep = true;
                 }
                 have = true;
             }
             if (need) ++steps;
-            u64 wv[J];
-            uint32_t wi[J];
+            if (!sweep) {
+                u64 wv[J];
+                uint32_t wi[J];
 #pragma unroll
-            for (int i = 0; i < J; ++i) {
-                const uint32_t off = 64u * (uint32_t)i + lane;
-                wi[i] = (uint32_t)(((u64)start + off) % L);
-                wv[i] = kFull;
-                if (off < L) {
-                    wv[i] = ld_word(&c.bitmap[wi[i]]);
-                    st.probes++;
+                for (int i = 0; i < J; ++i) {
+                    const uint32_t off = 64u * (uint32_t)i + lane;
+                    wi[i] = (uint32_t)(((u64)start + off) % L);
+                    wv[i] = kFull;
+                    if (off < L) {
+                        wv[i] = ld_word(&c.bitmap[wi[i]]);
+                        st.probes++;
+                    }
+                }
+                unsigned seen = 0, desig = 0;
+                bool examined_all = true;
+#pragma unroll
+                for (int i = 0; i < J; ++i) {
+                    if (!ballot(need)) {
+                        examined_all = false;
+                        break;
+                    }
+                    WindowResult wr = window_claim(c, wi[i], wv[i], need, id, c.order, st);
+                    seen += wr.seen_free;
+                    desig += wr.designated;
+                }
+                if (examined_all && seen <= desig) have = false;
+            } else {
+                u64 v[kSweepWords];
+#pragma unroll
+                for (uint32_t i = 0; i < kSweepWords; ++i) {
+                    const uint32_t off = lane * kSweepWords + i;
+                    uint32_t idx = start + off;
+                    if (idx >= L) idx -= L;
+                    v[i] = kFull;
+                    if (off < L) v[i] = ld_word(&c.bitmap[idx]);
+                }
+                st.probes += L < (lane + 1u) * kSweepWords
+                                 ? (L > lane * kSweepWords ? L - lane * kSweepWords : 0u)
+                                 : kSweepWords;
+                uint32_t cand = 0;
+                u64 cval = kFull;
+                bool more = false;
+#pragma unroll
+                for (uint32_t i = 0; i < kSweepWords; ++i) {
+                    if (v[i] != kFull) {
+                        if (cval == kFull) {
+                            uint32_t idx = start + lane * kSweepWords + i;
+                            cand = idx >= L ? idx - L : idx;
+                            cval = v[i];
+                        } else {
+                            more = true;
+                        }
+                    }
+                }
+                window_claim(c, cand, cval, need, id, c.order, st);
+                if (ballot(need) && !ballot(more)) {
+                    start = (uint32_t)(((u64)start + sweep_span) % L);
+                    if (++sweeps > max_sweeps) break;
                 }
             }
-            unsigned seen = 0, desig = 0;
-            bool examined_all = true;
-#pragma unroll
-            for (int i = 0; i < J; ++i) {
-                if (!ballot(need)) {
-                    examined_all = false;
-                    break;
-                }
-                WindowResult wr = window_claim(c, wi[i], wv[i], need, id, c.order, st);
-                seen += wr.seen_free;
-                desig += wr.designated;
-            }
-            if (examined_all && seen <= desig) have = false;
         }
         if (need) st.errors++;
         release_one(c, need);
@@ -141,12 +183,13 @@ __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint
         st.steps += steps;
     }
     flush_stats(c, st);
-    // The last wave to finish applies the reservation; every wave has read
-    // free_count before it counts itself done.
-    if (lane == 0) {
+    // The last workgroup to finish applies the reservation; every wave has
+    // read free_count before its workgroup counts itself done.
+    __syncthreads();
+    if (threadIdx.x == 0) {
         u64 prev = __hip_atomic_fetch_add((u64*)&c.phase[1], 1ull, __ATOMIC_ACQ_REL,
                                           __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == (u64)n_waves - 1u) {
+        if (prev == (u64)gridDim.x - 1u) {
             atom_add_ll(c.free_count, -granted);
             __hip_atomic_store(&c.phase[1], 0ll, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
@@ -174,6 +217,9 @@ __global__ __launch_bounds__(256) void k_free_batched(pbm_ctx c, uint32_t n,
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t n_chunks = (n + 63u) / 64u;
+    __shared__ uint32_t s_freed;
+    if (threadIdx.x == 0) s_freed = 0;
+    __syncthreads();
     Stats st;
     for (uint32_t ch = wave; ch < n_chunks; ch += n_waves) {
         const uint32_t r = ch * 64u + lane;
@@ -201,12 +247,14 @@ __global__ __launch_bounds__(256) void k_free_batched(pbm_ctx c, uint32_t n,
             st.errors += (uint32_t)__popcll(m & ~old);
         }
     }
-    // one reservation update per wave for all pages it returned
+    // one reservation update per workgroup for all pages it returned
     uint32_t fr = st.frees;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) fr += (uint32_t)__shfl_xor((int)fr, d, 64);
-    if (lane == 0 && fr) atom_add_ll(c.free_count, (long long)fr);
+    if (lane == 0 && fr) atomicAdd(&s_freed, fr);
     flush_stats(c, st);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_freed) atom_add_ll(c.free_count, (long long)s_freed);
 }
 
 // ---------------------------------------------------------- launches

@@ -103,24 +103,28 @@ def test_one_wave_batched_matches_oracle(gpu, cw, order):
         assert pool.free_count() == st.free[0]
 
 
-def test_one_wave_batched_budget_sweep_matches_oracle(gpu):
-    # walk budget 1: after one random window the wave sweeps linearly
-    T, n = 1 << 14, 600
-    log_case("one-wave batched budget=1 sweep")
-    with pbm.Pool(T, strategy=pbm.RW_BM, seed=2) as pool:
-        pool.prefill_bernoulli(0.99, 5)
+@pytest.mark.parametrize("T,fill,n,budget", [(1 << 14, 0.99, 600, 1), (1 << 18, 0.999, 400, 2),
+                                             (1 << 20, 0.9, 120000, 4)])
+@pytest.mark.parametrize("order", [pbm.ORDER_PACKED, pbm.ORDER_REFERENCE])
+def test_one_wave_batched_budget_sweep_matches_oracle(gpu, T, fill, n, budget, order):
+    # small walk budget: after it the wave sweeps 1024 words per step
+    log_case(f"one-wave batched sweep T={T} fill={fill} budget={budget} order={order}")
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=2, order=order) as pool:
+        pool.prefill_bernoulli(fill, 5)
         ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, cw=64, flags=pbm.LAUNCH_ONE_WAVE,
-                          budget=1)
-        st = orc.State(orc.prefill(T, 0.99, 5), T)
-        eids, esteps = st.batched(n, 0, 2, 0, 64, budget=1)
+                          budget=budget)
+        st = orc.State(orc.prefill(T, fill, 5), T)
+        eids, esteps = st.batched(n, 0, 2, order, 64, budget=budget)
         assert np.array_equal(ids, eids) and np.array_equal(steps, esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
 
 
 # ------------------------------------------------------------------ concurrent invariants
 
-def _check_cycle(torch, pool, T, before, n, get_mode, free_mode, cw=0, expect_prefix=False):
+def _check_cycle(torch, pool, T, before, n, get_mode, free_mode, cw=0, expect_prefix=False,
+                 budget=0):
     A = orc.count_free(before, T)
-    ids, steps = _get(torch, pool, n, get_mode, cw=cw)
+    ids, steps = _get(torch, pool, n, get_mode, cw=cw, budget=budget)
     after = pool.snapshot()
     rc, g = orc.check_alloc(before, after, T, ids)
     assert rc == 0, f"invariant {rc} violated"
@@ -150,7 +154,7 @@ def test_c1_plain_rwbm_cycle(gpu):
 
 @pytest.mark.parametrize("fill", [0.5, 0.9, 0.99])
 @pytest.mark.parametrize("kind", ["plain_rwbm", "plain_collab", "batched64", "batched128",
-                                  "batched256"])
+                                  "batched256", "batched64_b2"])
 def test_c2_cycle_invariants(gpu, fill, kind):
     # 1,048,576 pages, Bernoulli pre-fill, 262,144 requests then frees
     T, n = BIG, 262144
@@ -160,8 +164,9 @@ def test_c2_cycle_invariants(gpu, fill, kind):
         pool.prefill_bernoulli(fill, 42)
         before = orc.prefill(T, fill, 42)
         if kind.startswith("batched"):
+            cw, _, b = kind[7:].partition("_b")
             _check_cycle(gpu, pool, T, before, n, pbm.MODE_BATCHED, pbm.MODE_BATCHED,
-                         cw=int(kind[7:]), expect_prefix=True)
+                         cw=int(cw), expect_prefix=True, budget=int(b or 0))
         else:
             _check_cycle(gpu, pool, T, before, n, pbm.MODE_PLAIN, pbm.MODE_PLAIN)
         c = pool.counters()

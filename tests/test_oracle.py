@@ -407,3 +407,23 @@ def test_check_alloc_flags_planted_violations():
     assert orc.check_alloc(before, extra, T, ids)[0] == 5
     oob = ids.copy(); oob[0] = T
     assert orc.check_alloc(before, st.bm, T, oob)[0] == 1
+
+
+def test_batched_sweep_known_answer():
+    # Every page used except one bit in each of four words; budget 1: one
+    # random window, then the sweep from that window's start hands the bits
+    # out in map order starting at the window start.
+    T = 64 * 4096
+    L = T // 64
+    seed = 77
+    bm = orc.prefill(T, 1.0, 0)
+    words = [5, 1500, 2999, 4090]
+    for w in words:
+        bm[w] &= ~np.uint64(1 << 9)
+    wave_salt = 0x13198A2E03707344
+    start = (orc.lcg(orc.request_seed(seed ^ wave_salt, 0)) & 0xFFFFFFFF) % L
+    expect = [w * 64 + 9 for w in sorted(words, key=lambda w: (w - start) % L)]
+    st = orc.State(bm, T)
+    ids, steps = st.batched(4, 0, seed, 0, 64, budget=1)
+    assert ids.tolist() == expect
+    assert np.array_equal(st.bm, orc.prefill(T, 1.0, 0))
