@@ -209,6 +209,38 @@ int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
 int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
                    const pbm_launch* launch, void* stream);
 
+/* ---- multi-page requests and device malloc (RWMalloc) ------------------ */
+
+/* Serves n requests for d_counts[i] consecutive pages (0 <= count <= 64,
+ * one 64-page word); d_ids[i] = first page or -1 (0 for a count of 0, a
+ * coalesced member).  Wave-collective per 64 requests.  Replaces the
+ * drivers' getNPage kernels (unitTests/unitTest2.cu) calling getNPage
+ * (RWMalloc.cuh:198-248) and the legacy getXPage<Strategy>
+ * (parallelPage.cuh:56,64,72). */
+int pbm_get_n_pages(pbm_handle* h, uint32_t n, uint64_t req_base, const uint32_t* d_counts,
+                    int32_t* d_ids, uint32_t* d_steps, const pbm_launch* launch, void* stream);
+
+/* Returns the runs (d_ids[i], d_counts[i]); ids < 0 or counts 0 skipped.
+ * Replaces releasePages (RWMalloc.cuh:60-70). */
+int pbm_free_n_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids, const uint32_t* d_counts,
+                     const pbm_launch* launch, void* stream);
+
+/* mallocRW for n requests of d_sizes[i] bytes; d_ptrs[i] = device pointer
+ * or 0.  Needs a pool created with backing.  Replaces malloc_kernel
+ * (unitTests/unitTest3.cu:31-35) calling mallocRW (RWMalloc.cuh:299-326). */
+int pbm_malloc(pbm_handle* h, uint32_t n, uint64_t req_base, const uint64_t* d_sizes,
+               uint64_t* d_ptrs, uint32_t* d_steps, const pbm_launch* launch, void* stream);
+
+/* freeRW for n pointers (0 skipped).  Replaces free_kernel
+ * (unitTests/unitTest4.cu:37-43) calling freeRW (RWMalloc.cuh:329-334). */
+int pbm_free_ptrs(pbm_handle* h, uint32_t n, const uint64_t* d_ptrs, const pbm_launch* launch,
+                  void* stream);
+
+/* Host-side pageAddress / getPageID (Paging.cuh:30-52) over the pool's
+ * single contiguous allocation (the reference scans 1 GiB groups). */
+int pbm_page_address(pbm_handle* h, int32_t id, uint64_t* addr);
+int pbm_page_id(pbm_handle* h, uint64_t addr, int32_t* id);
+
 /* ---- graph-captured phases --------------------------------------------- */
 
 #define PBM_PHASE_ALLOC 1u

@@ -477,6 +477,557 @@ __device__ inline int32_t get_page_collab(const pbm_ctx& c, int32_t seed, uint32
     return id;
 }
 
+
+/* ---------------------------------------------------------------------
+ * CRW_BM (CRW_BM.cuh:75-142): clustered walk with a per-thread hint
+ * ------------------------------------------------------------------- */
+
+/* Claims the lowest free bit of word g until it is full, counting every
+ * atomicOr attempt as a step (CRW_BM.cuh:95,121). */
+__device__ __forceinline__ int32_t claim_counting(const pbm_ctx& c, uint32_t g, u64 w,
+                                                  uint32_t& steps, Stats& st) {
+    while (w != kFull) {
+        steps++;
+        unsigned b = (unsigned)__ffsll((long long)~w) - 1u;
+        u64 bit = 1ull << b;
+        u64 old = atom_or(&c.bitmap[g], bit);
+        st.atomics++;
+        if (!(old & bit)) return (int32_t)(g * 64u + b);
+        w = old | bit;
+    }
+    return -1;
+}
+
+/* getPage of CRW_BM for hint slot `slot` (slots >= max_threads have no
+ * hint).  First the word of the last page this slot got, moving to the
+ * next word when that page was the word's top bit (wrapping at n_words;
+ * the reference wraps at n_words-1, CRW_BM.cuh:91); then a random walk as
+ * RandomWalkBasic_BM, whose jumps to full words are not counted as steps.
+ * The hint is the last granted page id. */
+__device__ inline int32_t get_page_crw(const pbm_ctx& c, uint32_t slot, int32_t seed,
+                                       uint32_t* steps_out, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t L = c.n_words;
+    uint32_t steps = 0;
+    int32_t id = -1;
+    const bool has_slot = c.hint && slot < c.max_threads;
+    if (reserve_one(c, true)) {
+        if (has_slot) {
+            int32_t h = c.hint[slot];
+            if (h >= 0) {
+                uint32_t g = (uint32_t)h >> 6;
+                if (((uint32_t)h & 63u) == 63u) g = (g + 1u == L) ? 0u : g + 1u;
+                u64 w = ld_word(&c.bitmap[g]);
+                st.probes++;
+                id = claim_counting(c, g, w, steps, st);
+            }
+        }
+        uint32_t g = 0;
+        for (uint32_t j = 0; id < 0 && j < c.walk_budget; ++j) {
+            seed = lcg(seed);
+            g = (uint32_t)seed % L;
+            u64 w = ld_word(&c.bitmap[g]);
+            st.probes++;
+            id = claim_counting(c, g, w, steps, st);
+        }
+        if (id < 0) {
+            st.fallbacks++;
+            for (uint64_t k = 0; id < 0 && k < 4ull * L; ++k) {
+                g = (g + 1u == L) ? 0u : g + 1u;
+                u64 w = ld_word(&c.bitmap[g]);
+                st.probes++;
+                id = claim_counting(c, g, w, steps, st);
+            }
+        }
+        if (id < 0) { atom_add_ll(c.free_count, 1); st.errors++; }
+        else if (has_slot) c.hint[slot] = id;
+    }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* ---------------------------------------------------------------------
+ * SingleClock (parallelPage.cu:471-513): one global arm
+ * ------------------------------------------------------------------- */
+
+/* getPageSingleClock: every try takes the next arm position
+ * (atomicAdd(arm,1) % N) and claims that page if it is free.  Step = one
+ * arm advance.  The arm is 64-bit and the claim checks the old value (the
+ * reference's int arm overflows and its atomicExch result is ignored,
+ * parallelPage.cu:498-505).  Tickets are taken for the whole wave with one
+ * atomicAdd, in lane order. */
+__device__ inline int32_t get_page_single_clock(const pbm_ctx& c, uint32_t* steps_out,
+                                                Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t N = c.n_pages;
+    bool need = reserve_one(c, true);
+    const bool reserved = need;
+    uint32_t steps = 0;
+    int32_t id = -1;
+    const uint64_t max_tries = (uint64_t)c.walk_budget + 2ull * N;
+    for (uint64_t it = 0; it < max_tries; ++it) {
+        u64 m = ballot(need);
+        if (!m) break;
+        unsigned cnt = (unsigned)__popcll(m);
+        unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+        u64 t0 = 0;
+        if (lane_id() == leader) t0 = atom_add_u64(c.clock_arm, (u64)cnt);
+        t0 = readlane64(t0, leader);
+        if (need) {
+            steps++;
+            uint32_t p = (uint32_t)((t0 + rank_in(m)) % N);
+            u64 bit = 1ull << (p & 63u);
+            u64 w = ld_word(&c.bitmap[p >> 6]);
+            st.probes++;
+            if (!(w & bit)) {
+                u64 old = atom_or(&c.bitmap[p >> 6], bit);
+                st.atomics++;
+                if (!(old & bit)) { id = (int32_t)p; need = false; }
+            }
+        }
+    }
+    if (need) { atom_add_ll(c.free_count, 1); st.errors++; }
+    if (id >= 0) st.allocs++; else st.failures++;
+    (void)reserved;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* ---------------------------------------------------------------------
+ * DistClock (parallelPage.cu:537-596): per-thread arms
+ * ------------------------------------------------------------------- */
+
+/* getPageDistClock for arm slot `slot` (taken modulo max_threads): scan
+ * from the arm, claim the first free page, leave the arm one past the page
+ * examined last.  Step = one page examined, as the reference's arm
+ * advance; a whole word is examined at once and its pages counted. */
+__device__ inline int32_t get_page_dist_clock(const pbm_ctx& c, uint32_t slot,
+                                              uint32_t* steps_out, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t N = c.n_pages;
+    const uint32_t s = slot % c.max_threads;
+    uint32_t steps = 0;
+    int32_t id = -1;
+    if (reserve_one(c, true)) {
+        uint32_t arm = c.arms[s] % N;
+        const uint64_t max_pages = (uint64_t)c.walk_budget + 2ull * N;
+        uint64_t seen = 0;
+        while (seen < max_pages) {
+            const uint32_t g = arm >> 6;
+            const uint32_t b0 = arm & 63u;
+            u64 w = ld_word(&c.bitmap[g]);
+            st.probes++;
+            // pages b0..63 of this word (the padding bits past N are set)
+            u64 fr = ~w & (kFull << b0);
+            if (fr) {
+                unsigned b = (unsigned)__ffsll((long long)fr) - 1u;
+                u64 bit = 1ull << b;
+                steps += b - b0 + 1u;
+                seen += b - b0 + 1u;
+                u64 old = atom_or(&c.bitmap[g], bit);
+                st.atomics++;
+                arm = g * 64u + b + 1u;
+                if (arm >= N) arm = 0;
+                if (!(old & bit)) { id = (int32_t)(g * 64u + b); break; }
+            } else {
+                uint32_t last = g * 64u + 63u < N ? 64u : N - g * 64u;
+                steps += last - b0;
+                seen += last - b0;
+                arm = g * 64u + last;
+                if (arm >= N) arm = 0;
+            }
+        }
+        c.arms[s] = arm;
+        if (id < 0) { atom_add_ll(c.free_count, 1); st.errors++; }
+    }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* ---------------------------------------------------------------------
+ * LinkedList pool (parallelPage.cu:36-46, 342-418) as a ring FIFO
+ *
+ * The reference keeps a Node_t list guarded by sentinel spin locks on the
+ * head and tail ids.  Here the free list is a ring of n_pages slots; a
+ * slot holds (seq << 32 | page id).  Dequeue ticket t owns slot t % N and
+ * waits for seq == t+1 (filled by enqueue ticket t), then writes
+ * seq = t+N (free for enqueue ticket t+N); enqueue ticket t waits for
+ * seq == t, then writes seq = t+1 with the page.  Pages come back in free
+ * order (FIFO), and a page freed after the list ran empty is found again
+ * (the reference loses it, parallelPage.cu:357-362,405-414).  Tickets are
+ * taken once per wave.  The waits are bounded; a wave must not enqueue and
+ * dequeue in the same divergent region.  The page map mirrors the list so
+ * occupancy queries and checks see the same state.  Step = clock cycles
+ * spent in the call, as the reference counts them (parallelPage.cu:343,367).
+ * ------------------------------------------------------------------- */
+
+constexpr uint32_t kRingSpin = 1u << 22;
+
+__device__ __forceinline__ u64 ld_slot(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_slot(u64* p, u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline int32_t get_page_list(const pbm_ctx& c, uint32_t* steps_out,
+                                        Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const long long t_start = clock64();
+    const uint32_t N = c.n_pages;
+    bool want = reserve_one(c, true);
+    int32_t id = -1;
+    u64 m = ballot(want);
+    if (m) {
+        unsigned cnt = (unsigned)__popcll(m);
+        unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+        u64 base = 0;
+        if (lane_id() == leader) base = atom_add_u64(c.ring_head, (u64)cnt);
+        base = readlane64(base, leader);
+        if (want) {
+            const u64 t = base + rank_in(m);
+            u64* slot = &c.ring[t % N];
+            const uint32_t expect = (uint32_t)(t + 1u);
+            for (uint32_t k = 0; k < kRingSpin; ++k) {
+                u64 v = ld_slot(slot);
+                if ((uint32_t)(v >> 32) == expect) { id = (int32_t)(uint32_t)v; break; }
+            }
+            st.probes++;
+            if (id >= 0) {
+                st_slot(slot, (u64)(uint32_t)(t + N) << 32);
+                atom_or(&c.bitmap[(uint32_t)id >> 6], 1ull << ((uint32_t)id & 63u));
+                st.atomics++;
+            } else {
+                st.errors++;
+            }
+        }
+    }
+    if (id >= 0) st.allocs++; else st.failures++;
+    long long cyc = clock64() - t_start;
+    uint32_t steps = cyc > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)cyc;
+    st.steps += 1;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+__device__ inline void free_page_list(const pbm_ctx& c, int32_t id, uint32_t* steps_out = nullptr,
+                                      Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const long long t_start = clock64();
+    const uint32_t N = c.n_pages;
+    bool ok = false;
+    if (id >= 0 && (uint32_t)id < N) {
+        u64 bit = 1ull << ((uint32_t)id & 63u);
+        u64 old = atom_and(&c.bitmap[(uint32_t)id >> 6], ~bit);
+        st.atomics++;
+        ok = (old & bit) != 0;
+    }
+    if (!ok) st.errors++;
+    u64 m = ballot(ok);
+    if (m) {
+        unsigned cnt = (unsigned)__popcll(m);
+        unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+        u64 base = 0;
+        if (lane_id() == leader) base = atom_add_u64(c.ring_tail, (u64)cnt);
+        base = readlane64(base, leader);
+        if (ok) {
+            const u64 t = base + rank_in(m);
+            u64* slot = &c.ring[t % N];
+            const uint32_t expect = (uint32_t)t;
+            bool ready = false;
+            for (uint32_t k = 0; k < kRingSpin && !ready; ++k)
+                ready = (uint32_t)(ld_slot(slot) >> 32) == expect;
+            if (!ready) st.errors++;
+            st_slot(slot, ((u64)(uint32_t)(t + 1u) << 32) | (uint32_t)id);
+            st.frees++;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        release_one(c, ok);
+    }
+    if (steps_out) {
+        long long cyc = clock64() - t_start;
+        *steps_out = cyc > 0xFFFFFFFFll ? 0xFFFFFFFFu : (uint32_t)cyc;
+    }
+}
+
+/* ---------------------------------------------------------------------
+ * RWMalloc (RWMalloc.cuh:113-334)
+ * ------------------------------------------------------------------- */
+
+/* Clears `probe` on every lane whose word equals the word of a lower
+ * active lane, so one window never claims from a word image twice. */
+__device__ inline void dedupe_words(uint32_t word, bool& probe) {
+    u64 pending = ballot(probe);
+    while (pending) {
+        unsigned l = (unsigned)__ffsll((long long)pending) - 1u;
+        uint32_t v = readlane(word, l);
+        u64 same = ballot(probe && word == v);
+        if (probe && word == v && lane_id() != l) probe = false;
+        pending &= ~same;
+    }
+}
+
+/* getPage of RWMalloc: wave-collective like CollabRW_BM, but every lane
+ * walks its own LCG stream (RWMalloc.cuh:114-116,125-126), so a step
+ * probes up to 64 scattered words (duplicates probed once).  After the
+ * budget the wave sweeps a contiguous window from its lowest lane's last
+ * word. */
+__device__ inline int32_t get_page_rwmalloc(const pbm_ctx& c, int32_t seed, uint32_t* steps_out,
+                                            Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t L = c.n_words;
+    const unsigned lane = lane_id();
+    bool need = reserve_one(c, true);
+    int32_t id = -1;
+    uint32_t steps = 0, jumps = 0, sweeps = 0, word = 0, start = 0;
+    const uint32_t max_sweeps = 4u * ((L + 63u) / 64u) + 4u;
+    while (ballot(need)) {
+        bool probe;
+        if (jumps < c.walk_budget) {
+            seed = lcg(seed);
+            word = (uint32_t)seed % L;
+            probe = true;
+            ++jumps;
+            if (jumps == c.walk_budget) start = __builtin_amdgcn_readfirstlane(word);
+        } else {
+            if (sweeps == 0) st.fallbacks++;
+            start = (uint32_t)(((u64)start + 64u) % L);
+            if (++sweeps > max_sweeps) break;
+            word = (uint32_t)(((u64)start + lane) % L);
+            probe = lane < L;
+        }
+        if (need) steps++;
+        dedupe_words(word, probe);
+        window_step(c, probe, word, need, id, c.order, st);
+    }
+    if (need) { atom_add_ll(c.free_count, 1); st.errors++; }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* First run of zero bits of x: *start, returns its length (0 if none)
+ * and marks the run used in x.  find0Range (RWMalloc.cuh:159-180) means
+ * this but finds only runs starting at bit 0. */
+__device__ __forceinline__ unsigned first_zero_run(u64& x, unsigned& start) {
+    u64 z = ~x;
+    if (!z) return 0;
+    start = (unsigned)__ffsll((long long)z) - 1u;
+    u64 above = x >> start;                  // bit 0 is the run's first zero
+    unsigned len = above ? (unsigned)__ffsll((long long)above) - 1u : 64u - start;
+    u64 run = (len >= 64u ? kFull : ((1ull << len) - 1ull)) << start;
+    x |= run;
+    return len;
+}
+
+__device__ __forceinline__ u64 run_mask(unsigned start, unsigned n) {
+    return (n >= 64u ? kFull : ((1ull << n) - 1ull)) << start;
+}
+
+/* Takes `want` units per lane from the reservation counter with one atomic
+ * per wave; lanes are served in rank order while the running total fits. */
+__device__ inline bool reserve_n(const pbm_ctx& c, unsigned want) {
+    unsigned total = 0, pre = 0;
+    u64 act = ballot(want > 0);
+    if (!act) return false;
+    u64 m = act;
+    while (m) {
+        unsigned l = (unsigned)__ffsll((long long)m) - 1u;
+        m &= m - 1ull;
+        unsigned v = readlane(want, l);
+        if (lane_id() == l) pre = total;
+        total += v;
+    }
+    unsigned leader = (unsigned)__ffsll((long long)act) - 1u;
+    long long old = 0;
+    if (lane_id() == leader) old = atom_add_ll(c.free_count, -(long long)total);
+    old = (long long)readlane64((u64)old, leader);
+    bool ok = want > 0 && old >= (long long)(pre + want);
+    // give back what the refused lanes asked for
+    unsigned back = 0;
+    const u64 okm = ballot(ok);
+    m = act & ~okm;
+    while (m) {
+        unsigned l = (unsigned)__ffsll((long long)m) - 1u;
+        m &= m - 1ull;
+        back += readlane(want, l);
+    }
+    if (back && lane_id() == leader) atom_add_ll(c.free_count, (long long)back);
+    return ok;
+}
+
+constexpr uint32_t kNPageBudget = 64;
+
+/* getNPage (RWMalloc.cuh:198-248): wave-collective; lane asks for n
+ * consecutive pages inside one word (n <= 64; n == 0 is a coalesced member
+ * that needs nothing and gets 0).  Each step reads a window of consecutive
+ * words; in rounds, every lane offers the next zero-run of its word, and
+ * for each offering lane in lane order the lowest still-needing lane whose
+ * n fits takes the run's first n bits.  Takes are decided first and then
+ * claimed together; a claim that finds a bit taken gives back the bits it
+ * set (the reference leaks them, :236-237) and the lane keeps needing.
+ * At most kNPageBudget random windows, then one sweep of the whole map
+ * (the reference walks forever).  Returns the first page of the run, -1 on
+ * failure. */
+__device__ inline int32_t get_n_pages(const pbm_ctx& c, uint32_t n, int32_t seed,
+                                      uint32_t* steps_out, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t L = c.n_words;
+    const unsigned lane = lane_id();
+    seed = (int32_t)__builtin_amdgcn_readfirstlane(seed);
+    const bool too_big = n > 64u;
+    bool need = reserve_n(c, too_big ? 0u : n);
+    int32_t out = (n == 0) ? 0 : -1;
+    uint32_t steps = 0, jumps = 0, sweeps = 0, start = 0;
+    // Free pages need not form a long enough run, so the search is bounded:
+    // a few random windows, then one pass over the map, then failure.
+    const uint32_t budget = c.walk_budget < kNPageBudget ? c.walk_budget : kNPageBudget;
+    const uint32_t max_sweeps = (L + 63u) / 64u;
+    while (ballot(need)) {
+        if (jumps < budget) {
+            seed = lcg(seed);
+            start = (uint32_t)seed % L;
+            ++jumps;
+        } else {
+            if (sweeps == 0) st.fallbacks++;
+            start = (uint32_t)(((u64)start + 64u) % L);
+            if (++sweeps > max_sweeps) break;
+        }
+        if (need) steps++;
+        const uint32_t word = (uint32_t)(((u64)start + lane) % L);
+        // (a lane past the map's end probes nothing)
+        u64 img = kFull;
+        if (lane < L) { img = ld_word(&c.bitmap[word]); st.probes++; }
+        for (;;) {
+            unsigned rs = 0;
+            unsigned rl = first_zero_run(img, rs);
+            u64 H = ballot(rl > 0);
+            if (!H || !ballot(need)) break;
+            bool assigned = false;
+            unsigned src = 0, src_start = 0;
+            while (H) {
+                unsigned t = (unsigned)__ffsll((long long)H) - 1u;
+                H &= H - 1ull;
+                unsigned len_t = readlane(rl, t);
+                u64 tk = ballot(need && !assigned && n <= len_t);
+                if (tk) {
+                    unsigned k = (unsigned)__ffsll((long long)tk) - 1u;
+                    if (lane == k) {
+                        assigned = true;
+                        src = t;
+                        src_start = readlane(rs, t);
+                    }
+                }
+            }
+            uint32_t src_word = shfl32(word, assigned ? src : lane);
+            if (assigned) {
+                u64 mask = run_mask(src_start, n);
+                u64 old = atom_or(&c.bitmap[src_word], mask);
+                st.atomics++;
+                u64 owned = mask & ~old;
+                if (owned == mask) {
+                    out = (int32_t)(src_word * 64u + src_start);
+                    need = false;
+                } else if (owned) {
+                    atom_and(&c.bitmap[src_word], ~owned);
+                    st.atomics++;
+                }
+            }
+        }
+    }
+    if (need) { atom_add_ll(c.free_count, (long long)n); st.errors++; }
+    if (n > 0) {
+        if (out >= 0) st.allocs++; else st.failures++;
+    }
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return out;
+}
+
+/* releasePages (RWMalloc.cuh:60-70): returns the n pages starting at id
+ * (one word).  Pages that were not in use count as errors. */
+__device__ inline void free_n_pages(const pbm_ctx& c, int32_t id, uint32_t n,
+                                    Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    uint32_t freed = 0;
+    if (id >= 0 && (uint32_t)id < c.n_pages && n >= 1 && n <= 64 &&
+        ((uint32_t)id & 63u) + n <= 64u) {
+        u64 mask = run_mask((uint32_t)id & 63u, n);
+        u64 old = atom_and(&c.bitmap[(uint32_t)id >> 6], ~mask);
+        st.atomics++;
+        freed = (uint32_t)__popcll(old & mask);
+        st.frees += freed;
+        st.errors += n - freed;
+    } else if (n > 0) {
+        st.errors++;
+    }
+    if (freed) atom_add_ll(c.free_count, (long long)freed);
+}
+
+/* mallocRW (RWMalloc.cuh:299-326): wave-collective over the active lanes.
+ * size+4 bytes; requests smaller than a page are packed into one run
+ * taken by the lowest such lane (the coalescing leader), at offsets given
+ * in lane order (the reference orders them by a shared-memory atomicAdd)
+ * and rounded up to 4 bytes so every header is aligned; larger ones take
+ * ceil((size+4)/page_size) pages.  An int header before each pointer holds
+ * the run's page count (leader and large requests) or 0 (packed members).
+ * Needs backing memory; returns nullptr on failure or for runs over 64
+ * pages. */
+__device__ inline void* malloc_rw(const pbm_ctx& c, uint64_t size, int32_t seed,
+                                  uint32_t* steps_out, Stats* stats = nullptr) {
+    const uint64_t need_bytes = (size + 4u + 3u) & ~(uint64_t)3u;
+    const bool small = need_bytes < c.page_size;
+    const u64 small_mask = ballot(small);
+    const unsigned leader = small_mask ? (unsigned)__ffsll((long long)small_mask) - 1u : 0u;
+    uint64_t off = 0, total = 0;
+    u64 m = small_mask;
+    while (m) {
+        unsigned l = (unsigned)__ffsll((long long)m) - 1u;
+        m &= m - 1ull;
+        uint64_t v = readlane64(need_bytes, l);
+        if (lane_id() == l) off = total;
+        total += v;
+    }
+    uint64_t pages;
+    if (small) pages = lane_id() == leader ? (total + c.page_size - 1u) / c.page_size : 0u;
+    else pages = (need_bytes + c.page_size - 1u) / c.page_size;
+    uint32_t n = pages > 64u ? 65u : (uint32_t)pages;
+    int32_t pid = get_n_pages(c, n, seed, steps_out, stats);
+    if (small_mask) {
+        int32_t lp = (int32_t)readlane((uint32_t)pid, leader);
+        if (small) pid = lp;
+    }
+    if (pid < 0 || !c.pool) return nullptr;
+    char* p = c.pool + (size_t)(uint32_t)pid * c.page_size + (small ? off : 0u);
+    *(int32_t*)p = (int32_t)n;
+    return p + 4;
+}
+
+/* freeRW (RWMalloc.cuh:329-334): header 0 (a packed member) frees nothing;
+ * otherwise the run is returned.  Freeing the leader's pointer returns the
+ * pages of every member packed with it, as in the reference. */
+__device__ inline void free_rw(const pbm_ctx& c, void* ptr, Stats* stats = nullptr) {
+    if (!ptr) return;
+    int32_t n = *((int32_t*)ptr - 1);
+    if (n == 0) return;
+    int32_t id = page_id(c, (char*)ptr - 4);
+    free_n_pages(c, id, (uint32_t)n, stats);
+}
+
 }  // namespace pbm
 
 #endif  /* PBM_DEVICE_HPP */

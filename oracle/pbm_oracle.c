@@ -401,6 +401,417 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
     return got;
 }
 
+/* ------------------------------------------------------------ CRW_BM */
+
+/* Claim loop of CRW_BM (:84-107, :114-127): every atomicOr attempt is a
+ * step. */
+static int32_t claim_counting(uint64_t* bm, uint32_t g, uint32_t* steps) {
+    while (bm[g] != FULL) {
+        (*steps)++;
+        int b = __builtin_ctzll(~bm[g]);
+        u64 bit = (u64)1 << b;
+        u64 old = bm[g];
+        bm[g] |= bit;
+        if (!(old & bit)) return (int32_t)(g * 64u + (uint32_t)b);
+    }
+    return -1;
+}
+
+/* CRW_BM getPage (CRW_BM.cuh:75-142) for one thread whose hint is *hint
+ * (last granted page, -1 none): the hinted word (the next word when the
+ * hint was its top bit, wrapping at L), then the random walk, whose jumps
+ * to full words are not steps, then a sweep. */
+uint32_t orc_crw_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                        uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* hint,
+                        int32_t* ids, uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    uint32_t got = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t s = 0;
+        int32_t id = -1;
+        if (*free_count > 0) {
+            (*free_count)--;
+            if (*hint >= 0) {
+                uint32_t g = (uint32_t)*hint >> 6;
+                if (((uint32_t)*hint & 63u) == 63u) g = (g + 1u == L) ? 0u : g + 1u;
+                id = claim_counting(bm, g, &s);
+            }
+            int32_t sd = orc_request_seed(seed, req_base + r);
+            uint32_t g = 0;
+            for (uint32_t j = 0; id < 0 && j < budget; ++j) {
+                sd = orc_lcg(sd);
+                g = (uint32_t)sd % L;
+                id = claim_counting(bm, g, &s);
+            }
+            for (u64 k = 0; id < 0 && k < 4ull * L; ++k) {
+                g = (g + 1u == L) ? 0u : g + 1u;
+                id = claim_counting(bm, g, &s);
+            }
+            if (id < 0) (*free_count)++;
+            else *hint = id;
+        }
+        if (id >= 0) got++;
+        ids[r] = id;
+        if (steps) steps[r] = s;
+    }
+    return got;
+}
+
+/* ------------------------------------------------------------ clocks */
+
+/* getPageSingleClock (parallelPage.cu:491-509): take arm++ % N until the
+ * page there is free; step = arm advance. */
+uint32_t orc_single_clock_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                                 uint64_t* arm, uint32_t budget, int32_t* ids, uint32_t* steps) {
+    uint32_t got = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t s = 0;
+        int32_t id = -1;
+        if (*free_count > 0) {
+            (*free_count)--;
+            u64 max_tries = (u64)budget + 2ull * n_pages;
+            for (u64 it = 0; it < max_tries; ++it) {
+                s++;
+                uint32_t p = (uint32_t)((*arm)++ % n_pages);
+                u64 bit = (u64)1 << (p & 63u);
+                if (!(bm[p >> 6] & bit)) {
+                    bm[p >> 6] |= bit;
+                    id = (int32_t)p;
+                    break;
+                }
+            }
+            if (id < 0) (*free_count)++;
+        }
+        if (id >= 0) got++;
+        ids[r] = id;
+        if (steps) steps[r] = s;
+    }
+    return got;
+}
+
+/* Initial DistClock arm of slot t. */
+uint32_t orc_dist_clock_arm(uint64_t seed, uint32_t slot, uint32_t n_pages) {
+    return (uint32_t)orc_lcg(orc_request_seed(seed ^ 0xA4093822299F31D0ull, slot)) % n_pages;
+}
+
+/* getPageDistClock (parallelPage.cu:571-592) for one arm: examine pages
+ * from the arm on, take the first free one, leave the arm one past it;
+ * step = page examined. */
+uint32_t orc_dist_clock_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                               uint32_t* arm, uint32_t budget, int32_t* ids, uint32_t* steps) {
+    uint32_t got = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t s = 0;
+        int32_t id = -1;
+        if (*free_count > 0) {
+            (*free_count)--;
+            u64 max_pages = (u64)budget + 2ull * n_pages;
+            uint32_t a = *arm % n_pages;
+            for (u64 k = 0; k < max_pages; ++k) {
+                s++;
+                uint32_t p = a;
+                a = (p + 1u == n_pages) ? 0u : p + 1u;
+                u64 bit = (u64)1 << (p & 63u);
+                if (!(bm[p >> 6] & bit)) {
+                    bm[p >> 6] |= bit;
+                    id = (int32_t)p;
+                    break;
+                }
+            }
+            *arm = a;
+            if (id < 0) (*free_count)++;
+        }
+        if (id >= 0) got++;
+        ids[r] = id;
+        if (steps) steps[r] = s;
+    }
+    return got;
+}
+
+/* ------------------------------------------------------------ LinkedList */
+
+/* initPagesLinkedList (parallelPage.cu:303-337) on a pre-filled map: the
+ * free pages in ascending order form the list; returns how many. */
+uint32_t orc_list_build(const uint64_t* bm, uint32_t n_pages, int32_t* q) {
+    uint32_t k = 0;
+    for (uint32_t p = 0; p < n_pages; ++p)
+        if (!((bm[p >> 6] >> (p & 63u)) & 1u)) q[k++] = (int32_t)p;
+    return k;
+}
+
+/* getPageLinkedList (:342-369): pop the head; -1 when empty (the reference
+ * traps).  q is a ring of n_pages slots, head/tail are tickets. */
+void orc_list_get(uint64_t* bm, uint32_t n_pages, int32_t* q, uint64_t* head, uint64_t* tail,
+                  uint32_t n, int32_t* ids) {
+    for (uint32_t r = 0; r < n; ++r) {
+        if (*head == *tail) {
+            ids[r] = -1;
+            continue;
+        }
+        int32_t id = q[*head % n_pages];
+        (*head)++;
+        bm[(uint32_t)id >> 6] |= (u64)1 << ((uint32_t)id & 63u);
+        ids[r] = id;
+    }
+}
+
+/* freePageLinkedList (:399-418): append at the tail (ids < 0 and pages
+ * not in use are skipped). */
+void orc_list_put(uint64_t* bm, uint32_t n_pages, int32_t* q, uint64_t* head, uint64_t* tail,
+                  uint32_t n, const int32_t* ids) {
+    (void)head;
+    for (uint32_t r = 0; r < n; ++r) {
+        int32_t id = ids[r];
+        if (id < 0 || (uint32_t)id >= n_pages) continue;
+        u64 bit = (u64)1 << ((uint32_t)id & 63u);
+        if (!(bm[(uint32_t)id >> 6] & bit)) continue;
+        bm[(uint32_t)id >> 6] &= ~bit;
+        q[*tail % n_pages] = id;
+        (*tail)++;
+    }
+}
+
+/* ------------------------------------------------------------ RWMalloc */
+
+/* RWMalloc getPage (RWMalloc.cuh:113-149) on one wave: every active lane
+ * walks its own LCG stream; a word probed by several lanes is probed by
+ * the lowest of them only; after the budget the wave sweeps from its
+ * lowest lane's last word. */
+uint32_t orc_rwmalloc_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                             uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                             int32_t* ids, uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    const uint32_t max_sweeps = 4u * ((L + 63u) / 64u) + 4u;
+    uint32_t got = 0;
+    for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        int active[64], need[64], probe[64];
+        int32_t cid[64], sd[64];
+        uint32_t cst[64], word[64], n_act = 0;
+        for (int l = 0; l < 64; ++l) {
+            uint32_t r = ch * 64u + (uint32_t)l;
+            active[l] = r < n;
+            n_act += (uint32_t)active[l];
+            cid[l] = -1;
+            cst[l] = 0;
+            sd[l] = active[l] ? orc_request_seed(seed, req_base + r) : 0;
+            word[l] = 0;
+        }
+        int64_t old = *free_count;
+        int64_t g = old < 0 ? 0 : (old > (int64_t)n_act ? (int64_t)n_act : old);
+        *free_count -= g;
+        for (int l = 0; l < 64; ++l) need[l] = active[l] && l < g;
+        uint32_t jumps = 0, sweeps = 0, start = 0;
+        while (any(need)) {
+            if (jumps < budget) {
+                for (int l = 0; l < 64; ++l) {
+                    if (!active[l]) continue;
+                    sd[l] = orc_lcg(sd[l]);
+                    word[l] = (uint32_t)sd[l] % L;
+                    probe[l] = 1;
+                }
+                jumps++;
+                if (jumps == budget) start = word[0];
+            } else {
+                start = (uint32_t)(((u64)start + 64u) % L);
+                if (++sweeps > max_sweeps) break;
+                for (int l = 0; l < 64; ++l) {
+                    word[l] = (uint32_t)(((u64)start + (uint32_t)l) % L);
+                    probe[l] = active[l] && (uint32_t)l < L;
+                }
+            }
+            for (int l = 0; l < 64; ++l)
+                if (need[l]) cst[l]++;
+            for (int l = 0; l < 64; ++l) {
+                if (!probe[l] || !active[l]) { probe[l] = 0; continue; }
+                for (int k = 0; k < l; ++k)
+                    if (probe[k] && active[k] && word[k] == word[l]) { probe[l] = 0; break; }
+            }
+            uint32_t seen;
+            collab_claim(bm, probe, word, need, cid, order, &seen);
+        }
+        for (int l = 0; l < 64; ++l) {
+            if (need[l]) (*free_count)++;
+            if (!active[l]) continue;
+            ids[ch * 64u + l] = cid[l];
+            if (steps) steps[ch * 64u + l] = cst[l];
+            if (cid[l] >= 0) got++;
+        }
+    }
+    return got;
+}
+
+/* First zero run of *x: sets *start, returns its length (0 if none) and
+ * marks it used in *x (find0Range, RWMalloc.cuh:159-180, as intended). */
+static unsigned first_zero_run(u64* x, unsigned* start) {
+    u64 z = ~*x;
+    if (!z) return 0;
+    unsigned s = (unsigned)__builtin_ctzll(z);
+    u64 above = *x >> s;
+    unsigned len = above ? (unsigned)__builtin_ctzll(above) : 64u - s;
+    u64 run = (len >= 64u ? FULL : (((u64)1 << len) - 1u)) << s;
+    *x |= run;
+    *start = s;
+    return len;
+}
+
+static u64 run_mask(unsigned start, unsigned n) {
+    return (n >= 64u ? FULL : (((u64)1 << n) - 1u)) << start;
+}
+
+/* getNPage (RWMalloc.cuh:198-248) on one wave over chunks of 64 requests
+ * for counts[r] pages.  Reservation: lanes in order while the running sum
+ * of their counts fits the free pages.  Per step a window of consecutive
+ * words; per round every lane offers the next zero run of its word and,
+ * for each offering lane in order, the lowest lane that still needs and
+ * whose count fits takes the run's first pages.  At most 64 random
+ * windows, then one sweep of the map, then failure. */
+uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                             const uint32_t* counts, uint64_t req_base, uint64_t seed,
+                             uint32_t budget, int32_t* ids, uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    /* bounded search: at most 64 random windows, then one pass of the map */
+    const uint32_t max_sweeps = (L + 63u) / 64u;
+    if (budget > 64u) budget = 64u;
+    uint32_t got = 0;
+    for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        int active[64], need[64];
+        int32_t out[64];
+        uint32_t cst[64], cnt[64];
+        u64 pre = 0, total = 0;
+        for (int l = 0; l < 64; ++l) {
+            uint32_t r = ch * 64u + (uint32_t)l;
+            active[l] = r < n;
+            cnt[l] = active[l] ? counts[r] : 0;
+            out[l] = cnt[l] == 0 ? 0 : -1;
+            cst[l] = 0;
+            need[l] = 0;
+            if (active[l] && cnt[l] >= 1 && cnt[l] <= 64) total += cnt[l];
+        }
+        int64_t old = *free_count;
+        *free_count -= (int64_t)total;
+        u64 back = 0;
+        for (int l = 0; l < 64; ++l) {
+            if (!(active[l] && cnt[l] >= 1 && cnt[l] <= 64)) continue;
+            if (old >= (int64_t)(pre + cnt[l])) need[l] = 1;
+            else back += cnt[l];
+            pre += cnt[l];
+        }
+        *free_count += (int64_t)back;
+        int32_t sd = orc_request_seed(seed, req_base + (u64)ch * 64u);
+        uint32_t jumps = 0, sweeps = 0, start = 0;
+        while (any(need)) {
+            if (jumps < budget) {
+                sd = orc_lcg(sd);
+                start = (uint32_t)sd % L;
+                jumps++;
+            } else {
+                start = (uint32_t)(((u64)start + 64u) % L);
+                if (++sweeps > max_sweeps) break;
+            }
+            for (int l = 0; l < 64; ++l)
+                if (need[l]) cst[l]++;
+            u64 img[64];
+            uint32_t word[64];
+            for (int l = 0; l < 64; ++l) {
+                word[l] = (uint32_t)(((u64)start + (uint32_t)l) % L);
+                img[l] = (active[l] && (uint32_t)l < L) ? bm[word[l]] : FULL;
+            }
+            for (;;) {
+                unsigned rs[64], rl[64];
+                int has = 0;
+                for (int l = 0; l < 64; ++l) {
+                    rs[l] = 0;
+                    rl[l] = active[l] ? first_zero_run(&img[l], &rs[l]) : 0;
+                    has |= rl[l] > 0;
+                }
+                if (!has || !any(need)) break;
+                for (int t = 0; t < 64; ++t) {
+                    if (!rl[t]) continue;
+                    for (int k = 0; k < 64; ++k) {
+                        if (!need[k] || cnt[k] > rl[t]) continue;
+                        bm[word[t]] |= run_mask(rs[t], cnt[k]);
+                        out[k] = (int32_t)(word[t] * 64u + rs[t]);
+                        need[k] = 0;
+                        break;
+                    }
+                }
+            }
+        }
+        for (int l = 0; l < 64; ++l) {
+            if (need[l]) (*free_count) += cnt[l];
+            if (!active[l]) continue;
+            ids[ch * 64u + l] = out[l];
+            if (steps) steps[ch * 64u + l] = cst[l];
+            if (cnt[l] > 0 && out[l] >= 0) got++;
+        }
+    }
+    return got;
+}
+
+/* releasePages (RWMalloc.cuh:60-70). */
+uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                    const int32_t* ids, const uint32_t* counts) {
+    uint32_t freed = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        int32_t id = ids[r];
+        uint32_t c = counts[r];
+        if (id < 0 || c == 0 || c > 64 || (uint32_t)id >= n_pages || ((uint32_t)id & 63u) + c > 64u)
+            continue;
+        u64 m = run_mask((uint32_t)id & 63u, c);
+        freed += (uint32_t)__builtin_popcountll(bm[(uint32_t)id >> 6] & m);
+        bm[(uint32_t)id >> 6] &= ~m;
+    }
+    *free_count += freed;
+    return freed;
+}
+
+/* mallocRW (RWMalloc.cuh:299-326) on one wave: the plan of every request
+ * (page count asked, offset inside the run, header) and the getNPage
+ * grants.  Outputs per request: page (first page of its run, -1 on
+ * failure), offset (bytes from the page start to the header) and header. */
+uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                           int64_t* free_count, uint32_t n, const uint64_t* sizes,
+                           uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* page,
+                           uint64_t* offset, int32_t* header, uint32_t* steps) {
+    uint32_t got = 0;
+    for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        uint32_t m = n - ch * 64u < 64u ? n - ch * 64u : 64u;
+        uint32_t cnt[64];
+        u64 off[64], need_b[64], total = 0;
+        int small[64], leader = -1;
+        for (uint32_t l = 0; l < m; ++l) {
+            u64 nb = (sizes[ch * 64u + l] + 4u + 3u) & ~(u64)3u;
+            need_b[l] = nb;
+            small[l] = nb < page_size;
+            off[l] = 0;
+            if (small[l]) {
+                if (leader < 0) leader = (int)l;
+                off[l] = total;
+                total += nb;
+            }
+        }
+        for (uint32_t l = 0; l < m; ++l) {
+            u64 pg;
+            if (small[l]) pg = (int)l == leader ? (total + page_size - 1u) / page_size : 0;
+            else pg = (need_b[l] + page_size - 1u) / page_size;
+            cnt[l] = pg > 64u ? 65u : (uint32_t)pg;
+        }
+        int32_t ids[64];
+        uint32_t st[64];
+        orc_getnpage_serial(bm, n_pages, free_count, m, cnt, req_base + (u64)ch * 64u, seed,
+                            budget, ids, st);
+        for (uint32_t l = 0; l < m; ++l) {
+            int32_t pid = small[l] ? ids[leader] : ids[l];
+            page[ch * 64u + l] = pid;
+            offset[ch * 64u + l] = small[l] ? off[l] : 0;
+            header[ch * 64u + l] = (int32_t)cnt[l];
+            if (steps) steps[ch * 64u + l] = st[l];
+            if (pid >= 0) got++;
+        }
+    }
+    return got;
+}
+
 /* ------------------------------------------------------------ free */
 
 /* freePage (RandomWalkBasic_BM.cuh:98-102): clear the page's bit. */

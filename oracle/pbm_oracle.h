@@ -50,6 +50,41 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
                             uint64_t req_base, uint64_t seed, uint32_t budget, int order,
                             uint32_t cluster_words, int32_t* ids, uint32_t* steps);
 
+/* CRW_BM for one thread (hint in/out, -1 = none). */
+uint32_t orc_crw_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                        uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* hint,
+                        int32_t* ids, uint32_t* steps);
+
+/* SingleClock with a 64-bit arm (in/out). */
+uint32_t orc_single_clock_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                                 uint64_t* arm, uint32_t budget, int32_t* ids, uint32_t* steps);
+
+/* DistClock: initial arm of a slot, and one arm's serial requests. */
+uint32_t orc_dist_clock_arm(uint64_t seed, uint32_t slot, uint32_t n_pages);
+uint32_t orc_dist_clock_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                               uint32_t* arm, uint32_t budget, int32_t* ids, uint32_t* steps);
+
+/* LinkedList FIFO over a ring of n_pages slots. */
+uint32_t orc_list_build(const uint64_t* bm, uint32_t n_pages, int32_t* q);
+void orc_list_get(uint64_t* bm, uint32_t n_pages, int32_t* q, uint64_t* head, uint64_t* tail,
+                  uint32_t n, int32_t* ids);
+void orc_list_put(uint64_t* bm, uint32_t n_pages, int32_t* q, uint64_t* head, uint64_t* tail,
+                  uint32_t n, const int32_t* ids);
+
+/* RWMalloc getPage, getNPage, releasePages and mallocRW on one wave. */
+uint32_t orc_rwmalloc_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                             uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                             int32_t* ids, uint32_t* steps);
+uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                             const uint32_t* counts, uint64_t req_base, uint64_t seed,
+                             uint32_t budget, int32_t* ids, uint32_t* steps);
+uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                    const int32_t* ids, const uint32_t* counts);
+uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                           int64_t* free_count, uint32_t n, const uint64_t* sizes,
+                           uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* page,
+                           uint64_t* offset, int32_t* header, uint32_t* steps);
+
 /* freePage for every id >= 0; returns pages actually returned (bits that
  * were set).  free_count grows by that number. */
 uint32_t orc_free(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,

@@ -38,6 +38,24 @@ _SIGS = {
                                         C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, _P32,
                                         _PU32]),
     "orc_free": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _P32]),
+    "orc_crw_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                    C.c_uint64, C.c_uint32, _P32, _P32, _PU32]),
+    "orc_single_clock_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU64,
+                                             C.c_uint32, _P32, _PU32]),
+    "orc_dist_clock_arm": (C.c_uint32, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "orc_dist_clock_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32,
+                                           C.c_uint32, _P32, _PU32]),
+    "orc_list_build": (C.c_uint32, [_PU64, C.c_uint32, _P32]),
+    "orc_list_get": (None, [_PU64, C.c_uint32, _P32, _PU64, _PU64, C.c_uint32, _P32]),
+    "orc_list_put": (None, [_PU64, C.c_uint32, _P32, _PU64, _PU64, C.c_uint32, _P32]),
+    "orc_rwmalloc_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                         C.c_uint64, C.c_uint32, C.c_int, _P32, _PU32]),
+    "orc_getnpage_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32,
+                                         C.c_uint64, C.c_uint64, C.c_uint32, _P32, _PU32]),
+    "orc_free_n": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _P32, _PU32]),
+    "orc_malloc_serial": (C.c_uint32, [_PU64, C.c_uint32, C.c_uint32, _PI64, C.c_uint32, _PU64,
+                                       C.c_uint64, C.c_uint64, C.c_uint32, _P32, _PU64, _P32,
+                                       _PU32]),
     "orc_check_alloc": (C.c_int, [_PU64, _PU64, C.c_uint32, C.c_uint32, _P32, _PU32]),
     "orc_rwbm_expected_steps": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int]),
 }
@@ -134,6 +152,62 @@ class State:
                                  _p(steps, _PU32))
         return ids, steps
 
+    def crw(self, n: int, req_base: int, seed: int, hint: int = -1, budget: int = 1 << 20):
+        """CRW_BM for one thread; returns ids, steps, final hint."""
+        ids, steps = self._out(n)
+        h = np.array([hint], dtype=np.int32)
+        lib().orc_crw_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n, req_base,
+                             seed, budget, _p(h, _P32), _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps, int(h[0])
+
+    def single_clock(self, n: int, arm: int = 0, budget: int = 1 << 20):
+        ids, steps = self._out(n)
+        a = np.array([arm], dtype=np.uint64)
+        lib().orc_single_clock_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                      _p(a, _PU64), budget, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps, int(a[0])
+
+    def dist_clock(self, n: int, arm: int, budget: int = 1 << 20):
+        ids, steps = self._out(n)
+        a = np.array([arm], dtype=np.uint32)
+        lib().orc_dist_clock_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                    _p(a, _PU32), budget, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps, int(a[0])
+
+    def rwmalloc(self, n: int, req_base: int, seed: int, order: int, budget: int = 1 << 20):
+        ids, steps = self._out(n)
+        lib().orc_rwmalloc_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                  req_base, seed, budget, order, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps
+
+    def get_n(self, counts: np.ndarray, req_base: int, seed: int, budget: int = 1 << 20):
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        n = counts.size
+        ids, steps = self._out(n)
+        lib().orc_getnpage_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                  _p(counts, _PU32), req_base, seed, budget, _p(ids, _P32),
+                                  _p(steps, _PU32))
+        return ids, steps
+
+    def free_n(self, ids: np.ndarray, counts: np.ndarray) -> int:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        return int(lib().orc_free_n(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64),
+                                    ids.size, _p(ids, _P32), _p(counts, _PU32)))
+
+    def malloc(self, sizes: np.ndarray, page_size: int, req_base: int, seed: int,
+               budget: int = 1 << 20):
+        """(page, offset, header, steps) per request."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+        n = sizes.size
+        page, steps = self._out(n)
+        off = np.zeros(n, dtype=np.uint64)
+        hdr = np.zeros(n, dtype=np.int32)
+        lib().orc_malloc_serial(_p(self.bm, _PU64), self.n_pages, page_size, _p(self.free, _PI64),
+                                n, _p(sizes, _PU64), req_base, seed, budget, _p(page, _P32),
+                                _p(off, _PU64), _p(hdr, _P32), _p(steps, _PU32))
+        return page, off, hdr, steps
+
     def free_ids(self, ids: np.ndarray) -> int:
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         return int(lib().orc_free(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64),
@@ -153,3 +227,33 @@ def check_alloc(before: np.ndarray, after: np.ndarray, n_pages: int, ids: np.nda
 
 def expected_steps(T: float, A: float, n: float, W: int = 64) -> float:
     return float(lib().orc_rwbm_expected_steps(T, A, n, W))
+
+
+def dist_clock_arm(seed: int, slot: int, n_pages: int) -> int:
+    return int(lib().orc_dist_clock_arm(seed, slot, n_pages))
+
+
+class ListState:
+    """The LinkedList FIFO (a ring of n_pages slots) plus its page map."""
+
+    def __init__(self, bm: np.ndarray, n_pages: int):
+        self.bm = np.ascontiguousarray(bm, dtype=np.uint64).copy()
+        self.n_pages = n_pages
+        self.q = np.zeros(n_pages, dtype=np.int32)
+        self.ht = np.zeros(2, dtype=np.uint64)
+        self.ht[1] = lib().orc_list_build(_p(self.bm, _PU64), n_pages, _p(self.q, _P32))
+
+    def get(self, n: int) -> np.ndarray:
+        ids = np.full(n, -1, dtype=np.int32)
+        h, t = self.ht[0:1], self.ht[1:2]
+        lib().orc_list_get(_p(self.bm, _PU64), self.n_pages, _p(self.q, _P32), _p(h, _PU64),
+                           _p(t, _PU64), n, _p(ids, _P32))
+        self.ht[0], self.ht[1] = h[0], t[0]
+        return ids
+
+    def put(self, ids: np.ndarray) -> None:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        h, t = self.ht[0:1].copy(), self.ht[1:2].copy()
+        lib().orc_list_put(_p(self.bm, _PU64), self.n_pages, _p(self.q, _P32), _p(h, _PU64),
+                           _p(t, _PU64), ids.size, _p(ids, _P32))
+        self.ht[0], self.ht[1] = h[0], t[0]

@@ -294,10 +294,6 @@ pbm_ctx launch_ctx(const pbm_handle* h, const pbm_launch& L) {
 
 }  // namespace
 
-namespace pbmi {
-int ring_rebuild(pbm_handle* h, hipStream_t s);
-}
-
 extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
                              uint32_t* d_steps, const pbm_launch* launch, void* stream) {
     PBM_CHECK_HANDLE(h);
@@ -350,7 +346,7 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
             break;
         }
         default:
-            return pbmi::fail(PBM_ERR_UNSUPPORTED, "strategy not available in this build");
+            return pbmi::launch_get_strategy(h, c, L, n, req_base, d_ids, d_steps, s);
     }
     PBM_HIP(hipGetLastError());
     return PBM_OK;
@@ -366,8 +362,11 @@ extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
     if (rc) return rc;
     hipStream_t s = pbmi::as_stream(stream);
     pbm_ctx c = launch_ctx(h, L);
-    if (h->cfg.strategy == PBM_LINKED_LIST)
-        return pbmi::fail(PBM_ERR_UNSUPPORTED, "strategy not available in this build");
+    if (h->cfg.strategy == PBM_LINKED_LIST) {
+        if (L.mode == PBM_MODE_BATCHED)
+            return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
+        return pbmi::launch_free_list(c, L, n, d_ids, s);
+    }
     if (L.mode == PBM_MODE_BATCHED) {
         const uint32_t nb = batched_blocks(h, L, n);
         hipLaunchKernelGGL(k_free_batched, dim3(nb), dim3(256), 0, s, c, n, d_ids);
@@ -381,10 +380,3 @@ extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
     return PBM_OK;
 }
 
-namespace pbmi {
-int ring_rebuild(pbm_handle* h, hipStream_t s) {
-    (void)h;
-    (void)s;
-    return PBM_OK;
-}
-}  // namespace pbmi

@@ -48,6 +48,13 @@ int recount_free(pbm_handle* h, hipStream_t s);
 // ascending order and resets head/tail tickets.
 int ring_rebuild(pbm_handle* h, hipStream_t s);
 
+// One-page get launches of CRW_BM, RWMalloc, SingleClock, DistClock and
+// LinkedList (plain or serial), and the list's free launch.
+int launch_get_strategy(pbm_handle* h, const pbm_ctx& c, const pbm_launch& L, uint32_t n,
+                        uint64_t req_base, int32_t* d_ids, uint32_t* d_steps, hipStream_t s);
+int launch_free_list(const pbm_ctx& c, const pbm_launch& L, uint32_t n, const int32_t* d_ids,
+                     hipStream_t s);
+
 }  // namespace pbmi
 
 #define PBM_HIP(expr)                                                            \

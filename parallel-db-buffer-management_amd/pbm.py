@@ -92,6 +92,16 @@ _SIGS = {
                                 C.POINTER(Launch), C.c_void_p]),
     "pbm_free_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
                                  C.c_void_p]),
+    "pbm_get_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.POINTER(Launch), C.c_void_p]),
+    "pbm_free_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
+                                   C.POINTER(Launch), C.c_void_p]),
+    "pbm_malloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.POINTER(Launch), C.c_void_p]),
+    "pbm_free_ptrs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
+                                C.c_void_p]),
+    "pbm_page_address": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
+    "pbm_page_id": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
     "pbm_graph_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
                                    C.POINTER(Launch), C.POINTER(Launch), C.c_uint32,
                                    C.POINTER(C.c_void_p)]),
@@ -243,6 +253,36 @@ class Pool:
     def free_pages(self, n: int, ids, lnch: Launch | None = None, stream=None) -> None:
         check(lib().pbm_free_pages(self.h, n, _ptr(ids), C.byref(lnch or launch()),
                                    _stream(stream)), "pbm_free_pages")
+
+    def get_n_pages(self, n: int, counts, ids, steps=None, req_base: int = 0,
+                    lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_get_n_pages(self.h, n, req_base, _ptr(counts), _ptr(ids), _ptr(steps),
+                                    C.byref(lnch or launch()), _stream(stream)),
+              "pbm_get_n_pages")
+
+    def free_n_pages(self, n: int, ids, counts, lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_free_n_pages(self.h, n, _ptr(ids), _ptr(counts),
+                                     C.byref(lnch or launch()), _stream(stream)),
+              "pbm_free_n_pages")
+
+    def malloc(self, n: int, sizes, ptrs, steps=None, req_base: int = 0,
+               lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_malloc(self.h, n, req_base, _ptr(sizes), _ptr(ptrs), _ptr(steps),
+                               C.byref(lnch or launch()), _stream(stream)), "pbm_malloc")
+
+    def free_ptrs(self, n: int, ptrs, lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_free_ptrs(self.h, n, _ptr(ptrs), C.byref(lnch or launch()),
+                                  _stream(stream)), "pbm_free_ptrs")
+
+    def page_address(self, page: int) -> int:
+        v = C.c_uint64()
+        check(lib().pbm_page_address(self.h, page, C.byref(v)), "pbm_page_address")
+        return int(v.value)
+
+    def page_id(self, addr: int) -> int:
+        v = C.c_int32()
+        check(lib().pbm_page_id(self.h, addr, C.byref(v)), "pbm_page_id")
+        return int(v.value)
 
 
 class Graph:

@@ -427,3 +427,154 @@ def test_batched_sweep_known_answer():
     ids, steps = st.batched(4, 0, seed, 0, 64, budget=1)
     assert ids.tolist() == expect
     assert np.array_equal(st.bm, orc.prefill(T, 1.0, 0))
+
+
+# ---------------------------------------------------------------- CRW_BM
+
+def test_crw_single_thread_empty_pool_hands_out_consecutive_pages():
+    # known answer: one thread on an empty pool continues in its hinted
+    # word, so after the first random word it gets consecutive ids; at the
+    # word's top bit the hint moves to the next word
+    T = 64 * 32
+    st = orc.State(orc.cleared(T), T)
+    ids, steps, hint = st.crw(150, 0, 5)
+    first = int(ids[0])
+    assert first % 64 == 0
+    exp = [(first + k) % T for k in range(150)]
+    assert ids.tolist() == exp
+    assert steps.tolist() == [1] * 150  # one atomicOr attempt each
+    assert hint == exp[-1]
+
+
+def test_crw_hint_wraps_past_the_last_word():
+    # the reference wraps its hint at word L-1 and so skips the last word;
+    # here the last word is used and the hint wraps to word 0
+    T = 64 * 4
+    bm = orc.prefill(T, 1.0, 0)
+    bm[3] = np.uint64(0)
+    bm[0] = np.uint64(0)
+    st = orc.State(bm, T)
+    ids, _, _ = st.crw(66, 0, 1, hint=3 * 64 - 1)
+    assert ids[:64].tolist() == list(range(192, 256))
+    assert ids[64:].tolist() == [0, 1]
+
+
+# ---------------------------------------------------------------- clocks
+
+def test_single_clock_known_answers():
+    T = 64 * 16
+    st = orc.State(orc.cleared(T), T)
+    ids, steps, arm = st.single_clock(100)
+    assert ids.tolist() == list(range(100)) and steps.tolist() == [1] * 100 and arm == 100
+    # free every other page, wrap the arm to 0: gets return the freed pages
+    # in arm order, each costing two advances (free, then skip a used one)
+    st.free_ids(np.arange(0, 100, 2, dtype=np.int32))
+    ids, steps, _ = st.single_clock(10, arm=T)
+    assert ids.tolist() == list(range(0, 20, 2))
+    assert steps.tolist() == [1] + [2] * 9
+
+
+def test_dist_clock_scans_from_its_arm():
+    T = 64 * 8
+    bm = orc.cleared(T)
+    for p in (10, 11, 12, 300):
+        bm[p // 64] |= np.uint64(1 << (p % 64))
+    st = orc.State(bm, T)
+    ids, steps, arm = st.dist_clock(3, arm=10)
+    assert ids.tolist() == [13, 14, 15] and steps.tolist() == [4, 1, 1] and arm == 16
+    ids, steps, arm = st.dist_clock(2, arm=T - 1)   # wraps to page 0
+    assert ids.tolist() == [T - 1, 0] and arm == 1
+
+
+def test_dist_clock_initial_arm_is_in_range():
+    arms = [orc.dist_clock_arm(42, t, 1000) for t in range(2000)]
+    assert min(arms) >= 0 and max(arms) < 1000 and len(set(arms)) > 800
+
+
+# ---------------------------------------------------------------- LinkedList
+
+def test_list_fifo_known_answers():
+    T = 64 * 4
+    ls = orc.ListState(orc.cleared(T), T)
+    assert ls.get(5).tolist() == [0, 1, 2, 3, 4]
+    ls.put(np.array([3, 1, 4], dtype=np.int32))
+    # the rest of the initial list first, then the freed pages in free order
+    got = ls.get(T - 5 + 3)
+    assert got[: T - 5].tolist() == list(range(5, T)) and got[T - 5:].tolist() == [3, 1, 4]
+    assert ls.get(1).tolist() == [-1]        # empty
+    ls.put(np.array([7], dtype=np.int32))     # freed after it ran empty: found again
+    assert ls.get(1).tolist() == [7]
+
+
+def test_list_built_from_prefill_is_ascending_free_pages():
+    T = 64 * 16
+    bm = orc.prefill(T, 0.7, 3)
+    ls = orc.ListState(bm, T)
+    free = [p for p in range(T) if not (int(bm[p // 64]) >> (p % 64)) & 1]
+    assert ls.get(len(free) + 2).tolist() == free + [-1, -1]
+
+
+# ---------------------------------------------------------------- RWMalloc
+
+def test_rwmalloc_getpage_invariants():
+    T, n = 1 << 14, 4000
+    before = orc.prefill(T, 0.9, 6)
+    A = orc.count_free(before, T)
+    for order in (0, 1):
+        st = orc.State(before, T)
+        ids, _ = st.rwmalloc(n, 0, 6, order)
+        rc, g = orc.check_alloc(before, st.bm, T, ids)
+        assert rc == 0 and g == min(n, A)
+
+
+def test_getnpage_runs_known_answer():
+    # one window: word w0 free bits form runs [0,3) and [10,30); the lowest
+    # lanes needing 2 then 15 pages take the first run then the second
+    T = 64 * 64
+    L = 64
+    seed = None
+    for cand in range(100000):
+        if (orc.lcg(orc.request_seed(cand, 0)) & 0xFFFFFFFF) % L == 0:
+            seed = cand
+            break
+    bm = orc.prefill(T, 1.0, 0)
+    bm[0] = np.uint64(M64 ^ ((1 << 3) - 1) ^ (((1 << 20) - 1) << 10))
+    st = orc.State(bm, T)
+    ids, steps = st.get_n(np.array([2, 15, 0], dtype=np.uint32), 0, seed)
+    assert ids.tolist() == [0, 10, 0]
+    assert steps.tolist() == [1, 1, 0]
+    expect = M64 ^ (1 << 2) ^ (((1 << 5) - 1) << 25)
+    assert int(st.bm[0]) == expect
+
+
+def test_getnpage_zero_run_enumeration_brute_force():
+    # every granted run lies inside one word, was free before, and freeing
+    # the runs restores the map
+    rng = np.random.default_rng(11)
+    T = 1 << 14
+    before = orc.prefill(T, 0.6, 4)
+    counts = rng.integers(1, 9, size=3000).astype(np.uint32)
+    st = orc.State(before, T)
+    ids, _ = st.get_n(counts, 0, 4)
+    seen = np.zeros(T, dtype=bool)
+    for i, c in zip(ids, counts):
+        if i < 0:
+            continue
+        assert i % 64 + c <= 64
+        for p in range(i, i + c):
+            assert not (int(before[p // 64]) >> (p % 64)) & 1 and not seen[p]
+            seen[p] = True
+    st.free_n(ids, counts)
+    assert np.array_equal(st.bm, before)
+
+
+def test_malloc_packs_small_requests_behind_the_lowest_lane():
+    T, ps = 64 * 64, 4096
+    st = orc.State(orc.cleared(T), T)
+    sizes = np.array([100, 5000, 200, 4093, 7], dtype=np.uint64)
+    page, off, hdr, _ = st.malloc(sizes, ps, 0, 9)
+    # small: 100+4 -> 104, 200+4 -> 204, 7+4 -> 12 (rounded to 4): one page
+    assert hdr.tolist() == [1, 2, 0, 2, 0]
+    assert off.tolist() == [0, 0, 104, 0, 104 + 204]
+    assert page[2] == page[0] and page[4] == page[0]
+    assert len({int(page[0]), int(page[1]), int(page[3])}) == 3
