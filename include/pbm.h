@@ -108,6 +108,12 @@ typedef struct pbm_ctx {
     unsigned long long* ring_head;   /* LinkedList: next ticket to dequeue              */
     unsigned long long* ring_tail;   /* LinkedList: next ticket to enqueue              */
     long long*          phase;       /* batched calls: [1] wave completion counter      */
+    uint32_t*           sb_rank;     /* n_sb+1 entries: free pages before superblock s
+                                        (a superblock is 64 words)                      */
+    uint32_t*           w_rank;      /* n_words entries: free pages before word w inside
+                                        its superblock                                  */
+    unsigned long long* snap;        /* n_words entries: free bits of each word when
+                                        the rank pre-pass ran (1 = free)                */
     uint32_t n_pages;
     uint32_t n_words;
     uint32_t page_size;
@@ -115,6 +121,8 @@ typedef struct pbm_ctx {
     uint32_t max_threads;
     int32_t  order;
     uint64_t seed;
+    uint32_t n_sb;                   /* superblocks: ceil(n_words / 64)                 */
+    uint32_t reserved0;
 } pbm_ctx;
 
 /* Launch shape of a batch call. */
@@ -130,6 +138,10 @@ typedef struct pbm_launch {
 #define PBM_LAUNCH_NO_COUNTERS 1u  /* skip the device counter updates                  */
 #define PBM_LAUNCH_ONE_WAVE    2u  /* batched mode on a single wave: the schedule of the
                                       serial oracle (orc_batched_serial)                  */
+#define PBM_LAUNCH_RANK        4u  /* batched mode by rank: a pre-pass snapshots the free
+                                      bits and ranks them; request r of the call takes the
+                                      free page of rank r (map order), so the chunks of a
+                                      call claim disjoint pages with one atomicOr per word */
 
 typedef struct pbm_handle pbm_handle;
 

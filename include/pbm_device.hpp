@@ -67,6 +67,15 @@ __host__ __device__ __forceinline__ int32_t request_seed(u64 seed, u64 r) {
     return (int32_t)(uint32_t)(mix64(seed + (r + 1) * kGolden) >> 32);
 }
 
+/* Map an LCG state to [0, L) with its high bits.  The reference reduces
+ * with (unsigned)state % L (RandomWalkBasic_BM.cuh:71), which for a
+ * power-of-two L keeps the low bits: those follow x' = a*x + c mod L, one
+ * cycle shared by every walk, so concurrent walks trail each other through
+ * the same words.  The 31-bit residue's high bits do not. */
+__host__ __device__ __forceinline__ uint32_t lcg_to_range(int32_t state, uint32_t L) {
+    return (uint32_t)(((unsigned long long)((uint32_t)state & 0x7FFFFFFFu) * L) >> 31);
+}
+
 /* Pre-fill hash of page i: output i+1 of splitmix64 seeded with
  * mix64(seed ^ kPrefillSalt). */
 __host__ __device__ __forceinline__ u64 page_hash(u64 seed, u64 i) {
@@ -337,14 +346,17 @@ __device__ inline void free_page(const pbm_ctx& c, int32_t id, Stats* stats = nu
  * Collaborative window step (CollabRW_BM.cuh:114-160, wave64 rebuild)
  *
  * Probing lanes each read one word of the window; `need` lanes ask for a
- * page.  Free bits are designated to needy lanes in rank order, either
- * lane-major (PBM_ORDER_PACKED) or round-robin over words
- * (PBM_ORDER_REFERENCE, the order of the reference's has/need ballot
- * loop, :140-153).  Each giver claims only its designated bits with one
- * atomicOr and owns mask & ~old, so nothing is ever released; a needy lane
- * whose designated bit was taken by a racing wave stays needy.  Replaces
- * the reference's atomicExch of whole words and the leftover release
- * atomicExch(&w, ~r) (:155-156), which overwrites concurrent frees.
+ * page.  The lanes whose words the order reaches before the needy count
+ * (lane-major, PBM_ORDER_PACKED, or round-robin over words,
+ * PBM_ORDER_REFERENCE, the order of the reference's has/need ballot loop,
+ * :140-153) each take all free bits they saw with one atomicOr and own
+ * free & ~old, whatever no racing wave took first; the owned bits go to
+ * the needy lanes in the same order and the unused ones are given back
+ * with one atomicAnd.  Where the reference exchanges every word of the
+ * window to all-ones and releases leftovers with atomicExch(&w, ~r)
+ * (:126-128, :155-156), which overwrites concurrent frees, only the words
+ * needed are touched and the release is an atomicAnd.  A needy lane left
+ * without a bit (its bits went to a racing wave) stays needy.
  * ------------------------------------------------------------------- */
 
 struct WindowResult {
@@ -354,76 +366,93 @@ struct WindowResult {
 
 /* One step over the words held by probing lanes: `word` is the lane's word
  * index and `w` the value it read (kFull for lanes that do not probe).  On
- * return, a needy lane that got a page has need=false and id set. */
+ * return, a needy lane that got a page has need=false and id set, and
+ * *w_after (when given) is the lane's word as its own claim left it (w when
+ * the lane claimed nothing). */
 __device__ inline WindowResult window_claim(const pbm_ctx& c, uint32_t word, u64 w,
-                                            bool& need, int32_t& id, int32_t order, Stats& st) {
+                                            bool& need, int32_t& id, int32_t order, Stats& st,
+                                            u64* w_after = nullptr) {
+    if (w_after) *w_after = w;
     WindowResult res{0u, 0u};
-    u64 fr = ~w;
-    unsigned cnt = (unsigned)__popcll(fr);
-    u64 needy = ballot(need);
-    unsigned n_need = (unsigned)__popcll(needy);
-    unsigned q = rank_in(needy);
+    const u64 fr = ~w;
+    const unsigned cnt = (unsigned)__popcll(fr);
+    const u64 needy = ballot(need);
+    const unsigned n_need = (unsigned)__popcll(needy);
+    const unsigned q = rank_in(needy);
     unsigned total = 0;
-    unsigned pre = scan7(cnt, total);
+    const unsigned pre = scan7(cnt, total);
     res.seen_free = total;
     if (n_need == 0 || total == 0) return res;
+    res.designated = n_need < total ? n_need : total;
 
-    unsigned k = 0;          /* bits this lane gives                     */
-    unsigned giver = 0;      /* lane designated to serve this needy lane */
-    unsigned gbit = 0;       /* rank of that bit among the giver's free bits */
-    bool designated = false;
+    // Givers: the lanes the order reaches before n_need bits.  Round-robin
+    // reaches a lane only if round 0 does (a later round needs every lane
+    // of round 0 to have given).
+    bool giver;
+    if (order == PBM_ORDER_REFERENCE) {
+        const u64 B0 = ballot(cnt > 0);
+        giver = cnt > 0 && rank_in(B0) < n_need;
+    } else {
+        giver = cnt > 0 && pre < n_need;
+    }
+    // Grab every free bit a giver saw; own what no racing wave took first.
+    u64 owned = 0, old = 0;
+    if (giver) {
+        old = atom_or(&c.bitmap[word], fr);
+        st.atomics++;
+        owned = fr & ~old;
+    }
+    // Hand the owned bits to the needy lanes in the order.
+    const unsigned oc = (unsigned)__popcll(owned);
+    unsigned used = 0, src = 0, gbit = 0;
+    bool assigned = false;
     if (order == PBM_ORDER_REFERENCE) {
         unsigned H = 0;
         for (unsigned j = 0; j < 64u && H < n_need; ++j) {
-            u64 B = ballot(cnt > j);
-            unsigned h = (unsigned)__popcll(B);
+            const u64 B = ballot(oc > j);
+            const unsigned h = (unsigned)__popcll(B);
             if (h == 0) break;
-            unsigned take = (n_need - H < h) ? n_need - H : h;
-            if (cnt > j && rank_in(B) < take) k++;
+            const unsigned take = (n_need - H < h) ? n_need - H : h;
+            if (oc > j && rank_in(B) < take) used++;
             if (need && q >= H && q < H + take) {
-                giver = select_bit(B, q - H);
+                src = select_bit(B, q - H);
                 gbit = j;
-                designated = true;
+                assigned = true;
             }
             H += take;
         }
-        res.designated = H;
     } else {
-        unsigned avail = n_need > pre ? n_need - pre : 0u;
-        k = cnt < avail ? cnt : avail;
-        u64 G = ballot(k > 0);
+        unsigned o_tot = 0;
+        const unsigned opre = scan7(oc, o_tot);
+        const unsigned avail = n_need > opre ? n_need - opre : 0u;
+        used = oc < avail ? oc : avail;
+        u64 G = ballot(used > 0);
         unsigned base = 0;
         while (G) {
-            unsigned g = (unsigned)__ffsll((long long)G) - 1u;
+            const unsigned g = (unsigned)__ffsll((long long)G) - 1u;
             G &= G - 1ull;
-            unsigned kg = readlane(k, g);
-            if (need && q >= base && q < base + kg) {
-                giver = g;
+            const unsigned ug = readlane(used, g);
+            if (need && q >= base && q < base + ug) {
+                src = g;
                 gbit = q - base;
-                designated = true;
+                assigned = true;
             }
-            base += kg;
+            base += ug;
         }
-        res.designated = base;
     }
-
-    u64 mask = lowest_bits(fr, k);
-    u64 owned = 0;
-    if (mask) {
-        u64 old = atom_or(&c.bitmap[word], mask);
+    // Give back what the wave does not use.
+    const u64 extra = owned & ~lowest_bits(owned, used);
+    if (extra) {
+        atom_and(&c.bitmap[word], ~extra);
         st.atomics++;
-        owned = mask & ~old;
     }
-    unsigned src = designated ? giver : lane_id();
-    u64 g_owned = shfl64(owned, src);
-    u64 g_free = shfl64(fr, src);
-    uint32_t g_word = shfl32(word, src);
-    if (designated) {
-        unsigned b = select_bit(g_free, gbit);
-        if ((g_owned >> b) & 1ull) {
-            id = (int32_t)(g_word * 64u + b);
-            need = false;
-        }
+    if (w_after && giver) *w_after = (old | fr) & ~extra;
+    const unsigned from = assigned ? src : lane_id();
+    const u64 g_owned = shfl64(owned, from);
+    const uint32_t g_word = shfl32(word, from);
+    if (assigned) {
+        id = (int32_t)(g_word * 64u + select_bit(g_owned, gbit));
+        need = false;
     }
     return res;
 }
@@ -671,11 +700,13 @@ __device__ inline int32_t get_page_dist_clock(const pbm_ctx& c, uint32_t slot,
 
 constexpr uint32_t kRingSpin = 1u << 22;
 
+// A slot word carries both its sequence number and the page id, so relaxed
+// 64-bit atomics suffice: nothing else is published through it.
 __device__ __forceinline__ u64 ld_slot(const u64* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_slot(u64* p, u64 v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ inline int32_t get_page_list(const pbm_ctx& c, uint32_t* steps_out,
@@ -751,7 +782,8 @@ __device__ inline void free_page_list(const pbm_ctx& c, int32_t id, uint32_t* st
             st_slot(slot, ((u64)(uint32_t)(t + 1u) << 32) | (uint32_t)id);
             st.frees++;
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // a consumer that reserves this unit before the slot store is
+        // visible waits on the slot's sequence number
         release_one(c, ok);
     }
     if (steps_out) {

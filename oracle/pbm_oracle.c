@@ -53,6 +53,13 @@ int32_t orc_request_seed(uint64_t seed, uint64_t r) {
     return (int32_t)(uint32_t)(orc_mix64(seed + (r + 1) * GOLDEN) >> 32);
 }
 
+/* High-bit reduction of an LCG state to [0, L) used by the batched
+ * allocator's window jumps (the reference's (unsigned)state % L keeps the
+ * low bits, which cycle with period L for a power-of-two L). */
+static uint32_t lcg_to_range(int32_t state, uint32_t L) {
+    return (uint32_t)(((u64)((uint32_t)state & 0x7FFFFFFFu) * L) >> 31);
+}
+
 uint64_t orc_page_hash(uint64_t seed, uint64_t i) {
     return orc_mix64(orc_mix64(seed ^ PREFILL_SALT) + (i + 1) * GOLDEN);
 }
@@ -293,29 +300,42 @@ uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, 
     return granted_total;
 }
 
-#define SWEEP_WORDS 16u
+#define SWEEP_MAX 256u
 
 /* Batched persistent allocation on one wave (pbm_alloc.hip k_get_batched):
- * window mode makes up to `budget` random jumps per chunk to windows of
- * 64*J words, kept across chunks while they show free bits; after the
- * budget the wave sweeps for good: lane l scans words start+16l .. +15,
- * offers its first non-full word, and the sweep advances by 1024 words when
- * no lane saw a second non-full word. */
+ * chunk ch's first window starts at its home word ch*64*L/granted, then
+ * window mode makes up to `budget` random jumps to windows of 64*J words,
+ * kept while they show free bits; after the budget the chunk sweeps: with
+ * S = min(256, ceil(L/64)), lane l
+ * owns words start+S*l .. +S-1 with a cursor, offers the first non-full word
+ * from its cursor each step and moves past a word once it is full; when
+ * every lane has finished its segment the next span of 64*S words starts
+ * (a wave that never jumped first draws a random start). */
 uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                             uint64_t req_base, uint64_t seed, uint32_t budget, int order,
                             uint32_t cluster_words, int32_t* ids, uint32_t* steps) {
     const uint32_t L = orc_n_words(n_pages);
     const uint32_t J = cluster_words / 64u;
-    const uint32_t sweep_span = 64u * SWEEP_WORDS;
-    const uint32_t max_sweeps = 2u * ((L + sweep_span - 1u) / sweep_span) + 2u;
+    const uint32_t seg = (L + 63u) / 64u < SWEEP_MAX ? (L + 63u) / 64u : SWEEP_MAX;
+    const uint32_t sweep_span = 64u * seg;
+    const uint32_t max_dry = 2u * ((L + sweep_span - 1u) / sweep_span) + 2u;
     /* phase admission: the first min(n, free) requests by index */
     int64_t old = *free_count;
     int64_t granted = old < 0 ? 0 : (old > (int64_t)n ? (int64_t)n : old);
     *free_count -= granted;
     int32_t wseed = orc_request_seed(seed ^ WAVE_SALT, req_base + 0);
-    int have = 0, sweep = 0;
-    uint32_t start = 0, got = 0;
+    uint32_t got = 0;
+    uint32_t cur[64], seg_len[64];
+    for (int l = 0; l < 64; ++l) {
+        uint32_t first = (uint32_t)l * seg;
+        seg_len[l] = first >= L ? 0u : (L - first < seg ? L - first : seg);
+    }
     for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        /* the chunk's home word: its share of the grants */
+        const uint32_t home = granted > 0 ? (uint32_t)(((u64)ch * 64u * L) / (u64)granted % L) : 0u;
+        int have = 0, sweep = 0, homed = 0, found_pass = 0;
+        uint32_t start = 0;
+        for (int l = 0; l < 64; ++l) cur[l] = 0;
         int need[64], valid[64];
         int32_t cid[64];
         uint32_t cst[64];
@@ -326,17 +346,22 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
             cid[l] = -1;
             cst[l] = 0;
         }
-        uint32_t jumps = 0, sweeps = 0, iters = 0;
-        const uint32_t max_iters = budget + max_sweeps + 65536u;
+        uint32_t jumps = 0, dry = 0, iters = 0;
+        const uint32_t max_iters = budget + 4u * max_dry * seg + 65536u;
         while (any(need)) {
             if (++iters > max_iters) break;
             if (!have) {
-                if (jumps < budget) {
+                if (!homed) {
+                    start = home;
+                    homed = 1;
+                } else if (jumps < budget) {
                     wseed = orc_lcg(wseed);
-                    start = (uint32_t)wseed % L;
+                    start = lcg_to_range(wseed, L);
                     jumps++;
                 } else {
                     sweep = 1;
+                    for (int l = 0; l < 64; ++l) cur[l] = 0;
+                    found_pass = 0;
                 }
                 have = 1;
             }
@@ -363,30 +388,40 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
                 }
                 if (examined_all && seen_all <= desig_all) have = 0;
             } else {
-                int probe[64], more = 0;
+                int probe[64];
                 uint32_t word[64];
                 for (int l = 0; l < 64; ++l) {
                     probe[l] = 0;
                     word[l] = 0;
-                    for (uint32_t i = 0; i < SWEEP_WORDS; ++i) {
-                        uint32_t off = (uint32_t)l * SWEEP_WORDS + i;
-                        if (off >= L) break;
-                        uint32_t idx = start + off;
+                    while (cur[l] < seg_len[l]) {
+                        uint32_t idx = start + (uint32_t)l * seg + cur[l];
                         if (idx >= L) idx -= L;
-                        if (bm[idx] == FULL) continue;
-                        if (!probe[l]) {
+                        if (bm[idx] != FULL) {
                             probe[l] = 1;
                             word[l] = idx;
-                        } else {
-                            more = 1;
+                            break;
                         }
+                        cur[l]++;
                     }
                 }
                 uint32_t seen;
                 collab_claim(bm, probe, word, need, cid, order, &seen);
-                if (any(need) && !more) {
+                for (int l = 0; l < 64; ++l)
+                    if (probe[l] && bm[word[l]] == FULL) cur[l]++;
+                if (seen) found_pass = 1;
+                int open_seg = 0;
+                for (int l = 0; l < 64; ++l) open_seg |= cur[l] < seg_len[l];
+                if (!open_seg) {
+                    if (any(need)) {
+                        if (!found_pass) {
+                            if (++dry > max_dry) break;
+                        } else {
+                            dry = 0;
+                        }
+                    }
+                    found_pass = 0;
+                    for (int l = 0; l < 64; ++l) cur[l] = 0;
                     start = (uint32_t)(((u64)start + sweep_span) % L);
-                    if (++sweeps > max_sweeps) break;
                 }
             }
         }
@@ -399,6 +434,45 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
         }
     }
     return got;
+}
+
+/* Batched allocation by rank (PBM_LAUNCH_RANK, pbm_alloc.hip
+ * k_rank_local/k_rank_scan/rank_window): the call admits min(n, free)
+ * requests and request r takes the free page of rank r, i.e. the r-th free
+ * page in map order.  Chunk c's first window starts at the word holding
+ * rank 64c; a request served in the k-th 64-word window from there has
+ * steps k+1.  The outcome does not depend on the schedule. */
+uint32_t orc_batched_rank(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                          int32_t* ids, uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    int64_t old = *free_count;
+    int64_t granted = old < 0 ? 0 : (old > (int64_t)n ? (int64_t)n : old);
+    int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (granted > 0 ? (size_t)granted : 1u));
+    if (!order) return 0;
+    int64_t k = 0;
+    for (uint32_t w = 0; w < L && k < granted; ++w) {
+        u64 fr = ~bm[w];
+        while (fr && k < granted) {
+            int b = __builtin_ctzll(fr);
+            fr &= fr - 1u;
+            order[k++] = (int32_t)(w * 64u + (uint32_t)b);
+        }
+    }
+    *free_count -= k;
+    for (uint32_t r = 0; r < n; ++r) {
+        ids[r] = -1;
+        if (steps) steps[r] = 0;
+    }
+    for (int64_t r = 0; r < k; ++r) {
+        uint32_t w0 = (uint32_t)order[(r / 64) * 64] >> 6;
+        uint32_t wd = (uint32_t)order[r] >> 6;
+        ids[r] = order[r];
+        if (steps) steps[r] = (wd - w0) / 64u + 1u;
+    }
+    for (int64_t r = 0; r < k; ++r)
+        bm[(uint32_t)order[r] >> 6] |= (u64)1 << ((uint32_t)order[r] & 63u);
+    free(order);
+    return (uint32_t)k;
 }
 
 /* ------------------------------------------------------------ CRW_BM */

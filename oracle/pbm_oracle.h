@@ -50,6 +50,10 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
                             uint64_t req_base, uint64_t seed, uint32_t budget, int order,
                             uint32_t cluster_words, int32_t* ids, uint32_t* steps);
 
+/* Batched allocation by rank: request r takes the r-th free page. */
+uint32_t orc_batched_rank(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                          int32_t* ids, uint32_t* steps);
+
 /* CRW_BM for one thread (hint in/out, -1 = none). */
 uint32_t orc_crw_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                         uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* hint,

@@ -38,6 +38,7 @@ _SIGS = {
                                         C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, _P32,
                                         _PU32]),
     "orc_free": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _P32]),
+    "orc_batched_rank": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _P32, _PU32]),
     "orc_crw_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
                                     C.c_uint64, C.c_uint32, _P32, _P32, _PU32]),
     "orc_single_clock_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU64,
@@ -150,6 +151,12 @@ class State:
         lib().orc_batched_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
                                  req_base, seed, budget, order, cluster_words, _p(ids, _P32),
                                  _p(steps, _PU32))
+        return ids, steps
+
+    def batched_rank(self, n: int):
+        ids, steps = self._out(n)
+        lib().orc_batched_rank(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                               _p(ids, _P32), _p(steps, _PU32))
         return ids, steps
 
     def crw(self, n: int, req_base: int, seed: int, hint: int = -1, budget: int = 1 << 20):

@@ -59,16 +59,162 @@ __global__ __launch_bounds__(1024) void k_get_collab(pbm_ctx c, uint32_t n, uint
     pbm::flush_stats(c, st);
 }
 
-// Persistent batched allocation.  Each wave serves chunks of 64 requests.
-// Window mode: up to `budget` random jumps per chunk, each to a window of
-// 64*J consecutive words kept across chunks while it still shows free bits.
-// Sweep mode (after the budget, and for the rest of the wave): every lane
-// scans kSweepWords consecutive words and offers its first non-full one, so
-// one step covers 64*kSweepWords words of the map; the wave advances when
-// no lane has a second non-full word in its segment.
-constexpr uint32_t kSweepWords = 16;
+// ---- rank pre-pass: free-rank of every page at the start of a batched call
 
-template <int J>
+// Step 1: one wave per 64-word superblock snapshots the free bits of its
+// words, ranks the words inside the superblock, and stores the
+// superblock's free total at sb_rank[s+1].
+__global__ __launch_bounds__(256) void k_rank_local(pbm_ctx c) {
+    using namespace pbm;
+    const unsigned lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t sb = wave; sb < c.n_sb; sb += n_waves) {
+        const uint32_t w = sb * 64u + lane;
+        u64 fm = 0;
+        if (w < c.n_words) fm = ~ld_word(&c.bitmap[w]);
+        unsigned tot = 0;
+        const unsigned pre = scan7((unsigned)__popcll(fm), tot);
+        if (w < c.n_words) {
+            c.snap[w] = fm;
+            c.w_rank[w] = pre;
+        }
+        if (lane == 0) c.sb_rank[sb + 1u] = tot;
+    }
+}
+
+// Step 2: one workgroup turns the totals into prefixes in place:
+// sb_rank[s] = free pages before superblock s, sb_rank[n_sb] = all free.
+__global__ __launch_bounds__(1024) void k_rank_scan(pbm_ctx c) {
+    __shared__ uint32_t part[1024];
+    const uint32_t n = c.n_sb;
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (uint32_t i = 0; i < per && b + i < n; ++i) sum += c.sb_rank[b + i + 1u];
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {
+        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? part[threadIdx.x - 1u] : 0u;
+    for (uint32_t i = 0; i < per && b + i < n; ++i) {
+        run += c.sb_rank[b + i + 1u];
+        c.sb_rank[b + i + 1u] = run;
+    }
+    if (threadIdx.x == 0) c.sb_rank[0] = 0;
+}
+
+// First word holding free rank R (R < sb_rank[n_sb]): a 64-way search over
+// the superblock prefixes, then a ballot over the superblock's 64 words.
+__device__ inline uint32_t rank_search(const pbm_ctx& c, uint32_t R) {
+    using namespace pbm;
+    const unsigned lane = lane_id();
+    uint32_t lo = 0, hi = c.n_sb;
+    while (hi - lo > 1u) {
+        const uint32_t stride = (hi - lo + 63u) / 64u;
+        const uint32_t idx = lo + lane * stride;
+        const uint32_t v = idx < hi ? c.sb_rank[idx] : 0xFFFFFFFFu;
+        const u64 m = ballot(idx < hi && v <= R);
+        const unsigned t = 63u - (unsigned)__clzll((long long)m);
+        lo += t * stride;
+        hi = lo + stride < hi ? lo + stride : hi;
+    }
+    const uint32_t w = lo * 64u + lane;
+    bool hit = false;
+    if (w < c.n_words) {
+        const uint32_t rk = c.sb_rank[lo] + c.w_rank[w];
+        const uint32_t pc = (uint32_t)__popcll(c.snap[w]);
+        hit = rk <= R && R < rk + pc;
+    }
+    const u64 m = ballot(hit);
+    return lo * 64u + (uint32_t)__ffsll((long long)m) - 1u;
+}
+
+// One window of the rank path: lane l holds word w0+l, whose snapshot free
+// bits have ranks rk .. rk+pc-1; the bits with ranks in [R, R_end) belong
+// to this chunk and are claimed with one atomicOr per word.  Lane q of the
+// chunk (request of rank R+q) takes the bit of rank R+q.  *more tells
+// whether the chunk's range goes on past this window.
+__device__ inline void rank_window(const pbm_ctx& c, uint32_t w0, uint32_t R, uint32_t R_end,
+                                   bool& need, int32_t& id, pbm::Stats& st, bool& more) {
+    using namespace pbm;
+    const unsigned lane = lane_id();
+    const uint32_t wd = w0 + lane;
+    u64 fm = 0;
+    uint32_t rk = 0xFFFFFFFFu, pc = 0;
+    if (wd < c.n_words) {
+        fm = c.snap[wd];
+        rk = c.sb_rank[wd >> 6] + c.w_rank[wd];
+        pc = (uint32_t)__popcll(fm);
+        st.probes++;
+    }
+    uint32_t lo = 0, hi = 0;
+    if (wd < c.n_words) {
+        lo = rk >= R ? 0u : (R - rk < pc ? R - rk : pc);
+        hi = rk >= R_end ? 0u : (R_end - rk < pc ? R_end - rk : pc);
+        if (hi < lo) hi = lo;
+    }
+    const u64 mask = lowest_bits(fm, hi) & ~lowest_bits(fm, lo);
+    u64 owned = 0;
+    if (mask) {
+        const u64 old = atom_or(&c.bitmap[wd], mask);
+        st.atomics++;
+        owned = mask & ~old;
+    }
+    // the range continues iff the last word of the window ends below R_end
+    const u64 valid = ballot(wd < c.n_words);
+    more = false;
+    if (valid == kFull) {
+        const uint32_t end63 = readlane(rk + pc, 63u);
+        more = end63 < R_end;
+    }
+    unsigned src = lane, j = 0;
+    bool assigned = false;
+    u64 G = ballot(mask != 0);
+    while (G) {
+        const unsigned g = (unsigned)__ffsll((long long)G) - 1u;
+        G &= G - 1ull;
+        const uint32_t rk_g = readlane(rk, g), lo_g = readlane(lo, g), hi_g = readlane(hi, g);
+        const uint32_t first_q = rk_g + lo_g - R;
+        if (need && lane >= first_q && lane < first_q + (hi_g - lo_g)) {
+            src = g;
+            j = lo_g + (lane - first_q);
+            assigned = true;
+        }
+    }
+    const u64 s_owned = shfl64(owned, src);
+    const u64 s_fm = shfl64(fm, src);
+    const uint32_t s_wd = shfl32(wd, src);
+    if (assigned) {
+        const unsigned bit = select_bit(s_fm, j);
+        if ((s_owned >> bit) & 1ull) {
+            id = (int32_t)(s_wd * 64u + bit);
+            need = false;
+        }
+    }
+}
+
+// Persistent batched allocation.  Each wave serves chunks of 64 requests.
+// Window mode: chunk ch's first window starts at its home word
+// ch*64*L/granted (its share of the grants); after that, up to `budget`
+// random jumps (LCG steps mapped to a word by their high bits,
+// lcg_to_range).  A window is 64*J consecutive words, kept while it still
+// shows free bits.
+// Sweep mode (after the budget, for the rest of the chunk): the map is cut
+// into spans of 64*S words, S = min(kSweepMax, ceil(L/64)); lane l owns the
+// S words start + l*S .. and keeps a cursor in them.  Each step every lane
+// reads forward from its cursor to its first non-full word and offers it;
+// the cursor moves past a word once the lane has seen it full.  When every
+// lane has finished its segment the pass is over and the next span starts
+// (the same span again when one span covers the map).
+constexpr uint32_t kSweepMax = 256;
+constexpr uint32_t kSweepRound = 16;
+
+template <int J, bool RANK>
 __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                      int32_t* __restrict__ ids,
                                                      uint32_t* __restrict__ steps_out,
@@ -78,36 +224,65 @@ __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t L = c.n_words;
-    const uint32_t sweep_span = 64u * kSweepWords;
-    const uint32_t max_sweeps = 2u * ((L + sweep_span - 1u) / sweep_span) + 2u;
+    const uint32_t seg = (L + 63u) / 64u < kSweepMax ? (L + 63u) / 64u : kSweepMax;
+    const uint32_t sweep_span = 64u * seg;
+    const uint32_t max_dry = 2u * ((L + sweep_span - 1u) / sweep_span) + 2u;
     const uint32_t n_chunks = (n + 63u) / 64u;
     const long long avail =
         __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const long long granted = avail < 0 ? 0 : (avail > (long long)n ? (long long)n : avail);
     Stats st;
     int32_t wseed = request_seed(c.seed ^ kWaveSalt, req_base + wave);
-    bool have = false, sweep = false;
-    uint32_t start = 0;
+    const uint32_t first = lane * seg;
+    const uint32_t seg_len = first >= L ? 0u : (L - first < seg ? L - first : seg);
     for (uint32_t ch = wave; ch < n_chunks; ch += n_waves) {
         const uint32_t r = ch * 64u + lane;
         const bool valid = r < n;
         bool need = valid && (long long)r < granted;
         int32_t id = -1;
-        uint32_t steps = 0, jumps = 0, sweeps = 0, iters = 0;
-        // A kept window is retried without a new jump only while it showed
-        // more free bits than the wave designated (lost races); the cap
-        // bounds the loop even so.
-        const uint32_t max_iters = budget + max_sweeps + 65536u;
+        uint32_t steps = 0, jumps = 0, dry = 0, iters = 0;
+        // The chunk's first window starts in its own slice of the map, sized
+        // by its share of the call's grants, so concurrent chunks begin on
+        // disjoint words.
+        const uint32_t home =
+            granted > 0 ? (uint32_t)(((u64)ch * 64u * L) / (u64)granted % L) : 0u;
+        bool have = false, sweep = false, homed = false, found_pass = false;
+        uint32_t start = 0, cur = 0;
+        if (RANK) {
+            // Request r takes the free page of rank r; the walk below only
+            // serves what a concurrent writer took away.
+            const uint32_t total = c.sb_rank[c.n_sb];
+            const uint32_t R = ch * 64u;
+            const uint32_t lim = granted < (long long)total ? (uint32_t)granted : total;
+            if (R < lim) {
+                const uint32_t R_end = R + 64u < lim ? R + 64u : lim;
+                const uint32_t w0 = rank_search(c, R);
+                bool more = true;
+                for (uint32_t k = 0; more && ballot(need) && k <= L / 64u + 1u; ++k) {
+                    if (need) ++steps;
+                    rank_window(c, w0 + 64u * k, R, R_end, need, id, st, more);
+                }
+            }
+        }
+        // A kept window or a sweep position is retried only while it showed
+        // free bits (lost races); the cap bounds the loop even so.
+        const uint32_t max_iters = budget + 4u * max_dry * seg + 65536u;
         while (ballot(need)) {
             if (++iters > max_iters) break;
             if (!have) {
-                if (jumps < budget) {
+                if (!homed) {
+                    start = home;
+                    homed = true;
+                } else if (jumps < budget) {
                     wseed = lcg(wseed);
-                    start = (uint32_t)wseed % L;
+                    start = lcg_to_range(wseed, L);
                     ++jumps;
                 } else {
+                    // the sweep starts at the last window
                     st.fallbacks++;
                     sweep = true;
+                    cur = 0;
+                    found_pass = false;
                 }
                 have = true;
             }
@@ -139,37 +314,52 @@ __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint
                 }
                 if (examined_all && seen <= desig) have = false;
             } else {
-                u64 v[kSweepWords];
-#pragma unroll
-                for (uint32_t i = 0; i < kSweepWords; ++i) {
-                    const uint32_t off = lane * kSweepWords + i;
-                    uint32_t idx = start + off;
-                    if (idx >= L) idx -= L;
-                    v[i] = kFull;
-                    if (off < L) v[i] = ld_word(&c.bitmap[idx]);
-                }
-                st.probes += L < (lane + 1u) * kSweepWords
-                                 ? (L > lane * kSweepWords ? L - lane * kSweepWords : 0u)
-                                 : kSweepWords;
                 uint32_t cand = 0;
                 u64 cval = kFull;
-                bool more = false;
+                while (cur < seg_len && cval == kFull) {
+                    u64 v[kSweepRound];
 #pragma unroll
-                for (uint32_t i = 0; i < kSweepWords; ++i) {
-                    if (v[i] != kFull) {
-                        if (cval == kFull) {
-                            uint32_t idx = start + lane * kSweepWords + i;
-                            cand = idx >= L ? idx - L : idx;
-                            cval = v[i];
-                        } else {
-                            more = true;
+                    for (uint32_t i = 0; i < kSweepRound; ++i) {
+                        const uint32_t off = cur + i;
+                        uint32_t idx = start + first + off;
+                        if (idx >= L) idx -= L;
+                        v[i] = kFull;
+                        if (off < seg_len) {
+                            v[i] = ld_word(&c.bitmap[idx]);
+                            st.probes++;
                         }
                     }
+                    uint32_t hit = kSweepRound;
+#pragma unroll
+                    for (uint32_t i = kSweepRound; i-- > 0;)
+                        if (v[i] != kFull) hit = i;
+                    if (hit < kSweepRound) {
+                        cur += hit;
+                        uint32_t idx = start + first + cur;
+                        cand = idx >= L ? idx - L : idx;
+#pragma unroll
+                        for (uint32_t i = 0; i < kSweepRound; ++i)
+                            if (i == hit) cval = v[i];
+                    } else {
+                        cur = cur + kSweepRound < seg_len ? cur + kSweepRound : seg_len;
+                    }
                 }
-                window_claim(c, cand, cval, need, id, c.order, st);
-                if (ballot(need) && !ballot(more)) {
+                u64 after = kFull;
+                WindowResult wr = window_claim(c, cand, cval, need, id, c.order, st, &after);
+                if (cval != kFull && after == kFull) ++cur;
+                if (wr.seen_free) found_pass = true;
+                if (!ballot(cur < seg_len)) {
+                    // pass over this span finished
+                    if (ballot(need)) {
+                        if (!found_pass) {
+                            if (++dry > max_dry) break;
+                        } else {
+                            dry = 0;
+                        }
+                    }
+                    found_pass = false;
+                    cur = 0;
                     start = (uint32_t)(((u64)start + sweep_span) % L);
-                    if (++sweeps > max_sweeps) break;
                 }
             }
         }
@@ -183,15 +373,17 @@ __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint
         st.steps += steps;
     }
     flush_stats(c, st);
-    // The last workgroup to finish applies the reservation; every wave has
-    // read free_count before its workgroup counts itself done.
+    // The last workgroup to finish applies the reservation.  Every wave used
+    // the value it loaded from free_count before its workgroup counts itself
+    // done, so the subtraction cannot precede any of those loads; no fence
+    // is needed (an acquire/release here would write back and invalidate
+    // the whole L2 of the XCD under the workgroups still running).
     __syncthreads();
     if (threadIdx.x == 0) {
-        u64 prev = __hip_atomic_fetch_add((u64*)&c.phase[1], 1ull, __ATOMIC_ACQ_REL,
-                                          __HIP_MEMORY_SCOPE_AGENT);
+        u64 prev = atom_add_u64((u64*)&c.phase[1], 1ull);
         if (prev == (u64)gridDim.x - 1u) {
             atom_add_ll(c.free_count, -granted);
-            __hip_atomic_store(&c.phase[1], 0ll, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&c.phase[1], 0ll, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
@@ -310,22 +502,30 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         if (strat == PBM_LINKED_LIST)
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
         const bool one = (L.flags & PBM_LAUNCH_ONE_WAVE) != 0;
+        const bool rank = (L.flags & PBM_LAUNCH_RANK) != 0;
         const dim3 grid(one ? 1u : batched_blocks(h, L, n));
         const dim3 blk(one ? 64u : 256u);
-        switch (L.cluster_words) {
-            case 64:
-                hipLaunchKernelGGL(k_get_batched<1>, grid, blk, 0, s, c, n, req_base, d_ids,
-                                   d_steps, c.walk_budget);
-                break;
-            case 128:
-                hipLaunchKernelGGL(k_get_batched<2>, grid, blk, 0, s, c, n, req_base, d_ids,
-                                   d_steps, c.walk_budget);
-                break;
-            default:
-                hipLaunchKernelGGL(k_get_batched<4>, grid, blk, 0, s, c, n, req_base, d_ids,
-                                   d_steps, c.walk_budget);
-                break;
+        if (rank) {
+            uint32_t nb = (c.n_sb + 3u) / 4u;
+            if (nb > 4096u) nb = 4096u;
+            hipLaunchKernelGGL(k_rank_local, dim3(nb), dim3(256), 0, s, c);
+            hipLaunchKernelGGL(k_rank_scan, dim3(1), dim3(1024), 0, s, c);
         }
+#define PBM_LAUNCH_BATCHED(JJ)                                                              \
+    do {                                                                                    \
+        if (rank)                                                                           \
+            hipLaunchKernelGGL((k_get_batched<JJ, true>), grid, blk, 0, s, c, n, req_base,  \
+                               d_ids, d_steps, c.walk_budget);                              \
+        else                                                                                \
+            hipLaunchKernelGGL((k_get_batched<JJ, false>), grid, blk, 0, s, c, n, req_base, \
+                               d_ids, d_steps, c.walk_budget);                              \
+    } while (0)
+        switch (L.cluster_words) {
+            case 64: PBM_LAUNCH_BATCHED(1); break;
+            case 128: PBM_LAUNCH_BATCHED(2); break;
+            default: PBM_LAUNCH_BATCHED(4); break;
+        }
+#undef PBM_LAUNCH_BATCHED
         PBM_HIP(hipGetLastError());
         return PBM_OK;
     }

@@ -191,6 +191,10 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     PBM_ALLOC(h->d_counters, sizeof(pbm_counters));
     PBM_ALLOC(h->d_phase, 64u);
     PBM_ALLOC(h->d_scratch, 64u);
+    const uint32_t n_sb = (n_words + 63u) / 64u;
+    PBM_ALLOC(h->d_sb_rank, (size_t)(n_sb + 1u) * 4u);
+    PBM_ALLOC(h->d_w_rank, (size_t)n_words * 4u);
+    PBM_ALLOC(h->d_snap, (size_t)n_words * 8u);
     if (cfg->backing) PBM_ALLOC(h->d_pool, (size_t)cfg->n_pages * cfg->page_size);
     if (cfg->strategy == PBM_CRW_BM) PBM_ALLOC(h->d_hint, (size_t)h->cfg.max_threads * 4u);
     if (cfg->strategy == PBM_DIST_CLOCK) PBM_ALLOC(h->d_arms, (size_t)h->cfg.max_threads * 4u);
@@ -214,6 +218,10 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     c.ring_head = h->d_ring_ht;
     c.ring_tail = h->d_ring_ht ? h->d_ring_ht + 1 : nullptr;
     c.phase = h->d_phase;
+    c.sb_rank = h->d_sb_rank;
+    c.w_rank = h->d_w_rank;
+    c.snap = h->d_snap;
+    c.n_sb = n_sb;
     c.n_pages = cfg->n_pages;
     c.n_words = n_words;
     c.page_size = cfg->page_size;
@@ -239,7 +247,8 @@ extern "C" int pbm_destroy(pbm_handle* h) {
     if (hipGetDevice(&cur) == hipSuccess && cur != h->device) (void)hipSetDevice(h->device);
     (void)hipDeviceSynchronize();
     void* ptrs[] = {h->d_bitmap, h->d_free, h->d_counters, h->d_pool, h->d_hint, h->d_arms,
-                    h->d_clock, h->d_ring, h->d_ring_ht, h->d_phase, h->d_scratch};
+                    h->d_clock, h->d_ring, h->d_ring_ht, h->d_phase, h->d_scratch,
+                    h->d_sb_rank, h->d_w_rank, h->d_snap};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete h;

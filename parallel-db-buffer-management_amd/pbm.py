@@ -28,6 +28,7 @@ ORDER_PACKED, ORDER_REFERENCE = 0, 1
 MODE_PLAIN, MODE_BATCHED, MODE_SERIAL = 0, 1, 2
 LAUNCH_NO_COUNTERS = 1
 LAUNCH_ONE_WAVE = 2
+LAUNCH_RANK = 4
 
 PHASE_ALLOC, PHASE_FREE = 1, 2
 
@@ -54,10 +55,11 @@ class Ctx(C.Structure):
     _fields_ = [("bitmap", C.c_void_p), ("free_count", C.c_void_p), ("counters", C.c_void_p),
                 ("pool", C.c_void_p), ("hint", C.c_void_p), ("arms", C.c_void_p),
                 ("clock_arm", C.c_void_p), ("ring", C.c_void_p), ("ring_head", C.c_void_p),
-                ("ring_tail", C.c_void_p), ("phase", C.c_void_p),
+                ("ring_tail", C.c_void_p), ("phase", C.c_void_p), ("sb_rank", C.c_void_p),
+                ("w_rank", C.c_void_p), ("snap", C.c_void_p),
                 ("n_pages", C.c_uint32), ("n_words", C.c_uint32), ("page_size", C.c_uint32),
                 ("walk_budget", C.c_uint32), ("max_threads", C.c_uint32), ("order", C.c_int32),
-                ("seed", C.c_uint64)]
+                ("seed", C.c_uint64), ("n_sb", C.c_uint32), ("reserved0", C.c_uint32)]
 
 
 class Launch(C.Structure):

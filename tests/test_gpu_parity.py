@@ -119,6 +119,26 @@ def test_one_wave_batched_budget_sweep_matches_oracle(gpu, T, fill, n, budget, o
         assert np.array_equal(pool.snapshot(), st.bm)
 
 
+@pytest.mark.parametrize("T,fill,n", [(BIG, 0.9, 262144), (BIG, 0.5, 262144), (BIG, 0.99, 262144),
+                                      (65536, 0.0, 32768), (1000, 0.4, 700), (1 << 16, 0.999, 100)])
+@pytest.mark.parametrize("one_wave", [False, True])
+def test_batched_rank_matches_oracle_exactly(gpu, T, fill, n, one_wave):
+    # by rank, request r takes the r-th free page whatever the schedule, so
+    # even the full concurrent launch matches the oracle bit for bit
+    log_case(f"batched rank T={T} fill={fill} n={n} one_wave={one_wave}")
+    flags = pbm.LAUNCH_RANK | (pbm.LAUNCH_ONE_WAVE if one_wave else 0)
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=3) as pool:
+        pool.prefill_bernoulli(fill, 42)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, flags=flags)
+        st = orc.State(orc.prefill(T, fill, 42), T)
+        eids, esteps = st.batched_rank(n)
+        assert np.array_equal(ids, eids) and np.array_equal(steps, esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.free_count() == st.free[0]
+        _free(gpu, pool, ids, pbm.MODE_BATCHED)
+        assert np.array_equal(pool.snapshot(), orc.prefill(T, fill, 42))
+
+
 # ------------------------------------------------------------------ concurrent invariants
 
 def _check_cycle(torch, pool, T, before, n, get_mode, free_mode, cw=0, expect_prefix=False,

@@ -373,16 +373,15 @@ def test_batched_phase_admission_and_invariants(cw, order):
     assert np.array_equal(st.bm, before) and st.free[0] == A
 
 
-def test_batched_keeps_window_across_chunks():
-    # an empty pool: chunk 0 takes window w's 64 bits... every 64 requests
-    # fill exactly one word in packed order, so the window is exhausted only
-    # after 64 chunks and ids are consecutive within each word
+def test_batched_chunks_start_at_their_home_words():
+    # an empty pool of 256 words and 192 granted requests: chunk ch starts
+    # at word ch*64*256/192 and, in packed order, fills that word's 64 bits
     T = 64 * 256
     st = orc.State(orc.cleared(T), T)
     ids, steps = st.batched(64 * 3, 0, 5, 0, 64)
-    w0 = ids[0] // 64
-    assert ids[:64].tolist() == [w0 * 64 + b for b in range(64)]
-    assert ids[64] // 64 == (w0 + 1) % (T // 64)
+    for ch in range(3):
+        home = ch * 64 * 256 // 192
+        assert ids[64 * ch:64 * ch + 64].tolist() == [home * 64 + b for b in range(64)]
     assert steps.tolist() == [1] * (64 * 3)
 
 
@@ -410,19 +409,33 @@ def test_check_alloc_flags_planted_violations():
 
 
 def test_batched_sweep_known_answer():
-    # Every page used except one bit in each of four words; budget 1: one
-    # random window, then the sweep from that window's start hands the bits
-    # out in map order starting at the window start.
+    # Every page used except one bit in each of four words outside the home
+    # window (words 0..63); budget 1: the home window, one random window,
+    # then the sweep from that window's start.
     T = 64 * 4096
     L = T // 64
     seed = 77
     bm = orc.prefill(T, 1.0, 0)
-    words = [5, 1500, 2999, 4090]
+    words = [100, 1500, 2999, 4090]
     for w in words:
         bm[w] &= ~np.uint64(1 << 9)
     wave_salt = 0x13198A2E03707344
-    start = (orc.lcg(orc.request_seed(seed ^ wave_salt, 0)) & 0xFFFFFFFF) % L
-    expect = [w * 64 + 9 for w in sorted(words, key=lambda w: (w - start) % L)]
+    # batched windows map an LCG state to a word by its high bits
+    start = ((orc.lcg(orc.request_seed(seed ^ wave_salt, 0)) & 0x7FFFFFFF) * L) >> 31
+    rel = lambda w: (w - start) % L
+    # the random window takes its free words in map order; then lane l
+    # sweeps the S = L/64 words from start + S*l and offers its first
+    # non-full word: round k hands out the k-th free word of every segment
+    expect = [w * 64 + 9 for w in sorted(words, key=rel) if rel(w) < 64]
+    S = L // 64
+    segs = {}
+    for w in sorted(words, key=rel):
+        if rel(w) >= 64:
+            segs.setdefault(rel(w) // S, []).append(w)
+    for k in range(len(words)):
+        for sg in sorted(segs):
+            if len(segs[sg]) > k:
+                expect.append(segs[sg][k] * 64 + 9)
     st = orc.State(bm, T)
     ids, steps = st.batched(4, 0, seed, 0, 64, budget=1)
     assert ids.tolist() == expect
@@ -578,3 +591,21 @@ def test_malloc_packs_small_requests_behind_the_lowest_lane():
     assert off.tolist() == [0, 0, 104, 0, 104 + 204]
     assert page[2] == page[0] and page[4] == page[0]
     assert len({int(page[0]), int(page[1]), int(page[3])}) == 3
+
+
+def test_batched_rank_takes_free_pages_in_map_order():
+    T = 64 * 300
+    before = orc.prefill(T, 0.97, 12)
+    free = [p for p in range(T) if not (int(before[p // 64]) >> (p % 64)) & 1]
+    st = orc.State(before, T)
+    n = len(free) + 50
+    ids, steps = st.batched_rank(n)
+    assert ids.tolist() == free + [-1] * 50
+    # a request's window index counts 64-word windows from its chunk's
+    # first word
+    for r in range(len(free)):
+        w0 = free[(r // 64) * 64] // 64
+        assert steps[r] == (free[r] // 64 - w0) // 64 + 1
+    assert (steps[len(free):] == 0).all()
+    rc, g = orc.check_alloc(before, st.bm, T, ids)
+    assert rc == 0 and g == len(free)
