@@ -8,11 +8,14 @@ Workload (one step = one alloc phase + one free phase, inputs in HBM):
   granted page freed.  About 104,858 pages are free, so about 157,286
   requests fail by admission; allocs+frees counts the grants and the frees.
 
-Both variants run as captured HIP graphs (one graph per phase) on the same
-pool and stream; the batched variant uses the tuned launch (walk budget,
-window width, waves per CU) from the library's autotuner.  The CPU baseline
-is the serial oracle running the reference's RandomWalkBasic_BM policy on
-the identical pre-fill and request stream, on one host core.
+Every phase runs as a captured HIP graph on torch's current stream; HIP
+events between the graphs time each phase.  The batched launch (rank or
+walk homes, window width, walk budget, waves per CU) comes from the
+library's autotuner; the best walk-homes launch is timed too.  The plain
+variant is the reference's RandomWalkBasic_BM getPage/freePage, one thread
+per request.  The CPU baseline is the serial oracle running the reference's
+RandomWalkBasic_BM policy on the identical pre-fill and request stream, on
+one host core.
 
 Prints one JSON line.
 """
@@ -48,40 +51,37 @@ def walk_stats(steps: np.ndarray, ids: np.ndarray) -> dict:
     return out
 
 
-def time_cycle(torch, g_alloc, g_free, steps: int, warmup: int, stream):
+def time_phases(torch, graphs, steps: int, warmup: int, stream):
+    """Runs `steps` iterations of the graph sequence; returns the total ms
+    and the mean ms of every graph."""
     for _ in range(warmup):
-        g_alloc.launch(stream)
-        g_free.launch(stream)
+        for g in graphs:
+            g.launch(stream)
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(graphs) + 1)]
+          for _ in range(steps)]
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
     t0.record(stream)
     for k in range(steps):
-        ev[k][0].record(stream)
-        g_alloc.launch(stream)
-        ev[k][1].record(stream)
-        g_free.launch(stream)
-        ev[k][2].record(stream)
+        for i, g in enumerate(graphs):
+            ev[k][i].record(stream)
+            g.launch(stream)
+        ev[k][-1].record(stream)
     t1.record(stream)
     torch.cuda.synchronize()
-    total = t0.elapsed_time(t1)
-    a = float(np.mean([e[0].elapsed_time(e[1]) for e in ev]))
-    f = float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))
-    return total, a, f
+    per = [float(np.mean([e[i].elapsed_time(e[i + 1]) for e in ev])) for i in range(len(graphs))]
+    return t0.elapsed_time(t1), per
 
 
 def main() -> int:
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawTextHelpFormatter)
+    ap = argparse.ArgumentParser(description=__doc__,
+                                 formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="host time spent on the CPU-baseline sample (0 = skip)")
-    ap.add_argument("--cluster-words", type=int, default=0, help="0 = autotuned")
-    ap.add_argument("--waves-per-cu", type=int, default=0, help="0 = autotuned")
-    ap.add_argument("--walk-budget", type=int, default=0, help="0 = autotuned")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -107,62 +107,99 @@ def main() -> int:
     ids = torch.full((N_REQ,), -1, dtype=torch.int32, device="cuda")
     stp = torch.zeros((N_REQ,), dtype=torch.int32, device="cuda")
 
-    # ---- launch shape of the batched variant
-    tuned = {"cluster_words": args.cluster_words or 64, "waves_per_cu": args.waves_per_cu or 8,
-             "walk_budget": args.walk_budget or 0, "source": "default"}
-    if hasattr(pbm, "autotune") and not (args.cluster_words and args.waves_per_cu):
-        tuned = pbm.autotune(pool, N_REQ, stream=stream)
-        tuned["source"] = "autotune"
-    b_get = pbm.launch(mode=pbm.MODE_BATCHED, cluster_words=tuned["cluster_words"],
-                       waves_per_cu=tuned["waves_per_cu"], walk_budget=tuned["walk_budget"],
-                       flags=pbm.LAUNCH_NO_COUNTERS)
-    b_free = pbm.launch(mode=pbm.MODE_BATCHED, waves_per_cu=tuned["waves_per_cu"],
-                        flags=pbm.LAUNCH_NO_COUNTERS)
-    p_get = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
-    p_free = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
+    # ---- launch shapes from the autotuner
+    tune_rank = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_RANK_ONLY, stream)
+    tune_walk = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_WALK_ONLY, stream)
+    best = tune_rank if (tune_rank["ms_alloc"] + tune_rank["ms_free"]
+                         <= tune_walk["ms_alloc"] + tune_walk["ms_free"]) else tune_walk
+    assert np.array_equal(pool.snapshot(stream), before), "autotune changed the pool"
 
-    # ---- walk statistics and device counters (one untimed cycle each)
-    def counted_cycle(gl, fl):
+    def counted(gl, fl):
+        """One untimed cycle with device counters and step output."""
+        gc = pbm.Launch(gl.mode, gl.block, gl.waves_per_cu, gl.walk_budget, gl.cluster_words,
+                        gl.flags & ~pbm.LAUNCH_NO_COUNTERS & ~pbm.LAUNCH_RANK_READY)
+        fc = pbm.Launch(fl.mode, fl.block, fl.waves_per_cu, fl.walk_budget, fl.cluster_words,
+                        fl.flags & ~pbm.LAUNCH_NO_COUNTERS)
         pool.clear_counters(stream)
-        gl_c = pbm.launch(gl.mode, gl.block, gl.waves_per_cu, gl.walk_budget, gl.cluster_words, 0)
-        fl_c = pbm.launch(fl.mode, fl.block, fl.waves_per_cu, fl.walk_budget, fl.cluster_words, 0)
-        pool.get_pages(N_REQ, ids, stp, 0, gl_c, stream)
+        pool.get_pages(N_REQ, ids, stp, 0, gc, stream)
         torch.cuda.synchronize()
         c_alloc = pool.counters(stream)
         i_np = ids.cpu().numpy()
         s_np = stp.cpu().numpy().view(np.uint32)
-        pool.free_pages(N_REQ, ids, fl_c, stream)
+        pool.free_pages(N_REQ, ids, fc, stream)
         torch.cuda.synchronize()
         c_all = pool.counters(stream)
-        ok = bool(np.array_equal(pool.snapshot(stream), before))
         grants = int((i_np >= 0).sum())
-        uniq = len(np.unique(i_np[i_np >= 0])) == grants
-        return c_alloc, c_all, walk_stats(s_np, i_np), ok and uniq and grants == granted
+        ok = (bool(np.array_equal(pool.snapshot(stream), before)) and grants == granted
+              and len(np.unique(i_np[i_np >= 0])) == grants)
+        return c_alloc, c_all, walk_stats(s_np, i_np), ok
 
-    cb_alloc, cb_all, ws_b, ok_b = counted_cycle(b_get, b_free)
-    cp_alloc, cp_all, ws_p, ok_p = counted_cycle(p_get, p_free)
+    def graphs_for(tuned):
+        gl, fl = tuned["get_launch"], tuned["free_launch"]
+        if gl.flags & pbm.LAUNCH_RANK:
+            ready = pbm.Launch(gl.mode, gl.block, gl.waves_per_cu, gl.walk_budget,
+                               gl.cluster_words, gl.flags | pbm.LAUNCH_RANK_READY)
+            return ["rank_prepass", "alloc", "free"], [
+                pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_RANK),
+                pbm.Graph(pool, N_REQ, ids, None, 0, ready, fl, pbm.PHASE_ALLOC),
+                pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_FREE)]
+        return ["alloc", "free"], [pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_ALLOC),
+                                   pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_FREE)]
 
-    # ---- timed: graphs of each phase
-    gb_a = pbm.Graph(pool, N_REQ, ids, None, 0, b_get, b_free, pbm.PHASE_ALLOC)
-    gb_f = pbm.Graph(pool, N_REQ, ids, None, 0, b_get, b_free, pbm.PHASE_FREE)
-    tot_b, alloc_b, free_b = time_cycle(torch, gb_a, gb_f, args.steps, args.warmup, stream)
-    ok_b = ok_b and bool(np.array_equal(pool.snapshot(stream), before))
-    gp_a = pbm.Graph(pool, N_REQ, ids, None, 0, p_get, p_free, pbm.PHASE_ALLOC)
-    gp_f = pbm.Graph(pool, N_REQ, ids, None, 0, p_get, p_free, pbm.PHASE_FREE)
+    results = {}
+    for name, tuned in (("batched", best), ("batched_walk", tune_walk)):
+        c_alloc, c_all, ws, ok = counted(tuned["get_launch"], tuned["free_launch"])
+        names, graphs = graphs_for(tuned)
+        tot, per = time_phases(torch, graphs, args.steps, args.warmup, stream)
+        ok = ok and bool(np.array_equal(pool.snapshot(stream), before))
+        for g in graphs:
+            g.close()
+        results[name] = {"ms_per_step": tot / args.steps, "phases_ms": dict(zip(names, per)),
+                         "counters_alloc": c_alloc, "counters_cycle": c_all, "walk_steps": ws,
+                         "verified": ok, "tuned": {k: v for k, v in tuned.items()
+                                                   if k not in ("get_launch", "free_launch")}}
+
+    # ---- plain per-thread getPage / freePage (the reference's policy)
+    p_get = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
+    p_free = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
+    cp_alloc, cp_all, ws_p, ok_p = counted(p_get, p_free)
+    gp = [pbm.Graph(pool, N_REQ, ids, None, 0, p_get, p_free, pbm.PHASE_ALLOC),
+          pbm.Graph(pool, N_REQ, ids, None, 0, p_get, p_free, pbm.PHASE_FREE)]
     p_steps = max(3, min(args.steps, 10))
-    tot_p, alloc_p, free_p = time_cycle(torch, gp_a, gp_f, p_steps, 1, stream)
+    tot_p, per_p = time_phases(torch, gp, p_steps, 1, stream)
     ok_p = ok_p and bool(np.array_equal(pool.snapshot(stream), before))
+    for g in gp:
+        g.close()
 
-    ms_b = tot_b / args.steps
+    b = results["batched"]
+    ms_b = b["ms_per_step"]
     ms_p = tot_p / p_steps
     value = ops_per_step / (ms_b * 1e-3)
     value_p = ops_per_step / (ms_p * 1e-3)
 
-    # ---- roofline of the dominant kernel (the batched alloc phase)
-    bytes_alloc = 8 * cb_alloc["probes"] + 16 * cb_alloc["atomics"] + 4 * N_REQ
-    achieved = bytes_alloc / (alloc_b * 1e-3) / 1e9
-    atom_peak = pbm.atomic_peak(pool.n_words, 64)
-    atom_rate = cb_alloc["atomics"] / (alloc_b * 1e-3)
+    # ---- roofline of the dominant kernel of the batched step
+    ca, cc = b["counters_alloc"], b["counters_cycle"]
+    L = T_PAGES // 64
+    n_sb = (L + 63) // 64
+    rank = b["tuned"]["rank"]
+    # algorithmic bytes per launch of each kernel
+    bytes_k = {
+        # rank windows read snapshot (8 B) + word rank (4 B) + superblock rank (4 B) per
+        # word; walk windows read the word (8 B); claims are RMW (16 B); ids written (4 B)
+        "alloc": (16 if rank else 8) * ca["probes"] + 16 * ca["atomics"] + 4 * N_REQ,
+        # ids read (4 B), merged RMW per word (16 B)
+        "free": 4 * N_REQ + 16 * (cc["atomics"] - ca["atomics"]),
+    }
+    if rank:
+        # map read (8 B), snapshot (8 B) and word-rank (4 B) written, per word
+        bytes_k["rank_prepass"] = 20 * L + 8 * n_sb
+    kernel_names = {"alloc": f"k_get_batched<{b['tuned']['cluster_words'] // 64},{str(rank).lower()}>",
+                    "free": "k_free_batched", "rank_prepass": "k_rank_local+k_rank_scan"}
+    dom = max(b["phases_ms"], key=b["phases_ms"].get)
+    dom_ms = b["phases_ms"][dom]
+    achieved = bytes_k[dom] / (dom_ms * 1e-3) / 1e9
+    atom_peak = pbm.atomic_peak(L, 64)
+    atom_rate = (ca["atomics"] / (b["phases_ms"]["alloc"] * 1e-3))
 
     # ---- CPU baseline: reference policy, serial, same pre-fill and requests
     cpu = None
@@ -193,31 +230,31 @@ def main() -> int:
                                "262,144 one-page requests then frees (persistent batched)",
                    "pages": T_PAGES, "page_size": PAGE_SIZE, "fill": FILL,
                    "requests": N_REQ, "free_pages": A, "grants": granted,
-                   "cluster_words": tuned["cluster_words"],
-                   "waves_per_cu": tuned["waves_per_cu"], "walk_budget": tuned["walk_budget"],
-                   "launch_source": tuned["source"], "device": dev_name},
-        "phases_ms": {"batched_alloc": alloc_b, "batched_free": free_b,
-                      "plain_alloc": alloc_p, "plain_free": free_p},
+                   "launch": b["tuned"], "device": dev_name},
+        "phases_ms": b["phases_ms"],
+        "walk_steps": b["walk_steps"],
+        "verified": b["verified"],
         "plain": {"value": value_p, "ms_per_step": ms_p, "steps": p_steps,
-                  "walk_steps": ws_p, "probes": cp_alloc["probes"],
-                  "atomics": cp_all["atomics"], "verified": ok_p},
+                  "phases_ms": dict(zip(["alloc", "free"], per_p)), "walk_steps": ws_p,
+                  "probes": cp_alloc["probes"], "atomics": cp_all["atomics"],
+                  "verified": ok_p},
         "speedup_vs_plain": value / value_p,
-        "walk_steps": ws_b,
-        "verified": ok_b,
-        "counters": {"alloc": cb_alloc, "cycle": cb_all},
+        "batched_walk": {"value": ops_per_step / (results["batched_walk"]["ms_per_step"] * 1e-3),
+                         **{k: results["batched_walk"][k] for k in
+                            ("ms_per_step", "phases_ms", "walk_steps", "verified", "tuned")}},
+        "counters": {"alloc": ca, "cycle": cc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"k_get_batched<{tuned['cluster_words'] // 64}>",
-                     "bytes_per_launch": bytes_alloc,
+                     "kernel": kernel_names[dom], "phase": dom, "bytes_per_launch": bytes_k[dom],
+                     "launch_ms": dom_ms,
                      "atomic": {"achieved_ops": atom_rate, "peak_ops": atom_peak,
-                                "frac": atom_rate / atom_peak, "unit": "u64 RMW/s"}},
+                                "frac": atom_rate / atom_peak, "unit": "u64 RMW/s",
+                                "kernel": kernel_names["alloc"]}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
-    for g in (gb_a, gb_f, gp_a, gp_f):
-        g.close()
     pool.close()
-    return 0 if (ok_b and ok_p) else 1
+    return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]) else 1
 
 
 if __name__ == "__main__":

@@ -142,6 +142,8 @@ typedef struct pbm_launch {
                                       bits and ranks them; request r of the call takes the
                                       free page of rank r (map order), so the chunks of a
                                       call claim disjoint pages with one atomicOr per word */
+#define PBM_LAUNCH_RANK_READY  8u  /* with PBM_LAUNCH_RANK: skip the pre-pass because
+                                      pbm_rank_prepass just ran on the current map       */
 
 typedef struct pbm_handle pbm_handle;
 
@@ -221,6 +223,12 @@ int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
 int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
                    const pbm_launch* launch, void* stream);
 
+/* Runs the rank pre-pass of a PBM_LAUNCH_RANK call on its own (snapshot of
+ * the free bits, per-word and per-superblock free ranks), for a following
+ * pbm_get_pages with PBM_LAUNCH_RANK | PBM_LAUNCH_RANK_READY.  No reference
+ * counterpart. */
+int pbm_rank_prepass(pbm_handle* h, void* stream);
+
 /* ---- multi-page requests and device malloc (RWMalloc) ------------------ */
 
 /* Serves n requests for d_counts[i] consecutive pages (0 <= count <= 64,
@@ -257,6 +265,7 @@ int pbm_page_id(pbm_handle* h, uint64_t addr, int32_t* id);
 
 #define PBM_PHASE_ALLOC 1u
 #define PBM_PHASE_FREE  2u
+#define PBM_PHASE_RANK  4u  /* pbm_rank_prepass, captured before the other phases */
 
 typedef struct pbm_graph pbm_graph;
 
@@ -272,6 +281,30 @@ int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_id
                      const pbm_launch* free_launch, uint32_t phases, pbm_graph** out);
 int pbm_graph_launch(pbm_graph* g, void* stream);
 int pbm_graph_destroy(pbm_graph* g);
+
+/* ---- autotuning ------------------------------------------------------- */
+
+#define PBM_TUNE_QUICK     1u  /* 3 timed cycles per candidate instead of 9            */
+#define PBM_TUNE_WALK_ONLY 2u  /* random-walk homes only (no PBM_LAUNCH_RANK)          */
+#define PBM_TUNE_RANK_ONLY 4u  /* rank homes only                                      */
+
+typedef struct pbm_tune_result {
+    pbm_launch get_launch;   /* fastest batched launch for pbm_get_pages              */
+    pbm_launch free_launch;  /* its partner for pbm_free_pages                        */
+    float      ms_alloc;     /* median alloc phase of that launch, milliseconds       */
+    float      ms_free;      /* median free phase, milliseconds                       */
+    uint32_t   candidates;   /* launch shapes timed                                   */
+    uint32_t   reserved;
+} pbm_tune_result;
+
+/* Times alloc+free cycles of n requests (device buffer d_ids of n ids)
+ * over the batched launch space -- rank or walk homes, window width
+ * (64/128/256 words), walk budget (4/16/64 jumps), waves per CU
+ * (1/2/4/8/16) -- and returns the fastest.  Every cycle returns its pages,
+ * so the pool ends as it started; no other call may overlap.  No reference
+ * counterpart (its sweeps are hard-coded, unitTests/unitTest1.cu:14-18). */
+int pbm_autotune(pbm_handle* h, uint32_t n, int32_t* d_ids, uint32_t flags,
+                 pbm_tune_result* out, void* stream);
 
 /* ---- measurement ------------------------------------------------------ */
 

@@ -505,11 +505,9 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         const bool rank = (L.flags & PBM_LAUNCH_RANK) != 0;
         const dim3 grid(one ? 1u : batched_blocks(h, L, n));
         const dim3 blk(one ? 64u : 256u);
-        if (rank) {
-            uint32_t nb = (c.n_sb + 3u) / 4u;
-            if (nb > 4096u) nb = 4096u;
-            hipLaunchKernelGGL(k_rank_local, dim3(nb), dim3(256), 0, s, c);
-            hipLaunchKernelGGL(k_rank_scan, dim3(1), dim3(1024), 0, s, c);
+        if (rank && !(L.flags & PBM_LAUNCH_RANK_READY)) {
+            rc = pbm_rank_prepass(h, s);
+            if (rc) return rc;
         }
 #define PBM_LAUNCH_BATCHED(JJ)                                                              \
     do {                                                                                    \
@@ -548,6 +546,20 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         default:
             return pbmi::launch_get_strategy(h, c, L, n, req_base, d_ids, d_steps, s);
     }
+    PBM_HIP(hipGetLastError());
+    return PBM_OK;
+}
+
+extern "C" int pbm_rank_prepass(pbm_handle* h, void* stream) {
+    PBM_CHECK_HANDLE(h);
+    if (h->cfg.strategy == PBM_LINKED_LIST)
+        return pbmi::fail(PBM_ERR_UNSUPPORTED, "the rank pre-pass needs a bitmap strategy");
+    hipStream_t s = pbmi::as_stream(stream);
+    const pbm_ctx& c = h->ctx;
+    uint32_t nb = (c.n_sb + 3u) / 4u;
+    if (nb > 4096u) nb = 4096u;
+    hipLaunchKernelGGL(k_rank_local, dim3(nb), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(k_rank_scan, dim3(1), dim3(1024), 0, s, c);
     PBM_HIP(hipGetLastError());
     return PBM_OK;
 }

@@ -29,8 +29,9 @@ MODE_PLAIN, MODE_BATCHED, MODE_SERIAL = 0, 1, 2
 LAUNCH_NO_COUNTERS = 1
 LAUNCH_ONE_WAVE = 2
 LAUNCH_RANK = 4
+LAUNCH_RANK_READY = 8
 
-PHASE_ALLOC, PHASE_FREE = 1, 2
+PHASE_ALLOC, PHASE_FREE, PHASE_RANK = 1, 2, 4
 
 ERRORS = {0: "PBM_OK", -1: "PBM_ERR_INVALID", -2: "PBM_ERR_HIP", -3: "PBM_ERR_OOM",
           -4: "PBM_ERR_UNSUPPORTED", -5: "PBM_ERR_RANGE"}
@@ -67,6 +68,14 @@ class Launch(C.Structure):
                 ("walk_budget", C.c_uint32), ("cluster_words", C.c_uint32), ("flags", C.c_uint32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("get_launch", Launch), ("free_launch", Launch), ("ms_alloc", C.c_float),
+                ("ms_free", C.c_float), ("candidates", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+TUNE_QUICK, TUNE_WALK_ONLY, TUNE_RANK_ONLY = 1, 2, 4
+
+
 class PbmError(RuntimeError):
     def __init__(self, code: int, where: str, msg: str):
         super().__init__(f"{where} -> {ERRORS.get(code, code)}: {msg}")
@@ -94,6 +103,7 @@ _SIGS = {
                                 C.POINTER(Launch), C.c_void_p]),
     "pbm_free_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
                                  C.c_void_p]),
+    "pbm_rank_prepass": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pbm_get_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.POINTER(Launch), C.c_void_p]),
     "pbm_free_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
@@ -109,6 +119,8 @@ _SIGS = {
                                    C.POINTER(C.c_void_p)]),
     "pbm_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pbm_graph_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                               C.POINTER(TuneResult), C.c_void_p]),
     "pbm_atomic_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
     "pbm_last_error": (C.c_char_p, []),
     "pbm_version": (C.c_char_p, []),
@@ -256,6 +268,9 @@ class Pool:
         check(lib().pbm_free_pages(self.h, n, _ptr(ids), C.byref(lnch or launch()),
                                    _stream(stream)), "pbm_free_pages")
 
+    def rank_prepass(self, stream=None) -> None:
+        check(lib().pbm_rank_prepass(self.h, _stream(stream)), "pbm_rank_prepass")
+
     def get_n_pages(self, n: int, counts, ids, steps=None, req_base: int = 0,
                     lnch: Launch | None = None, stream=None) -> None:
         check(lib().pbm_get_n_pages(self.h, n, req_base, _ptr(counts), _ptr(ids), _ptr(steps),
@@ -292,7 +307,7 @@ class Graph:
 
     def __init__(self, pool: Pool, n: int, ids, steps=None, req_base: int = 0,
                  get_launch: Launch | None = None, free_launch: Launch | None = None,
-                 phases: int = PHASE_ALLOC | PHASE_FREE):
+                 phases: int = 3):
         g = C.c_void_p()
         check(lib().pbm_graph_create(pool.h, n, req_base, _ptr(ids), _ptr(steps),
                                      C.byref(get_launch or launch()),
@@ -321,3 +336,16 @@ def atomic_peak(words: int = 16384, rounds: int = 64, device: int = 0) -> float:
     v = C.c_double()
     check(lib().pbm_atomic_peak(words, rounds, device, C.byref(v)), "pbm_atomic_peak")
     return float(v.value)
+
+
+def autotune(pool: Pool, n: int, ids, flags: int = 0, stream=None) -> dict:
+    """Fastest batched launch for n requests (ids: device buffer of n int32)."""
+    r = TuneResult()
+    check(lib().pbm_autotune(pool.h, n, _ptr(ids), flags, C.byref(r), _stream(stream)),
+          "pbm_autotune")
+    g = r.get_launch
+    return {"get_launch": g, "free_launch": r.free_launch,
+            "rank": bool(g.flags & LAUNCH_RANK), "cluster_words": int(g.cluster_words),
+            "waves_per_cu": int(g.waves_per_cu), "walk_budget": int(g.walk_budget),
+            "ms_alloc": float(r.ms_alloc), "ms_free": float(r.ms_free),
+            "candidates": int(r.candidates)}

@@ -281,3 +281,21 @@ def test_snapshot_restore_round_trip(gpu):
         assert pool.free_count() == orc.count_free(bm, T)
         pool.reset()
         assert np.array_equal(pool.snapshot(), orc.cleared(T))
+
+
+def test_autotune_returns_a_working_launch_and_leaves_the_pool(gpu):
+    T, n = 1 << 18, 65536
+    log_case("autotune quick")
+    with pbm.Pool(T, seed=6) as pool:
+        pool.prefill_bernoulli(0.9, 42)
+        before = pool.snapshot()
+        ids = gpu.full((n,), -1, dtype=gpu.int32, device="cuda")
+        r = pbm.autotune(pool, n, ids, pbm.TUNE_QUICK)
+        assert r["candidates"] == 50 and r["ms_alloc"] > 0
+        assert np.array_equal(pool.snapshot(), before)
+        assert pool.free_count() == orc.count_free(before, T)
+        pool.get_pages(n, ids, None, 0, r["get_launch"])
+        gpu.cuda.synchronize()
+        got = ids.cpu().numpy()
+        rc, g = orc.check_alloc(before, pool.snapshot(), T, got)
+        assert rc == 0 and g == min(n, orc.count_free(before, T))
