@@ -8,8 +8,11 @@ Workload (one step = one alloc phase + one free phase, inputs in HBM):
   granted page freed.  About 104,858 pages are free, so about 157,286
   requests fail by admission; allocs+frees counts the grants and the frees.
 
-Every phase runs as a captured HIP graph on torch's current stream; HIP
-events between the graphs time each phase.  The batched launch (rank or
+A step is one launch of a captured HIP graph holding both phases, on
+torch's current stream; the step time is the mean over back-to-back
+launches between two HIP events.  Each phase kernel is also timed on its
+own from events around eager launches on the same stream (the roofline
+uses those durations).  The batched launch (rank or
 walk homes, window width, walk budget, waves per CU) comes from the
 library's autotuner; the best walk-homes launch is timed too.  The plain
 variant is the reference's RandomWalkBasic_BM getPage/freePage, one thread
@@ -51,27 +54,35 @@ def walk_stats(steps: np.ndarray, ids: np.ndarray) -> dict:
     return out
 
 
-def time_phases(torch, graphs, steps: int, warmup: int, stream):
-    """Runs `steps` iterations of the graph sequence; returns the total ms
-    and the mean ms of every graph."""
+def time_steps(torch, graph, steps: int, warmup: int, stream) -> float:
+    """Mean ms of one launch of `graph` (both phases), `steps` launches
+    back to back between two events."""
     for _ in range(warmup):
-        for g in graphs:
-            g.launch(stream)
+        graph.launch(stream)
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(len(graphs) + 1)]
-          for _ in range(steps)]
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(stream)
-    for k in range(steps):
-        for i, g in enumerate(graphs):
-            ev[k][i].record(stream)
-            g.launch(stream)
-        ev[k][-1].record(stream)
+    for _ in range(steps):
+        graph.launch(stream)
     t1.record(stream)
     torch.cuda.synchronize()
-    per = [float(np.mean([e[i].elapsed_time(e[i + 1]) for e in ev])) for i in range(len(graphs))]
-    return t0.elapsed_time(t1), per
+    return t0.elapsed_time(t1) / steps
+
+
+def time_phases(torch, pool, n, ids, gl, fl, steps: int, stream) -> dict:
+    """Mean ms of each phase kernel, from events recorded on the launch
+    stream around every eager alloc and free launch."""
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for e in ev:
+        e[0].record(stream)
+        pool.get_pages(n, ids, None, 0, gl, stream)
+        e[1].record(stream)
+        pool.free_pages(n, ids, fl, stream)
+        e[2].record(stream)
+    torch.cuda.synchronize()
+    return {"alloc": float(np.mean([e[0].elapsed_time(e[1]) for e in ev])),
+            "free": float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))}
 
 
 def main() -> int:
@@ -82,6 +93,12 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="host time spent on the CPU-baseline sample (0 = skip)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the autotuner; use fixed rank and walk launches "
+                         "(for counter-collection runs)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the line, copy a device buffer of this many MiB once "
+                         "(the known-size reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -108,8 +125,19 @@ def main() -> int:
     stp = torch.zeros((N_REQ,), dtype=torch.int32, device="cuda")
 
     # ---- launch shapes from the autotuner
-    tune_rank = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_RANK_ONLY, stream)
-    tune_walk = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_WALK_ONLY, stream)
+    if args.no_tune:
+        def fixed(rank, wpc, cw, budget):
+            flags = pbm.LAUNCH_NO_COUNTERS | (pbm.LAUNCH_RANK if rank else 0)
+            gl = pbm.launch(pbm.MODE_BATCHED, 256, wpc, budget, cw, flags)
+            fl = pbm.launch(pbm.MODE_BATCHED, 256, wpc, 0, 64, pbm.LAUNCH_NO_COUNTERS)
+            return {"get_launch": gl, "free_launch": fl, "rank": rank, "cluster_words": cw,
+                    "waves_per_cu": wpc, "walk_budget": budget, "ms_alloc": 0.0,
+                    "ms_free": 0.0, "candidates": 0}
+        tune_rank = fixed(True, 16, 64, 16)
+        tune_walk = fixed(False, 8, 256, 16)
+    else:
+        tune_rank = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_RANK_ONLY, stream)
+        tune_walk = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_WALK_ONLY, stream)
     best = tune_rank if (tune_rank["ms_alloc"] + tune_rank["ms_free"]
                          <= tune_walk["ms_alloc"] + tune_walk["ms_free"]) else tune_walk
     assert np.array_equal(pool.snapshot(stream), before), "autotune changed the pool"
@@ -117,7 +145,7 @@ def main() -> int:
     def counted(gl, fl):
         """One untimed cycle with device counters and step output."""
         gc = pbm.Launch(gl.mode, gl.block, gl.waves_per_cu, gl.walk_budget, gl.cluster_words,
-                        gl.flags & ~pbm.LAUNCH_NO_COUNTERS & ~pbm.LAUNCH_RANK_READY)
+                        gl.flags & ~pbm.LAUNCH_NO_COUNTERS)
         fc = pbm.Launch(fl.mode, fl.block, fl.waves_per_cu, fl.walk_budget, fl.cluster_words,
                         fl.flags & ~pbm.LAUNCH_NO_COUNTERS)
         pool.clear_counters(stream)
@@ -134,67 +162,51 @@ def main() -> int:
               and len(np.unique(i_np[i_np >= 0])) == grants)
         return c_alloc, c_all, walk_stats(s_np, i_np), ok
 
-    def graphs_for(tuned):
-        gl, fl = tuned["get_launch"], tuned["free_launch"]
-        if gl.flags & pbm.LAUNCH_RANK:
-            ready = pbm.Launch(gl.mode, gl.block, gl.waves_per_cu, gl.walk_budget,
-                               gl.cluster_words, gl.flags | pbm.LAUNCH_RANK_READY)
-            return ["rank_prepass", "alloc", "free"], [
-                pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_RANK),
-                pbm.Graph(pool, N_REQ, ids, None, 0, ready, fl, pbm.PHASE_ALLOC),
-                pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_FREE)]
-        return ["alloc", "free"], [pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_ALLOC),
-                                   pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_FREE)]
+    def measure(gl, fl, steps, warmup):
+        """Counted cycle, whole-step graph timing and per-phase timing."""
+        c_alloc, c_all, ws, ok = counted(gl, fl)
+        g = pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_ALLOC | pbm.PHASE_FREE)
+        ms = time_steps(torch, g, steps, warmup, stream)
+        g.close()
+        phases = time_phases(torch, pool, N_REQ, ids, gl, fl, steps, stream)
+        ok = ok and bool(np.array_equal(pool.snapshot(stream), before))
+        return {"ms_per_step": ms, "phases_ms": phases, "counters_alloc": c_alloc,
+                "counters_cycle": c_all, "walk_steps": ws, "verified": ok}
 
     results = {}
     for name, tuned in (("batched", best), ("batched_walk", tune_walk)):
-        c_alloc, c_all, ws, ok = counted(tuned["get_launch"], tuned["free_launch"])
-        names, graphs = graphs_for(tuned)
-        tot, per = time_phases(torch, graphs, args.steps, args.warmup, stream)
-        ok = ok and bool(np.array_equal(pool.snapshot(stream), before))
-        for g in graphs:
-            g.close()
-        results[name] = {"ms_per_step": tot / args.steps, "phases_ms": dict(zip(names, per)),
-                         "counters_alloc": c_alloc, "counters_cycle": c_all, "walk_steps": ws,
-                         "verified": ok, "tuned": {k: v for k, v in tuned.items()
-                                                   if k not in ("get_launch", "free_launch")}}
+        results[name] = measure(tuned["get_launch"], tuned["free_launch"], args.steps, args.warmup)
+        results[name]["tuned"] = {k: v for k, v in tuned.items()
+                                  if k not in ("get_launch", "free_launch")}
 
     # ---- plain per-thread getPage / freePage (the reference's policy)
     p_get = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
     p_free = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
-    cp_alloc, cp_all, ws_p, ok_p = counted(p_get, p_free)
-    gp = [pbm.Graph(pool, N_REQ, ids, None, 0, p_get, p_free, pbm.PHASE_ALLOC),
-          pbm.Graph(pool, N_REQ, ids, None, 0, p_get, p_free, pbm.PHASE_FREE)]
     p_steps = max(3, min(args.steps, 10))
-    tot_p, per_p = time_phases(torch, gp, p_steps, 1, stream)
-    ok_p = ok_p and bool(np.array_equal(pool.snapshot(stream), before))
-    for g in gp:
-        g.close()
+    rp = measure(p_get, p_free, p_steps, 1)
+    cp_alloc, cp_all, ws_p, ok_p = (rp["counters_alloc"], rp["counters_cycle"], rp["walk_steps"],
+                                    rp["verified"])
 
     b = results["batched"]
     ms_b = b["ms_per_step"]
-    ms_p = tot_p / p_steps
+    ms_p = rp["ms_per_step"]
     value = ops_per_step / (ms_b * 1e-3)
     value_p = ops_per_step / (ms_p * 1e-3)
 
     # ---- roofline of the dominant kernel of the batched step
     ca, cc = b["counters_alloc"], b["counters_cycle"]
     L = T_PAGES // 64
-    n_sb = (L + 63) // 64
     rank = b["tuned"]["rank"]
-    # algorithmic bytes per launch of each kernel
+    # algorithmic bytes per launch of each kernel: every map word read (8 B),
+    # every claim or release a read-modify-write (16 B), every id written by
+    # the alloc phase and read by the free phase (4 B)
     bytes_k = {
-        # rank windows read snapshot (8 B) + word rank (4 B) + superblock rank (4 B) per
-        # word; walk windows read the word (8 B); claims are RMW (16 B); ids written (4 B)
-        "alloc": (16 if rank else 8) * ca["probes"] + 16 * ca["atomics"] + 4 * N_REQ,
-        # ids read (4 B), merged RMW per word (16 B)
+        "alloc": 8 * ca["probes"] + 16 * ca["atomics"] + 4 * N_REQ,
         "free": 4 * N_REQ + 16 * (cc["atomics"] - ca["atomics"]),
     }
-    if rank:
-        # map read (8 B), snapshot (8 B) and word-rank (4 B) written, per word
-        bytes_k["rank_prepass"] = 20 * L + 8 * n_sb
-    kernel_names = {"alloc": f"k_get_batched<{b['tuned']['cluster_words'] // 64},{str(rank).lower()}>",
-                    "free": "k_free_batched", "rank_prepass": "k_rank_local+k_rank_scan"}
+    kernel_names = {"alloc": "k_get_rank" if rank else
+                    f"k_get_batched<{b['tuned']['cluster_words'] // 64}>",
+                    "free": "k_free_batched"}
     dom = max(b["phases_ms"], key=b["phases_ms"].get)
     dom_ms = b["phases_ms"][dom]
     achieved = bytes_k[dom] / (dom_ms * 1e-3) / 1e9
@@ -235,7 +247,7 @@ def main() -> int:
         "walk_steps": b["walk_steps"],
         "verified": b["verified"],
         "plain": {"value": value_p, "ms_per_step": ms_p, "steps": p_steps,
-                  "phases_ms": dict(zip(["alloc", "free"], per_p)), "walk_steps": ws_p,
+                  "phases_ms": rp["phases_ms"], "walk_steps": ws_p,
                   "probes": cp_alloc["probes"], "atomics": cp_all["atomics"],
                   "verified": ok_p},
         "speedup_vs_plain": value / value_p,
@@ -253,6 +265,12 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_mib > 0:
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
     pool.close()
     return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]) else 1
 

@@ -108,12 +108,10 @@ typedef struct pbm_ctx {
     unsigned long long* ring_head;   /* LinkedList: next ticket to dequeue              */
     unsigned long long* ring_tail;   /* LinkedList: next ticket to enqueue              */
     long long*          phase;       /* batched calls: [1] wave completion counter      */
-    uint32_t*           sb_rank;     /* n_sb+1 entries: free pages before superblock s
-                                        (a superblock is 64 words)                      */
-    uint32_t*           w_rank;      /* n_words entries: free pages before word w inside
-                                        its superblock                                  */
-    unsigned long long* snap;        /* n_words entries: free bits of each word when
-                                        the rank pre-pass ran (1 = free)                */
+    unsigned long long* scan;        /* n_tiles look-back words of the single-pass rank
+                                        scan (call parity, state, grant, prefix)        */
+    unsigned long long* scan_ctl;    /* rank scan: [0] tile tickets (n_tiles per call),
+                                        [1] look-back give-ups (stays 0)                */
     uint32_t n_pages;
     uint32_t n_words;
     uint32_t page_size;
@@ -121,7 +119,7 @@ typedef struct pbm_ctx {
     uint32_t max_threads;
     int32_t  order;
     uint64_t seed;
-    uint32_t n_sb;                   /* superblocks: ceil(n_words / 64)                 */
+    uint32_t n_tiles;                /* rank-scan tiles: ceil(n_words / 256)            */
     uint32_t reserved0;
 } pbm_ctx;
 
@@ -138,12 +136,11 @@ typedef struct pbm_launch {
 #define PBM_LAUNCH_NO_COUNTERS 1u  /* skip the device counter updates                  */
 #define PBM_LAUNCH_ONE_WAVE    2u  /* batched mode on a single wave: the schedule of the
                                       serial oracle (orc_batched_serial)                  */
-#define PBM_LAUNCH_RANK        4u  /* batched mode by rank: a pre-pass snapshots the free
-                                      bits and ranks them; request r of the call takes the
-                                      free page of rank r (map order), so the chunks of a
-                                      call claim disjoint pages with one atomicOr per word */
-#define PBM_LAUNCH_RANK_READY  8u  /* with PBM_LAUNCH_RANK: skip the pre-pass because
-                                      pbm_rank_prepass just ran on the current map       */
+#define PBM_LAUNCH_RANK        4u  /* batched mode by rank, one pass over the map: request r
+                                      of the call takes the free page of rank r (map order);
+                                      each 256-word tile learns the free pages before it by
+                                      decoupled look-back and claims its granted bits with
+                                      one atomicOr per word.  n < 2^30.                    */
 
 typedef struct pbm_handle pbm_handle;
 
@@ -223,12 +220,6 @@ int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
 int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
                    const pbm_launch* launch, void* stream);
 
-/* Runs the rank pre-pass of a PBM_LAUNCH_RANK call on its own (snapshot of
- * the free bits, per-word and per-superblock free ranks), for a following
- * pbm_get_pages with PBM_LAUNCH_RANK | PBM_LAUNCH_RANK_READY.  No reference
- * counterpart. */
-int pbm_rank_prepass(pbm_handle* h, void* stream);
-
 /* ---- multi-page requests and device malloc (RWMalloc) ------------------ */
 
 /* Serves n requests for d_counts[i] consecutive pages (0 <= count <= 64,
@@ -265,7 +256,6 @@ int pbm_page_id(pbm_handle* h, uint64_t addr, int32_t* id);
 
 #define PBM_PHASE_ALLOC 1u
 #define PBM_PHASE_FREE  2u
-#define PBM_PHASE_RANK  4u  /* pbm_rank_prepass, captured before the other phases */
 
 typedef struct pbm_graph pbm_graph;
 
@@ -286,7 +276,7 @@ int pbm_graph_destroy(pbm_graph* g);
 
 #define PBM_TUNE_QUICK     1u  /* 3 timed cycles per candidate instead of 9            */
 #define PBM_TUNE_WALK_ONLY 2u  /* random-walk homes only (no PBM_LAUNCH_RANK)          */
-#define PBM_TUNE_RANK_ONLY 4u  /* rank homes only                                      */
+#define PBM_TUNE_RANK_ONLY 4u  /* rank mode only                                       */
 
 typedef struct pbm_tune_result {
     pbm_launch get_launch;   /* fastest batched launch for pbm_get_pages              */

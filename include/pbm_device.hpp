@@ -39,6 +39,9 @@ constexpr u64 kGolden = 0x9E3779B97F4A7C15ull;
 constexpr u64 kPrefillSalt = 0x243F6A8885A308D3ull;
 constexpr u64 kWaveSalt = 0x13198A2E03707344ull;
 constexpr u64 kArmSalt = 0xA4093822299F31D0ull;
+/* Words per tile of the single-pass rank scan (one per thread of a
+ * 256-thread workgroup). */
+constexpr uint32_t kScanWords = 256;
 
 /* ---------------------------------------------------------------------
  * Random numbers

@@ -436,12 +436,11 @@ uint32_t orc_batched_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
     return got;
 }
 
-/* Batched allocation by rank (PBM_LAUNCH_RANK, pbm_alloc.hip
- * k_rank_local/k_rank_scan/rank_window): the call admits min(n, free)
- * requests and request r takes the free page of rank r, i.e. the r-th free
- * page in map order.  Chunk c's first window starts at the word holding
- * rank 64c; a request served in the k-th 64-word window from there has
- * steps k+1.  The outcome does not depend on the schedule. */
+/* Batched allocation by rank (PBM_LAUNCH_RANK, pbm_alloc.hip k_get_rank):
+ * the call admits min(n, free count) requests and request r takes the free
+ * page of rank r, i.e. the r-th free page in map order; each grant costs
+ * one probe of its word (steps 1), a refusal none.  The outcome does not
+ * depend on the schedule. */
 uint32_t orc_batched_rank(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                           int32_t* ids, uint32_t* steps) {
     const uint32_t L = orc_n_words(n_pages);
@@ -464,10 +463,8 @@ uint32_t orc_batched_rank(uint64_t* bm, uint32_t n_pages, int64_t* free_count, u
         if (steps) steps[r] = 0;
     }
     for (int64_t r = 0; r < k; ++r) {
-        uint32_t w0 = (uint32_t)order[(r / 64) * 64] >> 6;
-        uint32_t wd = (uint32_t)order[r] >> 6;
         ids[r] = order[r];
-        if (steps) steps[r] = (wd - w0) / 64u + 1u;
+        if (steps) steps[r] = 1u;
     }
     for (int64_t r = 0; r < k; ++r)
         bm[(uint32_t)order[r] >> 6] |= (u64)1 << ((uint32_t)order[r] & 63u);

@@ -59,143 +59,234 @@ __global__ __launch_bounds__(1024) void k_get_collab(pbm_ctx c, uint32_t n, uint
     pbm::flush_stats(c, st);
 }
 
-// ---- rank pre-pass: free-rank of every page at the start of a batched call
+// ---- batched allocation by rank, one pass over the map
+//
+// The map is cut into tiles of kScanWords words, one workgroup per tile,
+// tiles numbered in the order the workgroups start (an atomic ticket), so
+// every tile's predecessors are already running.  A tile counts its free
+// pages, publishes the count, and finds the free pages before it by
+// decoupled look-back: wave 0 reads the look-back words of the 64
+// preceding tiles at once and sums back to the nearest one that published
+// its inclusive prefix.  Tile 0 fixes the call's grant G = min(n, free
+// count) and passes it down the chain inside the same words.  Then the
+// free page of rank q < G belongs to request q: each word claims its
+// granted bits with one atomicOr and each wave writes ids[q] for its ranks
+// in index order.  A granted bit that a concurrent writer took between the
+// read and the claim sends its request to a short random walk.
+//
+// Tickets count up across calls and a call takes exactly n_tiles of them,
+// so ticket t is tile t mod n_tiles of call t div n_tiles; the parity of
+// the call number tells this call's look-back words from the last call's.
 
-// Step 1: one wave per 64-word superblock snapshots the free bits of its
-// words, ranks the words inside the superblock, and stores the
-// superblock's free total at sb_rank[s+1].
-__global__ __launch_bounds__(256) void k_rank_local(pbm_ctx c) {
-    using namespace pbm;
-    const unsigned lane = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t sb = wave; sb < c.n_sb; sb += n_waves) {
-        const uint32_t w = sb * 64u + lane;
-        u64 fm = 0;
-        if (w < c.n_words) fm = ~ld_word(&c.bitmap[w]);
-        unsigned tot = 0;
-        const unsigned pre = scan7((unsigned)__popcll(fm), tot);
-        if (w < c.n_words) {
-            c.snap[w] = fm;
-            c.w_rank[w] = pre;
-        }
-        if (lane == 0) c.sb_rank[sb + 1u] = tot;
-    }
+// Look-back word: [63] call parity, [62] inclusive prefix (else the tile's
+// own count), [61:31] grant G, [30:0] free pages (saturating).
+constexpr u64 kScanIncl = 1ull << 62;
+constexpr uint32_t kScanSat = 0x7FFFFFFFu;
+constexpr uint32_t kScanSpin = 1u << 20;  // look-back polls before giving up
+
+__device__ __forceinline__ u64 scan_word(uint32_t parity, bool incl, uint32_t grant, u64 v) {
+    const uint32_t sv = v > kScanSat ? kScanSat : (uint32_t)v;
+    return ((u64)parity << 63) | (incl ? kScanIncl : 0ull) | ((u64)grant << 31) | sv;
 }
 
-// Step 2: one workgroup turns the totals into prefixes in place:
-// sb_rank[s] = free pages before superblock s, sb_rank[n_sb] = all free.
-__global__ __launch_bounds__(1024) void k_rank_scan(pbm_ctx c) {
-    __shared__ uint32_t part[1024];
-    const uint32_t n = c.n_sb;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b = threadIdx.x * per;
-    uint32_t sum = 0;
-    for (uint32_t i = 0; i < per && b + i < n; ++i) sum += c.sb_rank[b + i + 1u];
-    part[threadIdx.x] = sum;
+__device__ __forceinline__ void st_scan(u64* p, u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A reserved request whose rank page is gone: up to 64 random words, then
+// sweeps.  Gives the reservation back when nothing is found.
+__device__ inline int32_t rank_fallback(const pbm_ctx& c, uint64_t req, uint32_t budget,
+                                        uint32_t& steps, pbm::Stats& st) {
+    int32_t seed = pbm::request_seed(c.seed, req);
+    int32_t id = pbm::walk_rwbm(c, seed, budget < 64u ? budget : 64u, steps, st);
+    if (id < 0) {
+        pbm::atom_add_ll(c.free_count, 1);
+        st.errors++;
+    }
+    return id;
+}
+
+__global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_t req_base,
+                                                  int32_t* __restrict__ ids,
+                                                  uint32_t* __restrict__ steps_out,
+                                                  uint32_t budget) {
+    using namespace pbm;
+    __shared__ uint32_t s_tile, s_parity, s_excl, s_grant;
+    __shared__ long long s_free;
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_pre[4][64];
+    __shared__ u64 s_fm[4][64];
+    const unsigned lane = lane_id();
+    const unsigned wv = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        const u64 t = atom_add_u64(&c.scan_ctl[0], 1ull);
+        s_free = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_tile = (uint32_t)(t % c.n_tiles);
+        s_parity = (uint32_t)((t / c.n_tiles) & 1u);
+    }
     __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {
-        const uint32_t v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
-        __syncthreads();
-        part[threadIdx.x] += v;
-        __syncthreads();
-    }
-    uint32_t run = threadIdx.x ? part[threadIdx.x - 1u] : 0u;
-    for (uint32_t i = 0; i < per && b + i < n; ++i) {
-        run += c.sb_rank[b + i + 1u];
-        c.sb_rank[b + i + 1u] = run;
-    }
-    if (threadIdx.x == 0) c.sb_rank[0] = 0;
-}
-
-// First word holding free rank R (R < sb_rank[n_sb]): a 64-way search over
-// the superblock prefixes, then a ballot over the superblock's 64 words.
-__device__ inline uint32_t rank_search(const pbm_ctx& c, uint32_t R) {
-    using namespace pbm;
-    const unsigned lane = lane_id();
-    uint32_t lo = 0, hi = c.n_sb;
-    while (hi - lo > 1u) {
-        const uint32_t stride = (hi - lo + 63u) / 64u;
-        const uint32_t idx = lo + lane * stride;
-        const uint32_t v = idx < hi ? c.sb_rank[idx] : 0xFFFFFFFFu;
-        const u64 m = ballot(idx < hi && v <= R);
-        const unsigned t = 63u - (unsigned)__clzll((long long)m);
-        lo += t * stride;
-        hi = lo + stride < hi ? lo + stride : hi;
-    }
-    const uint32_t w = lo * 64u + lane;
-    bool hit = false;
-    if (w < c.n_words) {
-        const uint32_t rk = c.sb_rank[lo] + c.w_rank[w];
-        const uint32_t pc = (uint32_t)__popcll(c.snap[w]);
-        hit = rk <= R && R < rk + pc;
-    }
-    const u64 m = ballot(hit);
-    return lo * 64u + (uint32_t)__ffsll((long long)m) - 1u;
-}
-
-// One window of the rank path: lane l holds word w0+l, whose snapshot free
-// bits have ranks rk .. rk+pc-1; the bits with ranks in [R, R_end) belong
-// to this chunk and are claimed with one atomicOr per word.  Lane q of the
-// chunk (request of rank R+q) takes the bit of rank R+q.  *more tells
-// whether the chunk's range goes on past this window.
-__device__ inline void rank_window(const pbm_ctx& c, uint32_t w0, uint32_t R, uint32_t R_end,
-                                   bool& need, int32_t& id, pbm::Stats& st, bool& more) {
-    using namespace pbm;
-    const unsigned lane = lane_id();
-    const uint32_t wd = w0 + lane;
+    const uint32_t tile = s_tile;
+    const uint32_t parity = s_parity;
+    const uint32_t w = tile * kScanWords + threadIdx.x;
+    Stats st;
     u64 fm = 0;
-    uint32_t rk = 0xFFFFFFFFu, pc = 0;
-    if (wd < c.n_words) {
-        fm = c.snap[wd];
-        rk = c.sb_rank[wd >> 6] + c.w_rank[wd];
-        pc = (uint32_t)__popcll(fm);
+    if (w < c.n_words) {
+        fm = ~ld_word(&c.bitmap[w]);  // padding bits past n_pages read as used
         st.probes++;
     }
-    uint32_t lo = 0, hi = 0;
-    if (wd < c.n_words) {
-        lo = rk >= R ? 0u : (R - rk < pc ? R - rk : pc);
-        hi = rk >= R_end ? 0u : (R_end - rk < pc ? R_end - rk : pc);
-        if (hi < lo) hi = lo;
+    const uint32_t pc = (uint32_t)__popcll(fm);
+    unsigned wtot = 0;
+    const unsigned pre = scan7(pc, wtot);
+    if (lane == 0) s_wsum[wv] = wtot;
+    __syncthreads();
+    uint32_t agg = 0, woff = 0;
+#pragma unroll
+    for (unsigned i = 0; i < 4; ++i) {
+        if (i < wv) woff += s_wsum[i];
+        agg += s_wsum[i];
     }
-    const u64 mask = lowest_bits(fm, hi) & ~lowest_bits(fm, lo);
-    u64 owned = 0;
+
+    if (wv == 0) {
+        uint32_t grant = 0;
+        u64 excl = 0;
+        if (tile == 0) {
+            const long long f = s_free;
+            grant = f <= 0 ? 0u : (f > (long long)n ? n : (uint32_t)f);
+            if (lane == 0) {
+                st_scan(&c.scan[0], scan_word(parity, true, grant, agg));
+                if (grant) atom_add_ll(c.free_count, -(long long)grant);
+            }
+        } else {
+            if (lane == 0) st_scan(&c.scan[tile], scan_word(parity, false, 0, agg));
+            int64_t k = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            for (;;) {
+                const int64_t idx = k - (int64_t)lane;
+                u64 v = 0;
+                bool ready = true, incl = false;
+                if (idx >= 0) {
+                    v = ld_word(&c.scan[idx]);
+                    ready = (uint32_t)(v >> 63) == parity;
+                    incl = ready && (v & kScanIncl);
+                }
+                const u64 P = ballot(incl);
+                const u64 R = ballot(ready);
+                const unsigned fp = P ? (unsigned)__ffsll((long long)P) - 1u : 63u;
+                const u64 upto = fp == 63u ? kFull : ((2ull << fp) - 1ull);
+                if ((R & upto) != upto) {
+                    if (++spins > kScanSpin) {
+                        // cannot happen while every started tile runs on;
+                        // recorded, and the chain goes on with no grant
+                        if (lane == 0) atom_add_u64(&c.scan_ctl[1], 1ull);
+                        st.errors++;
+                        grant = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                u64 part = (lane <= fp && idx >= 0) ? (v & kScanSat) : 0ull;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) part += shfl64(part, lane ^ (unsigned)d);
+                excl += part;
+                if (P) {
+                    grant = (uint32_t)((readlane64(v, fp) >> 31) & 0x7FFFFFFFull);
+                    break;
+                }
+                k -= 64;
+            }
+            if (lane == 0) st_scan(&c.scan[tile], scan_word(parity, true, grant, excl + agg));
+        }
+        if (lane == 0) {
+            s_excl = excl > kScanSat ? kScanSat : (uint32_t)excl;
+            s_grant = grant;
+        }
+    }
+    __syncthreads();
+
+    // claim the bits of rank < G in this thread's word; the result is
+    // checked once the ids are out
+    const uint32_t grant = s_grant;
+    const uint32_t base = s_excl + woff;  // rank of the wave's first free page
+    const uint32_t rk = base + pre;
+    uint32_t take = 0;
+    if (rk < grant) take = grant - rk < pc ? grant - rk : pc;
+    const u64 mask = lowest_bits(fm, take);
+    u64 old = 0;
     if (mask) {
-        const u64 old = atom_or(&c.bitmap[wd], mask);
+        old = atom_or(&c.bitmap[w], mask);
         st.atomics++;
-        owned = mask & ~old;
     }
-    // the range continues iff the last word of the window ends below R_end
-    const u64 valid = ballot(wd < c.n_words);
-    more = false;
-    if (valid == kFull) {
-        const uint32_t end63 = readlane(rk + pc, 63u);
-        more = end63 < R_end;
+    unsigned T = 0;
+    s_pre[wv][lane] = scan7(take, T);
+    s_fm[wv][lane] = fm;
+    __syncthreads();
+
+    // ids in rank order: output q of the wave lies in the last word whose
+    // prefix is <= q
+    const uint32_t wbase = tile * kScanWords + wv * 64u;
+    for (unsigned q = lane; q < T; q += 64u) {
+        unsigned s = 0;
+#pragma unroll
+        for (unsigned d = 32; d >= 1; d >>= 1)
+            if (s_pre[wv][s + d] <= q) s += d;
+        const unsigned bit = select_bit(s_fm[wv][s], q - s_pre[wv][s]);
+        const uint32_t r = base + q;
+        ids[r] = (int32_t)((wbase + s) * 64u + bit);
+        if (steps_out) steps_out[r] = 1u;
+        st.allocs++;
+        st.steps++;
     }
-    unsigned src = lane, j = 0;
-    bool assigned = false;
-    u64 G = ballot(mask != 0);
-    while (G) {
-        const unsigned g = (unsigned)__ffsll((long long)G) - 1u;
-        G &= G - 1ull;
-        const uint32_t rk_g = readlane(rk, g), lo_g = readlane(lo, g), hi_g = readlane(hi, g);
-        const uint32_t first_q = rk_g + lo_g - R;
-        if (need && lane >= first_q && lane < first_q + (hi_g - lo_g)) {
-            src = g;
-            j = lo_g + (lane - first_q);
-            assigned = true;
+    const u64 lost = mask & old;
+    if (lost) {
+        // a concurrent writer took granted bits: their requests walk, and
+        // overwrite what the loop above wrote (this wave's stores first)
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        u64 m = lost;
+        while (m) {
+            const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
+            m &= m - 1ull;
+            const uint32_t r = rk + (uint32_t)__popcll(fm & ((1ull << bit) - 1ull));
+            uint32_t steps = 1;
+            st.fallbacks++;
+            st.steps--;
+            const int32_t id = rank_fallback(c, req_base + r, budget, steps, st);
+            ids[r] = id;
+            if (steps_out) steps_out[r] = steps;
+            st.steps += steps;
+            if (id < 0) {
+                st.allocs--;
+                st.failures++;
+            }
         }
     }
-    const u64 s_owned = shfl64(owned, src);
-    const u64 s_fm = shfl64(fm, src);
-    const uint32_t s_wd = shfl32(wd, src);
-    if (assigned) {
-        const unsigned bit = select_bit(s_fm, j);
-        if ((s_owned >> bit) & 1ull) {
-            id = (int32_t)(s_wd * 64u + bit);
-            need = false;
+
+    // requests reserved beyond the free pages the scan saw (a concurrent
+    // free moved the count ahead of the map): the last tile walks for them
+    if (tile + 1u == c.n_tiles) {
+        const u64 seen = (u64)s_excl + agg;
+        for (u64 r = seen + threadIdx.x; r < grant; r += blockDim.x) {
+            uint32_t steps = 0;
+            const int32_t id = rank_fallback(c, req_base + r, budget, steps, st);
+            ids[r] = id;
+            if (steps_out) steps_out[r] = steps;
+            if (id >= 0) st.allocs++; else st.failures++;
+            st.steps += steps;
         }
     }
+    // requests past the grant fail; every tile writes its slice of them
+    {
+        const uint32_t rest = n - grant;
+        const uint32_t per = (uint32_t)(((u64)rest + c.n_tiles - 1u) / c.n_tiles);
+        const u64 lo = (u64)grant + (u64)tile * per;
+        const u64 hi = lo + per < (u64)n ? lo + per : (u64)n;
+        for (u64 r = lo + threadIdx.x; r < hi; r += blockDim.x) {
+            ids[r] = -1;
+            if (steps_out) steps_out[r] = 0;
+            st.failures++;
+        }
+    }
+    flush_stats(c, st);
 }
 
 // Persistent batched allocation.  Each wave serves chunks of 64 requests.
@@ -214,7 +305,7 @@ __device__ inline void rank_window(const pbm_ctx& c, uint32_t w0, uint32_t R, ui
 constexpr uint32_t kSweepMax = 256;
 constexpr uint32_t kSweepRound = 16;
 
-template <int J, bool RANK>
+template <int J>
 __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                      int32_t* __restrict__ ids,
                                                      uint32_t* __restrict__ steps_out,
@@ -248,22 +339,6 @@ __global__ __launch_bounds__(256) void k_get_batched(pbm_ctx c, uint32_t n, uint
             granted > 0 ? (uint32_t)(((u64)ch * 64u * L) / (u64)granted % L) : 0u;
         bool have = false, sweep = false, homed = false, found_pass = false;
         uint32_t start = 0, cur = 0;
-        if (RANK) {
-            // Request r takes the free page of rank r; the walk below only
-            // serves what a concurrent writer took away.
-            const uint32_t total = c.sb_rank[c.n_sb];
-            const uint32_t R = ch * 64u;
-            const uint32_t lim = granted < (long long)total ? (uint32_t)granted : total;
-            if (R < lim) {
-                const uint32_t R_end = R + 64u < lim ? R + 64u : lim;
-                const uint32_t w0 = rank_search(c, R);
-                bool more = true;
-                for (uint32_t k = 0; more && ballot(need) && k <= L / 64u + 1u; ++k) {
-                    if (need) ++steps;
-                    rank_window(c, w0 + 64u * k, R, R_end, need, id, st, more);
-                }
-            }
-        }
         // A kept window or a sweep position is retried only while it showed
         // free bits (lost races); the cap bounds the loop even so.
         const uint32_t max_iters = budget + 4u * max_dry * seg + 65536u;
@@ -401,9 +476,12 @@ __global__ __launch_bounds__(1024) void k_free_plain(pbm_ctx c, uint32_t n,
 }
 
 // Batched free: adjacent lanes whose pages share a word merge their bits
-// into one atomicAnd issued by the first lane of the run.
-__global__ __launch_bounds__(256) void k_free_batched(pbm_ctx c, uint32_t n,
-                                                      const int32_t* __restrict__ ids) {
+// into one atomicAnd issued by the first lane of the run.  Each wave loads
+// the ids of its next chunk before it works on the current one, and each
+// 1024-thread workgroup returns its pages to the reservation counter with
+// one atomic.
+__global__ __launch_bounds__(1024) void k_free_batched(pbm_ctx c, uint32_t n,
+                                                       const int32_t* __restrict__ ids) {
     using namespace pbm;
     const unsigned lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -413,9 +491,12 @@ __global__ __launch_bounds__(256) void k_free_batched(pbm_ctx c, uint32_t n,
     if (threadIdx.x == 0) s_freed = 0;
     __syncthreads();
     Stats st;
+    int32_t next = -1;
+    if (wave < n_chunks && wave * 64u + lane < n) next = ids[wave * 64u + lane];
     for (uint32_t ch = wave; ch < n_chunks; ch += n_waves) {
-        const uint32_t r = ch * 64u + lane;
-        const int32_t id = r < n ? ids[r] : -1;
+        const int32_t id = next;
+        const uint32_t nr = (ch + n_waves) * 64u + lane;
+        next = (ch + n_waves < n_chunks && nr < n) ? ids[nr] : -1;
         const bool valid = id >= 0 && (uint32_t)id < c.n_pages;
         if (id >= 0 && !valid) st.errors++;
         const uint32_t w = valid ? ((uint32_t)id >> 6) : (0x80000000u | lane);
@@ -501,29 +582,29 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
     if (L.mode == PBM_MODE_BATCHED) {
         if (strat == PBM_LINKED_LIST)
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
+        if (L.flags & PBM_LAUNCH_RANK) {
+            hipLaunchKernelGGL(k_get_rank, dim3(c.n_tiles), dim3(pbm::kScanWords), 0, s, c, n,
+                               req_base, d_ids, d_steps, c.walk_budget);
+            PBM_HIP(hipGetLastError());
+            return PBM_OK;
+        }
         const bool one = (L.flags & PBM_LAUNCH_ONE_WAVE) != 0;
-        const bool rank = (L.flags & PBM_LAUNCH_RANK) != 0;
         const dim3 grid(one ? 1u : batched_blocks(h, L, n));
         const dim3 blk(one ? 64u : 256u);
-        if (rank && !(L.flags & PBM_LAUNCH_RANK_READY)) {
-            rc = pbm_rank_prepass(h, s);
-            if (rc) return rc;
-        }
-#define PBM_LAUNCH_BATCHED(JJ)                                                              \
-    do {                                                                                    \
-        if (rank)                                                                           \
-            hipLaunchKernelGGL((k_get_batched<JJ, true>), grid, blk, 0, s, c, n, req_base,  \
-                               d_ids, d_steps, c.walk_budget);                              \
-        else                                                                                \
-            hipLaunchKernelGGL((k_get_batched<JJ, false>), grid, blk, 0, s, c, n, req_base, \
-                               d_ids, d_steps, c.walk_budget);                              \
-    } while (0)
         switch (L.cluster_words) {
-            case 64: PBM_LAUNCH_BATCHED(1); break;
-            case 128: PBM_LAUNCH_BATCHED(2); break;
-            default: PBM_LAUNCH_BATCHED(4); break;
+            case 64:
+                hipLaunchKernelGGL(k_get_batched<1>, grid, blk, 0, s, c, n, req_base, d_ids, d_steps,
+                                   c.walk_budget);
+                break;
+            case 128:
+                hipLaunchKernelGGL(k_get_batched<2>, grid, blk, 0, s, c, n, req_base, d_ids, d_steps,
+                                   c.walk_budget);
+                break;
+            default:
+                hipLaunchKernelGGL(k_get_batched<4>, grid, blk, 0, s, c, n, req_base, d_ids, d_steps,
+                                   c.walk_budget);
+                break;
         }
-#undef PBM_LAUNCH_BATCHED
         PBM_HIP(hipGetLastError());
         return PBM_OK;
     }
@@ -550,20 +631,6 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
     return PBM_OK;
 }
 
-extern "C" int pbm_rank_prepass(pbm_handle* h, void* stream) {
-    PBM_CHECK_HANDLE(h);
-    if (h->cfg.strategy == PBM_LINKED_LIST)
-        return pbmi::fail(PBM_ERR_UNSUPPORTED, "the rank pre-pass needs a bitmap strategy");
-    hipStream_t s = pbmi::as_stream(stream);
-    const pbm_ctx& c = h->ctx;
-    uint32_t nb = (c.n_sb + 3u) / 4u;
-    if (nb > 4096u) nb = 4096u;
-    hipLaunchKernelGGL(k_rank_local, dim3(nb), dim3(256), 0, s, c);
-    hipLaunchKernelGGL(k_rank_scan, dim3(1), dim3(1024), 0, s, c);
-    PBM_HIP(hipGetLastError());
-    return PBM_OK;
-}
-
 extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
                               const pbm_launch* launch, void* stream) {
     PBM_CHECK_HANDLE(h);
@@ -580,8 +647,8 @@ extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
         return pbmi::launch_free_list(c, L, n, d_ids, s);
     }
     if (L.mode == PBM_MODE_BATCHED) {
-        const uint32_t nb = batched_blocks(h, L, n);
-        hipLaunchKernelGGL(k_free_batched, dim3(nb), dim3(256), 0, s, c, n, d_ids);
+        const uint32_t nb = (batched_blocks(h, L, n) + 3u) / 4u;
+        hipLaunchKernelGGL(k_free_batched, dim3(nb), dim3(1024), 0, s, c, n, d_ids);
     } else if (L.mode == PBM_MODE_SERIAL) {
         hipLaunchKernelGGL(k_free_plain, dim3(1), dim3(1), 0, s, c, n, d_ids);
     } else {

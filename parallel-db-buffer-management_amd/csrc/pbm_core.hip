@@ -191,10 +191,9 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     PBM_ALLOC(h->d_counters, sizeof(pbm_counters));
     PBM_ALLOC(h->d_phase, 64u);
     PBM_ALLOC(h->d_scratch, 64u);
-    const uint32_t n_sb = (n_words + 63u) / 64u;
-    PBM_ALLOC(h->d_sb_rank, (size_t)(n_sb + 1u) * 4u);
-    PBM_ALLOC(h->d_w_rank, (size_t)n_words * 4u);
-    PBM_ALLOC(h->d_snap, (size_t)n_words * 8u);
+    const uint32_t n_tiles = (n_words + pbm::kScanWords - 1u) / pbm::kScanWords;
+    PBM_ALLOC(h->d_scan, (size_t)n_tiles * 8u);
+    PBM_ALLOC(h->d_scan_ctl, 32u);
     if (cfg->backing) PBM_ALLOC(h->d_pool, (size_t)cfg->n_pages * cfg->page_size);
     if (cfg->strategy == PBM_CRW_BM) PBM_ALLOC(h->d_hint, (size_t)h->cfg.max_threads * 4u);
     if (cfg->strategy == PBM_DIST_CLOCK) PBM_ALLOC(h->d_arms, (size_t)h->cfg.max_threads * 4u);
@@ -218,10 +217,9 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     c.ring_head = h->d_ring_ht;
     c.ring_tail = h->d_ring_ht ? h->d_ring_ht + 1 : nullptr;
     c.phase = h->d_phase;
-    c.sb_rank = h->d_sb_rank;
-    c.w_rank = h->d_w_rank;
-    c.snap = h->d_snap;
-    c.n_sb = n_sb;
+    c.scan = h->d_scan;
+    c.scan_ctl = h->d_scan_ctl;
+    c.n_tiles = n_tiles;
     c.n_pages = cfg->n_pages;
     c.n_words = n_words;
     c.page_size = cfg->page_size;
@@ -234,6 +232,15 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
     e = hipMemset(h->d_phase, 0, 64u);
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
+    // Look-back words start with call parity 0 and the tickets at n_tiles,
+    // so the first call has parity 1 and no zeroed word reads as published.
+    e = hipMemset(h->d_scan, 0, (size_t)n_tiles * 8u);
+    if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
+    {
+        const unsigned long long ctl[4] = {n_tiles, 0, 0, 0};
+        e = hipMemcpy(h->d_scan_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemcpy", __FILE__, __LINE__));
+    }
     if ((rc = pbm_reset(h, nullptr))) return cleanup(rc);
     e = hipDeviceSynchronize();
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipDeviceSynchronize", __FILE__, __LINE__));
@@ -248,7 +255,7 @@ extern "C" int pbm_destroy(pbm_handle* h) {
     (void)hipDeviceSynchronize();
     void* ptrs[] = {h->d_bitmap, h->d_free, h->d_counters, h->d_pool, h->d_hint, h->d_arms,
                     h->d_clock, h->d_ring, h->d_ring_ht, h->d_phase, h->d_scratch,
-                    h->d_sb_rank, h->d_w_rank, h->d_snap};
+                    h->d_scan, h->d_scan_ctl};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete h;

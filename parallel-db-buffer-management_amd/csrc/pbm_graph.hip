@@ -26,10 +26,9 @@ extern "C" int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, in
     PBM_CHECK_HANDLE(h);
     if (!out) return pbmi::fail(PBM_ERR_INVALID, "null output graph pointer");
     *out = nullptr;
-    if (phases == 0 || (phases & ~(PBM_PHASE_ALLOC | PBM_PHASE_FREE | PBM_PHASE_RANK)))
-        return pbmi::fail(PBM_ERR_INVALID, "phases must combine PBM_PHASE_RANK/ALLOC/FREE");
-    if ((phases & (PBM_PHASE_ALLOC | PBM_PHASE_FREE)) && (n == 0 || !d_ids))
-        return pbmi::fail(PBM_ERR_INVALID, "graph needs n > 0 and d_ids");
+    if (phases == 0 || (phases & ~(PBM_PHASE_ALLOC | PBM_PHASE_FREE)))
+        return pbmi::fail(PBM_ERR_INVALID, "phases must combine PBM_PHASE_ALLOC/FREE");
+    if (n == 0 || !d_ids) return pbmi::fail(PBM_ERR_INVALID, "graph needs n > 0 and d_ids");
     hipStream_t s = nullptr;
     PBM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     pbm_graph* g = new pbm_graph();
@@ -39,8 +38,7 @@ extern "C" int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, in
     if (e != hipSuccess) {
         rc = pbmi::hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
     } else {
-        if (phases & PBM_PHASE_RANK) rc = pbm_rank_prepass(h, s);
-        if (!rc && (phases & PBM_PHASE_ALLOC))
+        if (phases & PBM_PHASE_ALLOC)
             rc = pbm_get_pages(h, n, req_base, d_ids, d_steps, get_launch, s);
         if (!rc && (phases & PBM_PHASE_FREE)) rc = pbm_free_pages(h, n, d_ids, free_launch, s);
         hipGraph_t graph = nullptr;

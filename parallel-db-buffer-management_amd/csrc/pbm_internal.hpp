@@ -24,9 +24,8 @@ struct pbm_handle {
     unsigned long long* d_ring_ht = nullptr;  // [0] head, [1] tail
     long long* d_phase = nullptr;
     unsigned long long* d_scratch = nullptr;  // 8 words for reductions
-    uint32_t* d_sb_rank = nullptr;            // n_sb + 1 superblock ranks
-    uint32_t* d_w_rank = nullptr;             // per-word rank inside its superblock
-    unsigned long long* d_snap = nullptr;     // free bits at the last rank pre-pass
+    unsigned long long* d_scan = nullptr;     // rank scan: one look-back word per tile
+    unsigned long long* d_scan_ctl = nullptr; // rank scan: next tile, tiles done, epoch
 };
 
 namespace pbmi {

@@ -29,9 +29,8 @@ MODE_PLAIN, MODE_BATCHED, MODE_SERIAL = 0, 1, 2
 LAUNCH_NO_COUNTERS = 1
 LAUNCH_ONE_WAVE = 2
 LAUNCH_RANK = 4
-LAUNCH_RANK_READY = 8
 
-PHASE_ALLOC, PHASE_FREE, PHASE_RANK = 1, 2, 4
+PHASE_ALLOC, PHASE_FREE = 1, 2
 
 ERRORS = {0: "PBM_OK", -1: "PBM_ERR_INVALID", -2: "PBM_ERR_HIP", -3: "PBM_ERR_OOM",
           -4: "PBM_ERR_UNSUPPORTED", -5: "PBM_ERR_RANGE"}
@@ -56,11 +55,11 @@ class Ctx(C.Structure):
     _fields_ = [("bitmap", C.c_void_p), ("free_count", C.c_void_p), ("counters", C.c_void_p),
                 ("pool", C.c_void_p), ("hint", C.c_void_p), ("arms", C.c_void_p),
                 ("clock_arm", C.c_void_p), ("ring", C.c_void_p), ("ring_head", C.c_void_p),
-                ("ring_tail", C.c_void_p), ("phase", C.c_void_p), ("sb_rank", C.c_void_p),
-                ("w_rank", C.c_void_p), ("snap", C.c_void_p),
+                ("ring_tail", C.c_void_p), ("phase", C.c_void_p), ("scan", C.c_void_p),
+                ("scan_ctl", C.c_void_p),
                 ("n_pages", C.c_uint32), ("n_words", C.c_uint32), ("page_size", C.c_uint32),
                 ("walk_budget", C.c_uint32), ("max_threads", C.c_uint32), ("order", C.c_int32),
-                ("seed", C.c_uint64), ("n_sb", C.c_uint32), ("reserved0", C.c_uint32)]
+                ("seed", C.c_uint64), ("n_tiles", C.c_uint32), ("reserved0", C.c_uint32)]
 
 
 class Launch(C.Structure):
@@ -103,7 +102,6 @@ _SIGS = {
                                 C.POINTER(Launch), C.c_void_p]),
     "pbm_free_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
                                  C.c_void_p]),
-    "pbm_rank_prepass": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pbm_get_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.POINTER(Launch), C.c_void_p]),
     "pbm_free_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
@@ -267,9 +265,6 @@ class Pool:
     def free_pages(self, n: int, ids, lnch: Launch | None = None, stream=None) -> None:
         check(lib().pbm_free_pages(self.h, n, _ptr(ids), C.byref(lnch or launch()),
                                    _stream(stream)), "pbm_free_pages")
-
-    def rank_prepass(self, stream=None) -> None:
-        check(lib().pbm_rank_prepass(self.h, _stream(stream)), "pbm_rank_prepass")
 
     def get_n_pages(self, n: int, counts, ids, steps=None, req_base: int = 0,
                     lnch: Launch | None = None, stream=None) -> None:

@@ -120,7 +120,9 @@ def test_one_wave_batched_budget_sweep_matches_oracle(gpu, T, fill, n, budget, o
 
 
 @pytest.mark.parametrize("T,fill,n", [(BIG, 0.9, 262144), (BIG, 0.5, 262144), (BIG, 0.99, 262144),
-                                      (65536, 0.0, 32768), (1000, 0.4, 700), (1 << 16, 0.999, 100)])
+                                      (65536, 0.0, 32768), (1000, 0.4, 700), (1 << 16, 0.999, 100),
+                                      (1 << 22, 0.9, 300000), (1 << 24, 0.97, 200000),
+                                      (64 * 256 * 65 + 17, 0.6, 1 << 20)])
 @pytest.mark.parametrize("one_wave", [False, True])
 def test_batched_rank_matches_oracle_exactly(gpu, T, fill, n, one_wave):
     # by rank, request r takes the r-th free page whatever the schedule, so
@@ -137,6 +139,26 @@ def test_batched_rank_matches_oracle_exactly(gpu, T, fill, n, one_wave):
         assert pool.free_count() == st.free[0]
         _free(gpu, pool, ids, pbm.MODE_BATCHED)
         assert np.array_equal(pool.snapshot(), orc.prefill(T, fill, 42))
+
+
+def test_batched_rank_calls_in_a_row(gpu):
+    # each call flips the look-back epoch; calls of different sizes on the
+    # same pool must each match the oracle continuing from the last state
+    T = 1 << 21
+    log_case(f"batched rank calls in a row T={T}")
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=4) as pool:
+        pool.prefill_bernoulli(0.8, 9)
+        st = orc.State(orc.prefill(T, 0.8, 9), T)
+        for k, n in enumerate([1, 64, 5000, 100000, 777, 300000, 3, 150000]):
+            ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK)
+            eids, esteps = st.batched_rank(n)
+            assert np.array_equal(ids, eids), f"call {k} n={n}"
+            assert np.array_equal(steps, esteps), f"call {k} n={n}"
+            assert pool.free_count() == st.free[0]
+            if k % 3 == 2:
+                _free(gpu, pool, ids, pbm.MODE_BATCHED)
+                st.free_ids(ids)
+        assert np.array_equal(pool.snapshot(), st.bm)
 
 
 # ------------------------------------------------------------------ concurrent invariants

@@ -601,11 +601,8 @@ def test_batched_rank_takes_free_pages_in_map_order():
     n = len(free) + 50
     ids, steps = st.batched_rank(n)
     assert ids.tolist() == free + [-1] * 50
-    # a request's window index counts 64-word windows from its chunk's
-    # first word
-    for r in range(len(free)):
-        w0 = free[(r // 64) * 64] // 64
-        assert steps[r] == (free[r] // 64 - w0) // 64 + 1
-    assert (steps[len(free):] == 0).all()
+    # one probe of its word per grant, none per refusal
+    assert (steps[:len(free)] == 1).all() and (steps[len(free):] == 0).all()
+    assert st.free[0] == 0
     rc, g = orc.check_alloc(before, st.bm, T, ids)
     assert rc == 0 and g == len(free)
