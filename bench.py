@@ -8,11 +8,12 @@ Workload (one step = one alloc phase + one free phase, inputs in HBM):
   granted page freed.  About 104,858 pages are free, so about 157,286
   requests fail by admission; allocs+frees counts the grants and the frees.
 
-A step is one launch of a captured HIP graph holding both phases, on
-torch's current stream; the step time is the mean over back-to-back
-launches between two HIP events.  Each phase kernel is also timed on its
-own from events around eager launches on the same stream (the roofline
-uses those durations).  The batched launch (rank or
+A step is the alloc kernel then the free kernel, launched back to back on
+torch's current stream; the step time is the mean over the timed steps
+between two HIP events.  The same step captured as one HIP graph is timed
+too (ms_per_step_graph).  Each phase kernel is also timed on its own from
+events around it (the roofline uses those durations, which include the
+launch).  The batched launch (rank or
 walk homes, window width, walk budget, waves per CU) comes from the
 library's autotuner; the best walk-homes launch is timed too.  The plain
 variant is the reference's RandomWalkBasic_BM getPage/freePage, one thread
@@ -54,17 +55,17 @@ def walk_stats(steps: np.ndarray, ids: np.ndarray) -> dict:
     return out
 
 
-def time_steps(torch, graph, steps: int, warmup: int, stream) -> float:
-    """Mean ms of one launch of `graph` (both phases), `steps` launches
-    back to back between two events."""
+def time_region(torch, fn, steps: int, warmup: int, stream) -> float:
+    """Mean ms of fn() over `steps` calls launched back to back between two
+    events on `stream`, after `warmup` untimed calls."""
     for _ in range(warmup):
-        graph.launch(stream)
+        fn()
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(stream)
     for _ in range(steps):
-        graph.launch(stream)
+        fn()
     t1.record(stream)
     torch.cuda.synchronize()
     return t0.elapsed_time(t1) / steps
@@ -89,8 +90,8 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawTextHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="host time spent on the CPU-baseline sample (0 = skip)")
     ap.add_argument("--no-tune", action="store_true",
@@ -163,15 +164,23 @@ def main() -> int:
         return c_alloc, c_all, walk_stats(s_np, i_np), ok
 
     def measure(gl, fl, steps, warmup):
-        """Counted cycle, whole-step graph timing and per-phase timing."""
+        """Counted cycle; step time with the two phase kernels launched
+        back to back (eager) and as one captured graph; per-phase time."""
         c_alloc, c_all, ws, ok = counted(gl, fl)
+
+        def eager():
+            pool.get_pages(N_REQ, ids, None, 0, gl, stream)
+            pool.free_pages(N_REQ, ids, fl, stream)
+
+        ms = time_region(torch, eager, steps, warmup, stream)
         g = pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_ALLOC | pbm.PHASE_FREE)
-        ms = time_steps(torch, g, steps, warmup, stream)
+        ms_graph = time_region(torch, lambda: g.launch(stream), steps, warmup, stream)
         g.close()
-        phases = time_phases(torch, pool, N_REQ, ids, gl, fl, steps, stream)
+        phases = time_phases(torch, pool, N_REQ, ids, gl, fl, min(steps, 200), stream)
         ok = ok and bool(np.array_equal(pool.snapshot(stream), before))
-        return {"ms_per_step": ms, "phases_ms": phases, "counters_alloc": c_alloc,
-                "counters_cycle": c_all, "walk_steps": ws, "verified": ok}
+        return {"ms_per_step": ms, "ms_per_step_graph": ms_graph, "phases_ms": phases,
+                "counters_alloc": c_alloc, "counters_cycle": c_all, "walk_steps": ws,
+                "verified": ok}
 
     results = {}
     for name, tuned in (("batched", best), ("batched_walk", tune_walk)):
@@ -183,7 +192,7 @@ def main() -> int:
     p_get = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
     p_free = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
     p_steps = max(3, min(args.steps, 10))
-    rp = measure(p_get, p_free, p_steps, 1)
+    rp = measure(p_get, p_free, p_steps, 2)
     cp_alloc, cp_all, ws_p, ok_p = (rp["counters_alloc"], rp["counters_cycle"], rp["walk_steps"],
                                     rp["verified"])
 
@@ -243,6 +252,7 @@ def main() -> int:
                    "pages": T_PAGES, "page_size": PAGE_SIZE, "fill": FILL,
                    "requests": N_REQ, "free_pages": A, "grants": granted,
                    "launch": b["tuned"], "device": dev_name},
+        "ms_per_step_graph": b["ms_per_step_graph"],
         "phases_ms": b["phases_ms"],
         "walk_steps": b["walk_steps"],
         "verified": b["verified"],
@@ -253,7 +263,8 @@ def main() -> int:
         "speedup_vs_plain": value / value_p,
         "batched_walk": {"value": ops_per_step / (results["batched_walk"]["ms_per_step"] * 1e-3),
                          **{k: results["batched_walk"][k] for k in
-                            ("ms_per_step", "phases_ms", "walk_steps", "verified", "tuned")}},
+                            ("ms_per_step", "ms_per_step_graph", "phases_ms", "walk_steps",
+                             "verified", "tuned")}},
         "counters": {"alloc": ca, "cycle": cc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -118,6 +118,12 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
     __shared__ u64 s_fm[4][64];
     const unsigned lane = lane_id();
     const unsigned wv = threadIdx.x >> 6;
+    // The words of tile blockIdx.x are read while the ticket is in flight:
+    // tickets mostly follow dispatch order, and a tile read early is still
+    // a valid snapshot (the claims check every bit).
+    const uint32_t w_spec = blockIdx.x * kScanWords + threadIdx.x;
+    u64 spec = kFull;
+    if (w_spec < c.n_words) spec = ld_word(&c.bitmap[w_spec]);
     if (threadIdx.x == 0) {
         const u64 t = atom_add_u64(&c.scan_ctl[0], 1ull);
         s_free = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -131,7 +137,8 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
     Stats st;
     u64 fm = 0;
     if (w < c.n_words) {
-        fm = ~ld_word(&c.bitmap[w]);  // padding bits past n_pages read as used
+        // padding bits past n_pages read as used
+        fm = ~(tile == blockIdx.x ? spec : ld_word(&c.bitmap[w]));
         st.probes++;
     }
     const uint32_t pc = (uint32_t)__popcll(fm);
