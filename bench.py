@@ -219,6 +219,15 @@ def main() -> int:
     dom = max(b["phases_ms"], key=b["phases_ms"].get)
     dom_ms = b["phases_ms"][dom]
     achieved = bytes_k[dom] / (dom_ms * 1e-3) / 1e9
+    # HBM bytes per launch of that kernel from the committed FETCH_SIZE /
+    # WRITE_SIZE counter runs of this workload (null when not profiled)
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_c2_fill90.json"
+    if tfile.exists():
+        shapes = {"k_get_rank": (L + 255) // 256}
+        for k in json.loads(tfile.read_text())["kernels"]:
+            if k["kernel"] == kernel_names[dom] and k["workgroups"] == shapes.get(k["kernel"]):
+                traffic = k["hbm_bytes"]
     atom_peak = pbm.atomic_peak(L, 64)
     atom_rate = (ca["atomics"] / (b["phases_ms"]["alloc"] * 1e-3))
 
@@ -267,7 +276,7 @@ def main() -> int:
                              "verified", "tuned")}},
         "counters": {"alloc": ca, "cycle": cc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": kernel_names[dom], "phase": dom, "bytes_per_launch": bytes_k[dom],
                      "launch_ms": dom_ms,
                      "atomic": {"achieved_ops": atom_rate, "peak_ops": atom_peak,
