@@ -272,6 +272,91 @@ int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_id
 int pbm_graph_launch(pbm_graph* g, void* stream);
 int pbm_graph_destroy(pbm_graph* g);
 
+/* ---- serving loop (continuous batching) -------------------------------
+ *
+ * A device-resident request loop over one pool.  Sequences are submitted
+ * with an arrival step and a target length in pages; every step
+ *   1. admits waiting sequences in arrival order while a slot is free and
+ *      the pool's free-page count, less the pages still owed to admitted
+ *      sequences, covers the sequence's whole target (so an admitted
+ *      sequence never fails to grow);
+ *   2. grows every admitted sequence by one page with the pool strategy's
+ *      device getPage, called from one thread per slot (wave-collective
+ *      strategies serve the active lanes of each wave together), timing
+ *      each call with wall_clock64;
+ *   3. frees the page lists of the sequences that reached their target.
+ * Arrivals that come while others run join at the next step; all admitted
+ * sequences grow in the same kernel.  With a backed pool and
+ * PBM_SERVE_PAYLOAD, page j of a sequence holds a 64-bit payload that
+ * depends on (seed, the sequence's key, j) only; at the end its pages are read
+ * back, checked and summed, so a sequence's (length, checksum) is the same
+ * alone and in any mix.  No reference counterpart: the reference has no
+ * serving loop; its device getPage/freePage are what each step calls. */
+
+#define PBM_SERVE_PAYLOAD 1u  /* write, check and sum page payloads (backed pool)  */
+#define PBM_SERVE_LATENCY 2u  /* keep per-allocation latencies (first lat_capacity) */
+
+typedef struct pbm_serve_config {
+    uint32_t max_slots;     /* sequences admitted at once, 1 .. 2^22                  */
+    uint32_t max_pages;     /* page-list capacity of a slot; longer targets refused   */
+    uint32_t max_seqs;      /* sequences that can be submitted over the server's life */
+    uint32_t flags;         /* PBM_SERVE_*                                            */
+    uint32_t lat_capacity;  /* latency samples kept (PBM_SERVE_LATENCY)               */
+    uint32_t reserved;
+    uint64_t seed;          /* payload seed                                           */
+} pbm_serve_config;
+
+typedef struct pbm_serve_stats {
+    unsigned long long step;            /* steps run                                 */
+    unsigned long long submitted;       /* sequences submitted                       */
+    unsigned long long admitted;        /* sequences admitted                        */
+    unsigned long long finished;        /* sequences that reached their target       */
+    unsigned long long active;          /* admitted and not finished                 */
+    unsigned long long reserved_pages;  /* pages still owed to active sequences      */
+    unsigned long long allocs;          /* pages granted                             */
+    unsigned long long frees;           /* pages returned                            */
+    unsigned long long alloc_failures;  /* getPage calls that returned -1 (stays 0)  */
+    unsigned long long payload_errors;  /* payloads found changed at the end         */
+    unsigned long long lat_samples;     /* latencies recorded (may exceed capacity)  */
+} pbm_serve_stats;
+
+typedef struct pbm_server pbm_server;
+
+/* Creates a server on pool h (any strategy).  The pool must outlive it. */
+int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_server** out);
+int pbm_serve_destroy(pbm_server* s);
+
+/* Appends n sequences (host arrays): arrival_step[i] (non-decreasing, also
+ * across calls), target[i] in [1, max_pages] and key[i], the request's
+ * own id that its payloads derive from (NULL: the sequence id).  Sequence
+ * ids count up from 0 in submission order.  Synchronises the stream. */
+int pbm_serve_submit(pbm_server* s, uint32_t n, const uint32_t* arrival_step,
+                     const uint32_t* target, const uint32_t* key, void* stream);
+
+/* Runs n_steps steps (three kernels each) on the stream. */
+int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream);
+
+/* Captures n_steps steps as one HIP graph (launch with pbm_graph_launch;
+ * every launch runs the next n_steps steps). */
+int pbm_serve_graph(pbm_server* s, uint32_t n_steps, pbm_graph** out);
+
+int pbm_serve_read_stats(pbm_server* s, pbm_serve_stats* out, void* stream);
+
+/* Per-sequence results for ids first .. first+n-1 (host arrays, any may be
+ * NULL): step admitted, step finished (0 = not yet), pages held at the end
+ * and the sum of their payloads (0 without PBM_SERVE_PAYLOAD). */
+int pbm_serve_read_results(pbm_server* s, uint32_t first, uint32_t n, uint32_t* admit_step,
+                           uint32_t* finish_step, uint32_t* length,
+                           unsigned long long* checksum, void* stream);
+
+/* Copies up to n recorded latencies (wall-clock ticks) to host; *got =
+ * how many were copied. */
+int pbm_serve_read_latency(pbm_server* s, uint32_t* ticks, uint32_t n, uint32_t* got,
+                           void* stream);
+
+/* Frequency of the wall clock the latencies are counted in, Hz. */
+int pbm_serve_clock_hz(pbm_server* s, unsigned long long* hz);
+
 /* ---- autotuning ------------------------------------------------------- */
 
 #define PBM_TUNE_QUICK     1u  /* 3 timed cycles per candidate instead of 9            */

@@ -39,6 +39,7 @@ constexpr u64 kGolden = 0x9E3779B97F4A7C15ull;
 constexpr u64 kPrefillSalt = 0x243F6A8885A308D3ull;
 constexpr u64 kWaveSalt = 0x13198A2E03707344ull;
 constexpr u64 kArmSalt = 0xA4093822299F31D0ull;
+constexpr u64 kPayloadSalt = 0x452821E638D01377ull;
 /* Words per tile of the single-pass rank scan (one per thread of a
  * 256-thread workgroup). */
 constexpr uint32_t kScanWords = 256;
@@ -83,6 +84,13 @@ __host__ __device__ __forceinline__ uint32_t lcg_to_range(int32_t state, uint32_
  * mix64(seed ^ kPrefillSalt). */
 __host__ __device__ __forceinline__ u64 page_hash(u64 seed, u64 i) {
     return mix64(mix64(seed ^ kPrefillSalt) + (i + 1) * kGolden);
+}
+
+/* Payload the serving loop writes at the start of page j of sequence seq
+ * (and checks when the sequence ends): depends on (seed, seq, j) only, so
+ * a sequence's checksum is the same whatever else runs beside it. */
+__host__ __device__ __forceinline__ u64 serve_payload(u64 seed, uint32_t seq, uint32_t j) {
+    return mix64((seed ^ kPayloadSalt) + ((((u64)seq << 32) | j) + 1ull) * kGolden);
 }
 
 /* ---------------------------------------------------------------------

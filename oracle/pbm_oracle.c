@@ -939,3 +939,74 @@ double orc_rwbm_expected_steps(double T, double A, double n, int W) {
     for (double k = 0; k < n; k += 1.0) s += 1.0 / (1.0 - pow((T - A + k) / T, (double)W));
     return n > 0 ? s / n : 0.0;
 }
+
+/* ------------------------------------------------------ serving loop */
+
+/* Payload of page j of sequence seq (pbm_device.hpp serve_payload). */
+uint64_t orc_serve_payload(uint64_t seed, uint32_t seq, uint32_t j) {
+    return orc_mix64((seed ^ 0x452821E638D01377ull) +
+                     (((((u64)seq) << 32) | j) + 1ull) * 0x9E3779B97F4A7C15ull);
+}
+
+/* Checksum a sequence of `len` pages ends with: the wrapping sum of its
+ * payloads. */
+uint64_t orc_serve_checksum(uint64_t seed, uint32_t seq, uint32_t len) {
+    u64 s = 0;
+    for (uint32_t j = 0; j < len; ++j) s += orc_serve_payload(seed, seq, j);
+    return s;
+}
+
+/* The serving loop's schedule, counts only (pbm_serve.hip k_serve_admit /
+ * k_serve_grow / k_serve_release), which fix it: at step k = 1, 2, ...
+ * waiting sequences are admitted in arrival order, at most 1024 examined
+ * per step, while arrival <= k, a slot is free and free pages - pages owed
+ * to admitted sequences >= the target; then every admitted sequence gets
+ * one page, and those that reach their target finish at k and return all
+ * their pages before step k+1.  Writes admit/finish steps (0 = never) and
+ * totals[0..3] = allocs, frees, admitted, finished. */
+void orc_serve_sim(int64_t free_pages, uint32_t max_slots, uint32_t n_seqs,
+                   const uint32_t* arrival, const uint32_t* target, uint32_t n_steps,
+                   uint32_t* admit_step, uint32_t* finish_step, uint64_t* totals) {
+    uint32_t* act = (uint32_t*)malloc(sizeof(uint32_t) * (max_slots ? max_slots : 1u));
+    uint32_t* len = (uint32_t*)calloc(n_seqs ? n_seqs : 1u, sizeof(uint32_t));
+    if (!act || !len) { free(act); free(len); return; }
+    for (uint32_t i = 0; i < n_seqs; ++i) admit_step[i] = finish_step[i] = 0;
+    uint32_t n_act = 0, head = 0;
+    int64_t owed = 0;
+    u64 allocs = 0, frees = 0, admitted = 0, finished = 0;
+    for (uint32_t k = 1; k <= n_steps; ++k) {
+        uint32_t win = n_seqs - head < 1024u ? n_seqs - head : 1024u;
+        int64_t budget = free_pages - owed;
+        uint32_t m = 0;
+        while (m < win && arrival[head + m] <= k && n_act < max_slots &&
+               budget >= (int64_t)target[head + m]) {
+            budget -= target[head + m];
+            owed += target[head + m];
+            admit_step[head + m] = k;
+            act[n_act++] = head + m;
+            ++m;
+        }
+        head += m;
+        admitted += m;
+        /* grow: one page each */
+        free_pages -= n_act;
+        owed -= n_act;
+        allocs += n_act;
+        uint32_t keep = 0;
+        for (uint32_t i = 0; i < n_act; ++i) {
+            uint32_t q = act[i];
+            if (++len[q] == target[q]) {
+                finish_step[q] = k;
+                free_pages += len[q];
+                frees += len[q];
+                finished++;
+            } else {
+                act[keep++] = q;
+            }
+        }
+        n_act = keep;
+    }
+    if (totals) { totals[0] = allocs; totals[1] = frees; totals[2] = admitted; totals[3] = finished; }
+    free(act);
+    free(len);
+}

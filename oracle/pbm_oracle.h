@@ -105,6 +105,15 @@ int orc_check_alloc(const uint64_t* before, const uint64_t* after, uint32_t n_pa
  * with A free, words of W bits (theoretical_models.py:34-41 restated). */
 double orc_rwbm_expected_steps(double T, double A, double n, int W);
 
+
+/* Serving loop: payload of page j of sequence seq, a sequence's checksum,
+ * and the admission / finish schedule. */
+uint64_t orc_serve_payload(uint64_t seed, uint32_t seq, uint32_t j);
+uint64_t orc_serve_checksum(uint64_t seed, uint32_t seq, uint32_t len);
+void orc_serve_sim(int64_t free_pages, uint32_t max_slots, uint32_t n_seqs,
+                   const uint32_t* arrival, const uint32_t* target, uint32_t n_steps,
+                   uint32_t* admit_step, uint32_t* finish_step, uint64_t* totals);
+
 #ifdef __cplusplus
 }
 #endif

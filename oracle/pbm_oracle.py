@@ -58,6 +58,10 @@ _SIGS = {
                                        C.c_uint64, C.c_uint64, C.c_uint32, _P32, _PU64, _P32,
                                        _PU32]),
     "orc_check_alloc": (C.c_int, [_PU64, _PU64, C.c_uint32, C.c_uint32, _P32, _PU32]),
+    "orc_serve_payload": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "orc_serve_checksum": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "orc_serve_sim": (None, [C.c_int64, C.c_uint32, C.c_uint32, _PU32, _PU32, C.c_uint32, _PU32,
+                             _PU32, _PU64]),
     "orc_rwbm_expected_steps": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_int]),
 }
 
@@ -234,6 +238,28 @@ def check_alloc(before: np.ndarray, after: np.ndarray, n_pages: int, ids: np.nda
 
 def expected_steps(T: float, A: float, n: float, W: int = 64) -> float:
     return float(lib().orc_rwbm_expected_steps(T, A, n, W))
+
+
+def serve_payload(seed: int, seq: int, j: int) -> int:
+    return int(lib().orc_serve_payload(seed, seq, j))
+
+
+def serve_checksum(seed: int, seq: int, length: int) -> int:
+    return int(lib().orc_serve_checksum(seed, seq, length))
+
+
+def serve_sim(free_pages: int, max_slots: int, arrival: np.ndarray, target: np.ndarray,
+              n_steps: int):
+    """(admit_step, finish_step, totals) of the serving loop's schedule;
+    totals = allocs, frees, admitted, finished."""
+    a = np.ascontiguousarray(arrival, dtype=np.uint32)
+    t = np.ascontiguousarray(target, dtype=np.uint32)
+    adm = np.zeros(a.size, dtype=np.uint32)
+    fin = np.zeros(a.size, dtype=np.uint32)
+    tot = np.zeros(4, dtype=np.uint64)
+    lib().orc_serve_sim(free_pages, max_slots, a.size, _p(a, _PU32), _p(t, _PU32), n_steps,
+                        _p(adm, _PU32), _p(fin, _PU32), _p(tot, _PU64))
+    return adm, fin, tot
 
 
 def dist_clock_arm(seed: int, slot: int, n_pages: int) -> int:

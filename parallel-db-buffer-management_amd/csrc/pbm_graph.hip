@@ -5,12 +5,6 @@
 
 using pbm::u64;
 
-struct pbm_graph {
-    int device = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-};
-
 extern "C" int pbm_graph_destroy(pbm_graph* g) {
     if (!g) return PBM_OK;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);

@@ -28,6 +28,13 @@ struct pbm_handle {
     unsigned long long* d_scan_ctl = nullptr; // rank scan: next tile, tiles done, epoch
 };
 
+// An instantiated HIP graph and the device it runs on.
+struct pbm_graph {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 namespace pbmi {
 
 void set_error(const std::string& msg);

@@ -1,0 +1,574 @@
+// Continuous-batching serving loop: a device-resident slot table, arrival
+// queue and per-slot page lists; one step = admit, grow, release.
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "pbm_internal.hpp"
+
+using pbm::u64;
+
+namespace {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kAdmitWindow = 1024;  // waiting sequences examined per step
+
+// Words of the control block.
+enum : uint32_t {
+    kStep = 0,    // steps run
+    kHead,        // next sequence to admit
+    kTail,        // sequences submitted
+    kActive,      // slots admitted at the last grow (finished ones included)
+    kReserved,    // pages owed to admitted sequences after the last admission
+    kTop,         // free-slot stack height
+    kDone,        // sequences that finished at the last grow
+    kAdmitted,
+    kFinished,
+    kAllocs,
+    kFrees,
+    kAllocFail,
+    kPayloadErr,
+    kLatCount,
+    kCtlWords = 16
+};
+
+struct ServeDev {
+    pbm_ctx c;
+    const uint32_t* arr_step;  // [max_seqs] arrival step
+    const uint32_t* target;    // [max_seqs] target length
+    const uint32_t* key;       // [max_seqs] payload key
+    uint32_t* res_admit;       // [max_seqs]
+    uint32_t* res_finish;      // [max_seqs]
+    uint32_t* res_len;         // [max_seqs]
+    u64* res_sum;              // [max_seqs]
+    uint32_t* slot_seq;        // [max_slots] sequence in the slot or kEmpty
+    uint32_t* slot_len;        // [max_slots]
+    uint32_t* slot_target;     // [max_slots]
+    int32_t* pages;            // [max_slots * max_pages]
+    uint32_t* stack;           // [max_slots] free slots
+    uint32_t* done;            // [2 * max_slots] (slot, sequence) finished at the last grow
+    u64* ctl;                  // [kCtlWords]
+    uint32_t* lat;             // [lat_cap] wall-clock ticks per allocation
+    uint32_t lat_cap;
+    uint32_t max_slots;
+    uint32_t max_pages;
+    uint32_t max_seqs;
+    uint32_t flags;
+    u64 seed;
+};
+
+__device__ __forceinline__ u64 ld_ctl(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Admission (one workgroup).  First the bookkeeping of the last step: its
+// grow granted one page to each of kActive slots, and kDone of them
+// finished and go back on the free-slot stack.  Then waiting sequences are
+// admitted in arrival order while their arrival step has come, a slot is
+// free and free pages - owed pages >= the sequence's target.
+__global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) {
+    __shared__ u64 s_ctl[kCtlWords];
+    __shared__ long long s_free;
+    __shared__ uint32_t s_arr[kAdmitWindow];
+    __shared__ uint32_t s_tgt[kAdmitWindow];
+    __shared__ uint32_t s_m;
+    __shared__ long long s_add;
+    const uint32_t t = threadIdx.x;
+    if (t < kCtlWords) s_ctl[t] = ld_ctl(&d.ctl[t]);
+    if (t == 0) s_free = __hip_atomic_load(d.c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const u64 step = s_ctl[kStep] + 1u;
+    const u64 head = s_ctl[kHead];
+    const u64 tail = s_ctl[kTail];
+    const u64 done_n = s_ctl[kDone];
+    const u64 top = s_ctl[kTop] + done_n;
+    for (u64 i = t; i < done_n; i += blockDim.x) d.stack[s_ctl[kTop] + i] = d.done[2u * i];
+    const u64 active = s_ctl[kActive] - done_n;
+    const long long owed = (long long)s_ctl[kReserved] - (long long)s_ctl[kActive];
+    const uint32_t win = (uint32_t)(tail - head < kAdmitWindow ? tail - head : kAdmitWindow);
+    for (uint32_t i = t; i < win; i += blockDim.x) {
+        s_arr[i] = d.arr_step[head + i];
+        s_tgt[i] = d.target[head + i];
+    }
+    __syncthreads();
+    if (t == 0) {
+        long long budget = s_free - owed;
+        const u64 slots = (u64)d.max_slots - active;
+        uint32_t m = 0;
+        long long add = 0;
+        while (m < win && (u64)s_arr[m] <= step && m < slots && budget >= (long long)s_tgt[m]) {
+            budget -= s_tgt[m];
+            add += s_tgt[m];
+            ++m;
+        }
+        s_m = m;
+        s_add = add;
+    }
+    __syncthreads();
+    const uint32_t m = s_m;
+    for (uint32_t i = t; i < m; i += blockDim.x) {
+        const uint32_t slot = d.stack[top - 1u - i];
+        const uint32_t seq = (uint32_t)(head + i);
+        d.slot_seq[slot] = seq;
+        d.slot_len[slot] = 0;
+        d.slot_target[slot] = s_tgt[i];
+        d.res_admit[seq] = (uint32_t)step;
+        d.res_sum[seq] = 0;
+    }
+    if (t == 0) {
+        d.ctl[kStep] = step;
+        d.ctl[kHead] = head + m;
+        d.ctl[kActive] = active + m;
+        d.ctl[kReserved] = (u64)(owed + s_add);
+        d.ctl[kTop] = top - m;
+        d.ctl[kDone] = 0;
+        d.ctl[kAdmitted] = s_ctl[kAdmitted] + m;
+    }
+}
+
+// The strategy's device getPage for the calling lane of slot `slot`.
+template <int S>
+__device__ __forceinline__ int32_t serve_get(const pbm_ctx& c, uint32_t slot, int32_t seed,
+                                             pbm::Stats& st) {
+    if (S == PBM_RW_BM) return pbm::get_page_rwbm(c, seed, nullptr, &st);
+    if (S == PBM_COLLAB_RW_BM) return pbm::get_page_collab(c, seed, nullptr, &st);
+    if (S == PBM_CRW_BM) return pbm::get_page_crw(c, slot, seed, nullptr, &st);
+    if (S == PBM_RW_MALLOC) return pbm::get_page_rwmalloc(c, seed, nullptr, &st);
+    if (S == PBM_SINGLE_CLOCK) return pbm::get_page_single_clock(c, nullptr, &st);
+    if (S == PBM_DIST_CLOCK) return pbm::get_page_dist_clock(c, slot, nullptr, &st);
+    return pbm::get_page_list(c, nullptr, &st);
+}
+
+// Grow: every admitted slot gets one page.  A sequence that reaches its
+// target is recorded as finished and its slot listed for release.
+template <int S>
+__global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
+    using namespace pbm;
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const u64 step = ld_ctl(&d.ctl[kStep]);
+    Stats st;
+    uint32_t seq = kEmpty;
+    if (slot < d.max_slots) seq = d.slot_seq[slot];
+    const bool act = seq != kEmpty;
+    bool fin = false, got = false;
+    uint32_t ticks = 0;
+    if (act) {
+        const uint32_t len = d.slot_len[slot];
+        const uint32_t tgt = d.slot_target[slot];
+        const int32_t seed = request_seed(d.c.seed, (u64)seq * d.max_pages + len);
+        const u64 t0 = wall_clock64();
+        const int32_t id = serve_get<S>(d.c, slot, seed, st);
+        const u64 t1 = wall_clock64();
+        ticks = t1 - t0 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(t1 - t0);
+        if (id >= 0) {
+            got = true;
+            d.pages[(u64)slot * d.max_pages + len] = id;
+            if ((d.flags & PBM_SERVE_PAYLOAD) && d.c.pool)
+                *(u64*)page_address(d.c, id) = serve_payload(d.seed, d.key[seq], len);
+            d.slot_len[slot] = len + 1u;
+            if (len + 1u == tgt) {
+                fin = true;
+                d.res_finish[seq] = (uint32_t)step;
+                d.res_len[seq] = len + 1u;
+                d.slot_seq[slot] = kEmpty;
+            }
+        }
+    }
+    const unsigned lane = lane_id();
+    const u64 F = ballot(fin);
+    if (F) {
+        const unsigned ld = (unsigned)__ffsll((long long)F) - 1u;
+        u64 base = 0;
+        if (lane == ld) {
+            base = atom_add_u64(&d.ctl[kDone], (u64)__popcll(F));
+            atom_add_u64(&d.ctl[kFinished], (u64)__popcll(F));
+        }
+        base = readlane64(base, ld);
+        if (fin) {
+            const u64 k = base + rank_in(F);
+            d.done[2u * k] = slot;
+            d.done[2u * k + 1u] = seq;
+        }
+    }
+    const u64 G = ballot(got);
+    const u64 X = ballot(act && !got);
+    if (G) {
+        const unsigned ld = (unsigned)__ffsll((long long)G) - 1u;
+        u64 base = 0;
+        if (lane == ld) {
+            atom_add_u64(&d.ctl[kAllocs], (u64)__popcll(G));
+            if (d.flags & PBM_SERVE_LATENCY) base = atom_add_u64(&d.ctl[kLatCount], (u64)__popcll(G));
+        }
+        if (d.flags & PBM_SERVE_LATENCY) {
+            base = readlane64(base, ld);
+            const u64 k = base + rank_in(G);
+            if (got && k < d.lat_cap) d.lat[k] = ticks;
+        }
+    }
+    if (X && lane == (unsigned)__ffsll((long long)X) - 1u) {
+        // the page is still owed: the next admission must not count it
+        atom_add_u64(&d.ctl[kAllocFail], (u64)__popcll(X));
+        atom_add_u64(&d.ctl[kReserved], (u64)__popcll(X));
+    }
+    flush_stats(d.c, st);
+}
+
+// Release: the page lists of the sequences that finished at this step's
+// grow, cut into 64-page chunks dealt round-robin to the waves.  A chunk's
+// payloads are read back, checked and summed into the sequence's checksum,
+// then its pages go back to the pool.
+template <bool LIST>
+__global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
+    using namespace pbm;
+    const unsigned lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t done_n = (uint32_t)ld_ctl(&d.ctl[kDone]);
+    const bool payload = (d.flags & PBM_SERVE_PAYLOAD) && d.c.pool;
+    Stats st;
+    u64 bad = 0;
+    for (uint32_t g0 = 0; g0 < done_n; g0 += 64u) {
+        const uint32_t e = g0 + lane;
+        uint32_t slot = 0, seq = 0, len = 0;
+        if (e < done_n) {
+            slot = d.done[2u * e];
+            seq = d.done[2u * e + 1u];
+            len = d.slot_len[slot];
+        }
+        const uint32_t chunks = (len + 63u) / 64u;
+        uint32_t incl = chunks;
+#pragma unroll
+        for (unsigned s = 1; s < 64u; s <<= 1) {
+            const uint32_t v = shfl32(incl, lane >= s ? lane - s : lane);
+            if (lane >= s) incl += v;
+        }
+        const uint32_t total = readlane(incl, 63u);
+        for (uint32_t ci = wave; ci < total; ci += n_waves) {
+            const unsigned src = (unsigned)__ffsll((long long)ballot(incl > ci)) - 1u;
+            const uint32_t c_slot = readlane(slot, src);
+            const uint32_t c_seq = readlane(seq, src);
+            const uint32_t c_key = d.key[c_seq];
+            const uint32_t c_len = readlane(len, src);
+            const uint32_t c_first = readlane(incl - chunks, src);
+            const uint32_t j = (ci - c_first) * 64u + lane;
+            const bool valid = j < c_len;
+            int32_t id = -1;
+            u64 sum = 0;
+            if (valid) {
+                id = d.pages[(u64)c_slot * d.max_pages + j];
+                if (payload) {
+                    const u64 v = *(const u64*)page_address(d.c, id);
+                    sum = v;
+                    bad += v != serve_payload(d.seed, c_key, j);
+                }
+            }
+            if (payload) {
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) sum += shfl64(sum, lane ^ (unsigned)o);
+                if (lane == 0) atom_add_u64(&d.res_sum[c_seq], sum);
+            }
+            if (valid) {
+                if (LIST) free_page_list(d.c, id, nullptr, &st);
+                else free_page(d.c, id, &st);
+            }
+        }
+    }
+    uint32_t fr = st.frees;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        fr += (uint32_t)__shfl_xor((int)fr, o, 64);
+        bad += shfl64(bad, lane ^ (unsigned)o);
+    }
+    if (lane == 0 && fr) atom_add_u64(&d.ctl[kFrees], (u64)fr);
+    if (lane == 0 && bad) atom_add_u64(&d.ctl[kPayloadErr], bad);
+    flush_stats(d.c, st);
+}
+
+__global__ void k_serve_init(ServeDev d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.max_slots) {
+        d.slot_seq[i] = kEmpty;
+        d.slot_len[i] = 0;
+        d.stack[i] = d.max_slots - 1u - i;  // slot 0 is popped first
+    }
+    if (i < kCtlWords) d.ctl[i] = i == kTop ? (u64)d.max_slots : 0ull;
+}
+
+__global__ void k_serve_set_tail(u64* ctl, u64 tail) { ctl[kTail] = tail; }
+
+}  // namespace
+
+struct pbm_server {
+    pbm_handle* h = nullptr;
+    pbm_serve_config cfg{};
+    ServeDev d{};
+    uint32_t submitted = 0;
+    uint32_t last_arrival = 0;
+    uint32_t release_blocks = 1;
+    std::vector<void*> mem;
+};
+
+extern "C" int pbm_serve_destroy(pbm_server* s) {
+    if (!s) return PBM_OK;
+    if (s->h) (void)pbmi::use_device(s->h);
+    (void)hipDeviceSynchronize();
+    for (void* p : s->mem)
+        if (p) (void)hipFree(p);
+    delete s;
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_server** out) {
+    PBM_CHECK_HANDLE(h);
+    if (!cfg || !out) return pbmi::fail(PBM_ERR_INVALID, "null serve config or output");
+    *out = nullptr;
+    if (cfg->max_slots == 0 || cfg->max_slots > (1u << 22))
+        return pbmi::fail(PBM_ERR_RANGE, "max_slots must be in [1, 2^22]");
+    if (cfg->max_pages == 0 || cfg->max_seqs == 0)
+        return pbmi::fail(PBM_ERR_INVALID, "max_pages and max_seqs must be > 0");
+    if (cfg->flags & ~(PBM_SERVE_PAYLOAD | PBM_SERVE_LATENCY))
+        return pbmi::fail(PBM_ERR_INVALID, "unknown serve flags");
+    if ((cfg->flags & PBM_SERVE_PAYLOAD) && (!h->d_pool || h->cfg.page_size < 8))
+        return pbmi::fail(PBM_ERR_INVALID, "PBM_SERVE_PAYLOAD needs a backed pool");
+    pbm_server* s = new pbm_server();
+    s->h = h;
+    s->cfg = *cfg;
+    ServeDev& d = s->d;
+    d.c = h->ctx;
+    d.max_slots = cfg->max_slots;
+    d.max_pages = cfg->max_pages;
+    d.max_seqs = cfg->max_seqs;
+    d.flags = cfg->flags;
+    d.seed = cfg->seed;
+    d.lat_cap = (cfg->flags & PBM_SERVE_LATENCY) ? cfg->lat_capacity : 0u;
+    auto alloc = [&](void** p, size_t bytes) -> int {
+        hipError_t e = hipMalloc(p, bytes ? bytes : 8u);
+        if (e != hipSuccess) return pbmi::hip_fail(e, "hipMalloc(serve)", __FILE__, __LINE__);
+        s->mem.push_back(*p);
+        return PBM_OK;
+    };
+    const size_t ns = cfg->max_seqs, nl = cfg->max_slots;
+    int rc = PBM_OK;
+    void* p = nullptr;
+#define PBM_SALLOC(field, type, count)                          \
+    if (!rc) {                                                  \
+        rc = alloc(&p, (size_t)(count) * sizeof(type));         \
+        if (!rc) d.field = (type*)p;                            \
+    }
+    PBM_SALLOC(arr_step, uint32_t, ns)
+    PBM_SALLOC(target, uint32_t, ns)
+    PBM_SALLOC(key, uint32_t, ns)
+    PBM_SALLOC(res_admit, uint32_t, ns)
+    PBM_SALLOC(res_finish, uint32_t, ns)
+    PBM_SALLOC(res_len, uint32_t, ns)
+    PBM_SALLOC(res_sum, u64, ns)
+    PBM_SALLOC(slot_seq, uint32_t, nl)
+    PBM_SALLOC(slot_len, uint32_t, nl)
+    PBM_SALLOC(slot_target, uint32_t, nl)
+    PBM_SALLOC(pages, int32_t, nl * cfg->max_pages)
+    PBM_SALLOC(stack, uint32_t, nl)
+    PBM_SALLOC(done, uint32_t, 2u * nl)
+    PBM_SALLOC(ctl, u64, kCtlWords)
+    PBM_SALLOC(lat, uint32_t, d.lat_cap)
+#undef PBM_SALLOC
+    if (rc) {
+        pbm_serve_destroy(s);
+        return rc;
+    }
+    hipError_t e;
+    if ((e = hipMemset((void*)d.res_admit, 0, ns * 4u)) != hipSuccess ||
+        (e = hipMemset((void*)d.res_finish, 0, ns * 4u)) != hipSuccess ||
+        (e = hipMemset((void*)d.res_len, 0, ns * 4u)) != hipSuccess ||
+        (e = hipMemset((void*)d.res_sum, 0, ns * 8u)) != hipSuccess) {
+        pbm_serve_destroy(s);
+        return pbmi::hip_fail(e, "hipMemset(serve)", __FILE__, __LINE__);
+    }
+    const uint32_t n_init = cfg->max_slots > kCtlWords ? cfg->max_slots : kCtlWords;
+    hipLaunchKernelGGL(k_serve_init, dim3((n_init + 255u) / 256u), dim3(256), 0, nullptr, d);
+    if ((e = hipDeviceSynchronize()) != hipSuccess) {
+        pbm_serve_destroy(s);
+        return pbmi::hip_fail(e, "serve init", __FILE__, __LINE__);
+    }
+    s->release_blocks = (uint32_t)h->n_cu;  // four waves per CU
+    *out = s;
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_submit(pbm_server* s, uint32_t n, const uint32_t* arrival_step,
+                                const uint32_t* target, const uint32_t* key, void* stream) {
+    if (!s) return pbmi::fail(PBM_ERR_INVALID, "null server");
+    PBM_CHECK_HANDLE(s->h);
+    if (n == 0) return PBM_OK;
+    if (!arrival_step || !target) return pbmi::fail(PBM_ERR_INVALID, "null submit arrays");
+    if ((uint64_t)s->submitted + n > s->cfg.max_seqs)
+        return pbmi::fail(PBM_ERR_RANGE, "more sequences than max_seqs");
+    uint32_t last = s->last_arrival;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (arrival_step[i] < last) return pbmi::fail(PBM_ERR_INVALID, "arrival steps must not decrease");
+        last = arrival_step[i];
+        if (target[i] == 0 || target[i] > s->cfg.max_pages)
+            return pbmi::fail(PBM_ERR_RANGE, "target length must be in [1, max_pages]");
+    }
+    hipStream_t st = pbmi::as_stream(stream);
+    PBM_HIP(hipMemcpyAsync((void*)(s->d.arr_step + s->submitted), arrival_step, (size_t)n * 4u,
+                           hipMemcpyHostToDevice, st));
+    PBM_HIP(hipMemcpyAsync((void*)(s->d.target + s->submitted), target, (size_t)n * 4u,
+                           hipMemcpyHostToDevice, st));
+    std::vector<uint32_t> ids;
+    if (!key) {
+        ids.resize(n);
+        for (uint32_t i = 0; i < n; ++i) ids[i] = s->submitted + i;
+        key = ids.data();
+    }
+    PBM_HIP(hipMemcpyAsync((void*)(s->d.key + s->submitted), key, (size_t)n * 4u,
+                           hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_serve_set_tail, dim3(1), dim3(1), 0, st, s->d.ctl,
+                       (u64)s->submitted + n);
+    PBM_HIP(hipGetLastError());
+    PBM_HIP(hipStreamSynchronize(st));
+    s->submitted += n;
+    s->last_arrival = last;
+    return PBM_OK;
+}
+
+namespace {
+
+int launch_step(pbm_server* s, hipStream_t st) {
+    const ServeDev& d = s->d;
+    hipLaunchKernelGGL(k_serve_admit, dim3(1), dim3(1024), 0, st, d);
+    const dim3 g((d.max_slots + 255u) / 256u), b(256);
+    switch (s->h->cfg.strategy) {
+        case PBM_RW_BM: hipLaunchKernelGGL(k_serve_grow<PBM_RW_BM>, g, b, 0, st, d); break;
+        case PBM_COLLAB_RW_BM: hipLaunchKernelGGL(k_serve_grow<PBM_COLLAB_RW_BM>, g, b, 0, st, d); break;
+        case PBM_CRW_BM: hipLaunchKernelGGL(k_serve_grow<PBM_CRW_BM>, g, b, 0, st, d); break;
+        case PBM_RW_MALLOC: hipLaunchKernelGGL(k_serve_grow<PBM_RW_MALLOC>, g, b, 0, st, d); break;
+        case PBM_SINGLE_CLOCK: hipLaunchKernelGGL(k_serve_grow<PBM_SINGLE_CLOCK>, g, b, 0, st, d); break;
+        case PBM_DIST_CLOCK: hipLaunchKernelGGL(k_serve_grow<PBM_DIST_CLOCK>, g, b, 0, st, d); break;
+        default: hipLaunchKernelGGL(k_serve_grow<PBM_LINKED_LIST>, g, b, 0, st, d); break;
+    }
+    if (s->h->cfg.strategy == PBM_LINKED_LIST)
+        hipLaunchKernelGGL(k_serve_release<true>, dim3(s->release_blocks), dim3(256), 0, st, d);
+    else
+        hipLaunchKernelGGL(k_serve_release<false>, dim3(s->release_blocks), dim3(256), 0, st, d);
+    PBM_HIP(hipGetLastError());
+    return PBM_OK;
+}
+
+}  // namespace
+
+extern "C" int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream) {
+    if (!s) return pbmi::fail(PBM_ERR_INVALID, "null server");
+    PBM_CHECK_HANDLE(s->h);
+    hipStream_t st = pbmi::as_stream(stream);
+    for (uint32_t k = 0; k < n_steps; ++k) {
+        int rc = launch_step(s, st);
+        if (rc) return rc;
+    }
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_graph(pbm_server* s, uint32_t n_steps, pbm_graph** out) {
+    if (!s || !out) return pbmi::fail(PBM_ERR_INVALID, "null server or output graph");
+    PBM_CHECK_HANDLE(s->h);
+    *out = nullptr;
+    if (n_steps == 0) return pbmi::fail(PBM_ERR_INVALID, "a graph needs n_steps > 0");
+    hipStream_t st = nullptr;
+    PBM_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    pbm_graph* g = new pbm_graph();
+    g->device = s->h->device;
+    int rc = PBM_OK;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        rc = pbmi::hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
+    } else {
+        for (uint32_t k = 0; k < n_steps && !rc; ++k) rc = launch_step(s, st);
+        hipGraph_t graph = nullptr;
+        e = hipStreamEndCapture(st, &graph);
+        g->graph = graph;
+        if (!rc && e != hipSuccess) rc = pbmi::hip_fail(e, "hipStreamEndCapture", __FILE__, __LINE__);
+        if (!rc) {
+            e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) rc = pbmi::hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+        }
+    }
+    (void)hipStreamDestroy(st);
+    if (rc) {
+        pbm_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_read_stats(pbm_server* s, pbm_serve_stats* out, void* stream) {
+    if (!s || !out) return pbmi::fail(PBM_ERR_INVALID, "null server or output");
+    PBM_CHECK_HANDLE(s->h);
+    u64 ctl[kCtlWords];
+    hipStream_t st = pbmi::as_stream(stream);
+    PBM_HIP(hipMemcpyAsync(ctl, s->d.ctl, sizeof(ctl), hipMemcpyDeviceToHost, st));
+    PBM_HIP(hipStreamSynchronize(st));
+    out->step = ctl[kStep];
+    out->submitted = ctl[kTail];
+    out->admitted = ctl[kAdmitted];
+    out->finished = ctl[kFinished];
+    out->active = ctl[kActive] - ctl[kDone];
+    out->reserved_pages = ctl[kReserved] - ctl[kActive];
+    out->allocs = ctl[kAllocs];
+    out->frees = ctl[kFrees];
+    out->alloc_failures = ctl[kAllocFail];
+    out->payload_errors = ctl[kPayloadErr];
+    out->lat_samples = ctl[kLatCount];
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_read_results(pbm_server* s, uint32_t first, uint32_t n,
+                                      uint32_t* admit_step, uint32_t* finish_step,
+                                      uint32_t* length, unsigned long long* checksum,
+                                      void* stream) {
+    if (!s) return pbmi::fail(PBM_ERR_INVALID, "null server");
+    PBM_CHECK_HANDLE(s->h);
+    if ((uint64_t)first + n > s->cfg.max_seqs)
+        return pbmi::fail(PBM_ERR_RANGE, "result range beyond max_seqs");
+    hipStream_t st = pbmi::as_stream(stream);
+    if (admit_step)
+        PBM_HIP(hipMemcpyAsync(admit_step, s->d.res_admit + first, (size_t)n * 4u,
+                               hipMemcpyDeviceToHost, st));
+    if (finish_step)
+        PBM_HIP(hipMemcpyAsync(finish_step, s->d.res_finish + first, (size_t)n * 4u,
+                               hipMemcpyDeviceToHost, st));
+    if (length)
+        PBM_HIP(hipMemcpyAsync(length, s->d.res_len + first, (size_t)n * 4u,
+                               hipMemcpyDeviceToHost, st));
+    if (checksum)
+        PBM_HIP(hipMemcpyAsync(checksum, s->d.res_sum + first, (size_t)n * 8u,
+                               hipMemcpyDeviceToHost, st));
+    PBM_HIP(hipStreamSynchronize(st));
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_read_latency(pbm_server* s, uint32_t* ticks, uint32_t n, uint32_t* got,
+                                      void* stream) {
+    if (!s || !got) return pbmi::fail(PBM_ERR_INVALID, "null server or output");
+    PBM_CHECK_HANDLE(s->h);
+    *got = 0;
+    u64 cnt = 0;
+    hipStream_t st = pbmi::as_stream(stream);
+    PBM_HIP(hipMemcpyAsync(&cnt, s->d.ctl + kLatCount, 8u, hipMemcpyDeviceToHost, st));
+    PBM_HIP(hipStreamSynchronize(st));
+    u64 k = cnt < s->d.lat_cap ? cnt : s->d.lat_cap;
+    if (k > n) k = n;
+    if (k && !ticks) return pbmi::fail(PBM_ERR_INVALID, "null latency buffer");
+    if (k) {
+        PBM_HIP(hipMemcpyAsync(ticks, s->d.lat, (size_t)k * 4u, hipMemcpyDeviceToHost, st));
+        PBM_HIP(hipStreamSynchronize(st));
+    }
+    *got = (uint32_t)k;
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_clock_hz(pbm_server* s, unsigned long long* hz) {
+    if (!s || !hz) return pbmi::fail(PBM_ERR_INVALID, "null server or output");
+    PBM_CHECK_HANDLE(s->h);
+    int khz = 0;
+    PBM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->h->device));
+    *hz = (unsigned long long)khz * 1000ull;
+    return PBM_OK;
+}

@@ -75,6 +75,24 @@ class TuneResult(C.Structure):
 TUNE_QUICK, TUNE_WALK_ONLY, TUNE_RANK_ONLY = 1, 2, 4
 
 
+class ServeConfig(C.Structure):
+    _fields_ = [("max_slots", C.c_uint32), ("max_pages", C.c_uint32), ("max_seqs", C.c_uint32),
+                ("flags", C.c_uint32), ("lat_capacity", C.c_uint32), ("reserved", C.c_uint32),
+                ("seed", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("step", "submitted", "admitted", "finished", "active",
+                                          "reserved_pages", "allocs", "frees", "alloc_failures",
+                                          "payload_errors", "lat_samples")]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+SERVE_PAYLOAD, SERVE_LATENCY = 1, 2
+
+
 class PbmError(RuntimeError):
     def __init__(self, code: int, where: str, msg: str):
         super().__init__(f"{where} -> {ERRORS.get(code, code)}: {msg}")
@@ -120,6 +138,18 @@ _SIGS = {
     "pbm_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
                                C.POINTER(TuneResult), C.c_void_p]),
     "pbm_atomic_peak": (C.c_int, [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_double)]),
+    "pbm_serve_create": (C.c_int, [C.c_void_p, C.POINTER(ServeConfig), C.POINTER(C.c_void_p)]),
+    "pbm_serve_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_serve_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]),
+    "pbm_serve_run": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pbm_serve_graph": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pbm_serve_read_stats": (C.c_int, [C.c_void_p, C.POINTER(ServeStats), C.c_void_p]),
+    "pbm_serve_read_results": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbm_serve_read_latency": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32,
+                                         C.POINTER(C.c_uint32), C.c_void_p]),
+    "pbm_serve_clock_hz": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     "pbm_last_error": (C.c_char_p, []),
     "pbm_version": (C.c_char_p, []),
 }
@@ -324,6 +354,107 @@ class Graph:
             self.close()
         except Exception:
             pass
+
+
+class _StepGraph(Graph):
+    """A graph captured by another entry point (pbm_serve_graph)."""
+
+    def __init__(self, g: C.c_void_p, owner):
+        self.g = g
+        self.pool = owner
+
+
+class Server:
+    """Continuous-batching serving loop over a pool (pbm_serve_*)."""
+
+    def __init__(self, pool: Pool, max_slots: int, max_pages: int, max_seqs: int,
+                 flags: int = 0, lat_capacity: int = 0, seed: int = 0):
+        cfg = ServeConfig(max_slots, max_pages, max_seqs, flags, lat_capacity, 0, seed)
+        s = C.c_void_p()
+        check(lib().pbm_serve_create(pool.h, C.byref(cfg), C.byref(s)), "pbm_serve_create")
+        self.s = s
+        self.pool = pool
+        self.cfg = cfg
+
+    def submit(self, arrival_step, target, key=None, stream=None) -> None:
+        a = np.ascontiguousarray(arrival_step, dtype=np.uint32)
+        t = np.ascontiguousarray(target, dtype=np.uint32)
+        k = None if key is None else np.ascontiguousarray(key, dtype=np.uint32)
+        if a.shape != t.shape or (k is not None and k.shape != a.shape):
+            raise ValueError("arrival_step, target and key differ in length")
+        check(lib().pbm_serve_submit(self.s, a.size, a.ctypes.data_as(C.c_void_p),
+                                     t.ctypes.data_as(C.c_void_p),
+                                     None if k is None else k.ctypes.data_as(C.c_void_p),
+                                     _stream(stream)), "pbm_serve_submit")
+
+    def run(self, n_steps: int, stream=None) -> None:
+        check(lib().pbm_serve_run(self.s, n_steps, _stream(stream)), "pbm_serve_run")
+
+    def graph(self, n_steps: int) -> Graph:
+        g = C.c_void_p()
+        check(lib().pbm_serve_graph(self.s, n_steps, C.byref(g)), "pbm_serve_graph")
+        return _StepGraph(g, self)
+
+    def stats(self, stream=None) -> dict:
+        st = ServeStats()
+        check(lib().pbm_serve_read_stats(self.s, C.byref(st), _stream(stream)),
+              "pbm_serve_read_stats")
+        return st.as_dict()
+
+    def results(self, first: int = 0, n: int | None = None, stream=None) -> dict:
+        if n is None:
+            n = self.cfg.max_seqs - first
+        out = {k: np.zeros(n, dtype=np.uint32) for k in ("admit_step", "finish_step", "length")}
+        out["checksum"] = np.zeros(n, dtype=np.uint64)
+        check(lib().pbm_serve_read_results(
+            self.s, first, n, *(out[k].ctypes.data_as(C.c_void_p)
+                                for k in ("admit_step", "finish_step", "length", "checksum")),
+            _stream(stream)), "pbm_serve_read_results")
+        return out
+
+    def latency_ticks(self, n: int, stream=None) -> np.ndarray:
+        buf = np.zeros(max(n, 1), dtype=np.uint32)
+        got = C.c_uint32()
+        check(lib().pbm_serve_read_latency(self.s, buf.ctypes.data_as(C.c_void_p), n,
+                                           C.byref(got), _stream(stream)),
+              "pbm_serve_read_latency")
+        return buf[:got.value]
+
+    def clock_hz(self) -> int:
+        hz = C.c_uint64()
+        check(lib().pbm_serve_clock_hz(self.s, C.byref(hz)), "pbm_serve_clock_hz")
+        return int(hz.value)
+
+    def close(self) -> None:
+        if self.s:
+            check(lib().pbm_serve_destroy(self.s), "pbm_serve_destroy")
+            self.s = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_trace(n: int, rate_per_s: float, step_s: float, median: float, sigma: float,
+                seed: int, max_pages: int):
+    """Poisson arrivals at rate_per_s (exponential gaps from a seeded numpy
+    generator), binned into steps of step_s seconds (a sequence arriving in
+    [ (k-1) step_s, k step_s ) may be admitted from step k), and lognormal
+    target lengths (median, sigma) rounded and clipped to [1, max_pages]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate_per_s, size=n))
+    arrival = (np.floor(t / step_s) + 1).astype(np.uint32)
+    target = np.rint(np.exp(np.log(median) + sigma * rng.standard_normal(n)))
+    target = np.clip(target, 1, max_pages).astype(np.uint32)
+    return arrival, target
 
 
 def atomic_peak(words: int = 16384, rounds: int = 64, device: int = 0) -> float:

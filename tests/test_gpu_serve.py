@@ -1,0 +1,166 @@
+"""The serving loop through the C-ABI: its admission/finish schedule equals
+the serial oracle's, every sequence ends with its target length and the
+checksum of its own payloads (so no page was shared), each result is the
+same alone and in the mix, graph-captured steps equal eager ones, and the
+pool comes back to its starting state."""
+import numpy as np
+import pytest
+
+import pbm
+import pbm_oracle as orc
+from conftest import log_case
+
+pytestmark = pytest.mark.gpu
+
+SEED = 99
+
+
+def _trace(n, rate, median, sigma, max_pages, seed=7):
+    # arrivals per step = rate * step_s with step_s = 1 ms
+    return pbm.serve_trace(n, rate, 1e-3, median, sigma, seed, max_pages)
+
+
+def _run_to_end(server, n_steps, chunk=64):
+    done = 0
+    while done < n_steps:
+        k = min(chunk, n_steps - done)
+        server.run(k)
+        done += k
+
+
+def _check_results(res, keys, target, payload=True):
+    fin = res["finish_step"] > 0
+    assert fin.all()
+    assert np.array_equal(res["length"], target)
+    if payload:
+        exp = np.array([orc.serve_checksum(SEED, int(k), int(t)) for k, t in zip(keys, target)],
+                       dtype=np.uint64)
+        assert np.array_equal(res["checksum"], exp)
+
+
+@pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.COLLAB_RW_BM, pbm.CRW_BM, pbm.RW_MALLOC,
+                                      pbm.SINGLE_CLOCK, pbm.DIST_CLOCK, pbm.LINKED_LIST])
+def test_serve_schedule_matches_oracle(gpu, strategy):
+    T, n = 1 << 16, 2500
+    log_case(f"serve schedule strategy={strategy}")
+    arr, tgt = _trace(n, 8000, 60, 0.8, 2048)
+    with pbm.Pool(T, 64, strategy=strategy, backing=True, seed=5) as pool:
+        pool.prefill_bernoulli(0.5, 3)
+        A = pool.free_count()
+        adm, fin, tot = orc.serve_sim(A, 512, arr, tgt, 100000)
+        assert (fin > 0).all()
+        steps = int(fin.max()) + 2
+        with pbm.Server(pool, 512, 2048, n, pbm.SERVE_PAYLOAD, seed=SEED) as srv:
+            srv.submit(arr, tgt)
+            _run_to_end(srv, steps)
+            res = srv.results()
+            st = srv.stats()
+        assert np.array_equal(res["admit_step"], adm)
+        assert np.array_equal(res["finish_step"], fin)
+        _check_results(res, np.arange(n), tgt)
+        assert st["alloc_failures"] == 0 and st["payload_errors"] == 0
+        assert st["allocs"] == tot[0] and st["frees"] == tot[1]
+        assert st["admitted"] == st["finished"] == n and st["active"] == 0
+        assert pool.free_count() == A
+        assert np.array_equal(pool.snapshot(), orc.prefill(T, 0.5, 3))
+
+
+def test_serve_alone_equals_in_mix(gpu):
+    T, n = 1 << 16, 3000
+    log_case("serve alone vs in mix")
+    arr, tgt = _trace(n, 10000, 80, 0.8, 4096, seed=11)
+    keys = (np.arange(n) * 7919 + 13).astype(np.uint32)
+    with pbm.Pool(T, 64, strategy=pbm.RW_BM, backing=True, seed=8) as pool:
+        pool.prefill_bernoulli(0.3, 4)
+        with pbm.Server(pool, 1024, 4096, n, pbm.SERVE_PAYLOAD, seed=SEED) as srv:
+            srv.submit(arr, tgt, keys)
+            _, fin, _ = orc.serve_sim(pool.free_count(), 1024, arr, tgt, 100000)
+            _run_to_end(srv, int(fin.max()) + 2)
+            mix = srv.results()
+            assert srv.stats()["payload_errors"] == 0
+        _check_results(mix, keys, tgt)
+        pick = np.random.default_rng(0).choice(n, size=6, replace=False)
+        for q in pick:
+            with pbm.Server(pool, 4, 4096, 1, pbm.SERVE_PAYLOAD, seed=SEED) as one:
+                one.submit([1], [tgt[q]], [keys[q]])
+                one.run(int(tgt[q]) + 1)
+                alone = one.results()
+            assert alone["length"][0] == mix["length"][q]
+            assert alone["checksum"][0] == mix["checksum"][q]
+
+
+def test_serve_graph_steps_equal_eager_steps(gpu):
+    T, n = 1 << 15, 1500
+    log_case("serve graph vs eager")
+    arr, tgt = _trace(n, 6000, 40, 0.8, 1024, seed=3)
+    out = []
+    for use_graph in (False, True):
+        with pbm.Pool(T, 64, strategy=pbm.CRW_BM, backing=True, seed=2) as pool:
+            pool.prefill_bernoulli(0.4, 6)
+            with pbm.Server(pool, 256, 1024, n, pbm.SERVE_PAYLOAD, seed=SEED) as srv:
+                srv.submit(arr, tgt)
+                if use_graph:
+                    g = srv.graph(50)
+                    for _ in range(40):          # 2000 steps, eager work between replays
+                        g.launch()
+                        gpu.cuda.synchronize()
+                        srv.stats()
+                    g.close()
+                else:
+                    _run_to_end(srv, 2000)
+                res = srv.results()
+                out.append((res, srv.stats()))
+    (a, sa), (b, sb) = out
+    for k in ("admit_step", "finish_step", "length", "checksum"):
+        assert np.array_equal(a[k], b[k]), k
+    assert sa == sb
+
+
+def test_serve_latency_samples_and_clock(gpu):
+    log_case("serve latency")
+    arr, tgt = _trace(400, 5000, 30, 0.5, 256, seed=1)
+    with pbm.Pool(1 << 14, strategy=pbm.RW_BM, seed=1) as pool:
+        with pbm.Server(pool, 128, 256, 400, pbm.SERVE_LATENCY, lat_capacity=1000) as srv:
+            srv.submit(arr, tgt)
+            srv.run(400)
+            st = srv.stats()
+            lat = srv.latency_ticks(5000)
+            hz = srv.clock_hz()
+    assert st["lat_samples"] == st["allocs"] > 1000
+    assert lat.size == 1000 and (lat > 0).all()
+    assert 1e6 <= hz <= 1e10
+
+
+def test_serve_c5_pool_metadata_only(gpu):
+    # 67,108,864 pages, no page memory: schedule and counts against the oracle
+    T, n = 1 << 26, 20000
+    log_case("serve C5-size pool")
+    arr, tgt = _trace(n, 20000, 512, 0.8, 16384, seed=42)
+    with pbm.Pool(T, 4096, strategy=pbm.RW_BM, seed=42) as pool:
+        A = pool.free_count()
+        adm, fin, tot = orc.serve_sim(A, 32768, arr, tgt, 1 << 20)
+        steps = int(fin.max()) + 2
+        with pbm.Server(pool, 32768, 16384, n, 0) as srv:
+            srv.submit(arr, tgt)
+            _run_to_end(srv, steps, chunk=256)
+            res = srv.results()
+            st = srv.stats()
+        assert np.array_equal(res["admit_step"], adm)
+        assert np.array_equal(res["finish_step"], fin)
+        assert st["allocs"] == tot[0] == st["frees"] and st["alloc_failures"] == 0
+        assert pool.free_count() == A and pool.count_used() == 0
+
+
+def test_serve_rejects_bad_submissions(gpu):
+    log_case("serve bad submissions")
+    with pbm.Pool(1024, strategy=pbm.RW_BM) as pool:
+        with pbm.Server(pool, 8, 16, 4) as srv:
+            with pytest.raises(pbm.PbmError):
+                srv.submit([1], [17])             # longer than max_pages
+            with pytest.raises(pbm.PbmError):
+                srv.submit([3, 2], [1, 1])        # arrivals decrease
+            srv.submit([1, 1, 2, 2], [1, 2, 3, 4])
+            with pytest.raises(pbm.PbmError):
+                srv.submit([5], [1])              # beyond max_seqs
+        with pytest.raises(pbm.PbmError):
+            pbm.Server(pool, 8, 16, 4, pbm.SERVE_PAYLOAD)   # payload needs backing

@@ -606,3 +606,57 @@ def test_batched_rank_takes_free_pages_in_map_order():
     assert st.free[0] == 0
     rc, g = orc.check_alloc(before, st.bm, T, ids)
     assert rc == 0 and g == len(free)
+
+
+# ------------------------------------------------------------------ serving loop schedule
+
+def test_serve_sim_hand_schedule_slots_bound():
+    # 10 free pages, 2 slots, targets 4, 4, 3 all arriving at step 1:
+    # two admitted at once; the third waits for a slot, which frees when
+    # both finish at step 4 (pages 1..4), and runs steps 5..7
+    adm, fin, tot = orc.serve_sim(10, 2, np.array([1, 1, 1]), np.array([4, 4, 3]), 20)
+    assert adm.tolist() == [1, 1, 5] and fin.tolist() == [4, 4, 7]
+    assert tot.tolist() == [11, 11, 3, 3]
+
+
+def test_serve_sim_hand_schedule_pages_bound():
+    # 5 free pages: the second sequence (3 pages) does not fit beside the
+    # first (4 owed) until the first returns its pages after step 4
+    adm, fin, tot = orc.serve_sim(5, 10, np.array([1, 1]), np.array([4, 3]), 20)
+    assert adm.tolist() == [1, 5] and fin.tolist() == [4, 7]
+
+
+def test_serve_sim_fifo_head_blocks_later_arrivals():
+    # the head (6 pages) waits for pages, and a later small sequence that
+    # would fit waits behind it
+    adm, fin, _ = orc.serve_sim(8, 10, np.array([1, 2, 3]), np.array([5, 6, 1]), 30)
+    assert adm.tolist() == [1, 6, 6]
+    assert fin.tolist() == [5, 11, 6]
+
+
+def test_serve_sim_unbounded_pool_admits_on_arrival():
+    rng = np.random.default_rng(5)
+    arr = np.sort(rng.integers(1, 200, size=500)).astype(np.uint32)
+    tgt = rng.integers(1, 50, size=500).astype(np.uint32)
+    adm, fin, tot = orc.serve_sim(1 << 30, 1 << 20, arr, tgt, 400)
+    assert np.array_equal(adm, arr)
+    assert np.array_equal(fin, arr + tgt - 1)
+    assert tot[0] == tgt.sum() == tot[1] and tot[2] == tot[3] == 500
+
+
+def test_serve_sim_window_admits_at_most_1024_per_step():
+    arr = np.ones(3000, dtype=np.uint32)
+    tgt = np.ones(3000, dtype=np.uint32)
+    adm, fin, _ = orc.serve_sim(1 << 20, 1 << 20, arr, tgt, 10)
+    assert np.bincount(adm)[1:4].tolist() == [1024, 1024, 952]
+
+
+def test_serve_checksum_is_the_sum_of_payloads():
+    seed, seq = 77, 12345
+    s = sum(orc.serve_payload(seed, seq, j) for j in range(40)) % (1 << 64)
+    assert orc.serve_checksum(seed, seq, 40) == s
+    # payload restated from splitmix64's output function
+    M = (1 << 64) - 1
+    z = ((seed ^ 0x452821E638D01377) + (((seq << 32) | 7) + 1) * 0x9E3779B97F4A7C15) & M
+    assert orc.serve_payload(seed, seq, 7) == orc.mix64(z)
+    assert len({orc.serve_payload(seed, seq, j) for j in range(1000)}) == 1000
