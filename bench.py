@@ -86,6 +86,51 @@ def time_phases(torch, pool, n, ids, gl, fl, steps: int, stream) -> dict:
             "free": float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))}
 
 
+def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
+    """Serving loop on 67,108,864 x 4 KiB pages (page map only): Poisson
+    arrivals at 20,000/s binned into 1 ms steps, lognormal lengths (median
+    512 pages, sigma 0.8, capped at 16,384), one page per sequence per step,
+    admission by free-page count.  Returns ops/s over the timed steps and
+    the per-allocation latency percentiles of the device getPage calls
+    made in them."""
+    T, n_steps = 1 << 26, warm_steps + timed_steps
+    n = int(20000 * n_steps * 1e-3 * 1.2) + 64
+    arr, tgt = pbm.serve_trace(n, 20000.0, 1e-3, 512.0, 0.8, 7, 16384)
+    pool = pbm.Pool(T, 4096, strategy=pbm.RW_BM, backing=False, seed=7)
+    srv = pbm.Server(pool, 32768, 16384, n, pbm.SERVE_LATENCY, lat_capacity=1 << 26)
+    srv.submit(arr, tgt, stream=stream)
+    srv.run(warm_steps, stream)
+    torch.cuda.synchronize()
+    s0 = srv.stats(stream)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    srv.run(timed_steps, stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1)
+    s1 = srv.stats(stream)
+    lat = srv.latency_ticks(1 << 26, stream)[s0["lat_samples"]:s1["lat_samples"]]
+    hz = srv.clock_hz()
+    ops = (s1["allocs"] - s0["allocs"]) + (s1["frees"] - s0["frees"])
+    out = {
+        "workload": "67,108,864 x 4 KiB pages (page map only), Poisson 20,000 seq/s in 1 ms "
+                    "steps, lognormal length median 512 sigma 0.8, +1 page per step, RW_BM "
+                    "device getPage per slot",
+        "warm_steps": warm_steps, "timed_steps": timed_steps, "ms_per_step": ms / timed_steps,
+        "ops_per_s": ops / (ms * 1e-3), "allocs": s1["allocs"] - s0["allocs"],
+        "frees": s1["frees"] - s0["frees"], "active_at_end": s1["active"],
+        "pages_in_use_at_end": pool.count_used(stream),
+        "p50_alloc_us": float(np.percentile(lat, 50)) * 1e6 / hz if lat.size else None,
+        "p99_alloc_us": float(np.percentile(lat, 99)) * 1e6 / hz if lat.size else None,
+        "latency_samples": int(lat.size), "clock_hz": hz,
+        "alloc_failures": s1["alloc_failures"],
+    }
+    srv.close()
+    pool.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawTextHelpFormatter)
@@ -97,6 +142,10 @@ def main() -> int:
     ap.add_argument("--no-tune", action="store_true",
                     help="skip the autotuner; use fixed rank and walk launches "
                          "(for counter-collection runs)")
+    ap.add_argument("--serve-steps", type=int, default=1200,
+                    help="timed steps of the serving loop (after as many warm-up steps "
+                         "as --serve-warm; 0 = skip)")
+    ap.add_argument("--serve-warm", type=int, default=800)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the line, copy a device buffer of this many MiB once "
                          "(the known-size reference kernel for HBM byte counters)")
@@ -231,6 +280,9 @@ def main() -> int:
     atom_peak = pbm.atomic_peak(L, 64)
     atom_rate = (ca["atomics"] / (b["phases_ms"]["alloc"] * 1e-3))
 
+    serving = (serving_leg(torch, pbm, stream, args.serve_warm, args.serve_steps)
+               if args.serve_steps > 0 else None)
+
     # ---- CPU baseline: reference policy, serial, same pre-fill and requests
     cpu = None
     if args.cpu_seconds > 0:
@@ -283,6 +335,7 @@ def main() -> int:
                                 "frac": atom_rate / atom_peak, "unit": "u64 RMW/s",
                                 "kernel": kernel_names["alloc"]}},
         "cpu_baseline": cpu,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:

@@ -140,7 +140,9 @@ typedef struct pbm_launch {
                                       of the call takes the free page of rank r (map order);
                                       each 256-word tile learns the free pages before it by
                                       decoupled look-back and claims its granted bits with
-                                      one atomicOr per word.  n < 2^30.                    */
+                                      one atomicOr per word.                               */
+#define PBM_LAUNCH_COOPERATIVE 16u /* mallocRW bursts: every burst of the range in one
+                                      cooperative persistent kernel (batched mode)        */
 
 typedef struct pbm_handle pbm_handle;
 
@@ -380,6 +382,66 @@ typedef struct pbm_tune_result {
  * counterpart (its sweeps are hard-coded, unitTests/unitTest1.cu:14-18). */
 int pbm_autotune(pbm_handle* h, uint32_t n, int32_t* d_ids, uint32_t flags,
                  pbm_tune_result* out, void* stream);
+
+/* ---- bursts of mallocRW / freeRW --------------------------------------
+ *
+ * A trace of bursts with uneven request counts and sizes.  Burst b frees
+ * free_index[free_offset[b] .. free_offset[b+1]) (requests of earlier
+ * bursts, by global request index) and then mallocRWs requests
+ * alloc_offset[b] .. alloc_offset[b+1] of sizes[].  Each phase is one
+ * kernel (plain, serial or a persistent batched grid), so a range of
+ * bursts can be captured as one HIP graph; with PBM_LAUNCH_COOPERATIVE a
+ * range runs in one cooperative persistent kernel whose workgroups meet at
+ * a grid barrier between phases.  Requests packed together by mallocRW are
+ * the small requests of one 64-request chunk of a burst; freeing the
+ * leader returns the whole run (RWMalloc.cuh:329-334), so a trace should
+ * free a packed group together.  No reference counterpart: the reference
+ * drivers launch one malloc_kernel / free_kernel per test
+ * (unitTests/unitTest3.cu:31-35, unitTest4.cu:37-43). */
+
+typedef struct pbm_burst_trace {
+    uint32_t n_bursts;
+    uint32_t reserved;
+    const uint32_t* alloc_offset;  /* n_bursts + 1 entries, from 0                   */
+    const uint32_t* free_offset;   /* n_bursts + 1 entries, from 0                   */
+    const uint64_t* sizes;         /* alloc_offset[n_bursts] request sizes in bytes  */
+    const uint32_t* free_index;    /* free_offset[n_bursts] request indices          */
+} pbm_burst_trace;
+
+typedef struct pbm_bursts pbm_bursts;
+
+/* Uploads a trace (host arrays) for pool h (backed, bitmap strategy). */
+int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* trace, pbm_bursts** out);
+int pbm_bursts_destroy(pbm_bursts* b);
+
+/* Runs bursts first .. first+count-1. */
+int pbm_bursts_run(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch* launch,
+                   void* stream);
+
+/* Captures bursts first .. first+count-1 as one HIP graph (not cooperative). */
+int pbm_bursts_graph(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch* launch,
+                     pbm_graph** out);
+
+/* Pointers mallocRW returned for requests first .. first+n-1 (0 = failed
+ * or not yet run). */
+int pbm_bursts_read_ptrs(pbm_bursts* b, uint32_t first, uint32_t n, uint64_t* host,
+                         void* stream);
+
+/* Forgets every returned pointer (the pool is not touched). */
+int pbm_bursts_reset(pbm_bursts* b, void* stream);
+
+/* Grid-barrier polls that gave up (stays 0). */
+int pbm_bursts_barrier_timeouts(pbm_bursts* b, unsigned long long* out, void* stream);
+
+/* Times the first `count` bursts from the pool's current state over the
+ * batched launch space (waves per CU 1..16, walk budget 4/16/64, phase
+ * kernels or one cooperative kernel) and returns the fastest in
+ * get_launch (ms_alloc = its time).  The pool's map and the pointer table
+ * are put back after every candidate and at the end, so the pool must
+ * hold only what the trace's first burst expects. */
+int pbm_bursts_autotune(pbm_bursts* b, uint32_t count, uint32_t flags, pbm_tune_result* out,
+                        void* stream);
+
 
 /* ---- measurement ------------------------------------------------------ */
 

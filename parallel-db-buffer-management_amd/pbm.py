@@ -91,6 +91,13 @@ class ServeStats(C.Structure):
 
 
 SERVE_PAYLOAD, SERVE_LATENCY = 1, 2
+LAUNCH_COOPERATIVE = 16
+
+
+class BurstTrace(C.Structure):
+    _fields_ = [("n_bursts", C.c_uint32), ("reserved", C.c_uint32),
+                ("alloc_offset", C.c_void_p), ("free_offset", C.c_void_p),
+                ("sizes", C.c_void_p), ("free_index", C.c_void_p)]
 
 
 class PbmError(RuntimeError):
@@ -150,6 +157,18 @@ _SIGS = {
     "pbm_serve_read_latency": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32,
                                          C.POINTER(C.c_uint32), C.c_void_p]),
     "pbm_serve_clock_hz": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "pbm_bursts_create": (C.c_int, [C.c_void_p, C.POINTER(BurstTrace), C.POINTER(C.c_void_p)]),
+    "pbm_bursts_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_bursts_run": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(Launch),
+                                 C.c_void_p]),
+    "pbm_bursts_graph": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(Launch),
+                                   C.POINTER(C.c_void_p)]),
+    "pbm_bursts_read_ptrs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_void_p]),
+    "pbm_bursts_reset": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pbm_bursts_barrier_timeouts": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
+    "pbm_bursts_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(TuneResult),
+                                      C.c_void_p]),
     "pbm_last_error": (C.c_char_p, []),
     "pbm_version": (C.c_char_p, []),
 }
@@ -441,6 +460,130 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Bursts:
+    """A trace of mallocRW/freeRW bursts on a backed pool (pbm_bursts_*)."""
+
+    def __init__(self, pool: Pool, trace: dict):
+        self._keep = {k: np.ascontiguousarray(trace[k], dtype=t) for k, t in
+                      (("alloc_offset", np.uint32), ("free_offset", np.uint32),
+                       ("sizes", np.uint64), ("free_index", np.uint32))}
+        k = self._keep
+        tr = BurstTrace(len(k["alloc_offset"]) - 1, 0,
+                        *(k[n].ctypes.data_as(C.c_void_p) for n in
+                          ("alloc_offset", "free_offset", "sizes", "free_index")))
+        b = C.c_void_p()
+        check(lib().pbm_bursts_create(pool.h, C.byref(tr), C.byref(b)), "pbm_bursts_create")
+        self.b = b
+        self.pool = pool
+        self.n_bursts = tr.n_bursts
+        self.n_allocs = int(k["alloc_offset"][-1])
+
+    def run(self, first: int, count: int, launch_: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_bursts_run(self.b, first, count, C.byref(launch_ or launch()),
+                                   _stream(stream)), "pbm_bursts_run")
+
+    def graph(self, first: int, count: int, launch_: Launch | None = None) -> Graph:
+        g = C.c_void_p()
+        check(lib().pbm_bursts_graph(self.b, first, count, C.byref(launch_ or launch()),
+                                     C.byref(g)), "pbm_bursts_graph")
+        return _StepGraph(g, self)
+
+    def ptrs(self, first: int = 0, n: int | None = None, stream=None) -> np.ndarray:
+        n = self.n_allocs - first if n is None else n
+        out = np.zeros(max(n, 1), dtype=np.uint64)
+        check(lib().pbm_bursts_read_ptrs(self.b, first, n, out.ctypes.data_as(C.c_void_p),
+                                         _stream(stream)), "pbm_bursts_read_ptrs")
+        return out[:n]
+
+    def reset(self, stream=None) -> None:
+        check(lib().pbm_bursts_reset(self.b, _stream(stream)), "pbm_bursts_reset")
+
+    def barrier_timeouts(self, stream=None) -> int:
+        v = C.c_uint64()
+        check(lib().pbm_bursts_barrier_timeouts(self.b, C.byref(v), _stream(stream)),
+              "pbm_bursts_barrier_timeouts")
+        return int(v.value)
+
+    def autotune(self, count: int, flags: int = 0, stream=None) -> dict:
+        r = TuneResult()
+        check(lib().pbm_bursts_autotune(self.b, count, flags, C.byref(r), _stream(stream)),
+              "pbm_bursts_autotune")
+        g = r.get_launch
+        return {"launch": g, "waves_per_cu": int(g.waves_per_cu), "walk_budget": int(g.walk_budget),
+                "cooperative": bool(g.flags & LAUNCH_COOPERATIVE), "ms": float(r.ms_alloc),
+                "candidates": int(r.candidates)}
+
+    def close(self) -> None:
+        if self.b:
+            check(lib().pbm_bursts_destroy(self.b), "pbm_bursts_destroy")
+            self.b = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def burst_trace(n_bursts: int, median: float, sigma: float, min_size: int, max_size: int,
+                page_size: int, seed: int) -> dict:
+    """Bursts with lognormal request counts (median, sigma; at least 1),
+    each request an alloc or a free with probability 1/2 (a free needs a
+    live allocation of an earlier burst), alloc sizes log-uniform in
+    [min_size, max_size] bytes.  mallocRW packs the small requests
+    (size+4 rounded to 4 < page_size) of each 64-request chunk of a burst
+    behind one leader, and freeing the leader returns them all, so a free
+    picks a live packed group or a live large request and frees all of it.
+    Returns alloc_offset, free_offset, sizes, free_index (numpy arrays) and
+    the request counts."""
+    rng = np.random.default_rng(seed)
+    counts = np.maximum(1, np.rint(np.exp(np.log(median) + sigma * rng.standard_normal(n_bursts))))
+    counts = counts.astype(np.int64)
+    lo, hi = np.log(min_size), np.log(max_size)
+    units: list[np.ndarray] = []        # live units (arrays of request indices)
+    alloc_off, free_off = [0], [0]
+    sizes_l, frees_l = [], []
+    n_alloc = 0
+    for b in range(n_bursts):
+        c = int(counts[b])
+        want_free = int((rng.random(c) < 0.5).sum())
+        freed = 0
+        if want_free and units:
+            order = rng.permutation(len(units))
+            take = []
+            for u in order:
+                if freed >= want_free:
+                    break
+                take.append(u)
+                freed += len(units[u])
+            chosen = set(take)
+            frees_l.extend(np.concatenate([units[u] for u in take]).tolist())
+            units = [u for i, u in enumerate(units) if i not in chosen]
+        k = max(c - freed, 0)
+        sz = np.exp(rng.uniform(lo, hi, size=k)).astype(np.uint64)
+        sizes_l.append(sz)
+        small = ((sz + 4 + 3) & ~np.uint64(3)) < page_size
+        idx = np.arange(n_alloc, n_alloc + k)
+        for ch in range(0, k, 64):
+            s = small[ch:ch + 64]
+            if s.any():
+                units.append(idx[ch:ch + 64][s])
+            units.extend(idx[ch:ch + 64][~s].reshape(-1, 1))
+        n_alloc += k
+        alloc_off.append(n_alloc)
+        free_off.append(len(frees_l))
+    return {"alloc_offset": np.array(alloc_off, dtype=np.uint32),
+            "free_offset": np.array(free_off, dtype=np.uint32),
+            "sizes": np.concatenate(sizes_l) if sizes_l else np.zeros(0, np.uint64),
+            "free_index": np.array(frees_l, dtype=np.uint32), "counts": counts}
 
 
 def serve_trace(n: int, rate_per_s: float, step_s: float, median: float, sigma: float,
