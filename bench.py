@@ -42,6 +42,14 @@ T_PAGES, PAGE_SIZE, FILL, SEED, N_REQ = 1 << 20, 4096, 0.9, 42, 262144
 HBM_PEAK_GBS = 8000.0
 
 
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """One progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
 def walk_stats(steps: np.ndarray, ids: np.ndarray) -> dict:
     g = ids >= 0
     out = {"tas": float(steps[g].mean()) if g.any() else 0.0}
@@ -101,6 +109,7 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
     srv.submit(arr, tgt, stream=stream)
     srv.run(warm_steps, stream)
     torch.cuda.synchronize()
+    progress(f"serving: {warm_steps} warm-up steps done")
     s0 = srv.stats(stream)
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
@@ -109,6 +118,7 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
     t1.record(stream)
     torch.cuda.synchronize()
     ms = t0.elapsed_time(t1)
+    progress(f"serving: {timed_steps} timed steps, {ms:.1f} ms")
     s1 = srv.stats(stream)
     lat = srv.latency_ticks(1 << 26, stream)[s0["lat_samples"]:s1["lat_samples"]]
     hz = srv.clock_hz()
@@ -131,6 +141,65 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
     return out
 
 
+def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int) -> dict:
+    """Bursts of mallocRW/freeRW: 262,144 x 4 KiB pages (backed), empty at
+    the start (a page-level random pre-fill leaves no runs for multi-page
+    requests); n_bursts bursts with lognormal request counts
+    (median 2,048, sigma 1.0), each request a free of a live allocation or
+    a mallocRW of log-uniform 16 B .. 64 KiB with probability 1/2.  The
+    autotuner picks the launch on the first tune_bursts bursts; the whole
+    trace is then timed from the same starting state."""
+    T, PS = 1 << 18, 4096
+    tr = pbm.burst_trace(n_bursts, 2048.0, 1.0, 16, 65536, PS, 7)
+    pool = pbm.Pool(T, PS, strategy=pbm.RW_MALLOC, backing=True, seed=7)
+    start = pool.snapshot(stream)
+    used0 = pool.count_used(stream)
+    bs = pbm.Bursts(pool, tr)
+    progress(f"bursts: trace of {n_bursts} bursts built; autotuning on {tune_bursts}")
+    tune = bs.autotune(min(tune_bursts, n_bursts), 1, stream)
+    progress(f"bursts: tuned {tune['waves_per_cu']} waves/CU, budget {tune['walk_budget']}, "
+             f"cooperative {tune['cooperative']}, {tune['ms']:.2f} ms")
+    best = tune["launch"]
+    pool.restore(start, stream)
+    bs.reset(stream)
+    graphs = []
+    if not tune["cooperative"]:
+        for first in range(0, n_bursts, 256):
+            graphs.append(bs.graph(first, min(256, n_bursts - first), best))
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    if graphs:
+        for g in graphs:
+            g.launch(stream)
+    else:
+        bs.run(0, n_bursts, best, stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1)
+    progress(f"bursts: whole trace {ms:.1f} ms")
+    for g in graphs:
+        g.close()
+    ptrs = bs.ptrs(stream=stream)
+    n_req = int(tr["alloc_offset"][-1]) + int(tr["free_offset"][-1])
+    out = {
+        "workload": f"{n_bursts} bursts, lognormal count median 2048 sigma 1.0, 50/50 "
+                    "mallocRW (16 B..64 KiB log-uniform) / freeRW, 262,144 x 4 KiB pages "
+                    "empty at the start",
+        "requests": n_req, "mallocs": int(tr["alloc_offset"][-1]),
+        "frees": int(tr["free_offset"][-1]), "ms": ms, "requests_per_s": n_req / (ms * 1e-3),
+        "malloc_failures": int((ptrs == 0).sum()),
+        "launch": {k: tune[k] for k in ("waves_per_cu", "walk_budget", "cooperative")},
+        "tune_ms_first_bursts": tune["ms"], "tune_candidates": tune["candidates"],
+        "barrier_timeouts": bs.barrier_timeouts(stream),
+        "pages_live_at_end": pool.count_used(stream) - used0,
+    }
+    bs.close()
+    pool.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawTextHelpFormatter)
@@ -146,6 +215,8 @@ def main() -> int:
                     help="timed steps of the serving loop (after as many warm-up steps "
                          "as --serve-warm; 0 = skip)")
     ap.add_argument("--serve-warm", type=int, default=800)
+    ap.add_argument("--bursts", type=int, default=4096,
+                    help="bursts of the mallocRW/freeRW leg (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the line, copy a device buffer of this many MiB once "
                          "(the known-size reference kernel for HBM byte counters)")
@@ -187,7 +258,9 @@ def main() -> int:
         tune_walk = fixed(False, 8, 256, 16)
     else:
         tune_rank = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_RANK_ONLY, stream)
+        progress("alloc/free: rank launches tuned")
         tune_walk = pbm.autotune(pool, N_REQ, ids, pbm.TUNE_WALK_ONLY, stream)
+        progress("alloc/free: walk launches tuned")
     best = tune_rank if (tune_rank["ms_alloc"] + tune_rank["ms_free"]
                          <= tune_walk["ms_alloc"] + tune_walk["ms_free"]) else tune_walk
     assert np.array_equal(pool.snapshot(stream), before), "autotune changed the pool"
@@ -234,6 +307,7 @@ def main() -> int:
     results = {}
     for name, tuned in (("batched", best), ("batched_walk", tune_walk)):
         results[name] = measure(tuned["get_launch"], tuned["free_launch"], args.steps, args.warmup)
+        progress(f"alloc/free: {name} measured")
         results[name]["tuned"] = {k: v for k, v in tuned.items()
                                   if k not in ("get_launch", "free_launch")}
 
@@ -242,6 +316,7 @@ def main() -> int:
     p_free = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
     p_steps = max(3, min(args.steps, 10))
     rp = measure(p_get, p_free, p_steps, 2)
+    progress("alloc/free: plain measured")
     cp_alloc, cp_all, ws_p, ok_p = (rp["counters_alloc"], rp["counters_cycle"], rp["walk_steps"],
                                     rp["verified"])
 
@@ -282,10 +357,12 @@ def main() -> int:
 
     serving = (serving_leg(torch, pbm, stream, args.serve_warm, args.serve_steps)
                if args.serve_steps > 0 else None)
+    bursts = bursts_leg(torch, pbm, stream, args.bursts, 256) if args.bursts > 0 else None
 
     # ---- CPU baseline: reference policy, serial, same pre-fill and requests
     cpu = None
     if args.cpu_seconds > 0:
+        progress(f"cpu baseline: {args.cpu_seconds:.0f} s of the serial policy")
         sys.path.insert(0, str(ROOT / "oracle"))
         import pbm_oracle as orc
         t_end = time.perf_counter() + args.cpu_seconds
@@ -336,6 +413,7 @@ def main() -> int:
                                 "kernel": kernel_names["alloc"]}},
         "cpu_baseline": cpu,
         "serving": serving,
+        "bursts": bursts,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:
