@@ -1,0 +1,156 @@
+/*
+ * pbm_compat.hpp — the reference's host and device API names over the
+ * C-ABI, for code written against RandomWalkBasic_BM.cuh / CollabRW_BM.cuh
+ * / CRW_BM.cuh / RWMalloc.cuh.
+ *
+ * The reference picks its policy by which header a translation unit
+ * includes (unitTests/unitTest1.cu:7) and keeps one allocator per
+ * translation unit in file-static device globals set with
+ * cudaMemcpyToSymbol (Paging.cuh:23-28).  Here PBM_COMPAT_STRATEGY picks
+ * the policy before the include (default PBM_RW_BM) and this header keeps
+ * one pool per translation unit: a handle on the host and a pbm_ctx copy in
+ * a file-static __device__ variable.  Host errors print the library's
+ * message and exit with the error code, as gpuErrchk does
+ * (ErrorHandler.cuh:6-14).  Walk seeds mix the thread index with the wall
+ * clock, as the reference mixes it with clock() (RandomWalkBasic_BM.cuh:63).
+ *
+ * Build: hipcc --offload-arch=gfx950 -I include your.hip -L<dir> -lpbm
+ */
+#ifndef PBM_COMPAT_HPP
+#define PBM_COMPAT_HPP
+
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstdlib>
+#include "pbm.h"
+#include "pbm_device.hpp"
+
+#ifndef PBM_COMPAT_STRATEGY
+#define PBM_COMPAT_STRATEGY PBM_RW_BM
+#endif
+#ifndef TOTAL_SIZE_GB_DEFAULT
+#define TOTAL_SIZE_GB_DEFAULT 8   /* Paging.cuh:5 */
+#endif
+#ifndef PAGE_SIZE_DEFAULT
+#define PAGE_SIZE_DEFAULT 512     /* Paging.cuh:4 */
+#endif
+
+static __device__ pbm_ctx pbm_compat_ctx;
+static pbm_handle* pbm_compat_handle = nullptr;
+
+static inline void pbm_compat_check(int rc, const char* what) {
+    if (rc != PBM_OK) {
+        std::fprintf(stderr, "%s: %s\n", what, pbm_last_error());
+        std::exit(rc < 0 ? -rc : rc);
+    }
+}
+
+static inline void pbm_compat_publish() {
+    pbm_ctx c;
+    pbm_compat_check(pbm_get_ctx(pbm_compat_handle, &c), "pbm_get_ctx");
+    if (hipMemcpyToSymbol(HIP_SYMBOL(pbm_compat_ctx), &c, sizeof(c)) != hipSuccess) {
+        std::fprintf(stderr, "hipMemcpyToSymbol(pbm_compat_ctx) failed\n");
+        std::exit(2);
+    }
+}
+
+/* initMemoryManagement (RandomWalkBasic_BM.cuh:42-55): nGB GiB of pages of
+ * pageSize bytes, all free (h_total_n_pages = nGB * 2^30 / pageSize,
+ * Paging.cuh:76). */
+static inline void initMemoryManagement(int nGB = TOTAL_SIZE_GB_DEFAULT,
+                                        int pageSize = PAGE_SIZE_DEFAULT) {
+    pbm_config cfg = {};
+    cfg.n_pages = (uint32_t)(((uint64_t)nGB << 30) / (uint64_t)pageSize);
+    cfg.page_size = (uint32_t)pageSize;
+    cfg.strategy = PBM_COMPAT_STRATEGY;
+    cfg.backing = 1;
+    cfg.seed = 42;
+    if (pbm_compat_handle) pbm_destroy(pbm_compat_handle);
+    pbm_compat_check(pbm_create(&cfg, &pbm_compat_handle), "initMemoryManagement");
+    pbm_compat_publish();
+}
+
+/* resetMemoryManager (RandomWalkBasic_BM.cuh:104-106): every page free. */
+static inline void resetMemoryManager() {
+    pbm_compat_check(pbm_reset(pbm_compat_handle, nullptr), "resetMemoryManager");
+    pbm_compat_check(hipDeviceSynchronize() == hipSuccess ? PBM_OK : PBM_ERR_HIP, "resetMemoryManager");
+}
+
+/* getFreePagePercentage (RandomWalkBasic_BM.cuh:109-145): free / total. */
+static inline float getFreePagePercentage() {
+    float f = 0;
+    pbm_compat_check(pbm_free_page_percentage(pbm_compat_handle, &f, nullptr),
+                     "getFreePagePercentage");
+    return f;
+}
+
+/* prefillBuffer (RandomWalkBasic_BM.cuh:148-164): the allocator itself
+ * takes (int)((1 - freePercentage) * N) pages, leaving freePercentage free. */
+static inline void prefillBuffer(float freePercentage) {
+    pbm_compat_check(pbm_prefill_requests(pbm_compat_handle, freePercentage, nullptr),
+                     "prefillBuffer");
+    pbm_compat_check(hipDeviceSynchronize() == hipSuccess ? PBM_OK : PBM_ERR_HIP, "prefillBuffer");
+}
+
+/* The handle behind the names above, for calls the reference has no name
+ * for (batched phases, counters, snapshots). */
+static inline pbm_handle* pbmCompatHandle() { return pbm_compat_handle; }
+
+/* ---- device side ------------------------------------------------------ */
+
+static __device__ inline int32_t pbm_compat_seed() {
+    const uint64_t gid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    return pbm::request_seed(pbm_compat_ctx.seed, (gid << 32) ^ wall_clock64());
+}
+
+/* getPage (RandomWalkBasic_BM.cuh:60-89, CollabRW_BM.cuh:114-160,
+ * CRW_BM.cuh:75-142, RWMalloc.cuh:113-149): a page id, or -1 when the pool
+ * has none (the reference spins).  CollabRW and RWMalloc are collective
+ * over the calling wave's active lanes; collectWarpData is accepted and
+ * ignored. */
+static __device__ inline int getPage(int* stepCount = nullptr, bool collectWarpData = false) {
+    (void)collectWarpData;
+    uint32_t steps = 0;
+    int32_t id;
+    const pbm_ctx& c = pbm_compat_ctx;
+    if (PBM_COMPAT_STRATEGY == PBM_COLLAB_RW_BM)
+        id = pbm::get_page_collab(c, pbm_compat_seed(), &steps);
+    else if (PBM_COMPAT_STRATEGY == PBM_CRW_BM)
+        id = pbm::get_page_crw(c, blockIdx.x * blockDim.x + threadIdx.x, pbm_compat_seed(), &steps);
+    else if (PBM_COMPAT_STRATEGY == PBM_RW_MALLOC)
+        id = pbm::get_page_rwmalloc(c, pbm_compat_seed(), &steps);
+    else
+        id = pbm::get_page_rwbm(c, pbm_compat_seed(), &steps);
+    if (stepCount) *stepCount = (int)steps;
+    return id;
+}
+
+/* freePage (RandomWalkBasic_BM.cuh:98-102). */
+static __device__ inline void freePage(int pageID) { pbm::free_page(pbm_compat_ctx, pageID); }
+
+/* pageAddress / getPageID (Paging.cuh:30-52). */
+static __device__ inline void* pageAddress(int pageID) {
+    return pbm::page_address(pbm_compat_ctx, pageID);
+}
+static __device__ inline int getPageID(void* ptr) { return pbm::page_id(pbm_compat_ctx, ptr); }
+
+/* getPagePtr (RandomWalkBasic_BM.cuh:92-95). */
+static __device__ inline void* getPagePtr() { return pageAddress(getPage()); }
+
+/* getNPage (RWMalloc.cuh:198-248): n <= 64 consecutive pages in one word;
+ * wave-collective. */
+static __device__ inline int getNPage(int n, int* stepCount = nullptr) {
+    uint32_t steps = 0;
+    const int32_t id = pbm::get_n_pages(pbm_compat_ctx, n < 0 ? 65u : (uint32_t)n,
+                                        pbm_compat_seed(), &steps);
+    if (stepCount) *stepCount = (int)steps;
+    return id;
+}
+
+/* mallocRW / freeRW (RWMalloc.cuh:299-334); wave-collective. */
+static __device__ inline void* mallocRW(size_t size) {
+    return pbm::malloc_rw(pbm_compat_ctx, size, pbm_compat_seed(), nullptr);
+}
+static __device__ inline void freeRW(void* ptr) { pbm::free_rw(pbm_compat_ctx, ptr); }
+
+#endif /* PBM_COMPAT_HPP */

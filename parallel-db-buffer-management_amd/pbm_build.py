@@ -54,5 +54,30 @@ def build_library(jobs: int = 8) -> Path:
     return LIB
 
 
+COMPAT_SRC = ROOT / "tests" / "compat" / "compat_check.hip"
+COMPAT_STRATEGIES = {"rw_bm": "PBM_RW_BM", "collab": "PBM_COLLAB_RW_BM", "crw": "PBM_CRW_BM",
+                     "rw_malloc": "PBM_RW_MALLOC"}
+
+
+def compat_binary(name: str) -> Path:
+    return OBJDIR / f"compat_check_{name}"
+
+
+def build_compat_checks(jobs: int = 4) -> list[Path]:
+    """The reference-API driver (tests/compat/compat_check.hip) against
+    pbm_compat.hpp and libpbm.so, once per policy."""
+    OBJDIR.mkdir(exist_ok=True)
+    outs = [compat_binary(n) for n in COMPAT_STRATEGIES]
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, f"-DPBM_COMPAT_STRATEGY={m}", str(COMPAT_SRC),
+                                 "-o", str(o), "-L", str(PKG), "-lpbm",
+                                 "-Wl,-rpath,$ORIGIN/.."])
+                for m, o in zip(COMPAT_STRATEGIES.values(), outs)]
+        for f in futs:
+            f.result()
+    return outs
+
+
 if __name__ == "__main__":
     print(build_library(), file=sys.stderr)
+    print(build_compat_checks(), file=sys.stderr)
