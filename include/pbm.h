@@ -359,6 +359,67 @@ int pbm_serve_read_latency(pbm_server* s, uint32_t* ticks, uint32_t n, uint32_t*
 /* Frequency of the wall clock the latencies are counted in, Hz. */
 int pbm_serve_clock_hz(pbm_server* s, unsigned long long* hz);
 
+/* ---- buffer cache with CLOCK replacement ------------------------------
+ *
+ * A page cache of n_logical pages in `capacity` slots over pool h: a slot
+ * holds one frame (a pool page got with the pool strategy's device
+ * getPage, CRW_BM's clustered walk for a CRW_BM pool) and a reference bit.
+ * References are taken `batch` at a time: the pages a batch hits get their
+ * bit set; the first reference to an absent page is a miss and later ones
+ * in the batch are hits; misses fill empty slots first and then replace
+ * victims chosen by a CLOCK sweep from the arm over the slots occupied when
+ * the batch began (set bits are cleared and passed; if one pass has too few
+ * clear slots, the next pass takes slots in ring order), the sweep being
+ * one scan over the whole ring; loaded pages start with the bit set.  A
+ * batch of 1 is the textbook second-chance CLOCK.  No reference
+ * counterpart: the reference's clocks are allocators
+ * (parallelPage.cu:470-618) and it has no replacement. */
+
+typedef struct pbm_cache_config {
+    uint32_t n_logical;       /* logical pages, ids 0 .. n_logical-1               */
+    uint32_t capacity;        /* slots, 1 .. pool pages                           */
+    uint32_t batch;           /* references per batch, 1 .. capacity              */
+    uint32_t reserved;
+    uint64_t log_capacity;    /* evicted pages recorded in order (0 = none)       */
+} pbm_cache_config;
+
+typedef struct pbm_cache_stats {
+    unsigned long long hits;
+    unsigned long long misses;
+    unsigned long long evictions;
+    unsigned long long resident;        /* pages held now                         */
+    unsigned long long arm;             /* ring position of the clock arm         */
+    unsigned long long batches;
+    unsigned long long alloc_failures;  /* frames the pool could not give (0)     */
+    unsigned long long reserved;
+} pbm_cache_stats;
+
+typedef struct pbm_cache pbm_cache;
+
+int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_cache** out);
+int pbm_cache_destroy(pbm_cache* k);
+
+/* Runs n references (device array of logical page ids) through the cache. */
+int pbm_cache_run(pbm_cache* k, const int32_t* d_refs, uint64_t n, void* stream);
+
+int pbm_cache_read_stats(pbm_cache* k, pbm_cache_stats* out, void* stream);
+
+/* Evicted pages first .. first+n-1 of the log, in eviction order. */
+int pbm_cache_read_evictions(pbm_cache* k, uint64_t first, uint32_t n, int32_t* host,
+                             void* stream);
+
+/* Per slot (capacity entries each, any may be NULL): page held (-1 none),
+ * its frame, its reference bit. */
+int pbm_cache_read_slots(pbm_cache* k, int32_t* owner, int32_t* frame, uint32_t* refbit,
+                         void* stream);
+
+/* count references of a Zipf(s) trace over n_logical pages, starting at
+ * index first, into device memory: reference i is the first k with
+ * cdf[k] > u_i * cdf[n-1], cdf[k] = sum_{j<=k} (j+1)^-s summed in double in
+ * index order, u_i = (splitmix64 hash of (seed, i)) >> 11, over 2^53. */
+int pbm_zipf_trace(uint32_t n_logical, double s, uint64_t seed, uint64_t first, uint64_t count,
+                   int32_t* d_out, int device, void* stream);
+
 /* ---- autotuning ------------------------------------------------------- */
 
 #define PBM_TUNE_QUICK     1u  /* 3 timed cycles per candidate instead of 9            */

@@ -178,6 +178,9 @@ __device__ __forceinline__ long long atom_add_ll(long long* p, long long v) {
 __device__ __forceinline__ u64 atom_add_u64(u64* p, u64 v) {
     return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ u64 atom_max_u64(u64* p, u64 v) {
+    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 /* Per-thread event tally; batch kernels sum it over the wave and add it to
  * ctx.counters once (flush_stats). */

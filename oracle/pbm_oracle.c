@@ -1010,3 +1010,149 @@ void orc_serve_sim(int64_t free_pages, uint32_t max_slots, uint32_t n_seqs,
     free(act);
     free(len);
 }
+
+/* ------------------------------------------- buffer cache (CLOCK) */
+
+/* Zipf(s) over n logical pages: cdf[k] = sum_{j<=k} (j+1)^-s, summed in
+ * index order in double.  pbm_cache.hip builds the same table the same
+ * way. */
+void orc_zipf_cdf(uint32_t n, double s, double* cdf) {
+    double acc = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        acc += pow((double)k + 1.0, -s);
+        cdf[k] = acc;
+    }
+}
+
+/* Reference i of the trace: u = the top 53 bits of mix64 of (seed, i) as a
+ * double in [0,1), page = the first k with cdf[k] > u * cdf[n-1]. */
+int32_t orc_zipf_sample(const double* cdf, uint32_t n, uint64_t seed, uint64_t i) {
+    const u64 h = orc_mix64((seed ^ 0xBE5466CF34E90C6Cull) + (i + 1ull) * 0x9E3779B97F4A7C15ull);
+    const double u = (double)(h >> 11) * (1.0 / 9007199254740992.0);
+    const double t = u * cdf[n - 1];
+    uint32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        uint32_t mid = lo + (hi - lo) / 2u;
+        if (cdf[mid] > t) hi = mid; else lo = mid + 1u;
+    }
+    return (int32_t)lo;
+}
+
+void orc_zipf_trace(uint32_t n, double s, uint64_t seed, uint64_t first, uint64_t count,
+                    int32_t* out) {
+    double* cdf = (double*)malloc(sizeof(double) * n);
+    if (!cdf) return;
+    orc_zipf_cdf(n, s, cdf);
+    for (uint64_t i = 0; i < count; ++i) out[i] = orc_zipf_sample(cdf, n, seed, first + i);
+    free(cdf);
+}
+
+/* The buffer cache of pbm_cache.hip: `capacity` slots, CLOCK replacement,
+ * references taken `batch` at a time.  A batch (i) sets the reference bit
+ * of every slot it hits, as the cache stood when the batch began; (ii)
+ * counts the first reference to each absent page as a miss and later ones
+ * in the batch as hits; (iii) fills empty slots in order with the first
+ * misses, and for the rest takes victims by CLOCK over the slots occupied
+ * when the batch began, in ring order from the arm: a slot with its bit
+ * set has it cleared, one without is a victim; if one pass finds too few,
+ * every bit is clear and the next pass takes slots in ring order again;
+ * the arm moves past the last victim; (iv) loads the misses, in order,
+ * into the empty slots and then the victims' slots, with the bit set.
+ * With batch = 1 this is the textbook second-chance CLOCK.
+ * state: slot_of[n_logical] (-1 absent), owner[capacity], refbit[capacity],
+ * arm_res[2] = arm, resident.  stats += hits, misses, evictions.
+ * Evicted pages go to evict_log in order while it has room. */
+void orc_clock_run(uint32_t n_logical, uint32_t capacity, uint32_t batch, const int32_t* refs,
+                   uint64_t n_refs, int32_t* slot_of, int32_t* owner, uint8_t* refbit,
+                   uint32_t* arm_res, int32_t* evict_log, uint64_t log_cap, uint64_t* stats) {
+    const uint32_t C = capacity;
+    uint32_t arm = arm_res[0], resident = arm_res[1];
+    uint32_t* stamp = (uint32_t*)calloc(n_logical ? n_logical : 1u, sizeof(uint32_t));
+    int32_t* miss = (int32_t*)malloc(sizeof(int32_t) * (batch ? batch : 1u));
+    int32_t* victim = (int32_t*)malloc(sizeof(int32_t) * (batch ? batch : 1u));
+    int32_t* ones = (int32_t*)malloc(sizeof(int32_t) * (C ? C : 1u));
+    uint32_t* ones_pos = (uint32_t*)malloc(sizeof(uint32_t) * (C ? C : 1u));
+    if (!stamp || !miss || !victim || !ones || !ones_pos) goto out;
+    uint32_t bno = 0;
+    for (uint64_t b0 = 0; b0 < n_refs; b0 += batch, ++bno) {
+        const uint32_t nb = n_refs - b0 < batch ? (uint32_t)(n_refs - b0) : batch;
+        uint32_t K = 0;
+        for (uint32_t i = 0; i < nb; ++i) {
+            const int32_t l = refs[b0 + i];
+            const int32_t s = slot_of[l];
+            if (s >= 0) {
+                refbit[s] = 1;
+                stats[0]++;
+            } else if (stamp[l] != bno + 1u) {
+                stamp[l] = bno + 1u;
+                miss[K++] = l;
+                stats[1]++;
+            } else {
+                stats[0]++;
+            }
+        }
+        const uint32_t r0 = resident, empty = C - r0;
+        const uint32_t v = K > empty ? K - empty : 0u;
+        if (v > 0) {
+            uint32_t z = 0, n1 = 0, stop = 0;
+            for (uint32_t j = 0; j < C; ++j) {
+                const uint32_t s = (arm + j) % C;
+                if (s < r0 && !refbit[s]) z++;
+            }
+            uint32_t nv = 0;
+            if (v <= z) {
+                for (uint32_t j = 0; j < C && nv < v; ++j) {
+                    const uint32_t s = (arm + j) % C;
+                    if (s >= r0) continue;
+                    if (refbit[s]) {
+                        refbit[s] = 0;
+                    } else {
+                        victim[nv++] = (int32_t)s;
+                        stop = j;
+                    }
+                }
+            } else {
+                for (uint32_t j = 0; j < C; ++j) {
+                    const uint32_t s = (arm + j) % C;
+                    if (s >= r0) continue;
+                    if (refbit[s]) {
+                        refbit[s] = 0;
+                        ones[n1] = (int32_t)s;
+                        ones_pos[n1++] = j;
+                    } else {
+                        victim[nv++] = (int32_t)s;
+                    }
+                }
+                for (uint32_t k = 0; k < v - z; ++k) {
+                    victim[nv++] = ones[k];
+                    stop = ones_pos[k];
+                }
+            }
+            arm = (arm + stop + 1u) % C;
+        }
+        for (uint32_t i = 0; i < K; ++i) {
+            uint32_t s;
+            if (i < empty) {
+                s = r0 + i;
+            } else {
+                s = (uint32_t)victim[i - empty];
+                const int32_t old = owner[s];
+                slot_of[old] = -1;
+                if (evict_log && stats[2] < log_cap) evict_log[stats[2]] = old;
+                stats[2]++;
+            }
+            owner[s] = miss[i];
+            slot_of[miss[i]] = (int32_t)s;
+            refbit[s] = 1;
+        }
+        resident = r0 + K < C ? r0 + K : C;
+    }
+    arm_res[0] = arm;
+    arm_res[1] = resident;
+out:
+    free(stamp);
+    free(miss);
+    free(victim);
+    free(ones);
+    free(ones_pos);
+}

@@ -59,6 +59,10 @@ _SIGS = {
                                        _PU32]),
     "orc_check_alloc": (C.c_int, [_PU64, _PU64, C.c_uint32, C.c_uint32, _P32, _PU32]),
     "orc_serve_payload": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "orc_zipf_cdf": (None, [C.c_uint32, C.c_double, C.POINTER(C.c_double)]),
+    "orc_zipf_trace": (None, [C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, _P32]),
+    "orc_clock_run": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _P32, C.c_uint64, _P32, _P32,
+                             C.POINTER(C.c_uint8), _PU32, _P32, C.c_uint64, _PU64]),
     "orc_serve_checksum": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
     "orc_serve_sim": (None, [C.c_int64, C.c_uint32, C.c_uint32, _PU32, _PU32, C.c_uint32, _PU32,
                              _PU32, _PU64]),
@@ -260,6 +264,44 @@ def serve_sim(free_pages: int, max_slots: int, arrival: np.ndarray, target: np.n
     lib().orc_serve_sim(free_pages, max_slots, a.size, _p(a, _PU32), _p(t, _PU32), n_steps,
                         _p(adm, _PU32), _p(fin, _PU32), _p(tot, _PU64))
     return adm, fin, tot
+
+
+def zipf_cdf(n: int, s: float) -> np.ndarray:
+    cdf = np.zeros(n, dtype=np.float64)
+    lib().orc_zipf_cdf(n, s, cdf.ctypes.data_as(C.POINTER(C.c_double)))
+    return cdf
+
+
+def zipf_trace(n: int, s: float, seed: int, first: int, count: int) -> np.ndarray:
+    out = np.zeros(count, dtype=np.int32)
+    lib().orc_zipf_trace(n, s, seed, first, count, _p(out, _P32))
+    return out
+
+
+class ClockCache:
+    """The buffer cache's batched CLOCK schedule (orc_clock_run)."""
+
+    def __init__(self, n_logical: int, capacity: int, batch: int, log_cap: int = 0):
+        self.n_logical, self.capacity, self.batch = n_logical, capacity, batch
+        self.slot_of = np.full(n_logical, -1, dtype=np.int32)
+        self.owner = np.full(capacity, -1, dtype=np.int32)
+        self.refbit = np.zeros(capacity, dtype=np.uint8)
+        self.arm_res = np.zeros(2, dtype=np.uint32)
+        self.stats = np.zeros(3, dtype=np.uint64)      # hits, misses, evictions
+        self.log = np.zeros(max(log_cap, 1), dtype=np.int32)
+        self.log_cap = log_cap
+
+    def run(self, refs: np.ndarray) -> None:
+        refs = np.ascontiguousarray(refs, dtype=np.int32)
+        lib().orc_clock_run(self.n_logical, self.capacity, self.batch, _p(refs, _P32), refs.size,
+                            _p(self.slot_of, _P32), _p(self.owner, _P32),
+                            self.refbit.ctypes.data_as(C.POINTER(C.c_uint8)),
+                            _p(self.arm_res, _PU32), _p(self.log, _P32) if self.log_cap else None,
+                            self.log_cap, _p(self.stats, _PU64))
+
+    @property
+    def evictions(self) -> np.ndarray:
+        return self.log[:min(int(self.stats[2]), self.log_cap)]
 
 
 def dist_clock_arm(seed: int, slot: int, n_pages: int) -> int:

@@ -94,6 +94,19 @@ SERVE_PAYLOAD, SERVE_LATENCY = 1, 2
 LAUNCH_COOPERATIVE = 16
 
 
+class CacheConfig(C.Structure):
+    _fields_ = [("n_logical", C.c_uint32), ("capacity", C.c_uint32), ("batch", C.c_uint32),
+                ("reserved", C.c_uint32), ("log_capacity", C.c_uint64)]
+
+
+class CacheStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("hits", "misses", "evictions", "resident", "arm",
+                                          "batches", "alloc_failures", "reserved")]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != "reserved"}
+
+
 class BurstTrace(C.Structure):
     _fields_ = [("n_bursts", C.c_uint32), ("reserved", C.c_uint32),
                 ("alloc_offset", C.c_void_p), ("free_offset", C.c_void_p),
@@ -157,6 +170,16 @@ _SIGS = {
     "pbm_serve_read_latency": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32,
                                          C.POINTER(C.c_uint32), C.c_void_p]),
     "pbm_serve_clock_hz": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    "pbm_cache_create": (C.c_int, [C.c_void_p, C.POINTER(CacheConfig), C.POINTER(C.c_void_p)]),
+    "pbm_cache_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_cache_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pbm_cache_read_stats": (C.c_int, [C.c_void_p, C.POINTER(CacheStats), C.c_void_p]),
+    "pbm_cache_read_evictions": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
+                                           C.c_void_p]),
+    "pbm_cache_read_slots": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
+    "pbm_zipf_trace": (C.c_int, [C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
+                                 C.c_void_p, C.c_int, C.c_void_p]),
     "pbm_bursts_create": (C.c_int, [C.c_void_p, C.POINTER(BurstTrace), C.POINTER(C.c_void_p)]),
     "pbm_bursts_destroy": (C.c_int, [C.c_void_p]),
     "pbm_bursts_run": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(Launch),
@@ -460,6 +483,74 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Cache:
+    """Buffer cache with batched CLOCK replacement over a pool (pbm_cache_*)."""
+
+    def __init__(self, pool: Pool, n_logical: int, capacity: int, batch: int,
+                 log_capacity: int = 0):
+        cfg = CacheConfig(n_logical, capacity, batch, 0, log_capacity)
+        k = C.c_void_p()
+        check(lib().pbm_cache_create(pool.h, C.byref(cfg), C.byref(k)), "pbm_cache_create")
+        self.k = k
+        self.pool = pool
+        self.cfg = cfg
+
+    def run(self, refs, n: int | None = None, stream=None) -> None:
+        """refs: a device int32 tensor (or address) of logical page ids."""
+        n = refs.numel() if n is None else n
+        check(lib().pbm_cache_run(self.k, _ptr(refs), n, _stream(stream)), "pbm_cache_run")
+
+    def stats(self, stream=None) -> dict:
+        st = CacheStats()
+        check(lib().pbm_cache_read_stats(self.k, C.byref(st), _stream(stream)),
+              "pbm_cache_read_stats")
+        return st.as_dict()
+
+    def evictions(self, first: int = 0, n: int | None = None, stream=None) -> np.ndarray:
+        if n is None:
+            n = min(self.stats(stream)["evictions"], self.cfg.log_capacity) - first
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        check(lib().pbm_cache_read_evictions(self.k, first, n, out.ctypes.data_as(C.c_void_p),
+                                             _stream(stream)), "pbm_cache_read_evictions")
+        return out[:n]
+
+    def slots(self, stream=None):
+        cap = self.cfg.capacity
+        owner = np.zeros(cap, dtype=np.int32)
+        frame = np.zeros(cap, dtype=np.int32)
+        bit = np.zeros(cap, dtype=np.uint32)
+        check(lib().pbm_cache_read_slots(self.k, owner.ctypes.data_as(C.c_void_p),
+                                         frame.ctypes.data_as(C.c_void_p),
+                                         bit.ctypes.data_as(C.c_void_p), _stream(stream)),
+              "pbm_cache_read_slots")
+        return owner, frame, bit
+
+    def close(self) -> None:
+        if self.k:
+            check(lib().pbm_cache_destroy(self.k), "pbm_cache_destroy")
+            self.k = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def zipf_trace(out, n_logical: int, s: float, seed: int, first: int = 0, device: int = 0,
+               stream=None) -> None:
+    """Fills the device int32 tensor `out` with a Zipf(s) trace over
+    n_logical pages (pbm_zipf_trace)."""
+    check(lib().pbm_zipf_trace(n_logical, s, seed, first, out.numel(), _ptr(out), device,
+                               _stream(stream)), "pbm_zipf_trace")
 
 
 class Bursts:

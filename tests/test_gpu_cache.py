@@ -1,0 +1,96 @@
+"""The buffer cache (batched CLOCK over pool frames) through the C-ABI: the
+device Zipf trace equals the oracle's; hits, misses, the eviction order,
+the arm and every slot's page and bit equal the serial oracle's for
+batches of 1 (the textbook CLOCK) up to 65,536; frames are distinct pool
+pages, exactly as many as pages held."""
+import numpy as np
+import pytest
+
+import pbm
+import pbm_oracle as orc
+from conftest import log_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _trace(gpu, n_logical, s, seed, n):
+    t = gpu.empty(n, dtype=gpu.int32, device="cuda")
+    pbm.zipf_trace(t, n_logical, s, seed)
+    gpu.cuda.synchronize()
+    return t
+
+
+def test_zipf_trace_matches_oracle(gpu):
+    log_case("zipf trace")
+    t = _trace(gpu, 1 << 20, 1.1, 7, 1 << 20).cpu().numpy()
+    assert np.array_equal(t, orc.zipf_trace(1 << 20, 1.1, 7, 0, 1 << 20))
+
+
+def _check(gpu, pool, cache, cc, T):
+    st = cache.stats()
+    assert st["hits"] == int(cc.stats[0]) and st["misses"] == int(cc.stats[1])
+    assert st["evictions"] == int(cc.stats[2]) and st["alloc_failures"] == 0
+    assert st["arm"] == int(cc.arm_res[0]) and st["resident"] == int(cc.arm_res[1])
+    assert np.array_equal(cache.evictions(), cc.evictions)
+    owner, frame, bit = cache.slots()
+    assert np.array_equal(owner, cc.owner)
+    assert np.array_equal(bit.astype(np.uint8), cc.refbit)
+    held = frame[owner >= 0]
+    assert (held >= 0).all() and len(np.unique(held)) == held.size
+    assert pool.count_used() == held.size == st["resident"]
+    bm = pool.snapshot()
+    assert all((int(bm[int(f) // 64]) >> (int(f) % 64)) & 1 for f in held[:5000])
+
+
+@pytest.mark.parametrize("B,n", [(1, 20000), (7, 60000), (256, 300000), (2048, 1000000)])
+def test_cache_matches_oracle(gpu, B, n):
+    T, L, C = 1 << 16, 1 << 14, 2048
+    log_case(f"cache batch={B} refs={n}")
+    refs = _trace(gpu, L, 1.1, 3, n)
+    cc = orc.ClockCache(L, C, B, log_cap=n)
+    cc.run(refs.cpu().numpy())
+    with pbm.Pool(T, strategy=pbm.CRW_BM, seed=5) as pool:
+        with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
+            cache.run(refs)
+            gpu.cuda.synchronize()
+            _check(gpu, pool, cache, cc, T)
+
+
+@pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.DIST_CLOCK, pbm.COLLAB_RW_BM])
+def test_cache_frame_policies(gpu, strategy):
+    T, L, C, B, n = 1 << 15, 1 << 13, 4096, 512, 200000
+    log_case(f"cache frames strategy={strategy}")
+    refs = _trace(gpu, L, 0.9, 11, n)
+    cc = orc.ClockCache(L, C, B, log_cap=n)
+    cc.run(refs.cpu().numpy())
+    with pbm.Pool(T, strategy=strategy, seed=6) as pool:
+        with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
+            cache.run(refs)
+            gpu.cuda.synchronize()
+            _check(gpu, pool, cache, cc, T)
+
+
+def test_cache_c3_sizes(gpu):
+    # 16,777,216 frames (page map only), 4,194,304 logical pages, 1,048,576
+    # slots, Zipf(1.1) seed 7, batches of 65,536; 16M references
+    T, L, C, B, n = 1 << 24, 1 << 22, 1 << 20, 1 << 16, 1 << 24
+    log_case("cache C3 sizes")
+    refs = _trace(gpu, L, 1.1, 7, n)
+    cc = orc.ClockCache(L, C, B, log_cap=n)
+    cc.run(refs.cpu().numpy())
+    with pbm.Pool(T, strategy=pbm.CRW_BM, seed=7) as pool:
+        with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
+            cache.run(refs)
+            gpu.cuda.synchronize()
+            _check(gpu, pool, cache, cc, T)
+
+
+def test_cache_rejects_bad_configs(gpu):
+    log_case("cache bad configs")
+    with pbm.Pool(1024, strategy=pbm.CRW_BM) as pool:
+        for args in [(0, 16, 4), (100, 2048, 4), (100, 16, 0), (100, 16, 17)]:
+            with pytest.raises(pbm.PbmError):
+                pbm.Cache(pool, *args)
+    with pbm.Pool(1024, strategy=pbm.LINKED_LIST) as pool:
+        with pytest.raises(pbm.PbmError):
+            pbm.Cache(pool, 100, 16, 4)

@@ -660,3 +660,140 @@ def test_serve_checksum_is_the_sum_of_payloads():
     z = ((seed ^ 0x452821E638D01377) + (((seq << 32) | 7) + 1) * 0x9E3779B97F4A7C15) & M
     assert orc.serve_payload(seed, seq, 7) == orc.mix64(z)
     assert len({orc.serve_payload(seed, seq, j) for j in range(1000)}) == 1000
+
+
+# ------------------------------------------------------------------ buffer cache (CLOCK)
+
+def _textbook_clock(refs, C):
+    """Second-chance CLOCK, one reference at a time: hit sets the bit; a
+    miss fills the next empty frame, else sweeps from the hand clearing set
+    bits and replaces the first clear one; loaded pages start with the bit
+    set; the hand moves past the replaced frame."""
+    frames, bits, where, hand = [], [], {}, 0
+    hits, evicted = 0, []
+    for p in refs:
+        p = int(p)
+        if p in where:
+            bits[where[p]] = 1
+            hits += 1
+            continue
+        if len(frames) < C:
+            where[p] = len(frames)
+            frames.append(p)
+            bits.append(1)
+            continue
+        while bits[hand]:
+            bits[hand] = 0
+            hand = (hand + 1) % C
+        old = frames[hand]
+        del where[old]
+        evicted.append(old)
+        frames[hand], bits[hand], where[p] = p, 1, hand
+        hand = (hand + 1) % C
+    return hits, evicted, hand, frames, bits
+
+
+def _batched_clock(refs, C, B):
+    """The batch rule written out plainly: hits against the batch's start,
+    first references to absent pages in order, empty frames first, then two
+    CLOCK passes over the frames occupied at the start."""
+    frames, bits, where, arm = [], [], {}, 0
+    hits, evicted = 0, []
+    for b0 in range(0, len(refs), B):
+        batch = [int(x) for x in refs[b0:b0 + B]]
+        misses = []
+        for p in batch:
+            if p in where:
+                bits[where[p]] = 1
+                hits += 1
+            elif p in misses:
+                hits += 1
+            else:
+                misses.append(p)
+        r0 = len(frames)
+        empty = C - r0
+        v = max(0, len(misses) - empty)
+        victims = []
+        if v:
+            order = [(arm + j) % C for j in range(C)]
+            order = [s for s in order if s < r0]
+            zero = [s for s in order if not bits[s]]
+            if v <= len(zero):
+                for s in order:
+                    if bits[s]:
+                        bits[s] = 0
+                    else:
+                        victims.append(s)
+                        if len(victims) == v:
+                            break
+            else:
+                ones = [s for s in order if bits[s]]
+                for s in ones:
+                    bits[s] = 0
+                victims = zero + ones[:v - len(zero)]
+            arm = (victims[-1] + 1) % C
+        for i, p in enumerate(misses):
+            if i < empty:
+                frames.append(p)
+                bits.append(1)
+                where[p] = len(frames) - 1
+            else:
+                s = victims[i - empty]
+                old = frames[s]
+                del where[old]
+                evicted.append(old)
+                frames[s], bits[s], where[p] = p, 1, s
+    return hits, evicted, arm, frames, bits
+
+
+@pytest.mark.parametrize("C,n,seed", [(4, 60, 1), (16, 400, 2), (64, 3000, 3), (1, 50, 4)])
+def test_clock_batch_of_one_is_textbook_clock(C, n, seed):
+    refs = np.random.default_rng(seed).integers(0, 3 * C + 2, size=n).astype(np.int32)
+    cc = orc.ClockCache(3 * C + 2, C, 1, log_cap=n)
+    cc.run(refs)
+    hits, ev, hand, frames, bits = _textbook_clock(refs, C)
+    assert int(cc.stats[0]) == hits and int(cc.stats[1]) == n - hits
+    assert cc.evictions.tolist() == ev
+    assert int(cc.arm_res[0]) == hand % C
+    assert cc.owner[:len(frames)].tolist() == frames
+    assert cc.refbit[:len(bits)].tolist() == bits
+
+
+def test_clock_textbook_hand_example():
+    # 3 frames, references 1 2 3 4 1 2 5 1 2 3 4 5 (Belady's string):
+    # second-chance CLOCK gives 9 faults here
+    refs = np.array([1, 2, 3, 4, 1, 2, 5, 1, 2, 3, 4, 5], dtype=np.int32)
+    cc = orc.ClockCache(6, 3, 1, log_cap=12)
+    cc.run(refs)
+    assert int(cc.stats[1]) == 9 and int(cc.stats[0]) == 3
+    assert cc.evictions.tolist() == [1, 2, 3, 4, 1, 2]
+
+
+@pytest.mark.parametrize("C,B,n,seed", [(8, 3, 300, 5), (32, 16, 2000, 6), (64, 64, 4000, 7),
+                                        (50, 7, 3000, 8), (10, 10, 500, 9)])
+def test_clock_batches_follow_the_batch_rule(C, B, n, seed):
+    refs = np.random.default_rng(seed).integers(0, 4 * C, size=n).astype(np.int32)
+    cc = orc.ClockCache(4 * C, C, B, log_cap=n)
+    cc.run(refs)
+    hits, ev, arm, frames, bits = _batched_clock(refs, C, B)
+    assert int(cc.stats[0]) == hits
+    assert cc.evictions.tolist() == ev
+    assert int(cc.arm_res[0]) == arm
+    assert cc.owner[:len(frames)].tolist() == frames
+
+
+def test_zipf_trace_matches_its_distribution():
+    n, s = 1000, 1.1
+    t = orc.zipf_trace(n, s, 7, 0, 200000)
+    assert t.min() >= 0 and t.max() < n
+    p = np.arange(1, n + 1, dtype=np.float64) ** -s
+    p /= p.sum()
+    cnt = np.bincount(t, minlength=n)
+    # rank-1 and rank-2 frequencies within 5 sigma of their probabilities
+    for k in (0, 1, 9):
+        mu = p[k] * t.size
+        assert abs(cnt[k] - mu) < 5 * np.sqrt(mu), k
+    # the sample is a prefix-independent function of (seed, index)
+    assert np.array_equal(orc.zipf_trace(n, s, 7, 100, 50), t[100:150])
+    cdf = orc.zipf_cdf(n, s)
+    assert np.isclose(cdf[-1], (np.arange(1, n + 1, dtype=np.float64) ** -s).sum(), rtol=1e-12)
