@@ -200,6 +200,41 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int) -> dict:
     return out
 
 
+def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
+    """Buffer cache: 16,777,216 x 4 KiB frames (page map only) allocated
+    with CRW_BM, 4,194,304 logical pages held in 1,048,576 slots (the
+    resident cap), CLOCK replacement, a Zipf(1.1) trace of n_refs
+    references from seed 7, taken `batch` at a time."""
+    T, L, C = 1 << 24, 1 << 22, 1 << 20
+    refs = torch.empty(n_refs, dtype=torch.int32, device="cuda")
+    pbm.zipf_trace(refs, L, 1.1, 7, stream=stream)
+    pool = pbm.Pool(T, 4096, strategy=pbm.CRW_BM, backing=False, seed=7)
+    cache = pbm.Cache(pool, L, C, batch)
+    torch.cuda.synchronize()
+    progress(f"cache: {n_refs} Zipf references generated")
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    cache.run(refs, stream=stream)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1)
+    st = cache.stats(stream)
+    progress(f"cache: {ms:.1f} ms")
+    out = {
+        "workload": f"{n_refs} Zipf(1.1) references (seed 7) over 4,194,304 logical pages, "
+                    "1,048,576 slots, CLOCK replacement, frames from CRW_BM on 16,777,216 "
+                    f"x 4 KiB pages, batches of {batch}",
+        "ms": ms, "refs_per_s": n_refs / (ms * 1e-3), "hits": st["hits"], "misses": st["misses"],
+        "hit_rate": st["hits"] / n_refs, "evictions": st["evictions"],
+        "resident": st["resident"], "alloc_failures": st["alloc_failures"],
+    }
+    cache.close()
+    pool.close()
+    del refs
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__,
                                  formatter_class=argparse.RawTextHelpFormatter)
@@ -217,6 +252,9 @@ def main() -> int:
     ap.add_argument("--serve-warm", type=int, default=800)
     ap.add_argument("--bursts", type=int, default=4096,
                     help="bursts of the mallocRW/freeRW leg (0 = skip)")
+    ap.add_argument("--cache-refs", type=int, default=1 << 26,
+                    help="references of the buffer-cache leg (0 = skip)")
+    ap.add_argument("--cache-batch", type=int, default=1 << 16)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the line, copy a device buffer of this many MiB once "
                          "(the known-size reference kernel for HBM byte counters)")
@@ -358,6 +396,8 @@ def main() -> int:
     serving = (serving_leg(torch, pbm, stream, args.serve_warm, args.serve_steps)
                if args.serve_steps > 0 else None)
     bursts = bursts_leg(torch, pbm, stream, args.bursts, 256) if args.bursts > 0 else None
+    cache = (cache_leg(torch, pbm, stream, args.cache_refs, args.cache_batch)
+             if args.cache_refs > 0 else None)
 
     # ---- CPU baseline: reference policy, serial, same pre-fill and requests
     cpu = None
@@ -414,6 +454,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "serving": serving,
         "bursts": bursts,
+        "cache": cache,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:
