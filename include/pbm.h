@@ -45,7 +45,8 @@ typedef enum pbm_strategy {
     PBM_RW_MALLOC    = 3, /* RWMalloc.cuh:113-334            collaborative walk + getNPage  */
     PBM_LINKED_LIST  = 4, /* parallelPage.cu:36-46,300-466   FIFO free list                 */
     PBM_SINGLE_CLOCK = 5, /* parallelPage.cu:470-535         one global clock arm           */
-    PBM_DIST_CLOCK   = 6  /* parallelPage.cu:537-618         per-thread clock arms          */
+    PBM_DIST_CLOCK   = 6, /* parallelPage.cu:537-618         per-thread clock arms          */
+    PBM_RW_BASIC     = 7  /* RandomWalkBasic.cuh:54-84       random walk over single pages  */
 } pbm_strategy;
 
 /* Order in which a collaborative step hands the free bits of its window to
@@ -231,6 +232,15 @@ int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
  * (RWMalloc.cuh:198-248) and the legacy getXPage<Strategy>
  * (parallelPage.cuh:56,64,72). */
 int pbm_get_n_pages(pbm_handle* h, uint32_t n, uint64_t req_base, const uint32_t* d_counts,
+                    int32_t* d_ids, uint32_t* d_steps, const pbm_launch* launch, void* stream);
+
+/* The per-thread form: request i (one thread each) takes d_counts[i]
+ * consecutive pages inside one word (0 <= count <= 64) from a random walk
+ * over words.  Replaces the legacy getXPageRandomWalk_BM
+ * (parallelPage.cu:1104-1142) behind getXPage<Strategy>
+ * (parallelPage.cuh:56,64,72).  Plain or serial; free with
+ * pbm_free_n_pages. */
+int pbm_get_x_pages(pbm_handle* h, uint32_t n, uint64_t req_base, const uint32_t* d_counts,
                     int32_t* d_ids, uint32_t* d_steps, const pbm_launch* launch, void* stream);
 
 /* Returns the runs (d_ids[i], d_counts[i]); ids < 0 or counts 0 skipped.

@@ -103,7 +103,7 @@ static __device__ inline int32_t pbm_compat_seed() {
     return pbm::request_seed(pbm_compat_ctx.seed, (gid << 32) ^ wall_clock64());
 }
 
-/* getPage (RandomWalkBasic_BM.cuh:60-89, CollabRW_BM.cuh:114-160,
+/* getPage (RandomWalkBasic_BM.cuh:60-89, RandomWalkBasic.cuh:54-80, CollabRW_BM.cuh:114-160,
  * CRW_BM.cuh:75-142, RWMalloc.cuh:113-149): a page id, or -1 when the pool
  * has none (the reference spins).  CollabRW and RWMalloc are collective
  * over the calling wave's active lanes; collectWarpData is accepted and
@@ -119,6 +119,8 @@ static __device__ inline int getPage(int* stepCount = nullptr, bool collectWarpD
         id = pbm::get_page_crw(c, blockIdx.x * blockDim.x + threadIdx.x, pbm_compat_seed(), &steps);
     else if (PBM_COMPAT_STRATEGY == PBM_RW_MALLOC)
         id = pbm::get_page_rwmalloc(c, pbm_compat_seed(), &steps);
+    else if (PBM_COMPAT_STRATEGY == PBM_RW_BASIC)
+        id = pbm::get_page_rwbasic(c, pbm_compat_seed(), &steps);
     else
         id = pbm::get_page_rwbm(c, pbm_compat_seed(), &steps);
     if (stepCount) *stepCount = (int)steps;

@@ -339,6 +339,45 @@ __device__ inline int32_t get_page_rwbm(const pbm_ctx& c, int32_t seed, uint32_t
     return id;
 }
 
+/* getPage of RandomWalkBasic (RandomWalkBasic.cuh:54-80): a walk over
+ * single pages.  Each probe takes the next LCG state, page
+ * (unsigned)seed % n_pages, and claims it if its bit was clear; a step is
+ * one probe, the first included (:59-65).  The reference keeps one int per
+ * page (atomicExch to 1) and traps after n_pages/2 probes; here the pages
+ * are bits of the shared map and a walk that spends its budget sweeps. */
+__device__ inline int32_t get_page_rwbasic(const pbm_ctx& c, int32_t seed, uint32_t* steps_out,
+                                           Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    uint32_t steps = 0;
+    int32_t id = -1;
+    if (reserve_one(c, true)) {
+        uint32_t p = 0;
+        for (uint32_t j = 0; j < c.walk_budget; ++j) {
+            seed = lcg(seed);
+            p = (uint32_t)seed % c.n_pages;
+            steps++;
+            st.probes++;
+            const u64 bit = 1ull << (p & 63u);
+            const u64 old = atom_or(&c.bitmap[p >> 6], bit);
+            st.atomics++;
+            if (!(old & bit)) {
+                id = (int32_t)p;
+                break;
+            }
+        }
+        if (id < 0) {
+            st.fallbacks++;
+            id = sweep_claim(c, p >> 6, steps, st, 4u);
+        }
+        if (id < 0) { atom_add_ll(c.free_count, 1); st.errors++; }
+    }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
 /* freePage (RandomWalkBasic_BM.cuh:98-102): clears the page's bit and
  * returns a reservation unit.  Freeing a free page or an id out of range
  * changes nothing and counts an error. */
@@ -997,6 +1036,91 @@ __device__ inline int32_t get_n_pages(const pbm_ctx& c, uint32_t n, int32_t seed
     if (need) { atom_add_ll(c.free_count, (long long)n); st.errors++; }
     if (n > 0) {
         if (out >= 0) st.allocs++; else st.failures++;
+    }
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return out;
+}
+
+/* Start of the first run of at least x clear bits of w, or -1. */
+__device__ __forceinline__ int first_run_at_least(u64 w, uint32_t x) {
+    u64 img = w;
+    for (;;) {
+        unsigned start = 0;
+        const unsigned len = first_zero_run(img, start);
+        if (len == 0) return -1;
+        if (len >= x) return (int)start;
+    }
+}
+
+/* getXPage of RandomWalk_BM (parallelPage.cu:1104-1142), one thread: x
+ * consecutive pages (1 <= x <= 64) inside one word.  Each step jumps to
+ * word (unsigned)seed % n_words and, while the word holds a run of x clear
+ * bits, claims the first x bits of its first such run with one atomicOr;
+ * bits another thread set first make it give back only the bits it set and
+ * look at the word again.  The reference claims x bits from the word's
+ * first clear bit whether or not they are clear, which repeats forever on a
+ * word whose first run is short, and its revert clears bits it did not set
+ * (:1124-1136).  Step = one jump (:1112).  As in get_n_pages, a run may not
+ * exist, so the walk is bounded: min(walk_budget, kNPageBudget) jumps, one
+ * sweep of the map, then failure (-1).  x == 0 returns 0, x > 64 returns -1. */
+__device__ inline int32_t get_x_pages(const pbm_ctx& c, uint32_t x, int32_t seed,
+                                      uint32_t* steps_out, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    uint32_t steps = 0;
+    int32_t out = -1;
+    if (x == 0) {
+        out = 0;
+    } else if (x <= 64u) {
+        const long long old = atom_add_ll(c.free_count, -(long long)x);
+        if (old >= (long long)x) {
+            const uint32_t L = c.n_words;
+            uint32_t g = 0;
+            auto try_word = [&](uint32_t w_id) -> bool {
+                u64 w = ld_word(&c.bitmap[w_id]);
+                st.probes++;
+                for (int start = first_run_at_least(w, x); start >= 0;
+                     start = first_run_at_least(w, x)) {
+                    const u64 mask = run_mask((unsigned)start, x);
+                    const u64 prev = atom_or(&c.bitmap[w_id], mask);
+                    st.atomics++;
+                    const u64 owned = mask & ~prev;
+                    if (owned == mask) {
+                        out = (int32_t)(w_id * 64u + (uint32_t)start);
+                        return true;
+                    }
+                    if (owned) {
+                        atom_and(&c.bitmap[w_id], ~owned);
+                        st.atomics++;
+                    }
+                    w = prev;
+                }
+                return false;
+            };
+            bool found = false;
+            const uint32_t budget = c.walk_budget < kNPageBudget ? c.walk_budget : kNPageBudget;
+            for (uint32_t j = 0; j < budget && !found; ++j) {
+                seed = lcg(seed);
+                g = (uint32_t)seed % L;
+                steps++;
+                found = try_word(g);
+            }
+            if (!found) {
+                st.fallbacks++;
+                for (uint32_t k = 0; k < L && !found; ++k) {
+                    g = (g + 1u == L) ? 0u : g + 1u;
+                    steps++;
+                    found = try_word(g);
+                }
+            }
+            if (!found) { atom_add_ll(c.free_count, (long long)x); st.errors++; }
+        } else {
+            atom_add_ll(c.free_count, (long long)x);
+        }
+        if (out >= 0) st.allocs++; else st.failures++;
+    } else {
+        st.failures++;
     }
     st.steps += steps;
     if (steps_out) *steps_out = steps;

@@ -163,6 +163,117 @@ uint32_t orc_rwbm_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, ui
     return granted;
 }
 
+/* ------------------------------------------------------ RandomWalkBasic */
+
+/* getPage of RandomWalkBasic (RandomWalkBasic.cuh:54-80) on the page map:
+ * each probe takes the next LCG state and page (unsigned)seed % n_pages;
+ * a step is one probe, the first included.  After `budget` probes the
+ * sweep of rwbm_walk starts after the last probed page's word. */
+uint32_t orc_rwbasic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                            uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* ids,
+                            uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    uint32_t granted = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t s = 0;
+        int32_t id = -1;
+        if (*free_count > 0) {
+            (*free_count)--;
+            int32_t sd = orc_request_seed(seed, req_base + r);
+            uint32_t p = 0;
+            for (uint32_t j = 0; j < budget && id < 0; ++j) {
+                sd = orc_lcg(sd);
+                p = (uint32_t)sd % n_pages;
+                s++;
+                const u64 bit = (u64)1 << (p & 63u);
+                if (!(bm[p >> 6] & bit)) {
+                    bm[p >> 6] |= bit;
+                    id = (int32_t)p;
+                }
+            }
+            if (id < 0) {
+                uint32_t g = p >> 6;
+                for (u64 k = 0; k < 4ull * L && id < 0; ++k) {
+                    g = (g + 1u == L) ? 0u : g + 1u;
+                    s++;
+                    id = claim_lowest(bm, g);
+                }
+            }
+            if (id < 0) (*free_count)++;
+        }
+        if (id >= 0) granted++;
+        ids[r] = id;
+        if (steps) steps[r] = s;
+    }
+    return granted;
+}
+
+/* Start of the first run of at least x clear bits of w, or -1. */
+static int first_run_at_least(u64 w, uint32_t x) {
+    uint32_t b = 0;
+    while (b < 64u) {
+        if ((w >> b) & 1u) { b++; continue; }
+        uint32_t e = b;
+        while (e < 64u && !((w >> e) & 1u)) e++;
+        if (e - b >= x) return (int)b;
+        b = e;
+    }
+    return -1;
+}
+
+/* Per-thread getXPage (parallelPage.cu:1104-1142, intended semantics):
+ * random words, the first x bits of the first run of >= x clear bits,
+ * step = one jump; after min(budget, 64) jumps one sweep of the map. */
+uint32_t orc_getx_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                         const uint32_t* counts, uint64_t req_base, uint64_t seed,
+                         uint32_t budget, int32_t* ids, uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    uint32_t got = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint32_t x = counts[r];
+        uint32_t s = 0;
+        int32_t id = -1;
+        if (x == 0) {
+            id = 0;
+        } else if (x <= 64u && *free_count >= (int64_t)x) {
+            *free_count -= x;
+            int32_t sd = orc_request_seed(seed, req_base + r);
+            uint32_t g = 0;
+            const uint32_t jumps = budget < 64u ? budget : 64u;
+            for (uint32_t j = 0; j < jumps && id < 0; ++j) {
+                sd = orc_lcg(sd);
+                g = (uint32_t)sd % L;
+                s++;
+                int st = first_run_at_least(bm[g], x);
+                if (st >= 0) {
+                    bm[g] |= (x >= 64u ? FULL : (((u64)1 << x) - 1u)) << st;
+                    id = (int32_t)(g * 64u + (uint32_t)st);
+                }
+            }
+            for (uint32_t k = 0; k < L && id < 0; ++k) {
+                g = (g + 1u == L) ? 0u : g + 1u;
+                s++;
+                int st = first_run_at_least(bm[g], x);
+                if (st >= 0) {
+                    bm[g] |= (x >= 64u ? FULL : (((u64)1 << x) - 1u)) << st;
+                    id = (int32_t)(g * 64u + (uint32_t)st);
+                }
+            }
+            if (id < 0) *free_count += x;
+        }
+        if (id >= 0 && x > 0) got++;
+        ids[r] = id;
+        if (steps) steps[r] = s;
+    }
+    return got;
+}
+
+/* RW_TAS (theoretical_models.py:20-21), the expected mean probes of n
+ * serial page-granular walks on T pages with A free: T/n * ln(A/(A-n)). */
+double orc_rw_tas(double T, double A, double n) {
+    return T / n * log(A / (A - n));
+}
+
 /* ------------------------------------------------------------ CollabRW */
 
 /* One collaborative step over a window.  probe[l]: lane l read word

@@ -124,6 +124,16 @@ void orc_clock_run(uint32_t n_logical, uint32_t capacity, uint32_t batch, const 
                    uint64_t n_refs, int32_t* slot_of, int32_t* owner, uint8_t* refbit,
                    uint32_t* arm_res, int32_t* evict_log, uint64_t log_cap, uint64_t* stats);
 
+/* RandomWalkBasic (single-page walk) serial, and its closed form RW_TAS. */
+uint32_t orc_rwbasic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                            uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* ids,
+                            uint32_t* steps);
+double orc_rw_tas(double T, double A, double n);
+
+uint32_t orc_getx_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                         const uint32_t* counts, uint64_t req_base, uint64_t seed,
+                         uint32_t budget, int32_t* ids, uint32_t* steps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,11 @@ _SIGS = {
                                        _PU32]),
     "orc_check_alloc": (C.c_int, [_PU64, _PU64, C.c_uint32, C.c_uint32, _P32, _PU32]),
     "orc_serve_payload": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
+    "orc_rwbasic_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                        C.c_uint64, C.c_uint32, _P32, _PU32]),
+    "orc_rw_tas": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+    "orc_getx_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32, C.c_uint64,
+                                     C.c_uint64, C.c_uint32, _P32, _PU32]),
     "orc_zipf_cdf": (None, [C.c_uint32, C.c_double, C.POINTER(C.c_double)]),
     "orc_zipf_trace": (None, [C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, _P32]),
     "orc_clock_run": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _P32, C.c_uint64, _P32, _P32,
@@ -147,6 +152,12 @@ class State:
                               req_base, seed, budget, _p(ids, _P32), _p(steps, _PU32))
         return ids, steps
 
+    def rwbasic(self, n: int, req_base: int, seed: int, budget: int = 1 << 20):
+        ids, steps = self._out(n)
+        lib().orc_rwbasic_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                 req_base, seed, budget, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps
+
     def collab(self, n: int, req_base: int, seed: int, order: int, budget: int = 1 << 20):
         ids, steps = self._out(n)
         lib().orc_collab_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
@@ -204,6 +215,15 @@ class State:
                                   _p(steps, _PU32))
         return ids, steps
 
+    def get_x(self, counts: np.ndarray, req_base: int, seed: int, budget: int = 1 << 20):
+        """Per-thread getXPage, requests in index order."""
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        ids, steps = self._out(counts.size)
+        lib().orc_getx_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), counts.size,
+                              _p(counts, _PU32), req_base, seed, budget, _p(ids, _P32),
+                              _p(steps, _PU32))
+        return ids, steps
+
     def free_n(self, ids: np.ndarray, counts: np.ndarray) -> int:
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         counts = np.ascontiguousarray(counts, dtype=np.uint32)
@@ -238,6 +258,10 @@ def check_alloc(before: np.ndarray, after: np.ndarray, n_pages: int, ids: np.nda
     rc = lib().orc_check_alloc(_p(before, _PU64), _p(after, _PU64), n_pages, ids.size,
                                _p(ids, _P32), C.byref(g))
     return int(rc), int(g.value)
+
+
+def rw_tas(T: float, A: float, n: float) -> float:
+    return float(lib().orc_rw_tas(T, A, n))
 
 
 def expected_steps(T: float, A: float, n: float, W: int = 64) -> float:

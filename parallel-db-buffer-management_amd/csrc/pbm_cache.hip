@@ -196,6 +196,7 @@ __device__ __forceinline__ int32_t cache_get(const pbm_ctx& c, uint32_t slot, in
     if (S == PBM_CRW_BM) return pbm::get_page_crw(c, slot, seed, nullptr, &st);
     if (S == PBM_RW_MALLOC) return pbm::get_page_rwmalloc(c, seed, nullptr, &st);
     if (S == PBM_SINGLE_CLOCK) return pbm::get_page_single_clock(c, nullptr, &st);
+    if (S == PBM_RW_BASIC) return pbm::get_page_rwbasic(c, seed, nullptr, &st);
     return pbm::get_page_dist_clock(c, slot, nullptr, &st);
 }
 
@@ -377,6 +378,7 @@ int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
         case PBM_CRW_BM: hipLaunchKernelGGL(k_cache_load<PBM_CRW_BM>, gb, b, 0, s, d); break;
         case PBM_RW_MALLOC: hipLaunchKernelGGL(k_cache_load<PBM_RW_MALLOC>, gb, b, 0, s, d); break;
         case PBM_SINGLE_CLOCK: hipLaunchKernelGGL(k_cache_load<PBM_SINGLE_CLOCK>, gb, b, 0, s, d); break;
+        case PBM_RW_BASIC: hipLaunchKernelGGL(k_cache_load<PBM_RW_BASIC>, gb, b, 0, s, d); break;
         default: hipLaunchKernelGGL(k_cache_load<PBM_DIST_CLOCK>, gb, b, 0, s, d); break;
     }
     hipLaunchKernelGGL(k_cache_finish, dim3(1), dim3(1), 0, s, d);

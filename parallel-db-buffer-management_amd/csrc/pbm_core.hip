@@ -150,7 +150,7 @@ static int validate(const pbm_config* cfg) {
     if (cfg->n_pages == 0 || cfg->n_pages > (1u << 30))
         return pbmi::fail(PBM_ERR_RANGE, "n_pages must be in [1, 2^30]");
     if (cfg->page_size < 8) return pbmi::fail(PBM_ERR_INVALID, "page_size must be >= 8");
-    if (cfg->strategy < PBM_RW_BM || cfg->strategy > PBM_DIST_CLOCK)
+    if (cfg->strategy < PBM_RW_BM || cfg->strategy > PBM_RW_BASIC)
         return pbmi::fail(PBM_ERR_INVALID, "unknown strategy");
     if (cfg->order != PBM_ORDER_PACKED && cfg->order != PBM_ORDER_REFERENCE)
         return pbmi::fail(PBM_ERR_INVALID, "unknown collaborative order");

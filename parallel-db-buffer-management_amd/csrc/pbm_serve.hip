@@ -134,6 +134,7 @@ __device__ __forceinline__ int32_t serve_get(const pbm_ctx& c, uint32_t slot, in
     if (S == PBM_RW_MALLOC) return pbm::get_page_rwmalloc(c, seed, nullptr, &st);
     if (S == PBM_SINGLE_CLOCK) return pbm::get_page_single_clock(c, nullptr, &st);
     if (S == PBM_DIST_CLOCK) return pbm::get_page_dist_clock(c, slot, nullptr, &st);
+    if (S == PBM_RW_BASIC) return pbm::get_page_rwbasic(c, seed, nullptr, &st);
     return pbm::get_page_list(c, nullptr, &st);
 }
 
@@ -442,6 +443,7 @@ int launch_step(pbm_server* s, hipStream_t st) {
         case PBM_RW_MALLOC: hipLaunchKernelGGL(k_serve_grow<PBM_RW_MALLOC>, g, b, 0, st, d); break;
         case PBM_SINGLE_CLOCK: hipLaunchKernelGGL(k_serve_grow<PBM_SINGLE_CLOCK>, g, b, 0, st, d); break;
         case PBM_DIST_CLOCK: hipLaunchKernelGGL(k_serve_grow<PBM_DIST_CLOCK>, g, b, 0, st, d); break;
+        case PBM_RW_BASIC: hipLaunchKernelGGL(k_serve_grow<PBM_RW_BASIC>, g, b, 0, st, d); break;
         default: hipLaunchKernelGGL(k_serve_grow<PBM_LINKED_LIST>, g, b, 0, st, d); break;
     }
     if (s->h->cfg.strategy == PBM_LINKED_LIST)

@@ -25,6 +25,7 @@ __global__ __launch_bounds__(1024) void k_get_thread(pbm_ctx c, uint32_t n, uint
         if (S == PBM_CRW_BM) id = pbm::get_page_crw(c, gid, seed, &steps, &st);
         else if (S == PBM_SINGLE_CLOCK) id = pbm::get_page_single_clock(c, &steps, &st);
         else if (S == PBM_DIST_CLOCK) id = pbm::get_page_dist_clock(c, gid, &steps, &st);
+        else if (S == PBM_RW_BASIC) id = pbm::get_page_rwbasic(c, seed, &steps, &st);
         else id = pbm::get_page_list(c, &steps, &st);
         ids[r] = id;
         if (steps_out) steps_out[r] = steps;
@@ -88,6 +89,10 @@ int launch_get_strategy(pbm_handle* h, const pbm_ctx& c, const pbm_launch& L, ui
             break;
         case PBM_LINKED_LIST:
             hipLaunchKernelGGL(k_get_thread<PBM_LINKED_LIST>, tgrid, tblk, 0, s, c, n, req_base,
+                               d_ids, d_steps);
+            break;
+        case PBM_RW_BASIC:
+            hipLaunchKernelGGL(k_get_thread<PBM_RW_BASIC>, tgrid, tblk, 0, s, c, n, req_base,
                                d_ids, d_steps);
             break;
         case PBM_RW_MALLOC: {
@@ -228,6 +233,23 @@ __global__ __launch_bounds__(1024) void k_free_n_pages(pbm_ctx c, uint32_t n,
     pbm::flush_stats(c, st);
 }
 
+// Per-thread getXPage, one request per loop iteration.
+__global__ __launch_bounds__(1024) void k_get_x_pages(pbm_ctx c, uint32_t n, uint64_t req_base,
+                                                      const uint32_t* __restrict__ counts,
+                                                      int32_t* __restrict__ ids,
+                                                      uint32_t* __restrict__ steps_out) {
+    pbm::Stats st;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
+        uint32_t steps = 0;
+        const int32_t id = pbm::get_x_pages(c, counts[r], pbm::request_seed(c.seed, req_base + r),
+                                            &steps, &st);
+        ids[r] = id;
+        if (steps_out) steps_out[r] = steps;
+    }
+    pbm::flush_stats(c, st);
+}
+
 __global__ __launch_bounds__(1024) void k_malloc(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                  const uint64_t* __restrict__ sizes,
                                                  uint64_t* __restrict__ ptrs,
@@ -307,6 +329,22 @@ extern "C" int pbm_get_n_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
     if (rc) return rc;
     hipLaunchKernelGGL(k_get_n_pages, wave_grid(L, n), wave_block(L), 0, pbmi::as_stream(stream),
                        c, n, req_base, d_counts, d_ids, d_steps);
+    PBM_HIP(hipGetLastError());
+    return PBM_OK;
+}
+
+extern "C" int pbm_get_x_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
+                               const uint32_t* d_counts, int32_t* d_ids, uint32_t* d_steps,
+                               const pbm_launch* launch, void* stream) {
+    PBM_CHECK_HANDLE(h);
+    if (n == 0) return PBM_OK;
+    if (!d_counts || !d_ids) return pbmi::fail(PBM_ERR_INVALID, "null d_counts or d_ids");
+    pbm_launch L;
+    pbm_ctx c;
+    int rc = prep(h, launch, &L, &c);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_get_x_pages, thread_grid(L, n), thread_block(L), 0,
+                       pbmi::as_stream(stream), c, n, req_base, d_counts, d_ids, d_steps);
     PBM_HIP(hipGetLastError());
     return PBM_OK;
 }

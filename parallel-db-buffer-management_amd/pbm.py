@@ -16,11 +16,11 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libpbm.so"
 
 # pbm_strategy
-RW_BM, COLLAB_RW_BM, CRW_BM, RW_MALLOC, LINKED_LIST, SINGLE_CLOCK, DIST_CLOCK = range(7)
+RW_BM, COLLAB_RW_BM, CRW_BM, RW_MALLOC, LINKED_LIST, SINGLE_CLOCK, DIST_CLOCK, RW_BASIC = range(8)
 STRATEGY_NAMES = {
     RW_BM: "RandomWalkBasic_BM", COLLAB_RW_BM: "CollabRW_BM", CRW_BM: "CRW_BM",
     RW_MALLOC: "RWMalloc", LINKED_LIST: "LinkedList", SINGLE_CLOCK: "SingleClock",
-    DIST_CLOCK: "DistClock",
+    DIST_CLOCK: "DistClock", RW_BASIC: "RandomWalkBasic",
 }
 # pbm_order
 ORDER_PACKED, ORDER_REFERENCE = 0, 1
@@ -141,6 +141,8 @@ _SIGS = {
     "pbm_free_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
                                  C.c_void_p]),
     "pbm_get_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.POINTER(Launch), C.c_void_p]),
+    "pbm_get_x_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.POINTER(Launch), C.c_void_p]),
     "pbm_free_n_pages": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p,
                                    C.POINTER(Launch), C.c_void_p]),
@@ -343,6 +345,12 @@ class Pool:
         check(lib().pbm_get_n_pages(self.h, n, req_base, _ptr(counts), _ptr(ids), _ptr(steps),
                                     C.byref(lnch or launch()), _stream(stream)),
               "pbm_get_n_pages")
+
+    def get_x_pages(self, n: int, counts, ids, steps=None, req_base: int = 0,
+                    lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_get_x_pages(self.h, n, req_base, _ptr(counts), _ptr(ids), _ptr(steps),
+                                    C.byref(lnch or launch()), _stream(stream)),
+              "pbm_get_x_pages")
 
     def free_n_pages(self, n: int, ids, counts, lnch: Launch | None = None, stream=None) -> None:
         check(lib().pbm_free_n_pages(self.h, n, _ptr(ids), _ptr(counts),

@@ -44,6 +44,20 @@ def test_serial_crw_matches_oracle(gpu):
         assert np.array_equal(pool.snapshot(), st.bm)
 
 
+@pytest.mark.parametrize("T,fill", [(1 << 16, 0.9), (1_000_003, 0.99), (777, 0.3)])
+def test_serial_rwbasic_matches_oracle(gpu, T, fill):
+    n = 5000 if T > 1000 else 400
+    log_case(f"serial RandomWalkBasic T={T} fill={fill}")
+    with pbm.Pool(T, strategy=pbm.RW_BASIC, seed=19) as pool:
+        pool.prefill_bernoulli(fill, 3)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_SERIAL, req_base=11)
+        st = orc.State(orc.prefill(T, fill, 3), T)
+        eids, esteps = st.rwbasic(n, 11, 19)
+        assert np.array_equal(ids, eids) and np.array_equal(steps, esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.free_count() == st.free[0]
+
+
 def test_serial_single_clock_matches_oracle(gpu):
     T, n = 1 << 16, 20000
     log_case("serial single clock")
@@ -131,6 +145,57 @@ def test_serial_getnpage_matches_oracle(gpu, fill, hi):
         assert pool.free_count() == st.free[0]
 
 
+@pytest.mark.parametrize("fill,hi", [(0.3, 16), (0.7, 4), (0.0, 64)])
+def test_serial_getxpage_matches_oracle(gpu, fill, hi):
+    T, n = 1 << 15, 2000
+    log_case(f"serial getXPage fill={fill} hi={hi}")
+    counts = _counts(np.random.default_rng(5), n, hi)
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=29) as pool:
+        pool.prefill_bernoulli(fill, 7)
+        d_counts = gpu.from_numpy(counts.view(np.int32)).cuda()
+        ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
+        steps = gpu.zeros((n,), dtype=gpu.int32, device="cuda")
+        pool.get_x_pages(n, d_counts, ids, steps, 13, pbm.launch(mode=pbm.MODE_SERIAL))
+        gpu.cuda.synchronize()
+        st = orc.State(orc.prefill(T, fill, 7), T)
+        eids, esteps = st.get_x(counts, 13, 29)
+        assert np.array_equal(ids.cpu().numpy(), eids)
+        assert np.array_equal(steps.cpu().numpy().view(np.uint32), esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.free_count() == st.free[0]
+        pool.free_n_pages(n, ids, d_counts)
+        gpu.cuda.synchronize()
+        assert np.array_equal(pool.snapshot(), orc.prefill(T, fill, 7))
+
+
+def test_concurrent_getxpage_runs_are_disjoint(gpu):
+    T, n = 1 << 18, 1 << 16
+    log_case("concurrent getXPage")
+    counts = _counts(np.random.default_rng(6), n, 12)
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=30) as pool:
+        pool.prefill_bernoulli(0.5, 8)
+        before = pool.snapshot()
+        d_counts = gpu.from_numpy(counts.view(np.int32)).cuda()
+        ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
+        pool.get_x_pages(n, d_counts, ids, None, 0, pbm.launch(mode=pbm.MODE_PLAIN))
+        gpu.cuda.synchronize()
+        got = ids.cpu().numpy()
+        used = np.unpackbits(before.view(np.uint8), bitorder="little").astype(bool)[:T]
+        seen = np.zeros(T, dtype=bool)
+        for i, x in zip(got, counts):
+            if x == 0 or i < 0:
+                continue
+            assert i // 64 == (i + int(x) - 1) // 64
+            assert not seen[i:i + x].any() and not used[i:i + x].any()
+            seen[i:i + x] = True
+        after = np.unpackbits(pool.snapshot().view(np.uint8), bitorder="little").astype(bool)[:T]
+        assert np.array_equal(after, used | seen)
+        assert pool.free_count() == orc.count_free(before, T) - int(seen.sum())
+        pool.free_n_pages(n, ids, d_counts)
+        gpu.cuda.synchronize()
+        assert np.array_equal(pool.snapshot(), before)
+
+
 def _sizes(rng, n):
     # log-uniform 16 B .. 64 KiB
     return np.exp(rng.uniform(np.log(16), np.log(65536), size=n)).astype(np.uint64)
@@ -166,7 +231,7 @@ def test_serial_malloc_matches_oracle(gpu):
 # ------------------------------------------------------------------ concurrent
 
 @pytest.mark.parametrize("strategy", [pbm.CRW_BM, pbm.SINGLE_CLOCK, pbm.DIST_CLOCK,
-                                      pbm.LINKED_LIST, pbm.RW_MALLOC])
+                                      pbm.LINKED_LIST, pbm.RW_MALLOC, pbm.RW_BASIC])
 @pytest.mark.parametrize("fill", [0.5, 0.9])
 def test_concurrent_cycle_invariants(gpu, strategy, fill):
     T, n = 1 << 18, 65536

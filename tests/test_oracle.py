@@ -797,3 +797,97 @@ def test_zipf_trace_matches_its_distribution():
     assert np.array_equal(orc.zipf_trace(n, s, 7, 100, 50), t[100:150])
     cdf = orc.zipf_cdf(n, s)
     assert np.isclose(cdf[-1], (np.arange(1, n + 1, dtype=np.float64) ** -s).sum(), rtol=1e-12)
+
+
+# ------------------------------------------------------------------ RandomWalkBasic (single pages)
+
+def test_rwbasic_first_probe_is_the_lcg_page():
+    T = 1000
+    st = orc.State(orc.cleared(T), T)
+    ids, steps = st.rwbasic(1, 5, 77)
+    x = orc.lcg(orc.request_seed(77, 5))
+    assert ids.tolist() == [(x & 0xFFFFFFFF) % T] and steps.tolist() == [1]
+
+
+def test_rwbasic_probes_follow_the_lcg_stream():
+    # a map with one free page: the walk probes the LCG stream until it
+    # lands on it; the step count is the index of that page in the stream
+    T = 37
+    bm = orc.prefill(T, 1.0, 0)
+    free = 23
+    bm[0] &= ~np.uint64(1 << free)
+    st = orc.State(bm, T)
+    ids, steps = st.rwbasic(1, 0, 9)
+    x, k = orc.request_seed(9, 0), 0
+    while True:
+        x = orc.lcg(x)
+        k += 1
+        if (x & 0xFFFFFFFF) % T == free:
+            break
+    assert ids.tolist() == [free] and steps.tolist() == [k]
+
+
+def test_rwbasic_mean_steps_match_exact_expectation_and_rw_tas():
+    # probes of the k-th grant are geometric with p = (A-k)/T; T is not a
+    # power of two, so the LCG's low bits do not fold the walk onto one cycle
+    T, N = 1_000_003, 60_000
+    bm = orc.prefill(T, 0.9, 21)
+    A = orc.count_free(bm, T)
+    st = orc.State(bm, T)
+    ids, steps = st.rwbasic(N, 0, 5)
+    rc, g = orc.check_alloc(bm, st.bm, T, ids)
+    assert rc == 0 and g == N
+    p = (A - np.arange(N)) / T
+    mean = (1 / p).mean()
+    sd = np.sqrt(((1 - p) / p ** 2).sum()) / N
+    assert abs(steps.mean() - mean) < 5 * sd
+    assert abs(orc.rw_tas(T, A, N) - mean) / mean < 2e-3
+
+
+# ------------------------------------------------------------------ getXPage (per thread)
+
+def _one_word(free_bits):
+    w = (1 << 64) - 1
+    for b in free_bits:
+        w &= ~(1 << b)
+    return np.array([w], dtype=np.uint64)
+
+
+def test_getx_takes_the_first_long_enough_run():
+    # free bits 1,2 and 5..9: x=3 skips the short run at 1
+    st = orc.State(_one_word([1, 2, 5, 6, 7, 8, 9]), 64)
+    ids, steps = st.get_x(np.array([3], dtype=np.uint32), 0, 1)
+    assert ids.tolist() == [5] and steps.tolist() == [1]
+    assert int(st.bm[0]) == int(_one_word([1, 2, 8, 9])[0])
+
+
+def test_getx_word_with_enough_free_bits_but_no_run_is_passed_over():
+    # bits 0, 5, 6 free and x = 2: the reference would claim from bit 0
+    # forever; here the run at 5 is taken
+    st = orc.State(_one_word([0, 5, 6]), 64)
+    ids, _ = st.get_x(np.array([2], dtype=np.uint32), 0, 1)
+    assert ids.tolist() == [5]
+    st = orc.State(_one_word([0, 5, 9]), 64)
+    ids, steps = st.get_x(np.array([2], dtype=np.uint32), 0, 1, budget=4)
+    assert ids.tolist() == [-1] and steps.tolist() == [5] and st.free[0] == 3
+
+
+def test_getx_runs_are_disjoint_and_inside_one_word():
+    T = 64 * 200
+    before = orc.prefill(T, 0.4, 3)
+    counts = np.random.default_rng(4).integers(0, 17, size=800).astype(np.uint32)
+    st = orc.State(before, T)
+    ids, _ = st.get_x(counts, 0, 8)
+    seen = np.zeros(T, dtype=bool)
+    bm_bits = np.unpackbits(before.view(np.uint8), bitorder="little").astype(bool)[:T]
+    for i, x in zip(ids, counts):
+        if x == 0:
+            assert i == 0
+            continue
+        if i < 0:
+            continue
+        assert i // 64 == (i + x - 1) // 64
+        assert not seen[i:i + x].any() and not bm_bits[i:i + x].any()
+        seen[i:i + x] = True
+    after = np.unpackbits(st.bm.view(np.uint8), bitorder="little").astype(bool)[:T]
+    assert np.array_equal(after, bm_bits | seen)
