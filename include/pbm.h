@@ -264,6 +264,26 @@ int pbm_free_ptrs(pbm_handle* h, uint32_t n, const uint64_t* d_ptrs, const pbm_l
 int pbm_page_address(pbm_handle* h, int32_t id, uint64_t* addr);
 int pbm_page_id(pbm_handle* h, uint64_t addr, int32_t* id);
 
+/* ---- observation ------------------------------------------------------ */
+
+/* Free-page clusters: maximal runs of free pages, the page after the last
+ * following the first (a fully free pool is one cluster).  Replaces the
+ * cluster count of printNumPagesLeftClusteredRandomWalk_kernel
+ * (parallelPage.cu:269-291), which reports 0 for a fully free pool. */
+int pbm_count_clusters(pbm_handle* h, uint64_t* clusters, void* stream);
+
+/* Samples the pool's free pages `count` times, gap_us microseconds apart,
+ * from one workgroup on `stream` (run it on a stream of its own to watch
+ * allocations on others).  d_samples (device, 2*count words) receives
+ * (wall-clock tick, free pages) pairs.  Replaces monitorBufferRandomWalk
+ * (parallelPage.cu:209-223), which prints from the device.  At most 60 s
+ * per call. */
+int pbm_monitor(pbm_handle* h, uint32_t gap_us, uint32_t count, unsigned long long* d_samples,
+                void* stream);
+
+/* Frequency of the device wall clock (wall_clock64), Hz. */
+int pbm_wall_clock_hz(int device, unsigned long long* hz);
+
 /* ---- graph-captured phases --------------------------------------------- */
 
 #define PBM_PHASE_ALLOC 1u

@@ -1267,3 +1267,19 @@ out:
     free(ones);
     free(ones_pos);
 }
+
+/* --------------------------------------------------- clusters */
+
+/* Maximal runs of free pages, circular; a fully free pool is one run. */
+uint64_t orc_count_clusters(const uint64_t* bm, uint32_t n_pages) {
+    uint64_t c = 0, free_pages = 0;
+    for (uint32_t i = 0; i < n_pages; ++i) {
+        const uint32_t prev = i ? i - 1u : n_pages - 1u;
+        const int f = !((bm[i >> 6] >> (i & 63u)) & 1u);
+        const int fp = !((bm[prev >> 6] >> (prev & 63u)) & 1u);
+        free_pages += (uint64_t)f;
+        if (f && !fp) c++;
+    }
+    if (c == 0 && free_pages == n_pages) c = 1;
+    return c;
+}

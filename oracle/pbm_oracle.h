@@ -134,6 +134,8 @@ uint32_t orc_getx_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, ui
                          const uint32_t* counts, uint64_t req_base, uint64_t seed,
                          uint32_t budget, int32_t* ids, uint32_t* steps);
 
+uint64_t orc_count_clusters(const uint64_t* bm, uint32_t n_pages);
+
 #ifdef __cplusplus
 }
 #endif

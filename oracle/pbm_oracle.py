@@ -62,6 +62,7 @@ _SIGS = {
     "orc_rwbasic_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
                                         C.c_uint64, C.c_uint32, _P32, _PU32]),
     "orc_rw_tas": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+    "orc_count_clusters": (C.c_uint64, [_PU64, C.c_uint32]),
     "orc_getx_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32, C.c_uint64,
                                      C.c_uint64, C.c_uint32, _P32, _PU32]),
     "orc_zipf_cdf": (None, [C.c_uint32, C.c_double, C.POINTER(C.c_double)]),
@@ -258,6 +259,11 @@ def check_alloc(before: np.ndarray, after: np.ndarray, n_pages: int, ids: np.nda
     rc = lib().orc_check_alloc(_p(before, _PU64), _p(after, _PU64), n_pages, ids.size,
                                _p(ids, _P32), C.byref(g))
     return int(rc), int(g.value)
+
+
+def count_clusters(bm: np.ndarray, n_pages: int) -> int:
+    bm = np.ascontiguousarray(bm, dtype=np.uint64)
+    return int(lib().orc_count_clusters(_p(bm, _PU64), n_pages))
 
 
 def rw_tas(T: float, A: float, n: float) -> float:

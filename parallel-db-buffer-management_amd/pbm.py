@@ -152,6 +152,9 @@ _SIGS = {
                                 C.c_void_p]),
     "pbm_page_address": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "pbm_page_id": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
+    "pbm_count_clusters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
+    "pbm_monitor": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "pbm_wall_clock_hz": (C.c_int, [C.c_int, C.POINTER(C.c_uint64)]),
     "pbm_graph_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
                                    C.POINTER(Launch), C.POINTER(Launch), C.c_uint32,
                                    C.POINTER(C.c_void_p)]),
@@ -339,6 +342,17 @@ class Pool:
     def free_pages(self, n: int, ids, lnch: Launch | None = None, stream=None) -> None:
         check(lib().pbm_free_pages(self.h, n, _ptr(ids), C.byref(lnch or launch()),
                                    _stream(stream)), "pbm_free_pages")
+
+    def count_clusters(self, stream=None) -> int:
+        v = C.c_uint64()
+        check(lib().pbm_count_clusters(self.h, C.byref(v), _stream(stream)), "pbm_count_clusters")
+        return int(v.value)
+
+    def monitor(self, gap_us: int, count: int, samples, stream=None) -> None:
+        """Launches the sampling monitor; samples: device int64 tensor of
+        2*count entries (tick, free pages)."""
+        check(lib().pbm_monitor(self.h, gap_us, count, _ptr(samples), _stream(stream)),
+              "pbm_monitor")
 
     def get_n_pages(self, n: int, counts, ids, steps=None, req_base: int = 0,
                     lnch: Launch | None = None, stream=None) -> None:
@@ -697,6 +711,12 @@ def serve_trace(n: int, rate_per_s: float, step_s: float, median: float, sigma: 
     target = np.rint(np.exp(np.log(median) + sigma * rng.standard_normal(n)))
     target = np.clip(target, 1, max_pages).astype(np.uint32)
     return arrival, target
+
+
+def wall_clock_hz(device: int = 0) -> int:
+    hz = C.c_uint64()
+    check(lib().pbm_wall_clock_hz(device, C.byref(hz)), "pbm_wall_clock_hz")
+    return int(hz.value)
 
 
 def atomic_peak(words: int = 16384, rounds: int = 64, device: int = 0) -> float:

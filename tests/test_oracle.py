@@ -891,3 +891,32 @@ def test_getx_runs_are_disjoint_and_inside_one_word():
         seen[i:i + x] = True
     after = np.unpackbits(st.bm.view(np.uint8), bitorder="little").astype(bool)[:T]
     assert np.array_equal(after, bm_bits | seen)
+
+
+# ------------------------------------------------------------------ free-page clusters
+
+def _bits(bm, T):
+    return np.unpackbits(np.ascontiguousarray(bm).view(np.uint8), bitorder="little")[:T]
+
+
+@pytest.mark.parametrize("T,fill", [(64, 0.5), (1000, 0.3), (70001, 0.97), (4096, 0.0),
+                                    (4096, 1.0), (1, 0.0), (1, 1.0)])
+def test_cluster_count_is_the_number_of_free_runs(T, fill):
+    bm = orc.prefill(T, fill, 5)
+    free = _bits(bm, T) == 0
+    if free.all():
+        runs = 1
+    else:
+        # rotate so the map starts at a used page; runs = rising edges
+        k = int(np.argmin(free))
+        f = np.roll(free, -k).astype(np.int8)
+        runs = int((np.diff(f) == 1).sum())
+    assert orc.count_clusters(bm, T) == runs
+
+
+def test_cluster_count_hand_cases():
+    T = 128
+    alt = np.array([0xAAAAAAAAAAAAAAAA, 0xAAAAAAAAAAAAAAAA], dtype=np.uint64)  # odd pages used
+    assert orc.count_clusters(alt, T) == 64
+    wrap = np.array([0xFFFFFFFFFFFFFFF0, 0x0FFFFFFFFFFFFFFF], dtype=np.uint64)  # free 0-3, 124-127
+    assert orc.count_clusters(wrap, T) == 1
