@@ -120,8 +120,8 @@ typedef struct pbm_ctx {
     uint32_t max_threads;
     int32_t  order;
     uint64_t seed;
-    uint32_t n_tiles;                /* rank-scan tiles: ceil(n_words / 256)            */
-    uint32_t reserved0;
+    uint32_t n_tiles;                /* rank-scan tiles: ceil(n_words / (256*scan_wpt)) */
+    uint32_t scan_wpt;               /* rank scan: map words per thread (1 or 4)        */
 } pbm_ctx;
 
 /* Launch shape of a batch call. */
@@ -139,7 +139,7 @@ typedef struct pbm_launch {
                                       serial oracle (orc_batched_serial)                  */
 #define PBM_LAUNCH_RANK        4u  /* batched mode by rank, one pass over the map: request r
                                       of the call takes the free page of rank r (map order);
-                                      each 256-word tile learns the free pages before it by
+                                      each tile (256 or 1024 words) learns the free pages before it by
                                       decoupled look-back and claims its granted bits with
                                       one atomicOr per word.                               */
 #define PBM_LAUNCH_COOPERATIVE 16u /* mallocRW bursts: every burst of the range in one

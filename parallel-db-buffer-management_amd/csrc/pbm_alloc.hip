@@ -61,7 +61,8 @@ __global__ __launch_bounds__(1024) void k_get_collab(pbm_ctx c, uint32_t n, uint
 
 // ---- batched allocation by rank, one pass over the map
 //
-// The map is cut into tiles of kScanWords words, one workgroup per tile,
+// The map is cut into tiles of kScanThreads * scan_wpt words, one
+// workgroup per tile (each thread holding scan_wpt consecutive words),
 // tiles numbered in the order the workgroups start (an atomic ticket), so
 // every tile's predecessors are already running.  A tile counts its free
 // pages, publishes the count, and finds the free pages before it by
@@ -71,8 +72,9 @@ __global__ __launch_bounds__(1024) void k_get_collab(pbm_ctx c, uint32_t n, uint
 // count) and passes it down the chain inside the same words.  Then the
 // free page of rank q < G belongs to request q: each word claims its
 // granted bits with one atomicOr and each wave writes ids[q] for its ranks
-// in index order.  A granted bit that a concurrent writer took between the
-// read and the claim sends its request to a short random walk.
+// in rank order (its own bits, lowest first).  A granted bit that a
+// concurrent writer took between the read and the claim sends its request
+// to a short random walk.
 //
 // Tickets count up across calls and a call takes exactly n_tiles of them,
 // so ticket t is tile t mod n_tiles of call t div n_tiles; the parity of
@@ -106,24 +108,18 @@ __device__ inline int32_t rank_fallback(const pbm_ctx& c, uint64_t req, uint32_t
     return id;
 }
 
+template <int WPT>
 __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                   int32_t* __restrict__ ids,
                                                   uint32_t* __restrict__ steps_out,
                                                   uint32_t budget) {
     using namespace pbm;
+    constexpr uint32_t kTileWords = kScanThreads * WPT;
     __shared__ uint32_t s_tile, s_parity, s_excl, s_grant;
     __shared__ long long s_free;
     __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_pre[4][64];
-    __shared__ u64 s_fm[4][64];
     const unsigned lane = lane_id();
     const unsigned wv = threadIdx.x >> 6;
-    // The words of tile blockIdx.x are read while the ticket is in flight:
-    // tickets mostly follow dispatch order, and a tile read early is still
-    // a valid snapshot (the claims check every bit).
-    const uint32_t w_spec = blockIdx.x * kScanWords + threadIdx.x;
-    u64 spec = kFull;
-    if (w_spec < c.n_words) spec = ld_word(&c.bitmap[w_spec]);
     if (threadIdx.x == 0) {
         const u64 t = atom_add_u64(&c.scan_ctl[0], 1ull);
         s_free = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -133,18 +129,28 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t parity = s_parity;
-    const uint32_t w = tile * kScanWords + threadIdx.x;
+    // thread t holds WPT consecutive words of the tile
+    const uint32_t w0 = tile * kTileWords + threadIdx.x * WPT;
     Stats st;
-    u64 fm = 0;
-    if (w < c.n_words) {
-        // padding bits past n_pages read as used
-        fm = ~(tile == blockIdx.x ? spec : ld_word(&c.bitmap[w]));
-        st.probes++;
+    u64 fm[WPT];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+        fm[k] = 0;
+        if (w0 + k < c.n_words) {
+            fm[k] = ~ld_word(&c.bitmap[w0 + k]);  // padding bits past n_pages read as used
+            st.probes++;
+        }
+        cnt += (uint32_t)__popcll(fm[k]);
     }
-    const uint32_t pc = (uint32_t)__popcll(fm);
-    unsigned wtot = 0;
-    const unsigned pre = scan7(pc, wtot);
-    if (lane == 0) s_wsum[wv] = wtot;
+    uint32_t inc = cnt;  // inclusive scan of the free counts over the wave
+#pragma unroll
+    for (unsigned d = 1; d < 64u; d <<= 1) {
+        const uint32_t v = shfl32(inc, lane >= d ? lane - d : lane);
+        if (lane >= d) inc += v;
+    }
+    const uint32_t pre = inc - cnt;
+    if (lane == 63) s_wsum[wv] = inc;
     __syncthreads();
     uint32_t agg = 0, woff = 0;
 #pragma unroll
@@ -211,72 +217,75 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
     }
     __syncthreads();
 
-    // claim the bits of rank < G in this thread's word; the result is
+    // claim the bits of rank < G in this thread's words; the results are
     // checked once the ids are out
     const uint32_t grant = s_grant;
-    const uint32_t base = s_excl + woff;  // rank of the wave's first free page
-    const uint32_t rk = base + pre;
-    uint32_t take = 0;
-    if (rk < grant) take = grant - rk < pc ? grant - rk : pc;
-    const u64 mask = lowest_bits(fm, take);
-    u64 old = 0;
-    if (mask) {
-        old = atom_or(&c.bitmap[w], mask);
-        st.atomics++;
-    }
-    unsigned T = 0;
-    s_pre[wv][lane] = scan7(take, T);
-    s_fm[wv][lane] = fm;
-    __syncthreads();
-
-    // ids in rank order: output q of the wave lies in the last word whose
-    // prefix is <= q
-    const uint32_t wbase = tile * kScanWords + wv * 64u;
-    for (unsigned q = lane; q < T; q += 64u) {
-        unsigned s = 0;
+    const uint32_t base = s_excl + woff + pre;  // rank of this thread's first free page
+    u64 mask[WPT], old[WPT];
+    uint32_t rk = base;
 #pragma unroll
-        for (unsigned d = 32; d >= 1; d >>= 1)
-            if (s_pre[wv][s + d] <= q) s += d;
-        const unsigned bit = select_bit(s_fm[wv][s], q - s_pre[wv][s]);
-        const uint32_t r = base + q;
-        ids[r] = (int32_t)((wbase + s) * 64u + bit);
-        if (steps_out) steps_out[r] = 1u;
-        st.allocs++;
-        st.steps++;
+    for (int k = 0; k < WPT; ++k) {
+        const uint32_t pc = (uint32_t)__popcll(fm[k]);
+        uint32_t take = 0;
+        if (rk < grant) take = grant - rk < pc ? grant - rk : pc;
+        mask[k] = lowest_bits(fm[k], take);
+        old[k] = 0;
+        if (mask[k]) {
+            old[k] = atom_or(&c.bitmap[w0 + k], mask[k]);
+            st.atomics++;
+        }
+        rk += pc;
     }
-    const u64 lost = mask & old;
-    if (lost) {
-        // a concurrent writer took granted bits: their requests walk, and
-        // overwrite what the loop above wrote (this wave's stores first)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        u64 m = lost;
+    // ids: the granted bits in rank order, each thread its own
+    uint32_t r = base;
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+        u64 m = mask[k];
+        const uint32_t wb = (w0 + k) * 64u;
         while (m) {
             const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
             m &= m - 1ull;
-            const uint32_t r = rk + (uint32_t)__popcll(fm & ((1ull << bit) - 1ull));
+            ids[r] = (int32_t)(wb + bit);
+            if (steps_out) steps_out[r] = 1u;
+            ++r;
+        }
+    }
+    st.allocs += r - base;
+    st.steps += r - base;
+    // a concurrent writer took granted bits: their requests walk and
+    // overwrite what was written above
+    uint32_t rb = base;
+#pragma unroll
+    for (int k = 0; k < WPT; ++k) {
+        u64 lost = mask[k] & old[k];
+        while (lost) {
+            const unsigned bit = (unsigned)__ffsll((long long)lost) - 1u;
+            lost &= lost - 1ull;
+            const uint32_t q = rb + (uint32_t)__popcll(fm[k] & ((1ull << bit) - 1ull));
             uint32_t steps = 1;
             st.fallbacks++;
             st.steps--;
-            const int32_t id = rank_fallback(c, req_base + r, budget, steps, st);
-            ids[r] = id;
-            if (steps_out) steps_out[r] = steps;
+            const int32_t id = rank_fallback(c, req_base + q, budget, steps, st);
+            ids[q] = id;
+            if (steps_out) steps_out[q] = steps;
             st.steps += steps;
             if (id < 0) {
                 st.allocs--;
                 st.failures++;
             }
         }
+        rb += (uint32_t)__popcll(fm[k]);
     }
 
     // requests reserved beyond the free pages the scan saw (a concurrent
     // free moved the count ahead of the map): the last tile walks for them
     if (tile + 1u == c.n_tiles) {
         const u64 seen = (u64)s_excl + agg;
-        for (u64 r = seen + threadIdx.x; r < grant; r += blockDim.x) {
+        for (u64 q = seen + threadIdx.x; q < grant; q += blockDim.x) {
             uint32_t steps = 0;
-            const int32_t id = rank_fallback(c, req_base + r, budget, steps, st);
-            ids[r] = id;
-            if (steps_out) steps_out[r] = steps;
+            const int32_t id = rank_fallback(c, req_base + q, budget, steps, st);
+            ids[q] = id;
+            if (steps_out) steps_out[q] = steps;
             if (id >= 0) st.allocs++; else st.failures++;
             st.steps += steps;
         }
@@ -287,9 +296,9 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
         const uint32_t per = (uint32_t)(((u64)rest + c.n_tiles - 1u) / c.n_tiles);
         const u64 lo = (u64)grant + (u64)tile * per;
         const u64 hi = lo + per < (u64)n ? lo + per : (u64)n;
-        for (u64 r = lo + threadIdx.x; r < hi; r += blockDim.x) {
-            ids[r] = -1;
-            if (steps_out) steps_out[r] = 0;
+        for (u64 q = lo + threadIdx.x; q < hi; q += blockDim.x) {
+            ids[q] = -1;
+            if (steps_out) steps_out[q] = 0;
             st.failures++;
         }
     }
@@ -590,8 +599,12 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         if (strat == PBM_LINKED_LIST)
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
         if (L.flags & PBM_LAUNCH_RANK) {
-            hipLaunchKernelGGL(k_get_rank, dim3(c.n_tiles), dim3(pbm::kScanWords), 0, s, c, n,
-                               req_base, d_ids, d_steps, c.walk_budget);
+            if (c.scan_wpt == 4)
+                hipLaunchKernelGGL(k_get_rank<4>, dim3(c.n_tiles), dim3(pbm::kScanThreads), 0, s, c,
+                                   n, req_base, d_ids, d_steps, c.walk_budget);
+            else
+                hipLaunchKernelGGL(k_get_rank<1>, dim3(c.n_tiles), dim3(pbm::kScanThreads), 0, s, c,
+                                   n, req_base, d_ids, d_steps, c.walk_budget);
             PBM_HIP(hipGetLastError());
             return PBM_OK;
         }

@@ -191,7 +191,11 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     PBM_ALLOC(h->d_counters, sizeof(pbm_counters));
     PBM_ALLOC(h->d_phase, 64u);
     PBM_ALLOC(h->d_scratch, 64u);
-    const uint32_t n_tiles = (n_words + pbm::kScanWords - 1u) / pbm::kScanWords;
+    // four words per thread once a map has 8,192 words: fewer, larger
+    // tiles shorten dispatch and the look-back chain
+    const uint32_t scan_wpt = n_words >= (1u << 20) ? 4u : 1u;
+    const uint32_t tile_words = pbm::kScanThreads * scan_wpt;
+    const uint32_t n_tiles = (n_words + tile_words - 1u) / tile_words;
     PBM_ALLOC(h->d_scan, (size_t)n_tiles * 8u);
     PBM_ALLOC(h->d_scan_ctl, 32u);
     if (cfg->backing) PBM_ALLOC(h->d_pool, (size_t)cfg->n_pages * cfg->page_size);
@@ -220,6 +224,7 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     c.scan = h->d_scan;
     c.scan_ctl = h->d_scan_ctl;
     c.n_tiles = n_tiles;
+    c.scan_wpt = scan_wpt;
     c.n_pages = cfg->n_pages;
     c.n_words = n_words;
     c.page_size = cfg->page_size;

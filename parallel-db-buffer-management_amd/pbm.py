@@ -59,7 +59,7 @@ class Ctx(C.Structure):
                 ("scan_ctl", C.c_void_p),
                 ("n_pages", C.c_uint32), ("n_words", C.c_uint32), ("page_size", C.c_uint32),
                 ("walk_budget", C.c_uint32), ("max_threads", C.c_uint32), ("order", C.c_int32),
-                ("seed", C.c_uint64), ("n_tiles", C.c_uint32), ("reserved0", C.c_uint32)]
+                ("seed", C.c_uint64), ("n_tiles", C.c_uint32), ("scan_wpt", C.c_uint32)]
 
 
 class Launch(C.Structure):
