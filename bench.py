@@ -375,7 +375,8 @@ def main() -> int:
         "alloc": 8 * ca["probes"] + 16 * ca["atomics"] + 4 * N_REQ,
         "free": 4 * N_REQ + 16 * (cc["atomics"] - ca["atomics"]),
     }
-    kernel_names = {"alloc": "k_get_rank" if rank else
+    wpt = pool.ctx().scan_wpt
+    kernel_names = {"alloc": f"k_get_rank<{wpt}>" if rank else
                     f"k_get_batched<{b['tuned']['cluster_words'] // 64}>",
                     "free": "k_free_batched"}
     dom = max(b["phases_ms"], key=b["phases_ms"].get)
@@ -386,7 +387,7 @@ def main() -> int:
     traffic = None
     tfile = ROOT / "profiles" / "traffic_c2_fill90.json"
     if tfile.exists():
-        shapes = {"k_get_rank": (L + 255) // 256}
+        shapes = {f"k_get_rank<{wpt}>": (L + 256 * wpt - 1) // (256 * wpt)}
         for k in json.loads(tfile.read_text())["kernels"]:
             if k["kernel"] == kernel_names[dom] and k["workgroups"] == shapes.get(k["kernel"]):
                 traffic = k["hbm_bytes"]
