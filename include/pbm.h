@@ -39,14 +39,16 @@ extern "C" {
  * included header (unitTests/unitTest1.cu:4-7); here it is a field of the
  * pool configuration. */
 typedef enum pbm_strategy {
-    PBM_RW_BM        = 0, /* RandomWalkBasic_BM.cuh:60-102   plain bitmap random walk      */
-    PBM_COLLAB_RW_BM = 1, /* CollabRW_BM.cuh:114-160         wave-collaborative window walk */
-    PBM_CRW_BM       = 2, /* CRW_BM.cuh:75-156               clustered walk with hints      */
-    PBM_RW_MALLOC    = 3, /* RWMalloc.cuh:113-334            collaborative walk + getNPage  */
-    PBM_LINKED_LIST  = 4, /* parallelPage.cu:36-46,300-466   FIFO free list                 */
-    PBM_SINGLE_CLOCK = 5, /* parallelPage.cu:470-535         one global clock arm           */
-    PBM_DIST_CLOCK   = 6, /* parallelPage.cu:537-618         per-thread clock arms          */
-    PBM_RW_BASIC     = 7  /* RandomWalkBasic.cuh:54-84       random walk over single pages  */
+    PBM_RW_BM           = 0, /* RandomWalkBasic_BM.cuh:60-102  plain bitmap random walk       */
+    PBM_COLLAB_RW_BM    = 1, /* CollabRW_BM.cuh:114-160        wave-collaborative window walk */
+    PBM_CRW_BM          = 2, /* CRW_BM.cuh:75-156              clustered walk with hints      */
+    PBM_RW_MALLOC       = 3, /* RWMalloc.cuh:113-334           collaborative walk + getNPage  */
+    PBM_LINKED_LIST     = 4, /* parallelPage.cu:36-46,300-466  FIFO free list                 */
+    PBM_SINGLE_CLOCK    = 5, /* parallelPage.cu:470-535        one global clock arm           */
+    PBM_DIST_CLOCK      = 6, /* parallelPage.cu:537-618        per-thread clock arms          */
+    PBM_RW_BASIC        = 7, /* RandomWalkBasic.cuh:54-84      random walk over single pages  */
+    PBM_CRW_BASIC       = 8, /* parallelPage.cu:200-224        clustered page walk, page hint */
+    PBM_COLLAB_RW_BASIC = 9  /* parallelPage.cu:642-685        wave-collaborative page walk   */
 } pbm_strategy;
 
 /* Order in which a collaborative step hands the free bits of its window to

@@ -104,31 +104,27 @@ static __device__ inline int32_t pbm_compat_seed() {
 }
 
 /* getPage (RandomWalkBasic_BM.cuh:60-89, RandomWalkBasic.cuh:54-80, CollabRW_BM.cuh:114-160,
- * CRW_BM.cuh:75-142, RWMalloc.cuh:113-149): a page id, or -1 when the pool
- * has none (the reference spins).  CollabRW and RWMalloc are collective
- * over the calling wave's active lanes; collectWarpData is accepted and
- * ignored. */
+ * CRW_BM.cuh:75-142, RWMalloc.cuh:113-149, and the legacy
+ * getPageClusteredRandomWalk / getPageCollabRW, parallelPage.cu:200-218,
+ * 642-680): a page id, or -1 when the pool has none (the reference spins).
+ * The collaborative policies and RWMalloc are collective over the calling
+ * wave's active lanes; collectWarpData is accepted and ignored. */
 static __device__ inline int getPage(int* stepCount = nullptr, bool collectWarpData = false) {
     (void)collectWarpData;
     uint32_t steps = 0;
-    int32_t id;
-    const pbm_ctx& c = pbm_compat_ctx;
-    if (PBM_COMPAT_STRATEGY == PBM_COLLAB_RW_BM)
-        id = pbm::get_page_collab(c, pbm_compat_seed(), &steps);
-    else if (PBM_COMPAT_STRATEGY == PBM_CRW_BM)
-        id = pbm::get_page_crw(c, blockIdx.x * blockDim.x + threadIdx.x, pbm_compat_seed(), &steps);
-    else if (PBM_COMPAT_STRATEGY == PBM_RW_MALLOC)
-        id = pbm::get_page_rwmalloc(c, pbm_compat_seed(), &steps);
-    else if (PBM_COMPAT_STRATEGY == PBM_RW_BASIC)
-        id = pbm::get_page_rwbasic(c, pbm_compat_seed(), &steps);
-    else
-        id = pbm::get_page_rwbm(c, pbm_compat_seed(), &steps);
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t id = pbm::get_page_of<PBM_COMPAT_STRATEGY>(pbm_compat_ctx, gid, pbm_compat_seed(),
+                                                             &steps, nullptr);
     if (stepCount) *stepCount = (int)steps;
     return id;
 }
 
-/* freePage (RandomWalkBasic_BM.cuh:98-102). */
-static __device__ inline void freePage(int pageID) { pbm::free_page(pbm_compat_ctx, pageID); }
+/* freePage (RandomWalkBasic_BM.cuh:98-102; the list pool's
+ * freePageLinkedList, parallelPage.cu:399-418, appends to the ring). */
+static __device__ inline void freePage(int pageID) {
+    if (PBM_COMPAT_STRATEGY == PBM_LINKED_LIST) pbm::free_page_list(pbm_compat_ctx, pageID);
+    else pbm::free_page(pbm_compat_ctx, pageID);
+}
 
 /* pageAddress / getPageID (Paging.cuh:30-52). */
 static __device__ inline void* pageAddress(int pageID) {

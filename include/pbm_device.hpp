@@ -339,6 +339,26 @@ __device__ inline int32_t get_page_rwbm(const pbm_ctx& c, int32_t seed, uint32_t
     return id;
 }
 
+/* Page walk of RandomWalkBasic from LCG state `seed`: probe pages
+ * (unsigned)lcg^k(seed) % n_pages until one is claimed, a step per probe;
+ * after walk_budget probes, sweep from the last probed page's word. */
+__device__ inline int32_t walk_pages(const pbm_ctx& c, int32_t seed, uint32_t& steps,
+                                     Stats& st) {
+    uint32_t p = 0;
+    for (uint32_t j = 0; j < c.walk_budget; ++j) {
+        seed = lcg(seed);
+        p = (uint32_t)seed % c.n_pages;
+        steps++;
+        st.probes++;
+        const u64 bit = 1ull << (p & 63u);
+        const u64 old = atom_or(&c.bitmap[p >> 6], bit);
+        st.atomics++;
+        if (!(old & bit)) return (int32_t)p;
+    }
+    st.fallbacks++;
+    return sweep_claim(c, p >> 6, steps, st, 4u);
+}
+
 /* getPage of RandomWalkBasic (RandomWalkBasic.cuh:54-80): a walk over
  * single pages.  Each probe takes the next LCG state, page
  * (unsigned)seed % n_pages, and claims it if its bit was clear; a step is
@@ -352,24 +372,7 @@ __device__ inline int32_t get_page_rwbasic(const pbm_ctx& c, int32_t seed, uint3
     uint32_t steps = 0;
     int32_t id = -1;
     if (reserve_one(c, true)) {
-        uint32_t p = 0;
-        for (uint32_t j = 0; j < c.walk_budget; ++j) {
-            seed = lcg(seed);
-            p = (uint32_t)seed % c.n_pages;
-            steps++;
-            st.probes++;
-            const u64 bit = 1ull << (p & 63u);
-            const u64 old = atom_or(&c.bitmap[p >> 6], bit);
-            st.atomics++;
-            if (!(old & bit)) {
-                id = (int32_t)p;
-                break;
-            }
-        }
-        if (id < 0) {
-            st.fallbacks++;
-            id = sweep_claim(c, p >> 6, steps, st, 4u);
-        }
+        id = walk_pages(c, seed, steps, st);
         if (id < 0) { atom_add_ll(c.free_count, 1); st.errors++; }
     }
     if (id >= 0) st.allocs++; else st.failures++;
@@ -1196,6 +1199,143 @@ __device__ inline void free_rw(const pbm_ctx& c, void* ptr, Stats* stats = nullp
     if (n == 0) return;
     int32_t id = page_id(c, (char*)ptr - 4);
     free_n_pages(c, id, (uint32_t)n, stats);
+}
+
+/* ---------------------------------------------------------------------
+ * Legacy page-map walks (parallelPage.cu:185-224, 642-685)
+ *
+ * The legacy allocators keep one int per page (parallelPage.cu:117-121,
+ * 633-638) and claim with atomicExch(map[p], 1).  Here the same state is
+ * one bit per page of the shared map, claimed with atomicOr, so every
+ * occupancy query, free and batch entry point works on these pools too.
+ * ------------------------------------------------------------------- */
+
+/* getPageClusteredRandomWalk (parallelPage.cu:200-218) for hint slot
+ * `slot`: first the page after the slot's last grant (wrapping at
+ * n_pages), taken if its bit is clear, which counts one step; otherwise the
+ * page walk of RandomWalkBasic, counted as that walk counts (the hint probe
+ * is not a step then).  The hint becomes the granted page.  Slots >=
+ * max_threads have no hint. */
+__device__ inline int32_t get_page_crw_basic(const pbm_ctx& c, uint32_t slot, int32_t seed,
+                                             uint32_t* steps_out, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    uint32_t steps = 0;
+    int32_t id = -1;
+    const bool has_slot = c.hint && slot < c.max_threads;
+    if (reserve_one(c, true)) {
+        if (has_slot) {
+            const int32_t h = c.hint[slot];
+            if (h >= 0) {
+                const uint32_t p = ((uint32_t)h + 1u >= c.n_pages) ? 0u : (uint32_t)h + 1u;
+                const u64 bit = 1ull << (p & 63u);
+                st.probes++;
+                if (!(ld_word(&c.bitmap[p >> 6]) & bit)) {
+                    const u64 old = atom_or(&c.bitmap[p >> 6], bit);
+                    st.atomics++;
+                    if (!(old & bit)) {
+                        id = (int32_t)p;
+                        steps = 1;
+                    }
+                }
+            }
+        }
+        if (id < 0) id = walk_pages(c, seed, steps, st);
+        if (id < 0) { atom_add_ll(c.free_count, 1); st.errors++; }
+        else if (has_slot) c.hint[slot] = id;
+    }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* getPageCollabRW (parallelPage.cu:642-680), wave-collective over the
+ * active lanes.  Every active lane walks its own LCG stream over single
+ * pages: in each round it probes page (unsigned)seed % n_pages (a page
+ * drawn by several lanes is probed by the lowest of them) and claims it if
+ * its bit is clear.  The pages won in a round go, in lane order, to the
+ * requesting lanes still without one, in lane order; pages beyond what the
+ * wave still needs are given back at once (:668-670).  Step = one round,
+ * the same for every requesting lane (:657-660).  The reference gathers
+ * the pages in a shared array and hands them out with atomicSub
+ * (:677-679); here they move by rank through registers.  After
+ * walk_budget rounds the requests still open sweep the map one lane at a
+ * time. */
+__device__ inline int32_t get_page_collab_basic(const pbm_ctx& c, int32_t seed,
+                                                uint32_t* steps_out, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const bool req = reserve_one(c, true);
+    const u64 reqm = ballot(req);
+    const unsigned n_req = (unsigned)__popcll(reqm);
+    const unsigned q = rank_in(reqm);
+    int32_t id = -1;
+    uint32_t rounds = 0, p = 0;
+    unsigned filled = 0;
+    while (filled < n_req && rounds < c.walk_budget) {
+        rounds++;
+        seed = lcg(seed);
+        p = (uint32_t)seed % c.n_pages;
+        bool probe = true;
+        dedupe_words(p, probe);
+        const u64 bit = 1ull << (p & 63u);
+        bool won = false;
+        if (probe) {
+            st.probes++;
+            if (!(ld_word(&c.bitmap[p >> 6]) & bit)) {
+                const u64 old = atom_or(&c.bitmap[p >> 6], bit);
+                st.atomics++;
+                won = !(old & bit);
+            }
+        }
+        const u64 wm = ballot(won);
+        const unsigned k = (unsigned)__popcll(wm);
+        const unsigned take = k < n_req - filled ? k : n_req - filled;
+        if (won && rank_in(wm) >= take) {
+            atom_and(&c.bitmap[p >> 6], ~bit);
+            st.atomics++;
+        }
+        const bool mine = req && q >= filled && q < filled + take;
+        const unsigned src = mine ? select_bit(wm, q - filled) : lane_id();
+        const uint32_t got = shfl32(p, src);
+        if (mine) id = (int32_t)got;
+        filled += take;
+    }
+    uint32_t steps = req ? rounds : 0u;
+    u64 open = ballot(req && id < 0);
+    while (open) {
+        const unsigned l = (unsigned)__ffsll((long long)open) - 1u;
+        open &= open - 1ull;
+        if (lane_id() == l) {
+            st.fallbacks++;
+            id = sweep_claim(c, p >> 6, steps, st, 4u);
+        }
+    }
+    if (req && id < 0) { atom_add_ll(c.free_count, 1); st.errors++; }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* The strategy's one-page getPage for the calling lane, chosen at compile
+ * time: `slot` is the hint or arm slot of per-thread policies, `seed` the
+ * request's walk seed.  Wave-collective strategies must be called by whole
+ * waves' active lanes together. */
+template <int S>
+__device__ __forceinline__ int32_t get_page_of(const pbm_ctx& c, uint32_t slot, int32_t seed,
+                                               uint32_t* steps_out, Stats* st) {
+    if (S == PBM_RW_BM) return get_page_rwbm(c, seed, steps_out, st);
+    if (S == PBM_COLLAB_RW_BM) return get_page_collab(c, seed, steps_out, st);
+    if (S == PBM_CRW_BM) return get_page_crw(c, slot, seed, steps_out, st);
+    if (S == PBM_RW_MALLOC) return get_page_rwmalloc(c, seed, steps_out, st);
+    if (S == PBM_SINGLE_CLOCK) return get_page_single_clock(c, steps_out, st);
+    if (S == PBM_DIST_CLOCK) return get_page_dist_clock(c, slot, steps_out, st);
+    if (S == PBM_RW_BASIC) return get_page_rwbasic(c, seed, steps_out, st);
+    if (S == PBM_CRW_BASIC) return get_page_crw_basic(c, slot, seed, steps_out, st);
+    if (S == PBM_COLLAB_RW_BASIC) return get_page_collab_basic(c, seed, steps_out, st);
+    return get_page_list(c, steps_out, st);
 }
 
 }  // namespace pbm

@@ -165,40 +165,46 @@ uint32_t orc_rwbm_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, ui
 
 /* ------------------------------------------------------ RandomWalkBasic */
 
-/* getPage of RandomWalkBasic (RandomWalkBasic.cuh:54-80) on the page map:
- * each probe takes the next LCG state and page (unsigned)seed % n_pages;
- * a step is one probe, the first included.  After `budget` probes the
- * sweep of rwbm_walk starts after the last probed page's word. */
+/* Page walk of RandomWalkBasic (RandomWalkBasic.cuh:54-80) from LCG state
+ * sd: each probe takes the next LCG state and page (unsigned)sd % n_pages
+ * and claims it if free; a step is one probe, the first included.  After
+ * `budget` probes the sweep of rwbm_walk starts after the last probed
+ * page's word. */
+static int32_t page_walk(uint64_t* bm, uint32_t n_pages, int32_t sd, uint32_t budget,
+                         uint32_t* s) {
+    const uint32_t L = orc_n_words(n_pages);
+    uint32_t p = 0;
+    for (uint32_t j = 0; j < budget; ++j) {
+        sd = orc_lcg(sd);
+        p = (uint32_t)sd % n_pages;
+        (*s)++;
+        const u64 bit = (u64)1 << (p & 63u);
+        if (!(bm[p >> 6] & bit)) {
+            bm[p >> 6] |= bit;
+            return (int32_t)p;
+        }
+    }
+    uint32_t g = p >> 6;
+    for (u64 k = 0; k < 4ull * L; ++k) {
+        g = (g + 1u == L) ? 0u : g + 1u;
+        (*s)++;
+        int32_t id = claim_lowest(bm, g);
+        if (id >= 0) return id;
+    }
+    return -1;
+}
+
+/* getPage of RandomWalkBasic for requests in index order. */
 uint32_t orc_rwbasic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                             uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* ids,
                             uint32_t* steps) {
-    const uint32_t L = orc_n_words(n_pages);
     uint32_t granted = 0;
     for (uint32_t r = 0; r < n; ++r) {
         uint32_t s = 0;
         int32_t id = -1;
         if (*free_count > 0) {
             (*free_count)--;
-            int32_t sd = orc_request_seed(seed, req_base + r);
-            uint32_t p = 0;
-            for (uint32_t j = 0; j < budget && id < 0; ++j) {
-                sd = orc_lcg(sd);
-                p = (uint32_t)sd % n_pages;
-                s++;
-                const u64 bit = (u64)1 << (p & 63u);
-                if (!(bm[p >> 6] & bit)) {
-                    bm[p >> 6] |= bit;
-                    id = (int32_t)p;
-                }
-            }
-            if (id < 0) {
-                uint32_t g = p >> 6;
-                for (u64 k = 0; k < 4ull * L && id < 0; ++k) {
-                    g = (g + 1u == L) ? 0u : g + 1u;
-                    s++;
-                    id = claim_lowest(bm, g);
-                }
-            }
+            id = page_walk(bm, n_pages, orc_request_seed(seed, req_base + r), budget, &s);
             if (id < 0) (*free_count)++;
         }
         if (id >= 0) granted++;
@@ -206,6 +212,111 @@ uint32_t orc_rwbasic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
         if (steps) steps[r] = s;
     }
     return granted;
+}
+
+/* getPageClusteredRandomWalk (parallelPage.cu:200-218) for one thread
+ * whose hint is *hint (last granted page, -1 none): the page after the
+ * hint (wrapping at n_pages) if free, one step; otherwise the page walk,
+ * counted as that walk counts.  The hint becomes the granted page. */
+uint32_t orc_crw_basic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                              uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* hint,
+                              int32_t* ids, uint32_t* steps) {
+    uint32_t got = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t s = 0;
+        int32_t id = -1;
+        if (*free_count > 0) {
+            (*free_count)--;
+            if (*hint >= 0) {
+                const uint32_t p = ((uint32_t)*hint + 1u >= n_pages) ? 0u : (uint32_t)*hint + 1u;
+                const u64 bit = (u64)1 << (p & 63u);
+                if (!(bm[p >> 6] & bit)) {
+                    bm[p >> 6] |= bit;
+                    id = (int32_t)p;
+                    s = 1;
+                }
+            }
+            if (id < 0) id = page_walk(bm, n_pages, orc_request_seed(seed, req_base + r), budget, &s);
+            if (id < 0) (*free_count)++;
+            else *hint = id;
+        }
+        if (id >= 0) got++;
+        ids[r] = id;
+        if (steps) steps[r] = s;
+    }
+    return got;
+}
+
+/* getPageCollabRW (parallelPage.cu:642-680) on one wave per chunk of 64
+ * requests.  The wave reserves min(active, free) pages for its lowest
+ * lanes.  Each round every active lane takes the next state of its own LCG
+ * stream and probes page (unsigned)state % n_pages (a page drawn by a lower
+ * lane in the same round is not probed again); free pages are claimed in
+ * lane order.  The j-th page won in the round goes to the reserved lane
+ * filled+j if that lane exists and is given back otherwise.  Rounds stop
+ * when every reserved lane has a page or after `budget` rounds; step = the
+ * rounds, for every reserved lane.  Reserved lanes still without a page
+ * then sweep, in lane order, from their last probed page's word. */
+uint32_t orc_collab_basic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                                 uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* ids,
+                                 uint32_t* steps) {
+    const uint32_t L = orc_n_words(n_pages);
+    uint32_t got = 0;
+    for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        const uint32_t n_act = (n - ch * 64u) < 64u ? (n - ch * 64u) : 64u;
+        int32_t sd[64], cid[64];
+        uint32_t p[64], cst[64];
+        int probe[64];
+        for (uint32_t l = 0; l < n_act; ++l) {
+            sd[l] = orc_request_seed(seed, req_base + ch * 64u + l);
+            cid[l] = -1;
+            cst[l] = 0;
+            p[l] = 0;
+        }
+        int64_t old = *free_count;
+        const uint32_t g = (uint32_t)(old < 0 ? 0 : (old > (int64_t)n_act ? (int64_t)n_act : old));
+        *free_count -= g;
+        uint32_t filled = 0, rounds = 0;
+        while (filled < g && rounds < budget) {
+            rounds++;
+            for (uint32_t l = 0; l < n_act; ++l) {
+                sd[l] = orc_lcg(sd[l]);
+                p[l] = (uint32_t)sd[l] % n_pages;
+            }
+            for (uint32_t l = 0; l < n_act; ++l) {
+                probe[l] = 1;
+                for (uint32_t k = 0; k < l; ++k)
+                    if (p[k] == p[l]) { probe[l] = 0; break; }
+            }
+            uint32_t j = 0;
+            for (uint32_t l = 0; l < n_act; ++l) {
+                if (!probe[l]) continue;
+                const u64 bit = (u64)1 << (p[l] & 63u);
+                if (bm[p[l] >> 6] & bit) continue;
+                bm[p[l] >> 6] |= bit;
+                if (filled + j < g) cid[filled + j] = (int32_t)p[l];
+                else bm[p[l] >> 6] &= ~bit;
+                j++;
+            }
+            filled += (j < g - filled) ? j : g - filled;
+        }
+        for (uint32_t l = 0; l < g; ++l) cst[l] = rounds;
+        for (uint32_t l = filled; l < g; ++l) {
+            uint32_t w = p[l] >> 6;
+            for (u64 k = 0; k < 4ull * L && cid[l] < 0; ++k) {
+                w = (w + 1u == L) ? 0u : w + 1u;
+                cst[l]++;
+                cid[l] = claim_lowest(bm, w);
+            }
+            if (cid[l] < 0) (*free_count)++;
+        }
+        for (uint32_t l = 0; l < n_act; ++l) {
+            ids[ch * 64u + l] = cid[l];
+            if (steps) steps[ch * 64u + l] = cst[l];
+            if (cid[l] >= 0) got++;
+        }
+    }
+    return got;
 }
 
 /* Start of the first run of at least x clear bits of w, or -1. */

@@ -128,6 +128,16 @@ void orc_clock_run(uint32_t n_logical, uint32_t capacity, uint32_t batch, const 
 uint32_t orc_rwbasic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                             uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* ids,
                             uint32_t* steps);
+/* getPageClusteredRandomWalk (parallelPage.cu:200-218), one thread with
+ * hint *hint. */
+uint32_t orc_crw_basic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                              uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* hint,
+                              int32_t* ids, uint32_t* steps);
+/* getPageCollabRW (parallelPage.cu:642-680), one wave per chunk of 64
+ * requests. */
+uint32_t orc_collab_basic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                                 uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* ids,
+                                 uint32_t* steps);
 double orc_rw_tas(double T, double A, double n);
 
 uint32_t orc_getx_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,

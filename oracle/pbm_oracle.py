@@ -61,6 +61,10 @@ _SIGS = {
     "orc_serve_payload": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
     "orc_rwbasic_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
                                         C.c_uint64, C.c_uint32, _P32, _PU32]),
+    "orc_crw_basic_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                          C.c_uint64, C.c_uint32, _P32, _P32, _PU32]),
+    "orc_collab_basic_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                             C.c_uint64, C.c_uint32, _P32, _PU32]),
     "orc_rw_tas": (C.c_double, [C.c_double, C.c_double, C.c_double]),
     "orc_count_clusters": (C.c_uint64, [_PU64, C.c_uint32]),
     "orc_getx_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32, C.c_uint64,
@@ -157,6 +161,23 @@ class State:
         ids, steps = self._out(n)
         lib().orc_rwbasic_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
                                  req_base, seed, budget, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps
+
+    def crw_basic(self, n: int, req_base: int, seed: int, hint: int = -1,
+                  budget: int = 1 << 20):
+        """Legacy ClusteredRandomWalk for one thread; returns ids, steps, final hint."""
+        ids, steps = self._out(n)
+        h = np.array([hint], dtype=np.int32)
+        lib().orc_crw_basic_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                   req_base, seed, budget, _p(h, _P32), _p(ids, _P32),
+                                   _p(steps, _PU32))
+        return ids, steps, int(h[0])
+
+    def collab_basic(self, n: int, req_base: int, seed: int, budget: int = 1 << 20):
+        """Legacy CollabRW, one wave per chunk of 64 requests."""
+        ids, steps = self._out(n)
+        lib().orc_collab_basic_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                      req_base, seed, budget, _p(ids, _P32), _p(steps, _PU32))
         return ids, steps
 
     def collab(self, n: int, req_base: int, seed: int, order: int, budget: int = 1 << 20):

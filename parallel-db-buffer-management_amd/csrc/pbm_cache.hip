@@ -188,18 +188,6 @@ __global__ __launch_bounds__(256) void k_cache_evict(CacheDev d) {
     pbm::flush_stats(d.c, st);
 }
 
-template <int S>
-__device__ __forceinline__ int32_t cache_get(const pbm_ctx& c, uint32_t slot, int32_t seed,
-                                             pbm::Stats& st) {
-    if (S == PBM_RW_BM) return pbm::get_page_rwbm(c, seed, nullptr, &st);
-    if (S == PBM_COLLAB_RW_BM) return pbm::get_page_collab(c, seed, nullptr, &st);
-    if (S == PBM_CRW_BM) return pbm::get_page_crw(c, slot, seed, nullptr, &st);
-    if (S == PBM_RW_MALLOC) return pbm::get_page_rwmalloc(c, seed, nullptr, &st);
-    if (S == PBM_SINGLE_CLOCK) return pbm::get_page_single_clock(c, nullptr, &st);
-    if (S == PBM_RW_BASIC) return pbm::get_page_rwbasic(c, seed, nullptr, &st);
-    return pbm::get_page_dist_clock(c, slot, nullptr, &st);
-}
-
 // Misses load: into empty slots first, then into the victims' slots.
 template <int S>
 __global__ __launch_bounds__(256) void k_cache_load(CacheDev d) {
@@ -212,7 +200,7 @@ __global__ __launch_bounds__(256) void k_cache_load(CacheDev d) {
         const int32_t l = d.miss[i];
         const int32_t s = i < empty ? (int32_t)(r0 + i) : d.victim[i - empty];
         const int32_t seed = pbm::request_seed(d.c.seed, (batch_no << 32) | i);
-        const int32_t f = cache_get<S>(d.c, i, seed, st);
+        const int32_t f = pbm::get_page_of<S>(d.c, i, seed, nullptr, &st);
         fail = f < 0;
         d.frame[s] = f;
         d.owner[s] = l;
@@ -373,13 +361,19 @@ int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
     hipLaunchKernelGGL(k_cache_clear, gc, b, 0, s, d);
     hipLaunchKernelGGL(k_cache_evict, gb, b, 0, s, d);
     switch (k->h->cfg.strategy) {
-        case PBM_RW_BM: hipLaunchKernelGGL(k_cache_load<PBM_RW_BM>, gb, b, 0, s, d); break;
-        case PBM_COLLAB_RW_BM: hipLaunchKernelGGL(k_cache_load<PBM_COLLAB_RW_BM>, gb, b, 0, s, d); break;
-        case PBM_CRW_BM: hipLaunchKernelGGL(k_cache_load<PBM_CRW_BM>, gb, b, 0, s, d); break;
-        case PBM_RW_MALLOC: hipLaunchKernelGGL(k_cache_load<PBM_RW_MALLOC>, gb, b, 0, s, d); break;
-        case PBM_SINGLE_CLOCK: hipLaunchKernelGGL(k_cache_load<PBM_SINGLE_CLOCK>, gb, b, 0, s, d); break;
-        case PBM_RW_BASIC: hipLaunchKernelGGL(k_cache_load<PBM_RW_BASIC>, gb, b, 0, s, d); break;
-        default: hipLaunchKernelGGL(k_cache_load<PBM_DIST_CLOCK>, gb, b, 0, s, d); break;
+#define PBM_LOAD_CASE(S) \
+    case S: hipLaunchKernelGGL(k_cache_load<S>, gb, b, 0, s, d); break;
+        PBM_LOAD_CASE(PBM_RW_BM)
+        PBM_LOAD_CASE(PBM_COLLAB_RW_BM)
+        PBM_LOAD_CASE(PBM_CRW_BM)
+        PBM_LOAD_CASE(PBM_RW_MALLOC)
+        PBM_LOAD_CASE(PBM_SINGLE_CLOCK)
+        PBM_LOAD_CASE(PBM_DIST_CLOCK)
+        PBM_LOAD_CASE(PBM_RW_BASIC)
+        PBM_LOAD_CASE(PBM_CRW_BASIC)
+        PBM_LOAD_CASE(PBM_COLLAB_RW_BASIC)
+#undef PBM_LOAD_CASE
+        default: return pbmi::fail(PBM_ERR_UNSUPPORTED, "no cache load for this strategy");
     }
     hipLaunchKernelGGL(k_cache_finish, dim3(1), dim3(1), 0, s, d);
     PBM_HIP(hipGetLastError());

@@ -150,7 +150,7 @@ static int validate(const pbm_config* cfg) {
     if (cfg->n_pages == 0 || cfg->n_pages > (1u << 30))
         return pbmi::fail(PBM_ERR_RANGE, "n_pages must be in [1, 2^30]");
     if (cfg->page_size < 8) return pbmi::fail(PBM_ERR_INVALID, "page_size must be >= 8");
-    if (cfg->strategy < PBM_RW_BM || cfg->strategy > PBM_RW_BASIC)
+    if (cfg->strategy < PBM_RW_BM || cfg->strategy > PBM_COLLAB_RW_BASIC)
         return pbmi::fail(PBM_ERR_INVALID, "unknown strategy");
     if (cfg->order != PBM_ORDER_PACKED && cfg->order != PBM_ORDER_REFERENCE)
         return pbmi::fail(PBM_ERR_INVALID, "unknown collaborative order");
@@ -199,7 +199,8 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     PBM_ALLOC(h->d_scan, (size_t)n_tiles * 8u);
     PBM_ALLOC(h->d_scan_ctl, 32u);
     if (cfg->backing) PBM_ALLOC(h->d_pool, (size_t)cfg->n_pages * cfg->page_size);
-    if (cfg->strategy == PBM_CRW_BM) PBM_ALLOC(h->d_hint, (size_t)h->cfg.max_threads * 4u);
+    if (cfg->strategy == PBM_CRW_BM || cfg->strategy == PBM_CRW_BASIC)
+        PBM_ALLOC(h->d_hint, (size_t)h->cfg.max_threads * 4u);
     if (cfg->strategy == PBM_DIST_CLOCK) PBM_ALLOC(h->d_arms, (size_t)h->cfg.max_threads * 4u);
     if (cfg->strategy == PBM_SINGLE_CLOCK) PBM_ALLOC(h->d_clock, 8u);
     if (cfg->strategy == PBM_LINKED_LIST) {

@@ -124,20 +124,6 @@ __global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) {
     }
 }
 
-// The strategy's device getPage for the calling lane of slot `slot`.
-template <int S>
-__device__ __forceinline__ int32_t serve_get(const pbm_ctx& c, uint32_t slot, int32_t seed,
-                                             pbm::Stats& st) {
-    if (S == PBM_RW_BM) return pbm::get_page_rwbm(c, seed, nullptr, &st);
-    if (S == PBM_COLLAB_RW_BM) return pbm::get_page_collab(c, seed, nullptr, &st);
-    if (S == PBM_CRW_BM) return pbm::get_page_crw(c, slot, seed, nullptr, &st);
-    if (S == PBM_RW_MALLOC) return pbm::get_page_rwmalloc(c, seed, nullptr, &st);
-    if (S == PBM_SINGLE_CLOCK) return pbm::get_page_single_clock(c, nullptr, &st);
-    if (S == PBM_DIST_CLOCK) return pbm::get_page_dist_clock(c, slot, nullptr, &st);
-    if (S == PBM_RW_BASIC) return pbm::get_page_rwbasic(c, seed, nullptr, &st);
-    return pbm::get_page_list(c, nullptr, &st);
-}
-
 // Grow: every admitted slot gets one page.  A sequence that reaches its
 // target is recorded as finished and its slot listed for release.
 template <int S>
@@ -156,7 +142,7 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
         const uint32_t tgt = d.slot_target[slot];
         const int32_t seed = request_seed(d.c.seed, (u64)seq * d.max_pages + len);
         const u64 t0 = wall_clock64();
-        const int32_t id = serve_get<S>(d.c, slot, seed, st);
+        const int32_t id = get_page_of<S>(d.c, slot, seed, nullptr, &st);
         const u64 t1 = wall_clock64();
         ticks = t1 - t0 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(t1 - t0);
         if (id >= 0) {
@@ -437,14 +423,20 @@ int launch_step(pbm_server* s, hipStream_t st) {
     hipLaunchKernelGGL(k_serve_admit, dim3(1), dim3(1024), 0, st, d);
     const dim3 g((d.max_slots + 255u) / 256u), b(256);
     switch (s->h->cfg.strategy) {
-        case PBM_RW_BM: hipLaunchKernelGGL(k_serve_grow<PBM_RW_BM>, g, b, 0, st, d); break;
-        case PBM_COLLAB_RW_BM: hipLaunchKernelGGL(k_serve_grow<PBM_COLLAB_RW_BM>, g, b, 0, st, d); break;
-        case PBM_CRW_BM: hipLaunchKernelGGL(k_serve_grow<PBM_CRW_BM>, g, b, 0, st, d); break;
-        case PBM_RW_MALLOC: hipLaunchKernelGGL(k_serve_grow<PBM_RW_MALLOC>, g, b, 0, st, d); break;
-        case PBM_SINGLE_CLOCK: hipLaunchKernelGGL(k_serve_grow<PBM_SINGLE_CLOCK>, g, b, 0, st, d); break;
-        case PBM_DIST_CLOCK: hipLaunchKernelGGL(k_serve_grow<PBM_DIST_CLOCK>, g, b, 0, st, d); break;
-        case PBM_RW_BASIC: hipLaunchKernelGGL(k_serve_grow<PBM_RW_BASIC>, g, b, 0, st, d); break;
-        default: hipLaunchKernelGGL(k_serve_grow<PBM_LINKED_LIST>, g, b, 0, st, d); break;
+#define PBM_GROW_CASE(S) \
+    case S: hipLaunchKernelGGL(k_serve_grow<S>, g, b, 0, st, d); break;
+        PBM_GROW_CASE(PBM_RW_BM)
+        PBM_GROW_CASE(PBM_COLLAB_RW_BM)
+        PBM_GROW_CASE(PBM_CRW_BM)
+        PBM_GROW_CASE(PBM_RW_MALLOC)
+        PBM_GROW_CASE(PBM_LINKED_LIST)
+        PBM_GROW_CASE(PBM_SINGLE_CLOCK)
+        PBM_GROW_CASE(PBM_DIST_CLOCK)
+        PBM_GROW_CASE(PBM_RW_BASIC)
+        PBM_GROW_CASE(PBM_CRW_BASIC)
+        PBM_GROW_CASE(PBM_COLLAB_RW_BASIC)
+#undef PBM_GROW_CASE
+        default: return pbmi::fail(PBM_ERR_UNSUPPORTED, "no serving launch for this strategy");
     }
     if (s->h->cfg.strategy == PBM_LINKED_LIST)
         hipLaunchKernelGGL(k_serve_release<true>, dim3(s->release_blocks), dim3(256), 0, st, d);

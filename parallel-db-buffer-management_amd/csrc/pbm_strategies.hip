@@ -1,5 +1,6 @@
 // Plain and serial launches of the remaining policies (CRW_BM, RWMalloc,
-// SingleClock, DistClock, LinkedList), the list ring rebuild, and the
+// SingleClock, DistClock, LinkedList, RandomWalkBasic and the legacy
+// clustered and collaborative page walks), the list ring rebuild, and the
 // multi-page entry points (getNPage, mallocRW, freeRW).
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -20,24 +21,21 @@ __global__ __launch_bounds__(1024) void k_get_thread(pbm_ctx c, uint32_t n, uint
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t r = gid; r < n; r += stride) {
         uint32_t steps = 0;
-        int32_t id;
         const int32_t seed = pbm::request_seed(c.seed, req_base + r);
-        if (S == PBM_CRW_BM) id = pbm::get_page_crw(c, gid, seed, &steps, &st);
-        else if (S == PBM_SINGLE_CLOCK) id = pbm::get_page_single_clock(c, &steps, &st);
-        else if (S == PBM_DIST_CLOCK) id = pbm::get_page_dist_clock(c, gid, &steps, &st);
-        else if (S == PBM_RW_BASIC) id = pbm::get_page_rwbasic(c, seed, &steps, &st);
-        else id = pbm::get_page_list(c, &steps, &st);
+        const int32_t id = pbm::get_page_of<S>(c, gid, seed, &steps, &st);
         ids[r] = id;
         if (steps_out) steps_out[r] = steps;
     }
     pbm::flush_stats(c, st);
 }
 
-// RWMalloc getPage: wave-collective over chunks of 64 requests; every lane
-// walks the LCG stream of its own request.
-__global__ __launch_bounds__(1024) void k_get_rwmalloc(pbm_ctx c, uint32_t n, uint64_t req_base,
-                                                       int32_t* __restrict__ ids,
-                                                       uint32_t* __restrict__ steps_out) {
+// Wave-collective policies whose lanes each walk their own request's LCG
+// stream (RWMalloc getPage, the legacy CollabRW): a wave serves chunks of
+// 64 consecutive requests.
+template <int S>
+__global__ __launch_bounds__(1024) void k_get_wave(pbm_ctx c, uint32_t n, uint64_t req_base,
+                                                   int32_t* __restrict__ ids,
+                                                   uint32_t* __restrict__ steps_out) {
     pbm::Stats st;
     const unsigned lane = pbm::lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -47,8 +45,8 @@ __global__ __launch_bounds__(1024) void k_get_rwmalloc(pbm_ctx c, uint32_t n, ui
         const uint32_t r = ch * 64u + lane;
         if (r < n) {
             uint32_t steps = 0;
-            int32_t id = pbm::get_page_rwmalloc(c, pbm::request_seed(c.seed, req_base + r), &steps,
-                                                &st);
+            const int32_t id = pbm::get_page_of<S>(c, 0u, pbm::request_seed(c.seed, req_base + r),
+                                                   &steps, &st);
             ids[r] = id;
             if (steps_out) steps_out[r] = steps;
         }
@@ -74,34 +72,29 @@ int launch_get_strategy(pbm_handle* h, const pbm_ctx& c, const pbm_launch& L, ui
     const bool serial = L.mode == PBM_MODE_SERIAL;
     const dim3 tgrid = serial ? dim3(1) : dim3((n + L.block - 1) / L.block);
     const dim3 tblk = serial ? dim3(1) : dim3(L.block);
+    const uint32_t waves = (n + 63u) / 64u;
+    const dim3 wgrid = serial ? dim3(1) : dim3((waves * 64u + L.block - 1) / L.block);
+    const dim3 wblk = serial ? dim3(64) : dim3(L.block);
     switch (h->cfg.strategy) {
-        case PBM_CRW_BM:
-            hipLaunchKernelGGL(k_get_thread<PBM_CRW_BM>, tgrid, tblk, 0, s, c, n, req_base, d_ids,
+#define PBM_THREAD_CASE(S)                                                                      \
+    case S:                                                                                     \
+        hipLaunchKernelGGL(k_get_thread<S>, tgrid, tblk, 0, s, c, n, req_base, d_ids, d_steps); \
+        break;
+        PBM_THREAD_CASE(PBM_CRW_BM)
+        PBM_THREAD_CASE(PBM_SINGLE_CLOCK)
+        PBM_THREAD_CASE(PBM_DIST_CLOCK)
+        PBM_THREAD_CASE(PBM_LINKED_LIST)
+        PBM_THREAD_CASE(PBM_RW_BASIC)
+        PBM_THREAD_CASE(PBM_CRW_BASIC)
+#undef PBM_THREAD_CASE
+        case PBM_RW_MALLOC:
+            hipLaunchKernelGGL(k_get_wave<PBM_RW_MALLOC>, wgrid, wblk, 0, s, c, n, req_base, d_ids,
                                d_steps);
             break;
-        case PBM_SINGLE_CLOCK:
-            hipLaunchKernelGGL(k_get_thread<PBM_SINGLE_CLOCK>, tgrid, tblk, 0, s, c, n, req_base,
+        case PBM_COLLAB_RW_BASIC:
+            hipLaunchKernelGGL(k_get_wave<PBM_COLLAB_RW_BASIC>, wgrid, wblk, 0, s, c, n, req_base,
                                d_ids, d_steps);
             break;
-        case PBM_DIST_CLOCK:
-            hipLaunchKernelGGL(k_get_thread<PBM_DIST_CLOCK>, tgrid, tblk, 0, s, c, n, req_base,
-                               d_ids, d_steps);
-            break;
-        case PBM_LINKED_LIST:
-            hipLaunchKernelGGL(k_get_thread<PBM_LINKED_LIST>, tgrid, tblk, 0, s, c, n, req_base,
-                               d_ids, d_steps);
-            break;
-        case PBM_RW_BASIC:
-            hipLaunchKernelGGL(k_get_thread<PBM_RW_BASIC>, tgrid, tblk, 0, s, c, n, req_base,
-                               d_ids, d_steps);
-            break;
-        case PBM_RW_MALLOC: {
-            uint32_t waves = (n + 63u) / 64u;
-            dim3 grid = serial ? dim3(1) : dim3((waves * 64u + L.block - 1) / L.block);
-            dim3 blk = serial ? dim3(64) : dim3(L.block);
-            hipLaunchKernelGGL(k_get_rwmalloc, grid, blk, 0, s, c, n, req_base, d_ids, d_steps);
-            break;
-        }
         default:
             return fail(PBM_ERR_UNSUPPORTED, "strategy has no one-page launch here");
     }

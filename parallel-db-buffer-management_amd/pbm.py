@@ -16,11 +16,13 @@ PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libpbm.so"
 
 # pbm_strategy
-RW_BM, COLLAB_RW_BM, CRW_BM, RW_MALLOC, LINKED_LIST, SINGLE_CLOCK, DIST_CLOCK, RW_BASIC = range(8)
+(RW_BM, COLLAB_RW_BM, CRW_BM, RW_MALLOC, LINKED_LIST, SINGLE_CLOCK, DIST_CLOCK, RW_BASIC,
+ CRW_BASIC, COLLAB_RW_BASIC) = range(10)
 STRATEGY_NAMES = {
     RW_BM: "RandomWalkBasic_BM", COLLAB_RW_BM: "CollabRW_BM", CRW_BM: "CRW_BM",
     RW_MALLOC: "RWMalloc", LINKED_LIST: "LinkedList", SINGLE_CLOCK: "SingleClock",
     DIST_CLOCK: "DistClock", RW_BASIC: "RandomWalkBasic",
+    CRW_BASIC: "ClusteredRandomWalk", COLLAB_RW_BASIC: "CollabRW",
 }
 # pbm_order
 ORDER_PACKED, ORDER_REFERENCE = 0, 1

@@ -56,7 +56,8 @@ def build_library(jobs: int = 8) -> Path:
 
 COMPAT_SRC = ROOT / "tests" / "compat" / "compat_check.hip"
 COMPAT_STRATEGIES = {"rw_bm": "PBM_RW_BM", "collab": "PBM_COLLAB_RW_BM", "crw": "PBM_CRW_BM",
-                     "rw_malloc": "PBM_RW_MALLOC", "rw_basic": "PBM_RW_BASIC"}
+                     "rw_malloc": "PBM_RW_MALLOC", "rw_basic": "PBM_RW_BASIC",
+                     "crw_basic": "PBM_CRW_BASIC", "collab_basic": "PBM_COLLAB_RW_BASIC"}
 
 
 def compat_binary(name: str) -> Path:

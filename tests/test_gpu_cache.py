@@ -56,7 +56,8 @@ def test_cache_matches_oracle(gpu, B, n):
             _check(gpu, pool, cache, cc, T)
 
 
-@pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.DIST_CLOCK, pbm.COLLAB_RW_BM])
+@pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.DIST_CLOCK, pbm.COLLAB_RW_BM,
+                                      pbm.CRW_BASIC, pbm.COLLAB_RW_BASIC])
 def test_cache_frame_policies(gpu, strategy):
     T, L, C, B, n = 1 << 15, 1 << 13, 4096, 512, 200000
     log_case(f"cache frames strategy={strategy}")

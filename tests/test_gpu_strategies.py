@@ -1,5 +1,6 @@
-"""Parity of CRW_BM, SingleClock, DistClock, LinkedList and RWMalloc
-(getPage, getNPage, mallocRW/freeRW) with the serial oracle, through the
+"""Parity of CRW_BM, SingleClock, DistClock, LinkedList, RWMalloc
+(getPage, getNPage, mallocRW/freeRW), RandomWalkBasic and the legacy
+clustered and collaborative page walks with the serial oracle, through the
 C-ABI.  Serial launches must match the oracle bit for bit; concurrent ones
 are checked on the schedule-independent invariants."""
 import numpy as np
@@ -56,6 +57,56 @@ def test_serial_rwbasic_matches_oracle(gpu, T, fill):
         assert np.array_equal(ids, eids) and np.array_equal(steps, esteps)
         assert np.array_equal(pool.snapshot(), st.bm)
         assert pool.free_count() == st.free[0]
+
+
+@pytest.mark.parametrize("T,fill", [(1 << 16, 0.9), (777, 0.3), (1_000_003, 0.99)])
+def test_serial_crw_basic_matches_oracle(gpu, T, fill):
+    n = 5000 if T > 1000 else 400
+    log_case(f"serial ClusteredRandomWalk T={T} fill={fill}")
+    with pbm.Pool(T, strategy=pbm.CRW_BASIC, seed=37) as pool:
+        pool.prefill_bernoulli(fill, 4)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_SERIAL, req_base=5)
+        st = orc.State(orc.prefill(T, fill, 4), T)
+        eids, esteps, hint = st.crw_basic(n, 5, 37)
+        assert np.array_equal(ids, eids) and np.array_equal(steps, esteps)
+        # the hint of slot 0 carries into the next call
+        ids2, steps2 = _get(gpu, pool, 300, pbm.MODE_SERIAL, req_base=n + 5)
+        eids2, esteps2, _ = st.crw_basic(300, n + 5, 37, hint=hint)
+        assert np.array_equal(ids2, eids2) and np.array_equal(steps2, esteps2)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.free_count() == st.free[0]
+
+
+@pytest.mark.parametrize("T,fill,n", [(1 << 16, 0.9, 6000), (4096, 0.5, 3000), (100, 0.0, 130),
+                                      (1_000_003, 0.99, 20000)])
+def test_serial_collab_basic_matches_oracle(gpu, T, fill, n):
+    log_case(f"serial CollabRW T={T} fill={fill} n={n}")
+    with pbm.Pool(T, strategy=pbm.COLLAB_RW_BASIC, seed=41) as pool:
+        pool.prefill_bernoulli(fill, 6)
+        ids, steps = _get(gpu, pool, n, pbm.MODE_SERIAL, req_base=9)
+        st = orc.State(orc.prefill(T, fill, 6), T)
+        eids, esteps = st.collab_basic(n, 9, 41)
+        assert np.array_equal(ids, eids) and np.array_equal(steps, esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.free_count() == st.free[0]
+
+
+def test_serial_collab_basic_fallback_matches_oracle(gpu):
+    # a budget of 2 rounds on a 99%-full pool leaves most lanes to the sweep
+    T, n = 1 << 14, 640
+    log_case("serial CollabRW sweep fallback")
+    with pbm.Pool(T, strategy=pbm.COLLAB_RW_BASIC, seed=43) as pool:
+        pool.prefill_bernoulli(0.99, 8)
+        ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
+        steps = gpu.zeros((n,), dtype=gpu.int32, device="cuda")
+        pool.get_pages(n, ids, steps, 0, pbm.launch(mode=pbm.MODE_SERIAL, walk_budget=2))
+        gpu.cuda.synchronize()
+        st = orc.State(orc.prefill(T, 0.99, 8), T)
+        eids, esteps = st.collab_basic(n, 0, 43, budget=2)
+        assert np.array_equal(ids.cpu().numpy(), eids)
+        assert np.array_equal(steps.cpu().numpy().view(np.uint32), esteps)
+        assert np.array_equal(pool.snapshot(), st.bm)
+        assert pool.counters()["fallbacks"] > 0
 
 
 def test_serial_single_clock_matches_oracle(gpu):
@@ -231,7 +282,8 @@ def test_serial_malloc_matches_oracle(gpu):
 # ------------------------------------------------------------------ concurrent
 
 @pytest.mark.parametrize("strategy", [pbm.CRW_BM, pbm.SINGLE_CLOCK, pbm.DIST_CLOCK,
-                                      pbm.LINKED_LIST, pbm.RW_MALLOC, pbm.RW_BASIC])
+                                      pbm.LINKED_LIST, pbm.RW_MALLOC, pbm.RW_BASIC,
+                                      pbm.CRW_BASIC, pbm.COLLAB_RW_BASIC])
 @pytest.mark.parametrize("fill", [0.5, 0.9])
 def test_concurrent_cycle_invariants(gpu, strategy, fill):
     T, n = 1 << 18, 65536

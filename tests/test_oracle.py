@@ -844,6 +844,112 @@ def test_rwbasic_mean_steps_match_exact_expectation_and_rw_tas():
     assert abs(orc.rw_tas(T, A, N) - mean) / mean < 2e-3
 
 
+# ------------------------------------------------------------------ legacy page walks
+
+def _page(x: int, T: int) -> int:
+    return (x & 0xFFFFFFFF) % T
+
+
+def test_crw_basic_single_thread_takes_consecutive_pages_and_wraps():
+    # empty pool: the first request walks (one probe), every later one takes
+    # the page after the hint, wrapping from T-1 to 0 (parallelPage.cu:206-207)
+    T = 1000
+    st = orc.State(orc.cleared(T), T)
+    ids, steps, hint = st.crw_basic(40, 3, 11)
+    p0 = _page(orc.lcg(orc.request_seed(11, 3)), T)
+    assert ids.tolist() == [(p0 + k) % T for k in range(40)]
+    assert steps.tolist() == [1] * 40 and hint == ids[-1]
+    st2 = orc.State(orc.cleared(T), T)
+    ids2, steps2, _ = st2.crw_basic(3, 0, 11, hint=T - 1)
+    assert ids2.tolist() == [0, 1, 2] and steps2.tolist() == [1, 1, 1]
+
+
+def test_crw_basic_falls_back_to_the_page_walk_when_the_next_page_is_taken():
+    T = 4099
+    bm = orc.prefill(T, 0.5, 3)
+    h = next(i for i in range(T - 1) if (int(bm[(i + 1) // 64]) >> ((i + 1) % 64)) & 1)
+    st = orc.State(bm, T)
+    ids, steps, hint = st.crw_basic(1, 7, 19, hint=h)
+    ref = orc.State(bm, T)
+    rids, rsteps = ref.rwbasic(1, 7, 19)      # RandomWalkBasic from the same seed
+    assert ids.tolist() == rids.tolist() and steps.tolist() == rsteps.tolist()
+    assert hint == ids[0] and np.array_equal(st.bm, ref.bm)
+
+
+def test_collab_basic_one_lane_is_the_page_walk():
+    # a wave with one request walks exactly as RandomWalkBasic does
+    T = 1_000_003
+    bm = orc.prefill(T, 0.9, 5)
+    for r in range(20):
+        a, b = orc.State(bm, T), orc.State(bm, T)
+        ids, steps = a.collab_basic(1, r, 13)
+        rids, rsteps = b.rwbasic(1, r, 13)
+        assert ids.tolist() == rids.tolist() and steps.tolist() == rsteps.tolist()
+
+
+def test_collab_basic_empty_pool_first_round_serves_every_lane():
+    T = 1 << 20
+    st = orc.State(orc.cleared(T), T)
+    ids, steps = st.collab_basic(64, 0, 17)
+    first = [_page(orc.lcg(orc.request_seed(17, l)), T) for l in range(64)]
+    assert len(set(first)) == 64
+    assert ids.tolist() == first and steps.tolist() == [1] * 64
+
+
+def test_collab_basic_two_pages_go_to_the_first_lanes_to_draw_them():
+    # two free pages, 64 lanes: lane 0 gets the page lane 0 draws; lane 1 the
+    # other page, in the first round some lane draws it; the rest get none
+    T = 2
+    st = orc.State(orc.cleared(T), T)
+    ids, steps = st.collab_basic(64, 0, 23)
+    x = [orc.request_seed(23, l) for l in range(64)]
+    rounds, got = 0, []
+    while len(got) < 2:
+        rounds += 1
+        x = [orc.lcg(v) for v in x]
+        for v in x:
+            p = _page(v, T)
+            if p not in got:
+                got.append(p)
+    assert ids[:2].tolist() == got and (ids[2:] == -1).all()
+    assert steps[:2].tolist() == [rounds] * 2 and (steps[2:] == 0).all()
+    assert st.free[0] == 0
+
+
+def test_collab_basic_short_pool_hands_out_exactly_the_free_pages():
+    T = 1 << 12
+    bm = orc.prefill(T, 0.99, 9)
+    A = orc.count_free(bm, T)
+    assert 0 < A < 64
+    st = orc.State(bm, T)
+    ids, steps = st.collab_basic(64, 0, 29)
+    free_set = [i for i in range(T) if not (int(bm[i // 64]) >> (i % 64)) & 1]
+    assert sorted(ids[:A].tolist()) == free_set and (ids[A:] == -1).all()
+    assert len(set(steps[:A].tolist())) == 1 and (steps[A:] == 0).all()
+    assert orc.count_free(st.bm, T) == 0 and st.free[0] == 0
+
+
+def test_collab_basic_rounds_match_the_collaborative_model():
+    # 64 lanes probing single pages at free share p need about 1/p rounds
+    # (64 successes at 64 p per round) plus half a round of overshoot; wave
+    # c of the run sees A - 64 c free pages.  The reference's model
+    # (CoRWBM_WAS with W = 1, theoretical_models.py:54-62) takes the free
+    # share left at the end of the run and bounds the mean from above.
+    T, n = 1_000_003, 64 * 300
+    bm = orc.prefill(T, 0.9, 31)
+    A = orc.count_free(bm, T)
+    st = orc.State(bm, T)
+    ids, steps = st.collab_basic(n, 0, 3)
+    rc, g = orc.check_alloc(bm, st.bm, T, ids)
+    assert rc == 0 and g == n
+    per_wave = steps.reshape(-1, 64)
+    assert (per_wave == per_wave[:, :1]).all()
+    expect = (T / (A - 64 * np.arange(n // 64))).mean() + 0.5
+    assert abs(steps.mean() - expect) < 0.4
+    from models import CoRWBM_WAS
+    assert steps.mean() < CoRWBM_WAS(T, A, n, W=1, lanes=64) + 0.5
+
+
 # ------------------------------------------------------------------ getXPage (per thread)
 
 def _one_word(free_bits):
