@@ -266,6 +266,52 @@ int pbm_free_ptrs(pbm_handle* h, uint32_t n, const uint64_t* d_ptrs, const pbm_l
 int pbm_page_address(pbm_handle* h, int32_t id, uint64_t* addr);
 int pbm_page_id(pbm_handle* h, uint64_t addr, int32_t* id);
 
+/* Copies `bytes` of page memory from byte `offset` of the pool to host
+ * memory (synchronous on `stream`): for reading what device code wrote
+ * into pages, such as the kernel-malloc headers. */
+int pbm_read_pool(pbm_handle* h, uint64_t offset, uint64_t bytes, void* host, void* stream);
+
+/* ---- kernel malloc ---------------------------------------------------
+ * The MemoryManager classes of malloc.cuh:14-37 over a pool with backing
+ * and a page size that is a multiple of 8: sub-page spaces carved from
+ * runs of consecutive pages, with 8-byte chunk and space headers on the
+ * pages (malloc.cu:45-57, 200-211; the device API documents the layouts).
+ * The manager keeps a reference to the pool, which must outlive it. */
+#define PBM_KMALLOC_SIMPLE 0  /* KernelMallocSimple: per-thread bump chunks, free is a no-op */
+#define PBM_KMALLOC_COLLAB 1  /* KernelMallocCollab: a wave's small requests share a chunk  */
+
+typedef struct pbm_kmalloc pbm_kmalloc;
+
+/* Constructor (malloc.cu:76-84, 214-218).  max_threads: per-thread chunk
+ * slots of KernelMallocSimple (0 = 1<<20; its constructor argument),
+ * ignored for KernelMallocCollab. */
+int pbm_kmalloc_create(pbm_handle* h, int32_t kind, uint32_t max_threads, pbm_kmalloc** out);
+int pbm_kmalloc_destroy(pbm_kmalloc* k);
+
+/* reset (malloc.cu:99-104, 222-224): every page of the pool free and, for
+ * KernelMallocSimple, no current chunks. */
+int pbm_kmalloc_reset(pbm_kmalloc* k, void* stream);
+
+/* malloc(size) for n requests (d_sizes[r] bytes; d_ptrs[r] = address or
+ * 0; d_steps[r] = getNPage steps).  KernelMallocSimple: thread t serves
+ * requests t, t+T, ... from chunk slot t (T = min(n, max_threads); one
+ * thread and slot 0 in serial mode).  KernelMallocCollab: a wave serves
+ * each 64 consecutive requests together.  Replaces allocate_kernel
+ * (unitTests/unitTestMalloc1.cu:27-34) calling memManager.malloc. */
+int pbm_kmalloc_malloc(pbm_kmalloc* k, uint32_t n, uint64_t req_base, const uint32_t* d_sizes,
+                       uint64_t* d_ptrs, uint32_t* d_steps, const pbm_launch* launch,
+                       void* stream);
+
+/* free(ptr) for n pointers (0 skipped).  Replaces free_kernel
+ * (unitTests/unitTestMalloc1.cu:36-41).  KernelMallocSimple's free does
+ * nothing, as in the reference (malloc.cu:178-181). */
+int pbm_kmalloc_free(pbm_kmalloc* k, uint32_t n, const uint64_t* d_ptrs,
+                     const pbm_launch* launch, void* stream);
+
+/* KernelMallocSimple's current chunk (first page, -1 none) per slot, into
+ * host_current[max_threads] when not NULL; *max_threads when not NULL. */
+int pbm_kmalloc_current(pbm_kmalloc* k, int32_t* host_current, uint32_t* max_threads);
+
 /* ---- observation ------------------------------------------------------ */
 
 /* Free-page clusters: maximal runs of free pages, the page after the last

@@ -1202,6 +1202,187 @@ __device__ inline void free_rw(const pbm_ctx& c, void* ptr, Stats* stats = nullp
 }
 
 /* ---------------------------------------------------------------------
+ * Kernel malloc (malloc.cu): sub-page allocators over runs of pages
+ *
+ * Both take their chunks (runs of consecutive pages inside one map word)
+ * from getNPage, which is the wave-collective getXPage the reference's
+ * allocators call (GETXPAGE = getXPageCollabRW_BM, malloc.cu:10-13).
+ * Headers are 8 bytes, little-endian, and every space starts 8-byte
+ * aligned.  Both return nullptr on a pool without page memory or whose
+ * page size is not a multiple of 8.
+ * ------------------------------------------------------------------- */
+
+/* KernelMallocCollab headers (malloc.cu:200-211):
+ *   chunk  u16 n_pages | u16 occupied bytes | u32 first page (bits 0-30)
+ *   space  u32 offset of this header from the chunk start |
+ *          u32 space bytes with header (bits 0-30), is_chunk (bit 31)
+ * A request with a chunk to itself has no space header of its own: bit 31
+ * of its chunk header's last word is is_chunk (malloc.cu:262-265). */
+constexpr u64 kIsChunk = 1ull << 63;
+
+/* Largest `occupied` a collaborative group may hold: the u16 field, and
+ * at most 64 pages with the chunk header. */
+__device__ __forceinline__ uint32_t kmalloc_group_cap(uint32_t page_size) {
+    const u64 pages = 64ull * page_size - 8u;
+    return pages < 65535ull ? (uint32_t)pages : 65535u;
+}
+
+/* KernelMallocCollab::malloc (malloc.cu:231-290), wave-collective over the
+ * active lanes.  A request takes s = round8(size) + 8 bytes with its
+ * header.  Requests with s < page_size - 8 share chunks: in lane order
+ * they form groups whose total fits `occupied` (the whole wave when it
+ * fits, as in the reference; the reference's u16 field wraps beyond
+ * that); the group's lowest lane gets ceil((total + 8) / page_size) pages,
+ * writes the chunk header, and every member writes its space header at
+ * 8 + (its suffix sum - s), so the highest lane sits first (:243-251,
+ * :280-287).  Larger requests get ceil(s / page_size) pages to themselves
+ * and the data follows the chunk header (:276-278).  The reference's
+ * suffix sum reads lane targetLaneID instead of targetLaneID-1 (:248). */
+__device__ inline void* kmalloc_collab(const pbm_ctx& c, uint32_t size, int32_t seed,
+                                       uint32_t* steps_out, Stats* stats = nullptr) {
+    if (!c.pool || (c.page_size & 7u)) {
+        if (steps_out) *steps_out = 0;
+        return nullptr;
+    }
+    const uint32_t ps = c.page_size;
+    const u64 s = (((u64)size + 7u) & ~7ull) + 8u;
+    const uint32_t cap = kmalloc_group_cap(ps);
+    const bool collab = s + 8u < (u64)ps && s <= cap;
+    const u64 cm = ballot(collab);
+    // groups in lane order (forward), then suffix sums inside each group
+    unsigned leader = lane_id();
+    {
+        u64 m = cm;
+        uint32_t run = 0;
+        unsigned cur = 64;
+        while (m) {
+            const unsigned l = (unsigned)__ffsll((long long)m) - 1u;
+            m &= m - 1ull;
+            const uint32_t v = readlane((uint32_t)s, l);
+            if (cur == 64 || run + v > cap) { cur = l; run = 0; }
+            run += v;
+            if (lane_id() == l) leader = cur;
+        }
+    }
+    uint32_t suffix = 0;
+    {
+        u64 m = cm;
+        uint32_t run = 0;
+        unsigned cur = 64;
+        while (m) {
+            const unsigned l = 63u - (unsigned)__clzll((long long)m);
+            m &= ~(1ull << l);
+            const unsigned g = readlane(leader, l);
+            if (g != cur) { cur = g; run = 0; }
+            run += readlane((uint32_t)s, l);
+            if (lane_id() == l) suffix = run;
+        }
+    }
+    uint32_t n;
+    if (collab) n = (leader == lane_id()) ? (uint32_t)((suffix + 8u + ps - 1u) / ps) : 0u;
+    else n = (u64)(s + ps - 1u) / ps > 64u ? 65u : (uint32_t)((s + ps - 1u) / ps);
+    const int32_t pid = get_n_pages(c, n, seed, steps_out, stats);
+    const int32_t gpid = (int32_t)shfl32((uint32_t)pid, collab ? leader : lane_id());
+    if (gpid < 0) return nullptr;
+    char* chunk = c.pool + (size_t)(uint32_t)gpid * ps;
+    if (!collab) {
+        *(u64*)chunk = (u64)n | ((u64)(uint16_t)s << 16) | ((u64)(uint32_t)gpid << 32) | kIsChunk;
+        return chunk + 8;
+    }
+    if (leader == lane_id())
+        *(u64*)chunk = (u64)n | ((u64)suffix << 16) | ((u64)(uint32_t)gpid << 32);
+    const uint32_t root = 8u + suffix - (uint32_t)s;
+    *(u64*)(chunk + root) = (u64)root | ((u64)(uint32_t)s << 32);
+    return chunk + root + 8;
+}
+
+/* KernelMallocCollab::free (malloc.cu:300-320) as its comment describes
+ * it: a request with its own chunk returns the chunk's pages; a member
+ * takes its bytes off the chunk's `occupied` with one atomic, and the last
+ * member returns the pages.  The reference leaves the subtraction
+ * commented out (:311-312), so shared chunks are never returned. */
+__device__ inline void kfree_collab(const pbm_ctx& c, void* ptr, Stats* stats = nullptr) {
+    if (!ptr || !c.pool) return;
+    char* hdr = (char*)ptr - 8;
+    const u64 h = *(const u64*)hdr;
+    if (h & kIsChunk) {
+        free_n_pages(c, (int32_t)((uint32_t)(h >> 32) & 0x7FFFFFFFu), (uint32_t)(uint16_t)h, stats);
+        return;
+    }
+    const uint32_t root = (uint32_t)h;
+    const uint32_t bytes = (uint32_t)(h >> 32) & 0x7FFFFFFFu;
+    char* chunk = hdr - root;
+    const uint32_t old = __hip_atomic_fetch_sub((uint32_t*)chunk, bytes << 16, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+    if ((old >> 16) == bytes) {
+        const uint32_t first = __hip_atomic_load((const uint32_t*)(chunk + 4), __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+        free_n_pages(c, (int32_t)(first & 0x7FFFFFFFu), old & 0xFFFFu, stats);
+    }
+}
+
+/* KernelMallocSimple headers (malloc.cu:45-57), u16 fields:
+ *   chunk  allocated (space bytes with headers) | available (n*page_size - 8) |
+ *          last (offset of the last space header past the chunk header) | 0
+ *   space  prev (bytes of the previous space, 0 first) |
+ *          bytes (bits 0-14), is_last (bit 15) | header offset from the chunk start | 0 */
+constexpr uint32_t kSimpleMaxSize = 32760;
+
+/* KernelMallocSimple::malloc (malloc.cu:129-175) for the calling thread,
+ * whose current chunk is *current (a page id, -1 none): bump allocation
+ * inside the chunk; a request that does not fit starts a new chunk of
+ * ceil((size + 16) / page_size) pages (chunk and space header) and the old
+ * one is left as it is.  Collective over the active lanes because the new
+ * chunks come from getNPage.  Sizes are rounded up to 8 and limited to
+ * kSimpleMaxSize (a 15-bit field) and chunks to 65,535 available bytes;
+ * beyond that it returns nullptr.  As the reference intends: the fit test
+ * counts the chunk header once (it subtracts it twice, :136), chunks have
+ * room for the space header (:141), the pointer returned is the data after
+ * the space header (it returns the header, :174), the last space header
+ * is found past the chunk header (:160) and the first space has prev = 0. */
+__device__ inline void* kmalloc_simple(const pbm_ctx& c, int32_t* current, uint32_t size,
+                                       int32_t seed, uint32_t* steps_out, Stats* stats = nullptr) {
+    if (!c.pool || (c.page_size & 7u)) {
+        if (steps_out) *steps_out = 0;
+        return nullptr;
+    }
+    const uint32_t ps = c.page_size;
+    const uint32_t s = (size + 7u) & ~7u;
+    const u64 pages = ((u64)s + 16u + ps - 1u) / ps;
+    const bool valid = size <= kSimpleMaxSize && pages <= 64u && pages * ps - 8u <= 65535u;
+    int32_t cur = valid ? *current : -1;
+    bool fresh = true;
+    if (cur >= 0) {
+        const u64 m = *(const u64*)(c.pool + (size_t)(uint32_t)cur * ps);
+        const uint32_t alloc = (uint32_t)(m & 0xFFFFu), avail = (uint32_t)((m >> 16) & 0xFFFFu);
+        fresh = avail - alloc < s + 8u;
+    }
+    const int32_t pid = get_n_pages(c, valid && fresh ? (uint32_t)pages : 0u, seed, steps_out, stats);
+    if (!valid) return nullptr;
+    if (fresh) {
+        if (pid < 0) return nullptr;
+        cur = pid;
+        *current = pid;
+        *(u64*)(c.pool + (size_t)(uint32_t)pid * ps) = (pages * ps - 8u) << 16;
+    }
+    char* chunk = c.pool + (size_t)(uint32_t)cur * ps;
+    const u64 m = *(const u64*)chunk;
+    const uint32_t alloc = (uint32_t)(m & 0xFFFFu);
+    const uint32_t last = (uint32_t)((m >> 32) & 0xFFFFu);
+    const uint32_t off = 8u + alloc;
+    uint32_t prev = 0;
+    if (alloc) {
+        u64* lh = (u64*)(chunk + 8u + last);
+        const u64 l = *lh;
+        prev = (uint32_t)((l >> 16) & 0x7FFFu);
+        *lh = l & ~(1ull << 31);
+    }
+    *(u64*)(chunk + off) = (u64)prev | ((u64)(s | 0x8000u) << 16) | ((u64)off << 32);
+    *(u64*)chunk = (u64)(alloc + 8u + s) | (m & 0xFFFF0000ull) | ((u64)alloc << 32);
+    return chunk + off + 8;
+}
+
+/* ---------------------------------------------------------------------
  * Legacy page-map walks (parallelPage.cu:185-224, 642-685)
  *
  * The legacy allocators keep one int per page (parallelPage.cu:117-121,

@@ -1058,6 +1058,201 @@ uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_
     return freed;
 }
 
+/* ------------------------------------------------------- kernel malloc */
+
+#define OFF_NONE (~(u64)0)
+
+static void put64(uint8_t* mem, uint8_t* written, u64 at, u64 v) {
+    memcpy(mem + at, &v, 8);
+    if (written) memset(written + at, 1, 8);
+}
+
+static u64 get64(const uint8_t* mem, u64 at) {
+    u64 v;
+    memcpy(&v, mem + at, 8);
+    return v;
+}
+
+/* KernelMallocCollab::malloc (malloc.cu:231-290) on one wave per chunk of
+ * 64 requests, writing the headers into `mem` (the pool's bytes; `written`
+ * marks every byte written when not NULL).  s = round8(size) + 8; requests
+ * with s + 8 < page_size (and s within the group cap) form groups in lane
+ * order whose totals fit min(65535, 64 page_size - 8); a group's lowest
+ * lane asks getNPage for ceil((total + 8) / page_size) pages and writes the
+ * chunk header (n | total << 16 | page << 32); each member writes its space
+ * header (root | s << 32) at root = 8 + suffix - s, suffix being the sum
+ * of s over the group's lanes from itself up.  Other requests ask for
+ * ceil(s / page_size) pages and write (n | (s & 0xFFFF) << 16 | page << 32
+ * | 1 << 63).  off[r]: pool offset of the returned pointer, or ~0. */
+uint32_t orc_kmalloc_collab(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                            int64_t* free_count, uint8_t* mem, uint8_t* written, uint32_t n,
+                            const uint32_t* sizes, uint64_t req_base, uint64_t seed,
+                            uint32_t budget, uint64_t* off, uint32_t* steps) {
+    const u64 ps = page_size;
+    const u64 cap = 64u * ps - 8u < 65535u ? 64u * ps - 8u : 65535u;
+    uint32_t got = 0;
+    for (uint32_t ch = 0; ch * 64u < n; ++ch) {
+        const uint32_t n_act = (n - ch * 64u) < 64u ? (n - ch * 64u) : 64u;
+        u64 sz[64], suffix[64];
+        int collab[64], leader[64];
+        uint32_t cnt[64], st[64];
+        int32_t ids[64];
+        int cur = -1;
+        u64 run = 0;
+        for (uint32_t l = 0; l < n_act; ++l) {
+            const uint32_t r = ch * 64u + l;
+            sz[l] = (((u64)sizes[r] + 7u) & ~(u64)7) + 8u;
+            collab[l] = sz[l] + 8u < ps && sz[l] <= cap;
+            leader[l] = (int)l;
+            suffix[l] = 0;
+            if (!collab[l]) continue;
+            if (cur < 0 || run + sz[l] > cap) { cur = (int)l; run = 0; }
+            run += sz[l];
+            leader[l] = cur;
+        }
+        cur = -1;
+        run = 0;
+        for (int l = (int)n_act - 1; l >= 0; --l) {
+            if (!collab[l]) continue;
+            if (leader[l] != cur) { cur = leader[l]; run = 0; }
+            run += sz[l];
+            suffix[l] = run;
+        }
+        for (uint32_t l = 0; l < n_act; ++l) {
+            if (collab[l]) {
+                cnt[l] = leader[l] == (int)l ? (uint32_t)((suffix[l] + 8u + ps - 1u) / ps) : 0u;
+            } else {
+                const u64 p = (sz[l] + ps - 1u) / ps;
+                cnt[l] = p > 64u ? 65u : (uint32_t)p;
+            }
+        }
+        orc_getnpage_serial(bm, n_pages, free_count, n_act, cnt, req_base + (u64)ch * 64u, seed,
+                            budget, ids, st);
+        for (uint32_t l = 0; l < n_act; ++l) {
+            const uint32_t r = ch * 64u + l;
+            if (steps) steps[r] = st[l];
+            const int32_t gp = collab[l] ? ids[leader[l]] : ids[l];
+            if (gp < 0) {
+                off[r] = OFF_NONE;
+                continue;
+            }
+            got++;
+            const u64 chunk = (u64)(uint32_t)gp * ps;
+            if (!collab[l]) {
+                put64(mem, written, chunk,
+                      (u64)cnt[l] | ((u64)(uint16_t)sz[l] << 16) | ((u64)(uint32_t)gp << 32) |
+                          ((u64)1 << 63));
+                off[r] = chunk + 8u;
+                continue;
+            }
+            if (leader[l] == (int)l)
+                put64(mem, written, chunk,
+                      (u64)cnt[l] | (suffix[l] << 16) | ((u64)(uint32_t)gp << 32));
+            const u64 root = 8u + suffix[l] - sz[l];
+            put64(mem, written, chunk + root, root | (sz[l] << 32));
+            off[r] = chunk + root + 8u;
+        }
+    }
+    return got;
+}
+
+/* KernelMallocCollab::free as intended (malloc.cu:300-320), in index
+ * order: a header with bit 63 returns its chunk's pages; a space header
+ * takes its bytes off the chunk's occupied field and the last one returns
+ * the chunk's pages. */
+uint32_t orc_kfree_collab(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint8_t* mem,
+                          uint32_t n, const uint64_t* off) {
+    uint32_t freed = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        if (off[r] == OFF_NONE) continue;
+        const u64 hdr = off[r] - 8u;
+        const u64 h = get64(mem, hdr);
+        int32_t id;
+        uint32_t cnt;
+        if (h >> 63) {
+            id = (int32_t)((uint32_t)(h >> 32) & 0x7FFFFFFFu);
+            cnt = (uint32_t)(h & 0xFFFFu);
+        } else {
+            const u64 chunk = hdr - (uint32_t)h;
+            const uint32_t bytes = (uint32_t)(h >> 32) & 0x7FFFFFFFu;
+            uint32_t w;
+            memcpy(&w, mem + chunk, 4);
+            const uint32_t occ = w >> 16;
+            w -= bytes << 16;
+            memcpy(mem + chunk, &w, 4);
+            if (occ != bytes) continue;
+            uint32_t first;
+            memcpy(&first, mem + chunk + 4, 4);
+            id = (int32_t)(first & 0x7FFFFFFFu);
+            cnt = w & 0xFFFFu;
+        }
+        freed += orc_free_n(bm, n_pages, free_count, 1, &id, &cnt);
+    }
+    return freed;
+}
+
+/* KernelMallocSimple::malloc (malloc.cu:129-175) for one thread whose
+ * current chunk is *current, requests in index order.  s = round8(size);
+ * valid when size <= 32760, the chunk of ceil((s + 16) / page_size)
+ * pages has at most 64 pages and 65,535 available bytes.  A request that
+ * does not fit the current chunk (available - allocated < s + 8) asks
+ * getNPage for a new chunk, whose header becomes (available << 16).  The
+ * space header (prev | (s | 0x8000) << 16 | offset << 32) goes at offset
+ * 8 + allocated; the previous last space loses bit 15 of its second field
+ * and gives its size as prev; the chunk header becomes (allocated + 8 + s
+ * | available << 16 | old allocated << 32). */
+uint32_t orc_kmalloc_simple(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                            int64_t* free_count, uint8_t* mem, uint8_t* written, uint32_t n,
+                            const uint32_t* sizes, uint64_t req_base, uint64_t seed,
+                            uint32_t budget, int32_t* current, uint64_t* off, uint32_t* steps) {
+    const u64 ps = page_size;
+    uint32_t got = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        const uint32_t s = (sizes[r] + 7u) & ~7u;
+        const u64 pages = ((u64)s + 16u + ps - 1u) / ps;
+        const int valid = sizes[r] <= 32760u && pages <= 64u && pages * ps - 8u <= 65535u;
+        int32_t cur = valid ? *current : -1;
+        int fresh = 1;
+        if (cur >= 0) {
+            const u64 m = get64(mem, (u64)(uint32_t)cur * ps);
+            fresh = (uint32_t)((m >> 16) & 0xFFFFu) - (uint32_t)(m & 0xFFFFu) < s + 8u;
+        }
+        uint32_t st = 0;
+        int32_t pid = 0;
+        if (valid && fresh) {
+            const uint32_t c = (uint32_t)pages;
+            orc_getnpage_serial(bm, n_pages, free_count, 1, &c, req_base + r, seed, budget, &pid,
+                                &st);
+        }
+        if (steps) steps[r] = st;
+        off[r] = OFF_NONE;
+        if (!valid) continue;
+        if (fresh) {
+            if (pid < 0) continue;
+            cur = pid;
+            *current = pid;
+            put64(mem, written, (u64)(uint32_t)pid * ps, (pages * ps - 8u) << 16);
+        }
+        const u64 chunk = (u64)(uint32_t)cur * ps;
+        const u64 m = get64(mem, chunk);
+        const uint32_t alloc = (uint32_t)(m & 0xFFFFu);
+        const uint32_t last = (uint32_t)((m >> 32) & 0xFFFFu);
+        const uint32_t at = 8u + alloc;
+        uint32_t prev = 0;
+        if (alloc) {
+            const u64 l = get64(mem, chunk + 8u + last);
+            prev = (uint32_t)((l >> 16) & 0x7FFFu);
+            put64(mem, written, chunk + 8u + last, l & ~((u64)1 << 31));
+        }
+        put64(mem, written, chunk + at, (u64)prev | ((u64)(s | 0x8000u) << 16) | ((u64)at << 32));
+        put64(mem, written, chunk,
+              (u64)(alloc + 8u + s) | (m & 0xFFFF0000ull) | ((u64)alloc << 32));
+        off[r] = chunk + at + 8u;
+        got++;
+    }
+    return got;
+}
+
 /* mallocRW (RWMalloc.cuh:299-326) on one wave: the plan of every request
  * (page count asked, offset inside the run, header) and the getNPage
  * grants.  Outputs per request: page (first page of its run, -1 on

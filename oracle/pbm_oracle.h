@@ -84,6 +84,20 @@ uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count
                              uint32_t budget, int32_t* ids, uint32_t* steps);
 uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                     const int32_t* ids, const uint32_t* counts);
+/* KernelMallocCollab malloc / free (malloc.cu:231-320) and
+ * KernelMallocSimple malloc (malloc.cu:129-175) on a byte image `mem` of
+ * the pool; `written` (optional) marks the bytes written; off[r] = pool
+ * offset of the returned pointer or ~0. */
+uint32_t orc_kmalloc_collab(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                            int64_t* free_count, uint8_t* mem, uint8_t* written, uint32_t n,
+                            const uint32_t* sizes, uint64_t req_base, uint64_t seed,
+                            uint32_t budget, uint64_t* off, uint32_t* steps);
+uint32_t orc_kfree_collab(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint8_t* mem,
+                          uint32_t n, const uint64_t* off);
+uint32_t orc_kmalloc_simple(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                            int64_t* free_count, uint8_t* mem, uint8_t* written, uint32_t n,
+                            const uint32_t* sizes, uint64_t req_base, uint64_t seed,
+                            uint32_t budget, int32_t* current, uint64_t* off, uint32_t* steps);
 uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
                            int64_t* free_count, uint32_t n, const uint64_t* sizes,
                            uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* page,

@@ -20,6 +20,7 @@ _P32 = C.POINTER(C.c_int32)
 _PU32 = C.POINTER(C.c_uint32)
 _PU64 = C.POINTER(C.c_uint64)
 _PI64 = C.POINTER(C.c_int64)
+_PU8 = C.POINTER(C.c_uint8)
 
 _SIGS = {
     "orc_lcg": (C.c_int32, [C.c_int32]),
@@ -65,6 +66,13 @@ _SIGS = {
                                           C.c_uint64, C.c_uint32, _P32, _P32, _PU32]),
     "orc_collab_basic_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
                                              C.c_uint64, C.c_uint32, _P32, _PU32]),
+    "orc_kmalloc_collab": (C.c_uint32, [_PU64, C.c_uint32, C.c_uint32, _PI64, _PU8, _PU8,
+                                        C.c_uint32, _PU32, C.c_uint64, C.c_uint64, C.c_uint32,
+                                        _PU64, _PU32]),
+    "orc_kfree_collab": (C.c_uint32, [_PU64, C.c_uint32, _PI64, _PU8, C.c_uint32, _PU64]),
+    "orc_kmalloc_simple": (C.c_uint32, [_PU64, C.c_uint32, C.c_uint32, _PI64, _PU8, _PU8,
+                                        C.c_uint32, _PU32, C.c_uint64, C.c_uint64, C.c_uint32,
+                                        _P32, _PU64, _PU32]),
     "orc_rw_tas": (C.c_double, [C.c_double, C.c_double, C.c_double]),
     "orc_count_clusters": (C.c_uint64, [_PU64, C.c_uint32]),
     "orc_getx_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32, C.c_uint64,
@@ -179,6 +187,37 @@ class State:
         lib().orc_collab_basic_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
                                       req_base, seed, budget, _p(ids, _P32), _p(steps, _PU32))
         return ids, steps
+
+    def kmalloc_collab(self, mem: np.ndarray, written: np.ndarray, sizes: np.ndarray,
+                       page_size: int, req_base: int, seed: int, budget: int = 1 << 20):
+        """KernelMallocCollab::malloc; returns (pool offsets, ~0 for none; steps)."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        n = sizes.size
+        off = np.zeros(n, dtype=np.uint64)
+        steps = np.zeros(n, dtype=np.uint32)
+        lib().orc_kmalloc_collab(_p(self.bm, _PU64), self.n_pages, page_size, _p(self.free, _PI64),
+                                 _p(mem, _PU8), _p(written, _PU8), n, _p(sizes, _PU32), req_base,
+                                 seed, budget, _p(off, _PU64), _p(steps, _PU32))
+        return off, steps
+
+    def kfree_collab(self, mem: np.ndarray, off: np.ndarray) -> int:
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        return int(lib().orc_kfree_collab(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64),
+                                          _p(mem, _PU8), off.size, _p(off, _PU64)))
+
+    def kmalloc_simple(self, mem: np.ndarray, written: np.ndarray, sizes: np.ndarray,
+                       page_size: int, req_base: int, seed: int, current: int = -1,
+                       budget: int = 1 << 20):
+        """KernelMallocSimple::malloc for one thread; returns (offsets, steps, current)."""
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint32)
+        n = sizes.size
+        off = np.zeros(n, dtype=np.uint64)
+        steps = np.zeros(n, dtype=np.uint32)
+        cur = np.array([current], dtype=np.int32)
+        lib().orc_kmalloc_simple(_p(self.bm, _PU64), self.n_pages, page_size, _p(self.free, _PI64),
+                                 _p(mem, _PU8), _p(written, _PU8), n, _p(sizes, _PU32), req_base,
+                                 seed, budget, _p(cur, _P32), _p(off, _PU64), _p(steps, _PU32))
+        return off, steps, int(cur[0])
 
     def collab(self, n: int, req_base: int, seed: int, order: int, budget: int = 1 << 20):
         ids, steps = self._out(n)

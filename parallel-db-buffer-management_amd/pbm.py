@@ -154,6 +154,15 @@ _SIGS = {
                                 C.c_void_p]),
     "pbm_page_address": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "pbm_page_id": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
+    "pbm_read_pool": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
+    "pbm_kmalloc_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pbm_kmalloc_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_kmalloc_reset": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pbm_kmalloc_malloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.POINTER(Launch), C.c_void_p]),
+    "pbm_kmalloc_free": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
+                                   C.c_void_p]),
+    "pbm_kmalloc_current": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32)]),
     "pbm_count_clusters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
     "pbm_monitor": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "pbm_wall_clock_hz": (C.c_int, [C.c_int, C.POINTER(C.c_uint64)]),
@@ -391,6 +400,66 @@ class Pool:
         v = C.c_int32()
         check(lib().pbm_page_id(self.h, addr, C.byref(v)), "pbm_page_id")
         return int(v.value)
+
+    def read_pool(self, offset: int, nbytes: int, stream=None) -> np.ndarray:
+        """Bytes [offset, offset+nbytes) of page memory."""
+        out = np.empty(nbytes, dtype=np.uint8)
+        check(lib().pbm_read_pool(self.h, offset, nbytes, out.ctypes.data_as(C.c_void_p),
+                                  _stream(stream)), "pbm_read_pool")
+        return out
+
+
+KMALLOC_SIMPLE, KMALLOC_COLLAB = 0, 1
+
+
+class KMalloc:
+    """KernelMallocSimple / KernelMallocCollab (malloc.cuh:14-37) over a
+    pool with backing."""
+
+    def __init__(self, pool: Pool, kind: int, max_threads: int = 0):
+        self.pool = pool
+        h = C.c_void_p()
+        check(lib().pbm_kmalloc_create(pool.h, kind, max_threads, C.byref(h)),
+              "pbm_kmalloc_create")
+        self.k = h
+
+    def reset(self, stream=None) -> None:
+        check(lib().pbm_kmalloc_reset(self.k, _stream(stream)), "pbm_kmalloc_reset")
+
+    def malloc(self, n: int, sizes, ptrs, steps=None, req_base: int = 0,
+               lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_kmalloc_malloc(self.k, n, req_base, _ptr(sizes), _ptr(ptrs), _ptr(steps),
+                                       C.byref(lnch or launch()), _stream(stream)),
+              "pbm_kmalloc_malloc")
+
+    def free(self, n: int, ptrs, lnch: Launch | None = None, stream=None) -> None:
+        check(lib().pbm_kmalloc_free(self.k, n, _ptr(ptrs), C.byref(lnch or launch()),
+                                     _stream(stream)), "pbm_kmalloc_free")
+
+    def current(self) -> np.ndarray:
+        m = C.c_uint32()
+        check(lib().pbm_kmalloc_current(self.k, None, C.byref(m)), "pbm_kmalloc_current")
+        out = np.empty(m.value, dtype=np.int32)
+        check(lib().pbm_kmalloc_current(self.k, out.ctypes.data_as(C.c_void_p), None),
+              "pbm_kmalloc_current")
+        return out
+
+    def close(self) -> None:
+        if self.k:
+            check(lib().pbm_kmalloc_destroy(self.k), "pbm_kmalloc_destroy")
+            self.k = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Graph:

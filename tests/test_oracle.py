@@ -844,6 +844,104 @@ def test_rwbasic_mean_steps_match_exact_expectation_and_rw_tas():
     assert abs(orc.rw_tas(T, A, N) - mean) / mean < 2e-3
 
 
+# ------------------------------------------------------------------ kernel malloc headers
+
+def _u64(mem, at):
+    return int.from_bytes(mem[at:at + 8].tobytes(), "little")
+
+
+def _empty_pool(T, ps):
+    return (orc.State(orc.cleared(T), T), np.zeros(T * ps, np.uint8), np.zeros(T * ps, np.uint8))
+
+
+def test_kmalloc_collab_layout_known_answer():
+    # sizes 16, 100, 5000, 8, 3000, 24 at 4 KiB pages: with their 8-byte
+    # headers and rounding, s = 24, 112, 5008, 16, 3008, 32.  Lanes 0,1,3,4,5
+    # share one chunk (s + 8 < 4096) whose 3,192 bytes fit one page; lane 2
+    # gets two pages of its own (malloc.cu:239-290)
+    T, ps = 256, 4096
+    st, mem, wr = _empty_pool(T, ps)
+    sizes = np.array([16, 100, 5000, 8, 3000, 24], np.uint32)
+    off, steps = st.kmalloc_collab(mem, wr, sizes, ps, 0, 7)
+    chunk = int(off[5]) - 16                    # the highest member sits first
+    assert chunk % ps == 0
+    # suffix sums from each lane up: 3192, 3168, -, 3056, 3040, 32
+    assert [int(o) - chunk for o in off[[0, 1, 3, 4, 5]]] == [3184, 3072, 3056, 48, 16]
+    g = chunk // ps
+    assert _u64(mem, chunk) == 1 | (3192 << 16) | (g << 32)
+    for lane, root, sz in [(0, 3176, 24), (1, 3064, 112), (3, 3048, 16), (4, 40, 3008), (5, 8, 32)]:
+        assert _u64(mem, chunk + root) == root | (sz << 32)
+    own = int(off[2]) - 8
+    assert own % ps == 0
+    assert _u64(mem, own) == 2 | (5008 << 16) | ((own // ps) << 32) | (1 << 63)
+    assert steps.tolist() == [1, 0, 1, 0, 0, 0]   # members ask getNPage for nothing
+    assert orc.count_free(st.bm, T) == T - 3
+    # free: members give back their bytes; the last one frees the chunk
+    assert st.kfree_collab(mem, off[[0, 1, 3]]) == 0
+    assert (_u64(mem, chunk) >> 16) & 0xFFFF == 3192 - 24 - 112 - 16
+    assert st.kfree_collab(mem, off[[2]]) == 2
+    assert st.kfree_collab(mem, off[[4, 5]]) == 1
+    assert orc.count_free(st.bm, T) == T and st.free[0] == T
+
+
+def test_kmalloc_collab_groups_split_at_the_u16_field():
+    # 64 lanes of 3,000 bytes (s = 3,008) total 192,512 bytes: in lane order,
+    # groups of at most 65,535 bytes take 21 lanes (63,168) each, so 4 chunks
+    T, ps = 4096, 4096
+    st, mem, wr = _empty_pool(T, ps)
+    off, _ = st.kmalloc_collab(mem, wr, np.full(64, 3000, np.uint32), ps, 0, 3)
+    chunks = {}
+    for o in off:
+        h = _u64(mem, int(o) - 8)
+        chunks.setdefault(int(o) - 8 - (h & 0xFFFFFFFF), []).append(int(o))
+    assert sorted(len(v) for v in chunks.values()) == [1, 21, 21, 21]
+    pages = 0
+    for c, members in chunks.items():
+        h = _u64(mem, c)
+        assert (h >> 16) & 0xFFFF == 3008 * len(members)
+        assert h & 0xFFFF == -(-(3008 * len(members) + 8) // ps)
+        pages += h & 0xFFFF
+    assert orc.count_free(st.bm, T) == T - pages
+    assert st.kfree_collab(mem, off) == pages
+    assert orc.count_free(st.bm, T) == T
+
+
+def test_kmalloc_simple_layout_known_answer():
+    # one thread: sizes 10, 20, 4000 share the first one-page chunk (4,088
+    # bytes available); 30 does not fit the 24 left and starts a new chunk
+    T, ps, seed = 1024, 4096, 5
+    st, mem, wr = _empty_pool(T, ps)
+    off, steps, cur = st.kmalloc_simple(mem, wr, np.array([10, 20, 4000, 30], np.uint32), ps,
+                                        0, seed)
+    L = T // 64
+    w0 = (orc.lcg(orc.request_seed(seed, 0)) & 0xFFFFFFFF) % L
+    w3 = (orc.lcg(orc.request_seed(seed, 3)) & 0xFFFFFFFF) % L
+    assert w0 != w3
+    c0, c1 = w0 * 64 * ps, w3 * 64 * ps
+    assert [int(o) for o in off] == [c0 + 16, c0 + 40, c0 + 72, c1 + 16]
+    assert steps.tolist() == [1, 0, 0, 1] and cur == w3 * 64
+    # spaces: prev | (bytes | is_last) << 16 | header offset << 32
+    assert _u64(mem, c0 + 8) == 0 | (16 << 16) | (8 << 32)
+    assert _u64(mem, c0 + 32) == 16 | (24 << 16) | (32 << 32)
+    assert _u64(mem, c0 + 64) == 24 | ((4000 | 0x8000) << 16) | (64 << 32)
+    # chunk: allocated | available << 16 | last space offset << 32
+    assert _u64(mem, c0) == 4064 | (4088 << 16) | (56 << 32)
+    assert _u64(mem, c1) == 40 | (4088 << 16) | (0 << 32)
+    assert _u64(mem, c1 + 8) == 0 | ((32 | 0x8000) << 16) | (8 << 32)
+    assert orc.count_free(st.bm, T) == T - 2
+
+
+def test_kmalloc_simple_rejects_what_its_fields_cannot_hold():
+    T, ps = 1024, 4096
+    st, mem, wr = _empty_pool(T, ps)
+    off, steps, cur = st.kmalloc_simple(mem, wr, np.array([32761, 32760], np.uint32), ps, 0, 1)
+    assert off[0] == np.uint64(2 ** 64 - 1) and steps[0] == 0
+    # 32,760 + 16 bytes need 9 pages, 36,856 available: valid
+    assert off[1] != np.uint64(2 ** 64 - 1)
+    assert (_u64(mem, cur * ps) >> 16) & 0xFFFF == 9 * ps - 8
+    assert not wr[:cur * ps].any() and orc.count_free(st.bm, T) == T - 9
+
+
 # ------------------------------------------------------------------ legacy page walks
 
 def _page(x: int, T: int) -> int:
