@@ -64,19 +64,24 @@ def compat_binary(name: str) -> Path:
     return OBJDIR / f"compat_check_{name}"
 
 
+COMPAT_KMALLOC_SRC = ROOT / "tests" / "compat" / "compat_kmalloc.hip"
+
+
 def build_compat_checks(jobs: int = 4) -> list[Path]:
-    """The reference-API driver (tests/compat/compat_check.hip) against
-    pbm_compat.hpp and libpbm.so, once per policy."""
+    """The reference-API drivers against pbm_compat.hpp and libpbm.so:
+    tests/compat/compat_check.hip once per policy, and the kernel-malloc
+    driver tests/compat/compat_kmalloc.hip."""
     OBJDIR.mkdir(exist_ok=True)
-    outs = [compat_binary(n) for n in COMPAT_STRATEGIES]
+    jobs_list = [([f"-DPBM_COMPAT_STRATEGY={m}", str(COMPAT_SRC)], compat_binary(n))
+                 for n, m in COMPAT_STRATEGIES.items()]
+    jobs_list.append(([str(COMPAT_KMALLOC_SRC)], compat_binary("kmalloc")))
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, f"-DPBM_COMPAT_STRATEGY={m}", str(COMPAT_SRC),
-                                 "-o", str(o), "-L", str(PKG), "-lpbm",
+        futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, *src, "-o", str(o), "-L", str(PKG), "-lpbm",
                                  "-Wl,-rpath,$ORIGIN/.."])
-                for m, o in zip(COMPAT_STRATEGIES.values(), outs)]
+                for src, o in jobs_list]
         for f in futs:
             f.result()
-    return outs
+    return [o for _, o in jobs_list]
 
 
 if __name__ == "__main__":
