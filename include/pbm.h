@@ -48,7 +48,8 @@ typedef enum pbm_strategy {
     PBM_DIST_CLOCK      = 6, /* parallelPage.cu:537-618        per-thread clock arms          */
     PBM_RW_BASIC        = 7, /* RandomWalkBasic.cuh:54-84      random walk over single pages  */
     PBM_CRW_BASIC       = 8, /* parallelPage.cu:200-224        clustered page walk, page hint */
-    PBM_COLLAB_RW_BASIC = 9  /* parallelPage.cu:642-685        wave-collaborative page walk   */
+    PBM_COLLAB_RW_BASIC = 9, /* parallelPage.cu:642-685        wave-collaborative page walk   */
+    PBM_OUROBOROS       = 10 /* Ouroboros.cuh:22-62            queue of free pages, lazy chunks */
 } pbm_strategy;
 
 /* Order in which a collaborative step hands the free bits of its window to
@@ -107,9 +108,9 @@ typedef struct pbm_ctx {
     int32_t*            hint;        /* CRW: last granted page per slot, -1 = none      */
     unsigned*           arms;        /* DistClock: per-slot arm                         */
     unsigned long long* clock_arm;   /* SingleClock: global arm (64-bit, never wraps)   */
-    unsigned long long* ring;        /* LinkedList: n_pages slots (lap<<32 | page id)   */
-    unsigned long long* ring_head;   /* LinkedList: next ticket to dequeue              */
-    unsigned long long* ring_tail;   /* LinkedList: next ticket to enqueue              */
+    unsigned long long* ring;        /* LinkedList, Ouroboros: n_pages slots (lap<<32 | page id) */
+    unsigned long long* ring_head;   /* LinkedList, Ouroboros: next ticket to dequeue   */
+    unsigned long long* ring_tail;   /* LinkedList, Ouroboros: next ticket to enqueue   */
     long long*          phase;       /* batched calls: [1] wave completion counter      */
     unsigned long long* scan;        /* n_tiles look-back words of the single-pass rank
                                         scan (call parity, state, grant, prefix)        */
@@ -124,6 +125,11 @@ typedef struct pbm_ctx {
     uint64_t seed;
     uint32_t n_tiles;                /* rank-scan tiles: ceil(n_words / (256*scan_wpt)) */
     uint32_t scan_wpt;               /* rank scan: map words per thread (1 or 4)        */
+    unsigned*           fresh;       /* Ouroboros: pages free at the last reset, ascending */
+    unsigned long long* carve;       /* Ouroboros: [0] next fresh index, [1] fresh pages */
+    long long*          q_avail;     /* Ouroboros: queued pages no wave has been promised */
+    uint32_t chunk_pages;            /* Ouroboros: pages carved per chunk                */
+    uint32_t reserved_;
 } pbm_ctx;
 
 /* Launch shape of a batch call. */

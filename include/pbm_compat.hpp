@@ -1,7 +1,10 @@
 /*
  * pbm_compat.hpp — the reference's host and device API names over the
  * C-ABI, for code written against RandomWalkBasic_BM.cuh / CollabRW_BM.cuh
- * / CRW_BM.cuh / RWMalloc.cuh.
+ * / CRW_BM.cuh / RWMalloc.cuh / RandomWalkBasic.cuh, the legacy walks of
+ * parallelPage.cu, malloc.cuh, and (with PBM_COMPAT_OUROBOROS defined)
+ * Ouroboros.cuh, whose getPage returns a pointer and whose freePage takes
+ * one.
  *
  * The reference picks its policy by which header a translation unit
  * includes (unitTests/unitTest1.cu:7) and keeps one allocator per
@@ -26,7 +29,11 @@
 #include "pbm_device.hpp"
 
 #ifndef PBM_COMPAT_STRATEGY
+#ifdef PBM_COMPAT_OUROBOROS
+#define PBM_COMPAT_STRATEGY PBM_OUROBOROS
+#else
 #define PBM_COMPAT_STRATEGY PBM_RW_BM
+#endif
 #endif
 #ifndef TOTAL_SIZE_GB_DEFAULT
 #define TOTAL_SIZE_GB_DEFAULT 8   /* Paging.cuh:5 */
@@ -103,14 +110,13 @@ static __device__ inline int32_t pbm_compat_seed() {
     return pbm::request_seed(pbm_compat_ctx.seed, (gid << 32) ^ wall_clock64());
 }
 
-/* getPage (RandomWalkBasic_BM.cuh:60-89, RandomWalkBasic.cuh:54-80, CollabRW_BM.cuh:114-160,
- * CRW_BM.cuh:75-142, RWMalloc.cuh:113-149, and the legacy
- * getPageClusteredRandomWalk / getPageCollabRW, parallelPage.cu:200-218,
- * 642-680): a page id, or -1 when the pool has none (the reference spins).
- * The collaborative policies and RWMalloc are collective over the calling
- * wave's active lanes; collectWarpData is accepted and ignored. */
-static __device__ inline int getPage(int* stepCount = nullptr, bool collectWarpData = false) {
-    (void)collectWarpData;
+/* pageAddress / getPageID (Paging.cuh:30-52). */
+static __device__ inline void* pageAddress(int pageID) {
+    return pbm::page_address(pbm_compat_ctx, pageID);
+}
+static __device__ inline int getPageID(void* ptr) { return pbm::page_id(pbm_compat_ctx, ptr); }
+
+static __device__ inline int pbm_compat_get(int* stepCount) {
     uint32_t steps = 0;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t id = pbm::get_page_of<PBM_COMPAT_STRATEGY>(pbm_compat_ctx, gid, pbm_compat_seed(),
@@ -119,21 +125,36 @@ static __device__ inline int getPage(int* stepCount = nullptr, bool collectWarpD
     return id;
 }
 
+#ifdef PBM_COMPAT_OUROBOROS
+/* getPage / freePage of the Ouroboros wrapper (Ouroboros.cuh:16-17,
+ * 38-47): a page pointer (nullptr when the pool has no page for the
+ * request), and free by pointer. */
+static __device__ inline void* getPage() { return pageAddress(pbm_compat_get(nullptr)); }
+static __device__ inline void freePage(void* ptr) {
+    pbm::free_page_of<PBM_COMPAT_STRATEGY>(pbm_compat_ctx, getPageID(ptr), nullptr);
+}
+static __device__ inline void* getPagePtr() { return getPage(); }
+#else
+/* getPage (RandomWalkBasic_BM.cuh:60-89, RandomWalkBasic.cuh:54-80, CollabRW_BM.cuh:114-160,
+ * CRW_BM.cuh:75-142, RWMalloc.cuh:113-149, and the legacy
+ * getPageClusteredRandomWalk / getPageCollabRW, parallelPage.cu:200-218,
+ * 642-680): a page id, or -1 when the pool has none (the reference spins).
+ * The collaborative policies and RWMalloc are collective over the calling
+ * wave's active lanes; collectWarpData is accepted and ignored. */
+static __device__ inline int getPage(int* stepCount = nullptr, bool collectWarpData = false) {
+    (void)collectWarpData;
+    return pbm_compat_get(stepCount);
+}
+
 /* freePage (RandomWalkBasic_BM.cuh:98-102; the list pool's
  * freePageLinkedList, parallelPage.cu:399-418, appends to the ring). */
 static __device__ inline void freePage(int pageID) {
-    if (PBM_COMPAT_STRATEGY == PBM_LINKED_LIST) pbm::free_page_list(pbm_compat_ctx, pageID);
-    else pbm::free_page(pbm_compat_ctx, pageID);
+    pbm::free_page_of<PBM_COMPAT_STRATEGY>(pbm_compat_ctx, pageID, nullptr);
 }
-
-/* pageAddress / getPageID (Paging.cuh:30-52). */
-static __device__ inline void* pageAddress(int pageID) {
-    return pbm::page_address(pbm_compat_ctx, pageID);
-}
-static __device__ inline int getPageID(void* ptr) { return pbm::page_id(pbm_compat_ctx, ptr); }
 
 /* getPagePtr (RandomWalkBasic_BM.cuh:92-95). */
 static __device__ inline void* getPagePtr() { return pageAddress(getPage()); }
+#endif
 
 /* getNPage (RWMalloc.cuh:198-248): n <= 64 consecutive pages in one word;
  * wave-collective. */

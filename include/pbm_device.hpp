@@ -849,6 +849,152 @@ __device__ inline void free_page_list(const pbm_ctx& c, int32_t id, uint32_t* st
 }
 
 /* ---------------------------------------------------------------------
+ * Ouroboros page queue (Ouroboros.cuh:22-62 over Ouroboros' page queue,
+ * Ouroboros/include/device/queues/page/PageQueueVL_impl.cuh:36-111, and
+ * its bulk semaphore, Ouroboros/include/device/BulkSemaphore_impl.cuh:
+ * 117-232)
+ *
+ * A FIFO queue of free pages that fills lazily.  Pages enter it when a
+ * chunk of chunk_pages pages is carved, in order, from the pages that were
+ * free at the last reset (the bump allocation of chunks,
+ * Ouroboros/include/device/Ouroboros.cuh:201-225, whose pages are all
+ * enqueued at once, PageQueueVL_impl.cuh:36-48), or when they are freed
+ * (enqueued at the back, :104-121).  ctx.q_avail counts queued pages no
+ * request has been promised, the semaphore's count.  A wave promises
+ * itself its requests with one atomic; when the count falls short it
+ * carves enough chunks for the shortfall, enqueues their pages and adds
+ * them to the count (the semaphore's AllocateChunk path), then dequeues.
+ * Where the semaphore makes later waves reserve and sleep until the
+ * carving wave signals, here every short wave carves for itself, so no
+ * wave waits on another's promise; a chunk's leftover pages serve the
+ * next requests.  The ring, its tickets and slot sequence numbers are the
+ * list pool's (above); the page map mirrors the queue.  Reservation
+ * (ctx.free_count) bounds grants by the free pages, so a promised page is
+ * always queued or carvable.  Step = 1 per dequeue, 2 when the wave carved.
+ * ------------------------------------------------------------------- */
+
+/* Enqueues pages src[0..k) at consecutive tail tickets, wave-parallel
+ * (k and src uniform).  Returns false if a slot stayed busy. */
+__device__ inline bool enqueue_run(const pbm_ctx& c, const unsigned* src, uint32_t k) {
+    const uint32_t N = c.n_pages;
+    u64 t0 = 0;
+    const u64 act = ballot(true);
+    const unsigned leader = (unsigned)__ffsll((long long)act) - 1u;
+    const unsigned lanes = (unsigned)__popcll(act);
+    if (lane_id() == leader) t0 = atom_add_u64(c.ring_tail, (u64)k);
+    t0 = readlane64(t0, leader);
+    bool ok = true;
+    for (uint32_t j = rank_in(act); j < k; j += lanes) {
+        const u64 t = t0 + j;
+        u64* slot = &c.ring[t % N];
+        bool ready = false;
+        for (uint32_t s = 0; s < kRingSpin && !ready; ++s)
+            ready = (uint32_t)(ld_slot(slot) >> 32) == (uint32_t)t;
+        ok &= ready;
+        st_slot(slot, ((u64)(uint32_t)(t + 1u) << 32) | src[j]);
+    }
+    return ballot(!ok) == 0;
+}
+
+__device__ inline int32_t get_page_queue(const pbm_ctx& c, uint32_t* steps_out,
+                                         Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t N = c.n_pages;
+    const bool want = reserve_one(c, true);
+    int32_t id = -1;
+    uint32_t steps = 0;
+    const u64 m = ballot(want);
+    if (m) {
+        const unsigned cnt = (unsigned)__popcll(m);
+        const unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+        // promise; on a shortfall, carve whole chunks for it
+        u64 base = 0, take = 0;
+        if (lane_id() == leader) {
+            const long long old = atom_add_ll(c.q_avail, -(long long)cnt);
+            const long long have = old <= 0 ? 0 : (old >= (long long)cnt ? (long long)cnt : old);
+            const u64 shortfall = (u64)cnt - (u64)have;
+            if (shortfall) {
+                const u64 cp = c.chunk_pages;
+                const u64 ask = (shortfall + cp - 1u) / cp * cp;
+                const u64 n_fresh = ld_word(&c.carve[1]);
+                base = atom_add_u64(&c.carve[0], ask);
+                take = base >= n_fresh ? 0 : (ask < n_fresh - base ? ask : n_fresh - base);
+            }
+        }
+        base = readlane64(base, leader);
+        take = readlane64(take, leader);
+        if (take) {
+            if (!enqueue_run(c, c.fresh + base, (uint32_t)take)) st.errors++;
+            if (lane_id() == leader) atom_add_ll(c.q_avail, (long long)take);
+        }
+        u64 h0 = 0;
+        if (lane_id() == leader) h0 = atom_add_u64(c.ring_head, (u64)cnt);
+        h0 = readlane64(h0, leader);
+        if (want) {
+            const u64 t = h0 + rank_in(m);
+            u64* slot = &c.ring[t % N];
+            const uint32_t expect = (uint32_t)(t + 1u);
+            for (uint32_t k = 0; k < kRingSpin; ++k) {
+                const u64 v = ld_slot(slot);
+                if ((uint32_t)(v >> 32) == expect) { id = (int32_t)(uint32_t)v; break; }
+            }
+            st.probes++;
+            steps = take ? 2u : 1u;
+            if (id >= 0) {
+                st_slot(slot, (u64)(uint32_t)(t + N) << 32);
+                const u64 bit = 1ull << ((uint32_t)id & 63u);
+                if (atom_or(&c.bitmap[(uint32_t)id >> 6], bit) & bit) st.errors++;
+                st.atomics++;
+            } else {
+                st.errors++;
+            }
+        }
+    }
+    if (id >= 0) st.allocs++; else st.failures++;
+    st.steps += steps;
+    if (steps_out) *steps_out = steps;
+    return id;
+}
+
+/* freePage (Ouroboros.cuh:45-47): a page in use goes to the back of the
+ * queue and adds one to the count (PageQueueVL_impl.cuh:104-111).
+ * Freeing a free page or an id out of range counts an error. */
+__device__ inline void free_page_queue(const pbm_ctx& c, int32_t id, Stats* stats = nullptr) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const uint32_t N = c.n_pages;
+    bool ok = false;
+    if (id >= 0 && (uint32_t)id < N) {
+        const u64 bit = 1ull << ((uint32_t)id & 63u);
+        ok = (atom_and(&c.bitmap[(uint32_t)id >> 6], ~bit) & bit) != 0;
+        st.atomics++;
+    }
+    if (!ok) st.errors++;
+    const u64 m = ballot(ok);
+    if (!m) return;
+    const unsigned cnt = (unsigned)__popcll(m);
+    const unsigned leader = (unsigned)__ffsll((long long)m) - 1u;
+    u64 t0 = 0;
+    if (lane_id() == leader) t0 = atom_add_u64(c.ring_tail, (u64)cnt);
+    t0 = readlane64(t0, leader);
+    if (ok) {
+        const u64 t = t0 + rank_in(m);
+        u64* slot = &c.ring[t % N];
+        bool ready = false;
+        for (uint32_t k = 0; k < kRingSpin && !ready; ++k)
+            ready = (uint32_t)(ld_slot(slot) >> 32) == (uint32_t)t;
+        if (!ready) st.errors++;
+        st_slot(slot, ((u64)(uint32_t)(t + 1u) << 32) | (uint32_t)id);
+        st.frees++;
+    }
+    // signal after the slot stores: a promise made before they are visible
+    // waits on the slot's sequence number
+    if (lane_id() == leader) atom_add_ll(c.q_avail, (long long)cnt);
+    release_one(c, ok);
+}
+
+/* ---------------------------------------------------------------------
  * RWMalloc (RWMalloc.cuh:113-334)
  * ------------------------------------------------------------------- */
 
@@ -1516,7 +1662,16 @@ __device__ __forceinline__ int32_t get_page_of(const pbm_ctx& c, uint32_t slot, 
     if (S == PBM_RW_BASIC) return get_page_rwbasic(c, seed, steps_out, st);
     if (S == PBM_CRW_BASIC) return get_page_crw_basic(c, slot, seed, steps_out, st);
     if (S == PBM_COLLAB_RW_BASIC) return get_page_collab_basic(c, seed, steps_out, st);
+    if (S == PBM_OUROBOROS) return get_page_queue(c, steps_out, st);
     return get_page_list(c, steps_out, st);
+}
+
+/* The strategy's one-page freePage, chosen at compile time. */
+template <int S>
+__device__ __forceinline__ void free_page_of(const pbm_ctx& c, int32_t id, Stats* st) {
+    if (S == PBM_LINKED_LIST) free_page_list(c, id, nullptr, st);
+    else if (S == PBM_OUROBOROS) free_page_queue(c, id, st);
+    else free_page(c, id, st);
 }
 
 }  // namespace pbm

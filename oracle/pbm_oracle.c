@@ -864,6 +864,89 @@ void orc_list_put(uint64_t* bm, uint32_t n_pages, int32_t* q, uint64_t* head, ui
     }
 }
 
+/* ------------------------------------------------------ Ouroboros queue */
+
+/* Pages per chunk for requests of page_size bytes: the power-of-two size
+ * class of at least 16 bytes (Ouroboros/include/device/Helper.cuh:42-60)
+ * carved from 8 KiB chunks, 32 KiB chunks above 8 KiB
+ * (Ouroboros/include/InstanceDefinitions.cuh:26-28), one page per chunk
+ * above 32 KiB. */
+uint32_t orc_ouroboros_chunk_pages(uint32_t page_size) {
+    u64 cls = 16;
+    while (cls < page_size) cls <<= 1;
+    if (cls <= 8192u) return (uint32_t)(8192u / cls);
+    if (cls <= 32768u) return (uint32_t)(32768u / cls);
+    return 1u;
+}
+
+/* Queue state after a reset of the page map: fresh = the free pages,
+ * ascending; st = {carved, fresh pages, head, tail, q_avail}. */
+uint32_t orc_queue_reset(const uint64_t* bm, uint32_t n_pages, uint32_t* fresh, int64_t* st) {
+    uint32_t k = 0;
+    for (uint32_t p = 0; p < n_pages; ++p)
+        if (!((bm[p >> 6] >> (p & 63u)) & 1u)) fresh[k++] = p;
+    st[0] = 0;
+    st[1] = k;
+    st[2] = 0;
+    st[3] = 0;
+    st[4] = 0;
+    return k;
+}
+
+/* getPage of the Ouroboros comparator, one request at a time (a wave of
+ * one lane).  A reserved request promises itself one queued page; with
+ * none unpromised it carves the next chunk_pages fresh pages (fewer at the
+ * end) onto the back of the queue first.  It takes the front of the queue.
+ * Step = 1, 2 when it carved. */
+uint32_t orc_queue_get(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
+                       const uint32_t* fresh, int32_t* ring, int64_t* st, uint32_t chunk_pages,
+                       uint32_t n, int32_t* ids, uint32_t* steps) {
+    uint32_t got = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        ids[r] = -1;
+        if (steps) steps[r] = 0;
+        if (*free_count <= 0) continue;
+        (*free_count)--;
+        const int64_t old = st[4]--;
+        uint32_t s = 1;
+        if (old <= 0) {
+            const int64_t base = st[0];
+            st[0] += chunk_pages;
+            const int64_t take = base >= st[1] ? 0
+                                 : ((int64_t)chunk_pages < st[1] - base ? (int64_t)chunk_pages
+                                                                          : st[1] - base);
+            for (int64_t k = 0; k < take; ++k) ring[(st[3]++) % n_pages] = (int32_t)fresh[base + k];
+            st[4] += take;
+            if (take) s = 2;
+        }
+        const int32_t id = ring[(st[2]++) % n_pages];
+        bm[(uint32_t)id >> 6] |= (u64)1 << ((uint32_t)id & 63u);
+        ids[r] = id;
+        if (steps) steps[r] = s;
+        got++;
+    }
+    return got;
+}
+
+/* freePage of the comparator: a page in use goes to the back of the queue;
+ * other ids are skipped.  Returns the pages freed. */
+uint32_t orc_queue_put(uint64_t* bm, uint32_t n_pages, int64_t* free_count, int32_t* ring,
+                       int64_t* st, uint32_t n, const int32_t* ids) {
+    uint32_t freed = 0;
+    for (uint32_t r = 0; r < n; ++r) {
+        const int32_t id = ids[r];
+        if (id < 0 || (uint32_t)id >= n_pages) continue;
+        const u64 bit = (u64)1 << ((uint32_t)id & 63u);
+        if (!(bm[(uint32_t)id >> 6] & bit)) continue;
+        bm[(uint32_t)id >> 6] &= ~bit;
+        ring[(st[3]++) % n_pages] = id;
+        st[4]++;
+        (*free_count)++;
+        freed++;
+    }
+    return freed;
+}
+
 /* ------------------------------------------------------------ RWMalloc */
 
 /* RWMalloc getPage (RWMalloc.cuh:113-149) on one wave: every active lane

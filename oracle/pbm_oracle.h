@@ -84,6 +84,16 @@ uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count
                              uint32_t budget, int32_t* ids, uint32_t* steps);
 uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                     const int32_t* ids, const uint32_t* counts);
+/* Ouroboros comparator (Ouroboros.cuh:22-62): pages per chunk, queue
+ * reset, getPage and freePage, one request at a time; st = {carved, fresh
+ * pages, head, tail, unpromised queued pages}. */
+uint32_t orc_ouroboros_chunk_pages(uint32_t page_size);
+uint32_t orc_queue_reset(const uint64_t* bm, uint32_t n_pages, uint32_t* fresh, int64_t* st);
+uint32_t orc_queue_get(uint64_t* bm, uint32_t n_pages, int64_t* free_count,
+                       const uint32_t* fresh, int32_t* ring, int64_t* st, uint32_t chunk_pages,
+                       uint32_t n, int32_t* ids, uint32_t* steps);
+uint32_t orc_queue_put(uint64_t* bm, uint32_t n_pages, int64_t* free_count, int32_t* ring,
+                       int64_t* st, uint32_t n, const int32_t* ids);
 /* KernelMallocCollab malloc / free (malloc.cu:231-320) and
  * KernelMallocSimple malloc (malloc.cu:129-175) on a byte image `mem` of
  * the pool; `written` (optional) marks the bytes written; off[r] = pool

@@ -73,6 +73,11 @@ _SIGS = {
     "orc_kmalloc_simple": (C.c_uint32, [_PU64, C.c_uint32, C.c_uint32, _PI64, _PU8, _PU8,
                                         C.c_uint32, _PU32, C.c_uint64, C.c_uint64, C.c_uint32,
                                         _P32, _PU64, _PU32]),
+    "orc_ouroboros_chunk_pages": (C.c_uint32, [C.c_uint32]),
+    "orc_queue_reset": (C.c_uint32, [_PU64, C.c_uint32, _PU32, _PI64]),
+    "orc_queue_get": (C.c_uint32, [_PU64, C.c_uint32, _PI64, _PU32, _P32, _PI64, C.c_uint32,
+                                   C.c_uint32, _P32, _PU32]),
+    "orc_queue_put": (C.c_uint32, [_PU64, C.c_uint32, _PI64, _P32, _PI64, C.c_uint32, _P32]),
     "orc_rw_tas": (C.c_double, [C.c_double, C.c_double, C.c_double]),
     "orc_count_clusters": (C.c_uint64, [_PU64, C.c_uint32]),
     "orc_getx_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32, C.c_uint64,
@@ -308,6 +313,41 @@ class State:
         ids = np.ascontiguousarray(ids, dtype=np.int32)
         return int(lib().orc_free(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64),
                                   ids.size, _p(ids, _P32)))
+
+
+def ouroboros_chunk_pages(page_size: int) -> int:
+    return int(lib().orc_ouroboros_chunk_pages(page_size))
+
+
+class QueueState(State):
+    """The Ouroboros comparator's queue over a page map, one request at a
+    time: pages free at construction are carved in chunks of chunk_pages."""
+
+    def __init__(self, bm: np.ndarray, n_pages: int, chunk_pages: int):
+        super().__init__(bm, n_pages)
+        self.cp = chunk_pages
+        self.fresh = np.zeros(n_pages, dtype=np.uint32)
+        self.ring = np.zeros(n_pages, dtype=np.int32)
+        self.st = np.zeros(5, dtype=np.int64)
+        lib().orc_queue_reset(_p(self.bm, _PU64), n_pages, _p(self.fresh, _PU32),
+                              _p(self.st, _PI64))
+
+    def get(self, n: int):
+        ids, steps = self._out(n)
+        lib().orc_queue_get(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64),
+                            _p(self.fresh, _PU32), _p(self.ring, _P32), _p(self.st, _PI64),
+                            self.cp, n, _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps
+
+    def put(self, ids: np.ndarray) -> int:
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        return int(lib().orc_queue_put(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64),
+                                       _p(self.ring, _P32), _p(self.st, _PI64), ids.size,
+                                       _p(ids, _P32)))
+
+    def queued(self) -> list:
+        h, t = int(self.st[2]), int(self.st[3])
+        return [int(self.ring[k % self.n_pages]) for k in range(h, t)]
 
 
 def check_alloc(before: np.ndarray, after: np.ndarray, n_pages: int, ids: np.ndarray):

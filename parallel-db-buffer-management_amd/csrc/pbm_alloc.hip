@@ -596,7 +596,7 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
     const int strat = h->cfg.strategy;
 
     if (L.mode == PBM_MODE_BATCHED) {
-        if (strat == PBM_LINKED_LIST)
+        if (pbmi::is_queue_strategy(strat))
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
         if (L.flags & PBM_LAUNCH_RANK) {
             if (c.scan_wpt == 4)
@@ -661,10 +661,10 @@ extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
     if (rc) return rc;
     hipStream_t s = pbmi::as_stream(stream);
     pbm_ctx c = launch_ctx(h, L);
-    if (h->cfg.strategy == PBM_LINKED_LIST) {
+    if (pbmi::is_queue_strategy(h->cfg.strategy)) {
         if (L.mode == PBM_MODE_BATCHED)
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
-        return pbmi::launch_free_list(c, L, n, d_ids, s);
+        return pbmi::launch_free_list(c, h->cfg.strategy, L, n, d_ids, s);
     }
     if (L.mode == PBM_MODE_BATCHED) {
         const uint32_t nb = (batched_blocks(h, L, n) + 3u) / 4u;

@@ -144,7 +144,7 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
     if (!tr || !out) return pbmi::fail(PBM_ERR_INVALID, "null trace or output");
     *out = nullptr;
     if (!h->d_pool) return pbmi::fail(PBM_ERR_UNSUPPORTED, "mallocRW bursts need a backed pool");
-    if (h->cfg.strategy == PBM_LINKED_LIST)
+    if (pbmi::is_queue_strategy(h->cfg.strategy))
         return pbmi::fail(PBM_ERR_UNSUPPORTED, "mallocRW bursts need a bitmap strategy");
     if (tr->n_bursts == 0 || !tr->alloc_offset || !tr->free_offset)
         return pbmi::fail(PBM_ERR_INVALID, "a trace needs bursts and both offset arrays");

@@ -276,7 +276,7 @@ extern "C" int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_
     PBM_CHECK_HANDLE(h);
     if (!cfg || !out) return pbmi::fail(PBM_ERR_INVALID, "null cache config or output");
     *out = nullptr;
-    if (h->cfg.strategy == PBM_LINKED_LIST)
+    if (pbmi::is_queue_strategy(h->cfg.strategy))
         return pbmi::fail(PBM_ERR_UNSUPPORTED, "the cache's frames need a bitmap strategy");
     if (cfg->n_logical == 0 || cfg->n_logical > (1u << 30))
         return pbmi::fail(PBM_ERR_RANGE, "n_logical must be in [1, 2^30]");

@@ -150,7 +150,7 @@ static int validate(const pbm_config* cfg) {
     if (cfg->n_pages == 0 || cfg->n_pages > (1u << 30))
         return pbmi::fail(PBM_ERR_RANGE, "n_pages must be in [1, 2^30]");
     if (cfg->page_size < 8) return pbmi::fail(PBM_ERR_INVALID, "page_size must be >= 8");
-    if (cfg->strategy < PBM_RW_BM || cfg->strategy > PBM_COLLAB_RW_BASIC)
+    if (cfg->strategy < PBM_RW_BM || cfg->strategy > PBM_OUROBOROS)
         return pbmi::fail(PBM_ERR_INVALID, "unknown strategy");
     if (cfg->order != PBM_ORDER_PACKED && cfg->order != PBM_ORDER_REFERENCE)
         return pbmi::fail(PBM_ERR_INVALID, "unknown collaborative order");
@@ -203,9 +203,14 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
         PBM_ALLOC(h->d_hint, (size_t)h->cfg.max_threads * 4u);
     if (cfg->strategy == PBM_DIST_CLOCK) PBM_ALLOC(h->d_arms, (size_t)h->cfg.max_threads * 4u);
     if (cfg->strategy == PBM_SINGLE_CLOCK) PBM_ALLOC(h->d_clock, 8u);
-    if (cfg->strategy == PBM_LINKED_LIST) {
+    if (cfg->strategy == PBM_LINKED_LIST || cfg->strategy == PBM_OUROBOROS) {
         PBM_ALLOC(h->d_ring, (size_t)cfg->n_pages * 8u);
         PBM_ALLOC(h->d_ring_ht, 16u);
+    }
+    if (cfg->strategy == PBM_OUROBOROS) {
+        PBM_ALLOC(h->d_fresh, (size_t)cfg->n_pages * 4u);
+        PBM_ALLOC(h->d_carve, 16u);
+        PBM_ALLOC(h->d_qavail, 8u);
     }
 #undef PBM_ALLOC
 
@@ -233,6 +238,10 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     c.max_threads = h->cfg.max_threads;
     c.order = cfg->order;
     c.seed = cfg->seed;
+    c.fresh = h->d_fresh;
+    c.carve = h->d_carve;
+    c.q_avail = h->d_qavail;
+    c.chunk_pages = pbmi::ouroboros_chunk_pages(cfg->page_size);
 
     e = hipMemset(h->d_counters, 0, sizeof(pbm_counters));
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
@@ -261,7 +270,7 @@ extern "C" int pbm_destroy(pbm_handle* h) {
     (void)hipDeviceSynchronize();
     void* ptrs[] = {h->d_bitmap, h->d_free, h->d_counters, h->d_pool, h->d_hint, h->d_arms,
                     h->d_clock, h->d_ring, h->d_ring_ht, h->d_phase, h->d_scratch,
-                    h->d_scan, h->d_scan_ctl};
+                    h->d_scan, h->d_scan_ctl, h->d_fresh, h->d_carve, h->d_qavail};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete h;

@@ -26,6 +26,9 @@ struct pbm_handle {
     unsigned long long* d_scratch = nullptr;  // 8 words for reductions
     unsigned long long* d_scan = nullptr;     // rank scan: one look-back word per tile
     unsigned long long* d_scan_ctl = nullptr; // rank scan: next tile, tiles done, epoch
+    unsigned* d_fresh = nullptr;              // Ouroboros: free pages at the last reset
+    unsigned long long* d_carve = nullptr;    // Ouroboros: [0] carved, [1] fresh pages
+    long long* d_qavail = nullptr;            // Ouroboros: unpromised queued pages
 };
 
 // An instantiated HIP graph and the device it runs on.
@@ -41,6 +44,24 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 
+// Pages per chunk of the Ouroboros comparator: requests of page_size bytes
+// fall in the power-of-two size class of at least 16 bytes
+// (Ouroboros/include/device/Helper.cuh:42-60); a class up to 8 KiB is
+// carved from 8 KiB chunks, up to 32 KiB from 32 KiB chunks
+// (Ouroboros/include/Parameters.h:9-18, InstanceDefinitions.cuh:26-28);
+// larger pages are one per chunk.
+inline uint32_t ouroboros_chunk_pages(uint32_t page_size) {
+    uint64_t cls = 16;
+    while (cls < page_size) cls <<= 1;
+    if (cls <= 8192u) return (uint32_t)(8192u / cls);
+    if (cls <= 32768u) return (uint32_t)(32768u / cls);
+    return 1u;
+}
+
+// Strategies whose free pages live in a FIFO ring (LinkedList, Ouroboros):
+// no batched, multi-page, cache or burst calls on them.
+inline bool is_queue_strategy(int s) { return s == PBM_LINKED_LIST || s == PBM_OUROBOROS; }
+
 // Makes the handle's device current for the calling thread.
 int use_device(pbm_handle* h);
 
@@ -54,15 +75,17 @@ int reinit_strategy_state(pbm_handle* h, hipStream_t s);
 int recount_free(pbm_handle* h, hipStream_t s);
 
 // LinkedList: refills the FIFO ring with the free pages of the page map in
-// ascending order and resets head/tail tickets.
+// ascending order and resets head/tail tickets.  Ouroboros: empties the
+// ring and lists the free pages, ascending, as the pages chunks are carved
+// from.
 int ring_rebuild(pbm_handle* h, hipStream_t s);
 
 // One-page get launches of CRW_BM, RWMalloc, SingleClock, DistClock and
 // LinkedList (plain or serial), and the list's free launch.
 int launch_get_strategy(pbm_handle* h, const pbm_ctx& c, const pbm_launch& L, uint32_t n,
                         uint64_t req_base, int32_t* d_ids, uint32_t* d_steps, hipStream_t s);
-int launch_free_list(const pbm_ctx& c, const pbm_launch& L, uint32_t n, const int32_t* d_ids,
-                     hipStream_t s);
+int launch_free_list(const pbm_ctx& c, int strategy, const pbm_launch& L, uint32_t n,
+                     const int32_t* d_ids, hipStream_t s);
 
 }  // namespace pbmi
 

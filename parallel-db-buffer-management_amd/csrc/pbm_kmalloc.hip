@@ -101,7 +101,7 @@ extern "C" int pbm_kmalloc_create(pbm_handle* h, int32_t kind, uint32_t max_thre
     if (!h->d_pool) return pbmi::fail(PBM_ERR_UNSUPPORTED, "kernel malloc needs a pool with backing");
     if (h->cfg.page_size % 8u)
         return pbmi::fail(PBM_ERR_INVALID, "kernel malloc needs a page size that is a multiple of 8");
-    if (h->cfg.strategy == PBM_LINKED_LIST)
+    if (pbmi::is_queue_strategy(h->cfg.strategy))
         return pbmi::fail(PBM_ERR_UNSUPPORTED, "kernel malloc needs a bitmap strategy");
     pbm_kmalloc* k = new pbm_kmalloc();
     k->h = h;

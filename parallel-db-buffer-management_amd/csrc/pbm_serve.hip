@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
 // grow, cut into 64-page chunks dealt round-robin to the waves.  A chunk's
 // payloads are read back, checked and summed into the sequence's checksum,
 // then its pages go back to the pool.
-template <bool LIST>
+template <int S>
 __global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
     using namespace pbm;
     const unsigned lane = lane_id();
@@ -252,10 +252,7 @@ __global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
                 for (int o = 32; o >= 1; o >>= 1) sum += shfl64(sum, lane ^ (unsigned)o);
                 if (lane == 0) atom_add_u64(&d.res_sum[c_seq], sum);
             }
-            if (valid) {
-                if (LIST) free_page_list(d.c, id, nullptr, &st);
-                else free_page(d.c, id, &st);
-            }
+            if (valid) free_page_of<S>(d.c, id, &st);
         }
     }
     uint32_t fr = st.frees;
@@ -435,13 +432,17 @@ int launch_step(pbm_server* s, hipStream_t st) {
         PBM_GROW_CASE(PBM_RW_BASIC)
         PBM_GROW_CASE(PBM_CRW_BASIC)
         PBM_GROW_CASE(PBM_COLLAB_RW_BASIC)
+        PBM_GROW_CASE(PBM_OUROBOROS)
 #undef PBM_GROW_CASE
         default: return pbmi::fail(PBM_ERR_UNSUPPORTED, "no serving launch for this strategy");
     }
+    const dim3 rg(s->release_blocks), rb(256);
     if (s->h->cfg.strategy == PBM_LINKED_LIST)
-        hipLaunchKernelGGL(k_serve_release<true>, dim3(s->release_blocks), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_serve_release<PBM_LINKED_LIST>, rg, rb, 0, st, d);
+    else if (s->h->cfg.strategy == PBM_OUROBOROS)
+        hipLaunchKernelGGL(k_serve_release<PBM_OUROBOROS>, rg, rb, 0, st, d);
     else
-        hipLaunchKernelGGL(k_serve_release<false>, dim3(s->release_blocks), dim3(256), 0, st, d);
+        hipLaunchKernelGGL(k_serve_release<PBM_RW_BM>, rg, rb, 0, st, d);
     PBM_HIP(hipGetLastError());
     return PBM_OK;
 }

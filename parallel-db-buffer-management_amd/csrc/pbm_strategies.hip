@@ -54,13 +54,15 @@ __global__ __launch_bounds__(1024) void k_get_wave(pbm_ctx c, uint32_t n, uint64
     pbm::flush_stats(c, st);
 }
 
-__global__ __launch_bounds__(1024) void k_free_list(pbm_ctx c, uint32_t n,
+// Free for the ring strategies (LinkedList, Ouroboros).
+template <int S>
+__global__ __launch_bounds__(1024) void k_free_ring(pbm_ctx c, uint32_t n,
                                                     const int32_t* __restrict__ ids) {
     pbm::Stats st;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += stride) {
         int32_t id = ids[r];
-        if (id >= 0) pbm::free_page_list(c, id, nullptr, &st);
+        if (id >= 0) pbm::free_page_of<S>(c, id, &st);
     }
     pbm::flush_stats(c, st);
 }
@@ -86,6 +88,7 @@ int launch_get_strategy(pbm_handle* h, const pbm_ctx& c, const pbm_launch& L, ui
         PBM_THREAD_CASE(PBM_LINKED_LIST)
         PBM_THREAD_CASE(PBM_RW_BASIC)
         PBM_THREAD_CASE(PBM_CRW_BASIC)
+        PBM_THREAD_CASE(PBM_OUROBOROS)
 #undef PBM_THREAD_CASE
         case PBM_RW_MALLOC:
             hipLaunchKernelGGL(k_get_wave<PBM_RW_MALLOC>, wgrid, wblk, 0, s, c, n, req_base, d_ids,
@@ -102,11 +105,15 @@ int launch_get_strategy(pbm_handle* h, const pbm_ctx& c, const pbm_launch& L, ui
     return PBM_OK;
 }
 
-int launch_free_list(const pbm_ctx& c, const pbm_launch& L, uint32_t n, const int32_t* d_ids,
-                     hipStream_t s) {
+int launch_free_list(const pbm_ctx& c, int strategy, const pbm_launch& L, uint32_t n,
+                     const int32_t* d_ids, hipStream_t s) {
     const bool serial = L.mode == PBM_MODE_SERIAL;
-    hipLaunchKernelGGL(k_free_list, serial ? dim3(1) : dim3((n + L.block - 1) / L.block),
-                       serial ? dim3(1) : dim3(L.block), 0, s, c, n, d_ids);
+    const dim3 grid = serial ? dim3(1) : dim3((n + L.block - 1) / L.block);
+    const dim3 blk = serial ? dim3(1) : dim3(L.block);
+    if (strategy == PBM_OUROBOROS)
+        hipLaunchKernelGGL(k_free_ring<PBM_OUROBOROS>, grid, blk, 0, s, c, n, d_ids);
+    else
+        hipLaunchKernelGGL(k_free_ring<PBM_LINKED_LIST>, grid, blk, 0, s, c, n, d_ids);
     PBM_HIP(hipGetLastError());
     return PBM_OK;
 }
@@ -146,6 +153,28 @@ __global__ void k_ring_fill(const u64* bm, uint32_t n_words, const uint32_t* off
     }
 }
 
+// Ouroboros: the free pages of word w, ascending, at fresh positions
+// offs[w]..; the ring is empty and nothing is carved or queued yet.
+__global__ void k_fresh_fill(const u64* bm, uint32_t n_words, const uint32_t* offs,
+                             unsigned* fresh, u64* carve, u64* head_tail, long long* q_avail) {
+    uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n_words) return;
+    u64 fr = ~bm[w];
+    uint32_t pos = offs[w];
+    while (fr) {
+        unsigned b = (unsigned)__ffsll((long long)fr) - 1u;
+        fr &= fr - 1ull;
+        fresh[pos++] = w * 64u + b;
+    }
+    if (w == n_words - 1u) {
+        carve[0] = 0;
+        carve[1] = pos;
+        head_tail[0] = 0;
+        head_tail[1] = 0;
+        *q_avail = 0;
+    }
+}
+
 namespace pbmi {
 
 int ring_rebuild(pbm_handle* h, hipStream_t s) {
@@ -175,8 +204,12 @@ int ring_rebuild(pbm_handle* h, hipStream_t s) {
         }
         hipLaunchKernelGGL(k_ring_empty, dim3((c.n_pages + 255u) / 256u), dim3(256), 0, s,
                            h->d_ring, c.n_pages);
-        hipLaunchKernelGGL(k_ring_fill, dim3((L + 255u) / 256u), dim3(256), 0, s, h->d_bitmap, L,
-                           offs, h->d_ring, h->d_ring_ht);
+        if (h->cfg.strategy == PBM_OUROBOROS)
+            hipLaunchKernelGGL(k_fresh_fill, dim3((L + 255u) / 256u), dim3(256), 0, s, h->d_bitmap,
+                               L, offs, h->d_fresh, h->d_carve, h->d_ring_ht, h->d_qavail);
+        else
+            hipLaunchKernelGGL(k_ring_fill, dim3((L + 255u) / 256u), dim3(256), 0, s, h->d_bitmap,
+                               L, offs, h->d_ring, h->d_ring_ht);
         if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess) {
             rc = hip_fail(e, "ring rebuild", __FILE__, __LINE__);
             break;
@@ -289,7 +322,7 @@ int prep(pbm_handle* h, const pbm_launch* in, pbm_launch* L, pbm_ctx* c) {
     *c = h->ctx;
     if (x.walk_budget) c->walk_budget = x.walk_budget;
     if (x.flags & PBM_LAUNCH_NO_COUNTERS) c->counters = nullptr;
-    if (h->cfg.strategy == PBM_LINKED_LIST)
+    if (pbmi::is_queue_strategy(h->cfg.strategy))
         return pbmi::fail(PBM_ERR_UNSUPPORTED, "multi-page calls need a bitmap strategy");
     return PBM_OK;
 }

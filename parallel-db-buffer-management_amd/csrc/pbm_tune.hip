@@ -46,7 +46,7 @@ extern "C" int pbm_autotune(pbm_handle* h, uint32_t n, int32_t* d_ids, uint32_t 
                             pbm_tune_result* out, void* stream) {
     PBM_CHECK_HANDLE(h);
     if (!out || !d_ids || n == 0) return pbmi::fail(PBM_ERR_INVALID, "autotune needs n > 0, d_ids and out");
-    if (h->cfg.strategy == PBM_LINKED_LIST)
+    if (pbmi::is_queue_strategy(h->cfg.strategy))
         return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
     hipStream_t s = pbmi::as_stream(stream);
     int64_t before = 0;

@@ -17,12 +17,12 @@ LIB_PATH = PKG / "libpbm.so"
 
 # pbm_strategy
 (RW_BM, COLLAB_RW_BM, CRW_BM, RW_MALLOC, LINKED_LIST, SINGLE_CLOCK, DIST_CLOCK, RW_BASIC,
- CRW_BASIC, COLLAB_RW_BASIC) = range(10)
+ CRW_BASIC, COLLAB_RW_BASIC, OUROBOROS) = range(11)
 STRATEGY_NAMES = {
     RW_BM: "RandomWalkBasic_BM", COLLAB_RW_BM: "CollabRW_BM", CRW_BM: "CRW_BM",
     RW_MALLOC: "RWMalloc", LINKED_LIST: "LinkedList", SINGLE_CLOCK: "SingleClock",
     DIST_CLOCK: "DistClock", RW_BASIC: "RandomWalkBasic",
-    CRW_BASIC: "ClusteredRandomWalk", COLLAB_RW_BASIC: "CollabRW",
+    CRW_BASIC: "ClusteredRandomWalk", COLLAB_RW_BASIC: "CollabRW", OUROBOROS: "Ouroboros",
 }
 # pbm_order
 ORDER_PACKED, ORDER_REFERENCE = 0, 1
@@ -61,7 +61,9 @@ class Ctx(C.Structure):
                 ("scan_ctl", C.c_void_p),
                 ("n_pages", C.c_uint32), ("n_words", C.c_uint32), ("page_size", C.c_uint32),
                 ("walk_budget", C.c_uint32), ("max_threads", C.c_uint32), ("order", C.c_int32),
-                ("seed", C.c_uint64), ("n_tiles", C.c_uint32), ("scan_wpt", C.c_uint32)]
+                ("seed", C.c_uint64), ("n_tiles", C.c_uint32), ("scan_wpt", C.c_uint32),
+                ("fresh", C.c_void_p), ("carve", C.c_void_p), ("q_avail", C.c_void_p),
+                ("chunk_pages", C.c_uint32), ("reserved_", C.c_uint32)]
 
 
 class Launch(C.Structure):

@@ -29,3 +29,12 @@ def test_reference_kmalloc_driver(gpu):
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert proc.returncode == 0, proc.stdout + proc.stderr
     assert "compat kmalloc ok" in proc.stdout
+
+
+def test_reference_ouroboros_driver(gpu):
+    exe = pbm_build.compat_binary("ouroboros")
+    assert exe.exists(), f"{exe} is missing: run __graft_entry__.build()"
+    log_case("compat Ouroboros driver")
+    proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert "compat ouroboros ok" in proc.stdout

@@ -40,7 +40,8 @@ def _check_results(res, keys, target, payload=True):
 
 @pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.COLLAB_RW_BM, pbm.CRW_BM, pbm.RW_MALLOC,
                                       pbm.SINGLE_CLOCK, pbm.DIST_CLOCK, pbm.LINKED_LIST,
-                                      pbm.RW_BASIC, pbm.CRW_BASIC, pbm.COLLAB_RW_BASIC])
+                                      pbm.RW_BASIC, pbm.CRW_BASIC, pbm.COLLAB_RW_BASIC,
+                                      pbm.OUROBOROS])
 def test_serve_schedule_matches_oracle(gpu, strategy):
     T, n = 1 << 16, 2500
     log_case(f"serve schedule strategy={strategy}")

@@ -844,6 +844,44 @@ def test_rwbasic_mean_steps_match_exact_expectation_and_rw_tas():
     assert abs(orc.rw_tas(T, A, N) - mean) / mean < 2e-3
 
 
+# ------------------------------------------------------------------ Ouroboros queue
+
+def test_ouroboros_chunk_pages_follow_the_size_classes():
+    # queue index = max(ceil(log2 size) - 4, 0) and 512 >> index pages per
+    # 8 KiB chunk (Ouroboros/include/device/Helper.cuh:42-60); the second
+    # instance carves 32 KiB chunks for the 16 and 32 KiB classes
+    table = {8: 512, 16: 512, 17: 256, 264: 16, 512: 16, 4096: 2, 4097: 1, 8192: 1,
+             10000: 2, 16384: 2, 32768: 1, 65536: 1}
+    for ps, want in table.items():
+        assert orc.ouroboros_chunk_pages(ps) == want, ps
+
+
+def test_queue_known_answer_fifo_with_lazy_chunks():
+    q = orc.QueueState(orc.cleared(16), 16, 2)
+    ids, steps = q.get(3)
+    assert ids.tolist() == [0, 1, 2] and steps.tolist() == [2, 1, 2]   # two chunks carved
+    assert q.queued() == [3]
+    assert q.put(np.array([1, 0, 9], np.int32)) == 2                   # 9 is free: skipped
+    assert q.queued() == [3, 1, 0]
+    ids, steps = q.get(4)
+    assert ids.tolist() == [3, 1, 0, 4] and steps.tolist() == [1, 1, 1, 2]
+    assert q.queued() == [5]
+    assert orc.count_free(q.bm, 16) == 16 - 5 == q.free[0]
+
+
+def test_queue_carves_the_free_pages_in_order_and_runs_dry():
+    T = 4096
+    bm = orc.prefill(T, 0.7, 3)
+    A = orc.count_free(bm, T)
+    free_pages = [p for p in range(T) if not (int(bm[p // 64]) >> (p % 64)) & 1]
+    q = orc.QueueState(bm, T, 16)
+    ids, steps = q.get(A + 5)
+    assert ids[:A].tolist() == free_pages
+    assert (ids[A:] == -1).all() and (steps[A:] == 0).all()
+    assert (steps[:A] == np.where(np.arange(A) % 16 == 0, 2, 1)).all()
+    assert q.free[0] == 0 and orc.count_free(q.bm, T) == 0
+
+
 # ------------------------------------------------------------------ kernel malloc headers
 
 def _u64(mem, at):
