@@ -17,7 +17,11 @@ launch).  The batched launch (rank or
 walk homes, window width, walk budget, waves per CU) comes from the
 library's autotuner; the best walk-homes launch is timed too.  The plain
 variant is the reference's RandomWalkBasic_BM getPage/freePage, one thread
-per request.  The CPU baseline is the serial oracle running the reference's
+per request.  The same step under every other one-page policy (the other
+walks, the clocks, the FIFO list and the Ouroboros page queue) is timed with
+plain launches as comparators, and the kernel-malloc managers run the
+reference's malloc benchmark.
+The CPU baseline is the serial oracle running the reference's
 RandomWalkBasic_BM policy on the identical pre-fill and request stream, on
 one host core.
 
@@ -200,6 +204,75 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int) -> dict:
     return out
 
 
+def comparator_leg(torch, pbm, stream, strategy: int, steps: int) -> dict:
+    """The alloc+free step of the headline workload under another one-page
+    policy, plain launches (one thread per request, wave-collective
+    policies collective over each wave): the other bitmap walks, the clocks,
+    the FIFO list (parallelPage.cu:342-418) and the Ouroboros page queue
+    (Ouroboros.cuh:22-62)."""
+    pool = pbm.Pool(T_PAGES, PAGE_SIZE, strategy=strategy, seed=SEED)
+    pool.prefill_bernoulli(FILL, SEED, stream)
+    before = pool.snapshot(stream)
+    granted = min(N_REQ, pool.free_count(stream))
+    ids = torch.full((N_REQ,), -1, dtype=torch.int32, device="cuda")
+    lp = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
+
+    def step():
+        pool.get_pages(N_REQ, ids, None, 0, lp, stream)
+        pool.free_pages(N_REQ, ids, lp, stream)
+
+    ms = time_region(torch, step, steps, 2, stream)
+    ok = bool(np.array_equal(pool.snapshot(stream), before)) and pool.counters(stream)["errors"] == 0
+    pool.close()
+    progress(f"comparator {pbm.STRATEGY_NAMES[strategy]} measured")
+    return {"value": 2 * granted / (ms * 1e-3), "ms_per_step": ms, "steps": steps,
+            "verified": ok}
+
+
+def kmalloc_leg(torch, pbm, stream, n: int, steps: int) -> dict:
+    """KernelMallocCollab malloc + free and KernelMallocSimple malloc of the
+    reference's malloc benchmark (unitTests/unitTestMalloc1.cu:14-41):
+    n requests, request t asking (1103515245 t + 12345) % 8188 + 4 bytes,
+    on 262,144 x 4 KiB pages (page memory allocated), empty at the start.
+    A wave serves 64 consecutive requests (the reference's 32-thread blocks
+    are one warp each)."""
+    T = 1 << 18
+    t = np.arange(n, dtype=np.int64)
+    sizes = ((1103515245 * t + 12345) % 8188 + 4).astype(np.uint32)
+    d_sizes = torch.from_numpy(sizes.view(np.int32)).cuda()
+    ptrs = torch.zeros((n,), dtype=torch.int64, device="cuda")
+    lp = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
+    out = {"requests": n, "sizes": "(1103515245 t + 12345) % 8188 + 4 bytes", "pages": T}
+    with pbm.Pool(T, 4096, strategy=pbm.RW_BM, backing=True, seed=SEED) as pool:
+        with pbm.KMalloc(pool, pbm.KMALLOC_COLLAB) as km:
+            def cycle():
+                km.malloc(n, d_sizes, ptrs, None, 0, lp, stream)
+                km.free(n, ptrs, lp, stream)
+            ms = time_region(torch, cycle, steps, 2, stream)
+            ok = pool.count_used(stream) == 0 and pool.counters(stream)["errors"] == 0
+            out["collab"] = {"value": 2 * n / (ms * 1e-3), "unit": "mallocs+frees/s",
+                             "ms_per_cycle": ms, "verified": bool(ok)}
+        with pbm.KMalloc(pool, pbm.KMALLOC_SIMPLE, max_threads=n) as km:
+            total = 0.0
+            for k in range(steps + 2):
+                km.reset(stream)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                km.malloc(n, d_sizes, ptrs, None, 0, lp, stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                if k >= 2:
+                    total += e0.elapsed_time(e1)
+            p = ptrs.cpu().numpy()
+            ms = total / steps
+            out["simple"] = {"value": n / (ms * 1e-3), "unit": "mallocs/s", "ms_per_call": ms,
+                             "verified": bool((p != 0).all())}
+            km.reset(stream)
+    progress("kernel malloc measured")
+    return out
+
+
 def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
     """Buffer cache: 16,777,216 x 4 KiB frames (page map only) allocated
     with CRW_BM, 4,194,304 logical pages held in 1,048,576 slots (the
@@ -255,6 +328,10 @@ def main() -> int:
     ap.add_argument("--cache-refs", type=int, default=1 << 26,
                     help="references of the buffer-cache leg (0 = skip)")
     ap.add_argument("--cache-batch", type=int, default=1 << 16)
+    ap.add_argument("--comparators", type=int, default=1,
+                    help="time the headline step under every other one-page policy (0 = skip)")
+    ap.add_argument("--kmalloc", type=int, default=30000,
+                    help="requests of the kernel-malloc leg (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the line, copy a device buffer of this many MiB once "
                          "(the known-size reference kernel for HBM byte counters)")
@@ -358,6 +435,14 @@ def main() -> int:
     cp_alloc, cp_all, ws_p, ok_p = (rp["counters_alloc"], rp["counters_cycle"], rp["walk_steps"],
                                     rp["verified"])
 
+    comparators = {}
+    if args.comparators:
+        for s_ in (pbm.COLLAB_RW_BM, pbm.CRW_BM, pbm.RW_MALLOC, pbm.LINKED_LIST,
+                   pbm.SINGLE_CLOCK, pbm.DIST_CLOCK, pbm.RW_BASIC, pbm.CRW_BASIC,
+                   pbm.COLLAB_RW_BASIC, pbm.OUROBOROS):
+            comparators[pbm.STRATEGY_NAMES[s_]] = comparator_leg(torch, pbm, stream, s_, p_steps)
+    kmalloc = kmalloc_leg(torch, pbm, stream, args.kmalloc, 10) if args.kmalloc > 0 else None
+
     b = results["batched"]
     ms_b = b["ms_per_step"]
     ms_p = rp["ms_per_step"]
@@ -440,6 +525,7 @@ def main() -> int:
                   "probes": cp_alloc["probes"], "atomics": cp_all["atomics"],
                   "verified": ok_p},
         "speedup_vs_plain": value / value_p,
+        "comparators": comparators,
         "batched_walk": {"value": ops_per_step / (results["batched_walk"]["ms_per_step"] * 1e-3),
                          **{k: results["batched_walk"][k] for k in
                             ("ms_per_step", "ms_per_step_graph", "phases_ms", "walk_steps",
@@ -456,6 +542,7 @@ def main() -> int:
         "serving": serving,
         "bursts": bursts,
         "cache": cache,
+        "kmalloc": kmalloc,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:
@@ -465,7 +552,10 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
     pool.close()
-    return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]) else 1
+    return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]
+                 and all(v["verified"] for v in comparators.values())
+                 and (kmalloc is None or (kmalloc["collab"]["verified"]
+                                          and kmalloc["simple"]["verified"]))) else 1
 
 
 if __name__ == "__main__":
