@@ -114,6 +114,61 @@ def compute() -> dict:
                             "offset": off.tolist(), "header": hdr.tolist(),
                             "steps": steps.tolist()}
 
+    # RandomWalkBasic and the legacy clustered / collaborative page walks
+    st = pool(3000, 0.8, 9)
+    ids, steps = st.rwbasic(200, 0, 4)
+    g["rwbasic_3000_0.8"] = {"T": 3000, "fill": 0.8, "prefill_seed": 9, "n": 200, "seed": 4,
+                             "ids": ids.tolist(), "steps": steps.tolist()}
+    st = pool(3000, 0.8, 9)
+    ids, steps, hint = st.crw_basic(200, 0, 4)
+    g["crw_basic_3000_0.8"] = {"T": 3000, "fill": 0.8, "prefill_seed": 9, "n": 200, "seed": 4,
+                               "ids": ids.tolist(), "steps": steps.tolist(), "hint": hint}
+    st = pool(3000, 0.8, 9)
+    ids, steps = st.collab_basic(200, 0, 4)
+    g["collab_basic_3000_0.8"] = {"T": 3000, "fill": 0.8, "prefill_seed": 9, "n": 200,
+                                  "seed": 4, "ids": ids.tolist(), "steps": steps.tolist()}
+
+    # per-thread getXPage, runs of 1..12 pages on a 40% pool
+    xc = rng.integers(1, 13, size=80).astype(np.uint32)
+    st = pool(4096, 0.4, 10)
+    ids, steps = st.get_x(xc, 0, 6)
+    g["getxpage_4096_0.4"] = {"T": 4096, "fill": 0.4, "prefill_seed": 10, "seed": 6,
+                              "counts": xc.tolist(), "ids": ids.tolist(),
+                              "steps": steps.tolist()}
+
+    # kernel malloc headers: the bytes written, as (offset, 8-byte value)
+    def headers(mem, wr):
+        at = np.flatnonzero(wr[::8]) * 8
+        return [[int(a), int.from_bytes(mem[a:a + 8].tobytes(), "little")] for a in at]
+
+    T, ps = 1024, 4096
+    ksz = np.exp(rng.uniform(0, np.log(9000), size=100)).astype(np.uint32)
+    st = pool(T, 0.5, 11)
+    mem, wr = np.zeros(T * ps, np.uint8), np.zeros(T * ps, np.uint8)
+    off, steps = st.kmalloc_collab(mem, wr, ksz, ps, 0, 7)
+    g["kmalloc_collab_1024_0.5"] = {"T": T, "fill": 0.5, "prefill_seed": 11, "seed": 7,
+                                    "page_size": ps, "sizes": ksz.tolist(),
+                                    "offsets": [int(o) for o in off], "steps": steps.tolist(),
+                                    "headers": headers(mem, wr)}
+    st = pool(T, 0.5, 11)
+    mem, wr = np.zeros(T * ps, np.uint8), np.zeros(T * ps, np.uint8)
+    off, steps, cur = st.kmalloc_simple(mem, wr, ksz, ps, 0, 7)
+    g["kmalloc_simple_1024_0.5"] = {"T": T, "fill": 0.5, "prefill_seed": 11, "seed": 7,
+                                    "page_size": ps, "sizes": ksz.tolist(),
+                                    "offsets": [int(o) for o in off], "steps": steps.tolist(),
+                                    "current": cur, "headers": headers(mem, wr)}
+
+    # Ouroboros queue at 512-byte pages (16 per chunk): gets, frees, gets
+    q = orc.QueueState(orc.prefill(2048, 0.6, 12), 2048, orc.ouroboros_chunk_pages(512))
+    a, sa = q.get(120)
+    back = a[a >= 0][::3].copy()
+    q.put(back)
+    b, sb = q.get(120)
+    g["ouroboros_2048_0.6"] = {"T": 2048, "page_size": 512, "fill": 0.6, "prefill_seed": 12,
+                               "first": a.tolist(), "first_steps": sa.tolist(),
+                               "freed": back.tolist(), "second": b.tolist(),
+                               "second_steps": sb.tolist()}
+
     return g
 
 

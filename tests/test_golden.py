@@ -45,3 +45,39 @@ def test_golden_rwbm_ids_are_unique_and_were_free():
     ids = [i for i in g["ids"] if i >= 0]
     assert len(set(ids)) == len(ids)
     assert all((bm[i // 64] >> (i % 64)) & 1 for i in ids)
+
+
+def test_golden_crw_basic_takes_the_next_page_when_free():
+    # after a grant the next request probes the page after the hint first
+    # and counts one step when it is free (parallelPage.cu:200-211)
+    g = GOLDEN["crw_basic_3000_0.8"]
+    ids, steps = g["ids"], g["steps"]
+    for k in range(1, len(ids)):
+        if ids[k] == (ids[k - 1] + 1) % g["T"]:
+            assert steps[k] == 1
+    assert g["hint"] == ids[-1]
+
+
+def test_golden_kmalloc_collab_headers_are_consistent():
+    g = GOLDEN["kmalloc_collab_1024_0.5"]
+    hdr = dict((a, v) for a, v in g["headers"])
+    none = 2 ** 64 - 1
+    for o, size in zip(g["offsets"], g["sizes"]):
+        if o == none:
+            continue
+        h = hdr[o - 8]
+        s = ((size + 7) & ~7) + 8
+        if h >> 63:                                   # a chunk of its own
+            assert (h >> 16) & 0xFFFF == s & 0xFFFF and (o - 8) % g["page_size"] == 0
+        else:                                         # a member: offset, bytes, chunk
+            assert (h >> 32) == s and (o - 8 - (h & 0xFFFFFFFF)) in hdr
+
+
+def test_golden_ouroboros_is_fifo():
+    # after the frees the queue holds the leftover carved pages, then the
+    # freed pages in free order
+    g = GOLDEN["ouroboros_2048_0.6"]
+    freed = g["freed"]
+    second = [i for i in g["second"] if i >= 0]
+    pos = [second.index(f) for f in freed if f in second]
+    assert pos == sorted(pos)

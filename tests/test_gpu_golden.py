@@ -136,3 +136,69 @@ def test_golden_malloc(gpu):
         assert np.array_equal(p > 0, ok)
         assert np.array_equal(p[ok] - 4 - pool.page_address(0), page[ok] * ps + off[ok])
         assert steps.cpu().numpy().view(np.uint32).tolist() == g["steps"]
+
+
+@pytest.mark.parametrize("key,strategy", [("rwbasic_3000_0.8", pbm.RW_BASIC),
+                                          ("crw_basic_3000_0.8", pbm.CRW_BASIC),
+                                          ("collab_basic_3000_0.8", pbm.COLLAB_RW_BASIC)])
+def test_golden_page_walks(gpu, key, strategy):
+    g = G[key]
+    log_case(f"golden {key}")
+    with pbm.Pool(g["T"], strategy=strategy, seed=g["seed"]) as pool:
+        pool.prefill_bernoulli(g["fill"], g["prefill_seed"])
+        ids, steps = _get(gpu, pool, g["n"], pbm.MODE_SERIAL)
+        assert ids == g["ids"] and steps == g["steps"]
+
+
+def test_golden_getxpage(gpu):
+    g = G["getxpage_4096_0.4"]
+    log_case("golden getXPage")
+    n = len(g["counts"])
+    with pbm.Pool(g["T"], strategy=pbm.RW_BM, seed=g["seed"]) as pool:
+        pool.prefill_bernoulli(g["fill"], g["prefill_seed"])
+        counts = gpu.tensor(g["counts"], dtype=gpu.int32, device="cuda")
+        ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
+        steps = gpu.zeros((n,), dtype=gpu.int32, device="cuda")
+        pool.get_x_pages(n, counts, ids, steps, 0, pbm.launch(mode=pbm.MODE_SERIAL))
+        gpu.cuda.synchronize()
+        assert ids.cpu().numpy().tolist() == g["ids"]
+        assert steps.cpu().numpy().view(np.uint32).tolist() == g["steps"]
+
+
+@pytest.mark.parametrize("key,kind", [("kmalloc_collab_1024_0.5", pbm.KMALLOC_COLLAB),
+                                      ("kmalloc_simple_1024_0.5", pbm.KMALLOC_SIMPLE)])
+def test_golden_kmalloc_headers(gpu, key, kind):
+    g = G[key]
+    log_case(f"golden {key}")
+    n, ps = len(g["sizes"]), g["page_size"]
+    with pbm.Pool(g["T"], ps, strategy=pbm.RW_BM, backing=True, seed=g["seed"]) as pool:
+        pool.prefill_bernoulli(g["fill"], g["prefill_seed"])
+        with pbm.KMalloc(pool, kind, max_threads=64) as km:
+            sizes = gpu.tensor(g["sizes"], dtype=gpu.int32, device="cuda")
+            ptrs = gpu.zeros((n,), dtype=gpu.int64, device="cuda")
+            steps = gpu.zeros((n,), dtype=gpu.int32, device="cuda")
+            km.malloc(n, sizes, ptrs, steps, 0, pbm.launch(mode=pbm.MODE_SERIAL))
+            gpu.cuda.synchronize()
+            p = ptrs.cpu().numpy().view(np.uint64)
+            base = np.uint64(pool.page_address(0))
+            off = np.where(p == 0, np.uint64(2 ** 64 - 1), p - base)
+            assert [int(o) for o in off] == g["offsets"]
+            assert steps.cpu().numpy().view(np.uint32).tolist() == g["steps"]
+            mem = pool.read_pool(0, g["T"] * ps)
+            for at, val in g["headers"]:
+                assert int.from_bytes(mem[at:at + 8].tobytes(), "little") == val, at
+            if kind == pbm.KMALLOC_SIMPLE:
+                assert km.current()[0] == g["current"]
+
+
+def test_golden_ouroboros(gpu):
+    g = G["ouroboros_2048_0.6"]
+    log_case("golden Ouroboros")
+    with pbm.Pool(g["T"], g["page_size"], strategy=pbm.OUROBOROS) as pool:
+        pool.prefill_bernoulli(g["fill"], g["prefill_seed"])
+        a, sa = _get(gpu, pool, len(g["first"]), pbm.MODE_SERIAL)
+        assert a == g["first"] and sa == g["first_steps"]
+        back = gpu.tensor(g["freed"], dtype=gpu.int32, device="cuda")
+        pool.free_pages(len(g["freed"]), back, pbm.launch(mode=pbm.MODE_SERIAL))
+        b, sb = _get(gpu, pool, len(g["second"]), pbm.MODE_SERIAL)
+        assert b == g["second"] and sb == g["second_steps"]
