@@ -229,6 +229,54 @@ def comparator_leg(torch, pbm, stream, strategy: int, steps: int) -> dict:
             "verified": ok}
 
 
+def fill_sweep_leg(torch, pbm, stream, fills, steps: int) -> dict:
+    """The 1,048,576-page configuration at each pre-fill f: plain
+    RandomWalkBasic_BM and wave64 CollabRW_BM getPage (one thread per
+    request) against the rank-batched launch, 262,144 requests then frees.
+    Per policy: walk steps per grant (TAS, WAS over 64- and 32-lane
+    groups), failures against N - A, allocs+frees/s."""
+    out = {}
+    for f in fills:
+        row = {}
+        for name, strategy, lnch in (
+                ("plain_rw_bm", pbm.RW_BM, pbm.launch(mode=pbm.MODE_PLAIN)),
+                ("plain_collab_rw_bm", pbm.COLLAB_RW_BM, pbm.launch(mode=pbm.MODE_PLAIN)),
+                ("batched_rank", pbm.RW_BM,
+                 pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK))):
+            pool = pbm.Pool(T_PAGES, PAGE_SIZE, strategy=strategy, seed=SEED)
+            pool.prefill_bernoulli(f, SEED, stream)
+            before = pool.snapshot(stream)
+            A = pool.free_count(stream)
+            ids = torch.full((N_REQ,), -1, dtype=torch.int32, device="cuda")
+            stp = torch.zeros((N_REQ,), dtype=torch.int32, device="cuda")
+            fl = lnch if lnch.mode == pbm.MODE_BATCHED else pbm.launch(mode=pbm.MODE_PLAIN)
+            pool.get_pages(N_REQ, ids, stp, 0, lnch, stream)
+            torch.cuda.synchronize()
+            i_np = ids.cpu().numpy()
+            ws = walk_stats(stp.cpu().numpy().view(np.uint32), i_np)
+            fails = int((i_np < 0).sum())
+            pool.free_pages(N_REQ, ids, fl, stream)
+            quiet = pbm.Launch(lnch.mode, lnch.block, lnch.waves_per_cu, lnch.walk_budget,
+                               lnch.cluster_words, lnch.flags | pbm.LAUNCH_NO_COUNTERS)
+            qf = pbm.Launch(fl.mode, fl.block, fl.waves_per_cu, fl.walk_budget, fl.cluster_words,
+                            fl.flags | pbm.LAUNCH_NO_COUNTERS)
+
+            def step():
+                pool.get_pages(N_REQ, ids, None, 0, quiet, stream)
+                pool.free_pages(N_REQ, ids, qf, stream)
+
+            ms = time_region(torch, step, steps, 1, stream)
+            ok = (bool(np.array_equal(pool.snapshot(stream), before))
+                  and fails == max(0, N_REQ - A) and pool.counters(stream)["errors"] == 0)
+            row[name] = {"walk_steps": ws, "failures": fails, "expected_failures": max(0, N_REQ - A),
+                         "value": 2 * min(N_REQ, A) / (ms * 1e-3), "ms_per_step": ms,
+                         "verified": ok}
+            pool.close()
+        out[str(f)] = row
+        progress(f"fill sweep: f={f} measured")
+    return out
+
+
 def kmalloc_leg(torch, pbm, stream, n: int, steps: int) -> dict:
     """KernelMallocCollab malloc + free and KernelMallocSimple malloc of the
     reference's malloc benchmark (unitTests/unitTestMalloc1.cu:14-41):
@@ -330,6 +378,8 @@ def main() -> int:
     ap.add_argument("--cache-batch", type=int, default=1 << 16)
     ap.add_argument("--comparators", type=int, default=1,
                     help="time the headline step under every other one-page policy (0 = skip)")
+    ap.add_argument("--fill-sweep", type=int, default=1,
+                    help="plain RW_BM / CollabRW_BM / rank at fills 0.5, 0.9, 0.99 (0 = skip)")
     ap.add_argument("--kmalloc", type=int, default=30000,
                     help="requests of the kernel-malloc leg (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -442,6 +492,8 @@ def main() -> int:
                    pbm.COLLAB_RW_BASIC, pbm.OUROBOROS):
             comparators[pbm.STRATEGY_NAMES[s_]] = comparator_leg(torch, pbm, stream, s_, p_steps)
     kmalloc = kmalloc_leg(torch, pbm, stream, args.kmalloc, 10) if args.kmalloc > 0 else None
+    sweep = (fill_sweep_leg(torch, pbm, stream, (0.5, 0.9, 0.99), p_steps)
+             if args.fill_sweep else None)
 
     b = results["batched"]
     ms_b = b["ms_per_step"]
@@ -543,6 +595,7 @@ def main() -> int:
         "bursts": bursts,
         "cache": cache,
         "kmalloc": kmalloc,
+        "fill_sweep": sweep,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:
@@ -555,7 +608,9 @@ def main() -> int:
     return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]
                  and all(v["verified"] for v in comparators.values())
                  and (kmalloc is None or (kmalloc["collab"]["verified"]
-                                          and kmalloc["simple"]["verified"]))) else 1
+                                          and kmalloc["simple"]["verified"]))
+                 and (sweep is None or all(v["verified"] for row in sweep.values()
+                                           for v in row.values()))) else 1
 
 
 if __name__ == "__main__":
