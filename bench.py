@@ -186,7 +186,14 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int) -> dict:
     for g in graphs:
         g.close()
     ptrs = bs.ptrs(stream=stream)
-    n_req = int(tr["alloc_offset"][-1]) + int(tr["free_offset"][-1])
+    n_malloc, n_free = int(tr["alloc_offset"][-1]), int(tr["free_offset"][-1])
+    n_req = n_malloc + n_free
+    # algorithmic bytes: a mallocRW reads its map word (8 B), claims with one
+    # read-modify-write (16 B), writes its 4-byte header and its 8-byte
+    # pointer; a freeRW reads the pointer and the header and releases with
+    # one read-modify-write
+    alg = n_malloc * (8 + 16 + 4 + 8) + n_free * (8 + 4 + 16)
+    achieved = alg / (ms * 1e-3) / 1e9
     out = {
         "workload": f"{n_bursts} bursts, lognormal count median 2048 sigma 1.0, 50/50 "
                     "mallocRW (16 B..64 KiB log-uniform) / freeRW, 262,144 x 4 KiB pages "
@@ -198,6 +205,9 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int) -> dict:
         "tune_ms_first_bursts": tune["ms"], "tune_candidates": tune["candidates"],
         "barrier_timeouts": bs.barrier_timeouts(stream),
         "pages_live_at_end": pool.count_used(stream) - used0,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes": alg, "per_malloc": 36, "per_free": 28},
     }
     bs.close()
     pool.close()
@@ -229,50 +239,55 @@ def comparator_leg(torch, pbm, stream, strategy: int, steps: int) -> dict:
             "verified": ok}
 
 
+def policy_rows(torch, pbm, stream, n_pages: int, n_req: int, fill: float, steps: int) -> dict:
+    """One configuration under plain RandomWalkBasic_BM and wave64
+    CollabRW_BM getPage (one thread per request) and the rank-batched
+    launch: n_req requests then frees on n_pages x 4 KiB pages with a
+    Bernoulli pre-fill f (seed 42).  Per policy: walk steps per grant (TAS,
+    WAS over 64- and 32-lane groups, max), failures against N - A,
+    allocs+frees/s; verified = unique grants (the map returns exactly) and
+    failures = max(0, N - A)."""
+    row = {}
+    for name, strategy, lnch in (
+            ("plain_rw_bm", pbm.RW_BM, pbm.launch(mode=pbm.MODE_PLAIN)),
+            ("plain_collab_rw_bm", pbm.COLLAB_RW_BM, pbm.launch(mode=pbm.MODE_PLAIN)),
+            ("batched_rank", pbm.RW_BM, pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK))):
+        pool = pbm.Pool(n_pages, PAGE_SIZE, strategy=strategy, seed=SEED)
+        pool.prefill_bernoulli(fill, SEED, stream)
+        before = pool.snapshot(stream)
+        A = pool.free_count(stream)
+        ids = torch.full((n_req,), -1, dtype=torch.int32, device="cuda")
+        stp = torch.zeros((n_req,), dtype=torch.int32, device="cuda")
+        fl = lnch if lnch.mode == pbm.MODE_BATCHED else pbm.launch(mode=pbm.MODE_PLAIN)
+        pool.get_pages(n_req, ids, stp, 0, lnch, stream)
+        torch.cuda.synchronize()
+        i_np = ids.cpu().numpy()
+        ws = walk_stats(stp.cpu().numpy().view(np.uint32), i_np)
+        fails = int((i_np < 0).sum())
+        pool.free_pages(n_req, ids, fl, stream)
+        quiet = pbm.Launch(lnch.mode, lnch.block, lnch.waves_per_cu, lnch.walk_budget,
+                           lnch.cluster_words, lnch.flags | pbm.LAUNCH_NO_COUNTERS)
+        qf = pbm.Launch(fl.mode, fl.block, fl.waves_per_cu, fl.walk_budget, fl.cluster_words,
+                        fl.flags | pbm.LAUNCH_NO_COUNTERS)
+
+        def step():
+            pool.get_pages(n_req, ids, None, 0, quiet, stream)
+            pool.free_pages(n_req, ids, qf, stream)
+
+        ms = time_region(torch, step, steps, 1, stream)
+        ok = (bool(np.array_equal(pool.snapshot(stream), before))
+              and fails == max(0, n_req - A) and pool.counters(stream)["errors"] == 0)
+        row[name] = {"walk_steps": ws, "failures": fails, "expected_failures": max(0, n_req - A),
+                     "value": 2 * min(n_req, A) / (ms * 1e-3), "ms_per_step": ms, "verified": ok}
+        pool.close()
+    return row
+
+
 def fill_sweep_leg(torch, pbm, stream, fills, steps: int) -> dict:
-    """The 1,048,576-page configuration at each pre-fill f: plain
-    RandomWalkBasic_BM and wave64 CollabRW_BM getPage (one thread per
-    request) against the rank-batched launch, 262,144 requests then frees.
-    Per policy: walk steps per grant (TAS, WAS over 64- and 32-lane
-    groups), failures against N - A, allocs+frees/s."""
+    """The 1,048,576-page, 262,144-request configuration at each pre-fill f."""
     out = {}
     for f in fills:
-        row = {}
-        for name, strategy, lnch in (
-                ("plain_rw_bm", pbm.RW_BM, pbm.launch(mode=pbm.MODE_PLAIN)),
-                ("plain_collab_rw_bm", pbm.COLLAB_RW_BM, pbm.launch(mode=pbm.MODE_PLAIN)),
-                ("batched_rank", pbm.RW_BM,
-                 pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK))):
-            pool = pbm.Pool(T_PAGES, PAGE_SIZE, strategy=strategy, seed=SEED)
-            pool.prefill_bernoulli(f, SEED, stream)
-            before = pool.snapshot(stream)
-            A = pool.free_count(stream)
-            ids = torch.full((N_REQ,), -1, dtype=torch.int32, device="cuda")
-            stp = torch.zeros((N_REQ,), dtype=torch.int32, device="cuda")
-            fl = lnch if lnch.mode == pbm.MODE_BATCHED else pbm.launch(mode=pbm.MODE_PLAIN)
-            pool.get_pages(N_REQ, ids, stp, 0, lnch, stream)
-            torch.cuda.synchronize()
-            i_np = ids.cpu().numpy()
-            ws = walk_stats(stp.cpu().numpy().view(np.uint32), i_np)
-            fails = int((i_np < 0).sum())
-            pool.free_pages(N_REQ, ids, fl, stream)
-            quiet = pbm.Launch(lnch.mode, lnch.block, lnch.waves_per_cu, lnch.walk_budget,
-                               lnch.cluster_words, lnch.flags | pbm.LAUNCH_NO_COUNTERS)
-            qf = pbm.Launch(fl.mode, fl.block, fl.waves_per_cu, fl.walk_budget, fl.cluster_words,
-                            fl.flags | pbm.LAUNCH_NO_COUNTERS)
-
-            def step():
-                pool.get_pages(N_REQ, ids, None, 0, quiet, stream)
-                pool.free_pages(N_REQ, ids, qf, stream)
-
-            ms = time_region(torch, step, steps, 1, stream)
-            ok = (bool(np.array_equal(pool.snapshot(stream), before))
-                  and fails == max(0, N_REQ - A) and pool.counters(stream)["errors"] == 0)
-            row[name] = {"walk_steps": ws, "failures": fails, "expected_failures": max(0, N_REQ - A),
-                         "value": 2 * min(N_REQ, A) / (ms * 1e-3), "ms_per_step": ms,
-                         "verified": ok}
-            pool.close()
-        out[str(f)] = row
+        out[str(f)] = policy_rows(torch, pbm, stream, T_PAGES, N_REQ, f, steps)
         progress(f"fill sweep: f={f} measured")
     return out
 
@@ -379,7 +394,8 @@ def main() -> int:
     ap.add_argument("--comparators", type=int, default=1,
                     help="time the headline step under every other one-page policy (0 = skip)")
     ap.add_argument("--fill-sweep", type=int, default=1,
-                    help="plain RW_BM / CollabRW_BM / rank at fills 0.5, 0.9, 0.99 (0 = skip)")
+                    help="plain RW_BM / CollabRW_BM / rank at fills 0.5, 0.9, 0.99, and on "
+                         "65,536 empty pages (0 = skip)")
     ap.add_argument("--kmalloc", type=int, default=30000,
                     help="requests of the kernel-malloc leg (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -494,6 +510,11 @@ def main() -> int:
     kmalloc = kmalloc_leg(torch, pbm, stream, args.kmalloc, 10) if args.kmalloc > 0 else None
     sweep = (fill_sweep_leg(torch, pbm, stream, (0.5, 0.9, 0.99), p_steps)
              if args.fill_sweep else None)
+    c1 = None
+    if args.fill_sweep:
+        c1 = {"workload": "65,536 x 4 KiB pages, empty, 32,768 requests then frees",
+              **policy_rows(torch, pbm, stream, 1 << 16, 1 << 15, 0.0, p_steps)}
+        progress("65,536-page configuration measured")
 
     b = results["batched"]
     ms_b = b["ms_per_step"]
@@ -596,6 +617,7 @@ def main() -> int:
         "cache": cache,
         "kmalloc": kmalloc,
         "fill_sweep": sweep,
+        "small_pool": c1,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:
@@ -610,7 +632,9 @@ def main() -> int:
                  and (kmalloc is None or (kmalloc["collab"]["verified"]
                                           and kmalloc["simple"]["verified"]))
                  and (sweep is None or all(v["verified"] for row in sweep.values()
-                                           for v in row.values()))) else 1
+                                           for v in row.values()))
+                 and (c1 is None or all(v["verified"] for k, v in c1.items()
+                                        if k != "workload"))) else 1
 
 
 if __name__ == "__main__":
