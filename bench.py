@@ -11,9 +11,11 @@ Workload (one step = one alloc phase + one free phase, inputs in HBM):
 A step is the alloc kernel then the free kernel, launched back to back on
 torch's current stream; the step time is the mean over the timed steps
 between two HIP events.  The same step captured as one HIP graph is timed
-too (ms_per_step_graph).  Each phase kernel is also timed on its own from
-events around it (the roofline uses those durations, which include the
-launch).  The batched launch (rank or
+too (ms_per_step_graph).  Each phase kernel is also timed on its own, with
+HIP events the library records at the kernel's own start and end
+(hipExtLaunchKernelGGL): the roofline divides by those durations, which are
+the ones a kernel trace reports.  Events around each launch are reported
+beside them.  The batched launch (rank or
 walk homes, window width, walk budget, waves per CU) comes from the
 library's autotuner; the best walk-homes launch is timed too.  The plain
 variant is the reference's RandomWalkBasic_BM getPage/freePage, one thread
@@ -83,19 +85,36 @@ def time_region(torch, fn, steps: int, warmup: int, stream) -> float:
     return t0.elapsed_time(t1) / steps
 
 
-def time_phases(torch, pool, n, ids, gl, fl, steps: int, stream) -> dict:
-    """Mean ms of each phase kernel, from events recorded on the launch
-    stream around every eager alloc and free launch."""
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+def time_phases(torch, pbm, pool, n, ids, gl, fl, steps: int, stream) -> tuple[dict, dict]:
+    """Mean ms of each phase kernel over `steps` eager alloc+free steps
+    launched back to back.  First: each kernel's own duration, from HIP
+    events the library records at the kernel's start and end
+    (pbm_time_next_kernel, hipExtLaunchKernelGGL), the figure a kernel
+    trace reports.  Second: events recorded on the stream around every
+    launch, which add the dispatch of the launch and of the markers."""
+    ev = [[pbm.KernelEvent() for _ in range(4)] for _ in range(steps)]
     for e in ev:
+        pool.time_next_kernel(e[0], e[1])
+        pool.get_pages(n, ids, None, 0, gl, stream)
+        pool.time_next_kernel(e[2], e[3])
+        pool.free_pages(n, ids, fl, stream)
+    torch.cuda.synchronize()
+    own = {"alloc": float(np.mean([e[0].elapsed_ms(e[1]) for e in ev])),
+           "free": float(np.mean([e[2].elapsed_ms(e[3]) for e in ev]))}
+    for e in ev:
+        for x in e:
+            x.close()
+    br = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
+    for e in br:
         e[0].record(stream)
         pool.get_pages(n, ids, None, 0, gl, stream)
         e[1].record(stream)
         pool.free_pages(n, ids, fl, stream)
         e[2].record(stream)
     torch.cuda.synchronize()
-    return {"alloc": float(np.mean([e[0].elapsed_time(e[1]) for e in ev])),
-            "free": float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))}
+    bracket = {"alloc": float(np.mean([e[0].elapsed_time(e[1]) for e in br])),
+               "free": float(np.mean([e[1].elapsed_time(e[2]) for e in br]))}
+    return own, bracket
 
 
 def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
@@ -479,9 +498,11 @@ def main() -> int:
         g = pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_ALLOC | pbm.PHASE_FREE)
         ms_graph = time_region(torch, lambda: g.launch(stream), steps, warmup, stream)
         g.close()
-        phases = time_phases(torch, pool, N_REQ, ids, gl, fl, min(steps, 200), stream)
+        phases, bracket = time_phases(torch, pbm, pool, N_REQ, ids, gl, fl, min(steps, 200),
+                                      stream)
         ok = ok and bool(np.array_equal(pool.snapshot(stream), before))
         return {"ms_per_step": ms, "ms_per_step_graph": ms_graph, "phases_ms": phases,
+                "phases_ms_bracketed": bracket,
                 "counters_alloc": c_alloc, "counters_cycle": c_all, "walk_steps": ws,
                 "verified": ok}
 
@@ -591,6 +612,7 @@ def main() -> int:
                    "launch": b["tuned"], "device": dev_name},
         "ms_per_step_graph": b["ms_per_step_graph"],
         "phases_ms": b["phases_ms"],
+        "phases_ms_bracketed": b["phases_ms_bracketed"],
         "walk_steps": b["walk_steps"],
         "verified": b["verified"],
         "plain": {"value": value_p, "ms_per_step": ms_p, "steps": p_steps,

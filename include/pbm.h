@@ -318,6 +318,23 @@ int pbm_kmalloc_free(pbm_kmalloc* k, uint32_t n, const uint64_t* d_ptrs,
  * host_current[max_threads] when not NULL; *max_threads when not NULL. */
 int pbm_kmalloc_current(pbm_kmalloc* k, int32_t* host_current, uint32_t* max_threads);
 
+/* ---- kernel timing ----------------------------------------------------
+ * For measuring the allocation kernels without launch overhead: the next
+ * pbm_get_pages or pbm_free_pages call on h launches its main kernel with
+ * hipExtLaunchKernelGGL, which records start_event and stop_event (hipEvent_t
+ * handles, created with timing) at that kernel's own start and end, as a
+ * kernel trace reports them.  Rank-batched, batched-walk, plain
+ * RandomWalkBasic_BM / CollabRW_BM gets and bitmap frees honour it; every
+ * call clears it, recorded or not.  Not for calls inside a graph capture.
+ * No counterpart in the reference, which times launches with cudaEvents
+ * around them (unitTests/metrics.h:52-70). */
+int pbm_time_next_kernel(pbm_handle* h, void* start_event, void* stop_event);
+/* hipEventCreate / hipEventDestroy / elapsed ms (waits for stop_event),
+ * for callers without a HIP binding of their own. */
+int pbm_event_create(void** event);
+int pbm_event_destroy(void* event);
+int pbm_event_elapsed_ms(void* start_event, void* stop_event, float* ms);
+
 /* ---- observation ------------------------------------------------------ */
 
 /* Free-page clusters: maximal runs of free pages, the page after the last

@@ -583,9 +583,48 @@ pbm_ctx launch_ctx(const pbm_handle* h, const pbm_launch& L) {
 
 }  // namespace
 
+namespace {
+
+// Clears the events pbm_time_next_kernel armed when a batch call returns,
+// whichever path it took.
+struct Disarm {
+    pbm_handle* h;
+    ~Disarm() { h->t_start = h->t_stop = nullptr; }
+};
+
+}  // namespace
+
+extern "C" int pbm_time_next_kernel(pbm_handle* h, void* start_event, void* stop_event) {
+    if (!h) return pbmi::fail(PBM_ERR_INVALID, "null pbm_handle");
+    h->t_start = (hipEvent_t)start_event;
+    h->t_stop = (hipEvent_t)stop_event;
+    return PBM_OK;
+}
+
+extern "C" int pbm_event_create(void** event) {
+    if (!event) return pbmi::fail(PBM_ERR_INVALID, "null output");
+    hipEvent_t e = nullptr;
+    PBM_HIP(hipEventCreate(&e));
+    *event = (void*)e;
+    return PBM_OK;
+}
+
+extern "C" int pbm_event_destroy(void* event) {
+    if (event) PBM_HIP(hipEventDestroy((hipEvent_t)event));
+    return PBM_OK;
+}
+
+extern "C" int pbm_event_elapsed_ms(void* start_event, void* stop_event, float* ms) {
+    if (!start_event || !stop_event || !ms) return pbmi::fail(PBM_ERR_INVALID, "null argument");
+    PBM_HIP(hipEventSynchronize((hipEvent_t)stop_event));
+    PBM_HIP(hipEventElapsedTime(ms, (hipEvent_t)start_event, (hipEvent_t)stop_event));
+    return PBM_OK;
+}
+
 extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
                              uint32_t* d_steps, const pbm_launch* launch, void* stream) {
     PBM_CHECK_HANDLE(h);
+    Disarm disarm{h};
     if (n == 0) return PBM_OK;
     if (!d_ids) return pbmi::fail(PBM_ERR_INVALID, "null d_ids");
     pbm_launch L;
@@ -600,10 +639,10 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
         if (L.flags & PBM_LAUNCH_RANK) {
             if (c.scan_wpt == 4)
-                hipLaunchKernelGGL(k_get_rank<4>, dim3(c.n_tiles), dim3(pbm::kScanThreads), 0, s, c,
+                pbmi::launch_timed(h, k_get_rank<4>, dim3(c.n_tiles), dim3(pbm::kScanThreads), s, c,
                                    n, req_base, d_ids, d_steps, c.walk_budget);
             else
-                hipLaunchKernelGGL(k_get_rank<1>, dim3(c.n_tiles), dim3(pbm::kScanThreads), 0, s, c,
+                pbmi::launch_timed(h, k_get_rank<1>, dim3(c.n_tiles), dim3(pbm::kScanThreads), s, c,
                                    n, req_base, d_ids, d_steps, c.walk_budget);
             PBM_HIP(hipGetLastError());
             return PBM_OK;
@@ -613,15 +652,15 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         const dim3 blk(one ? 64u : 256u);
         switch (L.cluster_words) {
             case 64:
-                hipLaunchKernelGGL(k_get_batched<1>, grid, blk, 0, s, c, n, req_base, d_ids, d_steps,
+                pbmi::launch_timed(h, k_get_batched<1>, grid, blk, s, c, n, req_base, d_ids, d_steps,
                                    c.walk_budget);
                 break;
             case 128:
-                hipLaunchKernelGGL(k_get_batched<2>, grid, blk, 0, s, c, n, req_base, d_ids, d_steps,
+                pbmi::launch_timed(h, k_get_batched<2>, grid, blk, s, c, n, req_base, d_ids, d_steps,
                                    c.walk_budget);
                 break;
             default:
-                hipLaunchKernelGGL(k_get_batched<4>, grid, blk, 0, s, c, n, req_base, d_ids, d_steps,
+                pbmi::launch_timed(h, k_get_batched<4>, grid, blk, s, c, n, req_base, d_ids, d_steps,
                                    c.walk_budget);
                 break;
         }
@@ -634,14 +673,14 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         case PBM_RW_BM: {
             dim3 grid = serial ? dim3(1) : dim3((n + L.block - 1) / L.block);
             dim3 blk = serial ? dim3(1) : dim3(L.block);
-            hipLaunchKernelGGL(k_get_rwbm, grid, blk, 0, s, c, n, req_base, d_ids, d_steps);
+            pbmi::launch_timed(h, k_get_rwbm, grid, blk, s, c, n, req_base, d_ids, d_steps);
             break;
         }
         case PBM_COLLAB_RW_BM: {
             uint32_t waves = (n + 63u) / 64u;
             dim3 grid = serial ? dim3(1) : dim3((waves * 64u + L.block - 1) / L.block);
             dim3 blk = serial ? dim3(64) : dim3(L.block);
-            hipLaunchKernelGGL(k_get_collab, grid, blk, 0, s, c, n, req_base, d_ids, d_steps);
+            pbmi::launch_timed(h, k_get_collab, grid, blk, s, c, n, req_base, d_ids, d_steps);
             break;
         }
         default:
@@ -654,6 +693,7 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
 extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
                               const pbm_launch* launch, void* stream) {
     PBM_CHECK_HANDLE(h);
+    Disarm disarm{h};
     if (n == 0) return PBM_OK;
     if (!d_ids) return pbmi::fail(PBM_ERR_INVALID, "null d_ids");
     pbm_launch L;
@@ -668,11 +708,11 @@ extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
     }
     if (L.mode == PBM_MODE_BATCHED) {
         const uint32_t nb = (batched_blocks(h, L, n) + 3u) / 4u;
-        hipLaunchKernelGGL(k_free_batched, dim3(nb), dim3(1024), 0, s, c, n, d_ids);
+        pbmi::launch_timed(h, k_free_batched, dim3(nb), dim3(1024), s, c, n, d_ids);
     } else if (L.mode == PBM_MODE_SERIAL) {
-        hipLaunchKernelGGL(k_free_plain, dim3(1), dim3(1), 0, s, c, n, d_ids);
+        pbmi::launch_timed(h, k_free_plain, dim3(1), dim3(1), s, c, n, d_ids);
     } else {
-        hipLaunchKernelGGL(k_free_plain, dim3((n + L.block - 1) / L.block), dim3(L.block), 0, s, c,
+        pbmi::launch_timed(h, k_free_plain, dim3((n + L.block - 1) / L.block), dim3(L.block), s, c,
                            n, d_ids);
     }
     PBM_HIP(hipGetLastError());

@@ -3,6 +3,7 @@
 #define PBM_INTERNAL_HPP
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <string>
 #include "pbm.h"
 #include "pbm_device.hpp"
@@ -29,6 +30,10 @@ struct pbm_handle {
     unsigned* d_fresh = nullptr;              // Ouroboros: free pages at the last reset
     unsigned long long* d_carve = nullptr;    // Ouroboros: [0] carved, [1] fresh pages
     long long* d_qavail = nullptr;            // Ouroboros: unpromised queued pages
+    // events the next batch call's main kernel records at its own start and
+    // end (pbm_time_next_kernel); cleared by that call
+    hipEvent_t t_start = nullptr;
+    hipEvent_t t_stop = nullptr;
 };
 
 // An instantiated HIP graph and the device it runs on.
@@ -61,6 +66,19 @@ inline uint32_t ouroboros_chunk_pages(uint32_t page_size) {
 // Strategies whose free pages live in a FIFO ring (LinkedList, Ouroboros):
 // no batched, multi-page, cache or burst calls on them.
 inline bool is_queue_strategy(int s) { return s == PBM_LINKED_LIST || s == PBM_OUROBOROS; }
+
+// Launches `kernel`; when pbm_time_next_kernel armed the handle, with the
+// armed events recorded at the kernel's own start and end.
+template <typename F, typename... Args>
+inline void launch_timed(pbm_handle* h, F kernel, dim3 grid, dim3 block, hipStream_t s,
+                         Args... args) {
+    if (h->t_start || h->t_stop) {
+        hipExtLaunchKernelGGL(kernel, grid, block, 0, s, h->t_start, h->t_stop, 0, args...);
+        h->t_start = h->t_stop = nullptr;
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, 0, s, args...);
+    }
+}
 
 // Makes the handle's device current for the calling thread.
 int use_device(pbm_handle* h);

@@ -156,6 +156,10 @@ _SIGS = {
                                 C.c_void_p]),
     "pbm_page_address": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "pbm_page_id": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
+    "pbm_time_next_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pbm_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "pbm_event_destroy": (C.c_int, [C.c_void_p]),
+    "pbm_event_elapsed_ms": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "pbm_read_pool": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p]),
     "pbm_kmalloc_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "pbm_kmalloc_destroy": (C.c_int, [C.c_void_p]),
@@ -403,12 +407,42 @@ class Pool:
         check(lib().pbm_page_id(self.h, addr, C.byref(v)), "pbm_page_id")
         return int(v.value)
 
+    def time_next_kernel(self, start: "KernelEvent", stop: "KernelEvent") -> None:
+        """The next get_pages / free_pages call records start and stop at its
+        main kernel's own start and end (pbm_time_next_kernel)."""
+        check(lib().pbm_time_next_kernel(self.h, start.e, stop.e), "pbm_time_next_kernel")
+
     def read_pool(self, offset: int, nbytes: int, stream=None) -> np.ndarray:
         """Bytes [offset, offset+nbytes) of page memory."""
         out = np.empty(nbytes, dtype=np.uint8)
         check(lib().pbm_read_pool(self.h, offset, nbytes, out.ctypes.data_as(C.c_void_p),
                                   _stream(stream)), "pbm_read_pool")
         return out
+
+
+class KernelEvent:
+    """A hipEvent_t for pbm_time_next_kernel."""
+
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib().pbm_event_create(C.byref(e)), "pbm_event_create")
+        self.e = e
+
+    def elapsed_ms(self, stop: "KernelEvent") -> float:
+        v = C.c_float()
+        check(lib().pbm_event_elapsed_ms(self.e, stop.e, C.byref(v)), "pbm_event_elapsed_ms")
+        return float(v.value)
+
+    def close(self) -> None:
+        if self.e:
+            check(lib().pbm_event_destroy(self.e), "pbm_event_destroy")
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 KMALLOC_SIMPLE, KMALLOC_COLLAB = 0, 1
