@@ -3,7 +3,8 @@
 // getPageID, getFreePagePercentage, mallocRW, freeRW), compiled once per
 // policy with -DPBM_COMPAT_STRATEGY=...  It follows the flow of the
 // reference's drivers (unitTests/unitTest1.cu:22-44, unitTest3.cu:31-43):
-// pre-fill, one getPage per thread, check, freePage per thread, check.
+// pre-fill, one getPage per thread, check, freePage per thread, check;
+// then partial warps (unitTest5.cu, unitTest7.cu), then mallocRW.
 // Prints one line and exits 0 when every check holds.
 #include <algorithm>
 #include <cstdio>
@@ -13,6 +14,17 @@
 __global__ void get1page_kernel(int n, int* ids, int* steps) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid < n) ids[tid] = getPage(&steps[tid]);
+}
+
+// Partial-warp callers (unitTests/unitTest5.cu:24-35, unitTest7.cu:17-29):
+// only lanes with threadIdx.x % 32 < nalloc ask for a page; the others
+// record -2.
+__global__ void partial_kernel(int n, int nalloc, int* ids) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < n) {
+        if ((int)(threadIdx.x % 32) < nalloc) ids[tid] = getPage();
+        else ids[tid] = -2;
+    }
 }
 
 __global__ void freePage_kernel(int n, const int* ids) {
@@ -90,6 +102,29 @@ int main() {
     freePage_kernel<<<N / 256, 256>>>(N, d_ids);
     hipDeviceSynchronize();
     CHECK(getFreePagePercentage() == before, "every page came back");
+
+    // partial warps: 5,000 threads as the reference's drivers use, lane
+    // groups of 1, 16, 17, 31 and 32 of every 32
+    for (int nalloc : {1, 16, 17, 31, 32}) {
+        const int P = 5000;
+        partial_kernel<<<(P + 255) / 256, 256>>>(P, nalloc, d_ids);
+        std::vector<int> pid(P);
+        hipMemcpy(pid.data(), d_ids, P * 4, hipMemcpyDeviceToHost);
+        int asked = 0;
+        std::vector<int> pg;
+        for (int t = 0; t < P; ++t) {
+            if (pid[t] == -2) continue;
+            ++asked;
+            if (pid[t] >= 0) pg.push_back(pid[t]);
+            CHECK((t % 256) % 32 < nalloc, "only the asking lanes wrote");
+        }
+        std::sort(pg.begin(), pg.end());
+        CHECK(std::adjacent_find(pg.begin(), pg.end()) == pg.end(), "partial warps: a page twice");
+        CHECK((int)pg.size() == asked, "partial warps: every asking lane got a page");
+        freePage_kernel<<<(P + 255) / 256, 256>>>(P, d_ids);
+        hipDeviceSynchronize();
+        CHECK(getFreePagePercentage() == before, "partial warps: every page came back");
+    }
 
     int malloc_ok = -1;
     if (PBM_COMPAT_STRATEGY == PBM_RW_MALLOC) {
