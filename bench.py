@@ -43,6 +43,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "parallel-db-buffer-management_amd"))
 
+from metrics import walk_stats  # noqa: E402
+
 METRIC = "page allocs+frees/sec at 90% fill, persistent-batched vs plain; avg walk steps"
 T_PAGES, PAGE_SIZE, FILL, SEED, N_REQ = 1 << 20, 4096, 0.9, 42, 262144
 HBM_PEAK_GBS = 8000.0
@@ -54,19 +56,6 @@ _T0 = time.perf_counter()
 def progress(msg: str) -> None:
     """One progress line on stderr (the JSON result goes to stdout)."""
     print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
-
-
-def walk_stats(steps: np.ndarray, ids: np.ndarray) -> dict:
-    g = ids >= 0
-    out = {"tas": float(steps[g].mean()) if g.any() else 0.0}
-    for w in (64, 32):
-        m = len(ids) // w * w
-        s = steps[:m].reshape(-1, w)
-        gg = g[:m].reshape(-1, w)
-        rows = gg.any(axis=1)
-        out[f"was{w}"] = float(np.where(gg, s, 0).max(axis=1)[rows].mean()) if rows.any() else 0.0
-    out["max"] = int(steps[g].max()) if g.any() else 0
-    return out
 
 
 def time_region(torch, fn, steps: int, warmup: int, stream) -> float:

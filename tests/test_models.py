@@ -67,3 +67,33 @@ def test_sweep_rows():
     assert [r["fill"] for r in rows] == [0.5, 0.9, 0.99]
     assert rows[1]["A"] == 104858 and rows[1]["N"] == 104857
     assert all(r["RWBM_TAS"] >= 1 and r["RW_TAS"] >= 1 for r in rows)
+
+
+# ------------------------------------------------------------------ metrics (unitTests/metrics.h)
+
+def test_aggregate_metrics_known_answer():
+    from metrics import aggregate_metrics, sample_average
+    # 67 threads: two full warps and a partial third (metrics.h:21-30)
+    steps = [5] * 32 + [1] * 32 + [9, 0, 0]
+    avg, avg_max = aggregate_metrics(steps)
+    assert avg == pytest.approx((160 + 32 + 9) / 67, rel=1e-7)
+    assert avg_max == 5.0                                   # (5 + 1 + 9) / 3
+    assert sample_average([(1.0, 2.0, 3.0), (3.0, 4.0, 5.0)]) == (2.0, 3.0, 4.0)
+
+
+def test_walk_stats_equals_the_reference_metrics_when_all_granted():
+    from metrics import aggregate_metrics, walk_stats
+    rng = np.random.default_rng(1)
+    steps = rng.integers(1, 40, size=1000)
+    ws = walk_stats(steps, np.arange(1000))
+    avg, avg_max = aggregate_metrics(steps)
+    assert ws["tas"] == pytest.approx(avg, rel=1e-6)
+    assert ws["was32"] == pytest.approx(avg_max, rel=1e-6)
+    assert ws["max"] == steps.max()
+    # a failed request's steps do not count, nor does a group without grants
+    ids = np.arange(128)
+    ids[64:] = -1
+    st = np.ones(128, dtype=np.int64)
+    st[70] = 99
+    ws = walk_stats(st, ids)
+    assert ws["tas"] == 1.0 and ws["was64"] == 1.0 and ws["was32"] == 1.0 and ws["max"] == 1
