@@ -271,6 +271,21 @@ def policy_rows(torch, pbm, stream, n_pages: int, n_req: int, fill: float, steps
         torch.cuda.synchronize()
         i_np = ids.cpu().numpy()
         ws = walk_stats(stp.cpu().numpy().view(np.uint32), i_np)
+        yields = None
+        if strategy == pbm.COLLAB_RW_BM:
+            # collectWarpData on a second cycle: free bits each wave's
+            # window showed per step, against the pool's share 4096 A / T
+            pool.free_pages(n_req, ids, fl, stream)
+            waves, K = (n_req + 63) // 64, 8
+            y = torch.zeros((waves * K,), dtype=torch.int32, device="cuda")
+            ns = torch.zeros((waves,), dtype=torch.int32, device="cuda")
+            pool.collect_warp_data(n_req, ids, y, K, ns, 0, lnch, stream)
+            torch.cuda.synchronize()
+            y_np = y.cpu().numpy().reshape(waves, K)
+            ns_np = ns.cpu().numpy()
+            ran = ns_np > 0
+            yields = {"first_step_mean": float(y_np[ran, 0].mean()) if ran.any() else 0.0,
+                      "window_share": 4096.0 * A / n_pages, "waves": int(ran.sum())}
         fails = int((i_np < 0).sum())
         pool.free_pages(n_req, ids, fl, stream)
         quiet = pbm.Launch(lnch.mode, lnch.block, lnch.waves_per_cu, lnch.walk_budget,
@@ -287,6 +302,8 @@ def policy_rows(torch, pbm, stream, n_pages: int, n_req: int, fill: float, steps
               and fails == max(0, n_req - A) and pool.counters(stream)["errors"] == 0)
         row[name] = {"walk_steps": ws, "failures": fails, "expected_failures": max(0, n_req - A),
                      "value": 2 * min(n_req, A) / (ms * 1e-3), "ms_per_step": ms, "verified": ok}
+        if yields:
+            row[name]["warp_yield"] = yields
         pool.close()
     return row
 

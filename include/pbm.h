@@ -318,6 +318,18 @@ int pbm_kmalloc_free(pbm_kmalloc* k, uint32_t n, const uint64_t* d_ptrs,
  * host_current[max_threads] when not NULL; *max_threads when not NULL. */
 int pbm_kmalloc_current(pbm_kmalloc* k, int32_t* host_current, uint32_t* max_threads);
 
+/* collectWarpData (CollabRW_BM.cuh:131-138) over a batch, as the
+ * collectPageData driver gathers it (unitTests/collectPageData.cu:19-58):
+ * CollabRW_BM getPage for n requests, one wave per 64 consecutive requests
+ * (serial mode: one wave, in index order).  Wave w = request / 64 records
+ * the free bits its probing lanes found in each step s < max_steps at
+ * d_yields[w * max_steps + s] (later entries are left as they were) and
+ * its step count at d_n_steps[w].  Needs a PBM_COLLAB_RW_BM pool; plain or
+ * serial launches.  Granted pages stay allocated, as with pbm_get_pages. */
+int pbm_collect_warp_data(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
+                          uint32_t* d_yields, uint32_t max_steps, uint32_t* d_n_steps,
+                          const pbm_launch* launch, void* stream);
+
 /* ---- kernel timing ----------------------------------------------------
  * For measuring the allocation kernels without launch overhead: the next
  * pbm_get_pages or pbm_free_pages call on h launches its main kernel with

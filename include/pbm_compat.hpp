@@ -140,9 +140,22 @@ static __device__ inline void* getPagePtr() { return getPage(); }
  * getPageClusteredRandomWalk / getPageCollabRW, parallelPage.cu:200-218,
  * 642-680): a page id, or -1 when the pool has none (the reference spins).
  * The collaborative policies and RWMalloc are collective over the calling
- * wave's active lanes; collectWarpData is accepted and ignored. */
+ * wave's active lanes.  collectWarpData (CollabRW_BM only) prints the free
+ * bits the wave found in each step, one line per step. */
 static __device__ inline int getPage(int* stepCount = nullptr, bool collectWarpData = false) {
-    (void)collectWarpData;
+    if (PBM_COMPAT_STRATEGY == PBM_COLLAB_RW_BM && collectWarpData) {
+        // the wave's lowest lane prints the free bits found in each step,
+        // as CollabRW_BM.cuh:131-138 prints them (the first 64 steps)
+        uint32_t yields[64];
+        uint32_t it = 0, steps = 0;
+        const unsigned first = (unsigned)__ffsll((long long)pbm::ballot(true)) - 1u;
+        const int32_t id = pbm::get_page_collab(pbm_compat_ctx, pbm_compat_seed(), &steps,
+                                                nullptr, yields, 64u, &it);
+        if (pbm::lane_id() == first)
+            for (uint32_t k = 0; k < it && k < 64u; ++k) printf("%u\n", yields[k]);
+        if (stepCount) *stepCount = (int)steps;
+        return id;
+    }
     return pbm_compat_get(stepCount);
 }
 

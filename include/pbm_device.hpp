@@ -526,9 +526,15 @@ __device__ inline WindowResult window_step(const pbm_ctx& c, bool probe, uint32_
  * (uint32)lcg^s(seed) % n_words, lane l probing word start+l (mod n_words);
  * lanes l >= n_words do not probe, so a small map is never read twice in
  * one window.  seed is taken from the lowest active lane.  Step = one
- * window, the same for every lane served in it (CollabRW_BM.cuh:124). */
+ * window, the same for every lane served in it (CollabRW_BM.cuh:124).
+ *
+ * collectWarpData (:131-138), which prints the free bits the warp found in
+ * each step: with `yields`, the lowest active lane stores the free bits the
+ * wave's probing lanes read in step s at yields[s] for s < max_yields, and
+ * *n_iters (when given) receives the wave's step count. */
 __device__ inline int32_t get_page_collab(const pbm_ctx& c, int32_t seed, uint32_t* steps_out,
-                                          Stats* stats = nullptr) {
+                                          Stats* stats = nullptr, uint32_t* yields = nullptr,
+                                          uint32_t max_yields = 0, uint32_t* n_iters = nullptr) {
     Stats local;
     Stats& st = stats ? *stats : local;
     const uint32_t L = c.n_words;
@@ -539,8 +545,9 @@ __device__ inline int32_t get_page_collab(const pbm_ctx& c, int32_t seed, uint32
     const unsigned lane = lane_id();
     uint32_t budget = c.walk_budget;
     uint32_t start = 0;
-    uint32_t jumps = 0, sweeps = 0;
+    uint32_t jumps = 0, sweeps = 0, it = 0;
     const uint32_t max_sweeps = 4u * ((L + 63u) / 64u) + 4u;
+    const unsigned first = (unsigned)__ffsll((long long)ballot(true)) - 1u;
     while (ballot(need)) {
         if (jumps < budget) {
             seed = lcg(seed);
@@ -553,8 +560,11 @@ __device__ inline int32_t get_page_collab(const pbm_ctx& c, int32_t seed, uint32
         }
         if (need) steps++;
         uint32_t word = (uint32_t)(((u64)start + lane) % L);
-        window_step(c, lane < L, word, need, id, c.order, st);
+        const WindowResult res = window_step(c, lane < L, word, need, id, c.order, st);
+        if (yields && it < max_yields && lane == first) yields[it] = res.seen_free;
+        it++;
     }
+    if (n_iters && lane == first) *n_iters = it;
     if (need) { atom_add_ll(c.free_count, 1); st.errors++; }
     if (id >= 0) st.allocs++; else st.failures++;
     st.steps += steps;

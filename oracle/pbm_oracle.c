@@ -470,9 +470,34 @@ static int any(const int* v) {
     return 0;
 }
 
+/* CollabRW_BM getPage on one wave per chunk of 64 requests.  With yields,
+ * also collectWarpData (CollabRW_BM.cuh:131-138): yields[ch * max_steps + s]
+ * = free bits the wave's probing lanes read in its step s (s < max_steps),
+ * and n_steps[ch] = the wave's steps. */
+static uint32_t collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                              uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                              int32_t* ids, uint32_t* steps, uint32_t* yields,
+                              uint32_t max_steps, uint32_t* n_steps);
+
 uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                            uint64_t req_base, uint64_t seed, uint32_t budget, int order,
                            int32_t* ids, uint32_t* steps) {
+    return collab_serial(bm, n_pages, free_count, n, req_base, seed, budget, order, ids, steps,
+                         NULL, 0, NULL);
+}
+
+uint32_t orc_collab_yields(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                           uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                           int32_t* ids, uint32_t* yields, uint32_t max_steps,
+                           uint32_t* n_steps) {
+    return collab_serial(bm, n_pages, free_count, n, req_base, seed, budget, order, ids, NULL,
+                         yields, max_steps, n_steps);
+}
+
+static uint32_t collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                              uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                              int32_t* ids, uint32_t* steps, uint32_t* yields,
+                              uint32_t max_steps, uint32_t* n_steps) {
     const uint32_t L = orc_n_words(n_pages);
     const uint32_t max_sweeps = 4u * ((L + 63u) / 64u) + 4u;
     uint32_t granted_total = 0;
@@ -493,7 +518,7 @@ uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, 
         *free_count -= g;
         for (int l = 0; l < 64; ++l) need[l] = active[l] && l < g;
         int32_t sd = orc_request_seed(seed, req_base + (u64)ch * 64u);
-        uint32_t jumps = 0, sweeps = 0, start = 0;
+        uint32_t jumps = 0, sweeps = 0, start = 0, it = 0;
         while (any(need)) {
             if (jumps < budget) {
                 sd = orc_lcg(sd);
@@ -510,7 +535,10 @@ uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, 
             }
             uint32_t seen;
             collab_claim(bm, probe, word, need, cid, order, &seen);
+            if (yields && it < max_steps) yields[(u64)ch * max_steps + it] = seen;
+            it++;
         }
+        if (n_steps) n_steps[ch] = it;
         for (int l = 0; l < 64; ++l) {
             if (need[l]) (*free_count)++;
             if (!active[l]) continue;

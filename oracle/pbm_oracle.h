@@ -42,6 +42,12 @@ uint32_t orc_rwbm_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, ui
 uint32_t orc_collab_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                            uint64_t req_base, uint64_t seed, uint32_t budget, int order,
                            int32_t* ids, uint32_t* steps);
+/* The same with collectWarpData (CollabRW_BM.cuh:131-138): per wave, the
+ * free bits its probing lanes read in each step. */
+uint32_t orc_collab_yields(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                           uint64_t req_base, uint64_t seed, uint32_t budget, int order,
+                           int32_t* ids, uint32_t* yields, uint32_t max_steps,
+                           uint32_t* n_steps);
 
 /* Batched persistent allocation with one wave: phase reservation of
  * min(n, free) for the first requests by index, windows of 64*J words kept

@@ -78,6 +78,9 @@ _SIGS = {
     "orc_queue_get": (C.c_uint32, [_PU64, C.c_uint32, _PI64, _PU32, _P32, _PI64, C.c_uint32,
                                    C.c_uint32, _P32, _PU32]),
     "orc_queue_put": (C.c_uint32, [_PU64, C.c_uint32, _PI64, _P32, _PI64, C.c_uint32, _P32]),
+    "orc_collab_yields": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                       C.c_uint64, C.c_uint32, C.c_int, _P32, _PU32, C.c_uint32,
+                                       _PU32]),
     "orc_rw_tas": (C.c_double, [C.c_double, C.c_double, C.c_double]),
     "orc_count_clusters": (C.c_uint64, [_PU64, C.c_uint32]),
     "orc_getx_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32, C.c_uint64,
@@ -223,6 +226,19 @@ class State:
                                  _p(mem, _PU8), _p(written, _PU8), n, _p(sizes, _PU32), req_base,
                                  seed, budget, _p(cur, _P32), _p(off, _PU64), _p(steps, _PU32))
         return off, steps, int(cur[0])
+
+    def collab_yields(self, n: int, req_base: int, seed: int, order: int, max_steps: int,
+                      budget: int = 1 << 20):
+        """CollabRW_BM with collectWarpData: (ids, yields[waves, max_steps],
+        steps per wave)."""
+        waves = (n + 63) // 64
+        ids = np.full(n, -1, dtype=np.int32)
+        yields = np.zeros(waves * max_steps, dtype=np.uint32)
+        n_steps = np.zeros(waves, dtype=np.uint32)
+        lib().orc_collab_yields(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                req_base, seed, budget, order, _p(ids, _P32), _p(yields, _PU32),
+                                max_steps, _p(n_steps, _PU32))
+        return ids, yields.reshape(waves, max_steps), n_steps
 
     def collab(self, n: int, req_base: int, seed: int, order: int, budget: int = 1 << 20):
         ids, steps = self._out(n)

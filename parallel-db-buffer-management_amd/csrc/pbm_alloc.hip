@@ -59,6 +59,32 @@ __global__ __launch_bounds__(1024) void k_get_collab(pbm_ctx c, uint32_t n, uint
     pbm::flush_stats(c, st);
 }
 
+// CollabRW_BM getPage with collectWarpData: as k_get_collab, and wave w
+// (requests 64w..64w+63) records its per-step yields at
+// yields[w * max_steps ..] and its step count at n_steps[w].
+__global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t n,
+                                                            uint64_t req_base,
+                                                            int32_t* __restrict__ ids,
+                                                            uint32_t* __restrict__ yields,
+                                                            uint32_t max_steps,
+                                                            uint32_t* __restrict__ n_steps) {
+    pbm::Stats st;
+    const unsigned lane = pbm::lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x + 63u) >> 6;
+    const uint32_t n_chunks = (n + 63u) / 64u;
+    for (uint32_t ch = wave; ch < n_chunks; ch += n_waves) {
+        const uint32_t r = ch * 64u + lane;
+        if (r < n) {
+            int32_t id = pbm::get_page_collab(
+                c, pbm::request_seed(c.seed, req_base + (uint64_t)ch * 64u), nullptr, &st,
+                yields + (uint64_t)ch * max_steps, max_steps, &n_steps[ch]);
+            ids[r] = id;
+        }
+    }
+    pbm::flush_stats(c, st);
+}
+
 // ---- batched allocation by rank, one pass over the map
 //
 // The map is cut into tiles of kScanThreads * scan_wpt words, one
@@ -686,6 +712,33 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         default:
             return pbmi::launch_get_strategy(h, c, L, n, req_base, d_ids, d_steps, s);
     }
+    PBM_HIP(hipGetLastError());
+    return PBM_OK;
+}
+
+extern "C" int pbm_collect_warp_data(pbm_handle* h, uint32_t n, uint64_t req_base,
+                                     int32_t* d_ids, uint32_t* d_yields, uint32_t max_steps,
+                                     uint32_t* d_n_steps, const pbm_launch* launch,
+                                     void* stream) {
+    PBM_CHECK_HANDLE(h);
+    Disarm disarm{h};
+    if (n == 0) return PBM_OK;
+    if (!d_ids || !d_n_steps || (max_steps && !d_yields))
+        return pbmi::fail(PBM_ERR_INVALID, "null d_ids, d_yields or d_n_steps");
+    if (h->cfg.strategy != PBM_COLLAB_RW_BM)
+        return pbmi::fail(PBM_ERR_UNSUPPORTED, "collectWarpData belongs to CollabRW_BM");
+    pbm_launch L;
+    int rc = check_launch(launch, &L);
+    if (rc) return rc;
+    if (L.mode == PBM_MODE_BATCHED)
+        return pbmi::fail(PBM_ERR_UNSUPPORTED, "collectWarpData runs plain or serial");
+    pbm_ctx c = launch_ctx(h, L);
+    const bool serial = L.mode == PBM_MODE_SERIAL;
+    const uint32_t waves = (n + 63u) / 64u;
+    const dim3 grid = serial ? dim3(1) : dim3((waves * 64u + L.block - 1) / L.block);
+    const dim3 blk = serial ? dim3(64) : dim3(L.block);
+    hipLaunchKernelGGL(k_collect_warp_data, grid, blk, 0, pbmi::as_stream(stream), c, n, req_base,
+                       d_ids, d_yields, max_steps, d_n_steps);
     PBM_HIP(hipGetLastError());
     return PBM_OK;
 }

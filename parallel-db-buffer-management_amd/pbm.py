@@ -156,6 +156,9 @@ _SIGS = {
                                 C.c_void_p]),
     "pbm_page_address": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
     "pbm_page_id": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_int32)]),
+    "pbm_collect_warp_data": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p,
+                                        C.c_void_p, C.c_uint32, C.c_void_p, C.POINTER(Launch),
+                                        C.c_void_p]),
     "pbm_time_next_kernel": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pbm_event_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "pbm_event_destroy": (C.c_int, [C.c_void_p]),
@@ -406,6 +409,13 @@ class Pool:
         v = C.c_int32()
         check(lib().pbm_page_id(self.h, addr, C.byref(v)), "pbm_page_id")
         return int(v.value)
+
+    def collect_warp_data(self, n: int, ids, yields, max_steps: int, n_steps, req_base: int = 0,
+                          lnch: Launch | None = None, stream=None) -> None:
+        """CollabRW_BM getPage with collectWarpData (pbm_collect_warp_data)."""
+        check(lib().pbm_collect_warp_data(self.h, n, req_base, _ptr(ids), _ptr(yields), max_steps,
+                                          _ptr(n_steps), C.byref(lnch or launch()),
+                                          _stream(stream)), "pbm_collect_warp_data")
 
     def time_next_kernel(self, start: "KernelEvent", stop: "KernelEvent") -> None:
         """The next get_pages / free_pages call records start and stop at its

@@ -27,6 +27,13 @@ __global__ void partial_kernel(int n, int nalloc, int* ids) {
     }
 }
 
+// collectPageData (unitTests/collectPageData.cu:19-37): getPage with
+// collectWarpData, which prints the pages a warp found in each step.
+__global__ void collect_kernel(int n, int* ids) {
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid < n) ids[tid] = getPage(nullptr, true);
+}
+
 __global__ void freePage_kernel(int n, const int* ids) {
     int tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid < n && ids[tid] >= 0) freePage(ids[tid]);
@@ -124,6 +131,18 @@ int main() {
         freePage_kernel<<<(P + 255) / 256, 256>>>(P, d_ids);
         hipDeviceSynchronize();
         CHECK(getFreePagePercentage() == before, "partial warps: every page came back");
+    }
+
+    if (PBM_COMPAT_STRATEGY == PBM_COLLAB_RW_BM) {
+        std::printf("collect begin\n");
+        std::fflush(stdout);
+        collect_kernel<<<1, 32>>>(32, d_ids);
+        hipDeviceSynchronize();
+        std::fflush(stdout);
+        std::printf("collect end\n");
+        freePage_kernel<<<1, 32>>>(32, d_ids);
+        hipDeviceSynchronize();
+        CHECK(getFreePagePercentage() == before, "collectWarpData: every page came back");
     }
 
     int malloc_ok = -1;

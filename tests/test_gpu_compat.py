@@ -20,6 +20,11 @@ def test_reference_api_driver(gpu, name):
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert proc.returncode == 0, proc.stdout + proc.stderr
     assert "compat ok" in proc.stdout
+    if name == "collab":
+        # collectWarpData prints one line per step: the free bits found by
+        # the warp's 32 lanes (at most 32 words of 64 bits)
+        lines = proc.stdout.split("collect begin\n", 1)[1].split("collect end", 1)[0].split()
+        assert lines and all(0 <= int(v) <= 32 * 64 for v in lines)
 
 
 def test_reference_kmalloc_driver(gpu):

@@ -374,3 +374,35 @@ def test_concurrent_malloc_regions_are_disjoint(gpu):
         pool.free_ptrs(n, ptrs)
         gpu.cuda.synchronize()
         assert pool.count_used() == 0 and pool.free_count() == T
+
+
+@pytest.mark.parametrize("order", [pbm.ORDER_PACKED, pbm.ORDER_REFERENCE])
+def test_serial_collect_warp_data_matches_oracle(gpu, order):
+    # collectWarpData (CollabRW_BM.cuh:131-138): per wave, the free bits its
+    # lanes found in each step
+    T, n, K = 1 << 16, 64 * 120 + 17, 24
+    log_case(f"serial collectWarpData order={order}")
+    with pbm.Pool(T, strategy=pbm.COLLAB_RW_BM, seed=47, order=order) as pool:
+        pool.prefill_bernoulli(0.95, 9)
+        waves = -(-n // 64)
+        ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
+        yields = gpu.zeros((waves * K,), dtype=gpu.int32, device="cuda")
+        n_steps = gpu.zeros((waves,), dtype=gpu.int32, device="cuda")
+        pool.collect_warp_data(n, ids, yields, K, n_steps, 3, pbm.launch(mode=pbm.MODE_SERIAL))
+        gpu.cuda.synchronize()
+        st = orc.State(orc.prefill(T, 0.95, 9), T)
+        eids, ey, ens = st.collab_yields(n, 3, 47, order, K)
+        assert np.array_equal(ids.cpu().numpy(), eids)
+        assert np.array_equal(n_steps.cpu().numpy().view(np.uint32), ens)
+        got = yields.cpu().numpy().view(np.uint32).reshape(waves, K)
+        for w in range(waves):
+            k = min(int(ens[w]), K)
+            assert np.array_equal(got[w, :k], ey[w, :k]), w
+        assert np.array_equal(pool.snapshot(), st.bm)
+
+
+def test_collect_warp_data_needs_collab(gpu):
+    with pbm.Pool(4096, strategy=pbm.RW_BM) as pool:
+        ids = gpu.zeros((64,), dtype=gpu.int32, device="cuda")
+        with pytest.raises(pbm.PbmError):
+            pool.collect_warp_data(64, ids, ids, 1, ids)

@@ -844,6 +844,36 @@ def test_rwbasic_mean_steps_match_exact_expectation_and_rw_tas():
     assert abs(orc.rw_tas(T, A, N) - mean) / mean < 2e-3
 
 
+# ------------------------------------------------------------------ collectWarpData
+
+def test_collab_first_step_yield_is_the_window_share_of_free_pages():
+    # a wave's first window covers 64 words = 4,096 pages at a random start,
+    # so its free bits average 4096 A / T (the survey's 409.6 at 90% fill)
+    T = 1 << 20
+    bm = orc.prefill(T, 0.9, 42)
+    A = orc.count_free(bm, T)
+    first = []
+    for r in range(300):
+        st = orc.State(bm, T)
+        _, y, ns = st.collab_yields(64, r * 64, 5, 0, 4)
+        assert ns[0] >= 1 and y[0, 0] > 0
+        first.append(int(y[0, 0]))
+    first = np.array(first, dtype=np.float64)
+    expect = 4096 * A / T
+    assert abs(first.mean() - expect) < 5 * first.std() / np.sqrt(first.size)
+
+
+def test_collab_yields_count_the_steps():
+    T = 1 << 14
+    st = orc.State(orc.prefill(T, 0.99, 3), T)
+    ids, y, ns = st.collab_yields(64 * 10, 0, 7, 1, 200)
+    ref = orc.State(orc.prefill(T, 0.99, 3), T)
+    rids, rsteps = ref.collab(64 * 10, 0, 7, 1)
+    assert np.array_equal(ids, rids)          # recording changes nothing
+    assert (ns == rsteps.reshape(10, 64).max(axis=1)).all()
+    assert all((y[w, :ns[w]] >= 0).all() for w in range(10))
+
+
 # ------------------------------------------------------------------ Ouroboros queue
 
 def test_ouroboros_chunk_pages_follow_the_size_classes():
