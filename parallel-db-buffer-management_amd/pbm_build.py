@@ -66,18 +66,21 @@ def compat_binary(name: str) -> Path:
 
 COMPAT_KMALLOC_SRC = ROOT / "tests" / "compat" / "compat_kmalloc.hip"
 COMPAT_OUROBOROS_SRC = ROOT / "tests" / "compat" / "compat_ouroboros.hip"
+COMPAT_LEGACY_SRC = ROOT / "tests" / "compat" / "compat_legacy.hip"
 
 
 def build_compat_checks(jobs: int = 4) -> list[Path]:
     """The reference-API drivers against pbm_compat.hpp and libpbm.so:
     tests/compat/compat_check.hip once per policy, the kernel-malloc driver
-    tests/compat/compat_kmalloc.hip and the Ouroboros-wrapper driver
-    tests/compat/compat_ouroboros.hip."""
+    tests/compat/compat_kmalloc.hip, the Ouroboros-wrapper driver
+    tests/compat/compat_ouroboros.hip and the legacy-API driver
+    tests/compat/compat_legacy.hip."""
     OBJDIR.mkdir(exist_ok=True)
     jobs_list = [([f"-DPBM_COMPAT_STRATEGY={m}", str(COMPAT_SRC)], compat_binary(n))
                  for n, m in COMPAT_STRATEGIES.items()]
     jobs_list.append(([str(COMPAT_KMALLOC_SRC)], compat_binary("kmalloc")))
     jobs_list.append(([str(COMPAT_OUROBOROS_SRC)], compat_binary("ouroboros")))
+    jobs_list.append(([str(COMPAT_LEGACY_SRC)], compat_binary("legacy")))
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         futs = [ex.submit(_run, [HIPCC, *HIP_FLAGS, *src, "-o", str(o), "-L", str(PKG), "-lpbm",
                                  "-Wl,-rpath,$ORIGIN/.."])

@@ -43,3 +43,16 @@ def test_reference_ouroboros_driver(gpu):
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert proc.returncode == 0, proc.stdout + proc.stderr
     assert "compat ouroboros ok" in proc.stdout
+
+
+def test_reference_legacy_driver(gpu):
+    exe = pbm_build.compat_binary("legacy")
+    assert exe.exists(), f"{exe} is missing: run __graft_entry__.build()"
+    log_case("compat legacy driver")
+    proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert "compat legacy ok" in proc.stdout
+    # printNumPagesLeft of each family after its gets: 262,144 - 50,000
+    assert proc.stdout.count("Number of free pages: 212144 ") == 9
+    mon = proc.stdout.split("monitor begin\n", 1)[1].split("monitor end", 1)[0]
+    assert mon.count("[RW info] Number of free pages: 262144, percentage free: 100.0 %") == 3
