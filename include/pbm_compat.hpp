@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include "pbm.h"
 #include "pbm_device.hpp"
+#include "pbm_kmalloc_classes.hpp"
 
 #ifndef PBM_COMPAT_STRATEGY
 #ifdef PBM_COMPAT_OUROBOROS
@@ -186,88 +187,20 @@ static __device__ inline void* mallocRW(size_t size) {
 static __device__ inline void freeRW(void* ptr) { pbm::free_rw(pbm_compat_ctx, ptr); }
 
 /* ---- kernel malloc (malloc.cuh:14-37) ----------------------------------
- * Constructed on the host over this translation unit's pool
- * (initMemoryManagement runs with its defaults if it has not run) and
- * passed to kernels by value, as the reference's tests do
- * (unitTests/unitTestMalloc1.cu:14-41).  Like the reference's, the objects
- * have no destructor that frees anything: copies made for kernel launches
- * must not release the state.  malloc is collective over the calling
- * wave's active lanes (its chunks come from getNPage). */
+ * KernelMallocSimple / KernelMallocCollab over this translation unit's pool
+ * (initMemoryManagement runs with its defaults if it has not run). */
 
-static inline void pbm_compat_ensure_pool() {
-    if (!pbm_compat_handle) initMemoryManagement();
-}
-
-static inline pbm_ctx pbm_compat_host_ctx() {
-    pbm_ctx c;
-    pbm_compat_check(pbm_get_ctx(pbm_compat_handle, &c), "pbm_get_ctx");
-    if (!c.pool || (c.page_size & 7u)) {
-        std::fprintf(stderr, "kernel malloc needs page memory and a page size that is a multiple of 8\n");
-        std::exit(-PBM_ERR_UNSUPPORTED);
+struct PbmCompatPool {
+    static pbm_ctx ctx() {
+        if (!pbm_compat_handle) initMemoryManagement();
+        pbm_ctx c;
+        pbm_compat_check(pbm_get_ctx(pbm_compat_handle, &c), "pbm_get_ctx");
+        return c;
     }
-    return c;
-}
-
-class KernelMallocCollab {
-public:
-    /* KernelMallocCollab() (malloc.cu:214-218) */
-    __host__ KernelMallocCollab() {
-        pbm_compat_ensure_pool();
-        ctx_ = pbm_compat_host_ctx();
-    }
-    /* reset(): every page free (malloc.cu:222-224). */
-    __host__ void reset() { resetMemoryManager(); }
-    /* malloc / free (malloc.cu:231-320). */
-    __device__ void* malloc(int size) {
-        return pbm::kmalloc_collab(ctx_, size < 0 ? 0u : (uint32_t)size, pbm_compat_seed(), nullptr);
-    }
-    __device__ void free(void* addr) { pbm::kfree_collab(ctx_, addr); }
-
-private:
-    pbm_ctx ctx_;
+    static void reset() { resetMemoryManager(); }
 };
 
-class KernelMallocSimple {
-public:
-    /* KernelMallocSimple(maxNThreads) (malloc.cu:76-84): one current chunk
-     * per thread index below maxNThreads; threads beyond get nullptr. */
-    __host__ explicit KernelMallocSimple(int maxNThreads) : max_threads_(maxNThreads) {
-        pbm_compat_ensure_pool();
-        ctx_ = pbm_compat_host_ctx();
-        if (hipMalloc((void**)&current_, (size_t)max_threads_ * 4u) != hipSuccess) {
-            std::fprintf(stderr, "KernelMallocSimple: hipMalloc failed\n");
-            std::exit(-PBM_ERR_OOM);
-        }
-        clear_current();
-    }
-    /* reset(): every page free and no current chunks (malloc.cu:99-104). */
-    __host__ void reset() {
-        resetMemoryManager();
-        clear_current();
-    }
-    /* malloc (malloc.cu:129-175); free does nothing (malloc.cu:178-181). */
-    __device__ void* malloc(int size) {
-        const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-        int32_t dummy = -1;
-        const bool slot = tid < (uint32_t)max_threads_;
-        void* p = pbm::kmalloc_simple(ctx_, slot ? &current_[tid] : &dummy,
-                                      slot && size >= 0 ? (uint32_t)size : 0xFFFFFFFFu,
-                                      pbm_compat_seed(), nullptr);
-        return slot ? p : nullptr;
-    }
-    __device__ void free(void*) {}
-
-private:
-    __host__ void clear_current() {
-        if (hipMemset(current_, 0xFF, (size_t)max_threads_ * 4u) != hipSuccess ||
-            hipDeviceSynchronize() != hipSuccess) {
-            std::fprintf(stderr, "KernelMallocSimple: clearing the current chunks failed\n");
-            std::exit(-PBM_ERR_HIP);
-        }
-    }
-    pbm_ctx ctx_;
-    int32_t* current_ = nullptr;
-    int max_threads_ = 0;
-};
+typedef KernelMallocCollabT<PbmCompatPool> KernelMallocCollab;
+typedef KernelMallocSimpleT<PbmCompatPool> KernelMallocSimple;
 
 #endif /* PBM_COMPAT_HPP */

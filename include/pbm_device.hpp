@@ -1362,7 +1362,7 @@ __device__ inline void free_rw(const pbm_ctx& c, void* ptr, Stats* stats = nullp
  *
  * Both take their chunks (runs of consecutive pages inside one map word)
  * from getNPage, which is the wave-collective getXPage the reference's
- * allocators call (GETXPAGE = getXPageCollabRW_BM, malloc.cu:10-13).
+ * allocators call (GETXPAGE = getXPageCollabRW_BM, malloc.cu:9-12).
  * Headers are 8 bytes, little-endian, and every space starts 8-byte
  * aligned.  Both return nullptr on a pool without page memory or whose
  * page size is not a multiple of 8.

@@ -21,9 +21,10 @@
  * spins); getXPage takes x <= 64 pages inside one map word (per thread for
  * RandomWalk_BM and ClusteredRandomWalk_BM, wave-collective for
  * CollabRW_BM as getXPageCollabRW_BM is, parallelPage.cu:934-991).
- * pageAddress(id) resolves ids of the RandomWalk_BM pool, the one malloc.cu
- * builds on; pbm_compat.hpp's names cannot share a translation unit with
- * these.
+ * pageAddress(id) resolves ids of the RandomWalk_BM pool.  malloc.cuh's
+ * KernelMallocSimple / KernelMallocCollab allocate from the CollabRW_BM
+ * pool, as malloc.cu does.  pbm_compat.hpp's names cannot share a
+ * translation unit with these.
  *
  * Build: hipcc --offload-arch=gfx950 -I include your.hip -L<dir> -lpbm
  */
@@ -35,6 +36,7 @@
 #include <cstdlib>
 #include "pbm.h"
 #include "pbm_device.hpp"
+#include "pbm_kmalloc_classes.hpp"
 
 #ifndef PBM_LEGACY_TOTAL_N_PAGES
 #define PBM_LEGACY_TOTAL_N_PAGES 41900000u /* parallelPage.cuh:3 */
@@ -154,6 +156,23 @@ static __device__ inline int getXPageCollabRW_BM(int X, int* step_count = nullpt
 static __device__ inline void* pageAddress(int pageID) {
     return pbm::page_address(pbm_legacy_ctx_RandomWalk_BM, pageID);
 }
+
+/* malloc.cuh's classes over the CollabRW_BM pool, which malloc.cu
+ * allocates from (INITBM / RESETBM / GETXPAGE, malloc.cu:9-12); the pool is
+ * created by the first manager built (the reference re-creates it with
+ * every manager, malloc.cu:15-19). */
+struct PbmLegacyMallocPool {
+    static pbm_ctx ctx() {
+        if (!pbm_legacy_h_CollabRW_BM) initPagesCollabRW_BM();
+        pbm_ctx c;
+        pbm_legacy_check(pbm_get_ctx(pbm_legacy_h_CollabRW_BM, &c), "pbm_get_ctx");
+        return c;
+    }
+    static void reset() { resetBufferCollabRW_BM(); }
+};
+
+typedef KernelMallocCollabT<PbmLegacyMallocPool> KernelMallocCollab;
+typedef KernelMallocSimpleT<PbmLegacyMallocPool> KernelMallocSimple;
 
 /* monitorBufferRandomWalk (parallelPage.cu:209-223): on a stream of its
  * own, every msGap ms, maxCount times, prints the RandomWalk pool's free

@@ -79,6 +79,8 @@ def build_compat_checks(jobs: int = 4) -> list[Path]:
     jobs_list = [([f"-DPBM_COMPAT_STRATEGY={m}", str(COMPAT_SRC)], compat_binary(n))
                  for n, m in COMPAT_STRATEGIES.items()]
     jobs_list.append(([str(COMPAT_KMALLOC_SRC)], compat_binary("kmalloc")))
+    jobs_list.append((["-DPBM_KMALLOC_LEGACY", str(COMPAT_KMALLOC_SRC)],
+                      compat_binary("kmalloc_legacy")))
     jobs_list.append(([str(COMPAT_OUROBOROS_SRC)], compat_binary("ouroboros")))
     jobs_list.append(([str(COMPAT_LEGACY_SRC)], compat_binary("legacy")))
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:

@@ -6,7 +6,19 @@
 // checked, then free (Collab) or reset (Simple) and the pool must be all
 // free again.  Prints one line and exits 0 when every check holds.
 #include <cstdio>
+#ifdef PBM_KMALLOC_LEGACY
+// the legacy build: parallelPage.cuh + malloc.cuh (unitTests/unitTestMalloc1.cu:6-7)
+#define PBM_LEGACY_TOTAL_N_PAGES 262144u
+#define PBM_LEGACY_PAGE_SIZE 4096u
+#include "pbm_legacy.hpp"
+static float getFreePagePercentage() {
+    float f = 0;
+    pbm_free_page_percentage(pbmLegacyHandleCollabRW_BM(), &f, nullptr);
+    return f;
+}
+#else
 #include "pbm_compat.hpp"
+#endif
 
 __device__ unsigned request_size(int tid) {
     return (unsigned)((1103515245LL * (long long)tid + 12345) % 8188) + 4u;
@@ -77,7 +89,9 @@ int run(const char* name, M& mm, int n, void** d_ptrs, int* d_cnt, bool frees) {
 }
 
 int main() {
+#ifndef PBM_KMALLOC_LEGACY
     initMemoryManagement(1, 4096);  // 262,144 pages of 4 KiB
+#endif
     const int n = 30000;
     void** d_ptrs;
     int* d_cnt;
@@ -87,6 +101,10 @@ int main() {
     if (run("KernelMallocCollab", collab, n, d_ptrs, d_cnt, true)) return 1;
     KernelMallocSimple simple(1000000);
     if (run("KernelMallocSimple", simple, n, d_ptrs, d_cnt, false)) return 1;
+#ifdef PBM_KMALLOC_LEGACY
+    std::printf("compat kmalloc legacy ok\n");
+#else
     std::printf("compat kmalloc ok\n");
+#endif
     return 0;
 }

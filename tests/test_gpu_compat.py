@@ -27,13 +27,16 @@ def test_reference_api_driver(gpu, name):
         assert lines and all(0 <= int(v) <= 32 * 64 for v in lines)
 
 
-def test_reference_kmalloc_driver(gpu):
-    exe = pbm_build.compat_binary("kmalloc")
+@pytest.mark.parametrize("name,ok", [("kmalloc", "compat kmalloc ok"),
+                                     ("kmalloc_legacy", "compat kmalloc legacy ok")])
+def test_reference_kmalloc_driver(gpu, name, ok):
+    # malloc.cuh's classes with the current names and with the legacy header
+    exe = pbm_build.compat_binary(name)
     assert exe.exists(), f"{exe} is missing: run __graft_entry__.build()"
-    log_case("compat kernel malloc driver")
+    log_case(f"compat kernel malloc driver {name}")
     proc = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert proc.returncode == 0, proc.stdout + proc.stderr
-    assert "compat kmalloc ok" in proc.stdout
+    assert ok in proc.stdout
 
 
 def test_reference_ouroboros_driver(gpu):
