@@ -153,7 +153,7 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
     return out
 
 
-def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int) -> dict:
+def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int, cooperative: bool) -> dict:
     """Bursts of mallocRW/freeRW: 262,144 x 4 KiB pages (backed), empty at
     the start (a page-level random pre-fill leaves no runs for multi-page
     requests); n_bursts bursts with lognormal request counts
@@ -168,7 +168,8 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int) -> dict:
     used0 = pool.count_used(stream)
     bs = pbm.Bursts(pool, tr)
     progress(f"bursts: trace of {n_bursts} bursts built; autotuning on {tune_bursts}")
-    tune = bs.autotune(min(tune_bursts, n_bursts), 1, stream)
+    flags = pbm.TUNE_QUICK | (0 if cooperative else pbm.TUNE_NO_COOPERATIVE)
+    tune = bs.autotune(min(tune_bursts, n_bursts), flags, stream)
     progress(f"bursts: tuned {tune['waves_per_cu']} waves/CU, budget {tune['walk_budget']}, "
              f"cooperative {tune['cooperative']}, {tune['ms']:.2f} ms")
     best = tune["launch"]
@@ -413,6 +414,9 @@ def main() -> int:
     ap.add_argument("--serve-warm", type=int, default=800)
     ap.add_argument("--bursts", type=int, default=4096,
                     help="bursts of the mallocRW/freeRW leg (0 = skip)")
+    ap.add_argument("--bursts-cooperative", type=int, default=1,
+                    help="let the bursts autotuner pick the cooperative persistent kernel "
+                         "(0 = phase kernels only, replayed from HIP graphs)")
     ap.add_argument("--cache-refs", type=int, default=1 << 26,
                     help="references of the buffer-cache leg (0 = skip)")
     ap.add_argument("--cache-batch", type=int, default=1 << 16)
@@ -581,7 +585,8 @@ def main() -> int:
 
     serving = (serving_leg(torch, pbm, stream, args.serve_warm, args.serve_steps)
                if args.serve_steps > 0 else None)
-    bursts = bursts_leg(torch, pbm, stream, args.bursts, 256) if args.bursts > 0 else None
+    bursts = (bursts_leg(torch, pbm, stream, args.bursts, 256, bool(args.bursts_cooperative))
+              if args.bursts > 0 else None)
     cache = (cache_leg(torch, pbm, stream, args.cache_refs, args.cache_batch)
              if args.cache_refs > 0 else None)
 

@@ -538,6 +538,8 @@ int pbm_zipf_trace(uint32_t n_logical, double s, uint64_t seed, uint64_t first, 
 #define PBM_TUNE_QUICK     1u  /* 3 timed cycles per candidate instead of 9            */
 #define PBM_TUNE_WALK_ONLY 2u  /* random-walk homes only (no PBM_LAUNCH_RANK)          */
 #define PBM_TUNE_RANK_ONLY 4u  /* rank mode only                                       */
+#define PBM_TUNE_NO_COOPERATIVE 8u /* bursts: phase kernels only (a cooperative launch
+                                    * cannot be captured in a HIP graph)              */
 
 typedef struct pbm_tune_result {
     pbm_launch get_launch;   /* fastest batched launch for pbm_get_pages              */
@@ -609,7 +611,8 @@ int pbm_bursts_barrier_timeouts(pbm_bursts* b, unsigned long long* out, void* st
 
 /* Times the first `count` bursts from the pool's current state over the
  * batched launch space (waves per CU 1..16, walk budget 4/16/64, phase
- * kernels or one cooperative kernel) and returns the fastest in
+ * kernels or one cooperative kernel; phase kernels only with
+ * PBM_TUNE_NO_COOPERATIVE) and returns the fastest in
  * get_launch (ms_alloc = its time).  The pool's map and the pointer table
  * are put back after every candidate and at the end, so the pool must
  * hold only what the trace's first burst expects. */

@@ -372,7 +372,7 @@ extern "C" int pbm_bursts_autotune(pbm_bursts* b, uint32_t count, uint32_t flags
     const uint32_t budgets[] = {4, 16, 64};
     for (uint32_t wpc : wpcs)
         for (uint32_t bud : budgets)
-            for (int coop = 0; coop < 2; ++coop) {
+            for (int coop = 0; coop < ((flags & PBM_TUNE_NO_COOPERATIVE) ? 1 : 2); ++coop) {
                 pbm_launch l{};
                 l.mode = PBM_MODE_BATCHED;
                 l.waves_per_cu = wpc;

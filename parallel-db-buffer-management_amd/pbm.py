@@ -76,7 +76,7 @@ class TuneResult(C.Structure):
                 ("ms_free", C.c_float), ("candidates", C.c_uint32), ("reserved", C.c_uint32)]
 
 
-TUNE_QUICK, TUNE_WALK_ONLY, TUNE_RANK_ONLY = 1, 2, 4
+TUNE_QUICK, TUNE_WALK_ONLY, TUNE_RANK_ONLY, TUNE_NO_COOPERATIVE = 1, 2, 4, 8
 
 
 class ServeConfig(C.Structure):
@@ -529,6 +529,12 @@ class Graph:
         if self.g:
             check(lib().pbm_graph_destroy(self.g), "pbm_graph_destroy")
             self.g = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
         try:

@@ -142,3 +142,21 @@ def test_bursts_autotune_leaves_the_pool(gpu):
             gpu.cuda.synchronize()
             assert bs.barrier_timeouts() == 0
             assert bs.ptrs().any()
+
+
+def test_bursts_autotune_phase_kernels_only(gpu):
+    # PBM_TUNE_NO_COOPERATIVE keeps the tuner to the phase kernels, whose
+    # launch a HIP graph can capture
+    T = 1 << 17
+    log_case("bursts autotune, no cooperative candidates")
+    tr = pbm.burst_trace(24, 1024, 1.0, 16, 65536, PS, 9)
+    with pbm.Pool(T, PS, strategy=pbm.RW_MALLOC, backing=True, seed=1) as pool:
+        before = pool.snapshot()
+        with pbm.Bursts(pool, tr) as bs:
+            r = bs.autotune(24, pbm.TUNE_QUICK | pbm.TUNE_NO_COOPERATIVE)
+            assert r["candidates"] == 15 and not r["cooperative"]
+            assert np.array_equal(pool.snapshot(), before)
+            with bs.graph(0, 24, r["launch"]) as g:
+                g.launch()
+                gpu.cuda.synchronize()
+            assert bs.ptrs().any()
