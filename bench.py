@@ -390,6 +390,7 @@ def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
         "ms": ms, "refs_per_s": n_refs / (ms * 1e-3), "hits": st["hits"], "misses": st["misses"],
         "hit_rate": st["hits"] / n_refs, "evictions": st["evictions"],
         "resident": st["resident"], "alloc_failures": st["alloc_failures"],
+        "scan_timeouts": st["scan_timeouts"],
     }
     cache.close()
     pool.close()

@@ -504,7 +504,7 @@ typedef struct pbm_cache_stats {
     unsigned long long arm;             /* ring position of the clock arm         */
     unsigned long long batches;
     unsigned long long alloc_failures;  /* frames the pool could not give (0)     */
-    unsigned long long reserved;
+    unsigned long long scan_timeouts;   /* look-back waits given up (0)           */
 } pbm_cache_stats;
 
 typedef struct pbm_cache pbm_cache;

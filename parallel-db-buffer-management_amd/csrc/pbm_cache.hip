@@ -1,23 +1,32 @@
 // Buffer cache over a pool: logical pages are kept in `capacity` slots, each
 // slot holding one frame (a pool page) got with the pool strategy's device
-// getPage; CLOCK replacement.  References are taken a batch at a time:
-//   hits        every reference whose page is present sets its slot's bit;
-//               the first reference to an absent page is a miss, later ones
-//               in the batch are hits (an atomicMax per page keeps the first)
-//   misses      compacted in reference order by a scan of first-occurrence
-//               flags
-//   victims     a distributed CLOCK sweep: one scan over the ring of slots
-//               from the arm counts, per position, the clear and the set
-//               bits of slots occupied when the batch began; the first
-//               victims are the clear slots in ring order, and when a pass
-//               has too few, the set slots in ring order after every bit
-//               was cleared; bits passed before the last victim are cleared
-//   evict/load  victims' pages are dropped and their frames freed, then the
-//               misses load into empty slots first and victims' slots after,
-//               each with a new frame and its bit set
+// getPage; CLOCK replacement.  References are taken a batch at a time, in
+// five kernels:
+//   hits     every reference whose page is present sets its slot's bit;
+//            the first reference to an absent page is a miss, later ones
+//            in the batch are hits (an atomicMax per page keeps the first)
+//   misses   the misses, compacted in reference order (decoupled
+//            look-back over tiles of the batch); the last tile plans the
+//            batch: misses K, empty slots, victims needed V
+//   sweep    the CLOCK sweep from the arm: ring position j is slot
+//            (arm + j) mod capacity, and a slot is a candidate when it was
+//            occupied at batch start and its bit is clear.  Workgroups take
+//            tiles of the ring in order and rank the candidates by
+//            decoupled look-back; rank r < V is victim r, and a set bit is
+//            cleared when fewer than V candidates precede it.  Once the
+//            tile holding the V-th candidate is known no further tile is
+//            taken.  When the ring holds Z < V candidates, every set bit
+//            has been cleared and a second launch takes the first V - Z of
+//            the now-clear slots as victims Z.. (victims are marked 2 so it
+//            passes them).  A victim is evicted where it is taken: its page
+//            becomes absent and its frame goes back to the pool
+//   load     misses load into empty slots first and victims' slots after,
+//            each with a new frame and its bit set
+// The batch is closed (arm past the last victim, residency, counters) by
+// the next batch's plan, or by one small kernel at the end of a run.  Each
+// kernel clears the look-back words and tickets the next one uses.
 // With a batch of one reference this is the textbook second-chance CLOCK.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -34,17 +43,37 @@ enum : uint32_t {
     cMisses,
     cEvictions,
     cBatches,
-    cK,        // misses of the current batch
-    cV,        // victims of the current batch
-    cZ,        // clear slots in the ring at batch start
-    cStop,     // ring position of the last victim
+    cK,             // misses of the current batch
+    cV,             // victims of the current batch
+    cZ,             // candidates in the ring when the first sweep ran short
+    cStop,          // ring position of the last victim
     cAllocFail,
-    cR0,       // resident at batch start
-    cEmpty,    // empty slots at batch start
-    cCtlWords = 16
+    cR0,            // resident at batch start
+    cEmpty,         // empty slots at batch start
+    cV2,            // victims left for the second sweep
+    cMissTicket,    // tiles of the miss compaction taken
+    cSweepTicket0,  // tiles of the first sweep taken
+    cSweepTicket1,  // tiles of the second sweep taken
+    cSweepDone0,    // a tile at or past the first sweep's last victim (or kNoStop)
+    cSweepDone1,    // the same for the second sweep
+    cOpen,          // 1 while the last batch waits to be closed
+    cScanTimeouts,  // look-back waits given up (never, while tiles run on)
+    cCtlWords = 32
 };
 
 constexpr u64 kLow = 0xFFFFFFFFull;
+constexpr uint32_t kPer = 4;                                // items per thread
+constexpr uint32_t kTile = pbm::kScanThreads * kPer;        // items per look-back tile
+constexpr uint32_t kVictimBit = 2u;                         // bit value of a chosen victim
+constexpr uint32_t kGroups = 64;                            // workgroups of the sweep and load
+
+// Look-back word: [63:62] 0 unpublished, 1 the tile's own count, 2 its
+// inclusive prefix; [61:0] the count.  Cleared before every launch.
+constexpr u64 kLbOwn = 1ull << 62;
+constexpr u64 kLbIncl = 2ull << 62;
+constexpr u64 kLbCount = kLbOwn - 1ull;
+constexpr u64 kNoStop = ~0ull;
+constexpr uint32_t kLbSpin = 1u << 20;
 
 struct CacheDev {
     pbm_ctx c;
@@ -53,137 +82,309 @@ struct CacheDev {
     int32_t* owner;     // [capacity] page in the slot
     int32_t* frame;     // [capacity] pool page holding it
     uint32_t* refbit;   // [capacity]
-    uint32_t* mflag;    // [batch]
-    uint32_t* mpos;     // [batch]
     int32_t* miss;      // [batch]
-    u64* ring;          // [capacity] clear | set << 32, by ring position
-    u64* ring_rank;     // [capacity] exclusive scan of ring
     int32_t* victim;    // [batch]
+    u64* lb_miss;       // [miss_tiles] look-back words of the miss compaction
+    u64* lb_sweep;      // [2 * sweep_tiles] look-back words of the two sweeps
     int32_t* log;       // [log_cap] evicted pages in order
     u64 log_cap;
     u64* ctl;           // [cCtlWords]
     uint32_t n_logical, cap, batch;
+    uint32_t miss_tiles, sweep_tiles, load_groups;
 };
+
+// The batch counters, read all at once and written back all at once (one
+// thread; every load is issued before the first store).
+struct CtlState {
+    u64 arm, resident, hits, misses, evictions, batches, K, V, stop, r0, open;
+};
+
+__device__ __forceinline__ CtlState load_ctl(const CacheDev& d) {
+    CtlState c;
+    c.arm = d.ctl[cArm];
+    c.resident = d.ctl[cResident];
+    c.hits = d.ctl[cHits];
+    c.misses = d.ctl[cMisses];
+    c.evictions = d.ctl[cEvictions];
+    c.batches = d.ctl[cBatches];
+    c.K = d.ctl[cK];
+    c.V = d.ctl[cV];
+    c.stop = d.ctl[cStop];
+    c.r0 = d.ctl[cR0];
+    c.open = d.ctl[cOpen];
+    return c;
+}
+
+// Closes the last batch: the arm moves past its last victim.
+__device__ __forceinline__ void close_batch(const CacheDev& d, CtlState& c) {
+    if (!c.open) return;
+    if (c.V) c.arm = (c.arm + c.stop + 1u) % d.cap;
+    c.resident = c.r0 + c.K < d.cap ? c.r0 + c.K : d.cap;
+    c.evictions += c.V;
+    c.batches += 1u;
+    c.open = 0;
+}
+
+__device__ __forceinline__ void store_closed(const CacheDev& d, const CtlState& c) {
+    d.ctl[cArm] = c.arm;
+    d.ctl[cResident] = c.resident;
+    d.ctl[cEvictions] = c.evictions;
+    d.ctl[cBatches] = c.batches;
+    d.ctl[cOpen] = c.open;
+}
 
 __device__ __forceinline__ u64 ldc(const u64* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void stc(u64* p, u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 lb_clamp(u64 v) { return v > kLbCount ? kLbCount : v; }
 
-__device__ __forceinline__ void wave_count(u64* dst, bool x) {
-    const u64 m = pbm::ballot(x);
-    if (m && pbm::lane_id() == (unsigned)__ffsll((long long)m) - 1u)
-        pbm::atom_add_u64(dst, (u64)__popcll(m));
+// Workgroup scan of per-thread counts: this thread's offset in the tile
+// (returned) and the tile's total (agg).  Ends with the workgroup in step.
+__device__ __forceinline__ uint32_t tile_scan(uint32_t cnt, uint32_t* s_wsum, uint32_t& agg) {
+    const unsigned lane = pbm::lane_id(), wv = threadIdx.x >> 6;
+    uint32_t inc = cnt;
+#pragma unroll
+    for (unsigned d = 1; d < 64u; d <<= 1) {
+        const uint32_t v = pbm::shfl32(inc, lane >= d ? lane - d : lane);
+        if (lane >= d) inc += v;
+    }
+    if (lane == 63) s_wsum[wv] = inc;
+    __syncthreads();
+    uint32_t woff = 0;
+    agg = 0;
+#pragma unroll
+    for (unsigned i = 0; i < pbm::kScanThreads / 64u; ++i) {
+        if (i < wv) woff += s_wsum[i];
+        agg += s_wsum[i];
+    }
+    return woff + inc - cnt;
 }
 
-__global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* refs, uint32_t nb) {
+// Decoupled look-back, run by wave 0: publishes the tile's own count, sums
+// the counts of the preceding tiles back to the nearest one that published
+// its inclusive prefix (64 at a time), publishes this tile's inclusive
+// prefix and returns the exclusive one.  Every preceding tile was taken
+// earlier by a running workgroup, so the wait ends, unless that workgroup
+// stopped because the sweep's stop (*done) lies before it, and then so
+// does this tile's: kNoStop is returned.  A bounded number of polls is
+// kept all the same, and giving up is counted.
+__device__ u64 lb_prefix(u64* words, uint32_t tile, u64 agg, const u64* done, u64* timeouts) {
+    const unsigned lane = pbm::lane_id();
+    if (tile == 0) {
+        if (lane == 0) stc(&words[0], kLbIncl | lb_clamp(agg));
+        return 0;
+    }
+    if (lane == 0) stc(&words[tile], kLbOwn | lb_clamp(agg));
+    int64_t k = (int64_t)tile - 1;
+    u64 excl = 0;
+    uint32_t spins = 0;
+    for (;;) {
+        const int64_t idx = k - (int64_t)lane;
+        u64 v = 0;
+        bool ready = true, incl = false;
+        if (idx >= 0) {
+            v = ldc(&words[idx]);
+            ready = (v >> 62) != 0u;
+            incl = (v >> 62) == 2u;
+        }
+        const u64 P = pbm::ballot(incl);
+        const u64 R = pbm::ballot(ready);
+        const unsigned fp = P ? (unsigned)__ffsll((long long)P) - 1u : 63u;
+        const u64 upto = fp == 63u ? pbm::kFull : ((2ull << fp) - 1ull);
+        if ((R & upto) != upto) {
+            if (done && ldc(done) < tile) return kNoStop;
+            if (++spins > kLbSpin) {
+                if (lane == 0) pbm::atom_add_u64(timeouts, 1ull);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        // one inclusive word at most is summed, so the sum cannot wrap
+        u64 part = (lane <= fp && idx >= 0) ? (v & kLbCount) : 0ull;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) part += pbm::shfl64(part, lane ^ (unsigned)d);
+        excl += part;
+        if (P) break;
+        k -= 64;
+    }
+    if (lane == 0) stc(&words[tile], kLbIncl | lb_clamp(excl + agg));
+    return excl;
+}
+
+// tag: the batch's number since the cache was made, from 1.
+__global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* refs, uint32_t nb,
+                                                    u64 tag) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const u64 tag_hi = (ldc(&d.ctl[cBatches]) + 1u) << 32;
-    bool hit = false;
+    // the miss compaction's look-back words and tickets start clear
+    for (uint32_t w = i; w < d.miss_tiles; w += gridDim.x * blockDim.x) d.lb_miss[w] = 0;
+    if (i == 0) d.ctl[cMissTicket] = 0;
+    const u64 tag_hi = tag << 32;
     if (i < nb) {
         const int32_t l = refs[i];
         const int32_t s = d.slot_of[l];
         if (s >= 0) {
-            d.refbit[s] = 1u;
-            hit = true;
+            // hot slots are hit many times a batch: store only a change
+            if (d.refbit[s] != 1u) d.refbit[s] = 1u;
         } else {
-            pbm::atom_max_u64(&d.first[l], tag_hi | (kLow - i));
+            const u64 mine = tag_hi | (kLow - i);
+            if (ldc(&d.first[l]) < mine) pbm::atom_max_u64(&d.first[l], mine);
         }
     }
-    wave_count(&d.ctl[cHits], hit);
 }
 
-__global__ __launch_bounds__(256) void k_cache_flag(CacheDev d, const int32_t* refs, uint32_t nb) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const u64 tag_hi = (ldc(&d.ctl[cBatches]) + 1u) << 32;
-    bool later = false;
-    if (i < nb) {
-        const int32_t l = refs[i];
-        uint32_t f = 0;
-        if (d.slot_of[l] < 0) {
-            f = d.first[l] == (tag_hi | (kLow - i)) ? 1u : 0u;
-            later = !f;
+// First references to absent pages, compacted in reference order.  The
+// last tile closes the last batch and plans this one; hits are the
+// references that are not misses.
+__global__ __launch_bounds__(256) void k_cache_misses(CacheDev d, const int32_t* refs, uint32_t nb,
+                                                      u64 tag) {
+    __shared__ uint32_t s_tile, s_wsum[4];
+    __shared__ u64 s_excl;
+    {
+        // the sweeps' and the load's look-back words and tickets start clear
+        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+        for (uint32_t w = g; w < 2u * d.sweep_tiles; w += gridDim.x * blockDim.x) d.lb_sweep[w] = 0;
+        if (g == 0) {
+            d.ctl[cSweepTicket0] = 0;
+            d.ctl[cSweepTicket1] = 0;
+            d.ctl[cSweepDone0] = kNoStop;
+            d.ctl[cSweepDone1] = kNoStop;
         }
-        d.mflag[i] = f;
     }
-    wave_count(&d.ctl[cHits], later);
-}
-
-// Miss list and the batch plan: K misses, empty slots, victims needed.
-__global__ __launch_bounds__(256) void k_cache_plan(CacheDev d, const int32_t* refs, uint32_t nb) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nb && d.mflag[i]) d.miss[d.mpos[i]] = refs[i];
-    if (i == 0) {
-        const u64 K = (u64)d.mpos[nb - 1] + d.mflag[nb - 1];
-        const u64 r0 = d.ctl[cResident];
-        const u64 empty = d.cap - r0;
+    if (threadIdx.x == 0) s_tile = (uint32_t)pbm::atom_add_u64(&d.ctl[cMissTicket], 1ull);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const u64 tag_hi = tag << 32;
+    const uint32_t i0 = tile * kTile + threadIdx.x * kPer;
+    int32_t lv[kPer];
+    uint32_t fm = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t i = i0 + k;
+        lv[k] = -1;
+        if (i < nb) {
+            const int32_t l = refs[i];
+            if (d.slot_of[l] < 0 && d.first[l] == (tag_hi | (kLow - i))) {
+                lv[k] = l;
+                fm |= 1u << k;
+                ++cnt;
+            }
+        }
+    }
+    uint32_t agg = 0;
+    const uint32_t off = tile_scan(cnt, s_wsum, agg);
+    if (threadIdx.x < 64) {
+        const u64 e = lb_prefix(d.lb_miss, tile, agg, nullptr, &d.ctl[cScanTimeouts]);
+        if (threadIdx.x == 0) s_excl = e;
+    }
+    __syncthreads();
+    u64 r = s_excl + off;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k)
+        if (fm >> k & 1u) d.miss[r++] = lv[k];
+    if (tile == d.miss_tiles - 1u && threadIdx.x == 0) {
+        CtlState c = load_ctl(d);
+        close_batch(d, c);
+        const u64 K = s_excl + agg;
+        const u64 empty = d.cap - c.resident;
+        store_closed(d, c);
         d.ctl[cK] = K;
-        d.ctl[cR0] = r0;
+        d.ctl[cR0] = c.resident;
         d.ctl[cEmpty] = empty;
         d.ctl[cV] = K > empty ? K - empty : 0u;
-        d.ctl[cMisses] += K;
+        d.ctl[cV2] = 0;
+        d.ctl[cMisses] = c.misses + K;
+        d.ctl[cHits] = c.hits + (nb - K);
+        d.ctl[cOpen] = 1;
     }
 }
 
-// Ring position j is slot (arm + j) mod capacity.
-__global__ __launch_bounds__(256) void k_cache_ring(CacheDev d) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= d.cap || ldc(&d.ctl[cV]) == 0) return;
+// The first CLOCK sweep over the ring (pass 0), or the second one that
+// takes victims among the slots the first cleared (pass 1).  kGroups
+// workgroups take tiles in ring order until the stop is known.
+__global__ __launch_bounds__(256) void k_cache_sweep(CacheDev d, uint32_t pass) {
+    __shared__ uint32_t s_tile, s_wsum[4];
+    __shared__ u64 s_excl;
+    const u64 v = ldc(&d.ctl[pass ? cV2 : cV]);
+    if (v == 0) return;
+    u64* words = d.lb_sweep + (pass ? d.sweep_tiles : 0u);
+    u64* done = &d.ctl[pass ? cSweepDone1 : cSweepDone0];
+    u64* ticket = &d.ctl[pass ? cSweepTicket1 : cSweepTicket0];
     const u64 arm = ldc(&d.ctl[cArm]), r0 = ldc(&d.ctl[cR0]);
-    const uint32_t s = (uint32_t)((arm + j) % d.cap);
-    u64 f = 0;
-    if (s < r0) f = d.refbit[s] ? (1ull << 32) : 1ull;
-    d.ring[j] = f;
-}
-
-__global__ __launch_bounds__(256) void k_cache_select(CacheDev d) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    const u64 v = ldc(&d.ctl[cV]);
-    if (j >= d.cap || v == 0) return;
-    const u64 arm = ldc(&d.ctl[cArm]);
-    const u64 z = (d.ring_rank[d.cap - 1] + d.ring[d.cap - 1]) & kLow;
-    const u64 f = d.ring[j], r = d.ring_rank[j];
-    const int32_t s = (int32_t)((arm + j) % d.cap);
-    const u64 zr = r & kLow, orank = r >> 32;
-    if (j == 0) d.ctl[cZ] = z;
-    if (v <= z) {
-        if ((f & kLow) && zr < v) {
-            d.victim[zr] = s;
-            if (zr == v - 1) d.ctl[cStop] = j;
-        }
-    } else {
-        if (f & kLow) d.victim[zr] = s;
-        if ((f >> 32) && orank < v - z) {
-            d.victim[z + orank] = s;
-            if (orank == v - z - 1) d.ctl[cStop] = j;
-        }
-    }
-}
-
-// Bits the sweep passed are cleared: before the last victim in one pass,
-// all of them in two.
-__global__ __launch_bounds__(256) void k_cache_clear(CacheDev d) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    const u64 v = ldc(&d.ctl[cV]);
-    if (j >= d.cap || v == 0) return;
-    const u64 z = ldc(&d.ctl[cZ]), stop = ldc(&d.ctl[cStop]);
-    if ((d.ring[j] >> 32) && (v > z || j < stop)) {
-        const u64 arm = ldc(&d.ctl[cArm]);
-        d.refbit[(arm + j) % d.cap] = 0u;
-    }
-}
-
-// Victims leave: their pages become absent and their frames go back.
-__global__ __launch_bounds__(256) void k_cache_evict(CacheDev d) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const u64 v = ldc(&d.ctl[cV]);
+    const u64 vbase = pass ? ldc(&d.ctl[cZ]) : 0u;
+    const u64 ev0 = ldc(&d.ctl[cEvictions]);
     pbm::Stats st;
-    if (k < v) {
-        const int32_t s = d.victim[k];
-        const int32_t old = d.owner[s];
-        d.slot_of[old] = -1;
-        if (d.frame[s] >= 0) pbm::free_page(d.c, d.frame[s], &st);
-        const u64 at = ldc(&d.ctl[cEvictions]) + k;
-        if (at < d.log_cap) d.log[at] = old;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const u64 t = pbm::atom_add_u64(ticket, 1ull);
+            s_tile = (t >= d.sweep_tiles || ldc(done) < t) ? ~0u : (uint32_t)t;
+        }
+        __syncthreads();
+        const uint32_t tile = s_tile;
+        if (tile == ~0u) break;
+        const uint32_t j0 = tile * kTile + threadIdx.x * kPer;
+        uint32_t slot[kPer], bit[kPer];
+        uint32_t occ = 0, cnt = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            const uint32_t j = j0 + k;
+            slot[k] = 0;
+            bit[k] = 0;
+            if (j < d.cap) {
+                u64 s = arm + j;
+                if (s >= d.cap) s -= d.cap;
+                slot[k] = (uint32_t)s;
+                if (s < r0) {
+                    occ |= 1u << k;
+                    bit[k] = d.refbit[s];
+                    if (bit[k] == 0u) ++cnt;
+                }
+            }
+        }
+        uint32_t agg = 0;
+        const uint32_t off = tile_scan(cnt, s_wsum, agg);
+        if (threadIdx.x < 64) {
+            const u64 e = lb_prefix(words, tile, agg, done, &d.ctl[cScanTimeouts]);
+            if (threadIdx.x == 0) {
+                s_excl = e;
+                // a tile that holds the last victim, or lies past it,
+                // stops the taking of further tiles
+                if (e != kNoStop && e + agg >= v && ldc(done) > tile)
+                    __hip_atomic_fetch_min(done, (u64)tile, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                if (pass == 0 && e != kNoStop && tile == d.sweep_tiles - 1u && e + agg < v) {
+                    // fewer candidates than victims: every set bit is
+                    // cleared, and the second sweep takes the rest
+                    d.ctl[cZ] = e + agg;
+                    d.ctl[cV2] = v - (e + agg);
+                }
+            }
+        }
+        __syncthreads();
+        const u64 excl = s_excl;  // kNoStop (past the stop) is never below v
+        u64 r = excl + off;       // candidates before this thread's first position
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            if (excl >= v || !(occ >> k & 1u) || r >= v) continue;
+            const uint32_t s = slot[k];
+            if (bit[k] == 0u) {
+                const u64 vi = vbase + r;
+                d.victim[vi] = (int32_t)s;
+                d.refbit[s] = kVictimBit;
+                const int32_t old = d.owner[s];
+                d.slot_of[old] = -1;
+                if (d.frame[s] >= 0) pbm::free_page(d.c, d.frame[s], &st);
+                if (ev0 + vi < d.log_cap) d.log[ev0 + vi] = old;
+                if (r == v - 1u) d.ctl[cStop] = j0 + k;
+                ++r;
+            } else if (bit[k] == 1u) {
+                d.refbit[s] = 0u;  // the sweep passes a set bit
+            }
+        }
+        __syncthreads();  // the shared words are taken again
     }
     pbm::flush_stats(d.c, st);
 }
@@ -191,33 +392,36 @@ __global__ __launch_bounds__(256) void k_cache_evict(CacheDev d) {
 // Misses load: into empty slots first, then into the victims' slots.
 template <int S>
 __global__ __launch_bounds__(256) void k_cache_load(CacheDev d) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const u64 K = ldc(&d.ctl[cK]), empty = ldc(&d.ctl[cEmpty]), r0 = ldc(&d.ctl[cR0]);
     const u64 batch_no = ldc(&d.ctl[cBatches]);
+    const u64 stride = (u64)gridDim.x * blockDim.x;
     pbm::Stats st;
-    bool fail = false;
-    if (i < K) {
+    uint32_t fail = 0;
+    for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i - pbm::lane_id() < K; i += stride) {
+        if (i >= K) continue;
         const int32_t l = d.miss[i];
-        const int32_t s = i < empty ? (int32_t)(r0 + i) : d.victim[i - empty];
+        const u64 s = i < empty ? r0 + i : (u64)(uint32_t)d.victim[i - empty];
         const int32_t seed = pbm::request_seed(d.c.seed, (batch_no << 32) | i);
-        const int32_t f = pbm::get_page_of<S>(d.c, i, seed, nullptr, &st);
-        fail = f < 0;
-        d.frame[s] = f;
-        d.owner[s] = l;
-        d.slot_of[l] = s;
-        d.refbit[s] = 1u;
+        const int32_t f = pbm::get_page_of<S>(d.c, (uint32_t)i, seed, nullptr, &st);
+        if (s < d.cap) {
+            fail += f < 0;
+            d.frame[s] = f;
+            d.owner[s] = l;
+            d.slot_of[l] = (int32_t)s;
+            d.refbit[s] = 1u;
+        } else {
+            ++fail;  // no slot: cannot happen while the plan holds
+            if (f >= 0) pbm::free_page(d.c, f, &st);
+        }
     }
-    wave_count(&d.ctl[cAllocFail], fail);
+    if (fail) pbm::atom_add_u64(&d.ctl[cAllocFail], (u64)fail);
     pbm::flush_stats(d.c, st);
 }
 
-// End of batch: arm past the last victim, residency, counters.
-__global__ void k_cache_finish(CacheDev d) {
-    const u64 v = d.ctl[cV], K = d.ctl[cK], r0 = d.ctl[cR0];
-    if (v) d.ctl[cArm] = (d.ctl[cArm] + d.ctl[cStop] + 1u) % d.cap;
-    d.ctl[cResident] = r0 + K < d.cap ? r0 + K : d.cap;
-    d.ctl[cEvictions] += v;
-    d.ctl[cBatches] += 1u;
+__global__ void k_cache_close(CacheDev d) {
+    CtlState c = load_ctl(d);
+    close_batch(d, c);
+    store_closed(d, c);
 }
 
 __global__ void k_cache_init(CacheDev d) {
@@ -231,6 +435,8 @@ __global__ void k_cache_init(CacheDev d) {
         d.frame[i] = -1;
         d.refbit[i] = 0;
     }
+    if (i < d.miss_tiles) d.lb_miss[i] = 0;
+    if (i < 2u * d.sweep_tiles) d.lb_sweep[i] = 0;
     if (i < cCtlWords) d.ctl[i] = 0;
 }
 
@@ -258,8 +464,7 @@ struct pbm_cache {
     pbm_cache_config cfg{};
     CacheDev d{};
     std::vector<void*> mem;
-    void* scan_tmp = nullptr;
-    size_t scan_bytes_b = 0, scan_bytes_c = 0;
+    u64 tag = 0;  // batches launched
 };
 
 extern "C" int pbm_cache_destroy(pbm_cache* k) {
@@ -293,6 +498,9 @@ extern "C" int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_
     d.cap = cfg->capacity;
     d.batch = cfg->batch;
     d.log_cap = cfg->log_capacity;
+    d.miss_tiles = (cfg->batch + kTile - 1u) / kTile;
+    d.sweep_tiles = (cfg->capacity + kTile - 1u) / kTile;
+    d.load_groups = std::min((cfg->batch + 255u) / 256u, kGroups);
     auto alloc = [&](void** p, size_t bytes) -> int {
         hipError_t e = hipMalloc(p, bytes ? bytes : 8u);
         if (e != hipSuccess) return pbmi::hip_fail(e, "hipMalloc(cache)", __FILE__, __LINE__);
@@ -311,29 +519,19 @@ extern "C" int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_
     PBM_CALLOC(owner, int32_t, cfg->capacity)
     PBM_CALLOC(frame, int32_t, cfg->capacity)
     PBM_CALLOC(refbit, uint32_t, cfg->capacity)
-    PBM_CALLOC(mflag, uint32_t, cfg->batch)
-    PBM_CALLOC(mpos, uint32_t, cfg->batch)
     PBM_CALLOC(miss, int32_t, cfg->batch)
-    PBM_CALLOC(ring, u64, cfg->capacity)
-    PBM_CALLOC(ring_rank, u64, cfg->capacity)
     PBM_CALLOC(victim, int32_t, cfg->batch)
+    PBM_CALLOC(lb_miss, u64, d.miss_tiles)
+    PBM_CALLOC(lb_sweep, u64, 2u * d.sweep_tiles)
     PBM_CALLOC(log, int32_t, cfg->log_capacity)
     PBM_CALLOC(ctl, u64, cCtlWords)
 #undef PBM_CALLOC
-    if (!rc) {
-        hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, k->scan_bytes_b, d.mflag, d.mpos,
-                                                        (int)cfg->batch);
-        if (e == hipSuccess)
-            e = hipcub::DeviceScan::ExclusiveSum(nullptr, k->scan_bytes_c, d.ring, d.ring_rank,
-                                                 (int)cfg->capacity);
-        if (e != hipSuccess) rc = pbmi::hip_fail(e, "hipcub scan sizing", __FILE__, __LINE__);
-        else rc = alloc(&k->scan_tmp, std::max(k->scan_bytes_b, k->scan_bytes_c));
-    }
     if (rc) {
         pbm_cache_destroy(k);
         return rc;
     }
-    const uint32_t n_init = std::max(std::max(cfg->n_logical, cfg->capacity), (uint32_t)cCtlWords);
+    // (miss_tiles <= 2 * sweep_tiles <= capacity + 1)
+    const uint32_t n_init = std::max(std::max(cfg->n_logical, cfg->capacity + 1u), (uint32_t)cCtlWords);
     hipLaunchKernelGGL(k_cache_init, dim3((n_init + 255u) / 256u), dim3(256), 0, nullptr, d);
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -348,21 +546,16 @@ namespace {
 
 int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
     const CacheDev& d = k->d;
-    const dim3 b(256), gb((nb + 255u) / 256u), gc((d.cap + 255u) / 256u);
-    hipLaunchKernelGGL(k_cache_hits, gb, b, 0, s, d, refs, nb);
-    hipLaunchKernelGGL(k_cache_flag, gb, b, 0, s, d, refs, nb);
-    size_t bytes = k->scan_bytes_b;
-    PBM_HIP(hipcub::DeviceScan::ExclusiveSum(k->scan_tmp, bytes, d.mflag, d.mpos, (int)nb, s));
-    hipLaunchKernelGGL(k_cache_plan, gb, b, 0, s, d, refs, nb);
-    hipLaunchKernelGGL(k_cache_ring, gc, b, 0, s, d);
-    bytes = k->scan_bytes_c;
-    PBM_HIP(hipcub::DeviceScan::ExclusiveSum(k->scan_tmp, bytes, d.ring, d.ring_rank, (int)d.cap, s));
-    hipLaunchKernelGGL(k_cache_select, gc, b, 0, s, d);
-    hipLaunchKernelGGL(k_cache_clear, gc, b, 0, s, d);
-    hipLaunchKernelGGL(k_cache_evict, gb, b, 0, s, d);
+    const dim3 b(256);
+    const u64 tag = ++k->tag;
+    hipLaunchKernelGGL(k_cache_hits, dim3((nb + 255u) / 256u), b, 0, s, d, refs, nb, tag);
+    hipLaunchKernelGGL(k_cache_misses, dim3(d.miss_tiles), b, 0, s, d, refs, nb, tag);
+    const dim3 gs(std::min(d.sweep_tiles, kGroups));
+    hipLaunchKernelGGL(k_cache_sweep, gs, b, 0, s, d, 0u);
+    hipLaunchKernelGGL(k_cache_sweep, gs, b, 0, s, d, 1u);
     switch (k->h->cfg.strategy) {
 #define PBM_LOAD_CASE(S) \
-    case S: hipLaunchKernelGGL(k_cache_load<S>, gb, b, 0, s, d); break;
+    case S: hipLaunchKernelGGL(k_cache_load<S>, dim3(d.load_groups), b, 0, s, d); break;
         PBM_LOAD_CASE(PBM_RW_BM)
         PBM_LOAD_CASE(PBM_COLLAB_RW_BM)
         PBM_LOAD_CASE(PBM_CRW_BM)
@@ -375,7 +568,6 @@ int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
 #undef PBM_LOAD_CASE
         default: return pbmi::fail(PBM_ERR_UNSUPPORTED, "no cache load for this strategy");
     }
-    hipLaunchKernelGGL(k_cache_finish, dim3(1), dim3(1), 0, s, d);
     PBM_HIP(hipGetLastError());
     return PBM_OK;
 }
@@ -391,6 +583,10 @@ extern "C" int pbm_cache_run(pbm_cache* k, const int32_t* d_refs, uint64_t n, vo
         const uint32_t nb = (uint32_t)(n - b0 < k->d.batch ? n - b0 : k->d.batch);
         int rc = cache_batch(k, d_refs + b0, nb, s);
         if (rc) return rc;
+    }
+    if (n) {
+        hipLaunchKernelGGL(k_cache_close, dim3(1), dim3(1), 0, s, k->d);
+        PBM_HIP(hipGetLastError());
     }
     return PBM_OK;
 }
@@ -409,7 +605,7 @@ extern "C" int pbm_cache_read_stats(pbm_cache* k, pbm_cache_stats* out, void* st
     out->arm = ctl[cArm];
     out->batches = ctl[cBatches];
     out->alloc_failures = ctl[cAllocFail];
-    out->reserved = 0;
+    out->scan_timeouts = ctl[cScanTimeouts];
     return PBM_OK;
 }
 

@@ -105,10 +105,10 @@ class CacheConfig(C.Structure):
 
 class CacheStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("hits", "misses", "evictions", "resident", "arm",
-                                          "batches", "alloc_failures", "reserved")]
+                                          "batches", "alloc_failures", "scan_timeouts")]
 
     def as_dict(self) -> dict:
-        return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != "reserved"}
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
 class BurstTrace(C.Structure):

@@ -30,6 +30,7 @@ def _check(gpu, pool, cache, cc, T):
     st = cache.stats()
     assert st["hits"] == int(cc.stats[0]) and st["misses"] == int(cc.stats[1])
     assert st["evictions"] == int(cc.stats[2]) and st["alloc_failures"] == 0
+    assert st["scan_timeouts"] == 0
     assert st["arm"] == int(cc.arm_res[0]) and st["resident"] == int(cc.arm_res[1])
     assert np.array_equal(cache.evictions(), cc.evictions)
     owner, frame, bit = cache.slots()
@@ -50,6 +51,38 @@ def test_cache_matches_oracle(gpu, B, n):
     cc = orc.ClockCache(L, C, B, log_cap=n)
     cc.run(refs.cpu().numpy())
     with pbm.Pool(T, strategy=pbm.CRW_BM, seed=5) as pool:
+        with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
+            cache.run(refs)
+            gpu.cuda.synchronize()
+            _check(gpu, pool, cache, cc, T)
+
+
+def test_cache_runs_continue(gpu):
+    # a trace run in three calls (split at batch boundaries) ends where one
+    # call over the whole trace ends
+    T, L, C, B, n = 1 << 16, 1 << 14, 2048, 256, 300000
+    log_case("cache run in three calls")
+    refs = _trace(gpu, L, 1.1, 17, n)
+    cc = orc.ClockCache(L, C, B, log_cap=n)
+    cc.run(refs.cpu().numpy())
+    with pbm.Pool(T, strategy=pbm.CRW_BM, seed=5) as pool:
+        with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
+            for a, b in [(0, 256 * 300), (256 * 300, 256 * 301), (256 * 301, n)]:
+                cache.run(refs[a:b])
+                gpu.cuda.synchronize()
+            assert cache.stats()["batches"] == -(-n // B)
+            _check(gpu, pool, cache, cc, T)
+
+
+@pytest.mark.parametrize("C,B,n", [(3001, 1111, 150000), (1025, 1025, 150000), (5000, 1, 30000)])
+def test_cache_ragged_tiles(gpu, C, B, n):
+    # capacities and batches that end inside a look-back tile of 1,024
+    T, L = 1 << 15, 1 << 13
+    log_case(f"cache capacity={C} batch={B}")
+    refs = _trace(gpu, L, 1.0, 13, n)
+    cc = orc.ClockCache(L, C, B, log_cap=n)
+    cc.run(refs.cpu().numpy())
+    with pbm.Pool(T, strategy=pbm.CRW_BM, seed=4) as pool:
         with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
             cache.run(refs)
             gpu.cuda.synchronize()
