@@ -125,10 +125,18 @@ __global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) {
 }
 
 // Grow: every admitted slot gets one page.  A sequence that reaches its
-// target is recorded as finished and its slot listed for release.
+// target is recorded as finished and its slot listed for release.  The
+// step's tallies are summed over the workgroup in LDS and added to the
+// control block with one atomic per word per workgroup.
 template <int S>
 __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
     using namespace pbm;
+    // [0] finished, [1] allocations, [2] allocations still owed, [3..10]
+    // the pool counters (Stats order)
+    __shared__ uint32_t s_sum[11];
+    __shared__ u64 s_base[2];  // first done-list and latency entries of the workgroup
+    if (threadIdx.x < 11) s_sum[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const u64 step = ld_ctl(&d.ctl[kStep]);
     Stats st;
@@ -161,41 +169,63 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
     }
     const unsigned lane = lane_id();
     const u64 F = ballot(fin);
-    if (F) {
-        const unsigned ld = (unsigned)__ffsll((long long)F) - 1u;
-        u64 base = 0;
-        if (lane == ld) {
-            base = atom_add_u64(&d.ctl[kDone], (u64)__popcll(F));
-            atom_add_u64(&d.ctl[kFinished], (u64)__popcll(F));
-        }
-        base = readlane64(base, ld);
-        if (fin) {
-            const u64 k = base + rank_in(F);
-            d.done[2u * k] = slot;
-            d.done[2u * k + 1u] = seq;
-        }
-    }
     const u64 G = ballot(got);
     const u64 X = ballot(act && !got);
-    if (G) {
-        const unsigned ld = (unsigned)__ffsll((long long)G) - 1u;
-        u64 base = 0;
-        if (lane == ld) {
-            atom_add_u64(&d.ctl[kAllocs], (u64)__popcll(G));
-            if (d.flags & PBM_SERVE_LATENCY) base = atom_add_u64(&d.ctl[kLatCount], (u64)__popcll(G));
-        }
-        if (d.flags & PBM_SERVE_LATENCY) {
-            base = readlane64(base, ld);
-            const u64 k = base + rank_in(G);
-            if (got && k < d.lat_cap) d.lat[k] = ticks;
-        }
+    // this wave's place among the workgroup's finished and granted lanes
+    uint32_t wf = 0, wg = 0;
+    if (lane == 0) {
+        if (F) wf = atomicAdd(&s_sum[0], (uint32_t)__popcll(F));
+        if (G) wg = atomicAdd(&s_sum[1], (uint32_t)__popcll(G));
+        if (X) atomicAdd(&s_sum[2], (uint32_t)__popcll(X));
     }
-    if (X && lane == (unsigned)__ffsll((long long)X) - 1u) {
-        // the page is still owed: the next admission must not count it
-        atom_add_u64(&d.ctl[kAllocFail], (u64)__popcll(X));
-        atom_add_u64(&d.ctl[kReserved], (u64)__popcll(X));
+    wf = readlane(wf, 0);
+    wg = readlane(wg, 0);
+    uint32_t v[8] = {st.allocs, st.frees, st.failures, st.steps,
+                     st.probes, st.atomics, st.fallbacks, st.errors};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint32_t x = v[i];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+        if (lane == 0 && x) atomicAdd(&s_sum[3 + i], x);
     }
-    flush_stats(d.c, st);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // one lane per control word: finished (done-list base), finished
+        // total, allocations, latency base, failures, owed pages, then the
+        // pool counters
+        const uint32_t nf = s_sum[0], ng = s_sum[1], nx = s_sum[2];
+        const bool lat = (d.flags & PBM_SERVE_LATENCY) != 0;
+        u64* dst = nullptr;
+        u64 add = 0;
+        switch (lane) {
+            case 0: dst = &d.ctl[kDone]; add = nf; break;
+            case 1: dst = &d.ctl[kFinished]; add = nf; break;
+            case 2: dst = &d.ctl[kAllocs]; add = ng; break;
+            case 3: dst = lat ? &d.ctl[kLatCount] : nullptr; add = ng; break;
+            case 4: dst = &d.ctl[kAllocFail]; add = nx; break;
+            case 5: dst = &d.ctl[kReserved]; add = nx; break;
+            default:
+                if (lane >= 8 && lane < 16 && d.c.counters) {
+                    dst = (u64*)d.c.counters + (lane - 8u);
+                    add = s_sum[3 + (lane - 8u)];
+                }
+        }
+        u64 old = 0;
+        if (dst && add) old = atom_add_u64(dst, add);
+        if (lane == 0) s_base[0] = old;
+        if (lane == 3) s_base[1] = old;
+    }
+    __syncthreads();
+    if (fin) {
+        const u64 k = s_base[0] + wf + rank_in(F);
+        d.done[2u * k] = slot;
+        d.done[2u * k + 1u] = seq;
+    }
+    if ((d.flags & PBM_SERVE_LATENCY) && got) {
+        const u64 k = s_base[1] + wg + rank_in(G);
+        if (k < d.lat_cap) d.lat[k] = ticks;
+    }
 }
 
 // Release: the page lists of the sequences that finished at this step's
@@ -205,6 +235,9 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
 template <int S>
 __global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
     using namespace pbm;
+    __shared__ uint32_t s_sum[9];
+    if (threadIdx.x < 9) s_sum[threadIdx.x] = 0;
+    __syncthreads();
     const unsigned lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
@@ -255,15 +288,34 @@ __global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
             if (valid) free_page_of<S>(d.c, id, &st);
         }
     }
-    uint32_t fr = st.frees;
+    // tallies summed over the workgroup, then one atomic per word: [0]
+    // payload errors, [1..8] the pool counters (Stats order; frees is also
+    // the control block's count)
+    uint32_t v[9] = {(uint32_t)bad, st.allocs, st.frees, st.failures, st.steps,
+                     st.probes, st.atomics, st.fallbacks, st.errors};
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        fr += (uint32_t)__shfl_xor((int)fr, o, 64);
-        bad += shfl64(bad, lane ^ (unsigned)o);
+    for (int i = 0; i < 9; ++i) {
+        uint32_t x = v[i];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+        if (lane == 0 && x) atomicAdd(&s_sum[i], x);
     }
-    if (lane == 0 && fr) atom_add_u64(&d.ctl[kFrees], (u64)fr);
-    if (lane == 0 && bad) atom_add_u64(&d.ctl[kPayloadErr], bad);
-    flush_stats(d.c, st);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        u64* dst = nullptr;
+        u64 add = 0;
+        if (lane == 0) {
+            dst = &d.ctl[kPayloadErr];
+            add = s_sum[0];
+        } else if (lane == 1) {
+            dst = &d.ctl[kFrees];
+            add = s_sum[2];
+        } else if (lane >= 8 && lane < 16 && d.c.counters) {
+            dst = (u64*)d.c.counters + (lane - 8u);
+            add = s_sum[1 + (lane - 8u)];
+        }
+        if (dst && add) atom_add_u64(dst, add);
+    }
 }
 
 __global__ void k_serve_init(ServeDev d) {
