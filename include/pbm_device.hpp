@@ -247,6 +247,16 @@ __device__ inline void release_one(const pbm_ctx& c, bool give) {
     if (lane_id() == leader) atom_add_ll(c.free_count, (long long)__popcll(m));
 }
 
+/* Returns n units to ctx.free_count for every active lane (n may differ by
+ * lane, and be 0), one atomic per wave. */
+__device__ inline void release_n(const pbm_ctx& c, uint32_t n) {
+    const u64 act = ballot(n > 0u);
+    if (act == 0) return;
+    u64 total = 0;
+    for (u64 m = act; m; m &= m - 1ull) total += readlane(n, (unsigned)__ffsll((long long)m) - 1u);
+    if (lane_id() == (unsigned)__ffsll((long long)act) - 1u) atom_add_ll(c.free_count, (long long)total);
+}
+
 /* ---------------------------------------------------------------------
  * Paging (Paging.cuh:30-52): one contiguous pool, O(1) both ways
  * ------------------------------------------------------------------- */
@@ -1304,7 +1314,7 @@ __device__ inline void free_n_pages(const pbm_ctx& c, int32_t id, uint32_t n,
     } else if (n > 0) {
         st.errors++;
     }
-    if (freed) atom_add_ll(c.free_count, (long long)freed);
+    release_n(c, freed);
 }
 
 /* mallocRW (RWMalloc.cuh:299-326): wave-collective over the active lanes.
