@@ -482,11 +482,13 @@ int pbm_serve_clock_hz(pbm_server* s, unsigned long long* hz);
  * in the batch are hits; misses fill empty slots first and then replace
  * victims chosen by a CLOCK sweep from the arm over the slots occupied when
  * the batch began (set bits are cleared and passed; if one pass has too few
- * clear slots, the next pass takes slots in ring order), the sweep being
- * one scan over the whole ring; loaded pages start with the bit set.  A
- * batch of 1 is the textbook second-chance CLOCK.  No reference
- * counterpart: the reference's clocks are allocators
- * (parallelPage.cu:470-618) and it has no replacement. */
+ * clear slots, the next pass takes slots in ring order); the sweep stops
+ * at the last victim; loaded pages start with the bit set.  A batch of 1
+ * is the textbook second-chance CLOCK.  A cache's calls must be ordered
+ * (one stream, or synchronized): each batch's kernels hand tickets and
+ * look-back words to the next.  No reference counterpart: the reference's
+ * clocks are allocators (parallelPage.cu:470-618) and it has no
+ * replacement. */
 
 typedef struct pbm_cache_config {
     uint32_t n_logical;       /* logical pages, ids 0 .. n_logical-1               */
@@ -512,7 +514,9 @@ typedef struct pbm_cache pbm_cache;
 int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_cache** out);
 int pbm_cache_destroy(pbm_cache* k);
 
-/* Runs n references (device array of logical page ids) through the cache. */
+/* Runs n references (device array of logical page ids, each below
+ * n_logical) through the cache: five kernels per batch, and one that closes
+ * the last batch. */
 int pbm_cache_run(pbm_cache* k, const int32_t* d_refs, uint64_t n, void* stream);
 
 int pbm_cache_read_stats(pbm_cache* k, pbm_cache_stats* out, void* stream);
