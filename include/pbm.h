@@ -515,8 +515,8 @@ int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_cache** out
 int pbm_cache_destroy(pbm_cache* k);
 
 /* Runs n references (device array of logical page ids, each below
- * n_logical) through the cache: five kernels per batch, and one that closes
- * the last batch. */
+ * n_logical) through the cache: three kernels per batch, and one that
+ * closes the last batch. */
 int pbm_cache_run(pbm_cache* k, const int32_t* d_refs, uint64_t n, void* stream);
 
 int pbm_cache_read_stats(pbm_cache* k, pbm_cache_stats* out, void* stream);

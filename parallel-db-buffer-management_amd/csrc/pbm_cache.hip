@@ -1,30 +1,30 @@
 // Buffer cache over a pool: logical pages are kept in `capacity` slots, each
 // slot holding one frame (a pool page) got with the pool strategy's device
 // getPage; CLOCK replacement.  References are taken a batch at a time, in
-// five kernels:
-//   hits     every reference whose page is present sets its slot's bit;
-//            the first reference to an absent page is a miss, later ones
-//            in the batch are hits (an atomicMax per page keeps the first)
-//   misses   the misses, compacted in reference order (decoupled
-//            look-back over tiles of the batch); the last tile plans the
-//            batch: misses K, empty slots, victims needed V
-//   sweep    the CLOCK sweep from the arm: ring position j is slot
-//            (arm + j) mod capacity, and a slot is a candidate when it was
-//            occupied at batch start and its bit is clear.  Workgroups take
-//            tiles of the ring in order and rank the candidates by
-//            decoupled look-back; rank r < V is victim r, and a set bit is
-//            cleared when fewer than V candidates precede it.  Once the
-//            tile holding the V-th candidate is known no further tile is
-//            taken.  When the ring holds Z < V candidates, every set bit
-//            has been cleared and a second launch takes the first V - Z of
-//            the now-clear slots as victims Z.. (victims are marked 2 so it
-//            passes them).  A victim is evicted where it is taken: its page
-//            becomes absent and its frame goes back to the pool
-//   load     misses load into empty slots first and victims' slots after,
-//            each with a new frame and its bit set
-// The batch is closed (arm past the last victim, residency, counters) by
-// the next batch's plan, or by one small kernel at the end of a run.  Each
-// kernel clears the look-back words and tickets the next one uses.
+// three kernels:
+//   hits   closes the last batch (the arm moves past its last victim) and
+//          clears this batch's look-back state; every reference whose page
+//          is present sets its slot's bit, and the first reference to an
+//          absent page is its miss (an atomicMax per page keeps the first)
+//   plan   persistent workgroups take tiles by ticket: first the tiles of
+//          the batch, then the tiles of the ring.
+//          - a batch tile compacts its misses in reference order (decoupled
+//            look-back); the last one plans the batch (misses K, victims
+//            V) and publishes V with the batch's tag in one atomic word
+//          - a ring tile is the CLOCK sweep from the arm: ring position j is
+//            slot (arm + j) mod capacity, and a slot is a candidate when it
+//            was occupied at batch start and its bit is clear.  Look-back
+//            ranks the candidates; rank r < V is victim r, evicted where it
+//            is taken (its page becomes absent, its frame goes back to the
+//            pool), and a set bit is cleared when fewer than V candidates
+//            precede it.  Once a tile at or past the V-th candidate is
+//            known, no later tile is taken
+//   load   when the ring held Z < V candidates, every set bit has been
+//          cleared and the load kernel first sweeps again, taking the first
+//          V - Z now-clear slots as victims Z.. (victims carry bit value 2
+//          until loaded, so it passes them).  Then misses load into empty
+//          slots first and victims' slots after, each with a new frame and
+//          its bit set
 // With a batch of one reference this is the textbook second-chance CLOCK.
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -51,13 +51,13 @@ enum : uint32_t {
     cR0,            // resident at batch start
     cEmpty,         // empty slots at batch start
     cV2,            // victims left for the second sweep
-    cMissTicket,    // tiles of the miss compaction taken
-    cSweepTicket0,  // tiles of the first sweep taken
+    cTicket,        // tiles of the plan kernel taken (batch tiles, then ring tiles)
     cSweepTicket1,  // tiles of the second sweep taken
-    cSweepDone0,    // a tile at or past the first sweep's last victim (or kNoStop)
+    cSweepDone0,    // a ring tile at or past the first sweep's last victim (or kNoStop)
     cSweepDone1,    // the same for the second sweep
     cOpen,          // 1 while the last batch waits to be closed
-    cScanTimeouts,  // look-back waits given up (never, while tiles run on)
+    cScanTimeouts,  // waits given up (never, while tiles run on)
+    cPlan,          // (batch tag << 32) | V, published by the batch's last tile
     cCtlWords = 32
 };
 
@@ -66,6 +66,7 @@ constexpr uint32_t kPer = 4;                                // items per thread
 constexpr uint32_t kTile = pbm::kScanThreads * kPer;        // items per look-back tile
 constexpr uint32_t kVictimBit = 2u;                         // bit value of a chosen victim
 constexpr uint32_t kGroups = 64;                            // workgroups of the sweep and load
+constexpr uint32_t kNoTile = ~0u;
 
 // Look-back word: [63:62] 0 unpublished, 1 the tile's own count, 2 its
 // inclusive prefix; [61:0] the count.  Cleared before every launch.
@@ -90,21 +91,27 @@ struct CacheDev {
     u64 log_cap;
     u64* ctl;           // [cCtlWords]
     uint32_t n_logical, cap, batch;
-    uint32_t miss_tiles, sweep_tiles, load_groups;
+    uint32_t miss_tiles, sweep_tiles, plan_groups, load_groups;
 };
+
+__device__ __forceinline__ u64 ldc(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stc(u64* p, u64 v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 lb_clamp(u64 v) { return v > kLbCount ? kLbCount : v; }
 
 // The batch counters, read all at once and written back all at once (one
 // thread; every load is issued before the first store).
 struct CtlState {
-    u64 arm, resident, hits, misses, evictions, batches, K, V, stop, r0, open;
+    u64 arm, resident, evictions, batches, K, V, stop, r0, open;
 };
 
 __device__ __forceinline__ CtlState load_ctl(const CacheDev& d) {
     CtlState c;
     c.arm = d.ctl[cArm];
     c.resident = d.ctl[cResident];
-    c.hits = d.ctl[cHits];
-    c.misses = d.ctl[cMisses];
     c.evictions = d.ctl[cEvictions];
     c.batches = d.ctl[cBatches];
     c.K = d.ctl[cK];
@@ -116,30 +123,17 @@ __device__ __forceinline__ CtlState load_ctl(const CacheDev& d) {
 }
 
 // Closes the last batch: the arm moves past its last victim.
-__device__ __forceinline__ void close_batch(const CacheDev& d, CtlState& c) {
+__device__ __forceinline__ void close_batch(const CacheDev& d) {
+    CtlState c = load_ctl(d);
     if (!c.open) return;
     if (c.V) c.arm = (c.arm + c.stop + 1u) % d.cap;
     c.resident = c.r0 + c.K < d.cap ? c.r0 + c.K : d.cap;
-    c.evictions += c.V;
-    c.batches += 1u;
-    c.open = 0;
-}
-
-__device__ __forceinline__ void store_closed(const CacheDev& d, const CtlState& c) {
     d.ctl[cArm] = c.arm;
     d.ctl[cResident] = c.resident;
-    d.ctl[cEvictions] = c.evictions;
-    d.ctl[cBatches] = c.batches;
-    d.ctl[cOpen] = c.open;
+    d.ctl[cEvictions] = c.evictions + c.V;
+    d.ctl[cBatches] = c.batches + 1u;
+    d.ctl[cOpen] = 0;
 }
-
-__device__ __forceinline__ u64 ldc(const u64* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void stc(u64* p, u64 v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ u64 lb_clamp(u64 v) { return v > kLbCount ? kLbCount : v; }
 
 // Workgroup scan of per-thread counts: this thread's offset in the tile
 // (returned) and the tile's total (agg).  Ends with the workgroup in step.
@@ -215,13 +209,28 @@ __device__ u64 lb_prefix(u64* words, uint32_t tile, u64 agg, const u64* done, u6
     return excl;
 }
 
+// Shared words of a tile-taking workgroup.
+struct TileShared {
+    uint32_t tile, wsum[4];
+    u64 excl, v;
+};
+
 // tag: the batch's number since the cache was made, from 1.
 __global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* refs, uint32_t nb,
                                                     u64 tag) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    // the miss compaction's look-back words and tickets start clear
-    for (uint32_t w = i; w < d.miss_tiles; w += gridDim.x * blockDim.x) d.lb_miss[w] = 0;
-    if (i == 0) d.ctl[cMissTicket] = 0;
+    const uint32_t threads = gridDim.x * blockDim.x;
+    // the look-back words and tickets of this batch start clear
+    for (uint32_t w = i; w < d.miss_tiles; w += threads) d.lb_miss[w] = 0;
+    for (uint32_t w = i; w < 2u * d.sweep_tiles; w += threads) d.lb_sweep[w] = 0;
+    if (i == 0) {
+        close_batch(d);
+        d.ctl[cTicket] = 0;
+        d.ctl[cSweepTicket1] = 0;
+        d.ctl[cSweepDone0] = kNoStop;
+        d.ctl[cSweepDone1] = kNoStop;
+        d.ctl[cV2] = 0;
+    }
     const u64 tag_hi = tag << 32;
     if (i < nb) {
         const int32_t l = refs[i];
@@ -236,27 +245,11 @@ __global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* r
     }
 }
 
-// First references to absent pages, compacted in reference order.  The
-// last tile closes the last batch and plans this one; hits are the
-// references that are not misses.
-__global__ __launch_bounds__(256) void k_cache_misses(CacheDev d, const int32_t* refs, uint32_t nb,
-                                                      u64 tag) {
-    __shared__ uint32_t s_tile, s_wsum[4];
-    __shared__ u64 s_excl;
-    {
-        // the sweeps' and the load's look-back words and tickets start clear
-        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-        for (uint32_t w = g; w < 2u * d.sweep_tiles; w += gridDim.x * blockDim.x) d.lb_sweep[w] = 0;
-        if (g == 0) {
-            d.ctl[cSweepTicket0] = 0;
-            d.ctl[cSweepTicket1] = 0;
-            d.ctl[cSweepDone0] = kNoStop;
-            d.ctl[cSweepDone1] = kNoStop;
-        }
-    }
-    if (threadIdx.x == 0) s_tile = (uint32_t)pbm::atom_add_u64(&d.ctl[cMissTicket], 1ull);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+// A batch tile: first references to absent pages, compacted in reference
+// order.  The last tile plans the batch; hits are the references that are
+// not misses.
+__device__ void miss_tile(const CacheDev& d, const int32_t* refs, uint32_t nb, u64 tag,
+                          uint32_t tile, TileShared& sh) {
     const u64 tag_hi = tag << 32;
     const uint32_t i0 = tile * kTile + threadIdx.x * kPer;
     int32_t lv[kPer];
@@ -267,7 +260,10 @@ __global__ __launch_bounds__(256) void k_cache_misses(CacheDev d, const int32_t*
         lv[k] = -1;
         if (i < nb) {
             const int32_t l = refs[i];
-            if (d.slot_of[l] < 0 && d.first[l] == (tag_hi | (kLow - i))) {
+            // both loads in flight at once; the tag decides only for absent pages
+            const int32_t sl = d.slot_of[l];
+            const u64 fi = d.first[l];
+            if (sl < 0 && fi == (tag_hi | (kLow - i))) {
                 lv[k] = l;
                 fm |= 1u << k;
                 ++cnt;
@@ -275,132 +271,226 @@ __global__ __launch_bounds__(256) void k_cache_misses(CacheDev d, const int32_t*
         }
     }
     uint32_t agg = 0;
-    const uint32_t off = tile_scan(cnt, s_wsum, agg);
+    const uint32_t off = tile_scan(cnt, sh.wsum, agg);
     if (threadIdx.x < 64) {
         const u64 e = lb_prefix(d.lb_miss, tile, agg, nullptr, &d.ctl[cScanTimeouts]);
-        if (threadIdx.x == 0) s_excl = e;
+        if (threadIdx.x == 0) sh.excl = e;
     }
     __syncthreads();
-    u64 r = s_excl + off;
+    u64 r = sh.excl + off;
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k)
         if (fm >> k & 1u) d.miss[r++] = lv[k];
     if (tile == d.miss_tiles - 1u && threadIdx.x == 0) {
-        CtlState c = load_ctl(d);
-        close_batch(d, c);
-        const u64 K = s_excl + agg;
-        const u64 empty = d.cap - c.resident;
-        store_closed(d, c);
+        const u64 K = sh.excl + agg;
+        const u64 r0 = d.ctl[cResident], hits = d.ctl[cHits], misses = d.ctl[cMisses];
+        const u64 empty = d.cap - r0;
+        const u64 V = K > empty ? K - empty : 0u;
         d.ctl[cK] = K;
-        d.ctl[cR0] = c.resident;
+        d.ctl[cR0] = r0;
         d.ctl[cEmpty] = empty;
-        d.ctl[cV] = K > empty ? K - empty : 0u;
-        d.ctl[cV2] = 0;
-        d.ctl[cMisses] = c.misses + K;
-        d.ctl[cHits] = c.hits + (nb - K);
+        d.ctl[cV] = V;
+        d.ctl[cMisses] = misses + K;
+        d.ctl[cHits] = hits + (nb - K);
         d.ctl[cOpen] = 1;
+        // the ring tiles of this launch read V from here
+        stc(&d.ctl[cPlan], ((tag & kLow) << 32) | V);
     }
 }
 
-// The first CLOCK sweep over the ring (pass 0), or the second one that
-// takes victims among the slots the first cleared (pass 1).  kGroups
-// workgroups take tiles in ring order until the stop is known.
-__global__ __launch_bounds__(256) void k_cache_sweep(CacheDev d, uint32_t pass) {
-    __shared__ uint32_t s_tile, s_wsum[4];
-    __shared__ u64 s_excl;
-    const u64 v = ldc(&d.ctl[pass ? cV2 : cV]);
-    if (v == 0) return;
+// A ring tile of a CLOCK sweep (pass 0 in the plan kernel, pass 1 in the
+// load kernel).  v: the sweep's victims (kNoStop: read the plan word of
+// batch `tag` once the tile's candidates are counted).  Victims of pass 1
+// are published with a release store after their eviction, for the load
+// threads that wait on them.
+__device__ void sweep_tile(const CacheDev& d, uint32_t pass, uint32_t tile, u64 v, u64 tag,
+                           u64 arm, u64 r0, u64 vbase, u64 ev0, TileShared& sh, pbm::Stats& st) {
     u64* words = d.lb_sweep + (pass ? d.sweep_tiles : 0u);
     u64* done = &d.ctl[pass ? cSweepDone1 : cSweepDone0];
-    u64* ticket = &d.ctl[pass ? cSweepTicket1 : cSweepTicket0];
-    const u64 arm = ldc(&d.ctl[cArm]), r0 = ldc(&d.ctl[cR0]);
-    const u64 vbase = pass ? ldc(&d.ctl[cZ]) : 0u;
+    const uint32_t j0 = tile * kTile + threadIdx.x * kPer;
+    uint32_t slot[kPer], bit[kPer];
+    int32_t own[kPer], frm[kPer];
+    uint32_t occ = 0, cnt = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t j = j0 + k;
+        slot[k] = 0;
+        bit[k] = 0;
+        own[k] = -1;
+        frm[k] = -1;
+        if (j < d.cap) {
+            u64 s = arm + j;
+            if (s >= d.cap) s -= d.cap;
+            slot[k] = (uint32_t)s;
+            if (s < r0) {
+                occ |= 1u << k;
+                bit[k] = d.refbit[s];
+                if (bit[k] == 0u) {
+                    ++cnt;
+                    // a candidate may become a victim: its page and frame
+                    // are read now, off the path that waits for V
+                    own[k] = d.owner[s];
+                    frm[k] = d.frame[s];
+                }
+            }
+        }
+    }
+    uint32_t agg = 0;
+    const uint32_t off = tile_scan(cnt, sh.wsum, agg);
+    // the look-back does not need V, so it overlaps the batch tiles' work
+    if (threadIdx.x < 64) {
+        const u64 e = lb_prefix(words, tile, agg, done, &d.ctl[cScanTimeouts]);
+        if (threadIdx.x == 0) sh.excl = e;
+    }
+    if (threadIdx.x == 0) {
+        u64 vv = v;
+        if (vv == kNoStop) {
+            // the batch's last tile publishes V; it was taken before this one
+            const u64 want = tag & kLow;
+            uint32_t spins = 0;
+            u64 w = ldc(&d.ctl[cPlan]);
+            while ((w >> 32) != want) {
+                if (++spins > kLbSpin) {
+                    pbm::atom_add_u64(&d.ctl[cScanTimeouts], 1ull);
+                    w = want << 32;  // no victims
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                w = ldc(&d.ctl[cPlan]);
+            }
+            vv = w & kLow;
+        }
+        sh.v = vv;
+        const u64 e = sh.excl;
+        // a tile that holds the last victim, or lies past it, stops the
+        // taking of further tiles (with no victims, every tile does)
+        if (vv == 0)
+            __hip_atomic_fetch_min(done, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else if (e != kNoStop && e + agg >= vv && ldc(done) > tile)
+            __hip_atomic_fetch_min(done, (u64)tile, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (pass == 0 && e != kNoStop && tile == d.sweep_tiles - 1u && e + agg < vv) {
+            // fewer candidates than victims: every set bit is cleared here,
+            // and the second sweep takes the rest
+            d.ctl[cZ] = e + agg;
+            d.ctl[cV2] = vv - (e + agg);
+        }
+    }
+    __syncthreads();
+    v = sh.v;
+    if (v == 0) return;
+    const u64 excl = sh.excl;  // kNoStop (past the stop) is never below v
+    if (pass == 0 && excl != kNoStop && tile == d.sweep_tiles - 1u && excl + agg < v) {
+        // the second sweep's victims start unpublished
+        for (u64 k = excl + agg + threadIdx.x; k < v; k += blockDim.x) d.victim[k] = -1;
+    }
+    u64 r = excl + off;  // candidates before this thread's first position
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        if (excl >= v || !(occ >> k & 1u) || r >= v) continue;
+        const uint32_t s = slot[k];
+        if (bit[k] == 0u) {
+            const u64 vi = vbase + r;
+            d.refbit[s] = kVictimBit;
+            const int32_t old = own[k];
+            d.slot_of[old] = -1;
+            if (frm[k] >= 0) pbm::free_page(d.c, frm[k], &st);
+            if (ev0 + vi < d.log_cap) d.log[ev0 + vi] = old;
+            if (pass)
+                __hip_atomic_store(&d.victim[vi], (int32_t)s, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            else
+                d.victim[vi] = (int32_t)s;
+            if (r == v - 1u) d.ctl[cStop] = j0 + k;
+            ++r;
+        } else if (bit[k] == 1u) {
+            d.refbit[s] = 0u;  // the sweep passes a set bit
+        }
+    }
+}
+
+// Batch tiles, then ring tiles of the first sweep, taken by ticket.
+__global__ __launch_bounds__(256) void k_cache_plan(CacheDev d, const int32_t* refs, uint32_t nb,
+                                                    u64 tag) {
+    __shared__ TileShared sh;
+    const u64 arm = ldc(&d.ctl[cArm]), r0 = ldc(&d.ctl[cResident]);
     const u64 ev0 = ldc(&d.ctl[cEvictions]);
     pbm::Stats st;
     for (;;) {
         if (threadIdx.x == 0) {
-            const u64 t = pbm::atom_add_u64(ticket, 1ull);
-            s_tile = (t >= d.sweep_tiles || ldc(done) < t) ? ~0u : (uint32_t)t;
+            const u64 t = pbm::atom_add_u64(&d.ctl[cTicket], 1ull);
+            uint32_t tile = kNoTile;
+            if (t < d.miss_tiles) {
+                tile = (uint32_t)t;
+            } else {
+                const u64 rt = t - d.miss_tiles;
+                if (rt < d.sweep_tiles && !(ldc(&d.ctl[cSweepDone0]) < rt))
+                    tile = d.miss_tiles + (uint32_t)rt;
+            }
+            sh.tile = tile;
         }
         __syncthreads();
-        const uint32_t tile = s_tile;
-        if (tile == ~0u) break;
-        const uint32_t j0 = tile * kTile + threadIdx.x * kPer;
-        uint32_t slot[kPer], bit[kPer];
-        uint32_t occ = 0, cnt = 0;
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) {
-            const uint32_t j = j0 + k;
-            slot[k] = 0;
-            bit[k] = 0;
-            if (j < d.cap) {
-                u64 s = arm + j;
-                if (s >= d.cap) s -= d.cap;
-                slot[k] = (uint32_t)s;
-                if (s < r0) {
-                    occ |= 1u << k;
-                    bit[k] = d.refbit[s];
-                    if (bit[k] == 0u) ++cnt;
-                }
-            }
-        }
-        uint32_t agg = 0;
-        const uint32_t off = tile_scan(cnt, s_wsum, agg);
-        if (threadIdx.x < 64) {
-            const u64 e = lb_prefix(words, tile, agg, done, &d.ctl[cScanTimeouts]);
-            if (threadIdx.x == 0) {
-                s_excl = e;
-                // a tile that holds the last victim, or lies past it,
-                // stops the taking of further tiles
-                if (e != kNoStop && e + agg >= v && ldc(done) > tile)
-                    __hip_atomic_fetch_min(done, (u64)tile, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
-                if (pass == 0 && e != kNoStop && tile == d.sweep_tiles - 1u && e + agg < v) {
-                    // fewer candidates than victims: every set bit is
-                    // cleared, and the second sweep takes the rest
-                    d.ctl[cZ] = e + agg;
-                    d.ctl[cV2] = v - (e + agg);
-                }
-            }
-        }
-        __syncthreads();
-        const u64 excl = s_excl;  // kNoStop (past the stop) is never below v
-        u64 r = excl + off;       // candidates before this thread's first position
-#pragma unroll
-        for (uint32_t k = 0; k < kPer; ++k) {
-            if (excl >= v || !(occ >> k & 1u) || r >= v) continue;
-            const uint32_t s = slot[k];
-            if (bit[k] == 0u) {
-                const u64 vi = vbase + r;
-                d.victim[vi] = (int32_t)s;
-                d.refbit[s] = kVictimBit;
-                const int32_t old = d.owner[s];
-                d.slot_of[old] = -1;
-                if (d.frame[s] >= 0) pbm::free_page(d.c, d.frame[s], &st);
-                if (ev0 + vi < d.log_cap) d.log[ev0 + vi] = old;
-                if (r == v - 1u) d.ctl[cStop] = j0 + k;
-                ++r;
-            } else if (bit[k] == 1u) {
-                d.refbit[s] = 0u;  // the sweep passes a set bit
-            }
-        }
+        const uint32_t tile = sh.tile;
+        if (tile == kNoTile) break;
+        if (tile < d.miss_tiles)
+            miss_tile(d, refs, nb, tag, tile, sh);
+        else
+            sweep_tile(d, 0, tile - d.miss_tiles, kNoStop, tag, arm, r0, 0, ev0, sh, st);
         __syncthreads();  // the shared words are taken again
     }
     pbm::flush_stats(d.c, st);
 }
 
-// Misses load: into empty slots first, then into the victims' slots.
+// The second sweep when the first ran short, then the loads: misses go
+// into empty slots first, then into the victims' slots.
 template <int S>
 __global__ __launch_bounds__(256) void k_cache_load(CacheDev d) {
+    __shared__ TileShared sh;
     const u64 K = ldc(&d.ctl[cK]), empty = ldc(&d.ctl[cEmpty]), r0 = ldc(&d.ctl[cR0]);
     const u64 batch_no = ldc(&d.ctl[cBatches]);
-    const u64 stride = (u64)gridDim.x * blockDim.x;
+    const u64 v2 = ldc(&d.ctl[cV2]);
+    const u64 z = v2 ? ldc(&d.ctl[cZ]) : 0u;
     pbm::Stats st;
+    if (v2) {
+        const u64 arm = ldc(&d.ctl[cArm]), ev0 = ldc(&d.ctl[cEvictions]);
+        for (;;) {
+            if (threadIdx.x == 0) {
+                const u64 t = pbm::atom_add_u64(&d.ctl[cSweepTicket1], 1ull);
+                sh.tile = (t < d.sweep_tiles && !(ldc(&d.ctl[cSweepDone1]) < t)) ? (uint32_t)t
+                                                                                 : kNoTile;
+            }
+            __syncthreads();
+            const uint32_t tile = sh.tile;
+            if (tile == kNoTile) break;
+            sweep_tile(d, 1, tile, v2, 0, arm, r0, z, ev0, sh, st);
+            __syncthreads();
+        }
+    }
+    // every ring tile the second sweep needs was taken by a running
+    // workgroup before this one left the loop above
+    const u64 stride = (u64)gridDim.x * blockDim.x;
     uint32_t fail = 0;
     for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i - pbm::lane_id() < K; i += stride) {
         if (i >= K) continue;
         const int32_t l = d.miss[i];
-        const u64 s = i < empty ? r0 + i : (u64)(uint32_t)d.victim[i - empty];
+        u64 s = r0 + i;
+        if (i >= empty) {
+            const u64 k = i - empty;
+            int32_t vs;
+            if (v2 && k >= z) {
+                uint32_t spins = 0;
+                while ((vs = __hip_atomic_load(&d.victim[k], __ATOMIC_ACQUIRE,
+                                               __HIP_MEMORY_SCOPE_AGENT)) < 0) {
+                    if (++spins > kLbSpin) {
+                        pbm::atom_add_u64(&d.ctl[cScanTimeouts], 1ull);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            } else {
+                vs = d.victim[k];
+            }
+            s = (u64)(uint32_t)vs;
+        }
         const int32_t seed = pbm::request_seed(d.c.seed, (batch_no << 32) | i);
         const int32_t f = pbm::get_page_of<S>(d.c, (uint32_t)i, seed, nullptr, &st);
         if (s < d.cap) {
@@ -418,11 +508,7 @@ __global__ __launch_bounds__(256) void k_cache_load(CacheDev d) {
     pbm::flush_stats(d.c, st);
 }
 
-__global__ void k_cache_close(CacheDev d) {
-    CtlState c = load_ctl(d);
-    close_batch(d, c);
-    store_closed(d, c);
-}
+__global__ void k_cache_close(CacheDev d) { close_batch(d); }
 
 __global__ void k_cache_init(CacheDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -501,6 +587,7 @@ extern "C" int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_
     d.miss_tiles = (cfg->batch + kTile - 1u) / kTile;
     d.sweep_tiles = (cfg->capacity + kTile - 1u) / kTile;
     d.load_groups = std::min((cfg->batch + 255u) / 256u, kGroups);
+    d.plan_groups = d.miss_tiles + std::min(d.sweep_tiles, kGroups);
     auto alloc = [&](void** p, size_t bytes) -> int {
         hipError_t e = hipMalloc(p, bytes ? bytes : 8u);
         if (e != hipSuccess) return pbmi::hip_fail(e, "hipMalloc(cache)", __FILE__, __LINE__);
@@ -549,10 +636,7 @@ int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
     const dim3 b(256);
     const u64 tag = ++k->tag;
     hipLaunchKernelGGL(k_cache_hits, dim3((nb + 255u) / 256u), b, 0, s, d, refs, nb, tag);
-    hipLaunchKernelGGL(k_cache_misses, dim3(d.miss_tiles), b, 0, s, d, refs, nb, tag);
-    const dim3 gs(std::min(d.sweep_tiles, kGroups));
-    hipLaunchKernelGGL(k_cache_sweep, gs, b, 0, s, d, 0u);
-    hipLaunchKernelGGL(k_cache_sweep, gs, b, 0, s, d, 1u);
+    hipLaunchKernelGGL(k_cache_plan, dim3(d.plan_groups), b, 0, s, d, refs, nb, tag);
     switch (k->h->cfg.strategy) {
 #define PBM_LOAD_CASE(S) \
     case S: hipLaunchKernelGGL(k_cache_load<S>, dim3(d.load_groups), b, 0, s, d); break;
