@@ -299,10 +299,17 @@ def policy_rows(torch, pbm, stream, n_pages: int, n_req: int, fill: float, steps
             pool.free_pages(n_req, ids, qf, stream)
 
         ms = time_region(torch, step, steps, 1, stream)
+        own = None
+        if lnch.mode == pbm.MODE_BATCHED:
+            # on a small map the two kernels are shorter than the two host
+            # launches; their own durations show the device's part
+            own, _ = time_phases(torch, pbm, pool, n_req, ids, quiet, qf, steps, stream)
         ok = (bool(np.array_equal(pool.snapshot(stream), before))
               and fails == max(0, n_req - A) and pool.counters(stream)["errors"] == 0)
         row[name] = {"walk_steps": ws, "failures": fails, "expected_failures": max(0, n_req - A),
                      "value": 2 * min(n_req, A) / (ms * 1e-3), "ms_per_step": ms, "verified": ok}
+        if own:
+            row[name]["phases_ms"] = own
         if yields:
             row[name]["warp_yield"] = yields
         pool.close()
