@@ -584,7 +584,8 @@ def main() -> int:
     traffic = None
     tfile = ROOT / "profiles" / "traffic_c2_fill90.json"
     if tfile.exists():
-        shapes = {f"k_get_rank<{wpt}>": (L + 256 * wpt - 1) // (256 * wpt)}
+        tw = pool.ctx().scan_threads * wpt
+        shapes = {f"k_get_rank<{wpt}>": (L + tw - 1) // tw}
         for k in json.loads(tfile.read_text())["kernels"]:
             if k["kernel"] == kernel_names[dom] and k["workgroups"] == shapes.get(k["kernel"]):
                 traffic = k["hbm_bytes"]

@@ -123,13 +123,13 @@ typedef struct pbm_ctx {
     uint32_t max_threads;
     int32_t  order;
     uint64_t seed;
-    uint32_t n_tiles;                /* rank-scan tiles: ceil(n_words / (256*scan_wpt)) */
+    uint32_t n_tiles;                /* rank-scan tiles: ceil(n_words / (scan_threads*scan_wpt)) */
     uint32_t scan_wpt;               /* rank scan: map words per thread (1 or 4)        */
     unsigned*           fresh;       /* Ouroboros: pages free at the last reset, ascending */
     unsigned long long* carve;       /* Ouroboros: [0] next fresh index, [1] fresh pages */
     long long*          q_avail;     /* Ouroboros: queued pages no wave has been promised */
     uint32_t chunk_pages;            /* Ouroboros: pages carved per chunk                */
-    uint32_t reserved_;
+    uint32_t scan_threads;           /* rank scan: threads per tile (64 or 256)          */
 } pbm_ctx;
 
 /* Launch shape of a batch call. */

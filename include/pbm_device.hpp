@@ -40,8 +40,9 @@ constexpr u64 kPrefillSalt = 0x243F6A8885A308D3ull;
 constexpr u64 kWaveSalt = 0x13198A2E03707344ull;
 constexpr u64 kArmSalt = 0xA4093822299F31D0ull;
 constexpr u64 kPayloadSalt = 0x452821E638D01377ull;
-/* Threads per tile of the single-pass rank scan; a tile covers
- * kScanThreads * ctx.scan_wpt map words. */
+/* Threads per tile of the single-pass rank scan (ctx.scan_threads: one
+ * wave on small maps); a tile covers ctx.scan_threads * ctx.scan_wpt map
+ * words. */
 constexpr uint32_t kScanThreads = 256;
 
 /* ---------------------------------------------------------------------

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t 
 
 // ---- batched allocation by rank, one pass over the map
 //
-// The map is cut into tiles of kScanThreads * scan_wpt words, one
+// The map is cut into tiles of scan_threads * scan_wpt words, one
 // workgroup per tile (each thread holding scan_wpt consecutive words),
 // tiles numbered in the order the workgroups start (an atomic ticket), so
 // every tile's predecessors are already running.  A tile counts its free
@@ -134,16 +134,17 @@ __device__ inline int32_t rank_fallback(const pbm_ctx& c, uint64_t req, uint32_t
     return id;
 }
 
-template <int WPT>
-__global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_t req_base,
-                                                  int32_t* __restrict__ ids,
-                                                  uint32_t* __restrict__ steps_out,
-                                                  uint32_t budget) {
+// One tile of THREADS threads (the whole workgroup).
+template <int WPT, int THREADS>
+__device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t n, uint64_t req_base,
+                                          int32_t* __restrict__ ids,
+                                          uint32_t* __restrict__ steps_out, uint32_t budget) {
     using namespace pbm;
-    constexpr uint32_t kTileWords = kScanThreads * WPT;
+    constexpr uint32_t kTileWords = THREADS * WPT;
+    constexpr uint32_t kWaves = THREADS / 64;
     __shared__ uint32_t s_tile, s_parity, s_excl, s_grant;
     __shared__ long long s_free;
-    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_wsum[kWaves];
     const unsigned lane = lane_id();
     const unsigned wv = threadIdx.x >> 6;
     if (threadIdx.x == 0) {
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
     __syncthreads();
     uint32_t agg = 0, woff = 0;
 #pragma unroll
-    for (unsigned i = 0; i < 4; ++i) {
+    for (unsigned i = 0; i < kWaves; ++i) {
         if (i < wv) woff += s_wsum[i];
         agg += s_wsum[i];
     }
@@ -329,6 +330,22 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
         }
     }
     flush_stats(c, st);
+}
+
+template <int WPT>
+__global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_t req_base,
+                                                  int32_t* __restrict__ ids,
+                                                  uint32_t* __restrict__ steps_out,
+                                                  uint32_t budget) {
+    rank_tile<WPT, 256>(c, n, req_base, ids, steps_out, budget);
+}
+
+// One-wave tiles, for small maps: their grants spread over more CUs.
+__global__ __launch_bounds__(64) void k_get_rank_wave(pbm_ctx c, uint32_t n, uint64_t req_base,
+                                                      int32_t* __restrict__ ids,
+                                                      uint32_t* __restrict__ steps_out,
+                                                      uint32_t budget) {
+    rank_tile<1, 64>(c, n, req_base, ids, steps_out, budget);
 }
 
 // Persistent batched allocation.  Each wave serves chunks of 64 requests.
@@ -665,11 +682,14 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
         if (L.flags & PBM_LAUNCH_RANK) {
             if (c.scan_wpt == 4)
-                pbmi::launch_timed(h, k_get_rank<4>, dim3(c.n_tiles), dim3(pbm::kScanThreads), s, c,
-                                   n, req_base, d_ids, d_steps, c.walk_budget);
+                pbmi::launch_timed(h, k_get_rank<4>, dim3(c.n_tiles), dim3(256), s, c, n,
+                                   req_base, d_ids, d_steps, c.walk_budget);
+            else if (c.scan_threads == 64)
+                pbmi::launch_timed(h, k_get_rank_wave, dim3(c.n_tiles), dim3(64), s, c, n,
+                                   req_base, d_ids, d_steps, c.walk_budget);
             else
-                pbmi::launch_timed(h, k_get_rank<1>, dim3(c.n_tiles), dim3(pbm::kScanThreads), s, c,
-                                   n, req_base, d_ids, d_steps, c.walk_budget);
+                pbmi::launch_timed(h, k_get_rank<1>, dim3(c.n_tiles), dim3(256), s, c, n,
+                                   req_base, d_ids, d_steps, c.walk_budget);
             PBM_HIP(hipGetLastError());
             return PBM_OK;
         }

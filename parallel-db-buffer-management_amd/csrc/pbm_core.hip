@@ -193,8 +193,11 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     PBM_ALLOC(h->d_scratch, 64u);
     // four words per thread once a map has 8,192 words: fewer, larger
     // tiles shorten dispatch and the look-back chain
+    // rank-scan tiles: 4 words per thread on maps of 2^20 words and more;
+    // one wave per tile on small maps, so their grants spread over more CUs
     const uint32_t scan_wpt = n_words >= (1u << 20) ? 4u : 1u;
-    const uint32_t tile_words = pbm::kScanThreads * scan_wpt;
+    const uint32_t scan_threads = n_words <= PBM_SMALL_SCAN_WORDS ? 64u : pbm::kScanThreads;
+    const uint32_t tile_words = scan_threads * scan_wpt;
     const uint32_t n_tiles = (n_words + tile_words - 1u) / tile_words;
     PBM_ALLOC(h->d_scan, (size_t)n_tiles * 8u);
     PBM_ALLOC(h->d_scan_ctl, 32u);
@@ -231,6 +234,7 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     c.scan_ctl = h->d_scan_ctl;
     c.n_tiles = n_tiles;
     c.scan_wpt = scan_wpt;
+    c.scan_threads = scan_threads;
     c.n_pages = cfg->n_pages;
     c.n_words = n_words;
     c.page_size = cfg->page_size;

@@ -8,6 +8,11 @@
 #include "pbm.h"
 #include "pbm_device.hpp"
 
+// Maps of at most this many words scan with one-wave rank tiles.
+#ifndef PBM_SMALL_SCAN_WORDS
+#define PBM_SMALL_SCAN_WORDS 4096u
+#endif
+
 struct pbm_handle {
     pbm_config cfg;
     pbm_ctx ctx;

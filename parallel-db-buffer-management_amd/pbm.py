@@ -63,7 +63,7 @@ class Ctx(C.Structure):
                 ("walk_budget", C.c_uint32), ("max_threads", C.c_uint32), ("order", C.c_int32),
                 ("seed", C.c_uint64), ("n_tiles", C.c_uint32), ("scan_wpt", C.c_uint32),
                 ("fresh", C.c_void_p), ("carve", C.c_void_p), ("q_avail", C.c_void_p),
-                ("chunk_pages", C.c_uint32), ("reserved_", C.c_uint32)]
+                ("chunk_pages", C.c_uint32), ("scan_threads", C.c_uint32)]
 
 
 class Launch(C.Structure):
