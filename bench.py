@@ -441,7 +441,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the allocator shards by page range with no cross-GPU exchange; see DESIGN.md)",
+              "(the allocator would shard by page range, with no cross-GPU exchange)",
               file=sys.stderr)
         return 2
 
