@@ -619,6 +619,26 @@ def main() -> int:
                "sample": f"{reps} full cycles of the workload (RandomWalkBasic_BM policy, "
                          f"262,144 requests + frees on the same pre-fill), {busy:.1f} s on "
                          f"one of {os.cpu_count()} host cores"}
+        # the same policy on the host cores this process may use (at most
+        # 16), racing on one map with atomic claims: every grant, then every
+        # free, as in the device step
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        progress(f"cpu baseline: {args.cpu_seconds / 2:.0f} s of the policy on {threads} threads")
+        t_end = time.perf_counter() + args.cpu_seconds / 2
+        reps_t, busy_t = 0, 0.0
+        while time.perf_counter() < t_end:
+            st = orc.State(before, T_PAGES, A)
+            t = time.perf_counter()
+            _, _, g_t = st.rwbm_threads(N_REQ, 0, SEED, threads, free_after=True)
+            busy_t += time.perf_counter() - t
+            reps_t += 1
+            if g_t != granted or not np.array_equal(st.bm, before):
+                raise RuntimeError("threaded CPU baseline lost or doubled a page")
+        cpu["threads"] = {
+            "value": ops_per_step * reps_t / busy_t, "unit": "allocs+frees/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{reps_t} full cycles of the same workload on {threads} threads sharing "
+                      f"one map (atomic claims), {busy_t:.1f} s"}
 
     line = {
         "metric": METRIC, "value": value, "unit": "allocs+frees/s", "n_gpus": 1,

@@ -11,6 +11,8 @@
  *   orc_prefill_bernoulli popcount within binomial bounds of f*T
  *   orc_rwbm_serial       hand-computed known answers on tiny pools,
  *                         invariants, mean steps vs the closed form
+ *   orc_rwbm_threads      one thread equals orc_rwbm_serial; several keep
+ *                         grants unique and counted, and frees restore the map
  *   orc_collab_serial     hand-built window known answers (both orders),
  *                         invariants, brute-force set property
  *   orc_batched_serial    same as collab plus phase admission known answer
@@ -19,6 +21,7 @@
 #include "pbm_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -159,6 +162,140 @@ uint32_t orc_rwbm_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, ui
         if (id >= 0) granted++;
         ids[r] = id;
         if (steps) steps[r] = s;
+    }
+    return granted;
+}
+
+/* ------------------------------------------- RW_BM on host threads */
+
+/* claim_lowest with atomic reads and claims (the map is shared). */
+static int32_t claim_lowest_atomic(uint64_t* bm, uint32_t g) {
+    for (;;) {
+        const u64 v = __atomic_load_n(&bm[g], __ATOMIC_RELAXED);
+        if (v == FULL) return -1;
+        const u64 bit = (u64)1 << __builtin_ctzll(~v);
+        const u64 old = __atomic_fetch_or(&bm[g], bit, __ATOMIC_RELAXED);
+        if (!(old & bit)) return (int32_t)(g * 64u + (uint32_t)__builtin_ctzll(bit));
+    }
+}
+
+/* rwbm_walk with atomic claims. */
+static int32_t rwbm_walk_atomic(uint64_t* bm, uint32_t L, int32_t* seed, uint32_t budget,
+                                uint32_t* steps) {
+    uint32_t g = 0;
+    for (uint32_t j = 0; j < budget; ++j) {
+        (*steps)++;
+        *seed = orc_lcg(*seed);
+        g = (uint32_t)*seed % L;
+        int32_t id = claim_lowest_atomic(bm, g);
+        if (id >= 0) return id;
+    }
+    for (u64 k = 0; k < 4ull * L; ++k) {
+        g = (g + 1u == L) ? 0u : g + 1u;
+        (*steps)++;
+        int32_t id = claim_lowest_atomic(bm, g);
+        if (id >= 0) return id;
+    }
+    return -1;
+}
+
+typedef struct {
+    uint64_t* bm;
+    uint32_t n_pages, n, threads, t, budget;
+    int64_t* free_count;
+    uint64_t req_base, seed;
+    int phase;  /* 0: allocate, 1: free this thread's grants */
+    int32_t* ids;
+    uint32_t* steps;
+    uint32_t granted;
+} rwbm_job;
+
+static void* rwbm_thread(void* arg) {
+    rwbm_job* j = (rwbm_job*)arg;
+    const uint32_t L = orc_n_words(j->n_pages);
+    if (j->phase == 1) {
+        int64_t freed = 0;
+        for (uint64_t b0 = (uint64_t)j->t * 64u; b0 < j->n; b0 += (uint64_t)j->threads * 64u) {
+            for (uint32_t k = 0; k < 64u && b0 + k < j->n; ++k) {
+                const int32_t id = j->ids[b0 + k];
+                if (id < 0) continue;
+                const u64 bit = (u64)1 << ((uint32_t)id & 63u);
+                const u64 old =
+                    __atomic_fetch_and(&j->bm[(uint32_t)id >> 6], ~bit, __ATOMIC_RELAXED);
+                freed += (old & bit) != 0;
+            }
+        }
+        __atomic_fetch_add(j->free_count, freed, __ATOMIC_RELAXED);
+        return NULL;
+    }
+    /* blocks of 64 consecutive requests, dealt round-robin; each block
+     * reserves with one atomic, and its first grant-many requests walk (one
+     * thread thus serves the requests in order, as the serial policy) */
+    for (uint64_t b0 = (uint64_t)j->t * 64u; b0 < j->n; b0 += (uint64_t)j->threads * 64u) {
+        const uint32_t want = (uint32_t)(j->n - b0 < 64u ? j->n - b0 : 64u);
+        const int64_t old = __atomic_fetch_sub(j->free_count, (int64_t)want, __ATOMIC_RELAXED);
+        const uint32_t got = old <= 0 ? 0u : (old >= (int64_t)want ? want : (uint32_t)old);
+        if (got < want) __atomic_fetch_add(j->free_count, (int64_t)(want - got), __ATOMIC_RELAXED);
+        for (uint32_t k = 0; k < want; ++k) {
+            const uint32_t r = (uint32_t)b0 + k;
+            uint32_t s = 0;
+            int32_t id = -1;
+            if (k < got) {
+                int32_t sd = orc_request_seed(j->seed, j->req_base + r);
+                id = rwbm_walk_atomic(j->bm, L, &sd, j->budget, &s);
+                if (id < 0) __atomic_fetch_add(j->free_count, 1, __ATOMIC_RELAXED);
+            }
+            if (id >= 0) j->granted++;
+            j->ids[r] = id;
+            if (j->steps) j->steps[r] = s;
+        }
+    }
+    return NULL;
+}
+
+/* Runs one phase on every job: jobs[0] on the calling thread, the rest on
+ * threads of their own (a job whose thread cannot start runs here too). */
+static void rwbm_phase(rwbm_job* jobs, uint32_t threads) {
+    pthread_t tid[256];
+    int up[256];
+    for (uint32_t t = 1; t < threads; ++t)
+        up[t] = pthread_create(&tid[t], NULL, rwbm_thread, &jobs[t]) == 0;
+    rwbm_thread(&jobs[0]);
+    for (uint32_t t = 1; t < threads; ++t) {
+        if (up[t]) pthread_join(tid[t], NULL);
+        else rwbm_thread(&jobs[t]);
+    }
+}
+
+uint32_t orc_rwbm_threads(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                          uint64_t req_base, uint64_t seed, uint32_t budget, uint32_t threads,
+                          int free_after, int32_t* ids, uint32_t* steps) {
+    if (threads == 0) threads = 1;
+    if (threads > 256) threads = 256;
+    rwbm_job jobs[256];
+    uint32_t granted = 0;
+    for (uint32_t t = 0; t < threads; ++t) {
+        rwbm_job* j = &jobs[t];
+        j->bm = bm;
+        j->n_pages = n_pages;
+        j->n = n;
+        j->threads = threads;
+        j->t = t;
+        j->budget = budget;
+        j->free_count = free_count;
+        j->req_base = req_base;
+        j->seed = seed;
+        j->phase = 0;
+        j->ids = ids;
+        j->steps = steps;
+        j->granted = 0;
+    }
+    rwbm_phase(jobs, threads);
+    for (uint32_t t = 0; t < threads; ++t) granted += jobs[t].granted;
+    if (free_after) {
+        /* every grant is made before the first free */
+        for (uint32_t t = 0; t < threads; ++t) jobs[t].phase = 1;
+        rwbm_phase(jobs, threads);
     }
     return granted;
 }

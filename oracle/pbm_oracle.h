@@ -36,6 +36,17 @@ uint32_t orc_rwbm_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, ui
                          uint64_t req_base, uint64_t seed, uint32_t budget,
                          int32_t* ids, uint32_t* steps);
 
+/* The same policy on `threads` host threads sharing the map and the
+ * reservation counter (atomic claims and counts): blocks of 64 consecutive
+ * requests are dealt round-robin, each reserving with one atomic; with
+ * free_after, once every thread is done, each frees its grants.  With one
+ * thread the allocation equals orc_rwbm_serial.  Returns the number of
+ * grants.
+ * A CPU baseline for timing, not a schedule the device reproduces. */
+uint32_t orc_rwbm_threads(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                          uint64_t req_base, uint64_t seed, uint32_t budget, uint32_t threads,
+                          int free_after, int32_t* ids, uint32_t* steps);
+
 /* CollabRW_BM device getPage called by one wave over chunks of 64
  * requests (lanes of the last chunk beyond n inactive). order: 0 packed,
  * 1 reference round-robin. */

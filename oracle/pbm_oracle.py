@@ -33,6 +33,9 @@ _SIGS = {
     "orc_count_free": (C.c_int64, [_PU64, C.c_uint32]),
     "orc_rwbm_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
                                      C.c_uint64, C.c_uint32, _P32, _PU32]),
+    "orc_rwbm_threads": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
+                                      C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, _P32,
+                                      _PU32]),
     "orc_collab_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
                                        C.c_uint64, C.c_uint32, C.c_int, _P32, _PU32]),
     "orc_batched_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
@@ -172,6 +175,16 @@ class State:
         lib().orc_rwbm_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
                               req_base, seed, budget, _p(ids, _P32), _p(steps, _PU32))
         return ids, steps
+
+    def rwbm_threads(self, n: int, req_base: int, seed: int, threads: int,
+                     free_after: bool = False, budget: int = 1 << 20):
+        """The RandomWalkBasic_BM policy on `threads` host threads sharing
+        the map (a CPU baseline; one thread equals rwbm)."""
+        ids, steps = self._out(n)
+        g = lib().orc_rwbm_threads(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                   req_base, seed, budget, threads, int(free_after),
+                                   _p(ids, _P32), _p(steps, _PU32))
+        return ids, steps, int(g)
 
     def rwbasic(self, n: int, req_base: int, seed: int, budget: int = 1 << 20):
         ids, steps = self._out(n)

@@ -196,6 +196,35 @@ def test_rwbm_invariants_and_failures_exact():
     assert st.free[0] == A
 
 
+def test_rwbm_threads_one_thread_is_the_serial_policy():
+    T, n = 1 << 14, 6000
+    before = orc.prefill(T, 0.9, 42)
+    a, b = orc.State(before, T), orc.State(before, T)
+    ids, steps = a.rwbm(n, 7, 42)
+    tids, tsteps, g = b.rwbm_threads(n, 7, 42, 1)
+    assert np.array_equal(ids, tids) and np.array_equal(steps, tsteps)
+    assert np.array_equal(a.bm, b.bm) and a.free[0] == b.free[0]
+    assert g == int((ids >= 0).sum())
+
+
+@pytest.mark.parametrize("threads,fill,n", [(8, 0.9, 20000), (4, 0.5, 3000), (8, 0.99, 5000)])
+def test_rwbm_threads_grants_unique_and_counted(threads, fill, n):
+    # host threads race on one map: every reserved request is granted a
+    # page that was free, no page twice, and the frees restore the map
+    T = 1 << 16
+    before = orc.prefill(T, fill, 3)
+    A = orc.count_free(before, T)
+    st = orc.State(before, T)
+    ids, _, g = st.rwbm_threads(n, 0, 3, threads)
+    rc, granted = orc.check_alloc(before, st.bm, T, ids)
+    assert rc == 0 and granted == g == min(n, A)
+    assert st.free[0] == A - g
+    st2 = orc.State(before, T)
+    ids2, _, g2 = st2.rwbm_threads(n, 0, 3, threads, free_after=True)
+    assert g2 == min(n, A)
+    assert np.array_equal(st2.bm, before) and st2.free[0] == A
+
+
 def _conditional_mean_probes(before: np.ndarray, ids: np.ndarray) -> float:
     """Each jump picks a uniform word, so grant k needs L/NF_k jumps on
     average, NF_k = words not full before grant k (replayed from the ids)."""
