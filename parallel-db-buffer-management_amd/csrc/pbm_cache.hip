@@ -510,6 +510,12 @@ __global__ __launch_bounds__(256) void k_cache_load(CacheDev d) {
 
 __global__ void k_cache_close(CacheDev d) { close_batch(d); }
 
+// First-miss tags start again from batch 1 (the tag field is 32 bits).
+__global__ void k_cache_clear_first(CacheDev d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < d.n_logical) d.first[i] = 0;
+}
+
 __global__ void k_cache_init(CacheDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d.n_logical) {
@@ -550,7 +556,7 @@ struct pbm_cache {
     pbm_cache_config cfg{};
     CacheDev d{};
     std::vector<void*> mem;
-    u64 tag = 0;  // batches launched
+    u64 tag = 0;  // tag of the last batch launched (1 .. 2^32 - 1)
 };
 
 extern "C" int pbm_cache_destroy(pbm_cache* k) {
@@ -618,7 +624,8 @@ extern "C" int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_
         return rc;
     }
     // (miss_tiles <= 2 * sweep_tiles <= capacity + 1)
-    const uint32_t n_init = std::max(std::max(cfg->n_logical, cfg->capacity + 1u), (uint32_t)cCtlWords);
+    const uint32_t n_init =
+        std::max(std::max(cfg->n_logical, cfg->capacity + 1u), (uint32_t)cCtlWords);
     hipLaunchKernelGGL(k_cache_init, dim3((n_init + 255u) / 256u), dim3(256), 0, nullptr, d);
     hipError_t e = hipDeviceSynchronize();
     if (e != hipSuccess) {
@@ -634,6 +641,12 @@ namespace {
 int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
     const CacheDev& d = k->d;
     const dim3 b(256);
+    if (k->tag == kLow) {
+        // the next tag would not fit its 32 bits: every first-miss word is
+        // cleared and tags start again from 1
+        hipLaunchKernelGGL(k_cache_clear_first, dim3((d.n_logical + 255u) / 256u), b, 0, s, d);
+        k->tag = 0;
+    }
     const u64 tag = ++k->tag;
     hipLaunchKernelGGL(k_cache_hits, dim3((nb + 255u) / 256u), b, 0, s, d, refs, nb, tag);
     hipLaunchKernelGGL(k_cache_plan, dim3(d.plan_groups), b, 0, s, d, refs, nb, tag);
