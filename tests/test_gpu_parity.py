@@ -122,7 +122,9 @@ def test_one_wave_batched_budget_sweep_matches_oracle(gpu, T, fill, n, budget, o
 @pytest.mark.parametrize("T,fill,n", [(BIG, 0.9, 262144), (BIG, 0.5, 262144), (BIG, 0.99, 262144),
                                       (65536, 0.0, 32768), (1000, 0.4, 700), (1 << 16, 0.999, 100),
                                       (1 << 22, 0.9, 300000), (1 << 24, 0.97, 200000),
-                                      (64 * 256 * 65 + 17, 0.6, 1 << 20)])
+                                      (64 * 256 * 65 + 17, 0.6, 1 << 20),
+                                      # 2^20 map words and more: four words per thread
+                                      (1 << 26, 0.9, 300000), (1 << 26, 0.999, 100000)])
 @pytest.mark.parametrize("one_wave", [False, True])
 def test_batched_rank_matches_oracle_exactly(gpu, T, fill, n, one_wave):
     # by rank, request r takes the r-th free page whatever the schedule, so
