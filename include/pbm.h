@@ -507,6 +507,7 @@ typedef struct pbm_cache_stats {
     unsigned long long batches;
     unsigned long long alloc_failures;  /* frames the pool could not give (0)     */
     unsigned long long scan_timeouts;   /* look-back waits given up (0)           */
+    unsigned long long invalid_refs;    /* references to ids >= n_logical, skipped */
 } pbm_cache_stats;
 
 typedef struct pbm_cache pbm_cache;
@@ -514,9 +515,10 @@ typedef struct pbm_cache pbm_cache;
 int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_cache** out);
 int pbm_cache_destroy(pbm_cache* k);
 
-/* Runs n references (device array of logical page ids, each below
- * n_logical) through the cache: three kernels per batch, and one that
- * closes the last batch. */
+/* Runs n references (device array of logical page ids) through the cache:
+ * three kernels per batch, and one that closes the last batch.  A
+ * reference to an id outside 0 .. n_logical-1 is neither a hit nor a miss;
+ * it is skipped and counted (invalid_refs). */
 int pbm_cache_run(pbm_cache* k, const int32_t* d_refs, uint64_t n, void* stream);
 
 int pbm_cache_read_stats(pbm_cache* k, pbm_cache_stats* out, void* stream);

@@ -1744,6 +1744,7 @@ void orc_clock_run(uint32_t n_logical, uint32_t capacity, uint32_t batch, const 
         uint32_t K = 0;
         for (uint32_t i = 0; i < nb; ++i) {
             const int32_t l = refs[b0 + i];
+            if ((uint32_t)l >= n_logical) continue;  /* skipped: neither hit nor miss */
             const int32_t s = slot_of[l];
             if (s >= 0) {
                 refbit[s] = 1;

@@ -161,6 +161,7 @@ void orc_zipf_cdf(uint32_t n, double s, double* cdf);
 int32_t orc_zipf_sample(const double* cdf, uint32_t n, uint64_t seed, uint64_t i);
 void orc_zipf_trace(uint32_t n, double s, uint64_t seed, uint64_t first, uint64_t count,
                     int32_t* out);
+/* A reference outside 0 .. n_logical-1 is skipped (neither hit nor miss). */
 void orc_clock_run(uint32_t n_logical, uint32_t capacity, uint32_t batch, const int32_t* refs,
                    uint64_t n_refs, int32_t* slot_of, int32_t* owner, uint8_t* refbit,
                    uint32_t* arm_res, int32_t* evict_log, uint64_t log_cap, uint64_t* stats);

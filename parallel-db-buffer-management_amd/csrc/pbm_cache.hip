@@ -58,6 +58,8 @@ enum : uint32_t {
     cOpen,          // 1 while the last batch waits to be closed
     cScanTimeouts,  // waits given up (never, while tiles run on)
     cPlan,          // (batch tag << 32) | V, published by the batch's last tile
+    cInvalid,       // references to ids >= n_logical, skipped
+    cInvalidBatch,  // the same in the current batch (cleared by its plan)
     cCtlWords = 32
 };
 
@@ -215,7 +217,8 @@ struct TileShared {
     u64 excl, v;
 };
 
-// tag: the batch's number since the cache was made, from 1.
+// tag: the batch's number since the cache was made, from 1.  A reference
+// to an id >= n_logical is skipped and counted.
 __global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* refs, uint32_t nb,
                                                     u64 tag) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -232,7 +235,10 @@ __global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* r
         d.ctl[cV2] = 0;
     }
     const u64 tag_hi = tag << 32;
-    if (i < nb) {
+    bool bad = false;
+    if (i < nb && (uint32_t)refs[i] >= d.n_logical) {
+        bad = true;  // neither a hit nor a miss
+    } else if (i < nb) {
         const int32_t l = refs[i];
         const int32_t s = d.slot_of[l];
         if (s >= 0) {
@@ -242,6 +248,11 @@ __global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* r
             const u64 mine = tag_hi | (kLow - i);
             if (ldc(&d.first[l]) < mine) pbm::atom_max_u64(&d.first[l], mine);
         }
+    }
+    const u64 B = pbm::ballot(bad);
+    if (B && pbm::lane_id() == (unsigned)__ffsll((long long)B) - 1u) {
+        pbm::atom_add_u64(&d.ctl[cInvalid], (u64)__popcll(B));
+        pbm::atom_add_u64(&d.ctl[cInvalidBatch], (u64)__popcll(B));
     }
 }
 
@@ -258,7 +269,7 @@ __device__ void miss_tile(const CacheDev& d, const int32_t* refs, uint32_t nb, u
     for (uint32_t k = 0; k < kPer; ++k) {
         const uint32_t i = i0 + k;
         lv[k] = -1;
-        if (i < nb) {
+        if (i < nb && (uint32_t)refs[i] < d.n_logical) {
             const int32_t l = refs[i];
             // both loads in flight at once; the tag decides only for absent pages
             const int32_t sl = d.slot_of[l];
@@ -284,6 +295,7 @@ __device__ void miss_tile(const CacheDev& d, const int32_t* refs, uint32_t nb, u
     if (tile == d.miss_tiles - 1u && threadIdx.x == 0) {
         const u64 K = sh.excl + agg;
         const u64 r0 = d.ctl[cResident], hits = d.ctl[cHits], misses = d.ctl[cMisses];
+        const u64 bad = d.ctl[cInvalidBatch];
         const u64 empty = d.cap - r0;
         const u64 V = K > empty ? K - empty : 0u;
         d.ctl[cK] = K;
@@ -291,7 +303,8 @@ __device__ void miss_tile(const CacheDev& d, const int32_t* refs, uint32_t nb, u
         d.ctl[cEmpty] = empty;
         d.ctl[cV] = V;
         d.ctl[cMisses] = misses + K;
-        d.ctl[cHits] = hits + (nb - K);
+        d.ctl[cHits] = hits + (nb - K - bad);
+        d.ctl[cInvalidBatch] = 0;
         d.ctl[cOpen] = 1;
         // the ring tiles of this launch read V from here
         stc(&d.ctl[cPlan], ((tag & kLow) << 32) | V);
@@ -703,6 +716,7 @@ extern "C" int pbm_cache_read_stats(pbm_cache* k, pbm_cache_stats* out, void* st
     out->batches = ctl[cBatches];
     out->alloc_failures = ctl[cAllocFail];
     out->scan_timeouts = ctl[cScanTimeouts];
+    out->invalid_refs = ctl[cInvalid];
     return PBM_OK;
 }
 

@@ -105,7 +105,8 @@ class CacheConfig(C.Structure):
 
 class CacheStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in ("hits", "misses", "evictions", "resident", "arm",
-                                          "batches", "alloc_failures", "scan_timeouts")]
+                                          "batches", "alloc_failures", "scan_timeouts",
+                                          "invalid_refs")]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -74,6 +74,25 @@ def test_cache_runs_continue(gpu):
             _check(gpu, pool, cache, cc, T)
 
 
+def test_cache_skips_out_of_range_references(gpu):
+    # ids outside 0 .. n_logical-1 are neither hits nor misses; the rest of
+    # each batch runs as if they were absent
+    T, L, C, B, n = 1 << 16, 1 << 14, 2048, 256, 200000
+    log_case("cache with out-of-range references")
+    refs = _trace(gpu, L, 1.1, 23, n).cpu().numpy()
+    rng = np.random.default_rng(5)
+    bad = rng.choice(n, n // 100, replace=False)
+    refs[bad] = rng.choice(np.array([-1, -7, L, L + 5, 1 << 30], dtype=np.int32), bad.size)
+    cc = orc.ClockCache(L, C, B, log_cap=n)
+    cc.run(refs)
+    with pbm.Pool(T, strategy=pbm.CRW_BM, seed=5) as pool:
+        with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
+            cache.run(gpu.from_numpy(refs).cuda())
+            gpu.cuda.synchronize()
+            assert cache.stats()["invalid_refs"] == bad.size
+            _check(gpu, pool, cache, cc, T)
+
+
 @pytest.mark.parametrize("C,B,n", [(3001, 1111, 150000), (1025, 1025, 150000), (5000, 1, 30000)])
 def test_cache_ragged_tiles(gpu, C, B, n):
     # capacities and batches that end inside a look-back tile of 1,024

@@ -811,6 +811,24 @@ def test_clock_batches_follow_the_batch_rule(C, B, n, seed):
     assert cc.owner[:len(frames)].tolist() == frames
 
 
+def test_clock_skips_out_of_range_references():
+    # with batches of one reference a skipped reference is an empty batch,
+    # so the run equals the textbook CLOCK over the valid references only
+    C, n, L = 16, 600, 50
+    rng = np.random.default_rng(12)
+    refs = rng.integers(0, L, size=n).astype(np.int32)
+    bad = rng.choice(n, 40, replace=False)
+    refs[bad] = rng.choice(np.array([-1, -3, L, L + 9], dtype=np.int32), bad.size)
+    cc = orc.ClockCache(L, C, 1, log_cap=n)
+    cc.run(refs)
+    valid = refs[(refs >= 0) & (refs < L)]
+    hits, ev, hand, frames, bits = _textbook_clock(valid, C)
+    assert int(cc.stats[0]) == hits and int(cc.stats[1]) == valid.size - hits
+    assert cc.evictions.tolist() == ev
+    assert int(cc.arm_res[0]) == hand % C
+    assert cc.owner[:len(frames)].tolist() == frames
+
+
 def test_zipf_trace_matches_its_distribution():
     n, s = 1000, 1.1
     t = orc.zipf_trace(n, s, 7, 0, 200000)
