@@ -1362,9 +1362,14 @@ __device__ inline void* malloc_rw(const pbm_ctx& c, uint64_t size, int32_t seed,
  * pages of every member packed with it, as in the reference. */
 __device__ inline void free_rw(const pbm_ctx& c, void* ptr, Stats* stats = nullptr) {
     if (!ptr) return;
-    int32_t n = *((int32_t*)ptr - 1);
+    // a pointer outside the pool is an error, and its header is not read
+    const int32_t id = page_id(c, (char*)ptr - 4);
+    if (id < 0) {
+        if (stats) stats->errors++;
+        return;
+    }
+    const int32_t n = *((int32_t*)ptr - 1);
     if (n == 0) return;
-    int32_t id = page_id(c, (char*)ptr - 4);
     free_n_pages(c, id, (uint32_t)n, stats);
 }
 
@@ -1471,6 +1476,12 @@ __device__ inline void* kmalloc_collab(const pbm_ctx& c, uint32_t size, int32_t 
 __device__ inline void kfree_collab(const pbm_ctx& c, void* ptr, Stats* stats = nullptr) {
     if (!ptr || !c.pool) return;
     char* hdr = (char*)ptr - 8;
+    // a pointer outside the pool, or a space header pointing outside it, is
+    // an error, and nothing there is read or written
+    if (page_id(c, hdr) < 0) {
+        if (stats) stats->errors++;
+        return;
+    }
     const u64 h = *(const u64*)hdr;
     if (h & kIsChunk) {
         free_n_pages(c, (int32_t)((uint32_t)(h >> 32) & 0x7FFFFFFFu), (uint32_t)(uint16_t)h, stats);
@@ -1478,6 +1489,10 @@ __device__ inline void kfree_collab(const pbm_ctx& c, void* ptr, Stats* stats = 
     }
     const uint32_t root = (uint32_t)h;
     const uint32_t bytes = (uint32_t)(h >> 32) & 0x7FFFFFFFu;
+    if ((size_t)root > (size_t)(hdr - c.pool)) {
+        if (stats) stats->errors++;
+        return;
+    }
     char* chunk = hdr - root;
     const uint32_t old = __hip_atomic_fetch_sub((uint32_t*)chunk, bytes << 16, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);

@@ -162,3 +162,25 @@ def test_kmalloc_rejects_pools_it_cannot_serve(gpu):
     with pbm.Pool(1024, 4096, backing=True) as pool:
         with pytest.raises(pbm.PbmError):
             pbm.KMalloc(pool, 7)
+
+
+def test_frees_of_pointers_outside_the_pool_count_errors(gpu):
+    # freeRW and KernelMallocCollab's free check a pointer against the pool
+    # before they read the header in front of it
+    T, ps = 4096, 4096
+    log_case("frees of pointers outside the pool")
+    with pbm.Pool(T, ps, strategy=pbm.RW_MALLOC, backing=True, seed=2) as pool:
+        base = pool.page_address(0)
+        bad = np.array([8, base - 4096, base + T * ps + 64], dtype=np.uint64)
+        d_bad = gpu.from_numpy(bad.view(np.int64)).cuda()
+        before = pool.snapshot()
+        pool.clear_counters()
+        pool.free_ptrs(3, d_bad)
+        gpu.cuda.synchronize()
+        assert pool.counters()["errors"] == 3
+        with pbm.KMalloc(pool, pbm.KMALLOC_COLLAB) as km:
+            km.free(3, d_bad)
+            gpu.cuda.synchronize()
+        assert pool.counters()["errors"] == 6
+        assert np.array_equal(pool.snapshot(), before)
+        assert pool.free_count() == T
