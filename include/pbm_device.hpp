@@ -1170,20 +1170,43 @@ __device__ inline int32_t get_n_pages(const pbm_ctx& c, uint32_t n, int32_t seed
             unsigned rs = 0;
             unsigned rl = first_zero_run(img, rs);
             u64 H = ballot(rl > 0);
-            if (!H || !ballot(need)) break;
+            const u64 N = ballot(need);
+            if (!H || !N) break;
             bool assigned = false;
             unsigned src = 0, src_start = 0;
-            while (H) {
-                unsigned t = (unsigned)__ffsll((long long)H) - 1u;
-                H &= H - 1ull;
-                unsigned len_t = readlane(rl, t);
-                u64 tk = ballot(need && !assigned && n <= len_t);
-                if (tk) {
-                    unsigned k = (unsigned)__ffsll((long long)tk) - 1u;
-                    if (lane == k) {
-                        assigned = true;
-                        src = t;
-                        src_start = readlane(rs, t);
+            // shortest offered run, bit by bit over the offering lanes
+            unsigned min_len = 0;
+            {
+                u64 cand = H;
+                for (int b = 6; b >= 0; --b) {
+                    const u64 z = ballot(!((rl >> b) & 1u)) & cand;
+                    if (z) cand = z;
+                    else min_len |= 1u << b;
+                }
+            }
+            if (!ballot(need && n > min_len)) {
+                // every needy lane fits every offered run, so the word-by-word
+                // pass below would give the j-th offering word to the j-th
+                // needy lane: ranks give the same assignment at once
+                const unsigned j = rank_in(N);
+                if (need && j < (unsigned)__popcll(H)) {
+                    assigned = true;
+                    src = select_bit(H, j);
+                }
+                src_start = shfl32(rs, assigned ? src : lane);
+            } else {
+                while (H) {
+                    unsigned t = (unsigned)__ffsll((long long)H) - 1u;
+                    H &= H - 1ull;
+                    unsigned len_t = readlane(rl, t);
+                    u64 tk = ballot(need && !assigned && n <= len_t);
+                    if (tk) {
+                        unsigned k = (unsigned)__ffsll((long long)tk) - 1u;
+                        if (lane == k) {
+                            assigned = true;
+                            src = t;
+                            src_start = readlane(rs, t);
+                        }
                     }
                 }
             }
