@@ -1356,14 +1356,16 @@ __device__ inline void* malloc_rw(const pbm_ctx& c, uint64_t size, int32_t seed,
     const bool small = need_bytes < c.page_size;
     const u64 small_mask = ballot(small);
     const unsigned leader = small_mask ? (unsigned)__ffsll((long long)small_mask) - 1u : 0u;
+    // packed offsets: the exclusive sum of the small requests' bytes in
+    // lane order, one ballot per bit (need_bytes < page_size, a multiple of 4)
     uint64_t off = 0, total = 0;
-    u64 m = small_mask;
-    while (m) {
-        unsigned l = (unsigned)__ffsll((long long)m) - 1u;
-        m &= m - 1ull;
-        uint64_t v = readlane64(need_bytes, l);
-        if (lane_id() == l) off = total;
-        total += v;
+    if (small_mask) {
+        const unsigned bits = 64u - (unsigned)__builtin_clzll((u64)c.page_size);
+        for (unsigned b = 2; b < bits; ++b) {
+            const u64 mb = ballot(small && ((need_bytes >> b) & 1u));
+            off += (uint64_t)rank_in(mb) << b;
+            total += (uint64_t)__popcll(mb) << b;
+        }
     }
     uint64_t pages;
     if (small) pages = lane_id() == leader ? (total + c.page_size - 1u) / c.page_size : 0u;
