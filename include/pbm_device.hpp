@@ -1094,16 +1094,15 @@ __device__ __forceinline__ u64 run_mask(unsigned start, unsigned n) {
 /* Takes `want` units per lane from the reservation counter with one atomic
  * per wave; lanes are served in rank order while the running total fits. */
 __device__ inline bool reserve_n(const pbm_ctx& c, unsigned want) {
-    unsigned total = 0, pre = 0;
     u64 act = ballot(want > 0);
     if (!act) return false;
-    u64 m = act;
-    while (m) {
-        unsigned l = (unsigned)__ffsll((long long)m) - 1u;
-        m &= m - 1ull;
-        unsigned v = readlane(want, l);
-        if (lane_id() == l) pre = total;
-        total += v;
+    // the lanes' running total in lane order, one ballot per bit of want
+    unsigned total = 0, pre = 0;
+    for (unsigned b = 0; b < 32u; ++b) {
+        const u64 mb = ballot((want >> b) & 1u);
+        if (!mb && !ballot((want >> b) != 0u)) break;
+        pre += rank_in(mb) << b;
+        total += (unsigned)__popcll(mb) << b;
     }
     unsigned leader = (unsigned)__ffsll((long long)act) - 1u;
     long long old = 0;
@@ -1111,13 +1110,13 @@ __device__ inline bool reserve_n(const pbm_ctx& c, unsigned want) {
     old = (long long)readlane64((u64)old, leader);
     bool ok = want > 0 && old >= (long long)(pre + want);
     // give back what the refused lanes asked for
+    const u64 refused = act & ~ballot(ok);
     unsigned back = 0;
-    const u64 okm = ballot(ok);
-    m = act & ~okm;
-    while (m) {
-        unsigned l = (unsigned)__ffsll((long long)m) - 1u;
-        m &= m - 1ull;
-        back += readlane(want, l);
+    if (refused) {
+        for (unsigned b = 0; b < 32u; ++b) {
+            if (!ballot((want >> b) != 0u)) break;
+            back += (unsigned)__popcll(ballot((want >> b) & 1u) & refused) << b;
+        }
     }
     if (back && lane_id() == leader) atom_add_ll(c.free_count, (long long)back);
     return ok;
@@ -1446,33 +1445,50 @@ __device__ inline void* kmalloc_collab(const pbm_ctx& c, uint32_t size, int32_t 
     const uint32_t cap = kmalloc_group_cap(ps);
     const bool collab = s + 8u < (u64)ps && s <= cap;
     const u64 cm = ballot(collab);
-    // groups in lane order (forward), then suffix sums inside each group
     unsigned leader = lane_id();
-    {
-        u64 m = cm;
-        uint32_t run = 0;
-        unsigned cur = 64;
-        while (m) {
-            const unsigned l = (unsigned)__ffsll((long long)m) - 1u;
-            m &= m - 1ull;
-            const uint32_t v = readlane((uint32_t)s, l);
-            if (cur == 64 || run + v > cap) { cur = l; run = 0; }
-            run += v;
-            if (lane_id() == l) leader = cur;
-        }
-    }
     uint32_t suffix = 0;
-    {
-        u64 m = cm;
-        uint32_t run = 0;
-        unsigned cur = 64;
-        while (m) {
-            const unsigned l = 63u - (unsigned)__clzll((long long)m);
-            m &= ~(1ull << l);
-            const unsigned g = readlane(leader, l);
-            if (g != cur) { cur = g; run = 0; }
-            run += readlane((uint32_t)s, l);
-            if (lane_id() == l) suffix = run;
+    // the collaborative sizes' exclusive sum and total in lane order, one
+    // ballot per bit (s is a multiple of 8 below 2^16 for them)
+    uint32_t pre = 0, total = 0;
+    for (unsigned b = 3; b < 16u; ++b) {
+        const u64 mb = ballot(collab && ((s >> b) & 1u));
+        pre += rank_in(mb) << b;
+        total += (uint32_t)__popcll(mb) << b;
+    }
+    if (cm && total <= cap) {
+        // one group: its lowest lane leads, and a member's suffix sum is
+        // the total less what the lanes below it hold
+        if (collab) {
+            leader = (unsigned)__ffsll((long long)cm) - 1u;
+            suffix = total - pre;
+        }
+    } else {
+        // groups in lane order (forward), then suffix sums inside each group
+        {
+            u64 m = cm;
+            uint32_t run = 0;
+            unsigned cur = 64;
+            while (m) {
+                const unsigned l = (unsigned)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                const uint32_t v = readlane((uint32_t)s, l);
+                if (cur == 64 || run + v > cap) { cur = l; run = 0; }
+                run += v;
+                if (lane_id() == l) leader = cur;
+            }
+        }
+        {
+            u64 m = cm;
+            uint32_t run = 0;
+            unsigned cur = 64;
+            while (m) {
+                const unsigned l = 63u - (unsigned)__clzll((long long)m);
+                m &= ~(1ull << l);
+                const unsigned g = readlane(leader, l);
+                if (g != cur) { cur = g; run = 0; }
+                run += readlane((uint32_t)s, l);
+                if (lane_id() == l) suffix = run;
+            }
         }
     }
     uint32_t n;
