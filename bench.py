@@ -153,41 +153,41 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
     return out
 
 
-def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int, cooperative: bool) -> dict:
+def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int, persistent: bool) -> dict:
     """Bursts of mallocRW/freeRW: 262,144 x 4 KiB pages (backed), empty at
     the start (a page-level random pre-fill leaves no runs for multi-page
     requests); n_bursts bursts with lognormal request counts
     (median 2,048, sigma 1.0), each request a free of a live allocation or
     a mallocRW of log-uniform 16 B .. 64 KiB with probability 1/2.  The
-    autotuner picks the launch on the first tune_bursts bursts; the whole
-    trace is then timed from the same starting state."""
+    autotuner picks the launch (phase kernels or the persistent kernel,
+    waves per CU, walk budget, getNPage window) on the first tune_bursts
+    bursts; the whole trace is then timed from the same starting state,
+    replayed from HIP graphs, and its end state checked against the
+    trace: the live runs disjoint and in use, the map the start plus
+    exactly those runs, no wait given up."""
     T, PS = 1 << 18, 4096
     tr = pbm.burst_trace(n_bursts, 2048.0, 1.0, 16, 65536, PS, 7)
     pool = pbm.Pool(T, PS, strategy=pbm.RW_MALLOC, backing=True, seed=7)
     start = pool.snapshot(stream)
-    used0 = pool.count_used(stream)
     bs = pbm.Bursts(pool, tr)
     progress(f"bursts: trace of {n_bursts} bursts built; autotuning on {tune_bursts}")
-    flags = pbm.TUNE_QUICK | (0 if cooperative else pbm.TUNE_NO_COOPERATIVE)
+    flags = pbm.TUNE_QUICK | (0 if persistent else pbm.TUNE_NO_PERSISTENT)
     tune = bs.autotune(min(tune_bursts, n_bursts), flags, stream)
     progress(f"bursts: tuned {tune['waves_per_cu']} waves/CU, budget {tune['walk_budget']}, "
-             f"cooperative {tune['cooperative']}, {tune['ms']:.2f} ms")
+             f"window {tune['cluster_words']} words, persistent {tune['persistent']}, "
+             f"{tune['ms']:.2f} ms")
     best = tune["launch"]
     pool.restore(start, stream)
     bs.reset(stream)
-    graphs = []
-    if not tune["cooperative"]:
-        for first in range(0, n_bursts, 256):
-            graphs.append(bs.graph(first, min(256, n_bursts - first), best))
+    stride = n_bursts if tune["persistent"] else 256
+    graphs = [bs.graph(first, min(stride, n_bursts - first), best)
+              for first in range(0, n_bursts, stride)]
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(stream)
-    if graphs:
-        for g in graphs:
-            g.launch(stream)
-    else:
-        bs.run(0, n_bursts, best, stream)
+    for g in graphs:
+        g.launch(stream)
     t1.record(stream)
     torch.cuda.synchronize()
     ms = t0.elapsed_time(t1)
@@ -195,28 +195,67 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int, cooperative:
     for g in graphs:
         g.close()
     ptrs = bs.ptrs(stream=stream)
+    timeouts = bs.wait_timeouts(stream)
+    end = pbm.check_burst_end_state(tr, ptrs, pool.page_address(0), PS, start,
+                                    pool.snapshot(stream))
     n_malloc, n_free = int(tr["alloc_offset"][-1]), int(tr["free_offset"][-1])
     n_req = n_malloc + n_free
+    # the same run once more, eager, with device counters and HIP events at
+    # the main kernel's own start and end (the persistent kernel)
+    pool.restore(start, stream)
+    bs.reset(stream)
+    pool.clear_counters(stream)
+    counted = pbm.Launch(best.mode, best.block, best.waves_per_cu, best.walk_budget,
+                         best.cluster_words, best.flags & ~pbm.LAUNCH_NO_COUNTERS)
+    e0, e1 = pbm.KernelEvent(), pbm.KernelEvent()
+    pool.time_next_kernel(e0, e1)
+    bs.run(0, n_bursts, counted, stream)
+    torch.cuda.synchronize()
+    kernel_ms = e0.elapsed_ms(e1) if tune["persistent"] else None
+    e0.close()
+    e1.close()
+    cnt = pool.counters(stream)
+    end2 = pbm.check_burst_end_state(tr, bs.ptrs(stream=stream), pool.page_address(0), PS, start,
+                                     pool.snapshot(stream))
+    timeouts += bs.wait_timeouts(stream)
     # algorithmic bytes: a mallocRW reads its map word (8 B), claims with one
     # read-modify-write (16 B), writes its 4-byte header and its 8-byte
     # pointer; a freeRW reads the pointer and the header and releases with
     # one read-modify-write
     alg = n_malloc * (8 + 16 + 4 + 8) + n_free * (8 + 4 + 16)
-    achieved = alg / (ms * 1e-3) / 1e9
+    dom_ms = kernel_ms if kernel_ms else ms
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic_c4_bursts.json"
+    if tfile.exists() and tune["persistent"]:
+        for k in json.loads(tfile.read_text())["kernels"]:
+            if k["kernel"] == "k_bursts_persistent":
+                traffic = k["hbm_bytes"]
     out = {
         "workload": f"{n_bursts} bursts, lognormal count median 2048 sigma 1.0, 50/50 "
                     "mallocRW (16 B..64 KiB log-uniform) / freeRW, 262,144 x 4 KiB pages "
                     "empty at the start",
-        "requests": n_req, "mallocs": int(tr["alloc_offset"][-1]),
-        "frees": int(tr["free_offset"][-1]), "ms": ms, "requests_per_s": n_req / (ms * 1e-3),
+        "requests": n_req, "mallocs": n_malloc, "frees": n_free, "ms": ms,
+        "us_per_burst": 1e3 * ms / n_bursts, "requests_per_s": n_req / (ms * 1e-3),
+        "kernel_ms": kernel_ms,
         "malloc_failures": int((ptrs == 0).sum()),
-        "launch": {k: tune[k] for k in ("waves_per_cu", "walk_budget", "cooperative")},
+        "launch": {k: tune[k] for k in ("waves_per_cu", "walk_budget", "cluster_words",
+                                        "persistent")},
         "tune_ms_first_bursts": tune["ms"], "tune_candidates": tune["candidates"],
-        "barrier_timeouts": bs.barrier_timeouts(stream),
-        "pages_live_at_end": pool.count_used(stream) - used0,
+        "wait_timeouts": timeouts,
+        "live_runs_at_end": end["live_runs"], "pages_live_at_end": end["live_pages"],
+        # the serial oracle serves every request of this trace
+        "verified": bool(timeouts == 0 and int((ptrs == 0).sum()) == 0
+                         and end["failed_runs"] == 0
+                         and end2["live_pages"] == end["live_pages"]),
+        "counters": cnt,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes": alg, "per_malloc": 36, "per_free": 28},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_bursts_persistent" if kernel_ms else "k_burst_alloc+k_burst_free",
+                     "bytes_per_launch": alg, "launch_ms": dom_ms,
+                     "per_malloc": 36, "per_free": 28,
+                     "atomic": {"achieved_ops": cnt["atomics"] / (dom_ms * 1e-3),
+                                "unit": "u64 RMW/s on the page map"}},
     }
     bs.close()
     pool.close()
@@ -422,8 +461,8 @@ def main() -> int:
     ap.add_argument("--serve-warm", type=int, default=800)
     ap.add_argument("--bursts", type=int, default=4096,
                     help="bursts of the mallocRW/freeRW leg (0 = skip)")
-    ap.add_argument("--bursts-cooperative", type=int, default=1,
-                    help="let the bursts autotuner pick the cooperative persistent kernel "
+    ap.add_argument("--bursts-persistent", type=int, default=1,
+                    help="let the bursts autotuner pick the persistent kernel "
                          "(0 = phase kernels only, replayed from HIP graphs)")
     ap.add_argument("--cache-refs", type=int, default=1 << 26,
                     help="references of the buffer-cache leg (0 = skip)")
@@ -594,7 +633,7 @@ def main() -> int:
 
     serving = (serving_leg(torch, pbm, stream, args.serve_warm, args.serve_steps)
                if args.serve_steps > 0 else None)
-    bursts = (bursts_leg(torch, pbm, stream, args.bursts, 256, bool(args.bursts_cooperative))
+    bursts = (bursts_leg(torch, pbm, stream, args.bursts, 256, bool(args.bursts_persistent))
               if args.bursts > 0 else None)
     cache = (cache_leg(torch, pbm, stream, args.cache_refs, args.cache_batch)
              if args.cache_refs > 0 else None)
@@ -691,6 +730,7 @@ def main() -> int:
     pool.close()
     return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]
                  and all(v["verified"] for v in comparators.values())
+                 and (bursts is None or bursts["verified"])
                  and (kmalloc is None or (kmalloc["collab"]["verified"]
                                           and kmalloc["simple"]["verified"]))
                  and (sweep is None or all(v["verified"] for row in sweep.values()

@@ -138,20 +138,26 @@ typedef struct pbm_launch {
     uint32_t block;         /* threads per workgroup, multiple of 64 (0 = 256)           */
     uint32_t waves_per_cu;  /* batched: persistent waves per CU (0 = 8)                  */
     uint32_t walk_budget;   /* 0 = the pool's walk_budget                                */
-    uint32_t cluster_words; /* batched: words per window, 64 or 128 or 256 (0 = 64)      */
+    uint32_t cluster_words; /* batched: words per window, 64 or 128 or 256 (0 = 64);
+                               mallocRW bursts: getNPage's window (64 = the reference's
+                               one word per lane)                                       */
     uint32_t flags;         /* PBM_LAUNCH_* bits                                         */
 } pbm_launch;
 
 #define PBM_LAUNCH_NO_COUNTERS 1u  /* skip the device counter updates                  */
 #define PBM_LAUNCH_ONE_WAVE    2u  /* batched mode on a single wave: the schedule of the
-                                      serial oracle (orc_batched_serial)                  */
+                                      serial oracle (orc_batched_serial); with
+                                      PBM_LAUNCH_RANK, one workgroup scans every tile in
+                                      order                                              */
 #define PBM_LAUNCH_RANK        4u  /* batched mode by rank, one pass over the map: request r
                                       of the call takes the free page of rank r (map order);
                                       each tile (256 or 1024 words) learns the free pages before it by
                                       decoupled look-back and claims its granted bits with
                                       one atomicOr per word.                               */
-#define PBM_LAUNCH_COOPERATIVE 16u /* mallocRW bursts: every burst of the range in one
-                                      cooperative persistent kernel (batched mode)        */
+#define PBM_LAUNCH_PERSISTENT  16u /* mallocRW bursts: every burst of the range in one
+                                      persistent kernel (batched mode): waves take work
+                                      items in order from a ticket and wait on the
+                                      items they depend on                             */
 
 typedef struct pbm_handle pbm_handle;
 
@@ -197,7 +203,9 @@ int pbm_free_page_percentage(pbm_handle* h, float* out, void* stream);
 
 /* Copies the page map to / from host memory (`words` must equal n_words).
  * restore also resets the reservation counter to the free pages of the
- * restored map.  No reference counterpart (checkpoint/resume). */
+ * restored map, and sets the padding bits past n_pages in the last word
+ * whatever the host copy holds there.  No reference counterpart
+ * (checkpoint/resume). */
 int pbm_snapshot_bitmap(pbm_handle* h, uint64_t* host, size_t words, void* stream);
 int pbm_restore_bitmap(pbm_handle* h, const uint64_t* host, size_t words, void* stream);
 
@@ -544,8 +552,7 @@ int pbm_zipf_trace(uint32_t n_logical, double s, uint64_t seed, uint64_t first, 
 #define PBM_TUNE_QUICK     1u  /* 3 timed cycles per candidate instead of 9            */
 #define PBM_TUNE_WALK_ONLY 2u  /* random-walk homes only (no PBM_LAUNCH_RANK)          */
 #define PBM_TUNE_RANK_ONLY 4u  /* rank mode only                                       */
-#define PBM_TUNE_NO_COOPERATIVE 8u /* bursts: phase kernels only (a cooperative launch
-                                    * cannot be captured in a HIP graph)              */
+#define PBM_TUNE_NO_PERSISTENT 8u  /* bursts: phase kernels only                         */
 
 typedef struct pbm_tune_result {
     pbm_launch get_launch;   /* fastest batched launch for pbm_get_pages              */
@@ -570,16 +577,27 @@ int pbm_autotune(pbm_handle* h, uint32_t n, int32_t* d_ids, uint32_t flags,
  * A trace of bursts with uneven request counts and sizes.  Burst b frees
  * free_index[free_offset[b] .. free_offset[b+1]) (requests of earlier
  * bursts, by global request index) and then mallocRWs requests
- * alloc_offset[b] .. alloc_offset[b+1] of sizes[].  Each phase is one
- * kernel (plain, serial or a persistent batched grid), so a range of
- * bursts can be captured as one HIP graph; with PBM_LAUNCH_COOPERATIVE a
- * range runs in one cooperative persistent kernel whose workgroups meet at
- * a grid barrier between phases.  Requests packed together by mallocRW are
- * the small requests of one 64-request chunk of a burst; freeing the
- * leader returns the whole run (RWMalloc.cuh:329-334), so a trace should
- * free a packed group together.  No reference counterpart: the reference
- * drivers launch one malloc_kernel / free_kernel per test
- * (unitTests/unitTest3.cu:31-35, unitTest4.cu:37-43). */
+ * alloc_offset[b] .. alloc_offset[b+1] of sizes[], wave-collective over
+ * each 64 consecutive requests of the burst.
+ *
+ * Two ways to run a range of bursts, both capturable in a HIP graph:
+ *  - phase kernels: one free kernel and one alloc kernel per burst (plain,
+ *    serial or a batched grid);
+ *  - PBM_LAUNCH_PERSISTENT: one kernel for the whole range.  Each burst is
+ *    cut into free items (at most 64 frees, whole packed runs) and alloc
+ *    items (one 64-request chunk); waves take items in order from a ticket.
+ *    A free waits until the pointer it frees is published; an alloc item
+ *    waits until every free item of its burst is done.  Waits are bounded:
+ *    one that gives up is counted (pbm_bursts_wait_timeouts) and the run's
+ *    results are then not valid.
+ *
+ * Requests packed together by mallocRW are the small requests (size + 4,
+ * rounded up to 4, below the page size) of one 64-request chunk of a
+ * burst; freeing the leader returns the whole run (RWMalloc.cuh:329-334).
+ * pbm_bursts_create therefore rejects a trace that frees a request twice or
+ * frees one packed run's requests in more than one burst.  No reference
+ * counterpart: the reference drivers launch one malloc_kernel / free_kernel
+ * per test (unitTests/unitTest3.cu:31-35, unitTest4.cu:37-43). */
 
 typedef struct pbm_burst_trace {
     uint32_t n_bursts;
@@ -600,28 +618,31 @@ int pbm_bursts_destroy(pbm_bursts* b);
 int pbm_bursts_run(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch* launch,
                    void* stream);
 
-/* Captures bursts first .. first+count-1 as one HIP graph (not cooperative). */
+/* Captures bursts first .. first+count-1 as one HIP graph. */
 int pbm_bursts_graph(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch* launch,
                      pbm_graph** out);
 
 /* Pointers mallocRW returned for requests first .. first+n-1 (0 = failed
- * or not yet run). */
+ * or not yet run; 1 = not yet run inside a persistent run whose waits gave
+ * up). */
 int pbm_bursts_read_ptrs(pbm_bursts* b, uint32_t first, uint32_t n, uint64_t* host,
                          void* stream);
 
-/* Forgets every returned pointer (the pool is not touched). */
+/* Forgets every returned pointer and the wait time-out count (the pool is
+ * not touched). */
 int pbm_bursts_reset(pbm_bursts* b, void* stream);
 
-/* Grid-barrier polls that gave up (stays 0). */
-int pbm_bursts_barrier_timeouts(pbm_bursts* b, unsigned long long* out, void* stream);
+/* Dependency waits of persistent runs that gave up, since the last reset
+ * (stays 0). */
+int pbm_bursts_wait_timeouts(pbm_bursts* b, unsigned long long* out, void* stream);
 
 /* Times the first `count` bursts from the pool's current state over the
- * batched launch space (waves per CU 1..16, walk budget 4/16/64, phase
- * kernels or one cooperative kernel; phase kernels only with
- * PBM_TUNE_NO_COOPERATIVE) and returns the fastest in
- * get_launch (ms_alloc = its time).  The pool's map and the pointer table
- * are put back after every candidate and at the end, so the pool must
- * hold only what the trace's first burst expects. */
+ * batched launch space (waves per CU 1/2/4/8/16, walk budget 4/16/64,
+ * getNPage window 64/128/256 words, phase kernels or one persistent
+ * kernel; phase kernels only with PBM_TUNE_NO_PERSISTENT) and returns the
+ * fastest in get_launch (ms_alloc = its time).  The pool's map and the
+ * pointer table are put back after every candidate and at the end, so the
+ * pool must hold only what the trace's first burst expects. */
 int pbm_bursts_autotune(pbm_bursts* b, uint32_t count, uint32_t flags, pbm_tune_result* out,
                         void* stream);
 

@@ -253,8 +253,12 @@ __device__ inline void release_one(const pbm_ctx& c, bool give) {
 __device__ inline void release_n(const pbm_ctx& c, uint32_t n) {
     const u64 act = ballot(n > 0u);
     if (act == 0) return;
+    // the wave's sum, one ballot per bit of n
     u64 total = 0;
-    for (u64 m = act; m; m &= m - 1ull) total += readlane(n, (unsigned)__ffsll((long long)m) - 1u);
+    for (unsigned b = 0; b < 32u; ++b) {
+        if (!ballot((n >> b) != 0u)) break;
+        total += (u64)__popcll(ballot((n >> b) & 1u)) << b;
+    }
     if (lane_id() == (unsigned)__ffsll((long long)act) - 1u) atom_add_ll(c.free_count, (long long)total);
 }
 
@@ -1123,104 +1127,171 @@ __device__ inline bool reserve_n(const pbm_ctx& c, unsigned want) {
 }
 
 constexpr uint32_t kNPageBudget = 64;
+constexpr uint32_t kMaxWindowSpans = 4;  /* getNPage windows of up to 4 x 64 words */
 
 /* getNPage (RWMalloc.cuh:198-248): wave-collective; lane asks for n
  * consecutive pages inside one word (n <= 64; n == 0 is a coalesced member
  * that needs nothing and gets 0).  Each step reads a window of consecutive
- * words; in rounds, every lane offers the next zero-run of its word, and
- * for each offering lane in lane order the lowest still-needing lane whose
- * n fits takes the run's first n bits.  Takes are decided first and then
- * claimed together; a claim that finds a bit taken gives back the bits it
- * set (the reference leaks them, :236-237) and the lane keeps needing.
- * At most kNPageBudget random windows, then one sweep of the whole map
- * (the reference walks forever).  Returns the first page of the run, -1 on
- * failure. */
-__device__ inline int32_t get_n_pages(const pbm_ctx& c, uint32_t n, int32_t seed,
-                                      uint32_t* steps_out, Stats* stats = nullptr) {
-    Stats local;
-    Stats& st = stats ? *stats : local;
+ * words -- `spans` spans of 64 words, one word per lane per span (1 to 4;
+ * the reference's window is one word per lane) -- and the spans are
+ * searched in order.  In rounds, every lane offers the next zero-run of
+ * its word, and for each offering lane in lane order the lowest
+ * still-needing lane whose n fits takes the run's first n bits.  Takes are
+ * decided first and then claimed together; a claim that finds a bit taken
+ * gives back the bits it set (the reference leaks them, :236-237) and the
+ * lane keeps needing.  At most kNPageBudget random windows, then one sweep
+ * of the whole map (the reference walks forever).  Returns the first page
+ * of the run, -1 on failure. */
+/* The search of getNPage for lanes that already hold their reservation
+ * (need = true; reserve_n took it): returns the first page of the claimed
+ * run, 0 for n == 0, -1 when nothing was found (the reservation is then
+ * given back) or need is false. */
+/* A window of getNPage's map words read ahead of find_n_pages
+ * (wave-collective): lane l holds word start + 64 j + l of span j. */
+struct FirstWindow {
+    uint32_t start;
+    u64 imgs[kMaxWindowSpans];
+};
+
+__device__ inline FirstWindow load_window(const pbm_ctx& c, uint32_t start, uint32_t spans) {
+    FirstWindow w;
+    const uint32_t L = c.n_words;
+    w.start = start % L;
+#pragma unroll
+    for (uint32_t j = 0; j < kMaxWindowSpans; ++j) {
+        const uint32_t off = 64u * j + lane_id();
+        w.imgs[j] = kFull;
+        if (j < spans && off < L) w.imgs[j] = ld_word(&c.bitmap[(uint32_t)(((u64)w.start + off) % L)]);
+    }
+    return w;
+}
+
+/* With `first`, the search starts with that window (a step that takes no
+ * random jump), then jumps from the seed as without it. */
+__device__ inline int32_t find_n_pages(const pbm_ctx& c, uint32_t n, bool need, int32_t seed,
+                                       uint32_t* steps_out, Stats& st, uint32_t spans = 1,
+                                       const FirstWindow* first = nullptr) {
     const uint32_t L = c.n_words;
     const unsigned lane = lane_id();
     seed = (int32_t)__builtin_amdgcn_readfirstlane(seed);
-    const bool too_big = n > 64u;
-    bool need = reserve_n(c, too_big ? 0u : n);
+    spans = spans < 1u ? 1u : (spans > kMaxWindowSpans ? kMaxWindowSpans : spans);
     int32_t out = (n == 0) ? 0 : -1;
     uint32_t steps = 0, jumps = 0, sweeps = 0, start = 0;
     // Free pages need not form a long enough run, so the search is bounded:
     // a few random windows, then one pass over the map, then failure.
     const uint32_t budget = c.walk_budget < kNPageBudget ? c.walk_budget : kNPageBudget;
-    const uint32_t max_sweeps = (L + 63u) / 64u;
+    const uint32_t width = 64u * spans;
+    const uint32_t max_sweeps = (L + width - 1u) / width;
+    bool pre = first != nullptr;  // this step's window was read ahead
     while (ballot(need)) {
-        if (jumps < budget) {
+        if (pre) {
+            start = first->start;
+        } else if (jumps < budget) {
             seed = lcg(seed);
             start = (uint32_t)seed % L;
             ++jumps;
         } else {
             if (sweeps == 0) st.fallbacks++;
-            start = (uint32_t)(((u64)start + 64u) % L);
+            start = (uint32_t)(((u64)start + width) % L);
             if (++sweeps > max_sweeps) break;
         }
         if (need) steps++;
-        const uint32_t word = (uint32_t)(((u64)start + lane) % L);
-        // (a lane past the map's end probes nothing)
-        u64 img = kFull;
-        if (lane < L) { img = ld_word(&c.bitmap[word]); st.probes++; }
-        for (;;) {
-            unsigned rs = 0;
-            unsigned rl = first_zero_run(img, rs);
-            u64 H = ballot(rl > 0);
-            const u64 N = ballot(need);
-            if (!H || !N) break;
-            bool assigned = false;
-            unsigned src = 0, src_start = 0;
-            // shortest offered run, bit by bit over the offering lanes
-            unsigned min_len = 0;
-            {
-                u64 cand = H;
-                for (int b = 6; b >= 0; --b) {
-                    const u64 z = ballot(!((rl >> b) & 1u)) & cand;
-                    if (z) cand = z;
-                    else min_len |= 1u << b;
-                }
+        // every span's word is loaded before the first is searched
+        uint32_t words[kMaxWindowSpans];
+        u64 imgs[kMaxWindowSpans];
+#pragma unroll
+        for (uint32_t j = 0; j < kMaxWindowSpans; ++j) {
+            const uint32_t off = 64u * j + lane;
+            words[j] = (uint32_t)(((u64)start + off) % L);
+            imgs[j] = kFull;
+            // (a lane past the map's end probes nothing)
+            if (j < spans && off < L) {
+                imgs[j] = pre ? first->imgs[j] : ld_word(&c.bitmap[words[j]]);
+                st.probes++;
             }
-            if (!ballot(need && n > min_len)) {
-                // every needy lane fits every offered run, so the word-by-word
-                // pass below would give the j-th offering word to the j-th
-                // needy lane: ranks give the same assignment at once
-                const unsigned j = rank_in(N);
-                if (need && j < (unsigned)__popcll(H)) {
-                    assigned = true;
-                    src = select_bit(H, j);
-                }
-                src_start = shfl32(rs, assigned ? src : lane);
-            } else {
-                while (H) {
-                    unsigned t = (unsigned)__ffsll((long long)H) - 1u;
-                    H &= H - 1ull;
-                    unsigned len_t = readlane(rl, t);
-                    u64 tk = ballot(need && !assigned && n <= len_t);
-                    if (tk) {
-                        unsigned k = (unsigned)__ffsll((long long)tk) - 1u;
-                        if (lane == k) {
-                            assigned = true;
-                            src = t;
-                            src_start = readlane(rs, t);
-                        }
+        }
+        pre = false;
+#pragma unroll
+        for (uint32_t j = 0; j < kMaxWindowSpans; ++j) {
+            if (j >= spans || !ballot(need)) break;
+            const uint32_t word = words[j];
+            u64 img = imgs[j];
+            for (;;) {
+                unsigned rs = 0;
+                const unsigned rl = first_zero_run(img, rs);
+                const u64 N = ballot(need);
+                if (!N) break;
+                // the shortest run a needy lane asks for, bit by bit; an
+                // offered run shorter than that is taken by no lane below,
+                // so it is passed over at once
+                unsigned min_need = 0;
+                {
+                    u64 cand = N;
+                    for (int b = 6; b >= 0; --b) {
+                        const u64 z = ballot(!((n >> b) & 1u)) & cand;
+                        if (z) cand = z;
+                        else min_need |= 1u << b;
                     }
                 }
-            }
-            uint32_t src_word = shfl32(word, assigned ? src : lane);
-            if (assigned) {
-                u64 mask = run_mask(src_start, n);
-                u64 old = atom_or(&c.bitmap[src_word], mask);
-                st.atomics++;
-                u64 owned = mask & ~old;
-                if (owned == mask) {
-                    out = (int32_t)(src_word * 64u + src_start);
-                    need = false;
-                } else if (owned) {
-                    atom_and(&c.bitmap[src_word], ~owned);
+                u64 H = ballot(rl > 0 && rl >= min_need);
+                if (!ballot(rl > 0)) break;
+                if (!H) continue;
+                bool assigned = false;
+                unsigned src = 0;
+                // shortest offered run, bit by bit over the offering lanes
+                unsigned min_len = 0;
+                {
+                    u64 cand = H;
+                    for (int b = 6; b >= 0; --b) {
+                        const u64 z = ballot(!((rl >> b) & 1u)) & cand;
+                        if (z) cand = z;
+                        else min_len |= 1u << b;
+                    }
+                }
+                if (!ballot(need && n > min_len)) {
+                    // every needy lane fits every offered run, so the pass
+                    // below would give the q-th offering word to the q-th
+                    // needy lane: ranks give the same assignment at once
+                    const unsigned q = rank_in(N);
+                    if (need && q < (unsigned)__popcll(H)) {
+                        assigned = true;
+                        src = select_bit(H, q);
+                    }
+                } else {
+                    // offering lanes in lane order; each goes to the lowest
+                    // still-unassigned needy lane it fits.  The masks live in
+                    // scalar registers, and the pass ends once every needy
+                    // lane has a run.
+                    u64 rem = N;
+                    unsigned mine = 64u;
+                    while (H && rem) {
+                        const unsigned t = (unsigned)__ffsll((long long)H) - 1u;
+                        H &= H - 1ull;
+                        const unsigned len_t = readlane(rl, t);
+                        const u64 fit = ballot(n <= len_t) & rem;
+                        if (fit) {
+                            const unsigned k = (unsigned)__ffsll((long long)fit) - 1u;
+                            rem &= ~(1ull << k);
+                            mine = lane == k ? t : mine;
+                        }
+                    }
+                    assigned = mine < 64u;
+                    src = assigned ? mine : 0u;
+                }
+                const unsigned src_start = shfl32(rs, assigned ? src : lane);
+                uint32_t src_word = shfl32(word, assigned ? src : lane);
+                if (assigned) {
+                    u64 mask = run_mask(src_start, n);
+                    u64 old = atom_or(&c.bitmap[src_word], mask);
                     st.atomics++;
+                    u64 owned = mask & ~old;
+                    if (owned == mask) {
+                        out = (int32_t)(src_word * 64u + src_start);
+                        need = false;
+                    } else if (owned) {
+                        atom_and(&c.bitmap[src_word], ~owned);
+                        st.atomics++;
+                    }
                 }
             }
         }
@@ -1232,6 +1303,15 @@ __device__ inline int32_t get_n_pages(const pbm_ctx& c, uint32_t n, int32_t seed
     st.steps += steps;
     if (steps_out) *steps_out = steps;
     return out;
+}
+
+__device__ inline int32_t get_n_pages(const pbm_ctx& c, uint32_t n, int32_t seed,
+                                      uint32_t* steps_out, Stats* stats = nullptr,
+                                      uint32_t spans = 1) {
+    Stats local;
+    Stats& st = stats ? *stats : local;
+    const bool need = reserve_n(c, n > 64u ? 0u : n);
+    return find_n_pages(c, n, need, seed, steps_out, st, spans);
 }
 
 /* Start of the first run of at least x clear bits of w, or -1. */
@@ -1326,7 +1406,9 @@ __device__ inline void free_n_pages(const pbm_ctx& c, int32_t id, uint32_t n,
     Stats local;
     Stats& st = stats ? *stats : local;
     uint32_t freed = 0;
-    if (id >= 0 && (uint32_t)id < c.n_pages && n >= 1 && n <= 64 &&
+    // the run must lie in one word and below n_pages: padding bits past
+    // n_pages stay set, since the allocators hand out any clear bit
+    if (id >= 0 && n >= 1 && n <= 64 && (uint64_t)(uint32_t)id + n <= c.n_pages &&
         ((uint32_t)id & 63u) + n <= 64u) {
         u64 mask = run_mask((uint32_t)id & 63u, n);
         u64 old = atom_and(&c.bitmap[(uint32_t)id >> 6], ~mask);
@@ -1349,36 +1431,61 @@ __device__ inline void free_n_pages(const pbm_ctx& c, int32_t id, uint32_t n,
  * the run's page count (leader and large requests) or 0 (packed members).
  * Needs backing memory; returns nullptr on failure or for runs over 64
  * pages. */
-__device__ inline void* malloc_rw(const pbm_ctx& c, uint64_t size, int32_t seed,
-                                  uint32_t* steps_out, Stats* stats = nullptr) {
+/* What mallocRW asks getNPage for, per lane (wave-collective). */
+struct RunPlan {
+    uint32_t n;       /* pages of the lane's run: 0 for a packed member, 65 = too big */
+    uint64_t off;     /* a packed request's header offset inside the leader's run     */
+    bool small;       /* packed behind the coalescing leader                          */
+    unsigned leader;  /* lowest packed lane                                           */
+    u64 small_mask;
+};
+
+__device__ inline RunPlan plan_malloc_rw(const pbm_ctx& c, uint64_t size) {
+    RunPlan pl;
     const uint64_t need_bytes = (size + 4u + 3u) & ~(uint64_t)3u;
-    const bool small = need_bytes < c.page_size;
-    const u64 small_mask = ballot(small);
-    const unsigned leader = small_mask ? (unsigned)__ffsll((long long)small_mask) - 1u : 0u;
+    pl.small = need_bytes < c.page_size;
+    pl.small_mask = ballot(pl.small);
+    pl.leader = pl.small_mask ? (unsigned)__ffsll((long long)pl.small_mask) - 1u : 0u;
     // packed offsets: the exclusive sum of the small requests' bytes in
     // lane order, one ballot per bit (need_bytes < page_size, a multiple of 4)
     uint64_t off = 0, total = 0;
-    if (small_mask) {
+    if (pl.small_mask) {
         const unsigned bits = 64u - (unsigned)__builtin_clzll((u64)c.page_size);
         for (unsigned b = 2; b < bits; ++b) {
-            const u64 mb = ballot(small && ((need_bytes >> b) & 1u));
+            const u64 mb = ballot(pl.small && ((need_bytes >> b) & 1u));
             off += (uint64_t)rank_in(mb) << b;
             total += (uint64_t)__popcll(mb) << b;
         }
     }
     uint64_t pages;
-    if (small) pages = lane_id() == leader ? (total + c.page_size - 1u) / c.page_size : 0u;
+    if (pl.small) pages = lane_id() == pl.leader ? (total + c.page_size - 1u) / c.page_size : 0u;
     else pages = (need_bytes + c.page_size - 1u) / c.page_size;
-    uint32_t n = pages > 64u ? 65u : (uint32_t)pages;
-    int32_t pid = get_n_pages(c, n, seed, steps_out, stats);
-    if (small_mask) {
-        int32_t lp = (int32_t)readlane((uint32_t)pid, leader);
-        if (small) pid = lp;
+    pl.n = pages > 64u ? 65u : (uint32_t)pages;
+    pl.off = pl.small ? off : 0u;
+    return pl;
+}
+
+/* The pointer and header of a planned request whose run starts at page pid
+ * (the leader's pid for packed requests; wave-collective). */
+__device__ inline void* place_malloc_rw(const pbm_ctx& c, const RunPlan& pl, int32_t pid) {
+    if (pl.small_mask) {
+        int32_t lp = (int32_t)readlane((uint32_t)pid, pl.leader);
+        if (pl.small) pid = lp;
     }
     if (pid < 0 || !c.pool) return nullptr;
-    char* p = c.pool + (size_t)(uint32_t)pid * c.page_size + (small ? off : 0u);
-    *(int32_t*)p = (int32_t)n;
+    char* p = c.pool + (size_t)(uint32_t)pid * c.page_size + pl.off;
+    // device-scope store: a freeRW in another workgroup of the same kernel
+    // reads it once the pointer reaches it
+    __hip_atomic_store((int32_t*)p, (int32_t)pl.n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return p + 4;
+}
+
+__device__ inline void* malloc_rw(const pbm_ctx& c, uint64_t size, int32_t seed,
+                                  uint32_t* steps_out, Stats* stats = nullptr,
+                                  uint32_t spans = 1) {
+    const RunPlan pl = plan_malloc_rw(c, size);
+    const int32_t pid = get_n_pages(c, pl.n, seed, steps_out, stats, spans);
+    return place_malloc_rw(c, pl, pid);
 }
 
 /* freeRW (RWMalloc.cuh:329-334): header 0 (a packed member) frees nothing;
@@ -1392,7 +1499,8 @@ __device__ inline void free_rw(const pbm_ctx& c, void* ptr, Stats* stats = nullp
         if (stats) stats->errors++;
         return;
     }
-    const int32_t n = *((int32_t*)ptr - 1);
+    const int32_t n =
+        __hip_atomic_load((int32_t*)ptr - 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (n == 0) return;
     free_n_pages(c, id, (uint32_t)n, stats);
 }

@@ -1201,17 +1201,21 @@ static u64 run_mask(unsigned start, unsigned n) {
 
 /* getNPage (RWMalloc.cuh:198-248) on one wave over chunks of 64 requests
  * for counts[r] pages.  Reservation: lanes in order while the running sum
- * of their counts fits the free pages.  Per step a window of consecutive
- * words; per round every lane offers the next zero run of its word and,
- * for each offering lane in order, the lowest lane that still needs and
- * whose count fits takes the run's first pages.  At most 64 random
- * windows, then one sweep of the map, then failure. */
-uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
-                             const uint32_t* counts, uint64_t req_base, uint64_t seed,
-                             uint32_t budget, int32_t* ids, uint32_t* steps) {
+ * of their counts fits the free pages.  Per step a window of `spans` x 64
+ * consecutive words (the reference's window is one word per lane, spans =
+ * 1), searched span by span; per round every lane offers the next zero run
+ * of its word in the span and, for each offering lane in order, the lowest
+ * lane that still needs and whose count fits takes the run's first pages.
+ * At most 64 random windows, then one sweep of the map, then failure. */
+uint32_t orc_getnpage_spans(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                            const uint32_t* counts, uint64_t req_base, uint64_t seed,
+                            uint32_t budget, uint32_t spans, int32_t* ids, uint32_t* steps) {
     const uint32_t L = orc_n_words(n_pages);
+    if (spans < 1u) spans = 1u;
+    if (spans > 4u) spans = 4u;
+    const uint32_t width = 64u * spans;
     /* bounded search: at most 64 random windows, then one pass of the map */
-    const uint32_t max_sweeps = (L + 63u) / 64u;
+    const uint32_t max_sweeps = (L + width - 1u) / width;
     if (budget > 64u) budget = 64u;
     uint32_t got = 0;
     for (uint32_t ch = 0; ch * 64u < n; ++ch) {
@@ -1246,34 +1250,39 @@ uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count
                 start = (uint32_t)sd % L;
                 jumps++;
             } else {
-                start = (uint32_t)(((u64)start + 64u) % L);
+                start = (uint32_t)(((u64)start + width) % L);
                 if (++sweeps > max_sweeps) break;
             }
             for (int l = 0; l < 64; ++l)
                 if (need[l]) cst[l]++;
-            u64 img[64];
-            uint32_t word[64];
-            for (int l = 0; l < 64; ++l) {
-                word[l] = (uint32_t)(((u64)start + (uint32_t)l) % L);
-                img[l] = (active[l] && (uint32_t)l < L) ? bm[word[l]] : FULL;
-            }
-            for (;;) {
-                unsigned rs[64], rl[64];
-                int has = 0;
+            /* the spans hold distinct words, so searching them in order on
+             * a serial schedule is the same as reading them all first */
+            for (uint32_t j = 0; j < spans && any(need); ++j) {
+                u64 img[64];
+                uint32_t word[64];
                 for (int l = 0; l < 64; ++l) {
-                    rs[l] = 0;
-                    rl[l] = active[l] ? first_zero_run(&img[l], &rs[l]) : 0;
-                    has |= rl[l] > 0;
+                    const uint32_t off = 64u * j + (uint32_t)l;
+                    word[l] = (uint32_t)(((u64)start + off) % L);
+                    img[l] = (active[l] && off < L) ? bm[word[l]] : FULL;
                 }
-                if (!has || !any(need)) break;
-                for (int t = 0; t < 64; ++t) {
-                    if (!rl[t]) continue;
-                    for (int k = 0; k < 64; ++k) {
-                        if (!need[k] || cnt[k] > rl[t]) continue;
-                        bm[word[t]] |= run_mask(rs[t], cnt[k]);
-                        out[k] = (int32_t)(word[t] * 64u + rs[t]);
-                        need[k] = 0;
-                        break;
+                for (;;) {
+                    unsigned rs[64], rl[64];
+                    int has = 0;
+                    for (int l = 0; l < 64; ++l) {
+                        rs[l] = 0;
+                        rl[l] = active[l] ? first_zero_run(&img[l], &rs[l]) : 0;
+                        has |= rl[l] > 0;
+                    }
+                    if (!has || !any(need)) break;
+                    for (int t = 0; t < 64; ++t) {
+                        if (!rl[t]) continue;
+                        for (int k = 0; k < 64; ++k) {
+                            if (!need[k] || cnt[k] > rl[t]) continue;
+                            bm[word[t]] |= run_mask(rs[t], cnt[k]);
+                            out[k] = (int32_t)(word[t] * 64u + rs[t]);
+                            need[k] = 0;
+                            break;
+                        }
                     }
                 }
             }
@@ -1289,6 +1298,13 @@ uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count
     return got;
 }
 
+uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                             const uint32_t* counts, uint64_t req_base, uint64_t seed,
+                             uint32_t budget, int32_t* ids, uint32_t* steps) {
+    return orc_getnpage_spans(bm, n_pages, free_count, n, counts, req_base, seed, budget, 1u, ids,
+                              steps);
+}
+
 /* releasePages (RWMalloc.cuh:60-70). */
 uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                     const int32_t* ids, const uint32_t* counts) {
@@ -1296,7 +1312,9 @@ uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_
     for (uint32_t r = 0; r < n; ++r) {
         int32_t id = ids[r];
         uint32_t c = counts[r];
-        if (id < 0 || c == 0 || c > 64 || (uint32_t)id >= n_pages || ((uint32_t)id & 63u) + c > 64u)
+        /* a run past n_pages would clear padding bits, which must stay set */
+        if (id < 0 || c == 0 || c > 64 || (uint64_t)(uint32_t)id + c > n_pages ||
+            ((uint32_t)id & 63u) + c > 64u)
             continue;
         u64 m = run_mask((uint32_t)id & 63u, c);
         freed += (uint32_t)__builtin_popcountll(bm[(uint32_t)id >> 6] & m);
@@ -1505,10 +1523,10 @@ uint32_t orc_kmalloc_simple(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
  * (page count asked, offset inside the run, header) and the getNPage
  * grants.  Outputs per request: page (first page of its run, -1 on
  * failure), offset (bytes from the page start to the header) and header. */
-uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
-                           int64_t* free_count, uint32_t n, const uint64_t* sizes,
-                           uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* page,
-                           uint64_t* offset, int32_t* header, uint32_t* steps) {
+uint32_t orc_malloc_spans(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                          int64_t* free_count, uint32_t n, const uint64_t* sizes,
+                          uint64_t req_base, uint64_t seed, uint32_t budget, uint32_t spans,
+                          int32_t* page, uint64_t* offset, int32_t* header, uint32_t* steps) {
     uint32_t got = 0;
     for (uint32_t ch = 0; ch * 64u < n; ++ch) {
         uint32_t m = n - ch * 64u < 64u ? n - ch * 64u : 64u;
@@ -1534,8 +1552,8 @@ uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
         }
         int32_t ids[64];
         uint32_t st[64];
-        orc_getnpage_serial(bm, n_pages, free_count, m, cnt, req_base + (u64)ch * 64u, seed,
-                            budget, ids, st);
+        orc_getnpage_spans(bm, n_pages, free_count, m, cnt, req_base + (u64)ch * 64u, seed,
+                           budget, spans, ids, st);
         for (uint32_t l = 0; l < m; ++l) {
             int32_t pid = small[l] ? ids[leader] : ids[l];
             page[ch * 64u + l] = pid;
@@ -1546,6 +1564,14 @@ uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
         }
     }
     return got;
+}
+
+uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                           int64_t* free_count, uint32_t n, const uint64_t* sizes,
+                           uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* page,
+                           uint64_t* offset, int32_t* header, uint32_t* steps) {
+    return orc_malloc_spans(bm, n_pages, page_size, free_count, n, sizes, req_base, seed, budget,
+                            1u, page, offset, header, steps);
 }
 
 /* ------------------------------------------------------------ free */

@@ -99,6 +99,10 @@ uint32_t orc_rwmalloc_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count
 uint32_t orc_getnpage_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                              const uint32_t* counts, uint64_t req_base, uint64_t seed,
                              uint32_t budget, int32_t* ids, uint32_t* steps);
+/* getNPage with windows of `spans` x 64 words (1..4); _serial is spans = 1. */
+uint32_t orc_getnpage_spans(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
+                            const uint32_t* counts, uint64_t req_base, uint64_t seed,
+                            uint32_t budget, uint32_t spans, int32_t* ids, uint32_t* steps);
 uint32_t orc_free_n(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,
                     const int32_t* ids, const uint32_t* counts);
 /* Ouroboros comparator (Ouroboros.cuh:22-62): pages per chunk, queue
@@ -129,6 +133,10 @@ uint32_t orc_malloc_serial(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
                            int64_t* free_count, uint32_t n, const uint64_t* sizes,
                            uint64_t req_base, uint64_t seed, uint32_t budget, int32_t* page,
                            uint64_t* offset, int32_t* header, uint32_t* steps);
+uint32_t orc_malloc_spans(uint64_t* bm, uint32_t n_pages, uint32_t page_size,
+                          int64_t* free_count, uint32_t n, const uint64_t* sizes,
+                          uint64_t req_base, uint64_t seed, uint32_t budget, uint32_t spans,
+                          int32_t* page, uint64_t* offset, int32_t* header, uint32_t* steps);
 
 /* freePage for every id >= 0; returns pages actually returned (bits that
  * were set).  free_count grows by that number. */

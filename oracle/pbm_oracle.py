@@ -57,10 +57,16 @@ _SIGS = {
                                          C.c_uint64, C.c_uint32, C.c_int, _P32, _PU32]),
     "orc_getnpage_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32,
                                          C.c_uint64, C.c_uint64, C.c_uint32, _P32, _PU32]),
+    "orc_getnpage_spans": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _PU32,
+                                        C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _P32,
+                                        _PU32]),
     "orc_free_n": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, _P32, _PU32]),
     "orc_malloc_serial": (C.c_uint32, [_PU64, C.c_uint32, C.c_uint32, _PI64, C.c_uint32, _PU64,
                                        C.c_uint64, C.c_uint64, C.c_uint32, _P32, _PU64, _P32,
                                        _PU32]),
+    "orc_malloc_spans": (C.c_uint32, [_PU64, C.c_uint32, C.c_uint32, _PI64, C.c_uint32, _PU64,
+                                      C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, _P32,
+                                      _PU64, _P32, _PU32]),
     "orc_check_alloc": (C.c_int, [_PU64, _PU64, C.c_uint32, C.c_uint32, _P32, _PU32]),
     "orc_serve_payload": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
     "orc_rwbasic_serial": (C.c_uint32, [_PU64, C.c_uint32, _PI64, C.c_uint32, C.c_uint64,
@@ -301,13 +307,15 @@ class State:
                                   req_base, seed, budget, order, _p(ids, _P32), _p(steps, _PU32))
         return ids, steps
 
-    def get_n(self, counts: np.ndarray, req_base: int, seed: int, budget: int = 1 << 20):
+    def get_n(self, counts: np.ndarray, req_base: int, seed: int, budget: int = 1 << 20,
+              spans: int = 1):
+        """getNPage; windows of `spans` x 64 words (1 = the reference's)."""
         counts = np.ascontiguousarray(counts, dtype=np.uint32)
         n = counts.size
         ids, steps = self._out(n)
-        lib().orc_getnpage_serial(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
-                                  _p(counts, _PU32), req_base, seed, budget, _p(ids, _P32),
-                                  _p(steps, _PU32))
+        lib().orc_getnpage_spans(_p(self.bm, _PU64), self.n_pages, _p(self.free, _PI64), n,
+                                 _p(counts, _PU32), req_base, seed, budget, spans, _p(ids, _P32),
+                                 _p(steps, _PU32))
         return ids, steps
 
     def get_x(self, counts: np.ndarray, req_base: int, seed: int, budget: int = 1 << 20):
@@ -326,16 +334,17 @@ class State:
                                     ids.size, _p(ids, _P32), _p(counts, _PU32)))
 
     def malloc(self, sizes: np.ndarray, page_size: int, req_base: int, seed: int,
-               budget: int = 1 << 20):
-        """(page, offset, header, steps) per request."""
+               budget: int = 1 << 20, spans: int = 1):
+        """(page, offset, header, steps) per request; getNPage windows of
+        `spans` x 64 words."""
         sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
         n = sizes.size
         page, steps = self._out(n)
         off = np.zeros(n, dtype=np.uint64)
         hdr = np.zeros(n, dtype=np.int32)
-        lib().orc_malloc_serial(_p(self.bm, _PU64), self.n_pages, page_size, _p(self.free, _PI64),
-                                n, _p(sizes, _PU64), req_base, seed, budget, _p(page, _P32),
-                                _p(off, _PU64), _p(hdr, _P32), _p(steps, _PU32))
+        lib().orc_malloc_spans(_p(self.bm, _PU64), self.n_pages, page_size, _p(self.free, _PI64),
+                               n, _p(sizes, _PU64), req_base, seed, budget, spans, _p(page, _P32),
+                               _p(off, _PU64), _p(hdr, _P32), _p(steps, _PU32))
         return page, off, hdr, steps
 
     def free_ids(self, ids: np.ndarray) -> int:

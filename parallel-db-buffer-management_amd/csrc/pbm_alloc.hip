@@ -348,6 +348,21 @@ __global__ __launch_bounds__(64) void k_get_rank_wave(pbm_ctx c, uint32_t n, uin
     rank_tile<1, 64>(c, n, req_base, ids, steps_out, budget);
 }
 
+// The same scan on one workgroup that takes every tile of the call in
+// ticket order (PBM_LAUNCH_ONE_WAVE): each tile's look-back finds its
+// predecessor finished, the fully serial schedule of the same result.
+template <int WPT, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_get_rank_serial(pbm_ctx c, uint32_t n,
+                                                             uint64_t req_base,
+                                                             int32_t* __restrict__ ids,
+                                                             uint32_t* __restrict__ steps_out,
+                                                             uint32_t budget) {
+    for (uint32_t t = 0; t < c.n_tiles; ++t) {
+        rank_tile<WPT, THREADS>(c, n, req_base, ids, steps_out, budget);
+        __syncthreads();
+    }
+}
+
 // Persistent batched allocation.  Each wave serves chunks of 64 requests.
 // Window mode: chunk ch's first window starts at its home word
 // ch*64*L/granted (its share of the grants); after that, up to `budget`
@@ -681,6 +696,19 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
         if (pbmi::is_queue_strategy(strat))
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
         if (L.flags & PBM_LAUNCH_RANK) {
+            if (L.flags & PBM_LAUNCH_ONE_WAVE) {
+                if (c.scan_wpt == 4)
+                    pbmi::launch_timed(h, k_get_rank_serial<4, 256>, dim3(1), dim3(256), s, c, n,
+                                       req_base, d_ids, d_steps, c.walk_budget);
+                else if (c.scan_threads == 64)
+                    pbmi::launch_timed(h, k_get_rank_serial<1, 64>, dim3(1), dim3(64), s, c, n,
+                                       req_base, d_ids, d_steps, c.walk_budget);
+                else
+                    pbmi::launch_timed(h, k_get_rank_serial<1, 256>, dim3(1), dim3(256), s, c, n,
+                                       req_base, d_ids, d_steps, c.walk_budget);
+                PBM_HIP(hipGetLastError());
+                return PBM_OK;
+            }
             if (c.scan_wpt == 4)
                 pbmi::launch_timed(h, k_get_rank<4>, dim3(c.n_tiles), dim3(256), s, c, n,
                                    req_base, d_ids, d_steps, c.walk_budget);

@@ -1,8 +1,38 @@
 // Bursts of mallocRW / freeRW requests: each burst is a free phase (freeRW
 // of earlier bursts' allocations) then an alloc phase (mallocRW of its own
-// requests).  Phases run as separate kernels (capturable in a HIP graph)
-// or all bursts of a range inside one cooperative persistent kernel whose
-// workgroups meet at a grid barrier between phases.
+// requests).  Two ways to run a range of bursts:
+//
+//   phase kernels  one free kernel and one alloc kernel per burst (plain,
+//                  serial or a batched grid), capturable in a HIP graph
+//   persistent     PBM_LAUNCH_PERSISTENT: the whole range in one kernel.
+//                  The host cuts every burst into work items -- free items
+//                  of at most 64 frees, then alloc items of one 64-request
+//                  mallocRW chunk each -- and waves take the items in order
+//                  from an atomic ticket.  Dependencies are waited on where
+//                  they are, not with grid barriers: a free waits until the
+//                  pointer it frees has been published, and an alloc item
+//                  searches for runs only once every free item of its burst
+//                  is done (its plan and reservation are made before).  So
+//                  bursts overlap wherever the trace allows it, and a
+//                  burst's allocations still see every page its frees
+//                  returned.
+//
+// Progress: a wave waits only on items with lower tickets.  A wave takes a
+// ticket only while running and keeps at most two (the item it works on
+// and the next one, fetched ahead), so the lowest unfinished item is always
+// being worked on by a running wave, and its own dependencies are done.
+// Every wait is bounded anyway (2 s of wall clock); a wait that gives up
+// is counted, and the run's results are then not to be trusted.
+//
+// Memory ordering across workgroups (and XCDs, whose L2 caches are not
+// coherent with each other): a pointer is published with a device-scope
+// release store after its header, and a free item loads it and then
+// acquires before reading the header, so the header it reads is the one
+// written for that pointer, never an earlier owner's still held by its
+// XCD's L2.  A free item publishes nothing but device-scope atomic
+// read-modify-writes (map bits, reservation counter); it waits for them to
+// complete before it counts itself done, and the alloc items read those
+// words only with device-scope atomics, so no fence is needed that way.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <vector>
@@ -12,16 +42,40 @@ using pbm::u64;
 
 namespace {
 
-constexpr uint32_t kBarrierSpin = 1u << 20;  // polls of a grid barrier before giving up
+constexpr u64 kWaitTicks = 200000000ull;  // 2 s of the 100 MHz wall clock: a wait gives up
+// A table entry whose mallocRW has not run yet.  mallocRW returns 4-byte
+// aligned pointers (or 0 on failure), so 1 is never a result.
+constexpr u64 kPending = 1ull;
+constexpr uint32_t kMaxBursts = 1u << 24;  // burst numbers fit the item word
 
 struct BurstDev {
-    const uint32_t* alloc_off;  // [n_bursts + 1]
-    const uint32_t* free_off;   // [n_bursts + 1]
-    const uint64_t* sizes;      // [n_allocs]
-    const uint32_t* free_index; // [n_frees]
-    uint64_t* table;            // [n_allocs] pointer returned for each request
-    u64* bar;                   // [0] arrivals, [1] generation, [2] barrier timeouts
+    const uint32_t* alloc_off;   // [n_bursts + 1]
+    const uint32_t* free_off;    // [n_bursts + 1]
+    const uint64_t* sizes;       // [n_allocs]
+    const uint32_t* free_index;  // [n_frees] in trace order (phase kernels)
+    const uint32_t* free_order;  // [n_frees] grouped by packed run (persistent items)
+    const uint2* items;          // persistent work items, burst by burst
+    const uint32_t* n_fitems;    // [n_bursts] free items of each burst
+    uint32_t* fdone;             // [n_bursts] free items finished
+    uint64_t* table;             // [n_allocs] pointer returned for each request
+    u64* ctl;                    // [0] ticket, [1] waits that gave up
 };
+
+// Work item: x = first request (alloc) or first free_order entry (free);
+// y = burst << 8 | count << 1 | is_alloc, count in 1..64.
+__host__ __device__ inline uint32_t item_word(uint32_t burst, uint32_t count, bool alloc) {
+    return (burst << 8) | (count << 1) | (alloc ? 1u : 0u);
+}
+
+__device__ __forceinline__ uint32_t ld_u32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 ld_u64(const uint64_t* p) {
+    return __hip_atomic_load((const u64*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_u64(uint64_t* p, u64 v) {
+    __hip_atomic_store((u64*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Free phase: thread i frees the allocation free_index[i] points at.
 __global__ __launch_bounds__(1024) void k_burst_free(pbm_ctx c, BurstDev t, uint32_t f0,
@@ -37,7 +91,8 @@ __global__ __launch_bounds__(1024) void k_burst_free(pbm_ctx c, BurstDev t, uint
 // requests of the burst; the walk seed of a chunk is the seed of its first
 // request's global index (the plain mallocRW launch's rule).
 __device__ inline void alloc_chunks(const pbm_ctx& c, const BurstDev& t, uint32_t a0, uint32_t a1,
-                                    uint32_t wave, uint32_t n_waves, pbm::Stats& st) {
+                                    uint32_t wave, uint32_t n_waves, uint32_t spans,
+                                    pbm::Stats& st) {
     const unsigned lane = pbm::lane_id();
     const uint32_t n = a1 - a0;
     const uint32_t n_chunks = (n + 63u) / 64u;
@@ -46,71 +101,121 @@ __device__ inline void alloc_chunks(const pbm_ctx& c, const BurstDev& t, uint32_
         if (r < n) {
             void* p = pbm::malloc_rw(c, t.sizes[a0 + r],
                                      pbm::request_seed(c.seed, (u64)a0 + (u64)ch * 64u), nullptr,
-                                     &st);
+                                     &st, spans);
             t.table[a0 + r] = (uint64_t)p;
         }
     }
 }
 
 __global__ __launch_bounds__(1024) void k_burst_alloc(pbm_ctx c, BurstDev t, uint32_t a0,
-                                                      uint32_t a1) {
+                                                      uint32_t a1, uint32_t spans) {
     pbm::Stats st;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
-    alloc_chunks(c, t, a0, a1, wave, n_waves, st);
+    alloc_chunks(c, t, a0, a1, wave, n_waves, spans, st);
     pbm::flush_stats(c, st);
 }
 
-// Grid barrier of a cooperative launch.  The last workgroup to arrive
-// rewinds the arrival count and advances the generation the others poll.
-// The agent-scope fences publish each workgroup's plain stores (table
-// entries, mallocRW headers) before it arrives and make the others' visible
-// after it leaves.  A poll that outlasts kBarrierSpin is counted and the
-// workgroup goes on, and once one has, no later barrier of any launch
-// waits, so the kernel always ends.
-__device__ inline void grid_barrier(u64* bar, uint32_t n_groups) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const u64 gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const u64 arrived = pbm::atom_add_u64(&bar[0], 1ull);
-        if (arrived + 1u == n_groups) {
-            __hip_atomic_store(&bar[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __threadfence();
-            __hip_atomic_store(&bar[1], gen + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            uint32_t spins = 0;
-            while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                if (++spins > kBarrierSpin ||
-                    __hip_atomic_load(&bar[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                    pbm::atom_add_u64(&bar[2], 1ull);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(2);
-            }
-        }
-        __threadfence();
-    }
-    __syncthreads();
-}
-
-// Bursts first .. last-1 in one cooperative launch.
-__global__ __launch_bounds__(256) void k_bursts_coop(pbm_ctx c, BurstDev t, uint32_t first,
-                                                     uint32_t last) {
-    pbm::Stats st;
+// Before a persistent run of bursts b0 .. b1-1: their requests' table
+// entries become pending, their free-item counts zero, the ticket zero.
+__global__ void k_bursts_prepare(BurstDev t, uint32_t a0, uint32_t a1, uint32_t b0,
+                                 uint32_t b1) {
+    const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t n_threads = gridDim.x * blockDim.x;
-    const uint32_t wave = tid >> 6;
-    const uint32_t n_waves = n_threads >> 6;
-    for (uint32_t b = first; b < last; ++b) {
-        const uint32_t f0 = t.free_off[b], f1 = t.free_off[b + 1];
-        for (uint32_t i = f0 + tid; i < f1; i += n_threads)
-            pbm::free_rw(c, (void*)t.table[t.free_index[i]], &st);
-        grid_barrier(t.bar, gridDim.x);
-        alloc_chunks(c, t, t.alloc_off[b], t.alloc_off[b + 1], wave, n_waves, st);
-        grid_barrier(t.bar, gridDim.x);
+    for (uint32_t i = a0 + tid; i < a1; i += stride) t.table[i] = kPending;
+    for (uint32_t b = b0 + tid; b < b1; b += stride) t.fdone[b] = 0u;
+    if (tid == 0) t.ctl[0] = 0ull;
+}
+
+// Polls `ready` until it holds: a short sleep first, longer ones after.
+// Gives up (returns false) after kWaitTicks of the 100 MHz wall clock.
+template <typename Ready>
+__device__ inline bool wait_until(Ready ready) {
+    if (ready()) return true;
+    const u64 t0 = wall_clock64();
+    for (uint32_t spins = 1;; ++spins) {
+        if (spins < 64u) __builtin_amdgcn_s_sleep(1);
+        else __builtin_amdgcn_s_sleep(4);
+        if (ready()) return true;
+        if ((spins & 15u) == 0u && wall_clock64() - t0 > kWaitTicks) return false;
     }
-    pbm::flush_stats(c, st);
+}
+
+// Work items i0 .. i1-1, one wave per workgroup, items taken by ticket.
+__global__ __launch_bounds__(64) void k_bursts_persistent(pbm_ctx c, BurstDev t, uint32_t i0,
+                                                          uint32_t i1, uint32_t spans) {
+    using namespace pbm;
+    Stats st;
+    const unsigned lane = lane_id();
+    uint32_t gave_up = 0;  // wave-uniform
+    u64 tk = 0;
+    if (lane == 0) tk = atom_add_u64(&t.ctl[0], 1ull);
+    u64 it = (u64)i0 + readlane64(tk, 0);
+    while (it < (u64)i1) {
+        // the next ticket is in flight while this item runs
+        u64 nt = 0;
+        if (lane == 0) nt = atom_add_u64(&t.ctl[0], 1ull);
+        const uint2 item = t.items[it];
+        const uint32_t start = item.x;
+        const uint32_t burst = item.y >> 8;
+        const uint32_t cnt = (item.y >> 1) & 127u;
+        if (item.y & 1u) {
+            // alloc item: one mallocRW chunk, planned while the burst's
+            // frees may still run; it reserves and searches once they are
+            // done.  (Reserving before would let the items of bursts far
+            // ahead hold pages the bursts before them need.)  Its first
+            // window is the item's home, item i at word L * frac(i * phi),
+            // so the chunks running together start apart; random jumps from
+            // the chunk's seed follow.  The home window is read while the
+            // reservation is in flight.
+            RunPlan pl{};
+            if (lane < cnt) pl = plan_malloc_rw(c, t.sizes[start + lane]);
+            const uint32_t nf = t.n_fitems[burst];
+            if (!wait_until([&] { return ld_u32(&t.fdone[burst]) >= nf; })) gave_up++;
+            if (lane < cnt) {
+                const uint32_t home =
+                    (uint32_t)(((u64)(uint32_t)((uint32_t)it * 2654435769u) * c.n_words) >> 32);
+                const FirstWindow win = load_window(c, home, spans);
+                const bool need = reserve_n(c, pl.n > 64u ? 0u : pl.n);
+                const int32_t pid = find_n_pages(c, pl.n, need, request_seed(c.seed, start),
+                                                 nullptr, st, spans, &win);
+                void* p = place_malloc_rw(c, pl, pid);
+                // publish: the header written before the pointer is visible
+                // with it (release; the free's load is followed by an acquire)
+                __hip_atomic_store((u64*)&t.table[start + lane], (u64)p, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            // free item: whole packed runs, so every member reads its header
+            // (one load for the wave) before any run of the item is released
+            u64 v = kPending;
+            uint32_t r = 0;
+            if (lane < cnt) {
+                r = t.free_order[start + lane];
+                v = ld_u64(&t.table[r]);
+            }
+            if (ballot(v == kPending && lane < cnt)) {
+                const bool ok = wait_until([&] {
+                    if (lane < cnt && v == kPending) v = ld_u64(&t.table[r]);
+                    return !ballot(lane < cnt && v == kPending);
+                });
+                if (!ok) gave_up++;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            if (lane < cnt && v != kPending) free_rw(c, (void*)v, &st);
+            // every release has landed before the item counts as done.  The
+            // releases are device-scope atomics on the map and the
+            // reservation counter, performed where every workgroup sees
+            // them, and they are all the item publishes to the alloc items:
+            // waiting for them to complete orders them before the count.
+            __builtin_amdgcn_s_waitcnt(0);
+            if (lane == 0) __hip_atomic_fetch_add(&t.fdone[burst], 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        }
+        it = (u64)i0 + readlane64(nt, 0);
+    }
+    if (lane == 0 && gave_up) atom_add_u64(&t.ctl[1], (u64)gave_up);
+    flush_stats(c, st);
 }
 
 __global__ void k_zero_u64(uint64_t* p, uint32_t n) {
@@ -123,10 +228,9 @@ __global__ void k_zero_u64(uint64_t* p, uint32_t n) {
 struct pbm_bursts {
     pbm_handle* h = nullptr;
     uint32_t n_bursts = 0, n_allocs = 0, n_frees = 0;
-    std::vector<uint32_t> alloc_off, free_off;
+    std::vector<uint32_t> alloc_off, free_off, item_off;
     BurstDev d{};
     std::vector<void*> mem;
-    int coop_blocks_per_cu = 0;
 };
 
 extern "C" int pbm_bursts_destroy(pbm_bursts* b) {
@@ -139,6 +243,91 @@ extern "C" int pbm_bursts_destroy(pbm_bursts* b) {
     return PBM_OK;
 }
 
+namespace {
+
+// Work items of every burst, and the frees regrouped so that the requests
+// mallocRW packed into one run (the small requests of one 64-request chunk
+// of a burst) are freed by one item.  Rejects a trace that frees a request
+// twice or frees one packed run over several bursts: freeing the leader
+// returns the whole run, so a member freed later would read a header that
+// the run's next owner may have overwritten.
+int plan_items(const pbm_burst_trace* tr, uint32_t page_size, std::vector<uint2>& items,
+               std::vector<uint32_t>& item_off, std::vector<uint32_t>& n_fitems,
+               std::vector<uint32_t>& free_order) {
+    const uint32_t nb = tr->n_bursts;
+    const uint32_t na = tr->alloc_offset[nb];
+    // chunk start of each request, and whether mallocRW packs it
+    std::vector<uint32_t> chunk_of(na);
+    std::vector<uint8_t> small(na), freed(na, 0);
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t a0 = tr->alloc_offset[b], a1 = tr->alloc_offset[b + 1];
+        for (uint32_t r = a0; r < a1; ++r) {
+            chunk_of[r] = a0 + (r - a0) / 64u * 64u;
+            const uint64_t need = (tr->sizes[r] + 4u + 3u) & ~(uint64_t)3u;
+            small[r] = need < page_size;
+        }
+    }
+    std::vector<uint32_t> run_burst(na, UINT32_MAX);  // by chunk start: burst freeing its run
+    items.clear();
+    item_off.assign(nb + 1, 0);
+    n_fitems.assign(nb, 0);
+    free_order.clear();
+    free_order.reserve(tr->free_offset[nb]);
+    std::vector<std::pair<u64, uint32_t>> keyed;
+    for (uint32_t b = 0; b < nb; ++b) {
+        item_off[b] = (uint32_t)items.size();
+        keyed.clear();
+        for (uint32_t k = tr->free_offset[b]; k < tr->free_offset[b + 1]; ++k) {
+            const uint32_t r = tr->free_index[k];
+            if (freed[r]) return pbmi::fail(PBM_ERR_INVALID, "a request is freed twice");
+            freed[r] = 1;
+            u64 key = r;
+            if (small[r]) {
+                const uint32_t cs = chunk_of[r];
+                if (run_burst[cs] != UINT32_MAX && run_burst[cs] != b)
+                    return pbmi::fail(PBM_ERR_INVALID,
+                                      "the requests of one packed mallocRW run must be freed in "
+                                      "one burst");
+                run_burst[cs] = b;
+                key = (1ull << 63) | cs;
+            }
+            keyed.emplace_back(key, r);
+        }
+        std::stable_sort(keyed.begin(), keyed.end(),
+                         [](const std::pair<u64, uint32_t>& x, const std::pair<u64, uint32_t>& y) {
+                             return x.first < y.first;
+                         });
+        uint32_t item_start = (uint32_t)free_order.size(), item_n = 0;
+        for (size_t i = 0; i < keyed.size();) {
+            size_t j = i;
+            while (j < keyed.size() && keyed[j].first == keyed[i].first) ++j;
+            const uint32_t g = (uint32_t)(j - i);  // at most 64: one chunk's requests
+            if (item_n + g > 64u) {
+                items.push_back(make_uint2(item_start, item_word(b, item_n, false)));
+                n_fitems[b]++;
+                item_start = (uint32_t)free_order.size();
+                item_n = 0;
+            }
+            for (size_t q = i; q < j; ++q) free_order.push_back(keyed[q].second);
+            item_n += g;
+            i = j;
+        }
+        if (item_n) {
+            items.push_back(make_uint2(item_start, item_word(b, item_n, false)));
+            n_fitems[b]++;
+        }
+        const uint32_t a0 = tr->alloc_offset[b], a1 = tr->alloc_offset[b + 1];
+        for (uint32_t s = a0; s < a1; s += 64u) {
+            const uint32_t cnt = a1 - s < 64u ? a1 - s : 64u;
+            items.push_back(make_uint2(s, item_word(b, cnt, true)));
+        }
+    }
+    item_off[nb] = (uint32_t)items.size();
+    return PBM_OK;
+}
+
+}  // namespace
+
 extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_bursts** out) {
     PBM_CHECK_HANDLE(h);
     if (!tr || !out) return pbmi::fail(PBM_ERR_INVALID, "null trace or output");
@@ -149,6 +338,7 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
     if (tr->n_bursts == 0 || !tr->alloc_offset || !tr->free_offset)
         return pbmi::fail(PBM_ERR_INVALID, "a trace needs bursts and both offset arrays");
     const uint32_t nb = tr->n_bursts;
+    if (nb > kMaxBursts) return pbmi::fail(PBM_ERR_RANGE, "at most 2^24 bursts");
     if (tr->alloc_offset[0] != 0 || tr->free_offset[0] != 0)
         return pbmi::fail(PBM_ERR_INVALID, "offsets must start at 0");
     for (uint32_t i = 0; i < nb; ++i)
@@ -161,6 +351,10 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
         for (uint32_t k = tr->free_offset[i]; k < tr->free_offset[i + 1]; ++k)
             if (tr->free_index[k] >= tr->alloc_offset[i])
                 return pbmi::fail(PBM_ERR_INVALID, "a free must target an earlier burst's request");
+    std::vector<uint2> items;
+    std::vector<uint32_t> item_off, n_fitems, free_order;
+    int rc = plan_items(tr, h->ctx.page_size, items, item_off, n_fitems, free_order);
+    if (rc) return rc;
 
     pbm_bursts* b = new pbm_bursts();
     b->h = h;
@@ -169,42 +363,45 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
     b->n_frees = nf;
     b->alloc_off.assign(tr->alloc_offset, tr->alloc_offset + nb + 1);
     b->free_off.assign(tr->free_offset, tr->free_offset + nb + 1);
+    b->item_off = item_off;
     auto alloc = [&](void** p, size_t bytes) -> int {
         hipError_t e = hipMalloc(p, bytes ? bytes : 8u);
         if (e != hipSuccess) return pbmi::hip_fail(e, "hipMalloc(bursts)", __FILE__, __LINE__);
         b->mem.push_back(*p);
         return PBM_OK;
     };
-    void *p0 = nullptr, *p1 = nullptr, *p2 = nullptr, *p3 = nullptr, *p4 = nullptr, *p5 = nullptr;
-    int rc = alloc(&p0, (size_t)(nb + 1) * 4u);
-    if (!rc) rc = alloc(&p1, (size_t)(nb + 1) * 4u);
-    if (!rc) rc = alloc(&p2, (size_t)na * 8u);
-    if (!rc) rc = alloc(&p3, (size_t)nf * 4u);
-    if (!rc) rc = alloc(&p4, (size_t)na * 8u);
-    if (!rc) rc = alloc(&p5, 32u);
+    void* p[10] = {};
+    const size_t bytes[10] = {(size_t)(nb + 1) * 4u, (size_t)(nb + 1) * 4u, (size_t)na * 8u,
+                              (size_t)nf * 4u,       (size_t)nf * 4u,       items.size() * 8u,
+                              (size_t)nb * 4u,       (size_t)nb * 4u,       (size_t)na * 8u,
+                              64u};
+    for (int i = 0; i < 10 && !rc; ++i) rc = alloc(&p[i], bytes[i]);
     if (rc) {
         pbm_bursts_destroy(b);
         return rc;
     }
-    b->d.alloc_off = (const uint32_t*)p0;
-    b->d.free_off = (const uint32_t*)p1;
-    b->d.sizes = (const uint64_t*)p2;
-    b->d.free_index = (const uint32_t*)p3;
-    b->d.table = (uint64_t*)p4;
-    b->d.bar = (u64*)p5;
-    hipError_t e;
-    if ((e = hipMemcpy(p0, tr->alloc_offset, (size_t)(nb + 1) * 4u, hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(p1, tr->free_offset, (size_t)(nb + 1) * 4u, hipMemcpyHostToDevice)) ||
-        (na && (e = hipMemcpy(p2, tr->sizes, (size_t)na * 8u, hipMemcpyHostToDevice))) ||
-        (nf && (e = hipMemcpy(p3, tr->free_index, (size_t)nf * 4u, hipMemcpyHostToDevice))) ||
-        (e = hipMemset(p4, 0, (size_t)na * 8u + 8u * (na == 0))) || (e = hipMemset(p5, 0, 32u))) {
-        pbm_bursts_destroy(b);
-        return pbmi::hip_fail(e, "bursts upload", __FILE__, __LINE__);
-    }
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->coop_blocks_per_cu, k_bursts_coop, 256, 0);
+    b->d.alloc_off = (const uint32_t*)p[0];
+    b->d.free_off = (const uint32_t*)p[1];
+    b->d.sizes = (const uint64_t*)p[2];
+    b->d.free_index = (const uint32_t*)p[3];
+    b->d.free_order = (const uint32_t*)p[4];
+    b->d.items = (const uint2*)p[5];
+    b->d.n_fitems = (const uint32_t*)p[6];
+    b->d.fdone = (uint32_t*)p[7];
+    b->d.table = (uint64_t*)p[8];
+    b->d.ctl = (u64*)p[9];
+    hipError_t e = hipSuccess;
+    const void* src[8] = {tr->alloc_offset, tr->free_offset, tr->sizes, tr->free_index,
+                          free_order.data(), items.data(), n_fitems.data(), nullptr};
+    for (int i = 0; i < 7 && e == hipSuccess; ++i)
+        if (bytes[i] && src[i] && !(i == 2 && !na) && !((i == 3 || i == 4) && !nf))
+            e = hipMemcpy(p[i], src[i], bytes[i], hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(p[7], 0, bytes[7]);
+    if (e == hipSuccess) e = hipMemset(p[8], 0, bytes[8] ? bytes[8] : 8u);
+    if (e == hipSuccess) e = hipMemset(p[9], 0, 64u);
     if (e != hipSuccess) {
         pbm_bursts_destroy(b);
-        return pbmi::hip_fail(e, "hipOccupancyMaxActiveBlocksPerMultiprocessor", __FILE__, __LINE__);
+        return pbmi::hip_fail(e, "bursts upload", __FILE__, __LINE__);
     }
     *out = b;
     return PBM_OK;
@@ -213,7 +410,7 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
 namespace {
 
 int burst_launch(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch* in,
-                 hipStream_t s, bool capturing) {
+                 hipStream_t s) {
     pbm_handle* h = b->h;
     if ((uint64_t)first + count > b->n_bursts) return pbmi::fail(PBM_ERR_RANGE, "burst range beyond the trace");
     pbm_launch L{};
@@ -222,32 +419,48 @@ int burst_launch(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch
     if (L.block % 64 != 0 || L.block > 1024)
         return pbmi::fail(PBM_ERR_INVALID, "launch.block must be a multiple of 64, at most 1024");
     if (L.waves_per_cu == 0) L.waves_per_cu = 8;
+    if (L.cluster_words == 0) L.cluster_words = 64;
+    if (L.cluster_words != 64 && L.cluster_words != 128 && L.cluster_words != 256)
+        return pbmi::fail(PBM_ERR_INVALID, "launch.cluster_words must be 64, 128 or 256");
     if (L.mode < PBM_MODE_PLAIN || L.mode > PBM_MODE_SERIAL)
         return pbmi::fail(PBM_ERR_INVALID, "unknown launch mode");
+    // events armed by pbm_time_next_kernel time the persistent kernel, and
+    // are dropped by a phase-kernel run
+    struct Disarm {
+        pbm_handle* h;
+        ~Disarm() { h->t_start = h->t_stop = nullptr; }
+    } disarm{h};
+    const uint32_t spans = L.cluster_words / 64u;
     pbm_ctx c = h->ctx;
     if (L.walk_budget) c.walk_budget = L.walk_budget;
     if (L.flags & PBM_LAUNCH_NO_COUNTERS) c.counters = nullptr;
     const BurstDev& d = b->d;
-    if (L.flags & PBM_LAUNCH_COOPERATIVE) {
+    if (L.flags & PBM_LAUNCH_PERSISTENT) {
         if (L.mode != PBM_MODE_BATCHED)
-            return pbmi::fail(PBM_ERR_INVALID, "a cooperative launch runs in batched mode");
-        if (capturing)
-            return pbmi::fail(PBM_ERR_UNSUPPORTED, "a cooperative launch is not captured in a graph");
+            return pbmi::fail(PBM_ERR_INVALID, "a persistent launch runs in batched mode");
         if (count == 0) return PBM_OK;
-        uint32_t per_cu = (L.waves_per_cu + 3u) / 4u;
-        if (per_cu > (uint32_t)b->coop_blocks_per_cu) per_cu = (uint32_t)b->coop_blocks_per_cu;
-        if (per_cu == 0) return pbmi::fail(PBM_ERR_UNSUPPORTED, "cooperative kernel does not fit a CU");
         const uint32_t last = first + count;
-        void* args[] = {(void*)&c, (void*)&d, (void*)&first, (void*)&last};
-        PBM_HIP(hipLaunchCooperativeKernel((const void*)k_bursts_coop,
-                                           dim3(per_cu * (uint32_t)h->n_cu), dim3(256), args, 0, s));
+        const uint32_t a0 = b->alloc_off[first], a1 = b->alloc_off[last];
+        const uint32_t i0 = b->item_off[first], i1 = b->item_off[last];
+        uint32_t pb = (a1 - a0 + 255u) / 256u;
+        pb = pb < 1u ? 1u : (pb > 2048u ? 2048u : pb);
+        hipLaunchKernelGGL(k_bursts_prepare, dim3(pb), dim3(256), 0, s, d, a0, a1, first, last);
+        if (i1 > i0) {
+            uint64_t waves = (uint64_t)h->n_cu * L.waves_per_cu;
+            if (waves > i1 - i0) waves = i1 - i0;
+            pbmi::launch_timed(h, k_bursts_persistent, dim3((uint32_t)waves), dim3(64), s, c, d,
+                               i0, i1, spans);
+        }
+        PBM_HIP(hipGetLastError());
         return PBM_OK;
     }
     for (uint32_t k = first; k < first + count; ++k) {
         const uint32_t f0 = b->free_off[k], f1 = b->free_off[k + 1];
         const uint32_t a0 = b->alloc_off[k], a1 = b->alloc_off[k + 1];
         if (f1 > f0) {
-            dim3 grid(1), blk(1);
+            // serial: one wave.  Frees of distinct runs commute, so the map
+            // and the counter after the phase are those of any order.
+            dim3 grid(1), blk(64);
             if (L.mode == PBM_MODE_PLAIN) {
                 blk = dim3(L.block);
                 grid = dim3((f1 - f0 + L.block - 1) / L.block);
@@ -271,7 +484,7 @@ int burst_launch(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch
                 if (waves > chunks) waves = chunks;
                 grid = dim3((uint32_t)((waves * 64u + L.block - 1) / L.block));
             }
-            hipLaunchKernelGGL(k_burst_alloc, grid, blk, 0, s, c, d, a0, a1);
+            hipLaunchKernelGGL(k_burst_alloc, grid, blk, 0, s, c, d, a0, a1, spans);
         }
     }
     PBM_HIP(hipGetLastError());
@@ -284,7 +497,7 @@ extern "C" int pbm_bursts_run(pbm_bursts* b, uint32_t first, uint32_t count,
                               const pbm_launch* launch, void* stream) {
     if (!b) return pbmi::fail(PBM_ERR_INVALID, "null bursts");
     PBM_CHECK_HANDLE(b->h);
-    return burst_launch(b, first, count, launch, pbmi::as_stream(stream), false);
+    return burst_launch(b, first, count, launch, pbmi::as_stream(stream));
 }
 
 extern "C" int pbm_bursts_graph(pbm_bursts* b, uint32_t first, uint32_t count,
@@ -302,7 +515,7 @@ extern "C" int pbm_bursts_graph(pbm_bursts* b, uint32_t first, uint32_t count,
     if (e != hipSuccess) {
         rc = pbmi::hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
     } else {
-        rc = burst_launch(b, first, count, launch, st, true);
+        rc = burst_launch(b, first, count, launch, st);
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(st, &graph);
         g->graph = graph;
@@ -336,21 +549,22 @@ extern "C" int pbm_bursts_read_ptrs(pbm_bursts* b, uint32_t first, uint32_t n, u
 extern "C" int pbm_bursts_reset(pbm_bursts* b, void* stream) {
     if (!b) return pbmi::fail(PBM_ERR_INVALID, "null bursts");
     PBM_CHECK_HANDLE(b->h);
+    hipStream_t s = pbmi::as_stream(stream);
     if (b->n_allocs) {
         uint32_t blocks = (b->n_allocs + 255u) / 256u;
         if (blocks > 4096u) blocks = 4096u;
-        hipLaunchKernelGGL(k_zero_u64, dim3(blocks), dim3(256), 0, pbmi::as_stream(stream),
-                           b->d.table, b->n_allocs);
-        PBM_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_zero_u64, dim3(blocks), dim3(256), 0, s, b->d.table, b->n_allocs);
     }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, s, (uint64_t*)b->d.ctl, 8u);
+    PBM_HIP(hipGetLastError());
     return PBM_OK;
 }
 
-extern "C" int pbm_bursts_barrier_timeouts(pbm_bursts* b, unsigned long long* out, void* stream) {
+extern "C" int pbm_bursts_wait_timeouts(pbm_bursts* b, unsigned long long* out, void* stream) {
     if (!b || !out) return pbmi::fail(PBM_ERR_INVALID, "null bursts or output");
     PBM_CHECK_HANDLE(b->h);
     hipStream_t st = pbmi::as_stream(stream);
-    PBM_HIP(hipMemcpyAsync(out, b->d.bar + 2, 8u, hipMemcpyDeviceToHost, st));
+    PBM_HIP(hipMemcpyAsync(out, b->d.ctl + 1, 8u, hipMemcpyDeviceToHost, st));
     PBM_HIP(hipStreamSynchronize(st));
     return PBM_OK;
 }
@@ -366,34 +580,36 @@ extern "C" int pbm_bursts_autotune(pbm_bursts* b, uint32_t count, uint32_t flags
     std::vector<uint64_t> map(L);
     int rc = pbm_snapshot_bitmap(h, map.data(), L, stream);
     if (rc) return rc;
-    struct Cand { pbm_launch l; };
-    std::vector<Cand> space;
+    std::vector<pbm_launch> space;
     const uint32_t wpcs[] = {1, 2, 4, 8, 16};
     const uint32_t budgets[] = {4, 16, 64};
+    const uint32_t widths[] = {64, 128, 256};
     for (uint32_t wpc : wpcs)
         for (uint32_t bud : budgets)
-            for (int coop = 0; coop < ((flags & PBM_TUNE_NO_COOPERATIVE) ? 1 : 2); ++coop) {
-                pbm_launch l{};
-                l.mode = PBM_MODE_BATCHED;
-                l.waves_per_cu = wpc;
-                l.walk_budget = bud;
-                l.flags = PBM_LAUNCH_NO_COUNTERS | (coop ? PBM_LAUNCH_COOPERATIVE : 0u);
-                space.push_back({l});
-            }
+            for (uint32_t cw : widths)
+                for (int pers = 0; pers < ((flags & PBM_TUNE_NO_PERSISTENT) ? 1 : 2); ++pers) {
+                    pbm_launch l{};
+                    l.mode = PBM_MODE_BATCHED;
+                    l.waves_per_cu = wpc;
+                    l.walk_budget = bud;
+                    l.cluster_words = cw;
+                    l.flags = PBM_LAUNCH_NO_COUNTERS | (pers ? PBM_LAUNCH_PERSISTENT : 0u);
+                    space.push_back(l);
+                }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     PBM_HIP(hipEventCreate(&e0));
     PBM_HIP(hipEventCreate(&e1));
     const int reps = (flags & PBM_TUNE_QUICK) ? 1 : 3;
     float best = 1e30f;
     pbm_tune_result res{};
-    for (const Cand& k : space) {
+    for (const pbm_launch& l : space) {
         std::vector<float> t;
         for (int r = 0; r < reps && !rc; ++r) {
             rc = pbm_restore_bitmap(h, map.data(), L, stream);
             if (!rc) rc = pbm_bursts_reset(b, stream);
             if (rc) break;
             (void)hipEventRecord(e0, s);
-            rc = pbm_bursts_run(b, 0, count, &k.l, stream);
+            rc = pbm_bursts_run(b, 0, count, &l, stream);
             (void)hipEventRecord(e1, s);
             if (hipEventSynchronize(e1) != hipSuccess) rc = pbmi::fail(PBM_ERR_HIP, "bursts autotune run");
             float ms = 0;
@@ -405,8 +621,8 @@ extern "C" int pbm_bursts_autotune(pbm_bursts* b, uint32_t count, uint32_t flags
         const float ms = t[t.size() / 2];
         if (ms < best) {
             best = ms;
-            res.get_launch = k.l;
-            res.free_launch = k.l;
+            res.get_launch = l;
+            res.free_launch = l;
             res.ms_alloc = ms;
         }
     }

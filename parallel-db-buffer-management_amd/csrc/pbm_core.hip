@@ -83,6 +83,15 @@ __global__ void k_count_free(const u64* bm, uint32_t n_words, uint32_t n_pages, 
     if (pbm::lane_id() == 0 && v) pbm::atom_add_u64(out, v);
 }
 
+// Sets the padding bits past n_pages in the last word (a map restored from
+// the host may have them clear; the allocators hand out any clear bit).
+__global__ void k_set_padding(u64* bm, uint32_t n_words, uint32_t n_pages) {
+    if (blockIdx.x != 0 || threadIdx.x != 0 || n_words == 0) return;
+    const uint64_t first = (uint64_t)(n_words - 1u) * 64u;
+    const uint32_t valid = (uint32_t)(n_pages - first);
+    if (valid < 64u) bm[n_words - 1u] |= ~((1ull << valid) - 1ull);
+}
+
 __global__ void k_set_u64(u64* p, uint32_t n, u64 v) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -383,6 +392,9 @@ extern "C" int pbm_restore_bitmap(pbm_handle* h, const uint64_t* host, size_t wo
         return pbmi::fail(PBM_ERR_INVALID, "restore buffer must hold exactly n_words words");
     hipStream_t s = pbmi::as_stream(stream);
     PBM_HIP(hipMemcpyAsync(h->d_bitmap, host, words * 8u, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_set_padding, dim3(1), dim3(64), 0, s, h->d_bitmap, h->ctx.n_words,
+                       h->ctx.n_pages);
+    PBM_HIP(hipGetLastError());
     PBM_HIP(hipStreamSynchronize(s));
     int rc = pbmi::recount_free(h, s);
     if (rc) return rc;

@@ -76,7 +76,7 @@ class TuneResult(C.Structure):
                 ("ms_free", C.c_float), ("candidates", C.c_uint32), ("reserved", C.c_uint32)]
 
 
-TUNE_QUICK, TUNE_WALK_ONLY, TUNE_RANK_ONLY, TUNE_NO_COOPERATIVE = 1, 2, 4, 8
+TUNE_QUICK, TUNE_WALK_ONLY, TUNE_RANK_ONLY, TUNE_NO_PERSISTENT = 1, 2, 4, 8
 
 
 class ServeConfig(C.Structure):
@@ -95,7 +95,7 @@ class ServeStats(C.Structure):
 
 
 SERVE_PAYLOAD, SERVE_LATENCY = 1, 2
-LAUNCH_COOPERATIVE = 16
+LAUNCH_PERSISTENT = 16
 
 
 class CacheConfig(C.Structure):
@@ -215,7 +215,7 @@ _SIGS = {
     "pbm_bursts_read_ptrs": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
                                        C.c_void_p]),
     "pbm_bursts_reset": (C.c_int, [C.c_void_p, C.c_void_p]),
-    "pbm_bursts_barrier_timeouts": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
+    "pbm_bursts_wait_timeouts": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_void_p]),
     "pbm_bursts_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(TuneResult),
                                       C.c_void_p]),
     "pbm_last_error": (C.c_char_p, []),
@@ -737,10 +737,11 @@ class Bursts:
     def reset(self, stream=None) -> None:
         check(lib().pbm_bursts_reset(self.b, _stream(stream)), "pbm_bursts_reset")
 
-    def barrier_timeouts(self, stream=None) -> int:
+    def wait_timeouts(self, stream=None) -> int:
+        """Dependency waits of persistent runs that gave up since the last reset."""
         v = C.c_uint64()
-        check(lib().pbm_bursts_barrier_timeouts(self.b, C.byref(v), _stream(stream)),
-              "pbm_bursts_barrier_timeouts")
+        check(lib().pbm_bursts_wait_timeouts(self.b, C.byref(v), _stream(stream)),
+              "pbm_bursts_wait_timeouts")
         return int(v.value)
 
     def autotune(self, count: int, flags: int = 0, stream=None) -> dict:
@@ -749,7 +750,8 @@ class Bursts:
               "pbm_bursts_autotune")
         g = r.get_launch
         return {"launch": g, "waves_per_cu": int(g.waves_per_cu), "walk_budget": int(g.walk_budget),
-                "cooperative": bool(g.flags & LAUNCH_COOPERATIVE), "ms": float(r.ms_alloc),
+                "cluster_words": int(g.cluster_words) or 64,
+                "persistent": bool(g.flags & LAUNCH_PERSISTENT), "ms": float(r.ms_alloc),
                 "candidates": int(r.candidates)}
 
     def close(self) -> None:
@@ -821,6 +823,73 @@ def burst_trace(n_bursts: int, median: float, sigma: float, min_size: int, max_s
             "free_offset": np.array(free_off, dtype=np.uint32),
             "sizes": np.concatenate(sizes_l) if sizes_l else np.zeros(0, np.uint64),
             "free_index": np.array(frees_l, dtype=np.uint32), "counts": counts}
+
+
+def burst_live_runs(trace: dict, page_size: int) -> dict:
+    """The runs a burst trace leaves allocated, from its requests alone:
+    {request that owns the run: (pages, requests packed into it)}.
+    mallocRW packs the small requests (size + 4, rounded up to 4, below the
+    page size) of each 64-request chunk of a burst into one run owned by
+    the first of them (ceil(bytes / page_size) pages); a larger request
+    owns ceil((size + 4) / page_size) pages.  A run is live while the
+    request that owns it is not freed (freeing a packed member frees
+    nothing)."""
+    a_off, sizes = trace["alloc_offset"], np.asarray(trace["sizes"], dtype=np.uint64)
+    freed = np.zeros(len(sizes), dtype=bool)
+    freed[np.asarray(trace["free_index"], dtype=np.int64)] = True
+    need = (sizes + np.uint64(4 + 3)) & ~np.uint64(3)
+    small = need < np.uint64(page_size)
+    # chunk start of every request
+    burst = np.repeat(np.arange(len(a_off) - 1), np.diff(a_off.astype(np.int64)))
+    idx = np.arange(len(sizes), dtype=np.int64)
+    start = a_off[burst].astype(np.int64)
+    chunk = start + (idx - start) // 64 * 64
+    runs = {}
+    # packed runs: per chunk, the small requests' bytes
+    cs = chunk[small]
+    if cs.size:
+        uniq, first_pos, inv = np.unique(cs, return_index=True, return_inverse=True)
+        tot = np.bincount(inv, weights=need[small].astype(np.float64)).astype(np.int64)
+        members = np.split(idx[small][np.argsort(inv, kind="stable")],
+                           np.cumsum(np.bincount(inv))[:-1])
+        leaders = idx[small][first_pos]
+        for u in np.nonzero(~freed[leaders])[0]:
+            runs[int(leaders[u])] = (int((tot[u] + page_size - 1) // page_size), members[u])
+    big = np.nonzero(~small & ~freed)[0]
+    pages = ((need[big] + np.uint64(page_size - 1)) // np.uint64(page_size)).astype(np.int64)
+    for r, n in zip(big.tolist(), pages.tolist()):
+        runs[r] = (n, np.array([r]))
+    return runs
+
+
+def check_burst_end_state(trace: dict, ptrs: np.ndarray, base: int, page_size: int,
+                          before: np.ndarray, after: np.ndarray) -> dict:
+    """Checks the end state of a burst run against the trace: every live
+    run starts on a page inside the pool, live runs are disjoint, each of
+    their pages is used now and was free in `before`, and the pool holds
+    exactly `before` plus the live runs' pages.  Returns the live runs'
+    leader pointers and page count; raises AssertionError on a violation."""
+    runs = burst_live_runs(trace, page_size)
+    lead = np.array(sorted(runs), dtype=np.int64)
+    n_pages = np.array([runs[k][0] for k in lead], dtype=np.int64)
+    p = ptrs[lead].astype(np.int64) if lead.size else np.zeros(0, np.int64)
+    ok = p > 0
+    start = (p[ok] - 4 - base) // page_size
+    assert ((p[ok] - 4 - base) % page_size == 0).all(), "a live run does not start on a page"
+    cnt = n_pages[ok]
+    used = np.zeros(before.size * 64, dtype=bool)
+    order = np.argsort(start)
+    s, c = start[order], cnt[order]
+    assert (s >= 0).all() and (s + c <= before.size * 64).all(), "a live run leaves the pool"
+    assert (s[1:] >= s[:-1] + c[:-1]).all(), "two live runs overlap"
+    for a, n in zip(s.tolist(), c.tolist()):
+        used[a:a + n] = True
+    bits_b = np.unpackbits(before.view(np.uint8), bitorder="little").astype(bool)
+    bits_a = np.unpackbits(after.view(np.uint8), bitorder="little").astype(bool)
+    assert not (used & bits_b).any(), "a live run holds a page used before the trace"
+    assert np.array_equal(bits_a, bits_b | used), "the map is not the pre-fill plus the live runs"
+    return {"leaders": p[ok].astype(np.uint64), "live_runs": int(ok.sum()),
+            "live_pages": int(c.sum()), "failed_runs": int((~ok).sum())}
 
 
 def serve_trace(n: int, rate_per_s: float, step_s: float, median: float, sigma: float,

@@ -307,6 +307,43 @@ def test_snapshot_restore_round_trip(gpu):
         assert np.array_equal(pool.snapshot(), orc.cleared(T))
 
 
+def test_free_n_pages_run_crossing_n_pages_is_an_error(gpu):
+    # 1,000 pages: the last word holds 40 pages and 24 padding bits, which
+    # must stay set whatever a free asks for
+    T = 1000
+    with pbm.Pool(T, strategy=pbm.RW_MALLOC) as pool:
+        pool.prefill_bernoulli(1.0, 1)
+        before = pool.snapshot()
+        pool.clear_counters()
+        ids = gpu.tensor([995, 990, 1 << 30], dtype=gpu.int32, device="cuda")
+        counts = gpu.tensor([10, 10, 1], dtype=gpu.int32, device="cuda")
+        pool.free_n_pages(1, ids[:1], counts[:1])   # 995..1004 crosses n_pages
+        gpu.cuda.synchronize()
+        assert np.array_equal(pool.snapshot(), before)
+        assert pool.free_count() == 0
+        c = pool.counters()
+        assert c["errors"] == 1 and c["frees"] == 0
+        pool.free_n_pages(1, ids[1:2], counts[1:2])  # 990..999 is the pool's last 10 pages
+        gpu.cuda.synchronize()
+        after = pool.snapshot()
+        assert int(after[-1]) == int(before[-1]) & ~(((1 << 10) - 1) << (990 - 960))
+        assert pool.free_count() == 10 and pool.counters()["frees"] == 10
+        # nothing past n_pages is ever granted afterwards
+        got, _ = _get(gpu, pool, 64, pbm.MODE_PLAIN)
+        assert sorted(got[got >= 0].tolist()) == list(range(990, 1000))
+
+
+def test_restore_sets_the_padding_bits(gpu):
+    T = 1000
+    with pbm.Pool(T) as pool:
+        pool.restore(np.zeros(orc.n_words(T), dtype=np.uint64))  # padding bits clear
+        assert np.array_equal(pool.snapshot(), orc.cleared(T))
+        assert pool.free_count() == T
+        ids, _ = _get(gpu, pool, 1200, pbm.MODE_PLAIN)
+        got = ids[ids >= 0]
+        assert len(got) == T and got.max() < T and len(set(got.tolist())) == T
+
+
 def test_autotune_returns_a_working_launch_and_leaves_the_pool(gpu):
     T, n = 1 << 18, 65536
     log_case("autotune quick")

@@ -610,6 +610,62 @@ def test_getnpage_zero_run_enumeration_brute_force():
     assert np.array_equal(st.bm, before)
 
 
+def test_free_n_rejects_a_run_past_n_pages():
+    # 1,000 pages: the last word's bits 40..63 are padding and stay set
+    T = 1000
+    st = orc.State(orc.prefill(T, 1.0, 1), T, 0)
+    before = st.bm.copy()
+    assert st.free_n(np.array([995]), np.array([10])) == 0
+    assert np.array_equal(st.bm, before) and st.free[0] == 0
+    assert st.free_n(np.array([990]), np.array([10])) == 10
+    assert int(st.bm[-1]) == int(before[-1]) & ~(((1 << 10) - 1) << 30)
+
+
+def test_getnpage_wider_window_takes_a_run_from_its_second_span():
+    # the first jump's window: its first 64 words full, the next 64 words
+    # free.  One span per step must jump again; two spans find the run in
+    # the first step, at the first word of the second span.
+    T, seed, base = 64 * 512, 5, 0
+    L = T // 64
+    start = (orc.lcg(orc.request_seed(seed, base)) & 0xFFFFFFFF) % L
+    bm = orc.cleared(T)
+    for k in range(64):
+        bm[(start + k) % L] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    st1 = orc.State(bm.copy(), T)
+    ids1, steps1 = st1.get_n(np.array([3], dtype=np.uint32), base, seed, spans=1)
+    st2 = orc.State(bm.copy(), T)
+    ids2, steps2 = st2.get_n(np.array([3], dtype=np.uint32), base, seed, spans=2)
+    assert steps1[0] >= 2 and steps2[0] == 1
+    assert ids2[0] == ((start + 64) % L) * 64
+    # spans = 1 is the reference's window: the function it extends
+    st0 = orc.State(bm.copy(), T)
+    ids0, steps0 = st0.get_n(np.array([3], dtype=np.uint32), base, seed)
+    assert ids0[0] == ids1[0] and steps0[0] == steps1[0]
+
+
+@pytest.mark.parametrize("spans", [2, 4])
+def test_getnpage_wide_windows_keep_the_run_invariants(spans):
+    rng = np.random.default_rng(21)
+    T = 1 << 14
+    before = orc.prefill(T, 0.6, 8)
+    counts = rng.integers(1, 9, size=2000).astype(np.uint32)
+    st = orc.State(before, T)
+    ids, _ = st.get_n(counts, 0, 4, spans=spans)
+    seen = np.zeros(T, dtype=bool)
+    for i, c in zip(ids, counts):
+        if i < 0:
+            continue
+        assert i % 64 + c <= 64
+        for q in range(i, i + c):
+            assert not (int(before[q // 64]) >> (q % 64)) & 1 and not seen[q]
+            seen[q] = True
+    # the reservation counter tracks the map: every granted page is counted
+    assert st.free[0] == orc.count_free(st.bm, T)
+    assert (ids >= 0).sum() > 0
+    st.free_n(ids, counts)
+    assert np.array_equal(st.bm, before)
+
+
 def test_malloc_packs_small_requests_behind_the_lowest_lane():
     T, ps = 64 * 64, 4096
     st = orc.State(orc.cleared(T), T)
