@@ -20,6 +20,9 @@ int fail(int code, const std::string& msg) {
 int hip_fail(hipError_t e, const char* what, const char* file, int line) {
     g_last_error = std::string("HIP error '") + hipGetErrorString(e) + "' in " + what + " at " +
                    file + ":" + std::to_string(line);
+    // the failure is reported by the return code: take it off the thread's
+    // HIP error state, or the caller's next HIP call would report it again
+    (void)hipGetLastError();
     return e == hipErrorOutOfMemory ? PBM_ERR_OOM : PBM_ERR_HIP;
 }
 

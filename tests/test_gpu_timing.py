@@ -44,5 +44,8 @@ def test_every_call_disarms_the_events(gpu):
         gpu.cuda.synchronize()
         with pytest.raises(pbm.PbmError):      # never recorded
             a.elapsed_ms(b)
+        # the failed HIP call is reported once, by the library: the caller's
+        # next HIP work runs clean
+        assert int(gpu.full((8,), 3, dtype=gpu.int32, device="cuda").sum().item()) == 24
         a.close()
         b.close()
