@@ -547,6 +547,18 @@ int pbm_cache_read_slots(pbm_cache* k, int32_t* owner, int32_t* frame, uint32_t*
 int pbm_zipf_trace(uint32_t n_logical, double s, uint64_t seed, uint64_t first, uint64_t count,
                    int32_t* d_out, int device, void* stream);
 
+/* pbm_zipf_trace with no device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  The CDF goes
+ * to a static device array of 4,194,304 doubles (32 MiB, one per device,
+ * held until the process ends) instead of a hipMalloc'd table: n_logical
+ * above 4,194,304 is rejected with PBM_ERR_RANGE.  The output is the same
+ * as pbm_zipf_trace's.  Calls that overlap on one device race on the array
+ * (calls from host threads are serialized; the call returns after its
+ * kernel). */
+int pbm_zipf_trace_noalloc(uint32_t n_logical, double s, uint64_t seed, uint64_t first,
+                           uint64_t count, int32_t* d_out, int device, void* stream);
+
 /* ---- autotuning ------------------------------------------------------- */
 
 #define PBM_TUNE_QUICK     1u  /* 3 timed cycles per candidate instead of 9            */

@@ -28,6 +28,7 @@
 // With a batch of one reference this is the textbook second-chance CLOCK.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <vector>
 #include "pbm_internal.hpp"
@@ -742,6 +743,77 @@ extern "C" int pbm_cache_read_slots(pbm_cache* k, int32_t* owner, int32_t* frame
     if (frame) PBM_HIP(hipMemcpyAsync(frame, k->d.frame, bytes, hipMemcpyDeviceToHost, s));
     if (refbit) PBM_HIP(hipMemcpyAsync(refbit, k->d.refbit, bytes, hipMemcpyDeviceToHost, s));
     PBM_HIP(hipStreamSynchronize(s));
+    return PBM_OK;
+}
+
+// ---- Zipf trace without allocation: the CDF lives in a static scratch
+// array, one copy per device (32 MiB: up to 2^22 logical pages), which each
+// device keeps until the process ends.  In host code the array's name is
+// not a device address: every use goes through the address
+// hipGetSymbolAddress returned for the device (zipf_scratch_ptr).
+constexpr uint32_t kZipfScratchPages = 1u << 22;
+constexpr uint32_t kZipfStage = 1u << 15;  // CDF entries uploaded per copy
+constexpr int kZipfMaxDevices = 64;
+static __device__ __align__(256) double zipf_scratch[kZipfScratchPages];
+
+namespace {
+
+std::mutex g_zipf_mu;                        // guards the two arrays below
+double* g_zipf_ptr[kZipfMaxDevices] = {};    // zipf_scratch of each device
+double g_zipf_stage[kZipfStage];             // host staging of the CDF
+
+// The device address of zipf_scratch on `device` (current), looked up once.
+int zipf_scratch_ptr(int device, double** out) {
+    if (!g_zipf_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(zipf_scratch));
+        if (e != hipSuccess || !p) return pbmi::hip_fail(e, "hipGetSymbolAddress(zipf_scratch)",
+                                                         __FILE__, __LINE__);
+        g_zipf_ptr[device] = (double*)p;
+    }
+    *out = g_zipf_ptr[device];
+    return PBM_OK;
+}
+
+}  // namespace
+
+extern "C" int pbm_zipf_trace_noalloc(uint32_t n_logical, double s, uint64_t seed,
+                                      uint64_t first, uint64_t count, int32_t* d_out, int device,
+                                      void* stream) {
+    if (n_logical == 0 || !(s > 0.0)) return pbmi::fail(PBM_ERR_INVALID, "need n_logical > 0, s > 0");
+    if (count && !d_out) return pbmi::fail(PBM_ERR_INVALID, "null output");
+    if (n_logical > kZipfScratchPages)
+        return pbmi::fail(PBM_ERR_RANGE, "pbm_zipf_trace_noalloc holds the CDF of at most "
+                                         "4,194,304 logical pages in its static scratch");
+    if (device < 0 || device >= kZipfMaxDevices)
+        return pbmi::fail(PBM_ERR_INVALID, "device ordinal out of range");
+    if (!count) return PBM_OK;
+    PBM_HIP(hipSetDevice(device));
+    hipStream_t st = pbmi::as_stream(stream);
+    std::lock_guard<std::mutex> lock(g_zipf_mu);
+    double* cdf = nullptr;
+    int rc = zipf_scratch_ptr(device, &cdf);
+    if (rc) return rc;
+    // the same running sum as pbm_zipf_trace, uploaded a stage at a time;
+    // every entry the kernel reads (0 .. n_logical-1) is written here, so
+    // what an earlier call left behind is never read
+    double acc = 0;
+    for (uint32_t k0 = 0; k0 < n_logical; k0 += kZipfStage) {
+        const uint32_t m = n_logical - k0 < kZipfStage ? n_logical - k0 : kZipfStage;
+        for (uint32_t i = 0; i < m; ++i) {
+            acc += std::pow((double)(k0 + i) + 1.0, -s);
+            g_zipf_stage[i] = acc;
+        }
+        PBM_HIP(hipMemcpyAsync(cdf + k0, g_zipf_stage, (size_t)m * 8u, hipMemcpyHostToDevice, st));
+        PBM_HIP(hipStreamSynchronize(st));  // the stage is refilled next
+    }
+    const u64 blocks = (count + 255u) / 256u;
+    hipLaunchKernelGGL(k_zipf, dim3((uint32_t)blocks), dim3(256), 0, st, cdf, n_logical, seed,
+                       first, count, d_out);
+    PBM_HIP(hipGetLastError());
+    // the next call on this device rewrites the CDF: it must not overlap
+    // this kernel
+    PBM_HIP(hipStreamSynchronize(st));
     return PBM_OK;
 }
 

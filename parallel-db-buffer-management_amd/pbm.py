@@ -206,6 +206,8 @@ _SIGS = {
                                        C.c_void_p]),
     "pbm_zipf_trace": (C.c_int, [C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64,
                                  C.c_void_p, C.c_int, C.c_void_p]),
+    "pbm_zipf_trace_noalloc": (C.c_int, [C.c_uint32, C.c_double, C.c_uint64, C.c_uint64,
+                                         C.c_uint64, C.c_void_p, C.c_int, C.c_void_p]),
     "pbm_bursts_create": (C.c_int, [C.c_void_p, C.POINTER(BurstTrace), C.POINTER(C.c_void_p)]),
     "pbm_bursts_destroy": (C.c_int, [C.c_void_p]),
     "pbm_bursts_run": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(Launch),
@@ -697,6 +699,14 @@ def zipf_trace(out, n_logical: int, s: float, seed: int, first: int = 0, device:
     n_logical pages (pbm_zipf_trace)."""
     check(lib().pbm_zipf_trace(n_logical, s, seed, first, out.numel(), _ptr(out), device,
                                _stream(stream)), "pbm_zipf_trace")
+
+
+def zipf_trace_noalloc(out, n_logical: int, s: float, seed: int, first: int = 0,
+                       device: int = 0, stream=None) -> None:
+    """zipf_trace with no device allocation (pbm_zipf_trace_noalloc): the
+    CDF of at most 4,194,304 pages in the library's static scratch."""
+    check(lib().pbm_zipf_trace_noalloc(n_logical, s, seed, first, out.numel(), _ptr(out),
+                                       device, _stream(stream)), "pbm_zipf_trace_noalloc")
 
 
 class Bursts:
