@@ -48,6 +48,7 @@ from metrics import walk_stats  # noqa: E402
 METRIC = "page allocs+frees/sec at 90% fill, persistent-batched vs plain; avg walk steps"
 T_PAGES, PAGE_SIZE, FILL, SEED, N_REQ = 1 << 20, 4096, 0.9, 42, 262144
 HBM_PEAK_GBS = 8000.0
+GRAPH_STEPS = 32  # alloc+free steps captured in one replayed graph
 
 
 _T0 = time.perf_counter()
@@ -106,47 +107,116 @@ def time_phases(torch, pbm, pool, n, ids, gl, fl, steps: int, stream) -> tuple[d
     return own, bracket
 
 
-def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int) -> dict:
+def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: int = 50) -> dict:
     """Serving loop on 67,108,864 x 4 KiB pages (page map only): Poisson
     arrivals at 20,000/s binned into 1 ms steps, lognormal lengths (median
     512 pages, sigma 0.8, capped at 16,384), one page per sequence per step,
-    admission by free-page count.  Returns ops/s over the timed steps and
-    the per-allocation latency percentiles of the device getPage calls
-    made in them."""
-    T, n_steps = 1 << 26, warm_steps + timed_steps
+    admission by free-page count.  Arrivals are live: before each block of
+    `block` steps is put on the stream, the sequences arriving in it are
+    submitted (stream-ordered, no host synchronisation), while earlier
+    blocks still run.  Measured: ops/s over `timed_steps` eager steps and
+    over as many steps replayed from a captured `block`-step graph, the
+    per-allocation latency of the device getPage calls and the
+    arrival-to-admission wait of the sequences admitted in the eager
+    region, and each step kernel's own duration."""
+    T, n_steps = 1 << 26, warm_steps + 2 * timed_steps + 200 + 2 * block
     n = int(20000 * n_steps * 1e-3 * 1.2) + 64
     arr, tgt = pbm.serve_trace(n, 20000.0, 1e-3, 512.0, 0.8, 7, 16384)
     pool = pbm.Pool(T, 4096, strategy=pbm.RW_BM, backing=False, seed=7)
     srv = pbm.Server(pool, 32768, 16384, n, pbm.SERVE_LATENCY, lat_capacity=1 << 26)
-    srv.submit(arr, tgt, stream=stream)
-    srv.run(warm_steps, stream)
+    state = {"sub": 0, "done": 0}
+
+    def submit_until(step_hi: int) -> None:
+        j = int(np.searchsorted(arr, step_hi, side="right"))
+        if j > state["sub"]:
+            srv.submit(arr[state["sub"]:j], tgt[state["sub"]:j], stream=stream)
+            state["sub"] = j
+
+    def steps(k: int, graph=None) -> None:
+        for _ in range(k // block):
+            submit_until(state["done"] + block)
+            if graph is not None:
+                graph.launch(stream)
+            else:
+                srv.run(block, stream)
+            state["done"] += block
+
+    steps(warm_steps)
     torch.cuda.synchronize()
     progress(f"serving: {warm_steps} warm-up steps done")
     s0 = srv.stats(stream)
+    c0 = pool.counters(stream)
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     t0.record(stream)
-    srv.run(timed_steps, stream)
+    steps(timed_steps)
     t1.record(stream)
     torch.cuda.synchronize()
     ms = t0.elapsed_time(t1)
-    progress(f"serving: {timed_steps} timed steps, {ms:.1f} ms")
     s1 = srv.stats(stream)
+    progress(f"serving: {timed_steps} timed eager steps, {ms:.1f} ms")
+    g = srv.graph(block)
+    t2 = torch.cuda.Event(enable_timing=True)
+    t3 = torch.cuda.Event(enable_timing=True)
+    t2.record(stream)
+    steps(timed_steps, graph=g)
+    t3.record(stream)
+    torch.cuda.synchronize()
+    ms_graph = t2.elapsed_time(t3)
+    g.close()
+    s2 = srv.stats(stream)
+    progress(f"serving: {timed_steps} steps replayed from {block}-step graphs, {ms_graph:.1f} ms")
+    # each kernel's own duration over 200 more steps, with device counters
+    submit_until(state["done"] + 200)
+    pool.clear_counters(stream)
+    s3 = srv.stats(stream)
+    kms = srv.run_timed(200, stream)
+    state["done"] += 200
+    s4 = srv.stats(stream)
+    c4 = pool.counters(stream)
     lat = srv.latency_ticks(1 << 26, stream)[s0["lat_samples"]:s1["lat_samples"]]
+    res = srv.results(0, state["sub"], stream)
+    waits = srv.wait_ticks(0, state["sub"], stream)
     hz = srv.clock_hz()
+    in_eager = (res["admit_step"] > s0["step"]) & (res["admit_step"] <= s1["step"])
+    w = waits[in_eager].astype(np.float64)
+    w_steps = (res["admit_step"][in_eager].astype(np.int64) - arr[:state["sub"]][in_eager])
     ops = (s1["allocs"] - s0["allocs"]) + (s1["frees"] - s0["frees"])
+    ops_graph = (s2["allocs"] - s1["allocs"]) + (s2["frees"] - s1["frees"])
+    # roofline of the grow kernel: every slot's state word is scanned (4 B);
+    # a grant reads the slot's length and target (8 B), reads map words
+    # (8 B per probe), claims with read-modify-writes (16 B each), writes
+    # the page id, the new length and a latency sample (12 B)
+    grants = s4["allocs"] - s3["allocs"]
+    frees = s4["frees"] - s3["frees"]
+    grow_atomics = c4["atomics"] - c4["frees"]
+    grow_bytes = (4 * 32768 * 200 + 8 * c4["probes"] + 16 * grow_atomics + 20 * grants) / 200
+    achieved = grow_bytes / (kms["grow"] * 1e-3) / 1e9
     out = {
         "workload": "67,108,864 x 4 KiB pages (page map only), Poisson 20,000 seq/s in 1 ms "
                     "steps, lognormal length median 512 sigma 0.8, +1 page per step, RW_BM "
-                    "device getPage per slot",
+                    f"device getPage per slot; arrivals submitted live, {block} steps ahead",
         "warm_steps": warm_steps, "timed_steps": timed_steps, "ms_per_step": ms / timed_steps,
         "ops_per_s": ops / (ms * 1e-3), "allocs": s1["allocs"] - s0["allocs"],
-        "frees": s1["frees"] - s0["frees"], "active_at_end": s1["active"],
-        "pages_in_use_at_end": pool.count_used(stream),
+        "frees": s1["frees"] - s0["frees"],
+        "graph": {"steps_per_graph": block, "ms_per_step": ms_graph / timed_steps,
+                  "ops_per_s": ops_graph / (ms_graph * 1e-3)},
+        "kernel_ms": kms,
+        "active_at_end": s4["active"], "pages_in_use_at_end": pool.count_used(stream),
         "p50_alloc_us": float(np.percentile(lat, 50)) * 1e6 / hz if lat.size else None,
         "p99_alloc_us": float(np.percentile(lat, 99)) * 1e6 / hz if lat.size else None,
         "latency_samples": int(lat.size), "clock_hz": hz,
-        "alloc_failures": s1["alloc_failures"],
+        "admitted_timed": int(in_eager.sum()),
+        "p50_wait_us": float(np.percentile(w, 50)) * 1e6 / hz if w.size else None,
+        "p99_wait_us": float(np.percentile(w, 99)) * 1e6 / hz if w.size else None,
+        "mean_wait_steps": float(w_steps.mean()) if w.size else None,
+        "alloc_failures": s4["alloc_failures"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_serve_grow<0>", "bytes_per_launch": grow_bytes,
+                     "launch_ms": kms["grow"], "grants_per_launch": grants / 200,
+                     "frees_per_launch": frees / 200},
+        "_cpu_sample_map": pool.snapshot(stream), "_cpu_sample_grants": int(grants / 200 * 100),
     }
     srv.close()
     pool.close()
@@ -257,6 +327,7 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int, persistent: 
                      "atomic": {"achieved_ops": cnt["atomics"] / (dom_ms * 1e-3),
                                 "unit": "u64 RMW/s on the page map"}},
     }
+    out["_cpu_trace"] = tr
     bs.close()
     pool.close()
     return out
@@ -412,7 +483,9 @@ def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
     """Buffer cache: 16,777,216 x 4 KiB frames (page map only) allocated
     with CRW_BM, 4,194,304 logical pages held in 1,048,576 slots (the
     resident cap), CLOCK replacement, a Zipf(1.1) trace of n_refs
-    references from seed 7, taken `batch` at a time."""
+    references from seed 7, taken `batch` at a time.  The trace is timed
+    whole, then run again on a fresh cache with each kernel's own duration
+    recorded (pbm_cache_run_timed)."""
     T, L, C = 1 << 24, 1 << 22, 1 << 20
     refs = torch.empty(n_refs, dtype=torch.int32, device="cuda")
     pbm.zipf_trace(refs, L, 1.1, 7, stream=stream)
@@ -428,7 +501,28 @@ def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
     torch.cuda.synchronize()
     ms = t0.elapsed_time(t1)
     st = cache.stats(stream)
+    cache.close()
+    pool.close()
     progress(f"cache: {ms:.1f} ms")
+    pool = pbm.Pool(T, 4096, strategy=pbm.CRW_BM, backing=False, seed=7)
+    cache = pbm.Cache(pool, L, C, batch)
+    kms = cache.run_timed(refs, stream=stream)
+    st2 = cache.stats(stream)
+    cache.close()
+    pool.close()
+    n_batches = (n_refs + batch - 1) // batch
+    m = st["misses"] / n_batches       # misses per batch
+    v = st["evictions"] / n_batches    # victims per batch
+    # algorithmic bytes per launch (one batch): hits reads each reference
+    # and its page's slot (8 B) and raises the first-miss tag of each miss
+    # (16 B read-modify-write); plan reads each reference and its tag
+    # (12 B), writes each miss (4 B) and evicts each victim (owner, page and
+    # frame words: 16 B); load takes a frame per miss (8 B probe, 16 B
+    # claim) and writes the slot, its page and bit (16 B)
+    bytes_k = {"hits": 8 * batch + 16 * m, "plan": 12 * batch + 4 * m + 16 * v,
+               "load": 40 * m}
+    dom = max(kms, key=kms.get)
+    achieved = bytes_k[dom] / (kms[dom] * 1e-3) / 1e9
     out = {
         "workload": f"{n_refs} Zipf(1.1) references (seed 7) over 4,194,304 logical pages, "
                     "1,048,576 slots, CLOCK replacement, frames from CRW_BM on 16,777,216 "
@@ -436,12 +530,151 @@ def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
         "ms": ms, "refs_per_s": n_refs / (ms * 1e-3), "hits": st["hits"], "misses": st["misses"],
         "hit_rate": st["hits"] / n_refs, "evictions": st["evictions"],
         "resident": st["resident"], "alloc_failures": st["alloc_failures"],
-        "scan_timeouts": st["scan_timeouts"],
+        "scan_timeouts": st["scan_timeouts"] + st2["scan_timeouts"],
+        "kernel_ms": kms, "verified": bool(st2["hits"] == st["hits"]
+                                           and st2["evictions"] == st["evictions"]),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": {"hits": "k_cache_hits", "plan": "k_cache_plan",
+                                "load": "k_cache_load<2>"}[dom],
+                     "bytes_per_launch": bytes_k[dom], "launch_ms": kms[dom]},
     }
-    cache.close()
-    pool.close()
+    # the references on the host for the serial CLOCK baseline
+    out["_cpu_refs"] = refs.cpu().numpy()
     del refs
     return out
+
+
+def rank_at_scale_leg(torch, pbm, stream, n_pages: int, fill: float, n_req: int,
+                      steps: int) -> dict:
+    """The rank allocation at the serving pool's size: n_pages x 4 KiB
+    (page map only), Bernoulli pre-fill `fill` (seed 42), n_req one-page
+    requests then frees, with each kernel's own duration (the tile scan
+    holds four map words per thread on maps of 2^20 words and more)."""
+    pool = pbm.Pool(n_pages, PAGE_SIZE, strategy=pbm.RW_BM, backing=False, seed=SEED)
+    pool.prefill_bernoulli(fill, SEED, stream)
+    before = pool.snapshot(stream)
+    A = pool.free_count(stream)
+    ids = torch.full((n_req,), -1, dtype=torch.int32, device="cuda")
+    gl = pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK)
+    fl = pbm.launch(pbm.MODE_BATCHED)
+    pool.clear_counters(stream)
+    pool.get_pages(n_req, ids, None, 0, gl, stream)
+    torch.cuda.synchronize()
+    ca = pool.counters(stream)
+    i_np = ids.cpu().numpy()
+    pool.free_pages(n_req, ids, fl, stream)
+    torch.cuda.synchronize()
+    cc = pool.counters(stream)
+    grants = int((i_np >= 0).sum())
+    ok = (grants == min(n_req, A) and len(np.unique(i_np[i_np >= 0])) == grants
+          and bool(np.array_equal(pool.snapshot(stream), before)))
+    q = pbm.Launch(gl.mode, gl.block, gl.waves_per_cu, gl.walk_budget, gl.cluster_words,
+                   gl.flags | pbm.LAUNCH_NO_COUNTERS)
+    qf = pbm.Launch(fl.mode, fl.block, fl.waves_per_cu, fl.walk_budget, fl.cluster_words,
+                    fl.flags | pbm.LAUNCH_NO_COUNTERS)
+
+    def step():
+        pool.get_pages(n_req, ids, None, 0, q, stream)
+        pool.free_pages(n_req, ids, qf, stream)
+
+    ms = time_region(torch, step, steps, 2, stream)
+    own, _ = time_phases(torch, pbm, pool, n_req, ids, q, qf, steps, stream)
+    L = n_pages // 64
+    alloc_bytes = 8 * L + 16 * ca["atomics"] + 4 * n_req
+    free_bytes = 4 * n_req + 16 * (cc["atomics"] - ca["atomics"])
+    achieved = alloc_bytes / (own["alloc"] * 1e-3) / 1e9
+    pool.close()
+    return {"workload": f"{n_pages:,} x 4 KiB pages (page map only), Bernoulli pre-fill "
+                        f"f={fill} seed 42, {n_req:,} one-page requests then frees, rank "
+                        "allocation",
+            "free_pages": A, "grants": grants, "ms_per_step": ms,
+            "value": 2 * grants / (ms * 1e-3), "phases_ms": own, "verified": ok,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "k_get_rank<4>", "bytes_per_launch": alloc_bytes,
+                         "launch_ms": own["alloc"], "free_bytes_per_launch": free_bytes,
+                         "free_gbs": free_bytes / (own["free"] * 1e-3) / 1e9}}
+
+
+def cpu_legs(orc, c1, bursts, cache, serving, seconds: float) -> None:
+    """The CPU baseline of each configuration leg: the oracle's serial
+    restatement of the reference's policy on one host core, over the same
+    input (bounded samples of about `seconds` each, stated in `sample`)."""
+    if c1:
+        T, n = 1 << 16, 1 << 15
+        empty = orc.cleared(T)
+        t_end = time.perf_counter() + seconds
+        reps, busy = 0, 0.0
+        while time.perf_counter() < t_end:
+            st = orc.State(empty, T)
+            t = time.perf_counter()
+            ids, _ = st.rwbm(n, 0, SEED)
+            st.free_ids(ids)
+            busy += time.perf_counter() - t
+            reps += 1
+        c1["cpu_baseline"] = {
+            "value": 2 * n * reps / busy, "unit": "allocs+frees/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} cycles of 32,768 RandomWalkBasic_BM getPage + freePage on the "
+                      f"empty 65,536-page map, {busy:.1f} s on one core"}
+        progress("cpu baseline: 65,536-page configuration")
+    if bursts and "_cpu_trace" in bursts:
+        tr, T, PS = bursts["_cpu_trace"], 1 << 18, 4096
+        st = orc.State(orc.cleared(T), T)
+        n = int(tr["alloc_offset"][-1])
+        page = np.full(n, -1, dtype=np.int64)
+        hdr = np.zeros(n, dtype=np.int64)
+        nb = len(tr["alloc_offset"]) - 1
+        t = time.perf_counter()
+        for b in range(nb):
+            fi = tr["free_index"][tr["free_offset"][b]:tr["free_offset"][b + 1]]
+            live = (page[fi] >= 0) & (hdr[fi] > 0)
+            if live.any():
+                st.free_n(page[fi][live], hdr[fi][live])
+            a0, a1 = int(tr["alloc_offset"][b]), int(tr["alloc_offset"][b + 1])
+            if a1 > a0:
+                p_, _, h_, _ = st.malloc(tr["sizes"][a0:a1], PS, a0, 7)
+                page[a0:a1], hdr[a0:a1] = p_, h_
+        busy = time.perf_counter() - t
+        bursts["cpu_baseline"] = {
+            "value": bursts["requests"] / busy, "unit": "requests/s", "cores": 1, "kind": "port",
+            "sample": f"the whole trace ({nb} bursts) by the serial mallocRW/freeRW policy, "
+                      f"{busy:.1f} s on one core", "malloc_failures": int((page < 0).sum()),
+            "pages_live_at_end": int(T - st.free[0])}
+        progress("cpu baseline: bursts")
+    if cache and "_cpu_refs" in cache:
+        refs = cache["_cpu_refs"]
+        cc = orc.ClockCache(1 << 22, 1 << 20, 1)
+        t = time.perf_counter()
+        cc.run_textbook(refs)
+        busy = time.perf_counter() - t
+        cache["cpu_baseline"] = {
+            "value": refs.size / busy, "unit": "references/s", "cores": 1, "kind": "port",
+            "sample": f"the whole trace by textbook second-chance CLOCK, one reference at a "
+                      f"time, {busy:.2f} s on one core"}
+        cache["serial_clock"] = {
+            "hits": int(cc.stats[0]), "misses": int(cc.stats[1]), "evictions": int(cc.stats[2]),
+            "hit_rate": float(cc.stats[0]) / refs.size,
+            "batched_minus_serial_hits": cache["hits"] - int(cc.stats[0])}
+        progress("cpu baseline: cache")
+    if serving and "_cpu_sample_map" in serving:
+        bm, g = serving["_cpu_sample_map"], serving["_cpu_sample_grants"]
+        T = bm.size * 64
+        t_end = time.perf_counter() + seconds
+        reps, busy = 0, 0.0
+        while time.perf_counter() < t_end:
+            st = orc.State(bm, T)
+            t = time.perf_counter()
+            ids, _ = st.rwbm(g, 0, SEED)
+            st.free_ids(ids)
+            busy += time.perf_counter() - t
+            reps += 1
+        serving["cpu_baseline"] = {
+            "value": 2 * g * reps / busy, "unit": "allocs+frees/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} rounds of {g:,} RandomWalkBasic_BM getPage + freePage (100 "
+                      f"steps' grants) on the serving pool's map at the end of the run, "
+                      f"{busy:.1f} s on one core"}
+        progress("cpu baseline: serving")
 
 
 def main() -> int:
@@ -472,6 +705,9 @@ def main() -> int:
     ap.add_argument("--fill-sweep", type=int, default=1,
                     help="plain RW_BM / CollabRW_BM / rank at fills 0.5, 0.9, 0.99, and on "
                          "65,536 empty pages (0 = skip)")
+    ap.add_argument("--rank-scale", type=int, default=1,
+                    help="time the rank allocation on 67,108,864 pages at 90%% fill with "
+                         "4,194,304 requests (0 = skip)")
     ap.add_argument("--kmalloc", type=int, default=30000,
                     help="requests of the kernel-malloc leg (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -553,12 +789,19 @@ def main() -> int:
 
         ms = time_region(torch, eager, steps, warmup, stream)
         g = pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_ALLOC | pbm.PHASE_FREE)
-        ms_graph = time_region(torch, lambda: g.launch(stream), steps, warmup, stream)
+        ms_graph1 = time_region(torch, lambda: g.launch(stream), steps, warmup, stream)
+        g.close()
+        # GRAPH_STEPS steps per graph: a replay runs that many steps
+        g = pbm.Graph(pool, N_REQ, ids, None, 0, gl, fl, pbm.PHASE_ALLOC | pbm.PHASE_FREE,
+                      n_steps=GRAPH_STEPS)
+        reps = max(1, steps // GRAPH_STEPS)
+        ms_graph = time_region(torch, lambda: g.launch(stream), reps, 1, stream) / GRAPH_STEPS
         g.close()
         phases, bracket = time_phases(torch, pbm, pool, N_REQ, ids, gl, fl, min(steps, 200),
                                       stream)
         ok = ok and bool(np.array_equal(pool.snapshot(stream), before))
-        return {"ms_per_step": ms, "ms_per_step_graph": ms_graph, "phases_ms": phases,
+        return {"ms_per_step": ms, "ms_per_step_graph": ms_graph,
+                "ms_per_step_graph_1": ms_graph1, "phases_ms": phases,
                 "phases_ms_bracketed": bracket,
                 "counters_alloc": c_alloc, "counters_cycle": c_all, "walk_steps": ws,
                 "verified": ok}
@@ -592,7 +835,24 @@ def main() -> int:
     if args.fill_sweep:
         c1 = {"workload": "65,536 x 4 KiB pages, empty, 32,768 requests then frees",
               **policy_rows(torch, pbm, stream, 1 << 16, 1 << 15, 0.0, p_steps)}
+        # roofline of the rank row's dominant kernel: the alloc scan reads
+        # all 1,024 map words, claims the 512 it grants from with one
+        # read-modify-write each and writes 32,768 ids; the free reads the
+        # ids and releases the 512 words
+        ph = c1["batched_rank"]["phases_ms"]
+        c1_bytes = {"alloc": 8 * 1024 + 16 * 512 + 4 * 32768, "free": 4 * 32768 + 16 * 512}
+        c1_dom = max(ph, key=ph.get)
+        c1_ach = c1_bytes[c1_dom] / (ph[c1_dom] * 1e-3) / 1e9
+        c1["roofline"] = {"bound": "hbm", "achieved": c1_ach, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": c1_ach / HBM_PEAK_GBS, "traffic": None,
+                          "kernel": "k_get_rank_wave" if c1_dom == "alloc" else "k_free_batched",
+                          "bytes_per_launch": c1_bytes[c1_dom], "launch_ms": ph[c1_dom]}
         progress("65,536-page configuration measured")
+    rank_scale = None
+    if args.rank_scale:
+        rank_scale = rank_at_scale_leg(torch, pbm, stream, 1 << 26, 0.9, 1 << 22,
+                                       max(3, min(args.steps, 50)))
+        progress("rank allocation at 67,108,864 pages measured")
 
     b = results["batched"]
     ms_b = b["ms_per_step"]
@@ -678,6 +938,11 @@ def main() -> int:
             "cores": threads, "kind": "port",
             "sample": f"{reps_t} full cycles of the same workload on {threads} threads sharing "
                       f"one map (atomic claims), {busy_t:.1f} s"}
+        cpu_legs(orc, c1, bursts, cache, serving, args.cpu_seconds / 4)
+    for leg in (bursts, cache, serving):
+        if leg:
+            for k in [k for k in leg if k.startswith("_cpu")]:
+                del leg[k]
 
     line = {
         "metric": METRIC, "value": value, "unit": "allocs+frees/s", "n_gpus": 1,
@@ -690,6 +955,7 @@ def main() -> int:
                    "requests": N_REQ, "free_pages": A, "grants": granted,
                    "launch": b["tuned"], "device": dev_name},
         "ms_per_step_graph": b["ms_per_step_graph"],
+        "ms_per_step_graph_1": b["ms_per_step_graph_1"], "graph_steps": GRAPH_STEPS,
         "phases_ms": b["phases_ms"],
         "phases_ms_bracketed": b["phases_ms_bracketed"],
         "walk_steps": b["walk_steps"],
@@ -719,6 +985,7 @@ def main() -> int:
         "kmalloc": kmalloc,
         "fill_sweep": sweep,
         "small_pool": c1,
+        "rank_at_scale": rank_scale,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:
@@ -731,12 +998,15 @@ def main() -> int:
     return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]
                  and all(v["verified"] for v in comparators.values())
                  and (bursts is None or bursts["verified"])
+                 and (cache is None or cache["verified"])
+                 and (rank_scale is None or rank_scale["verified"])
                  and (kmalloc is None or (kmalloc["collab"]["verified"]
                                           and kmalloc["simple"]["verified"]))
                  and (sweep is None or all(v["verified"] for row in sweep.values()
                                            for v in row.values()))
-                 and (c1 is None or all(v["verified"] for k, v in c1.items()
-                                        if k != "workload"))) else 1
+                 and (c1 is None or all(c1[k]["verified"] for k in
+                                        ("plain_rw_bm", "plain_collab_rw_bm", "batched_rank")))
+                 ) else 1
 
 
 if __name__ == "__main__":

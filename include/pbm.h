@@ -392,6 +392,14 @@ typedef struct pbm_graph pbm_graph;
 int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
                      uint32_t* d_steps, const pbm_launch* get_launch,
                      const pbm_launch* free_launch, uint32_t phases, pbm_graph** out);
+
+/* The same with n_steps (1 .. 1024) copies of the captured phases in one
+ * graph, one after another: a launch runs n_steps steps.  No reference
+ * counterpart. */
+int pbm_graph_create_steps(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
+                           uint32_t* d_steps, const pbm_launch* get_launch,
+                           const pbm_launch* free_launch, uint32_t phases, uint32_t n_steps,
+                           pbm_graph** out);
 int pbm_graph_launch(pbm_graph* g, void* stream);
 int pbm_graph_destroy(pbm_graph* g);
 
@@ -452,12 +460,32 @@ int pbm_serve_destroy(pbm_server* s);
 /* Appends n sequences (host arrays): arrival_step[i] (non-decreasing, also
  * across calls), target[i] in [1, max_pages] and key[i], the request's
  * own id that its payloads derive from (NULL: the sequence id).  Sequence
- * ids count up from 0 in submission order.  Synchronises the stream. */
+ * ids count up from 0 in submission order.  Stream-ordered and without
+ * host synchronisation: the arrays are copied (through the server's
+ * pinned mirror) and made visible to admission after the work already on
+ * the stream, so sequences can be submitted while steps run; a sequence
+ * is admissible from its arrival step or from the first step after its
+ * submission, whichever is later.  The host arrays may be reused when the
+ * call returns. */
 int pbm_serve_submit(pbm_server* s, uint32_t n, const uint32_t* arrival_step,
                      const uint32_t* target, const uint32_t* key, void* stream);
 
+/* Per sequence first .. first+n-1: wall-clock ticks (pbm_serve_clock_hz)
+ * from its arrival -- the later of its submission reaching the device and
+ * the start of its arrival step -- to the start of the step that admitted
+ * it; 0xFFFFFFFF for a sequence not admitted (or whose arrival step lies
+ * more than 65,536 steps before its admission). */
+int pbm_serve_read_waits(pbm_server* s, uint32_t first, uint32_t n, uint32_t* ticks,
+                         void* stream);
+
 /* Runs n_steps steps (three kernels each) on the stream. */
 int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream);
+
+/* Runs n_steps steps (1 .. 4096) as pbm_serve_run does, each kernel
+ * recording events at its own start and end, waits for them and writes the
+ * mean duration of each kernel in ms: kernel_ms[0] admit, [1] grow, [2]
+ * release.  A measurement helper; no reference counterpart. */
+int pbm_serve_run_timed(pbm_server* s, uint32_t n_steps, float* kernel_ms, void* stream);
 
 /* Captures n_steps steps as one HIP graph (launch with pbm_graph_launch;
  * every launch runs the next n_steps steps). */
@@ -528,6 +556,13 @@ int pbm_cache_destroy(pbm_cache* k);
  * reference to an id outside 0 .. n_logical-1 is neither a hit nor a miss;
  * it is skipped and counted (invalid_refs). */
 int pbm_cache_run(pbm_cache* k, const int32_t* d_refs, uint64_t n, void* stream);
+
+/* pbm_cache_run of at most 4,096 batches with each kernel recording events
+ * at its own start and end; waits for them and writes the mean duration in
+ * ms of kernel_ms[0] hits, [1] plan, [2] load.  A measurement helper; no
+ * reference counterpart. */
+int pbm_cache_run_timed(pbm_cache* k, const int32_t* d_refs, uint64_t n, float* kernel_ms,
+                        void* stream);
 
 int pbm_cache_read_stats(pbm_cache* k, pbm_cache_stats* out, void* stream);
 

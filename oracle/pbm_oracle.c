@@ -1849,6 +1849,54 @@ out:
     free(ones_pos);
 }
 
+/* Textbook second-chance CLOCK, one reference at a time (the batch rule
+ * at batch 1, in time linear in the references rather than in the frames
+ * per miss): a hit sets the frame's bit; a miss fills the next empty frame,
+ * else the hand clears set bits until it finds a clear one, replaces that
+ * page and moves past it.  Loaded pages start with the bit set.  Every
+ * bit the hand clears was set by a reference, so the sweeps total at most
+ * the references.  References outside 0 .. n_logical-1 are skipped.
+ * stats = {hits, misses, evictions}; the page evicted k-th goes to
+ * evict_log[k] while k < log_cap. */
+void orc_clock_textbook(uint32_t n_logical, uint32_t capacity, const int32_t* refs,
+                        uint64_t n_refs, int32_t* slot_of, int32_t* owner, uint8_t* refbit,
+                        uint32_t* arm_res, int32_t* evict_log, uint64_t log_cap,
+                        uint64_t* stats) {
+    const uint32_t C = capacity;
+    uint32_t hand = arm_res[0], resident = arm_res[1];
+    for (uint64_t i = 0; i < n_refs; ++i) {
+        const int32_t l = refs[i];
+        if ((uint32_t)l >= n_logical) continue;
+        const int32_t s0 = slot_of[l];
+        if (s0 >= 0) {
+            refbit[s0] = 1;
+            stats[0]++;
+            continue;
+        }
+        stats[1]++;
+        uint32_t s;
+        if (resident < C) {
+            s = resident++;
+        } else {
+            while (refbit[hand]) {
+                refbit[hand] = 0;
+                hand = hand + 1u == C ? 0u : hand + 1u;
+            }
+            s = hand;
+            const int32_t old = owner[s];
+            slot_of[old] = -1;
+            if (evict_log && stats[2] < log_cap) evict_log[stats[2]] = old;
+            stats[2]++;
+            hand = hand + 1u == C ? 0u : hand + 1u;
+        }
+        owner[s] = l;
+        slot_of[l] = (int32_t)s;
+        refbit[s] = 1;
+    }
+    arm_res[0] = hand;
+    arm_res[1] = resident;
+}
+
 /* --------------------------------------------------- clusters */
 
 /* Maximal runs of free pages, circular; a fully free pool is one run. */

@@ -173,6 +173,12 @@ void orc_zipf_trace(uint32_t n, double s, uint64_t seed, uint64_t first, uint64_
 void orc_clock_run(uint32_t n_logical, uint32_t capacity, uint32_t batch, const int32_t* refs,
                    uint64_t n_refs, int32_t* slot_of, int32_t* owner, uint8_t* refbit,
                    uint32_t* arm_res, int32_t* evict_log, uint64_t log_cap, uint64_t* stats);
+/* Textbook second-chance CLOCK, one reference at a time, in time linear in
+ * the references (the batch rule of orc_clock_run at batch 1). */
+void orc_clock_textbook(uint32_t n_logical, uint32_t capacity, const int32_t* refs,
+                        uint64_t n_refs, int32_t* slot_of, int32_t* owner, uint8_t* refbit,
+                        uint32_t* arm_res, int32_t* evict_log, uint64_t log_cap,
+                        uint64_t* stats);
 
 /* RandomWalkBasic (single-page walk) serial, and its closed form RW_TAS. */
 uint32_t orc_rwbasic_serial(uint64_t* bm, uint32_t n_pages, int64_t* free_count, uint32_t n,

@@ -98,6 +98,8 @@ _SIGS = {
     "orc_zipf_trace": (None, [C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_uint64, _P32]),
     "orc_clock_run": (None, [C.c_uint32, C.c_uint32, C.c_uint32, _P32, C.c_uint64, _P32, _P32,
                              C.POINTER(C.c_uint8), _PU32, _P32, C.c_uint64, _PU64]),
+    "orc_clock_textbook": (None, [C.c_uint32, C.c_uint32, _P32, C.c_uint64, _P32, _P32,
+                                  C.POINTER(C.c_uint8), _PU32, _P32, C.c_uint64, _PU64]),
     "orc_serve_checksum": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32]),
     "orc_serve_sim": (None, [C.c_int64, C.c_uint32, C.c_uint32, _PU32, _PU32, C.c_uint32, _PU32,
                              _PU32, _PU64]),
@@ -466,6 +468,17 @@ class ClockCache:
                             self.refbit.ctypes.data_as(C.POINTER(C.c_uint8)),
                             _p(self.arm_res, _PU32), _p(self.log, _P32) if self.log_cap else None,
                             self.log_cap, _p(self.stats, _PU64))
+
+    def run_textbook(self, refs: np.ndarray) -> None:
+        """The references one at a time by textbook second-chance CLOCK
+        (orc_clock_textbook): the batch rule at batch 1, in linear time."""
+        refs = np.ascontiguousarray(refs, dtype=np.int32)
+        lib().orc_clock_textbook(self.n_logical, self.capacity, _p(refs, _P32), refs.size,
+                                 _p(self.slot_of, _P32), _p(self.owner, _P32),
+                                 self.refbit.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                 _p(self.arm_res, _PU32),
+                                 _p(self.log, _P32) if self.log_cap else None, self.log_cap,
+                                 _p(self.stats, _PU64))
 
     @property
     def evictions(self) -> np.ndarray:

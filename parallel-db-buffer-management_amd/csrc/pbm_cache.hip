@@ -652,7 +652,18 @@ extern "C" int pbm_cache_create(pbm_handle* h, const pbm_cache_config* cfg, pbm_
 
 namespace {
 
-int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
+// A launch that records e0 / e1 at the kernel's own start and end when
+// they are given.
+template <typename K, typename... A>
+void launch_ev(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t e0, hipEvent_t e1, A... a) {
+    if (e0 || e1) hipExtLaunchKernelGGL(kernel, g, b, 0, st, e0, e1, 0, a...);
+    else hipLaunchKernelGGL(kernel, g, b, 0, st, a...);
+}
+
+// One batch: hits, plan, load.  With ev (six events), kernel k records
+// ev[2k] and ev[2k+1] at its own start and end.
+int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s,
+                hipEvent_t* ev = nullptr) {
     const CacheDev& d = k->d;
     const dim3 b(256);
     if (k->tag == kLow) {
@@ -662,11 +673,14 @@ int cache_batch(pbm_cache* k, const int32_t* refs, uint32_t nb, hipStream_t s) {
         k->tag = 0;
     }
     const u64 tag = ++k->tag;
-    hipLaunchKernelGGL(k_cache_hits, dim3((nb + 255u) / 256u), b, 0, s, d, refs, nb, tag);
-    hipLaunchKernelGGL(k_cache_plan, dim3(d.plan_groups), b, 0, s, d, refs, nb, tag);
+    hipEvent_t e[6] = {};
+    if (ev)
+        for (int i = 0; i < 6; ++i) e[i] = ev[i];
+    launch_ev(k_cache_hits, dim3((nb + 255u) / 256u), b, s, e[0], e[1], d, refs, nb, tag);
+    launch_ev(k_cache_plan, dim3(d.plan_groups), b, s, e[2], e[3], d, refs, nb, tag);
     switch (k->h->cfg.strategy) {
 #define PBM_LOAD_CASE(S) \
-    case S: hipLaunchKernelGGL(k_cache_load<S>, dim3(d.load_groups), b, 0, s, d); break;
+    case S: launch_ev(k_cache_load<S>, dim3(d.load_groups), b, s, e[4], e[5], d); break;
         PBM_LOAD_CASE(PBM_RW_BM)
         PBM_LOAD_CASE(PBM_COLLAB_RW_BM)
         PBM_LOAD_CASE(PBM_CRW_BM)
@@ -699,6 +713,50 @@ extern "C" int pbm_cache_run(pbm_cache* k, const int32_t* d_refs, uint64_t n, vo
         hipLaunchKernelGGL(k_cache_close, dim3(1), dim3(1), 0, s, k->d);
         PBM_HIP(hipGetLastError());
     }
+    return PBM_OK;
+}
+
+extern "C" int pbm_cache_run_timed(pbm_cache* k, const int32_t* d_refs, uint64_t n,
+                                   float* kernel_ms, void* stream) {
+    if (!k || !kernel_ms) return pbmi::fail(PBM_ERR_INVALID, "null cache or output");
+    PBM_CHECK_HANDLE(k->h);
+    if (n && !d_refs) return pbmi::fail(PBM_ERR_INVALID, "null references");
+    const uint64_t nbat = (n + k->d.batch - 1u) / k->d.batch;
+    if (nbat == 0 || nbat > 4096u)
+        return pbmi::fail(PBM_ERR_RANGE, "pbm_cache_run_timed runs 1 .. 4096 batches");
+    hipStream_t s = pbmi::as_stream(stream);
+    std::vector<hipEvent_t> ev((size_t)nbat * 6u, nullptr);
+    int rc = PBM_OK;
+    for (hipEvent_t& e : ev) {
+        const hipError_t he = hipEventCreate(&e);
+        if (he != hipSuccess) {
+            rc = pbmi::hip_fail(he, "hipEventCreate", __FILE__, __LINE__);
+            break;
+        }
+    }
+    for (uint64_t q = 0; q < nbat && !rc; ++q) {
+        const uint64_t b0 = q * k->d.batch;
+        const uint32_t nb = (uint32_t)(n - b0 < k->d.batch ? n - b0 : k->d.batch);
+        rc = cache_batch(k, d_refs + b0, nb, s, &ev[q * 6u]);
+    }
+    if (!rc) {
+        hipLaunchKernelGGL(k_cache_close, dim3(1), dim3(1), 0, s, k->d);
+        hipError_t he = hipGetLastError();
+        if (he == hipSuccess) he = hipStreamSynchronize(s);
+        if (he != hipSuccess) rc = pbmi::hip_fail(he, "cache timed run", __FILE__, __LINE__);
+    }
+    double sum[3] = {0, 0, 0};
+    for (uint64_t q = 0; q < nbat && !rc; ++q)
+        for (int j = 0; j < 3 && !rc; ++j) {
+            float ms = 0;
+            const hipError_t he = hipEventElapsedTime(&ms, ev[q * 6u + 2u * j], ev[q * 6u + 2u * j + 1u]);
+            if (he != hipSuccess) rc = pbmi::hip_fail(he, "hipEventElapsedTime", __FILE__, __LINE__);
+            sum[j] += ms;
+        }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    for (int j = 0; j < 3; ++j) kernel_ms[j] = (float)(sum[j] / (double)nbat);
     return PBM_OK;
 }
 

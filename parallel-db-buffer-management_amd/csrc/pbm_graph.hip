@@ -13,16 +13,17 @@ extern "C" int pbm_graph_destroy(pbm_graph* g) {
     return PBM_OK;
 }
 
-extern "C" int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
-                                uint32_t* d_steps, const pbm_launch* get_launch,
-                                const pbm_launch* free_launch, uint32_t phases,
-                                pbm_graph** out) {
+extern "C" int pbm_graph_create_steps(pbm_handle* h, uint32_t n, uint64_t req_base,
+                                      int32_t* d_ids, uint32_t* d_steps,
+                                      const pbm_launch* get_launch, const pbm_launch* free_launch,
+                                      uint32_t phases, uint32_t n_steps, pbm_graph** out) {
     PBM_CHECK_HANDLE(h);
     if (!out) return pbmi::fail(PBM_ERR_INVALID, "null output graph pointer");
     *out = nullptr;
     if (phases == 0 || (phases & ~(PBM_PHASE_ALLOC | PBM_PHASE_FREE)))
         return pbmi::fail(PBM_ERR_INVALID, "phases must combine PBM_PHASE_ALLOC/FREE");
     if (n == 0 || !d_ids) return pbmi::fail(PBM_ERR_INVALID, "graph needs n > 0 and d_ids");
+    if (n_steps == 0 || n_steps > 1024u) return pbmi::fail(PBM_ERR_RANGE, "n_steps must be in [1, 1024]");
     hipStream_t s = nullptr;
     PBM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     pbm_graph* g = new pbm_graph();
@@ -32,9 +33,11 @@ extern "C" int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, in
     if (e != hipSuccess) {
         rc = pbmi::hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
     } else {
-        if (phases & PBM_PHASE_ALLOC)
-            rc = pbm_get_pages(h, n, req_base, d_ids, d_steps, get_launch, s);
-        if (!rc && (phases & PBM_PHASE_FREE)) rc = pbm_free_pages(h, n, d_ids, free_launch, s);
+        for (uint32_t k = 0; k < n_steps && !rc; ++k) {
+            if (phases & PBM_PHASE_ALLOC)
+                rc = pbm_get_pages(h, n, req_base, d_ids, d_steps, get_launch, s);
+            if (!rc && (phases & PBM_PHASE_FREE)) rc = pbm_free_pages(h, n, d_ids, free_launch, s);
+        }
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(s, &graph);
         g->graph = graph;
@@ -51,6 +54,14 @@ extern "C" int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, in
     }
     *out = g;
     return PBM_OK;
+}
+
+extern "C" int pbm_graph_create(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
+                                uint32_t* d_steps, const pbm_launch* get_launch,
+                                const pbm_launch* free_launch, uint32_t phases,
+                                pbm_graph** out) {
+    return pbm_graph_create_steps(h, n, req_base, d_ids, d_steps, get_launch, free_launch, phases,
+                                  1u, out);
 }
 
 extern "C" int pbm_graph_launch(pbm_graph* g, void* stream) {

@@ -1,6 +1,7 @@
 // Continuous-batching serving loop: a device-resident slot table, arrival
 // queue and per-slot page lists; one step = admit, grow, release.
 #include <hip/hip_runtime.h>
+#include <cstring>
 #include <vector>
 #include "pbm_internal.hpp"
 
@@ -10,6 +11,8 @@ namespace {
 
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kAdmitWindow = 1024;  // waiting sequences examined per step
+constexpr uint32_t kStepRing = 1u << 16; // start times kept for the last 65,536 steps
+constexpr uint32_t kWaitUnknown = 0xFFFFFFFFu;
 
 // Words of the control block.
 enum : uint32_t {
@@ -47,6 +50,9 @@ struct ServeDev {
     uint32_t* done;            // [2 * max_slots] (slot, sequence) finished at the last grow
     u64* ctl;                  // [kCtlWords]
     uint32_t* lat;             // [lat_cap] wall-clock ticks per allocation
+    u64* seq_visible;          // [max_seqs] wall clock when the sequence was submitted
+    u64* step_tick;            // [kStepRing] wall clock at the start of step s (s mod ring)
+    uint32_t* res_wait;        // [max_seqs] ticks from arrival to admission
     uint32_t lat_cap;
     uint32_t max_slots;
     uint32_t max_pages;
@@ -71,11 +77,16 @@ __global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) {
     __shared__ uint32_t s_tgt[kAdmitWindow];
     __shared__ uint32_t s_m;
     __shared__ long long s_add;
+    __shared__ u64 s_now;
     const uint32_t t = threadIdx.x;
     if (t < kCtlWords) s_ctl[t] = ld_ctl(&d.ctl[t]);
-    if (t == 0) s_free = __hip_atomic_load(d.c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == 0) {
+        s_free = __hip_atomic_load(d.c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_now = wall_clock64();
+    }
     __syncthreads();
     const u64 step = s_ctl[kStep] + 1u;
+    const u64 now = s_now;
     const u64 head = s_ctl[kHead];
     const u64 tail = s_ctl[kTail];
     const u64 done_n = s_ctl[kDone];
@@ -112,8 +123,22 @@ __global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) {
         d.slot_target[slot] = s_tgt[i];
         d.res_admit[seq] = (uint32_t)step;
         d.res_sum[seq] = 0;
+        // arrival: the later of its submission and the start of its arrival
+        // step (known for the last kStepRing steps; step 0 is before the
+        // first step)
+        const uint32_t as = s_arr[i];
+        bool known = true;
+        u64 a = 0;
+        if (as == step) a = now;
+        else if (as > 0u && step - as < kStepRing) a = d.step_tick[as % kStepRing];
+        else if (as > 0u) known = false;
+        const u64 vis = d.seq_visible[seq];
+        const u64 arrive = a > vis ? a : vis;
+        d.res_wait[seq] = !known ? kWaitUnknown
+                          : (now - arrive > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)(now - arrive));
     }
     if (t == 0) {
+        d.step_tick[step % kStepRing] = now;
         d.ctl[kStep] = step;
         d.ctl[kHead] = head + m;
         d.ctl[kActive] = active + m;
@@ -328,7 +353,15 @@ __global__ void k_serve_init(ServeDev d) {
     if (i < kCtlWords) d.ctl[i] = i == kTop ? (u64)d.max_slots : 0ull;
 }
 
-__global__ void k_serve_set_tail(u64* ctl, u64 tail) { ctl[kTail] = tail; }
+// Makes sequences first .. tail-1 (copied in before this kernel on the
+// same stream) visible to admission, stamping when that happened.
+__global__ void k_serve_set_tail(ServeDev d, u64 first, u64 tail) {
+    const u64 now = wall_clock64();
+    for (u64 i = first + blockIdx.x * blockDim.x + threadIdx.x; i < tail;
+         i += (u64)gridDim.x * blockDim.x)
+        d.seq_visible[i] = now;
+    if (blockIdx.x == 0 && threadIdx.x == 0) d.ctl[kTail] = tail;
+}
 
 }  // namespace
 
@@ -340,6 +373,11 @@ struct pbm_server {
     uint32_t last_arrival = 0;
     uint32_t release_blocks = 1;
     std::vector<void*> mem;
+    // pinned host mirror of arr_step, target and key: a submit copies into
+    // it and from it to the device without waiting for the stream; no
+    // sequence id is ever written twice, so a copy in flight is never
+    // overwritten
+    uint32_t* h_mirror = nullptr;  // [3 * max_seqs]
 };
 
 extern "C" int pbm_serve_destroy(pbm_server* s) {
@@ -348,6 +386,7 @@ extern "C" int pbm_serve_destroy(pbm_server* s) {
     (void)hipDeviceSynchronize();
     for (void* p : s->mem)
         if (p) (void)hipFree(p);
+    if (s->h_mirror) (void)hipHostFree(s->h_mirror);
     delete s;
     return PBM_OK;
 }
@@ -404,7 +443,14 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
     PBM_SALLOC(done, uint32_t, 2u * nl)
     PBM_SALLOC(ctl, u64, kCtlWords)
     PBM_SALLOC(lat, uint32_t, d.lat_cap)
+    PBM_SALLOC(seq_visible, u64, ns)
+    PBM_SALLOC(step_tick, u64, kStepRing)
+    PBM_SALLOC(res_wait, uint32_t, ns)
 #undef PBM_SALLOC
+    if (!rc) {
+        const hipError_t he = hipHostMalloc((void**)&s->h_mirror, 3u * ns * 4u, hipHostMallocDefault);
+        if (he != hipSuccess) rc = pbmi::hip_fail(he, "hipHostMalloc(serve mirror)", __FILE__, __LINE__);
+    }
     if (rc) {
         pbm_serve_destroy(s);
         return rc;
@@ -413,7 +459,9 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
     if ((e = hipMemset((void*)d.res_admit, 0, ns * 4u)) != hipSuccess ||
         (e = hipMemset((void*)d.res_finish, 0, ns * 4u)) != hipSuccess ||
         (e = hipMemset((void*)d.res_len, 0, ns * 4u)) != hipSuccess ||
-        (e = hipMemset((void*)d.res_sum, 0, ns * 8u)) != hipSuccess) {
+        (e = hipMemset((void*)d.res_sum, 0, ns * 8u)) != hipSuccess ||
+        (e = hipMemset((void*)d.res_wait, 0xFF, ns * 4u)) != hipSuccess ||
+        (e = hipMemset((void*)d.step_tick, 0, (size_t)kStepRing * 8u)) != hipSuccess) {
         pbm_serve_destroy(s);
         return pbmi::hip_fail(e, "hipMemset(serve)", __FILE__, __LINE__);
     }
@@ -444,36 +492,63 @@ extern "C" int pbm_serve_submit(pbm_server* s, uint32_t n, const uint32_t* arriv
             return pbmi::fail(PBM_ERR_RANGE, "target length must be in [1, max_pages]");
     }
     hipStream_t st = pbmi::as_stream(stream);
-    PBM_HIP(hipMemcpyAsync((void*)(s->d.arr_step + s->submitted), arrival_step, (size_t)n * 4u,
+    const size_t ns = s->cfg.max_seqs, at = s->submitted;
+    uint32_t* m_arr = s->h_mirror + at;
+    uint32_t* m_tgt = s->h_mirror + ns + at;
+    uint32_t* m_key = s->h_mirror + 2u * ns + at;
+    std::memcpy(m_arr, arrival_step, (size_t)n * 4u);
+    std::memcpy(m_tgt, target, (size_t)n * 4u);
+    for (uint32_t i = 0; i < n; ++i) m_key[i] = key ? key[i] : s->submitted + i;
+    PBM_HIP(hipMemcpyAsync((void*)(s->d.arr_step + at), m_arr, (size_t)n * 4u,
                            hipMemcpyHostToDevice, st));
-    PBM_HIP(hipMemcpyAsync((void*)(s->d.target + s->submitted), target, (size_t)n * 4u,
+    PBM_HIP(hipMemcpyAsync((void*)(s->d.target + at), m_tgt, (size_t)n * 4u,
                            hipMemcpyHostToDevice, st));
-    std::vector<uint32_t> ids;
-    if (!key) {
-        ids.resize(n);
-        for (uint32_t i = 0; i < n; ++i) ids[i] = s->submitted + i;
-        key = ids.data();
-    }
-    PBM_HIP(hipMemcpyAsync((void*)(s->d.key + s->submitted), key, (size_t)n * 4u,
+    PBM_HIP(hipMemcpyAsync((void*)(s->d.key + at), m_key, (size_t)n * 4u,
                            hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_serve_set_tail, dim3(1), dim3(1), 0, st, s->d.ctl,
-                       (u64)s->submitted + n);
+    const uint32_t blocks = n > 65536u ? 256u : (n + 255u) / 256u;
+    hipLaunchKernelGGL(k_serve_set_tail, dim3(blocks), dim3(256), 0, st, s->d, (u64)at,
+                       (u64)at + n);
     PBM_HIP(hipGetLastError());
-    PBM_HIP(hipStreamSynchronize(st));
     s->submitted += n;
     s->last_arrival = last;
     return PBM_OK;
 }
 
+extern "C" int pbm_serve_read_waits(pbm_server* s, uint32_t first, uint32_t n, uint32_t* ticks,
+                                    void* stream) {
+    if (!s || (n && !ticks)) return pbmi::fail(PBM_ERR_INVALID, "null server or output");
+    PBM_CHECK_HANDLE(s->h);
+    if ((uint64_t)first + n > s->cfg.max_seqs)
+        return pbmi::fail(PBM_ERR_RANGE, "wait range beyond max_seqs");
+    if (!n) return PBM_OK;
+    hipStream_t st = pbmi::as_stream(stream);
+    PBM_HIP(hipMemcpyAsync(ticks, s->d.res_wait + first, (size_t)n * 4u, hipMemcpyDeviceToHost, st));
+    PBM_HIP(hipStreamSynchronize(st));
+    return PBM_OK;
+}
+
 namespace {
 
-int launch_step(pbm_server* s, hipStream_t st) {
+// A launch that records e0 / e1 at the kernel's own start and end when
+// they are given (a plain launch otherwise, which graph capture takes).
+template <typename K, typename... A>
+void launch_ev(K kernel, dim3 g, dim3 b, hipStream_t st, hipEvent_t e0, hipEvent_t e1, A... a) {
+    if (e0 || e1) hipExtLaunchKernelGGL(kernel, g, b, 0, st, e0, e1, 0, a...);
+    else hipLaunchKernelGGL(kernel, g, b, 0, st, a...);
+}
+
+// One step: admit, grow, release.  With ev (six events), each kernel
+// records ev[2k] and ev[2k+1] at its own start and end.
+int launch_step(pbm_server* s, hipStream_t st, hipEvent_t* ev = nullptr) {
     const ServeDev& d = s->d;
-    hipLaunchKernelGGL(k_serve_admit, dim3(1), dim3(1024), 0, st, d);
+    hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr;
+    launch_ev(k_serve_admit, dim3(1), dim3(1024), st, e0, e1, d);
     const dim3 g((d.max_slots + 255u) / 256u), b(256);
+    e0 = ev ? ev[2] : nullptr;
+    e1 = ev ? ev[3] : nullptr;
     switch (s->h->cfg.strategy) {
 #define PBM_GROW_CASE(S) \
-    case S: hipLaunchKernelGGL(k_serve_grow<S>, g, b, 0, st, d); break;
+    case S: launch_ev(k_serve_grow<S>, g, b, st, e0, e1, d); break;
         PBM_GROW_CASE(PBM_RW_BM)
         PBM_GROW_CASE(PBM_COLLAB_RW_BM)
         PBM_GROW_CASE(PBM_CRW_BM)
@@ -489,12 +564,14 @@ int launch_step(pbm_server* s, hipStream_t st) {
         default: return pbmi::fail(PBM_ERR_UNSUPPORTED, "no serving launch for this strategy");
     }
     const dim3 rg(s->release_blocks), rb(256);
+    e0 = ev ? ev[4] : nullptr;
+    e1 = ev ? ev[5] : nullptr;
     if (s->h->cfg.strategy == PBM_LINKED_LIST)
-        hipLaunchKernelGGL(k_serve_release<PBM_LINKED_LIST>, rg, rb, 0, st, d);
+        launch_ev(k_serve_release<PBM_LINKED_LIST>, rg, rb, st, e0, e1, d);
     else if (s->h->cfg.strategy == PBM_OUROBOROS)
-        hipLaunchKernelGGL(k_serve_release<PBM_OUROBOROS>, rg, rb, 0, st, d);
+        launch_ev(k_serve_release<PBM_OUROBOROS>, rg, rb, st, e0, e1, d);
     else
-        hipLaunchKernelGGL(k_serve_release<PBM_RW_BM>, rg, rb, 0, st, d);
+        launch_ev(k_serve_release<PBM_RW_BM>, rg, rb, st, e0, e1, d);
     PBM_HIP(hipGetLastError());
     return PBM_OK;
 }
@@ -509,6 +586,43 @@ extern "C" int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream) {
         int rc = launch_step(s, st);
         if (rc) return rc;
     }
+    return PBM_OK;
+}
+
+extern "C" int pbm_serve_run_timed(pbm_server* s, uint32_t n_steps, float* kernel_ms,
+                                   void* stream) {
+    if (!s || !kernel_ms) return pbmi::fail(PBM_ERR_INVALID, "null server or output");
+    PBM_CHECK_HANDLE(s->h);
+    if (n_steps == 0 || n_steps > 4096u)
+        return pbmi::fail(PBM_ERR_RANGE, "pbm_serve_run_timed runs 1 .. 4096 steps");
+    hipStream_t st = pbmi::as_stream(stream);
+    std::vector<hipEvent_t> ev((size_t)n_steps * 6u, nullptr);
+    int rc = PBM_OK;
+    for (hipEvent_t& e : ev) {
+        const hipError_t he = hipEventCreate(&e);
+        if (he != hipSuccess) {
+            rc = pbmi::hip_fail(he, "hipEventCreate", __FILE__, __LINE__);
+            break;
+        }
+    }
+    for (uint32_t k = 0; k < n_steps && !rc; ++k) rc = launch_step(s, st, &ev[(size_t)k * 6u]);
+    double sum[3] = {0, 0, 0};
+    if (!rc) {
+        const hipError_t he = hipStreamSynchronize(st);
+        if (he != hipSuccess) rc = pbmi::hip_fail(he, "hipStreamSynchronize", __FILE__, __LINE__);
+    }
+    for (uint32_t k = 0; k < n_steps && !rc; ++k)
+        for (int j = 0; j < 3 && !rc; ++j) {
+            float ms = 0;
+            const hipError_t he =
+                hipEventElapsedTime(&ms, ev[(size_t)k * 6u + 2u * j], ev[(size_t)k * 6u + 2u * j + 1u]);
+            if (he != hipSuccess) rc = pbmi::hip_fail(he, "hipEventElapsedTime", __FILE__, __LINE__);
+            sum[j] += ms;
+        }
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    for (int j = 0; j < 3; ++j) kernel_ms[j] = (float)(sum[j] / n_steps);
     return PBM_OK;
 }
 

@@ -179,6 +179,9 @@ _SIGS = {
     "pbm_graph_create": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
                                    C.POINTER(Launch), C.POINTER(Launch), C.c_uint32,
                                    C.POINTER(C.c_void_p)]),
+    "pbm_graph_create_steps": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p,
+                                         C.c_void_p, C.POINTER(Launch), C.POINTER(Launch),
+                                         C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "pbm_graph_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pbm_graph_destroy": (C.c_int, [C.c_void_p]),
     "pbm_autotune": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
@@ -188,7 +191,10 @@ _SIGS = {
     "pbm_serve_destroy": (C.c_int, [C.c_void_p]),
     "pbm_serve_submit": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p]),
+    "pbm_serve_read_waits": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                                       C.c_void_p]),
     "pbm_serve_run": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pbm_serve_run_timed": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "pbm_serve_graph": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "pbm_serve_read_stats": (C.c_int, [C.c_void_p, C.POINTER(ServeStats), C.c_void_p]),
     "pbm_serve_read_results": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
@@ -199,6 +205,8 @@ _SIGS = {
     "pbm_cache_create": (C.c_int, [C.c_void_p, C.POINTER(CacheConfig), C.POINTER(C.c_void_p)]),
     "pbm_cache_destroy": (C.c_int, [C.c_void_p]),
     "pbm_cache_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    "pbm_cache_run_timed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p,
+                                      C.c_void_p]),
     "pbm_cache_read_stats": (C.c_int, [C.c_void_p, C.POINTER(CacheStats), C.c_void_p]),
     "pbm_cache_read_evictions": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_void_p,
                                            C.c_void_p]),
@@ -516,12 +524,13 @@ class Graph:
 
     def __init__(self, pool: Pool, n: int, ids, steps=None, req_base: int = 0,
                  get_launch: Launch | None = None, free_launch: Launch | None = None,
-                 phases: int = 3):
+                 phases: int = 3, n_steps: int = 1):
         g = C.c_void_p()
-        check(lib().pbm_graph_create(pool.h, n, req_base, _ptr(ids), _ptr(steps),
-                                     C.byref(get_launch or launch()),
-                                     C.byref(free_launch or launch()), phases, C.byref(g)),
-              "pbm_graph_create")
+        check(lib().pbm_graph_create_steps(pool.h, n, req_base, _ptr(ids), _ptr(steps),
+                                           C.byref(get_launch or launch()),
+                                           C.byref(free_launch or launch()), phases, n_steps,
+                                           C.byref(g)),
+              "pbm_graph_create_steps")
         self.g = g
         self.pool = pool  # keeps the pool alive while the graph exists
 
@@ -580,6 +589,13 @@ class Server:
     def run(self, n_steps: int, stream=None) -> None:
         check(lib().pbm_serve_run(self.s, n_steps, _stream(stream)), "pbm_serve_run")
 
+    def run_timed(self, n_steps: int, stream=None) -> dict:
+        """Runs n_steps steps; the mean kernel-own ms of admit, grow, release."""
+        out = (C.c_float * 3)()
+        check(lib().pbm_serve_run_timed(self.s, n_steps, out, _stream(stream)),
+              "pbm_serve_run_timed")
+        return {"admit": float(out[0]), "grow": float(out[1]), "release": float(out[2])}
+
     def graph(self, n_steps: int) -> Graph:
         g = C.c_void_p()
         check(lib().pbm_serve_graph(self.s, n_steps, C.byref(g)), "pbm_serve_graph")
@@ -601,6 +617,16 @@ class Server:
                                 for k in ("admit_step", "finish_step", "length", "checksum")),
             _stream(stream)), "pbm_serve_read_results")
         return out
+
+    def wait_ticks(self, first: int = 0, n: int | None = None, stream=None) -> np.ndarray:
+        """Per sequence: wall-clock ticks from arrival to admission
+        (0xFFFFFFFF: not admitted)."""
+        if n is None:
+            n = self.cfg.max_seqs - first
+        buf = np.zeros(max(n, 1), dtype=np.uint32)
+        check(lib().pbm_serve_read_waits(self.s, first, n, buf.ctypes.data_as(C.c_void_p),
+                                         _stream(stream)), "pbm_serve_read_waits")
+        return buf[:n]
 
     def latency_ticks(self, n: int, stream=None) -> np.ndarray:
         buf = np.zeros(max(n, 1), dtype=np.uint32)
@@ -649,6 +675,14 @@ class Cache:
         """refs: a device int32 tensor (or address) of logical page ids."""
         n = refs.numel() if n is None else n
         check(lib().pbm_cache_run(self.k, _ptr(refs), n, _stream(stream)), "pbm_cache_run")
+
+    def run_timed(self, refs, n: int | None = None, stream=None) -> dict:
+        """run(), with the mean kernel-own ms of hits, plan and load per batch."""
+        n = refs.numel() if n is None else n
+        out = (C.c_float * 3)()
+        check(lib().pbm_cache_run_timed(self.k, _ptr(refs), n, out, _stream(stream)),
+              "pbm_cache_run_timed")
+        return {"hits": float(out[0]), "plan": float(out[1]), "load": float(out[2])}
 
     def stats(self, stream=None) -> dict:
         st = CacheStats()

@@ -124,18 +124,26 @@ def test_cache_frame_policies(gpu, strategy):
 
 
 def test_cache_c3_sizes(gpu):
-    # 16,777,216 frames (page map only), 4,194,304 logical pages, 1,048,576
-    # slots, Zipf(1.1) seed 7, batches of 65,536; 16M references
-    T, L, C, B, n = 1 << 24, 1 << 22, 1 << 20, 1 << 16, 1 << 24
-    log_case("cache C3 sizes")
+    # the buffer-cache workload whole: 16,777,216 frames (page map only),
+    # 4,194,304 logical pages, 1,048,576 slots, Zipf(1.1) seed 7, batches
+    # of 65,536, all 67,108,864 references
+    T, L, C, B, n = 1 << 24, 1 << 22, 1 << 20, 1 << 16, 1 << 26
+    log_case("cache C3 sizes, 64M references")
     refs = _trace(gpu, L, 1.1, 7, n)
+    h_refs = refs.cpu().numpy()
     cc = orc.ClockCache(L, C, B, log_cap=n)
-    cc.run(refs.cpu().numpy())
+    cc.run(h_refs)
     with pbm.Pool(T, strategy=pbm.CRW_BM, seed=7) as pool:
         with pbm.Cache(pool, L, C, B, log_capacity=n) as cache:
             cache.run(refs)
             gpu.cuda.synchronize()
             _check(gpu, pool, cache, cc, T)
+    # the textbook CLOCK (one reference at a time) on the same trace: the
+    # batch rule's hit count lies within 0.01% of the references of it
+    tb = orc.ClockCache(L, C, 1)
+    tb.run_textbook(h_refs)
+    assert int(tb.stats[0] + tb.stats[1]) == n
+    assert abs(int(cc.stats[0]) - int(tb.stats[0])) < n // 10000
 
 
 def test_cache_rejects_bad_configs(gpu):

@@ -72,3 +72,31 @@ def test_walk_graph_replays_keep_the_invariants(gpu):
             gpu.cuda.synchronize()
             assert np.array_equal(pool.snapshot(), before), k
         g.close()
+
+
+def test_multi_step_graph_replays_match_the_oracle(gpu):
+    # 16 alloc+free steps in one graph: each replay runs all of them, the
+    # ids are the last step's (the oracle's rank answer), the map comes back
+    T, n, S = 1 << 20, 262144, 16
+    log_case("rank graph of 16 steps")
+    rank = pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK)
+    free = pbm.launch(pbm.MODE_BATCHED)
+    with pbm.Pool(T, strategy=pbm.RW_BM, seed=42) as pool:
+        pool.prefill_bernoulli(0.9, 42)
+        before = pool.snapshot()
+        eids, _ = orc.State(before, T).batched_rank(n)
+        granted = int((eids >= 0).sum())
+        ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
+        pool.clear_counters()
+        with pbm.Graph(pool, n, ids, None, 0, rank, free, pbm.PHASE_ALLOC | pbm.PHASE_FREE,
+                       n_steps=S) as g:
+            for k in range(3):
+                ids.fill_(-7)
+                g.launch()
+                gpu.cuda.synchronize()
+                assert np.array_equal(ids.cpu().numpy(), eids), k
+                assert np.array_equal(pool.snapshot(), before), k
+        c = pool.counters()
+        assert c["allocs"] == c["frees"] == 3 * S * granted and c["errors"] == 0
+        with pytest.raises(pbm.PbmError):
+            pbm.Graph(pool, n, ids, None, 0, rank, free, 3, n_steps=0)

@@ -118,6 +118,67 @@ def test_serve_graph_steps_equal_eager_steps(gpu):
     assert sa == sb
 
 
+def test_serve_live_submission_matches_the_oracle_schedule(gpu):
+    # sequences are submitted block by block while earlier steps are still
+    # queued or running (no host synchronisation in between): each block
+    # is submitted before the steps of its arrivals are enqueued, so the
+    # schedule is the one of the whole trace submitted up front
+    T, n, B = 1 << 16, 3000, 25
+    log_case("serve live submission")
+    arr, tgt = _trace(n, 10000, 60, 0.8, 2048, seed=21)
+    keys = (np.arange(n) * 31 + 5).astype(np.uint32)
+    with pbm.Pool(T, 64, strategy=pbm.RW_BM, backing=True, seed=4) as pool:
+        pool.prefill_bernoulli(0.5, 9)
+        A = pool.free_count()
+        adm, fin, tot = orc.serve_sim(A, 512, arr, tgt, 100000)
+        steps = int(fin.max()) + 2
+        with pbm.Server(pool, 512, 2048, n, pbm.SERVE_PAYLOAD, seed=SEED) as srv:
+            i = done = 0
+            while done < steps:
+                j = int(np.searchsorted(arr, done + B, side="right"))
+                if j > i:
+                    srv.submit(arr[i:j], tgt[i:j], keys[i:j])
+                    i = j
+                srv.run(B)
+                done += B
+            res = srv.results()
+            st = srv.stats()
+            waits = srv.wait_ticks()
+        assert np.array_equal(res["admit_step"], adm)
+        assert np.array_equal(res["finish_step"], fin)
+        _check_results(res, keys, tgt)
+        assert st["allocs"] == tot[0] and st["frees"] == tot[1] and st["payload_errors"] == 0
+        assert pool.free_count() == A
+        # admitted at their arrival step: no wait; queued: a positive wait,
+        # longer for the ones that queued longer
+        assert (waits != 0xFFFFFFFF).all()
+        on_time = adm == arr
+        assert (waits[on_time] == 0).all()
+        late = adm > arr
+        assert late.any() and (waits[late] > 0).all()
+        d = adm[late] - arr[late]
+        assert np.median(waits[late][d >= 10]) > np.median(waits[late][d <= 2])
+
+
+def test_serve_sequences_submitted_after_their_arrival_step_wait_for_the_next_step(gpu):
+    log_case("serve late submission")
+    with pbm.Pool(1 << 12, 64, strategy=pbm.RW_BM, backing=True, seed=1) as pool:
+        with pbm.Server(pool, 16, 64, 8, pbm.SERVE_PAYLOAD, seed=SEED) as srv:
+            srv.submit([1, 2], [3, 3])
+            srv.run(10)
+            srv.submit([4, 4, 12], [2, 5, 1])   # steps 1..10 are already on the stream
+            srv.run(10)
+            res = srv.results(0, 5)
+            waits = srv.wait_ticks(0, 5)
+            hz = srv.clock_hz()
+    assert res["admit_step"].tolist() == [1, 2, 11, 11, 12]
+    assert res["finish_step"].tolist() == [3, 4, 12, 15, 12]
+    # the late ones waited from their submission to step 11's start: a
+    # positive span, and well under a second
+    assert waits[0] == 0 and waits[1] == 0 and waits[4] == 0
+    assert 0 < waits[2] < hz and 0 < waits[3] < hz
+
+
 def test_serve_latency_samples_and_clock(gpu):
     log_case("serve latency")
     arr, tgt = _trace(400, 5000, 30, 0.5, 256, seed=1)

@@ -844,6 +844,35 @@ def test_clock_batch_of_one_is_textbook_clock(C, n, seed):
     assert cc.refbit[:len(bits)].tolist() == bits
 
 
+@pytest.mark.parametrize("C,n,seed", [(7, 400, 1), (64, 5000, 2), (300, 20000, 3)])
+def test_linear_textbook_clock_equals_both_restatements(C, n, seed):
+    # orc_clock_textbook against the Python textbook CLOCK and against the
+    # batched oracle at batch 1: hits, evictions in order, hand and frames
+    L = 4 * C
+    refs = orc.zipf_trace(L, 1.1, seed, 0, n)
+    hits, evicted, hand, frames, bits = _textbook_clock(refs, C)
+    lin = orc.ClockCache(L, C, 1, log_cap=n)
+    lin.run_textbook(refs)
+    assert int(lin.stats[0]) == hits and lin.evictions.tolist() == evicted
+    assert int(lin.arm_res[0]) == hand % C and lin.owner[:len(frames)].tolist() == frames
+    assert lin.refbit[:len(bits)].tolist() == bits
+    bat = orc.ClockCache(L, C, 1, log_cap=n)
+    bat.run(refs)
+    assert np.array_equal(bat.stats, lin.stats) and np.array_equal(bat.evictions, lin.evictions)
+
+
+def test_linear_textbook_clock_runs_in_pieces_and_skips_out_of_range():
+    L, C = 500, 50
+    refs = orc.zipf_trace(L, 0.9, 4, 0, 6000)
+    one = orc.ClockCache(L, C, 1, log_cap=6000)
+    one.run_textbook(refs)
+    two = orc.ClockCache(L, C, 1, log_cap=6000)
+    two.run_textbook(refs[:2500])
+    two.run_textbook(np.concatenate([refs[2500:4000], [-1, L, L + 7]]))
+    two.run_textbook(refs[4000:])
+    assert np.array_equal(one.stats, two.stats) and np.array_equal(one.evictions, two.evictions)
+
+
 def test_clock_textbook_hand_example():
     # 3 frames, references 1 2 3 4 1 2 5 1 2 3 4 5 (Belady's string):
     # second-chance CLOCK gives 9 faults here
