@@ -898,6 +898,17 @@ def main() -> int:
     cache = (cache_leg(torch, pbm, stream, args.cache_refs, args.cache_batch)
              if args.cache_refs > 0 else None)
 
+    # ---- achievable HBM bandwidth: a 1 GiB device-to-device copy (read +
+    # write bytes over the copy's time), the ceiling the rooflines' 8 TB/s
+    # nominal peak is checked against
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    ms_copy = time_region(torch, lambda: dst.copy_(src), 5, 2, stream)
+    hbm_copy = {"bytes": 2 * src.numel() * 4, "ms": ms_copy,
+                "gbs": 2 * src.numel() * 4 / (ms_copy * 1e-3) / 1e9}
+    del src, dst
+    progress(f"hbm copy: {hbm_copy['gbs']:.0f} GB/s")
+
     # ---- CPU baseline: reference policy, serial, same pre-fill and requests
     cpu = None
     if args.cpu_seconds > 0:
@@ -986,6 +997,7 @@ def main() -> int:
         "fill_sweep": sweep,
         "small_pool": c1,
         "rank_at_scale": rank_scale,
+        "hbm_copy": hbm_copy,
     }
     print(json.dumps(line), flush=True)
     if args.copy_mib > 0:

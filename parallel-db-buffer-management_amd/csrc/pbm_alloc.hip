@@ -111,6 +111,7 @@ __global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t 
 constexpr u64 kScanIncl = 1ull << 62;
 constexpr uint32_t kScanSat = 0x7FFFFFFFu;
 constexpr uint32_t kScanSpin = 1u << 20;  // look-back polls before giving up
+constexpr uint32_t kRankStage = 4096;     // ids staged in LDS per pass (large tiles)
 
 __device__ __forceinline__ u64 scan_word(uint32_t parity, bool incl, uint32_t grant, u64 v) {
     const uint32_t sv = v > kScanSat ? kScanSat : (uint32_t)v;
@@ -263,18 +264,55 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t n, uint64_t
         }
         rk += pc;
     }
-    // ids: the granted bits in rank order, each thread its own
+    // ids: the granted bits in rank order
     uint32_t r = base;
+    if constexpr (WPT >= 4) {
+        // large tiles (thousands of grants): the tile's ids go through LDS
+        // a stage at a time and out in consecutive 4-byte stores, so every
+        // wave writes whole lines instead of one scattered id per lane
+        __shared__ int32_t s_ids[kRankStage];
+        uint32_t mine = 0;
 #pragma unroll
-    for (int k = 0; k < WPT; ++k) {
-        u64 m = mask[k];
-        const uint32_t wb = (w0 + k) * 64u;
-        while (m) {
-            const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
-            m &= m - 1ull;
-            ids[r] = (int32_t)(wb + bit);
-            if (steps_out) steps_out[r] = 1u;
-            ++r;
+        for (int k = 0; k < WPT; ++k) mine += (uint32_t)__popcll(mask[k]);
+        const uint32_t t_lo = s_excl;
+        const uint32_t t_hi = grant > s_excl ? (grant - s_excl < agg ? grant : s_excl + agg) : s_excl;
+        for (uint32_t lo = t_lo; lo < t_hi; lo += kRankStage) {
+            const uint32_t hi = t_hi - lo < kRankStage ? t_hi : lo + kRankStage;
+            if (base < hi && base + mine > lo) {
+                uint32_t q = base;
+#pragma unroll
+                for (int k = 0; k < WPT; ++k) {
+                    u64 m = mask[k];
+                    const uint32_t wb = (w0 + k) * 64u;
+                    while (m && q < hi) {
+                        const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
+                        m &= m - 1ull;
+                        if (q >= lo) s_ids[q - lo] = (int32_t)(wb + bit);
+                        ++q;
+                    }
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < hi - lo; i += THREADS) {
+                ids[lo + i] = s_ids[i];
+                if (steps_out) steps_out[lo + i] = 1u;
+            }
+            __syncthreads();
+        }
+        r = base + mine;
+    } else {
+        // each thread its own
+#pragma unroll
+        for (int k = 0; k < WPT; ++k) {
+            u64 m = mask[k];
+            const uint32_t wb = (w0 + k) * 64u;
+            while (m) {
+                const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
+                m &= m - 1ull;
+                ids[r] = (int32_t)(wb + bit);
+                if (steps_out) steps_out[r] = 1u;
+                ++r;
+            }
         }
     }
     st.allocs += r - base;
