@@ -597,6 +597,151 @@ def rank_at_scale_leg(torch, pbm, stream, n_pages: int, fill: float, n_req: int,
                          "free_gbs": free_bytes / (own["free"] * 1e-3) / 1e9}}
 
 
+SWEEP_PAGES, SWEEP_PAGE_SIZE = 1 << 24, 512  # the reference's default pool: 8 GiB of 512 B pages
+SWEEP_FREE = [round(0.1 - 0.001 * i, 3) for i in range(96)]  # unitTest1.cu:17 (0.1 .. 0.005)
+SWEEP_THREADS = [1 << i for i in range(21)]                   # unitTest1.5.cu:16
+SWEEP_SIZES = [4, 16, 64, 128] + list(range(192, 257, 64)) + list(range(512, 8193, 256))
+
+
+def reference_sweeps_leg(torch, pbm, stream, samples: int) -> dict:
+    """The reference's three measurement grids on its default pool
+    (16,777,216 pages of 512 B, Paging.cuh:4-7), with `samples` samples per
+    point where the reference takes 96, 30 and 100:
+
+    free_fraction (unitTests/unitTest1.cu:14-17, 95-125): 5,000 one-page
+      requests at each free fraction 0.1, 0.099, .., 0.005, for the three
+      bitmap walks.  The pool is filled to each point by the strategy's own
+      plain getPage requests, (int)((1 - p) T) of them as prefillBuffer
+      counts them; each sample restores that map and walks with new
+      request streams.  TAS, WAS (64- and 32-lane groups) and the kernel
+      time, beside the closed forms of models.sweep at the same T, A, N.
+    free_time (unitTests/unitTest1.5.cu:14-16, 66-87): N one-page grants on
+      the empty pool, then the timed free of all N, N = 1 .. 1,048,576; for
+      the array walk RandomWalkBasic and the bitmap RandomWalkBasic_BM
+      (plain, one thread per page) and the batched bitmap free.
+    getnpage_size (unitTests/unitTest2.cu:12-18, 70-92): 10,000 and 100,000
+      RWMalloc getNPage requests of ceil(size / 512) pages each on the empty
+      pool, sizes 4 B .. 8 KiB; the reference's window (one word per lane)
+      and the 256-word window.
+
+    Times are events recorded on the stream around each launch, as the
+    reference's KernelTiming brackets its kernels.  Every sample checks
+    its grants: min(N, A) of them, no page twice, the map restored."""
+    import models
+    T = SWEEP_PAGES
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn) -> float:
+        ev0.record(stream)
+        fn()
+        ev1.record(stream)
+        ev1.synchronize()
+        return ev0.elapsed_time(ev1)
+
+    ok = True
+    plain = pbm.launch(mode=pbm.MODE_PLAIN)
+    plain_q = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
+
+    # ---- free fraction grid, 5,000 requests
+    n_req = 5000
+    ff = {"pages": T, "page_size": SWEEP_PAGE_SIZE, "threads": n_req, "samples": samples,
+          "free": SWEEP_FREE}
+    ids = torch.full((n_req,), -1, dtype=torch.int32, device="cuda")
+    stp = torch.zeros((n_req,), dtype=torch.int32, device="cuda")
+    fill_ids = torch.empty((T,), dtype=torch.int32, device="cuda")
+    for name, strategy in (("RandomWalkBasic_BM", pbm.RW_BM), ("CollabRW_BM", pbm.COLLAB_RW_BM),
+                           ("CRW_BM", pbm.CRW_BM)):
+        cols = {"tas": [], "was64": [], "was32": [], "ms": []}
+        with pbm.Pool(T, SWEEP_PAGE_SIZE, strategy=strategy, seed=SEED) as pool:
+            filled = 0
+            for p in SWEEP_FREE:
+                target = int(np.float32(1.0 - np.float32(p)) * np.float32(T))
+                if target > filled:
+                    pool.get_pages(target - filled, fill_ids, None, (0xFFFE << 48) + filled,
+                                   plain_q, stream)
+                    filled = target
+                words = pool.snapshot(stream)
+                A = pool.free_count(stream)
+                tas, w64, w32, ms = [], [], [], []
+                for k in range(samples):
+                    ids.fill_(-1)
+                    ms.append(timed(lambda: pool.get_pages(n_req, ids, stp, k * n_req, plain,
+                                                           stream)))
+                    i_np = ids.cpu().numpy()
+                    ws = walk_stats(stp.cpu().numpy().view(np.uint32), i_np)
+                    g = i_np[i_np >= 0]
+                    ok = ok and g.size == min(n_req, A) and np.unique(g).size == g.size
+                    tas.append(ws["tas"])
+                    w64.append(ws["was64"])
+                    w32.append(ws["was32"])
+                    pool.restore(words, stream)
+                cols["tas"].append(float(np.mean(tas)))
+                cols["was64"].append(float(np.mean(w64)))
+                cols["was32"].append(float(np.mean(w32)))
+                cols["ms"].append(float(np.mean(ms)))
+            ok = ok and filled == int(np.float32(1.0 - np.float32(SWEEP_FREE[-1])) * np.float32(T))
+        ff[name] = cols
+        progress(f"reference sweeps: free fraction grid, {name}")
+    del fill_ids
+    rows = models.sweep(T, n_req, [1.0 - p for p in SWEEP_FREE])
+    ff["model"] = {k: [r[k] for r in rows] for k in
+                   ("RW_TAS", "RWBM_TAS", "RWBM_WAS", "CoRWBM_WAS")}
+
+    # ---- timed free over 1 .. 1,048,576 threads
+    ft = {"pages": T, "threads": SWEEP_THREADS, "samples": samples}
+    big = torch.full((SWEEP_THREADS[-1],), -1, dtype=torch.int32, device="cuda")
+    batched = pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_NO_COUNTERS)
+    for name, strategy, fl in (("RandomWalkBasic", pbm.RW_BASIC, plain_q),
+                               ("RandomWalkBasic_BM", pbm.RW_BM, plain_q),
+                               ("batched_bitmap_free", pbm.RW_BM, batched)):
+        col = []
+        with pbm.Pool(T, SWEEP_PAGE_SIZE, strategy=strategy, seed=SEED) as pool:
+            for n in SWEEP_THREADS:
+                ms = []
+                for k in range(samples):
+                    pool.get_pages(n, big, None, k * n, plain_q, stream)
+                    ms.append(timed(lambda: pool.free_pages(n, big, fl, stream)))
+                col.append(float(np.mean(ms)))
+            ok = ok and pool.free_count(stream) == T
+        ft[name] = col
+        progress(f"reference sweeps: timed free, {name}")
+    del big
+
+    # ---- getNPage over request sizes, 10,000 and 100,000 requests
+    gs = {"pages": T, "page_size": SWEEP_PAGE_SIZE, "sizes": SWEEP_SIZES,
+          "n_pages": [-(-s_ // SWEEP_PAGE_SIZE) for s_ in SWEEP_SIZES], "samples": samples}
+    with pbm.Pool(T, SWEEP_PAGE_SIZE, strategy=pbm.RW_MALLOC, seed=SEED) as pool:
+        for n_req in (10000, 100000):
+            ids = torch.full((n_req,), -1, dtype=torch.int32, device="cuda")
+            stp = torch.zeros((n_req,), dtype=torch.int32, device="cuda")
+            cnt = torch.empty((n_req,), dtype=torch.int32, device="cuda")
+            for name, gl in (("plain", plain),
+                             ("plain_wide", pbm.launch(mode=pbm.MODE_PLAIN, cluster_words=256))):
+                cols = {"tas": [], "was64": [], "ms": []}
+                for npg in gs["n_pages"]:
+                    cnt.fill_(npg)
+                    tas, w64, ms = [], [], []
+                    for k in range(samples):
+                        pool.reset(stream)
+                        ids.fill_(-1)
+                        ms.append(timed(lambda: pool.get_n_pages(n_req, cnt, ids, stp, k * n_req,
+                                                                 gl, stream)))
+                        i_np = ids.cpu().numpy()
+                        ws = walk_stats(stp.cpu().numpy().view(np.uint32), i_np)
+                        g = i_np[i_np >= 0]
+                        ok = (ok and g.size == n_req
+                              and pool.free_count(stream) == T - n_req * npg)
+                        tas.append(ws["tas"])
+                        w64.append(ws["was64"])
+                    cols["tas"].append(float(np.mean(tas)))
+                    cols["was64"].append(float(np.mean(w64)))
+                    cols["ms"].append(float(np.mean(ms)))
+                gs[f"{name}_{n_req}"] = cols
+                progress(f"reference sweeps: getNPage sizes, {name}, {n_req} requests")
+        pool.reset(stream)
+    return {"free_fraction": ff, "free_time": ft, "getnpage_size": gs, "verified": bool(ok)}
+
+
 def cpu_legs(orc, c1, bursts, cache, serving, seconds: float) -> None:
     """The CPU baseline of each configuration leg: the oracle's serial
     restatement of the reference's policy on one host core, over the same
@@ -708,6 +853,9 @@ def main() -> int:
     ap.add_argument("--rank-scale", type=int, default=1,
                     help="time the rank allocation on 67,108,864 pages at 90%% fill with "
                          "4,194,304 requests (0 = skip)")
+    ap.add_argument("--ref-sweeps", type=int, default=4,
+                    help="samples per point of the reference's free-fraction, timed-free and "
+                         "getNPage-size grids (0 skips them)")
     ap.add_argument("--kmalloc", type=int, default=30000,
                     help="requests of the kernel-malloc leg (0 = skip)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -853,6 +1001,8 @@ def main() -> int:
         rank_scale = rank_at_scale_leg(torch, pbm, stream, 1 << 26, 0.9, 1 << 22,
                                        max(3, min(args.steps, 50)))
         progress("rank allocation at 67,108,864 pages measured")
+    ref_sweeps = (reference_sweeps_leg(torch, pbm, stream, args.ref_sweeps)
+                  if args.ref_sweeps > 0 else None)
 
     b = results["batched"]
     ms_b = b["ms_per_step"]
@@ -997,6 +1147,7 @@ def main() -> int:
         "fill_sweep": sweep,
         "small_pool": c1,
         "rank_at_scale": rank_scale,
+        "reference_sweeps": ref_sweeps,
         "hbm_copy": hbm_copy,
     }
     print(json.dumps(line), flush=True)
@@ -1012,6 +1163,7 @@ def main() -> int:
                  and (bursts is None or bursts["verified"])
                  and (cache is None or cache["verified"])
                  and (rank_scale is None or rank_scale["verified"])
+                 and (ref_sweeps is None or ref_sweeps["verified"])
                  and (kmalloc is None or (kmalloc["collab"]["verified"]
                                           and kmalloc["simple"]["verified"]))
                  and (sweep is None or all(v["verified"] for row in sweep.values()

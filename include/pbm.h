@@ -139,8 +139,9 @@ typedef struct pbm_launch {
     uint32_t waves_per_cu;  /* batched: persistent waves per CU (0 = 8)                  */
     uint32_t walk_budget;   /* 0 = the pool's walk_budget                                */
     uint32_t cluster_words; /* batched: words per window, 64 or 128 or 256 (0 = 64);
-                               mallocRW bursts: getNPage's window (64 = the reference's
-                               one word per lane)                                       */
+                               getNPage, mallocRW and their bursts: the words one
+                               search round reads (64 = the reference's one word per
+                               lane)                                                    */
     uint32_t flags;         /* PBM_LAUNCH_* bits                                         */
 } pbm_launch;
 
@@ -693,6 +694,54 @@ int pbm_bursts_wait_timeouts(pbm_bursts* b, unsigned long long* out, void* strea
 int pbm_bursts_autotune(pbm_bursts* b, uint32_t count, uint32_t flags, pbm_tune_result* out,
                         void* stream);
 
+
+/* ---- debug: owner shadow ---------------------------------------------- */
+
+/* What the owner shadow has seen since pbm_shadow_enable. */
+typedef struct pbm_shadow_report {
+    uint64_t double_grants; /* grants of a page the shadow did not hold as free   */
+    uint64_t bad_frees;     /* frees of a page it held as free, or past n_pages   */
+    int64_t  grant_page;    /* one double-granted page (-1 = none)                */
+    uint64_t grant_owner;   /* the request number that was granted it            */
+    uint64_t grant_holder;  /* the request number holding it then (PBM_SHADOW_HELD:
+                               held before the shadow last read the map;
+                               PBM_SHADOW_FREE: a page past n_pages)            */
+    int64_t  free_page;     /* one page of a bad free (-1 = none)                 */
+} pbm_shadow_report;
+
+#define PBM_SHADOW_FREE 0xFFFFFFFFFFFFFFFFull  /* owner word of a free page        */
+#define PBM_SHADOW_HELD 0xFFFFFFFFFFFFFFFEull  /* owner word of a page held before
+                                                  the shadow last read the map     */
+
+/* Debug aid with no reference counterpart (the reference checks only the
+ * final map; a double grant shows there only as a lost page).  on != 0:
+ * keeps one owner word per page (8 bytes x n_pages of device memory), set
+ * from the map, and clears the report; on = 0 frees it.  While on, every
+ * successful pbm_get_pages, pbm_free_pages, pbm_get_n_pages,
+ * pbm_get_x_pages and pbm_free_n_pages launches one more kernel on the same
+ * stream after its own: each granted page moves from free to its request
+ * number (req_base + i) by compare-and-swap, so a page granted twice is
+ * reported at the call that granted it the second time; each freed page
+ * moves back to free.  pbm_reset, pbm_prefill_bernoulli and
+ * pbm_restore_bitmap set the owners from the new map.  Other calls that
+ * grant or free pages (mallocRW, bursts, the cache, the server) do not
+ * update it: call pbm_shadow_enable(h, 1) again after them, or record
+ * their pages with pbm_shadow_note.  Calls on several streams at once see
+ * the owners in no defined order.  Bitmap strategies only. */
+int pbm_shadow_enable(pbm_handle* h, int on, void* stream);
+
+/* Records the grants (freeing = 0) or frees (freeing = 1) of n requests
+ * (d_ids[i], d_counts[i] pages; d_counts null = one page each; ids < 0 or
+ * counts 0 skipped) made outside the entry points above, for callers with
+ * kernels of their own. */
+int pbm_shadow_note(pbm_handle* h, uint32_t n, uint64_t req_base, const int32_t* d_ids,
+                    const uint32_t* d_counts, int freeing, void* stream);
+
+/* Reads the report (synchronizes the stream). */
+int pbm_shadow_read(pbm_handle* h, pbm_shadow_report* out, void* stream);
+
+/* Copies the owner words (`words` must equal n_pages) to host memory. */
+int pbm_shadow_owners(pbm_handle* h, uint64_t* host, size_t words, void* stream);
 
 /* ---- measurement ------------------------------------------------------ */
 

@@ -717,9 +717,8 @@ extern "C" int pbm_event_elapsed_ms(void* start_event, void* stop_event, float* 
     return PBM_OK;
 }
 
-extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
-                             uint32_t* d_steps, const pbm_launch* launch, void* stream) {
-    PBM_CHECK_HANDLE(h);
+static int get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
+                     uint32_t* d_steps, const pbm_launch* launch, void* stream) {
     Disarm disarm{h};
     if (n == 0) return PBM_OK;
     if (!d_ids) return pbmi::fail(PBM_ERR_INVALID, "null d_ids");
@@ -802,6 +801,14 @@ extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32
     return PBM_OK;
 }
 
+extern "C" int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
+                             uint32_t* d_steps, const pbm_launch* launch, void* stream) {
+    PBM_CHECK_HANDLE(h);
+    int rc = get_pages(h, n, req_base, d_ids, d_steps, launch, stream);
+    if (rc) return rc;
+    return pbmi::shadow_note(h, n, req_base, d_ids, nullptr, false, pbmi::as_stream(stream));
+}
+
 extern "C" int pbm_collect_warp_data(pbm_handle* h, uint32_t n, uint64_t req_base,
                                      int32_t* d_ids, uint32_t* d_yields, uint32_t max_steps,
                                      uint32_t* d_n_steps, const pbm_launch* launch,
@@ -840,6 +847,8 @@ extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
     if (rc) return rc;
     hipStream_t s = pbmi::as_stream(stream);
     pbm_ctx c = launch_ctx(h, L);
+    // the shadow sees the frees before the kernel makes the pages grantable
+    if ((rc = pbmi::shadow_note(h, n, 0, d_ids, nullptr, true, s))) return rc;
     if (pbmi::is_queue_strategy(h->cfg.strategy)) {
         if (L.mode == PBM_MODE_BATCHED)
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");

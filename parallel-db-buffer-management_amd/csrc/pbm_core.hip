@@ -137,6 +137,7 @@ int recount_free(pbm_handle* h, hipStream_t s) {
 
 int reinit_strategy_state(pbm_handle* h, hipStream_t s) {
     const pbm_ctx& c = h->ctx;
+    if (int rc = shadow_sync(h, s)) return rc;
     if (h->d_hint)
         hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(c.max_threads, 256)), dim3(256), 0, s,
                            h->d_hint, c.max_threads, -1);
@@ -286,7 +287,8 @@ extern "C" int pbm_destroy(pbm_handle* h) {
     (void)hipDeviceSynchronize();
     void* ptrs[] = {h->d_bitmap, h->d_free, h->d_counters, h->d_pool, h->d_hint, h->d_arms,
                     h->d_clock, h->d_ring, h->d_ring_ht, h->d_phase, h->d_scratch,
-                    h->d_scan, h->d_scan_ctl, h->d_fresh, h->d_carve, h->d_qavail};
+                    h->d_scan, h->d_scan_ctl, h->d_fresh, h->d_carve, h->d_qavail,
+                    h->d_shadow, h->d_shadow_rep};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     delete h;

@@ -35,6 +35,10 @@ struct pbm_handle {
     unsigned* d_fresh = nullptr;              // Ouroboros: free pages at the last reset
     unsigned long long* d_carve = nullptr;    // Ouroboros: [0] carved, [1] fresh pages
     long long* d_qavail = nullptr;            // Ouroboros: unpromised queued pages
+    // debug shadow of page owners (pbm_shadow_enable): one word per page and
+    // the report words; null while off
+    unsigned long long* d_shadow = nullptr;
+    unsigned long long* d_shadow_rep = nullptr;
     // events the next batch call's main kernel records at its own start and
     // end (pbm_time_next_kernel); cleared by that call
     hipEvent_t t_start = nullptr;
@@ -93,6 +97,13 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // Re-initialises strategy state after the page map changed wholesale
 // (reset, prefill, restore): hints, arms, clock, free list.
 int reinit_strategy_state(pbm_handle* h, hipStream_t s);
+
+// Shadow owners: set every page's owner from the map (held or free), and
+// record the grants or frees of a finished id-returning call.  No-ops while
+// the shadow is off.
+int shadow_sync(pbm_handle* h, hipStream_t s);
+int shadow_note(pbm_handle* h, uint32_t n, uint64_t req_base, const int32_t* d_ids,
+                const uint32_t* d_counts, bool freeing, hipStream_t s);
 
 // Recomputes the reservation counter from the page map.
 int recount_free(pbm_handle* h, hipStream_t s);

@@ -227,6 +227,9 @@ int ring_rebuild(pbm_handle* h, hipStream_t s) {
 
 // getNPage: wave-collective over chunks of 64 requests; the chunk's walk
 // seed is the seed of its first request.
+// SPANS: 64-word spans of the window each search round reads (1 = one
+// word per lane, RWMalloc.cuh:198-248).
+template <int SPANS>
 __global__ __launch_bounds__(1024) void k_get_n_pages(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                       const uint32_t* __restrict__ counts,
                                                       int32_t* __restrict__ ids,
@@ -241,7 +244,8 @@ __global__ __launch_bounds__(1024) void k_get_n_pages(pbm_ctx c, uint32_t n, uin
         if (r < n) {
             uint32_t steps = 0;
             int32_t id = pbm::get_n_pages(
-                c, counts[r], pbm::request_seed(c.seed, req_base + (uint64_t)ch * 64u), &steps, &st);
+                c, counts[r], pbm::request_seed(c.seed, req_base + (uint64_t)ch * 64u), &steps, &st,
+                SPANS);
             ids[r] = id;
             if (steps_out) steps_out[r] = steps;
         }
@@ -276,6 +280,7 @@ __global__ __launch_bounds__(1024) void k_get_x_pages(pbm_ctx c, uint32_t n, uin
     pbm::flush_stats(c, st);
 }
 
+template <int SPANS>
 __global__ __launch_bounds__(1024) void k_malloc(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                  const uint64_t* __restrict__ sizes,
                                                  uint64_t* __restrict__ ptrs,
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(1024) void k_malloc(pbm_ctx c, uint32_t n, uint64_t
             uint32_t steps = 0;
             void* p = pbm::malloc_rw(c, sizes[r],
                                      pbm::request_seed(c.seed, req_base + (uint64_t)ch * 64u),
-                                     &steps, &st);
+                                     &steps, &st, SPANS);
             ptrs[r] = (uint64_t)p;
             if (steps_out) steps_out[r] = steps;
         }
@@ -318,6 +323,9 @@ int prep(pbm_handle* h, const pbm_launch* in, pbm_launch* L, pbm_ctx* c) {
         return pbmi::fail(PBM_ERR_INVALID, "launch.block must be a multiple of 64, at most 1024");
     if (x.mode != PBM_MODE_PLAIN && x.mode != PBM_MODE_SERIAL)
         return pbmi::fail(PBM_ERR_UNSUPPORTED, "multi-page calls run plain or serial");
+    if (x.cluster_words == 0) x.cluster_words = 64;
+    if (x.cluster_words != 64 && x.cluster_words != 128 && x.cluster_words != 256)
+        return pbmi::fail(PBM_ERR_INVALID, "launch.cluster_words must be 64, 128 or 256");
     *L = x;
     *c = h->ctx;
     if (x.walk_budget) c->walk_budget = x.walk_budget;
@@ -353,10 +361,12 @@ extern "C" int pbm_get_n_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
     pbm_ctx c;
     int rc = prep(h, launch, &L, &c);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_get_n_pages, wave_grid(L, n), wave_block(L), 0, pbmi::as_stream(stream),
-                       c, n, req_base, d_counts, d_ids, d_steps);
+    auto k = L.cluster_words == 256 ? k_get_n_pages<4>
+             : L.cluster_words == 128 ? k_get_n_pages<2> : k_get_n_pages<1>;
+    hipLaunchKernelGGL(k, wave_grid(L, n), wave_block(L), 0, pbmi::as_stream(stream), c, n,
+                       req_base, d_counts, d_ids, d_steps);
     PBM_HIP(hipGetLastError());
-    return PBM_OK;
+    return pbmi::shadow_note(h, n, req_base, d_ids, d_counts, false, pbmi::as_stream(stream));
 }
 
 extern "C" int pbm_get_x_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
@@ -372,7 +382,7 @@ extern "C" int pbm_get_x_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
     hipLaunchKernelGGL(k_get_x_pages, thread_grid(L, n), thread_block(L), 0,
                        pbmi::as_stream(stream), c, n, req_base, d_counts, d_ids, d_steps);
     PBM_HIP(hipGetLastError());
-    return PBM_OK;
+    return pbmi::shadow_note(h, n, req_base, d_ids, d_counts, false, pbmi::as_stream(stream));
 }
 
 extern "C" int pbm_free_n_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
@@ -384,6 +394,8 @@ extern "C" int pbm_free_n_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
     pbm_ctx c;
     int rc = prep(h, launch, &L, &c);
     if (rc) return rc;
+    if ((rc = pbmi::shadow_note(h, n, 0, d_ids, d_counts, true, pbmi::as_stream(stream))))
+        return rc;
     hipLaunchKernelGGL(k_free_n_pages, thread_grid(L, n), thread_block(L), 0,
                        pbmi::as_stream(stream), c, n, d_ids, d_counts);
     PBM_HIP(hipGetLastError());
@@ -401,7 +413,8 @@ extern "C" int pbm_malloc(pbm_handle* h, uint32_t n, uint64_t req_base, const ui
     pbm_ctx c;
     int rc = prep(h, launch, &L, &c);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_malloc, wave_grid(L, n), wave_block(L), 0, pbmi::as_stream(stream), c, n,
+    auto k = L.cluster_words == 256 ? k_malloc<4> : L.cluster_words == 128 ? k_malloc<2> : k_malloc<1>;
+    hipLaunchKernelGGL(k, wave_grid(L, n), wave_block(L), 0, pbmi::as_stream(stream), c, n,
                        req_base, d_sizes, d_ptrs, d_steps);
     PBM_HIP(hipGetLastError());
     return PBM_OK;

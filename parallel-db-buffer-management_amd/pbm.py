@@ -79,6 +79,18 @@ class TuneResult(C.Structure):
 TUNE_QUICK, TUNE_WALK_ONLY, TUNE_RANK_ONLY, TUNE_NO_PERSISTENT = 1, 2, 4, 8
 
 
+class ShadowReport(C.Structure):
+    _fields_ = [("double_grants", C.c_uint64), ("bad_frees", C.c_uint64),
+                ("grant_page", C.c_int64), ("grant_owner", C.c_uint64),
+                ("grant_holder", C.c_uint64), ("free_page", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+SHADOW_FREE, SHADOW_HELD = 0xFFFFFFFFFFFFFFFF, 0xFFFFFFFFFFFFFFFE
+
+
 class ServeConfig(C.Structure):
     _fields_ = [("max_slots", C.c_uint32), ("max_pages", C.c_uint32), ("max_seqs", C.c_uint32),
                 ("flags", C.c_uint32), ("lat_capacity", C.c_uint32), ("reserved", C.c_uint32),
@@ -138,6 +150,11 @@ _SIGS = {
     "pbm_snapshot_bitmap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "pbm_restore_bitmap": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "pbm_read_free_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_void_p]),
+    "pbm_shadow_enable": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "pbm_shadow_note": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p,
+                                  C.c_int, C.c_void_p]),
+    "pbm_shadow_read": (C.c_int, [C.c_void_p, C.POINTER(ShadowReport), C.c_void_p]),
+    "pbm_shadow_owners": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "pbm_get_ctx": (C.c_int, [C.c_void_p, C.POINTER(Ctx)]),
     "pbm_read_counters": (C.c_int, [C.c_void_p, C.POINTER(Counters), C.c_void_p]),
     "pbm_clear_counters": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -338,6 +355,27 @@ class Pool:
         out = np.empty(self.n_words, dtype=np.uint64)
         check(lib().pbm_snapshot_bitmap(self.h, out.ctypes.data_as(C.c_void_p), self.n_words,
                                         _stream(stream)), "pbm_snapshot_bitmap")
+        return out
+
+    def shadow_enable(self, on: bool = True, stream=None) -> None:
+        """Debug owner shadow on (set from the map, report cleared) or off."""
+        check(lib().pbm_shadow_enable(self.h, int(bool(on)), _stream(stream)),
+              "pbm_shadow_enable")
+
+    def shadow_note(self, n: int, ids, counts=None, req_base: int = 0, freeing: bool = False,
+                    stream=None) -> None:
+        check(lib().pbm_shadow_note(self.h, n, req_base, _ptr(ids), _ptr(counts),
+                                    int(bool(freeing)), _stream(stream)), "pbm_shadow_note")
+
+    def shadow_report(self, stream=None) -> dict:
+        r = ShadowReport()
+        check(lib().pbm_shadow_read(self.h, C.byref(r), _stream(stream)), "pbm_shadow_read")
+        return r.as_dict()
+
+    def shadow_owners(self, stream=None) -> np.ndarray:
+        out = np.empty(self.n_pages, dtype=np.uint64)
+        check(lib().pbm_shadow_owners(self.h, out.ctypes.data_as(C.c_void_p), self.n_pages,
+                                      _stream(stream)), "pbm_shadow_owners")
         return out
 
     def restore(self, words: np.ndarray, stream=None) -> None:

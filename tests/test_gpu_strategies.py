@@ -171,20 +171,22 @@ def _counts(rng, n, hi):
     return c
 
 
-@pytest.mark.parametrize("fill,hi", [(0.5, 8), (0.3, 40), (0.8, 3)])
-def test_serial_getnpage_matches_oracle(gpu, fill, hi):
+@pytest.mark.parametrize("fill,hi,spans", [(0.5, 8, 1), (0.3, 40, 1), (0.8, 3, 1),
+                                           (0.5, 8, 4), (0.8, 3, 2), (0.3, 40, 4)])
+def test_serial_getnpage_matches_oracle(gpu, fill, hi, spans):
     T, n = 1 << 15, 3000
-    log_case(f"serial getNPage fill={fill} hi={hi}")
+    log_case(f"serial getNPage fill={fill} hi={hi} spans={spans}")
     counts = _counts(np.random.default_rng(1), n, hi)
     with pbm.Pool(T, strategy=pbm.RW_MALLOC, seed=23) as pool:
         pool.prefill_bernoulli(fill, 6)
         d_counts = gpu.from_numpy(counts.view(np.int32)).cuda()
         ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
         steps = gpu.zeros((n,), dtype=gpu.int32, device="cuda")
-        pool.get_n_pages(n, d_counts, ids, steps, 11, pbm.launch(mode=pbm.MODE_SERIAL))
+        pool.get_n_pages(n, d_counts, ids, steps, 11,
+                         pbm.launch(mode=pbm.MODE_SERIAL, cluster_words=64 * spans))
         gpu.cuda.synchronize()
         st = orc.State(orc.prefill(T, fill, 6), T)
-        eids, esteps = st.get_n(counts, 11, 23)
+        eids, esteps = st.get_n(counts, 11, 23, spans=spans)
         assert np.array_equal(ids.cpu().numpy(), eids)
         assert np.array_equal(steps.cpu().numpy().view(np.uint32), esteps)
         assert np.array_equal(pool.snapshot(), st.bm)
@@ -252,19 +254,21 @@ def _sizes(rng, n):
     return np.exp(rng.uniform(np.log(16), np.log(65536), size=n)).astype(np.uint64)
 
 
-def test_serial_malloc_matches_oracle(gpu):
+@pytest.mark.parametrize("spans", [1, 4])
+def test_serial_malloc_matches_oracle(gpu, spans):
     T, ps, n = 1 << 15, 4096, 3000
-    log_case("serial mallocRW/freeRW")
+    log_case(f"serial mallocRW/freeRW spans={spans}")
     sizes = _sizes(np.random.default_rng(2), n)
     with pbm.Pool(T, ps, strategy=pbm.RW_MALLOC, backing=True, seed=29) as pool:
         pool.prefill_bernoulli(0.4, 9)
         d_sizes = gpu.from_numpy(sizes.view(np.int64)).cuda()
         ptrs = gpu.zeros((n,), dtype=gpu.int64, device="cuda")
         steps = gpu.zeros((n,), dtype=gpu.int32, device="cuda")
-        pool.malloc(n, d_sizes, ptrs, steps, 0, pbm.launch(mode=pbm.MODE_SERIAL))
+        pool.malloc(n, d_sizes, ptrs, steps, 0,
+                    pbm.launch(mode=pbm.MODE_SERIAL, cluster_words=64 * spans))
         gpu.cuda.synchronize()
         st = orc.State(orc.prefill(T, 0.4, 9), T)
-        page, off, hdr, esteps = st.malloc(sizes, ps, 0, 29)
+        page, off, hdr, esteps = st.malloc(sizes, ps, 0, 29, spans=spans)
         p = ptrs.cpu().numpy().view(np.uint64).astype(np.int64)
         base = pool.page_address(0)
         # a packed group may span several pages: compare byte positions
@@ -325,16 +329,28 @@ def test_list_refills_after_running_empty(gpu):
         assert np.array_equal(np.sort(again[again >= 0]), np.sort(back))
 
 
-def test_concurrent_getnpage_runs_are_disjoint(gpu):
+def test_multipage_calls_reject_other_window_widths(gpu):
+    with pbm.Pool(4096, strategy=pbm.RW_MALLOC) as pool:
+        c = gpu.ones((64,), dtype=gpu.int32, device="cuda")
+        ids = gpu.full((64,), -1, dtype=gpu.int32, device="cuda")
+        with pytest.raises(pbm.PbmError, match="cluster_words"):
+            pool.get_n_pages(64, c, ids, None, 0, pbm.launch(mode=pbm.MODE_PLAIN,
+                                                             cluster_words=192))
+        assert (ids.cpu().numpy() == -1).all()
+
+
+@pytest.mark.parametrize("cw", [64, 256])
+def test_concurrent_getnpage_runs_are_disjoint(gpu, cw):
     T, n = 1 << 18, 30000
-    log_case("concurrent getNPage")
+    log_case(f"concurrent getNPage cluster_words={cw}")
     counts = _counts(np.random.default_rng(7), n, 17)
     with pbm.Pool(T, strategy=pbm.RW_MALLOC, seed=3) as pool:
         pool.prefill_bernoulli(0.5, 1)
         before = orc.prefill(T, 0.5, 1)
         d_counts = gpu.from_numpy(counts.view(np.int32)).cuda()
         ids = gpu.full((n,), -7, dtype=gpu.int32, device="cuda")
-        pool.get_n_pages(n, d_counts, ids)
+        pool.get_n_pages(n, d_counts, ids, None, 0,
+                         pbm.launch(mode=pbm.MODE_PLAIN, cluster_words=cw))
         gpu.cuda.synchronize()
         got = ids.cpu().numpy()
         after = pool.snapshot()

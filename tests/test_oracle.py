@@ -1324,3 +1324,47 @@ def test_cluster_count_hand_cases():
     assert orc.count_clusters(alt, T) == 64
     wrap = np.array([0xFFFFFFFFFFFFFFF0, 0x0FFFFFFFFFFFFFFF], dtype=np.uint64)  # free 0-3, 124-127
     assert orc.count_clusters(wrap, T) == 1
+
+
+def test_lcg_low_bits_give_every_walk_the_same_cycle():
+    # RNG_LCG.cuh:6-12 works modulo 2^31 with an odd increment and a
+    # multiplier = 1 mod 4: its low k bits are an LCG modulo 2^k of full
+    # period.  On a map of 2^k words (word = state % 2^k) every walk visits
+    # the words in one fixed cyclic order; two seeds only start at different
+    # places on it.
+    k = 12
+    M = 1 << k
+
+    def low_cycle(x):
+        out = []
+        for _ in range(M):
+            x = orc.lcg(x)
+            out.append(x % M)
+        return out
+
+    a, b = low_cycle(12345), low_cycle(987654321)
+    assert sorted(a) == list(range(M))
+    i = a.index(b[0])
+    assert b == a[i:] + a[:i]
+
+
+def test_request_filled_power_of_two_map_clusters_along_the_cycle():
+    # A map filled by such walks (prefillBuffer, RandomWalkBasic_BM.cuh:154-164)
+    # holds long runs of full words along the common cycle, the primary
+    # clustering of linear probing: at 1% free, later walks on 2^12 words are
+    # an order of magnitude longer than on 4,093 words, where state % L mixes
+    # the high bits in.
+    tas = {}
+    for T in (1 << 18, (1 << 18) - 192):
+        st = orc.State(orc.cleared(T), T)
+        st.rwbm(int(np.float32(0.99) * np.float32(T)), 1 << 40, 42)
+        ids, steps = st.rwbm(2000, 0, 42)
+        tas[T] = float(np.mean(steps[ids >= 0]))
+    assert tas[1 << 18] > 8 * tas[(1 << 18) - 192]
+    # a Bernoulli pre-fill of the same map shows no such gap
+    b = {}
+    for T in (1 << 18, (1 << 18) - 192):
+        st = orc.State(orc.prefill(T, 0.99, 3), T)
+        ids, steps = st.rwbm(2000, 0, 42)
+        b[T] = float(np.mean(steps[ids >= 0]))
+    assert b[1 << 18] < 2 * b[(1 << 18) - 192]
