@@ -114,8 +114,7 @@ typedef struct pbm_ctx {
     long long*          phase;       /* batched calls: [1] wave completion counter      */
     unsigned long long* scan;        /* n_tiles look-back words of the single-pass rank
                                         scan (call parity, state, grant, prefix)        */
-    unsigned long long* scan_ctl;    /* rank scan: [0] tile tickets (n_tiles per call),
-                                        [1] look-back give-ups (stays 0)                */
+    unsigned long long* scan_ctl;    /* rank scan: [1] look-back give-ups (stays 0)  */
     uint32_t n_pages;
     uint32_t n_words;
     uint32_t page_size;

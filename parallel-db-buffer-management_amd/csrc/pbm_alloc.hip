@@ -88,9 +88,11 @@ __global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t 
 // ---- batched allocation by rank, one pass over the map
 //
 // The map is cut into tiles of scan_threads * scan_wpt words, one
-// workgroup per tile (each thread holding scan_wpt consecutive words),
-// tiles numbered in the order the workgroups start (an atomic ticket), so
-// every tile's predecessors are already running.  A tile counts its free
+// workgroup per tile (each thread holding scan_wpt consecutive words), tile
+// = workgroup index.  A tile only ever waits for lower-numbered tiles: the
+// dispatcher starts the workgroups of an XCD in index order, so a waited-for
+// tile on the same XCD has started, and one on another XCD waits for that
+// XCD's slots only, never for slots the waiting tiles hold.  A tile counts its free
 // pages, publishes the count, and finds the free pages before it by
 // decoupled look-back: wave 0 reads the look-back words of the 64
 // preceding tiles at once and sums back to the nearest one that published
@@ -102,9 +104,11 @@ __global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t 
 // concurrent writer took between the read and the claim sends its request
 // to a short random walk.
 //
-// Tickets count up across calls and a call takes exactly n_tiles of them,
-// so ticket t is tile t mod n_tiles of call t div n_tiles; the parity of
-// the call number tells this call's look-back words from the last call's.
+// Every call runs every tile once, and a tile's look-back word carries the
+// parity of the call that wrote it: a tile takes this call's parity as the
+// flip of its own word's, so no counter is shared between tiles.  (Tiles
+// numbered by an atomic ticket at their start would wait in line at one
+// address, about 12 ns each: 12 us for the 1,024 tiles of a 2^20-word map.)
 
 // Look-back word: [63] call parity, [62] inclusive prefix (else the tile's
 // own count), [61:31] grant G, [30:0] free pages (saturating).
@@ -137,26 +141,17 @@ __device__ inline int32_t rank_fallback(const pbm_ctx& c, uint64_t req, uint32_t
 
 // One tile of THREADS threads (the whole workgroup).
 template <int WPT, int THREADS>
-__device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t n, uint64_t req_base,
-                                          int32_t* __restrict__ ids,
+__device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint32_t n,
+                                          uint64_t req_base, int32_t* __restrict__ ids,
                                           uint32_t* __restrict__ steps_out, uint32_t budget) {
     using namespace pbm;
     constexpr uint32_t kTileWords = THREADS * WPT;
     constexpr uint32_t kWaves = THREADS / 64;
-    __shared__ uint32_t s_tile, s_parity, s_excl, s_grant;
+    __shared__ uint32_t s_parity, s_excl, s_grant;
     __shared__ long long s_free;
     __shared__ uint32_t s_wsum[kWaves];
     const unsigned lane = lane_id();
     const unsigned wv = threadIdx.x >> 6;
-    if (threadIdx.x == 0) {
-        const u64 t = atom_add_u64(&c.scan_ctl[0], 1ull);
-        s_free = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_tile = (uint32_t)(t % c.n_tiles);
-        s_parity = (uint32_t)((t / c.n_tiles) & 1u);
-    }
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t parity = s_parity;
     // thread t holds WPT consecutive words of the tile
     const uint32_t w0 = tile * kTileWords + threadIdx.x * WPT;
     Stats st;
@@ -171,6 +166,15 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t n, uint64_t
         }
         cnt += (uint32_t)__popcll(fm[k]);
     }
+    // this tile's word from the last call, and the free count for tile 0's
+    // grant, read while the map words are in flight
+    u64 own = 0;
+    long long fr = 0;
+    if (threadIdx.x == 0) {
+        own = ld_word(&c.scan[tile]);
+        if (tile == 0)
+            fr = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     uint32_t inc = cnt;  // inclusive scan of the free counts over the wave
 #pragma unroll
     for (unsigned d = 1; d < 64u; d <<= 1) {
@@ -179,7 +183,12 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t n, uint64_t
     }
     const uint32_t pre = inc - cnt;
     if (lane == 63) s_wsum[wv] = inc;
+    if (threadIdx.x == 0) {
+        s_parity = (uint32_t)(own >> 63) ^ 1u;
+        s_free = fr;
+    }
     __syncthreads();
+    const uint32_t parity = s_parity;
     uint32_t agg = 0, woff = 0;
 #pragma unroll
     for (unsigned i = 0; i < kWaves; ++i) {
@@ -375,7 +384,7 @@ __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_
                                                   int32_t* __restrict__ ids,
                                                   uint32_t* __restrict__ steps_out,
                                                   uint32_t budget) {
-    rank_tile<WPT, 256>(c, n, req_base, ids, steps_out, budget);
+    rank_tile<WPT, 256>(c, blockIdx.x, n, req_base, ids, steps_out, budget);
 }
 
 // One-wave tiles, for small maps: their grants spread over more CUs.
@@ -383,12 +392,12 @@ __global__ __launch_bounds__(64) void k_get_rank_wave(pbm_ctx c, uint32_t n, uin
                                                       int32_t* __restrict__ ids,
                                                       uint32_t* __restrict__ steps_out,
                                                       uint32_t budget) {
-    rank_tile<1, 64>(c, n, req_base, ids, steps_out, budget);
+    rank_tile<1, 64>(c, blockIdx.x, n, req_base, ids, steps_out, budget);
 }
 
 // The same scan on one workgroup that takes every tile of the call in
-// ticket order (PBM_LAUNCH_ONE_WAVE): each tile's look-back finds its
-// predecessor finished, the fully serial schedule of the same result.
+// order (PBM_LAUNCH_ONE_WAVE): each tile's look-back finds its predecessor
+// finished, the fully serial schedule of the same result.
 template <int WPT, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_get_rank_serial(pbm_ctx c, uint32_t n,
                                                              uint64_t req_base,
@@ -396,7 +405,7 @@ __global__ __launch_bounds__(THREADS) void k_get_rank_serial(pbm_ctx c, uint32_t
                                                              uint32_t* __restrict__ steps_out,
                                                              uint32_t budget) {
     for (uint32_t t = 0; t < c.n_tiles; ++t) {
-        rank_tile<WPT, THREADS>(c, n, req_base, ids, steps_out, budget);
+        rank_tile<WPT, THREADS>(c, t, n, req_base, ids, steps_out, budget);
         __syncthreads();
     }
 }
@@ -588,10 +597,14 @@ __global__ __launch_bounds__(1024) void k_free_plain(pbm_ctx c, uint32_t n,
 }
 
 // Batched free: adjacent lanes whose pages share a word merge their bits
-// into one atomicAnd issued by the first lane of the run.  Each wave loads
-// the ids of its next chunk before it works on the current one, and each
+// into one atomicAnd issued by the first lane of the run.  Each wave frees
+// a contiguous range of chunks of 64 ids, kFreeUnroll chunks at a time,
+// and issues all their atomics before it reads any result, so their round
+// trips to the memory side overlap instead of following one another.  Each
 // 1024-thread workgroup returns its pages to the reservation counter with
 // one atomic.
+constexpr uint32_t kFreeUnroll = 8;
+
 __global__ __launch_bounds__(1024) void k_free_batched(pbm_ctx c, uint32_t n,
                                                        const int32_t* __restrict__ ids) {
     using namespace pbm;
@@ -599,37 +612,52 @@ __global__ __launch_bounds__(1024) void k_free_batched(pbm_ctx c, uint32_t n,
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t n_chunks = (n + 63u) / 64u;
+    const uint32_t per = (n_chunks + n_waves - 1u) / n_waves;
+    const uint64_t c_lo = (uint64_t)wave * per;
+    const uint64_t c_hi = c_lo + per < n_chunks ? c_lo + per : n_chunks;
     __shared__ uint32_t s_freed;
     if (threadIdx.x == 0) s_freed = 0;
     __syncthreads();
     Stats st;
-    int32_t next = -1;
-    if (wave < n_chunks && wave * 64u + lane < n) next = ids[wave * 64u + lane];
-    for (uint32_t ch = wave; ch < n_chunks; ch += n_waves) {
-        const int32_t id = next;
-        const uint32_t nr = (ch + n_waves) * 64u + lane;
-        next = (ch + n_waves < n_chunks && nr < n) ? ids[nr] : -1;
-        const bool valid = id >= 0 && (uint32_t)id < c.n_pages;
-        if (id >= 0 && !valid) st.errors++;
-        const uint32_t w = valid ? ((uint32_t)id >> 6) : (0x80000000u | lane);
-        u64 m = valid ? (1ull << ((uint32_t)id & 63u)) : 0ull;
-        const uint32_t prev = shfl32(w, lane == 0 ? 0u : lane - 1u);
-        const bool head = lane == 0 || prev != w;
-        const u64 H = ballot(head);
-        const u64 upto = lane == 63 ? kFull : ((2ull << lane) - 1ull);
-        const unsigned seg = 63u - (unsigned)__clzll((long long)(H & upto));
+    const u64 upto = lane == 63 ? kFull : ((2ull << lane) - 1ull);
+    for (uint64_t cb = c_lo; cb < c_hi; cb += kFreeUnroll) {
+        int32_t id[kFreeUnroll];
 #pragma unroll
-        for (unsigned d = 1; d < 64u; d <<= 1) {
-            const unsigned src = lane + d < 64u ? lane + d : lane;
-            const u64 o = shfl64(m, src);
-            const uint32_t s2 = shfl32(seg, src);
-            if (lane + d < 64u && s2 == seg) m |= o;
+        for (uint32_t u = 0; u < kFreeUnroll; ++u) {
+            const uint64_t r = (cb + u) * 64u + lane;
+            id[u] = (cb + u < c_hi && r < n) ? ids[r] : -1;
         }
-        if (head && valid) {
-            const u64 old = atom_and(&c.bitmap[w], ~m);
-            st.atomics++;
-            st.frees += (uint32_t)__popcll(old & m);
-            st.errors += (uint32_t)__popcll(m & ~old);
+        u64 m[kFreeUnroll], old[kFreeUnroll];
+#pragma unroll
+        for (uint32_t u = 0; u < kFreeUnroll; ++u) {
+            m[u] = 0;
+            old[u] = 0;
+            if (cb + u >= c_hi) continue;  // the same for the whole wave
+            const bool valid = id[u] >= 0 && (uint32_t)id[u] < c.n_pages;
+            if (id[u] >= 0 && !valid) st.errors++;
+            const uint32_t w = valid ? ((uint32_t)id[u] >> 6) : (0x80000000u | lane);
+            u64 mu = valid ? (1ull << ((uint32_t)id[u] & 63u)) : 0ull;
+            const uint32_t prev = shfl32(w, lane == 0 ? 0u : lane - 1u);
+            const bool head = lane == 0 || prev != w;
+            const u64 H = ballot(head);
+            const unsigned seg = 63u - (unsigned)__clzll((long long)(H & upto));
+#pragma unroll
+            for (unsigned d = 1; d < 64u; d <<= 1) {
+                const unsigned src = lane + d < 64u ? lane + d : lane;
+                const u64 o = shfl64(mu, src);
+                const uint32_t s2 = shfl32(seg, src);
+                if (lane + d < 64u && s2 == seg) mu |= o;
+            }
+            if (head && valid) {
+                m[u] = mu;
+                old[u] = atom_and(&c.bitmap[w], ~mu);
+                st.atomics++;
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kFreeUnroll; ++u) {
+            st.frees += (uint32_t)__popcll(old[u] & m[u]);
+            st.errors += (uint32_t)__popcll(m[u] & ~old[u]);
         }
     }
     // one reservation update per workgroup for all pages it returned

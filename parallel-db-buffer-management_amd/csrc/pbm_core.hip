@@ -264,12 +264,12 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
     e = hipMemset(h->d_phase, 0, 64u);
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
-    // Look-back words start with call parity 0 and the tickets at n_tiles,
-    // so the first call has parity 1 and no zeroed word reads as published.
+    // Look-back words start with call parity 0: the first call has parity 1
+    // and no zeroed word reads as published by it.
     e = hipMemset(h->d_scan, 0, (size_t)n_tiles * 8u);
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
     {
-        const unsigned long long ctl[4] = {n_tiles, 0, 0, 0};
+        const unsigned long long ctl[4] = {0, 0, 0, 0};
         e = hipMemcpy(h->d_scan_ctl, ctl, sizeof(ctl), hipMemcpyHostToDevice);
         if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemcpy", __FILE__, __LINE__));
     }
