@@ -556,8 +556,10 @@ def rank_at_scale_leg(torch, pbm, stream, n_pages: int, fill: float, n_req: int,
     before = pool.snapshot(stream)
     A = pool.free_count(stream)
     ids = torch.full((n_req,), -1, dtype=torch.int32, device="cuda")
+    # the free's waves per CU from the autotuner (the rank scan has one shape)
+    tuned = pbm.autotune(pool, n_req, ids, pbm.TUNE_RANK_ONLY | pbm.TUNE_QUICK, stream)
     gl = pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK)
-    fl = pbm.launch(pbm.MODE_BATCHED)
+    fl = pbm.launch(pbm.MODE_BATCHED, waves_per_cu=tuned["free_waves_per_cu"])
     pool.clear_counters(stream)
     pool.get_pages(n_req, ids, None, 0, gl, stream)
     torch.cuda.synchronize()
@@ -589,6 +591,7 @@ def rank_at_scale_leg(torch, pbm, stream, n_pages: int, fill: float, n_req: int,
                         f"f={fill} seed 42, {n_req:,} one-page requests then frees, rank "
                         "allocation",
             "free_pages": A, "grants": grants, "ms_per_step": ms,
+            "free_waves_per_cu": tuned["free_waves_per_cu"],
             "value": 2 * grants / (ms * 1e-3), "phases_ms": own, "verified": ok,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
@@ -893,8 +896,8 @@ def main() -> int:
             gl = pbm.launch(pbm.MODE_BATCHED, 256, wpc, budget, cw, flags)
             fl = pbm.launch(pbm.MODE_BATCHED, 256, wpc, 0, 64, pbm.LAUNCH_NO_COUNTERS)
             return {"get_launch": gl, "free_launch": fl, "rank": rank, "cluster_words": cw,
-                    "waves_per_cu": wpc, "walk_budget": budget, "ms_alloc": 0.0,
-                    "ms_free": 0.0, "candidates": 0}
+                    "waves_per_cu": wpc, "walk_budget": budget, "free_waves_per_cu": wpc,
+                    "ms_alloc": 0.0, "ms_free": 0.0, "candidates": 0}
         tune_rank = fixed(True, 16, 64, 16)
         tune_walk = fixed(False, 8, 256, 16)
     else:

@@ -603,18 +603,21 @@ int pbm_zipf_trace_noalloc(uint32_t n_logical, double s, uint64_t seed, uint64_t
 
 typedef struct pbm_tune_result {
     pbm_launch get_launch;   /* fastest batched launch for pbm_get_pages              */
-    pbm_launch free_launch;  /* its partner for pbm_free_pages                        */
-    float      ms_alloc;     /* median alloc phase of that launch, milliseconds       */
-    float      ms_free;      /* median free phase, milliseconds                       */
+    pbm_launch free_launch;  /* fastest batched launch for pbm_free_pages after it    */
+    float      ms_alloc;     /* median alloc kernel of get_launch, milliseconds       */
+    float      ms_free;      /* median free kernel of free_launch, milliseconds       */
     uint32_t   candidates;   /* launch shapes timed                                   */
     uint32_t   reserved;
 } pbm_tune_result;
 
 /* Times alloc+free cycles of n requests (device buffer d_ids of n ids)
- * over the batched launch space -- rank or walk homes, window width
- * (64/128/256 words), walk budget (4/16/64 jumps), waves per CU
- * (1/2/4/8/16) -- and returns the fastest.  Every cycle returns its pages,
- * so the pool ends as it started; no other call may overlap.  No reference
+ * over the batched launch space and returns the fastest alloc launch --
+ * the rank scan, or walk homes over window width (64/128/256 words), walk
+ * budget (4/16/64 jumps) and waves per CU (1/2/4/8/16) -- and, timed after
+ * it, the fastest free launch (waves per CU 1/2/4/8/16/32).  Times are the
+ * kernels' own (events at each kernel's start and end), medians of 9
+ * cycles (3 with PBM_TUNE_QUICK).  Every cycle returns its pages, so the
+ * pool ends as it started; no other call may overlap.  No reference
  * counterpart (its sweeps are hard-coded, unitTests/unitTest1.cu:14-18). */
 int pbm_autotune(pbm_handle* h, uint32_t n, int32_t* d_ids, uint32_t flags,
                  pbm_tune_result* out, void* stream);

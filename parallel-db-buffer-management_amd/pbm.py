@@ -1002,7 +1002,8 @@ def atomic_peak(words: int = 16384, rounds: int = 64, device: int = 0) -> float:
 
 
 def autotune(pool: Pool, n: int, ids, flags: int = 0, stream=None) -> dict:
-    """Fastest batched launch for n requests (ids: device buffer of n int32)."""
+    """Fastest batched alloc launch for n requests (ids: device buffer of n
+    int32) and the fastest free launch after it."""
     r = TuneResult()
     check(lib().pbm_autotune(pool.h, n, _ptr(ids), flags, C.byref(r), _stream(stream)),
           "pbm_autotune")
@@ -1010,5 +1011,6 @@ def autotune(pool: Pool, n: int, ids, flags: int = 0, stream=None) -> dict:
     return {"get_launch": g, "free_launch": r.free_launch,
             "rank": bool(g.flags & LAUNCH_RANK), "cluster_words": int(g.cluster_words),
             "waves_per_cu": int(g.waves_per_cu), "walk_budget": int(g.walk_budget),
+            "free_waves_per_cu": int(r.free_launch.waves_per_cu),
             "ms_alloc": float(r.ms_alloc), "ms_free": float(r.ms_free),
             "candidates": int(r.candidates)}
