@@ -59,6 +59,20 @@ def progress(msg: str) -> None:
     print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
+def traffic_of(leg: str, kernel: str):
+    """HBM bytes per launch of `kernel` in `leg`, from the committed
+    rocprofv3 FETCH_SIZE and WRITE_SIZE runs of these workloads (each
+    scaled by what the counter reads for a 1 GiB device copy); None when
+    that kernel was not profiled."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    for k in json.loads(f.read_text())["kernels"]:
+        if k["leg"] == leg and k["kernel"] == kernel:
+            return k["hbm_bytes"]
+    return None
+
+
 def time_region(torch, fn, steps: int, warmup: int, stream) -> float:
     """Mean ms of fn() over `steps` calls launched back to back between two
     events on `stream`, after `warmup` untimed calls."""
@@ -212,7 +226,8 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
         "mean_wait_steps": float(w_steps.mean()) if w.size else None,
         "alloc_failures": s4["alloc_failures"],
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_of("serving", "k_serve_grow<0>"),
                      "kernel": "k_serve_grow<0>", "bytes_per_launch": grow_bytes,
                      "launch_ms": kms["grow"], "grants_per_launch": grants / 200,
                      "frees_per_launch": frees / 200},
@@ -295,12 +310,7 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int, persistent: 
     alg = n_malloc * (8 + 16 + 4 + 8) + n_free * (8 + 4 + 16)
     dom_ms = kernel_ms if kernel_ms else ms
     achieved = alg / (dom_ms * 1e-3) / 1e9
-    traffic = None
-    tfile = ROOT / "profiles" / "traffic_c4_bursts.json"
-    if tfile.exists() and tune["persistent"]:
-        for k in json.loads(tfile.read_text())["kernels"]:
-            if k["kernel"] == "k_bursts_persistent":
-                traffic = k["hbm_bytes"]
+    traffic = traffic_of("bursts", "k_bursts_persistent") if tune["persistent"] else None
     out = {
         "workload": f"{n_bursts} bursts, lognormal count median 2048 sigma 1.0, 50/50 "
                     "mallocRW (16 B..64 KiB log-uniform) / freeRW, 262,144 x 4 KiB pages "
@@ -534,7 +544,9 @@ def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
         "kernel_ms": kms, "verified": bool(st2["hits"] == st["hits"]
                                            and st2["evictions"] == st["evictions"]),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_of("cache", {"hits": "k_cache_hits", "plan": "k_cache_plan",
+                                                     "load": "k_cache_load<2>"}[dom]),
                      "kernel": {"hits": "k_cache_hits", "plan": "k_cache_plan",
                                 "load": "k_cache_load<2>"}[dom],
                      "bytes_per_launch": bytes_k[dom], "launch_ms": kms[dom]},
@@ -586,6 +598,26 @@ def rank_at_scale_leg(torch, pbm, stream, n_pages: int, fill: float, n_req: int,
     alloc_bytes = 8 * L + 16 * ca["atomics"] + 4 * n_req
     free_bytes = 4 * n_req + 16 * (cc["atomics"] - ca["atomics"])
     achieved = alloc_bytes / (own["alloc"] * 1e-3) / 1e9
+    # the same step with exclusive claims (plain stores; nothing else runs
+    # on the pool meanwhile): one checked cycle, then timed
+    qx = pbm.Launch(q.mode, q.block, q.waves_per_cu, q.walk_budget, q.cluster_words,
+                    q.flags | pbm.LAUNCH_EXCLUSIVE)
+    pool.get_pages(n_req, ids, None, 0, qx, stream)
+    torch.cuda.synchronize()
+    x_np = ids.cpu().numpy()
+    pool.free_pages(n_req, ids, qf, stream)
+    torch.cuda.synchronize()
+    ok_x = (np.array_equal(x_np, i_np) and bool(np.array_equal(pool.snapshot(stream), before)))
+
+    def step_x():
+        pool.get_pages(n_req, ids, None, 0, qx, stream)
+        pool.free_pages(n_req, ids, qf, stream)
+
+    ms_x = time_region(torch, step_x, steps, 2, stream)
+    own_x, _ = time_phases(torch, pbm, pool, n_req, ids, qx, qf, steps, stream)
+    # exclusive claims write each claimed word (8 B) instead of a 16 B RMW
+    x_bytes = 8 * L + 8 * ca["atomics"] + 4 * n_req
+    x_ach = x_bytes / (own_x["alloc"] * 1e-3) / 1e9
     pool.close()
     return {"workload": f"{n_pages:,} x 4 KiB pages (page map only), Bernoulli pre-fill "
                         f"f={fill} seed 42, {n_req:,} one-page requests then frees, rank "
@@ -594,10 +626,22 @@ def rank_at_scale_leg(torch, pbm, stream, n_pages: int, fill: float, n_req: int,
             "free_waves_per_cu": tuned["free_waves_per_cu"],
             "value": 2 * grants / (ms * 1e-3), "phases_ms": own, "verified": ok,
             "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "k_get_rank<4>", "bytes_per_launch": alloc_bytes,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": traffic_of("rank_at_scale", "k_get_rank<4, false>"),
+                         "kernel": "k_get_rank<4, false>", "bytes_per_launch": alloc_bytes,
                          "launch_ms": own["alloc"], "free_bytes_per_launch": free_bytes,
-                         "free_gbs": free_bytes / (own["free"] * 1e-3) / 1e9}}
+                         "free_gbs": free_bytes / (own["free"] * 1e-3) / 1e9,
+                         "free_traffic": traffic_of("rank_at_scale", "k_free_batched")},
+            "exclusive": {"ms_per_step": ms_x, "value": 2 * grants / (ms_x * 1e-3),
+                          "phases_ms": own_x, "verified": ok_x,
+                          "same_ids_as_atomic_claims": bool(np.array_equal(x_np, i_np)),
+                          "roofline": {"bound": "hbm", "achieved": x_ach, "peak": HBM_PEAK_GBS,
+                                       "unit": "GB/s", "frac": x_ach / HBM_PEAK_GBS,
+                                       "traffic": traffic_of("rank_at_scale",
+                                                             "k_get_rank<4, true>"),
+                                       "kernel": "k_get_rank<4, true>",
+                                       "bytes_per_launch": x_bytes,
+                                       "launch_ms": own_x["alloc"]}}}
 
 
 SWEEP_PAGES, SWEEP_PAGE_SIZE = 1 << 24, 512  # the reference's default pool: 8 GiB of 512 B pages
@@ -963,6 +1007,13 @@ def main() -> int:
         progress(f"alloc/free: {name} measured")
         results[name]["tuned"] = {k: v for k, v in tuned.items()
                                   if k not in ("get_launch", "free_launch")}
+    # the rank launch with exclusive claims (plain stores of the claimed
+    # words; the step runs nothing else on the pool)
+    rg = tune_rank["get_launch"]
+    xg = pbm.Launch(rg.mode, rg.block, rg.waves_per_cu, rg.walk_budget, rg.cluster_words,
+                    rg.flags | pbm.LAUNCH_EXCLUSIVE)
+    results["batched_exclusive"] = measure(xg, tune_rank["free_launch"], args.steps, args.warmup)
+    progress("alloc/free: batched_exclusive measured")
 
     # ---- plain per-thread getPage / freePage (the reference's policy)
     p_get = pbm.launch(mode=pbm.MODE_PLAIN, flags=pbm.LAUNCH_NO_COUNTERS)
@@ -995,8 +1046,11 @@ def main() -> int:
         c1_dom = max(ph, key=ph.get)
         c1_ach = c1_bytes[c1_dom] / (ph[c1_dom] * 1e-3) / 1e9
         c1["roofline"] = {"bound": "hbm", "achieved": c1_ach, "peak": HBM_PEAK_GBS,
-                          "unit": "GB/s", "frac": c1_ach / HBM_PEAK_GBS, "traffic": None,
-                          "kernel": "k_get_rank_wave" if c1_dom == "alloc" else "k_free_batched",
+                          "unit": "GB/s", "frac": c1_ach / HBM_PEAK_GBS,
+                          "traffic": traffic_of("small_pool", "k_get_rank_wave<false>"
+                                                if c1_dom == "alloc" else "k_free_batched"),
+                          "kernel": "k_get_rank_wave<false>" if c1_dom == "alloc"
+                          else "k_free_batched",
                           "bytes_per_launch": c1_bytes[c1_dom], "launch_ms": ph[c1_dom]}
         progress("65,536-page configuration measured")
     rank_scale = None
@@ -1025,7 +1079,7 @@ def main() -> int:
         "free": 4 * N_REQ + 16 * (cc["atomics"] - ca["atomics"]),
     }
     wpt = pool.ctx().scan_wpt
-    kernel_names = {"alloc": f"k_get_rank<{wpt}>" if rank else
+    kernel_names = {"alloc": f"k_get_rank<{wpt}, false>" if rank else
                     f"k_get_batched<{b['tuned']['cluster_words'] // 64}>",
                     "free": "k_free_batched"}
     dom = max(b["phases_ms"], key=b["phases_ms"].get)
@@ -1033,14 +1087,7 @@ def main() -> int:
     achieved = bytes_k[dom] / (dom_ms * 1e-3) / 1e9
     # HBM bytes per launch of that kernel from the committed FETCH_SIZE /
     # WRITE_SIZE counter runs of this workload (null when not profiled)
-    traffic = None
-    tfile = ROOT / "profiles" / "traffic_c2_fill90.json"
-    if tfile.exists():
-        tw = pool.ctx().scan_threads * wpt
-        shapes = {f"k_get_rank<{wpt}>": (L + tw - 1) // tw}
-        for k in json.loads(tfile.read_text())["kernels"]:
-            if k["kernel"] == kernel_names[dom] and k["workgroups"] == shapes.get(k["kernel"]):
-                traffic = k["hbm_bytes"]
+    traffic = traffic_of("headline", kernel_names[dom])
     atom_peak = pbm.atomic_peak(L, 64)
     atom_rate = (ca["atomics"] / (b["phases_ms"]["alloc"] * 1e-3))
 
@@ -1134,6 +1181,10 @@ def main() -> int:
                          **{k: results["batched_walk"][k] for k in
                             ("ms_per_step", "ms_per_step_graph", "phases_ms", "walk_steps",
                              "verified", "tuned")}},
+        "batched_exclusive": {
+            "value": ops_per_step / (results["batched_exclusive"]["ms_per_step"] * 1e-3),
+            **{k: results["batched_exclusive"][k] for k in
+               ("ms_per_step", "ms_per_step_graph", "phases_ms", "verified")}},
         "counters": {"alloc": ca, "cycle": cc},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -1162,10 +1213,12 @@ def main() -> int:
         torch.cuda.synchronize()
     pool.close()
     return 0 if (b["verified"] and ok_p and results["batched_walk"]["verified"]
+                 and results["batched_exclusive"]["verified"]
                  and all(v["verified"] for v in comparators.values())
                  and (bursts is None or bursts["verified"])
                  and (cache is None or cache["verified"])
-                 and (rank_scale is None or rank_scale["verified"])
+                 and (rank_scale is None or (rank_scale["verified"]
+                                             and rank_scale["exclusive"]["verified"]))
                  and (ref_sweeps is None or ref_sweeps["verified"])
                  and (kmalloc is None or (kmalloc["collab"]["verified"]
                                           and kmalloc["simple"]["verified"]))

@@ -158,6 +158,15 @@ typedef struct pbm_launch {
                                       persistent kernel (batched mode): waves take work
                                       items in order from a ticket and wait on the
                                       items they depend on                             */
+#define PBM_LAUNCH_EXCLUSIVE   32u /* with PBM_LAUNCH_RANK: the caller guarantees that no
+                                      other kernel reads or writes the pool's map while
+                                      the call runs (calls on one stream, or ordered by
+                                      events).  Each tile then claims its words with
+                                      plain stores of the word it read plus its granted
+                                      bits, instead of device-scope atomics, which run
+                                      at the memory side on MI355X.  Same ids and map as
+                                      without it; no reference counterpart.  Ignored
+                                      without PBM_LAUNCH_RANK                          */
 
 typedef struct pbm_handle pbm_handle;
 

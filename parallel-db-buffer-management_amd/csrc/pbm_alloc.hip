@@ -94,7 +94,7 @@ __global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t 
 // tile on the same XCD has started, and one on another XCD waits for that
 // XCD's slots only, never for slots the waiting tiles hold.  A tile counts its free
 // pages, publishes the count, and finds the free pages before it by
-// decoupled look-back: wave 0 reads the look-back words of the 64
+// decoupled look-back: wave 0 reads the look-back words of the 256
 // preceding tiles at once and sums back to the nearest one that published
 // its inclusive prefix.  Tile 0 fixes the call's grant G = min(n, free
 // count) and passes it down the chain inside the same words.  Then the
@@ -116,6 +116,7 @@ constexpr u64 kScanIncl = 1ull << 62;
 constexpr uint32_t kScanSat = 0x7FFFFFFFu;
 constexpr uint32_t kScanSpin = 1u << 20;  // look-back polls before giving up
 constexpr uint32_t kRankStage = 4096;     // ids staged in LDS per pass (large tiles)
+constexpr int kLookRounds = 4;            // look-back reads 64 * kLookRounds words at once
 
 __device__ __forceinline__ u64 scan_word(uint32_t parity, bool incl, uint32_t grant, u64 v) {
     const uint32_t sv = v > kScanSat ? kScanSat : (uint32_t)v;
@@ -139,8 +140,11 @@ __device__ inline int32_t rank_fallback(const pbm_ctx& c, uint64_t req, uint32_t
     return id;
 }
 
-// One tile of THREADS threads (the whole workgroup).
-template <int WPT, int THREADS>
+// One tile of THREADS threads (the whole workgroup).  EXCL: the caller
+// keeps every other kernel off the map for the call (PBM_LAUNCH_EXCLUSIVE),
+// so a tile, the only writer of its words, stores each claimed word as read
+// plus its granted bits.
+template <int WPT, int THREADS, bool EXCL>
 __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint32_t n,
                                           uint64_t req_base, int32_t* __restrict__ ids,
                                           uint32_t* __restrict__ steps_out, uint32_t budget) {
@@ -211,19 +215,40 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             int64_t k = (int64_t)tile - 1;
             uint32_t spins = 0;
             for (;;) {
-                const int64_t idx = k - (int64_t)lane;
-                u64 v = 0;
-                bool ready = true, incl = false;
-                if (idx >= 0) {
-                    v = ld_word(&c.scan[idx]);
-                    ready = (uint32_t)(v >> 63) == parity;
-                    incl = ready && (v & kScanIncl);
+                // 64 * kLookRounds predecessors at a time: tile k - (64 j + lane)
+                u64 v[kLookRounds];
+                u64 P[kLookRounds], R[kLookRounds];
+#pragma unroll
+                for (int j = 0; j < kLookRounds; ++j) {
+                    const int64_t idx = k - (int64_t)(64 * j) - (int64_t)lane;
+                    v[j] = 0;
+                    bool ready = true, incl = false;
+                    if (idx >= 0) {
+                        v[j] = ld_word(&c.scan[idx]);
+                        ready = (uint32_t)(v[j] >> 63) == parity;
+                        incl = ready && (v[j] & kScanIncl);
+                    }
+                    P[j] = ballot(incl);
+                    R[j] = ballot(ready);
                 }
-                const u64 P = ballot(incl);
-                const u64 R = ballot(ready);
-                const unsigned fp = P ? (unsigned)__ffsll((long long)P) - 1u : 63u;
-                const u64 upto = fp == 63u ? kFull : ((2ull << fp) - 1ull);
-                if ((R & upto) != upto) {
+                // the nearest inclusive word (jf, fp); every nearer one must
+                // have published its count
+                int jf = kLookRounds;
+                unsigned fp = 63u;
+#pragma unroll
+                for (int j = kLookRounds - 1; j >= 0; --j)
+                    if (P[j]) {
+                        jf = j;
+                        fp = (unsigned)__ffsll((long long)P[j]) - 1u;
+                    }
+                bool ok = true;
+#pragma unroll
+                for (int j = 0; j < kLookRounds; ++j) {
+                    const u64 need = j < jf ? kFull
+                                     : j == jf ? (fp == 63u ? kFull : ((2ull << fp) - 1ull)) : 0ull;
+                    if ((R[j] & need) != need) ok = false;
+                }
+                if (!ok) {
                     if (++spins > kScanSpin) {
                         // cannot happen while every started tile runs on;
                         // recorded, and the chain goes on with no grant
@@ -235,15 +260,20 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                u64 part = (lane <= fp && idx >= 0) ? (v & kScanSat) : 0ull;
+                u64 part = 0, vf = 0;
+#pragma unroll
+                for (int j = 0; j < kLookRounds; ++j) {
+                    if (j < jf || (j == jf && lane <= fp)) part += v[j] & kScanSat;
+                    if (j == jf) vf = v[j];
+                }
 #pragma unroll
                 for (int d = 32; d >= 1; d >>= 1) part += shfl64(part, lane ^ (unsigned)d);
                 excl += part;
-                if (P) {
-                    grant = (uint32_t)((readlane64(v, fp) >> 31) & 0x7FFFFFFFull);
+                if (jf < kLookRounds) {
+                    grant = (uint32_t)((readlane64(vf, fp) >> 31) & 0x7FFFFFFFull);
                     break;
                 }
-                k -= 64;
+                k -= 64 * kLookRounds;
             }
             if (lane == 0) st_scan(&c.scan[tile], scan_word(parity, true, grant, excl + agg));
         }
@@ -268,8 +298,12 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
         mask[k] = lowest_bits(fm[k], take);
         old[k] = 0;
         if (mask[k]) {
-            old[k] = atom_or(&c.bitmap[w0 + k], mask[k]);
-            st.atomics++;
+            if constexpr (EXCL) {
+                c.bitmap[w0 + k] = ~(fm[k] & ~mask[k]);
+            } else {
+                old[k] = atom_or(&c.bitmap[w0 + k], mask[k]);
+                st.atomics++;
+            }
         }
         rk += pc;
     }
@@ -379,33 +413,34 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
     flush_stats(c, st);
 }
 
-template <int WPT>
+template <int WPT, bool EXCL>
 __global__ __launch_bounds__(256) void k_get_rank(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                   int32_t* __restrict__ ids,
                                                   uint32_t* __restrict__ steps_out,
                                                   uint32_t budget) {
-    rank_tile<WPT, 256>(c, blockIdx.x, n, req_base, ids, steps_out, budget);
+    rank_tile<WPT, 256, EXCL>(c, blockIdx.x, n, req_base, ids, steps_out, budget);
 }
 
 // One-wave tiles, for small maps: their grants spread over more CUs.
+template <bool EXCL>
 __global__ __launch_bounds__(64) void k_get_rank_wave(pbm_ctx c, uint32_t n, uint64_t req_base,
                                                       int32_t* __restrict__ ids,
                                                       uint32_t* __restrict__ steps_out,
                                                       uint32_t budget) {
-    rank_tile<1, 64>(c, blockIdx.x, n, req_base, ids, steps_out, budget);
+    rank_tile<1, 64, EXCL>(c, blockIdx.x, n, req_base, ids, steps_out, budget);
 }
 
 // The same scan on one workgroup that takes every tile of the call in
 // order (PBM_LAUNCH_ONE_WAVE): each tile's look-back finds its predecessor
 // finished, the fully serial schedule of the same result.
-template <int WPT, int THREADS>
+template <int WPT, int THREADS, bool EXCL>
 __global__ __launch_bounds__(THREADS) void k_get_rank_serial(pbm_ctx c, uint32_t n,
                                                              uint64_t req_base,
                                                              int32_t* __restrict__ ids,
                                                              uint32_t* __restrict__ steps_out,
                                                              uint32_t budget) {
     for (uint32_t t = 0; t < c.n_tiles; ++t) {
-        rank_tile<WPT, THREADS>(c, t, n, req_base, ids, steps_out, budget);
+        rank_tile<WPT, THREADS, EXCL>(c, t, n, req_base, ids, steps_out, budget);
         __syncthreads();
     }
 }
@@ -745,6 +780,31 @@ extern "C" int pbm_event_elapsed_ms(void* start_event, void* stop_event, float* 
     return PBM_OK;
 }
 
+template <bool EXCL>
+static void launch_rank(pbm_handle* h, const pbm_ctx& c, const pbm_launch& L, uint32_t n,
+                 uint64_t req_base, int32_t* d_ids, uint32_t* d_steps, hipStream_t s) {
+    if (L.flags & PBM_LAUNCH_ONE_WAVE) {
+        if (c.scan_wpt == 4)
+            pbmi::launch_timed(h, k_get_rank_serial<4, 256, EXCL>, dim3(1), dim3(256), s, c, n,
+                               req_base, d_ids, d_steps, c.walk_budget);
+        else if (c.scan_threads == 64)
+            pbmi::launch_timed(h, k_get_rank_serial<1, 64, EXCL>, dim3(1), dim3(64), s, c, n,
+                               req_base, d_ids, d_steps, c.walk_budget);
+        else
+            pbmi::launch_timed(h, k_get_rank_serial<1, 256, EXCL>, dim3(1), dim3(256), s, c, n,
+                               req_base, d_ids, d_steps, c.walk_budget);
+    } else if (c.scan_wpt == 4) {
+        pbmi::launch_timed(h, k_get_rank<4, EXCL>, dim3(c.n_tiles), dim3(256), s, c, n, req_base,
+                           d_ids, d_steps, c.walk_budget);
+    } else if (c.scan_threads == 64) {
+        pbmi::launch_timed(h, k_get_rank_wave<EXCL>, dim3(c.n_tiles), dim3(64), s, c, n,
+                           req_base, d_ids, d_steps, c.walk_budget);
+    } else {
+        pbmi::launch_timed(h, k_get_rank<1, EXCL>, dim3(c.n_tiles), dim3(256), s, c, n, req_base,
+                           d_ids, d_steps, c.walk_budget);
+    }
+}
+
 static int get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_ids,
                      uint32_t* d_steps, const pbm_launch* launch, void* stream) {
     Disarm disarm{h};
@@ -761,28 +821,10 @@ static int get_pages(pbm_handle* h, uint32_t n, uint64_t req_base, int32_t* d_id
         if (pbmi::is_queue_strategy(strat))
             return pbmi::fail(PBM_ERR_UNSUPPORTED, "batched mode needs a bitmap strategy");
         if (L.flags & PBM_LAUNCH_RANK) {
-            if (L.flags & PBM_LAUNCH_ONE_WAVE) {
-                if (c.scan_wpt == 4)
-                    pbmi::launch_timed(h, k_get_rank_serial<4, 256>, dim3(1), dim3(256), s, c, n,
-                                       req_base, d_ids, d_steps, c.walk_budget);
-                else if (c.scan_threads == 64)
-                    pbmi::launch_timed(h, k_get_rank_serial<1, 64>, dim3(1), dim3(64), s, c, n,
-                                       req_base, d_ids, d_steps, c.walk_budget);
-                else
-                    pbmi::launch_timed(h, k_get_rank_serial<1, 256>, dim3(1), dim3(256), s, c, n,
-                                       req_base, d_ids, d_steps, c.walk_budget);
-                PBM_HIP(hipGetLastError());
-                return PBM_OK;
-            }
-            if (c.scan_wpt == 4)
-                pbmi::launch_timed(h, k_get_rank<4>, dim3(c.n_tiles), dim3(256), s, c, n,
-                                   req_base, d_ids, d_steps, c.walk_budget);
-            else if (c.scan_threads == 64)
-                pbmi::launch_timed(h, k_get_rank_wave, dim3(c.n_tiles), dim3(64), s, c, n,
-                                   req_base, d_ids, d_steps, c.walk_budget);
+            if (L.flags & PBM_LAUNCH_EXCLUSIVE)
+                launch_rank<true>(h, c, L, n, req_base, d_ids, d_steps, s);
             else
-                pbmi::launch_timed(h, k_get_rank<1>, dim3(c.n_tiles), dim3(256), s, c, n,
-                                   req_base, d_ids, d_steps, c.walk_budget);
+                launch_rank<false>(h, c, L, n, req_base, d_ids, d_steps, s);
             PBM_HIP(hipGetLastError());
             return PBM_OK;
         }

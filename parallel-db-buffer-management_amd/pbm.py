@@ -108,6 +108,7 @@ class ServeStats(C.Structure):
 
 SERVE_PAYLOAD, SERVE_LATENCY = 1, 2
 LAUNCH_PERSISTENT = 16
+LAUNCH_EXCLUSIVE = 32
 
 
 class CacheConfig(C.Structure):

@@ -125,12 +125,15 @@ def test_one_wave_batched_budget_sweep_matches_oracle(gpu, T, fill, n, budget, o
                                       (64 * 256 * 65 + 17, 0.6, 1 << 20),
                                       # 2^20 map words and more: four words per thread
                                       (1 << 26, 0.9, 300000), (1 << 26, 0.999, 100000)])
-@pytest.mark.parametrize("one_wave", [False, True])
-def test_batched_rank_matches_oracle_exactly(gpu, T, fill, n, one_wave):
+@pytest.mark.parametrize("one_wave,excl", [(False, False), (True, False), (False, True),
+                                           (True, True)])
+def test_batched_rank_matches_oracle_exactly(gpu, T, fill, n, one_wave, excl):
     # by rank, request r takes the r-th free page whatever the schedule, so
-    # even the full concurrent launch matches the oracle bit for bit
-    log_case(f"batched rank T={T} fill={fill} n={n} one_wave={one_wave}")
-    flags = pbm.LAUNCH_RANK | (pbm.LAUNCH_ONE_WAVE if one_wave else 0)
+    # even the full concurrent launch matches the oracle bit for bit, with
+    # atomic claims or with the plain stores of an exclusive call
+    log_case(f"batched rank T={T} fill={fill} n={n} one_wave={one_wave} exclusive={excl}")
+    flags = (pbm.LAUNCH_RANK | (pbm.LAUNCH_ONE_WAVE if one_wave else 0)
+             | (pbm.LAUNCH_EXCLUSIVE if excl else 0))
     with pbm.Pool(T, strategy=pbm.RW_BM, seed=3) as pool:
         pool.prefill_bernoulli(fill, 42)
         ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, flags=flags)
@@ -143,16 +146,19 @@ def test_batched_rank_matches_oracle_exactly(gpu, T, fill, n, one_wave):
         assert np.array_equal(pool.snapshot(), orc.prefill(T, fill, 42))
 
 
-def test_batched_rank_calls_in_a_row(gpu):
-    # each call flips the look-back epoch; calls of different sizes on the
-    # same pool must each match the oracle continuing from the last state
-    T = 1 << 21
+@pytest.mark.parametrize("T", [1 << 21, 1 << 26])
+def test_batched_rank_calls_in_a_row(gpu, T):
+    # each call flips every tile's look-back parity; calls of different
+    # sizes and kinds (atomic or exclusive claims, one workgroup or all) on
+    # the same pool must each match the oracle continuing from the last state
     log_case(f"batched rank calls in a row T={T}")
+    kinds = [pbm.LAUNCH_RANK, pbm.LAUNCH_RANK | pbm.LAUNCH_EXCLUSIVE,
+             pbm.LAUNCH_RANK | pbm.LAUNCH_ONE_WAVE]
     with pbm.Pool(T, strategy=pbm.RW_BM, seed=4) as pool:
         pool.prefill_bernoulli(0.8, 9)
         st = orc.State(orc.prefill(T, 0.8, 9), T)
         for k, n in enumerate([1, 64, 5000, 100000, 777, 300000, 3, 150000]):
-            ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK)
+            ids, steps = _get(gpu, pool, n, pbm.MODE_BATCHED, flags=kinds[k % 3])
             eids, esteps = st.batched_rank(n)
             assert np.array_equal(ids, eids), f"call {k} n={n}"
             assert np.array_equal(steps, esteps), f"call {k} n={n}"
