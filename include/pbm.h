@@ -167,6 +167,13 @@ typedef struct pbm_launch {
                                       at the memory side on MI355X.  Same ids and map as
                                       without it; no reference counterpart.  Ignored
                                       without PBM_LAUNCH_RANK                          */
+#define PBM_LAUNCH_ORDERED     64u /* with PBM_LAUNCH_PERSISTENT: every alloc item waits
+                                      for all free items of its burst before it reserves
+                                      (the order of the phase kernels).  Without it an
+                                      alloc item runs ahead of its burst's frees while at
+                                      least a quarter of the pool's pages are unreserved,
+                                      and one that then finds no run waits for them and
+                                      tries once more                                  */
 
 typedef struct pbm_handle pbm_handle;
 
@@ -645,8 +652,14 @@ int pbm_autotune(pbm_handle* h, uint32_t n, int32_t* d_ids, uint32_t flags,
  *  - PBM_LAUNCH_PERSISTENT: one kernel for the whole range.  Each burst is
  *    cut into free items (at most 64 frees, whole packed runs) and alloc
  *    items (one 64-request chunk); waves take items in order from a ticket.
- *    A free waits until the pointer it frees is published; an alloc item
- *    waits until every free item of its burst is done.  Waits are bounded:
+ *    A free waits until the pointer it frees is published.  An alloc item
+ *    of burst b first waits until an earlier burst has finished (every
+ *    item): the earliest one after which the bursts up to b ask for at most
+ *    half the pool's pages, counted from the trace when it is created.  It
+ *    then runs ahead of its own burst's frees while a quarter of the pool
+ *    is unreserved; otherwise, or with PBM_LAUNCH_ORDERED, it waits until
+ *    every free item of its burst is done, and one that ran ahead and found
+ *    no run for a request waits for them and tries again.  Waits are bounded:
  *    one that gives up is counted (pbm_bursts_wait_timeouts) and the run's
  *    results are then not valid.
  *

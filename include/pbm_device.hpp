@@ -1096,8 +1096,10 @@ __device__ __forceinline__ u64 run_mask(unsigned start, unsigned n) {
 }
 
 /* Takes `want` units per lane from the reservation counter with one atomic
- * per wave; lanes are served in rank order while the running total fits. */
-__device__ inline bool reserve_n(const pbm_ctx& c, unsigned want) {
+ * per wave; lanes are served in rank order while the running total fits.
+ * With `seen`, the counter's value before the wave's take is stored there
+ * (unchanged when no lane wants anything). */
+__device__ inline bool reserve_n(const pbm_ctx& c, unsigned want, long long* seen = nullptr) {
     u64 act = ballot(want > 0);
     if (!act) return false;
     // the lanes' running total in lane order, one ballot per bit of want
@@ -1112,6 +1114,7 @@ __device__ inline bool reserve_n(const pbm_ctx& c, unsigned want) {
     long long old = 0;
     if (lane_id() == leader) old = atom_add_ll(c.free_count, -(long long)total);
     old = (long long)readlane64((u64)old, leader);
+    if (seen) *seen = old;
     bool ok = want > 0 && old >= (long long)(pre + want);
     // give back what the refused lanes asked for
     const u64 refused = act & ~ballot(ok);

@@ -10,17 +10,27 @@
 //                  mallocRW chunk each -- and waves take the items in order
 //                  from an atomic ticket.  Dependencies are waited on where
 //                  they are, not with grid barriers: a free waits until the
-//                  pointer it frees has been published, and an alloc item
-//                  searches for runs only once every free item of its burst
-//                  is done (its plan and reservation are made before).  So
-//                  bursts overlap wherever the trace allows it, and a
-//                  burst's allocations still see every page its frees
-//                  returned.
+//                  pointer it frees has been published.  An alloc item of
+//                  burst b waits until burst behind[b] has finished, the
+//                  earliest burst after which the bursts up to b ask for at
+//                  most half the pool (from the trace's sizes), so the
+//                  allocations running ahead of the frees that later give
+//                  their pages back stay bounded.  It runs ahead of its own
+//                  burst's frees while a quarter of the pool is unreserved,
+//                  and otherwise (or with PBM_LAUNCH_ORDERED) searches only
+//                  once every free item of its burst is done; one that ran
+//                  ahead and found no run waits for them and tries again.
+//                  So bursts overlap wherever the trace allows it, and a
+//                  failure is reported only after the burst's frees have
+//                  returned their pages.
 //
 // Progress: a wave waits only on items with lower tickets.  A wave takes a
-// ticket only while running and keeps at most two (the item it works on
-// and the next one, fetched ahead), so the lowest unfinished item is always
-// being worked on by a running wave, and its own dependencies are done.
+// ticket only while running, and takes its next one only once its current
+// item has nothing left to wait for, so every item taken is being worked on
+// by a running wave: the lowest unfinished item always is, and its own
+// dependencies are done.  (Fetching the next ticket ahead, while the
+// current item may still wait, leaves items that others wait on held by a
+// wave that is itself waiting; such chains grew with the wave count.)
 // Every wait is bounded anyway (2 s of wall clock); a wait that gives up
 // is counted, and the run's results are then not to be trusted.
 //
@@ -35,6 +45,7 @@
 // words only with device-scope atomics, so no fence is needed that way.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <climits>
 #include <vector>
 #include "pbm_internal.hpp"
 
@@ -57,6 +68,9 @@ struct BurstDev {
     const uint2* items;          // persistent work items, burst by burst
     const uint32_t* n_fitems;    // [n_bursts] free items of each burst
     uint32_t* fdone;             // [n_bursts] free items finished
+    uint32_t* adone;             // [n_bursts] alloc items finished
+    const uint32_t* behind;      // [n_bursts] burst b's alloc items wait until burst
+                                 // behind[b] has finished (behind[b] == b: no wait)
     uint64_t* table;             // [n_allocs] pointer returned for each request
     u64* ctl;                    // [0] ticket, [1] waits that gave up
 };
@@ -123,19 +137,22 @@ __global__ void k_bursts_prepare(BurstDev t, uint32_t a0, uint32_t a1, uint32_t 
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     for (uint32_t i = a0 + tid; i < a1; i += stride) t.table[i] = kPending;
-    for (uint32_t b = b0 + tid; b < b1; b += stride) t.fdone[b] = 0u;
+    for (uint32_t b = b0 + tid; b < b1; b += stride) t.fdone[b] = t.adone[b] = 0u;
     if (tid == 0) t.ctl[0] = 0ull;
 }
 
-// Polls `ready` until it holds: a short sleep first, longer ones after.
-// Gives up (returns false) after kWaitTicks of the 100 MHz wall clock.
+// Polls `ready` until it holds, sleeping longer the longer it waits (every
+// poll is a device-scope load performed at the memory side, and hundreds of
+// waiting waves polling at full rate slow the waves they wait for).  Gives
+// up (returns false) after kWaitTicks of the 100 MHz wall clock.
 template <typename Ready>
 __device__ inline bool wait_until(Ready ready) {
     if (ready()) return true;
     const u64 t0 = wall_clock64();
     for (uint32_t spins = 1;; ++spins) {
-        if (spins < 64u) __builtin_amdgcn_s_sleep(1);
-        else __builtin_amdgcn_s_sleep(4);
+        if (spins < 8u) __builtin_amdgcn_s_sleep(2);
+        else if (spins < 32u) __builtin_amdgcn_s_sleep(8);
+        else __builtin_amdgcn_s_sleep(32);
         if (ready()) return true;
         if ((spins & 15u) == 0u && wall_clock64() - t0 > kWaitTicks) return false;
     }
@@ -143,48 +160,108 @@ __device__ inline bool wait_until(Ready ready) {
 
 // Work items i0 .. i1-1, one wave per workgroup, items taken by ticket.
 __global__ __launch_bounds__(64) void k_bursts_persistent(pbm_ctx c, BurstDev t, uint32_t i0,
-                                                          uint32_t i1, uint32_t spans) {
+                                                          uint32_t i1, uint32_t b0,
+                                                          uint32_t spans, long long ahead_min) {
     using namespace pbm;
     Stats st;
     const unsigned lane = lane_id();
     uint32_t gave_up = 0;  // wave-uniform
     u64 tk = 0;
     if (lane == 0) tk = atom_add_u64(&t.ctl[0], 1ull);
+    long long room = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t seen_done = b0;  // bursts b0 .. seen_done-1 are known to be finished
     u64 it = (u64)i0 + readlane64(tk, 0);
     while (it < (u64)i1) {
-        // the next ticket is in flight while this item runs
+        // the next ticket is taken once this item has nothing left to wait
+        // for, and is in flight while the item finishes
         u64 nt = 0;
-        if (lane == 0) nt = atom_add_u64(&t.ctl[0], 1ull);
+        bool took = false;
+        auto take_next = [&] {
+            if (took) return;
+            if (lane == 0) nt = atom_add_u64(&t.ctl[0], 1ull);
+            took = true;
+        };
         const uint2 item = t.items[it];
         const uint32_t start = item.x;
         const uint32_t burst = item.y >> 8;
         const uint32_t cnt = (item.y >> 1) & 127u;
         if (item.y & 1u) {
-            // alloc item: one mallocRW chunk, planned while the burst's
-            // frees may still run; it reserves and searches once they are
-            // done.  (Reserving before would let the items of bursts far
-            // ahead hold pages the bursts before them need.)  Its first
-            // window is the item's home, item i at word L * frac(i * phi),
-            // so the chunks running together start apart; random jumps from
-            // the chunk's seed follow.  The home window is read while the
-            // reservation is in flight.
+            // alloc item: one mallocRW chunk.  While at least ahead_min
+            // pages are unreserved it runs ahead of its burst's frees; below
+            // that it waits for them before it reserves (so the items of
+            // bursts far ahead cannot hold pages the bursts before them
+            // need).  An item that ran ahead and found no run for a lane
+            // waits for its burst's frees and tries that lane once more, so
+            // a failure is reported only once every page the burst's frees
+            // return is back.  Its first window is the item's home, item i
+            // at word L * frac(i * phi), so the chunks running together
+            // start apart; random jumps from the chunk's seed follow.  The
+            // home window is read while the reservation is in flight.
+            // (room: the reservation counter as this wave's last reserve
+            // saw it, so the decision costs no extra round trip)
             RunPlan pl{};
             if (lane < cnt) pl = plan_malloc_rw(c, t.sizes[start + lane]);
-            const uint32_t nf = t.n_fitems[burst];
-            if (!wait_until([&] { return ld_u32(&t.fdone[burst]) >= nf; })) gave_up++;
+            // window: the bursts after behind[burst] ask for at most half
+            // the pool's pages, and it must have finished (all its items)
+            // before this item allocates, so the allocations running ahead
+            // of the frees the trace gives back after them stay bounded
+            const uint32_t wb = t.behind[burst];
+            if (wb < burst && wb >= seen_done) {
+                const uint32_t nf = t.n_fitems[wb];
+                const uint32_t na = (t.alloc_off[wb + 1] - t.alloc_off[wb] + 63u) / 64u;
+                if (!wait_until([&] {
+                        return ld_u32(&t.fdone[wb]) >= nf && ld_u32(&t.adone[wb]) >= na;
+                    }))
+                    gave_up++;
+                seen_done = wb + 1u;
+            }
+            bool done = false;
+            if (room < ahead_min) {
+                const uint32_t nf = t.n_fitems[burst];
+                if (!wait_until([&] { return ld_u32(&t.fdone[burst]) >= nf; })) gave_up++;
+                done = true;
+                take_next();
+            }
+            const uint32_t want = (lane < cnt && pl.n <= 64u) ? pl.n : 0u;
+            int32_t pid = -1;
+            bool held = false;
+            const int32_t seed = request_seed(c.seed, start);
             if (lane < cnt) {
                 const uint32_t home =
                     (uint32_t)(((u64)(uint32_t)((uint32_t)it * 2654435769u) * c.n_words) >> 32);
                 const FirstWindow win = load_window(c, home, spans);
-                const bool need = reserve_n(c, pl.n > 64u ? 0u : pl.n);
-                const int32_t pid = find_n_pages(c, pl.n, need, request_seed(c.seed, start),
-                                                 nullptr, st, spans, &win);
+                held = reserve_n(c, want, &room);
+                pid = find_n_pages(c, pl.n, held, seed, nullptr, st, spans, &win);
+            }
+            room = (long long)readlane64((u64)room, 0);  // lane 0 is always in the item
+            if (!done && ballot(lane < cnt && want > 0u && pid < 0)) {
+                const uint32_t nf = t.n_fitems[burst];
+                if (!wait_until([&] { return ld_u32(&t.fdone[burst]) >= nf; })) gave_up++;
+                const bool again = lane < cnt && want > 0u && pid < 0;
+                if (again) {
+                    // the first try counted a failure, and an error when it
+                    // held a reservation it could not place
+                    st.failures--;
+                    if (held) st.errors--;
+                }
+                if (lane < cnt) {
+                    const bool held2 = reserve_n(c, again ? want : 0u, &room);
+                    const int32_t pid2 =
+                        find_n_pages(c, again ? want : 0u, held2, seed, nullptr, st, spans);
+                    if (again) pid = pid2;
+                }
+                room = (long long)readlane64((u64)room, 0);
+            }
+            take_next();
+            if (lane < cnt) {
                 void* p = place_malloc_rw(c, pl, pid);
                 // publish: the header written before the pointer is visible
                 // with it (release; the free's load is followed by an acquire)
                 __hip_atomic_store((u64*)&t.table[start + lane], (u64)p, __ATOMIC_RELEASE,
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
+            if (lane == 0) __hip_atomic_fetch_add(&t.adone[burst], 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
         } else {
             // free item: whole packed runs, so every member reads its header
             // (one load for the wave) before any run of the item is released
@@ -195,12 +272,21 @@ __global__ __launch_bounds__(64) void k_bursts_persistent(pbm_ctx c, BurstDev t,
                 v = ld_u64(&t.table[r]);
             }
             if (ballot(v == kPending && lane < cnt)) {
+                // one pending pointer is polled per round; once it is
+                // published (its chunk's pointers are published together)
+                // every pending lane reads its own again
                 const bool ok = wait_until([&] {
+                    const u64 pend = ballot(lane < cnt && v == kPending);
+                    if (!pend) return true;
+                    const unsigned q = (unsigned)__ffsll((long long)pend) - 1u;
+                    if (lane == q) v = ld_u64(&t.table[r]);
+                    if (readlane64(v, q) == kPending) return false;
                     if (lane < cnt && v == kPending) v = ld_u64(&t.table[r]);
                     return !ballot(lane < cnt && v == kPending);
                 });
                 if (!ok) gave_up++;
             }
+            take_next();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             if (lane < cnt && v != kPending) free_rw(c, (void*)v, &st);
             // every release has landed before the item counts as done.  The
@@ -212,6 +298,7 @@ __global__ __launch_bounds__(64) void k_bursts_persistent(pbm_ctx c, BurstDev t,
             if (lane == 0) __hip_atomic_fetch_add(&t.fdone[burst], 1u, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
         }
+        take_next();
         it = (u64)i0 + readlane64(nt, 0);
     }
     if (lane == 0 && gave_up) atom_add_u64(&t.ctl[1], (u64)gave_up);
@@ -251,9 +338,10 @@ namespace {
 // twice or frees one packed run over several bursts: freeing the leader
 // returns the whole run, so a member freed later would read a header that
 // the run's next owner may have overwritten.
-int plan_items(const pbm_burst_trace* tr, uint32_t page_size, std::vector<uint2>& items,
-               std::vector<uint32_t>& item_off, std::vector<uint32_t>& n_fitems,
-               std::vector<uint32_t>& free_order) {
+int plan_items(const pbm_burst_trace* tr, uint32_t page_size, uint32_t n_pages,
+               std::vector<uint2>& items, std::vector<uint32_t>& item_off,
+               std::vector<uint32_t>& n_fitems, std::vector<uint32_t>& free_order,
+               std::vector<uint32_t>& behind) {
     const uint32_t nb = tr->n_bursts;
     const uint32_t na = tr->alloc_offset[nb];
     // chunk start of each request, and whether mallocRW packs it
@@ -323,6 +411,44 @@ int plan_items(const pbm_burst_trace* tr, uint32_t page_size, std::vector<uint2>
         }
     }
     item_off[nb] = (uint32_t)items.size();
+    // pages each burst's mallocRW chunks ask for: a large request its own
+    // run, the small ones of a chunk one packed run (runs over 64 pages
+    // fail and take nothing)
+    std::vector<uint64_t> apages(nb, 0);
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t a0 = tr->alloc_offset[b], a1 = tr->alloc_offset[b + 1];
+        for (uint32_t s = a0; s < a1; s += 64u) {
+            uint64_t packed = 0;
+            for (uint32_t r = s; r < a1 && r < s + 64u; ++r) {
+                const uint64_t need = (tr->sizes[r] + 4u + 3u) & ~(uint64_t)3u;
+                if (small[r]) {
+                    packed += need;
+                } else {
+                    const uint64_t n = (need + page_size - 1u) / page_size;
+                    if (n <= 64u) apages[b] += n;
+                }
+            }
+            const uint64_t n = (packed + page_size - 1u) / page_size;
+            if (n <= 64u) apages[b] += n;
+        }
+    }
+    // behind[b]: the earliest burst l < b such that bursts l+1 .. b ask for
+    // at most half the pool (b - 1 when b alone asks for more); b itself,
+    // meaning no wait, while bursts 0 .. b together ask for at most that
+    behind.assign(nb, 0);
+    const uint64_t budget = n_pages / 2u;
+    std::vector<uint64_t> pre(nb + 1, 0);  // pre[i] = apages[0 .. i-1]
+    for (uint32_t b = 0; b < nb; ++b) pre[b + 1] = pre[b] + apages[b];
+    for (uint32_t b = 0; b < nb; ++b) {
+        if (b == 0 || pre[b + 1] <= budget) {
+            behind[b] = b;
+            continue;
+        }
+        const uint64_t target = pre[b + 1] - budget;  // need pre[l + 1] >= target
+        const uint32_t j = (uint32_t)(std::lower_bound(pre.begin() + 1, pre.begin() + b + 1,
+                                                       target) - pre.begin());
+        behind[b] = j <= b ? j - 1u : b - 1u;
+    }
     return PBM_OK;
 }
 
@@ -352,8 +478,9 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
             if (tr->free_index[k] >= tr->alloc_offset[i])
                 return pbmi::fail(PBM_ERR_INVALID, "a free must target an earlier burst's request");
     std::vector<uint2> items;
-    std::vector<uint32_t> item_off, n_fitems, free_order;
-    int rc = plan_items(tr, h->ctx.page_size, items, item_off, n_fitems, free_order);
+    std::vector<uint32_t> item_off, n_fitems, free_order, behind;
+    int rc = plan_items(tr, h->ctx.page_size, h->ctx.n_pages, items, item_off, n_fitems,
+                        free_order, behind);
     if (rc) return rc;
 
     pbm_bursts* b = new pbm_bursts();
@@ -370,12 +497,12 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
         b->mem.push_back(*p);
         return PBM_OK;
     };
-    void* p[10] = {};
-    const size_t bytes[10] = {(size_t)(nb + 1) * 4u, (size_t)(nb + 1) * 4u, (size_t)na * 8u,
+    void* p[12] = {};
+    const size_t bytes[12] = {(size_t)(nb + 1) * 4u, (size_t)(nb + 1) * 4u, (size_t)na * 8u,
                               (size_t)nf * 4u,       (size_t)nf * 4u,       items.size() * 8u,
                               (size_t)nb * 4u,       (size_t)nb * 4u,       (size_t)na * 8u,
-                              64u};
-    for (int i = 0; i < 10 && !rc; ++i) rc = alloc(&p[i], bytes[i]);
+                              64u,                   (size_t)nb * 4u,       (size_t)nb * 4u};
+    for (int i = 0; i < 12 && !rc; ++i) rc = alloc(&p[i], bytes[i]);
     if (rc) {
         pbm_bursts_destroy(b);
         return rc;
@@ -390,6 +517,8 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
     b->d.fdone = (uint32_t*)p[7];
     b->d.table = (uint64_t*)p[8];
     b->d.ctl = (u64*)p[9];
+    b->d.adone = (uint32_t*)p[10];
+    b->d.behind = (const uint32_t*)p[11];
     hipError_t e = hipSuccess;
     const void* src[8] = {tr->alloc_offset, tr->free_offset, tr->sizes, tr->free_index,
                           free_order.data(), items.data(), n_fitems.data(), nullptr};
@@ -397,6 +526,8 @@ extern "C" int pbm_bursts_create(pbm_handle* h, const pbm_burst_trace* tr, pbm_b
         if (bytes[i] && src[i] && !(i == 2 && !na) && !((i == 3 || i == 4) && !nf))
             e = hipMemcpy(p[i], src[i], bytes[i], hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(p[7], 0, bytes[7]);
+    if (e == hipSuccess) e = hipMemset(p[10], 0, bytes[10]);
+    if (e == hipSuccess) e = hipMemcpy(p[11], behind.data(), bytes[11], hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemset(p[8], 0, bytes[8] ? bytes[8] : 8u);
     if (e == hipSuccess) e = hipMemset(p[9], 0, 64u);
     if (e != hipSuccess) {
@@ -448,8 +579,12 @@ int burst_launch(pbm_bursts* b, uint32_t first, uint32_t count, const pbm_launch
         if (i1 > i0) {
             uint64_t waves = (uint64_t)h->n_cu * L.waves_per_cu;
             if (waves > i1 - i0) waves = i1 - i0;
+            // run-ahead floor: a quarter of the pool unreserved
+            const long long ahead_min = (L.flags & PBM_LAUNCH_ORDERED)
+                                            ? LLONG_MAX
+                                            : (long long)((h->ctx.n_pages + 3u) / 4u);
             pbmi::launch_timed(h, k_bursts_persistent, dim3((uint32_t)waves), dim3(64), s, c, d,
-                               i0, i1, spans);
+                               i0, i1, first, spans, ahead_min);
         }
         PBM_HIP(hipGetLastError());
         return PBM_OK;

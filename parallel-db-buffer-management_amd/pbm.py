@@ -109,6 +109,7 @@ class ServeStats(C.Structure):
 SERVE_PAYLOAD, SERVE_LATENCY = 1, 2
 LAUNCH_PERSISTENT = 16
 LAUNCH_EXCLUSIVE = 32
+LAUNCH_ORDERED = 64
 
 
 class CacheConfig(C.Structure):

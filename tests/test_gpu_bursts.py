@@ -100,6 +100,8 @@ def _modes():
             "batched": pbm.launch(mode=b, waves_per_cu=4, walk_budget=16),
             "graph_phases": pbm.launch(mode=b, waves_per_cu=8, walk_budget=16),
             "persistent": pbm.launch(mode=b, waves_per_cu=8, walk_budget=16, flags=per),
+            "persistent_ordered": pbm.launch(mode=b, waves_per_cu=8, walk_budget=16,
+                                             flags=per | pbm.LAUNCH_ORDERED),
             "persistent_wide": pbm.launch(mode=b, waves_per_cu=2, walk_budget=4,
                                           cluster_words=256, flags=per),
             "graph_persistent": pbm.launch(mode=b, waves_per_cu=16, walk_budget=16, flags=per)}
@@ -132,7 +134,8 @@ def test_bursts_concurrent_end_state(gpu, mode):
         assert st["live_runs"] > 0
 
 
-@pytest.mark.parametrize("mode", ["graph_phases", "persistent", "graph_persistent"])
+@pytest.mark.parametrize("mode", ["graph_phases", "persistent", "persistent_ordered",
+                                  "graph_persistent"])
 def test_c4_trace_concurrent_end_state(gpu, c4_trace, mode):
     # the benchmark's configuration: 4,096 bursts on 262,144 empty pages
     log_case(f"bursts C4 trace concurrent {mode}")
@@ -146,6 +149,30 @@ def test_c4_trace_concurrent_end_state(gpu, c4_trace, mode):
             st = _end_state(gpu, pool, bs, c4_trace, before)
         assert st["failed_runs"] == 0 and st["live_pages"] == 38875
         assert pool.free_count() == C4_PAGES
+
+
+@pytest.mark.parametrize("ordered", [False, True])
+def test_persistent_on_a_fragmented_pool(gpu, ordered):
+    # 30% of the pages free, scattered: alloc items run ahead of their
+    # burst's frees (more than a quarter of the pool is unreserved), most
+    # multi-page runs find no room, wait for the burst's frees and try again;
+    # requests that still find none fail.  Live runs stay disjoint and in
+    # use, and every page comes back.
+    T = 1 << 16
+    log_case(f"bursts persistent, fragmented pool, ordered={ordered}")
+    tr = pbm.burst_trace(64, 512, 1.0, 16, 65536, PS, 21)
+    flags = pbm.LAUNCH_PERSISTENT | (pbm.LAUNCH_ORDERED if ordered else 0)
+    with pbm.Pool(T, PS, strategy=pbm.RW_MALLOC, backing=True, seed=5) as pool:
+        pool.prefill_bernoulli(0.7, 6)
+        before = pool.snapshot()
+        with pbm.Bursts(pool, tr) as bs:
+            bs.run(0, bs.n_bursts, pbm.launch(mode=pbm.MODE_BATCHED, waves_per_cu=8,
+                                              walk_budget=16, flags=flags))
+            gpu.cuda.synchronize()
+            fails = int((bs.ptrs() == 0).sum())
+            st = _end_state(gpu, pool, bs, tr, before)
+        assert fails > 0 and st["failed_runs"] > 0
+        assert pool.free_count() == orc.count_free(before, T)
 
 
 def test_persistent_runs_in_pieces_match_one_run(gpu):
