@@ -59,6 +59,16 @@ def progress(msg: str) -> None:
     print(f"[bench {time.perf_counter() - _T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
 
 
+def _rwbm_tas(T: int, A: int, N: int):
+    """The reference's RWBM_TAS closed form for 64-bit words (None when it
+    does not apply: every free page granted makes its last term infinite)."""
+    import models
+    try:
+        return models.RWBM_TAS(T, A, min(N, A - 1), W=64)
+    except (ValueError, ZeroDivisionError, OverflowError):
+        return None
+
+
 def traffic_of(leg: str, kernel: str):
     """HBM bytes per launch of `kernel` in `leg`, from the committed
     rocprofv3 FETCH_SIZE and WRITE_SIZE runs of these workloads (each
@@ -1161,7 +1171,10 @@ def main() -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
         "data": "synthetic",
         "config": {"workload": "1,048,576 x 4 KiB pages, Bernoulli pre-fill f=0.9 seed 42, "
-                               "262,144 one-page requests then frees (persistent batched)",
+                               "262,144 one-page requests then frees; batched launch "
+                               + ("(single-pass rank allocation: request r takes the free page "
+                                  "of rank r in map order)" if rank else
+                                  "(persistent random-walk windows)"),
                    "pages": T_PAGES, "page_size": PAGE_SIZE, "fill": FILL,
                    "requests": N_REQ, "free_pages": A, "grants": granted,
                    "launch": b["tuned"], "device": dev_name},
@@ -1176,6 +1189,14 @@ def main() -> int:
                   "probes": cp_alloc["probes"], "atomics": cp_all["atomics"],
                   "verified": ok_p},
         "speedup_vs_plain": value / value_p,
+        # against the reference's collaborative walk on the same step
+        "speedup_vs_plain_collab": (value / comparators["CollabRW_BM"]["value"]
+                                    if "CollabRW_BM" in comparators else None),
+        # the plain walk's mean steps against the reference's closed form
+        # (theoretical_models.py RWBM_TAS, 64-bit words, the grants as
+        # requests): (unsigned)seed % L on this power-of-two map keeps the
+        # LCG's low 14 bits, whose single cycle every walk follows
+        "plain_tas_closed_form": _rwbm_tas(T_PAGES, A, granted),
         "comparators": comparators,
         "batched_walk": {"value": ops_per_step / (results["batched_walk"]["ms_per_step"] * 1e-3),
                          **{k: results["batched_walk"][k] for k in
