@@ -139,11 +139,12 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
     `block` steps is put on the stream, the sequences arriving in it are
     submitted (stream-ordered, no host synchronisation), while earlier
     blocks still run.  Measured: ops/s over `timed_steps` eager steps and
-    over as many steps replayed from a captured `block`-step graph, the
+    over as many steps replayed from a captured `block`-step graph and as
+    many run by the device-resident loop (`block` steps per launch), the
     per-allocation latency of the device getPage calls and the
     arrival-to-admission wait of the sequences admitted in the eager
     region, and each step kernel's own duration."""
-    T, n_steps = 1 << 26, warm_steps + 2 * timed_steps + 200 + 2 * block
+    T, n_steps = 1 << 26, warm_steps + 3 * timed_steps + 200 + 2 * block
     n = int(20000 * n_steps * 1e-3 * 1.2) + 64
     arr, tgt = pbm.serve_trace(n, 20000.0, 1e-3, 512.0, 0.8, 7, 16384)
     pool = pbm.Pool(T, 4096, strategy=pbm.RW_BM, backing=False, seed=7)
@@ -156,11 +157,13 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
             srv.submit(arr[state["sub"]:j], tgt[state["sub"]:j], stream=stream)
             state["sub"] = j
 
-    def steps(k: int, graph=None) -> None:
+    def steps(k: int, graph=None, loop=False) -> None:
         for _ in range(k // block):
             submit_until(state["done"] + block)
             if graph is not None:
                 graph.launch(stream)
+            elif loop:
+                srv.run_loop(block, stream)
             else:
                 srv.run(block, stream)
             state["done"] += block
@@ -190,6 +193,18 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
     g.close()
     s2 = srv.stats(stream)
     progress(f"serving: {timed_steps} steps replayed from {block}-step graphs, {ms_graph:.1f} ms")
+    # the device-resident loop: `block` steps per kernel launch
+    t4 = torch.cuda.Event(enable_timing=True)
+    t5 = torch.cuda.Event(enable_timing=True)
+    t4.record(stream)
+    steps(timed_steps, loop=True)
+    t5.record(stream)
+    torch.cuda.synchronize()
+    ms_loop = t4.elapsed_time(t5)
+    s2l = srv.stats(stream)
+    loop_timeouts = srv.loop_timeouts(stream)
+    progress(f"serving: {timed_steps} steps in {block}-step device-resident loop launches, "
+             f"{ms_loop:.1f} ms")
     # each kernel's own duration over 200 more steps, with device counters
     submit_until(state["done"] + 200)
     pool.clear_counters(stream)
@@ -207,6 +222,7 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
     w_steps = (res["admit_step"][in_eager].astype(np.int64) - arr[:state["sub"]][in_eager])
     ops = (s1["allocs"] - s0["allocs"]) + (s1["frees"] - s0["frees"])
     ops_graph = (s2["allocs"] - s1["allocs"]) + (s2["frees"] - s1["frees"])
+    ops_loop = (s2l["allocs"] - s2["allocs"]) + (s2l["frees"] - s2["frees"])
     # roofline of the grow kernel: every slot's state word is scanned (4 B);
     # a grant reads the slot's length and target (8 B), reads map words
     # (8 B per probe), claims with read-modify-writes (16 B each), writes
@@ -225,6 +241,8 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
         "frees": s1["frees"] - s0["frees"],
         "graph": {"steps_per_graph": block, "ms_per_step": ms_graph / timed_steps,
                   "ops_per_s": ops_graph / (ms_graph * 1e-3)},
+        "loop": {"steps_per_launch": block, "ms_per_step": ms_loop / timed_steps,
+                 "ops_per_s": ops_loop / (ms_loop * 1e-3), "wait_timeouts": loop_timeouts},
         "kernel_ms": kms,
         "active_at_end": s4["active"], "pages_in_use_at_end": pool.count_used(stream),
         "p50_alloc_us": float(np.percentile(lat, 50)) * 1e6 / hz if lat.size else None,

@@ -497,6 +497,26 @@ int pbm_serve_read_waits(pbm_server* s, uint32_t first, uint32_t n, uint32_t* ti
 /* Runs n_steps steps (three kernels each) on the stream. */
 int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream);
 
+/* Runs n_steps steps in one kernel: the device-resident loop.  One
+ * 1,024-thread workgroup per 1,024 slots; each step workgroup 0 admits, every
+ * workgroup grows its slots, then every workgroup frees its share of the
+ * finished sequences' pages, the phases ordered by device-side counters
+ * (no kernel boundary between steps).  The same schedule and results as
+ * pbm_serve_run.  Every workgroup waits for the others, so all must be
+ * resident at once: PBM_ERR_UNSUPPORTED when the device cannot hold them,
+ * and a wait that gives up (2 s, another kernel holding the device) is
+ * counted by pbm_serve_loop_timeouts and leaves the results invalid.
+ * Capturable in a HIP graph.  Sequences submitted before the call are seen
+ * by its steps; later ones by the next call. */
+int pbm_serve_run_loop(pbm_server* s, uint32_t n_steps, void* stream);
+
+/* Captures one pbm_serve_run_loop call of n_steps steps as a HIP graph
+ * (every launch runs the next n_steps steps). */
+int pbm_serve_graph_loop(pbm_server* s, uint32_t n_steps, pbm_graph** out);
+
+/* Waits of device-resident loop runs that gave up (stays 0). */
+int pbm_serve_loop_timeouts(pbm_server* s, unsigned long long* out, void* stream);
+
 /* Runs n_steps steps (1 .. 4096) as pbm_serve_run does, each kernel
  * recording events at its own start and end, waits for them and writes the
  * mean duration of each kernel in ms: kernel_ms[0] admit, [1] grow, [2]

@@ -70,7 +70,7 @@ __device__ __forceinline__ u64 ld_ctl(const u64* p) {
 // finished and go back on the free-slot stack.  Then waiting sequences are
 // admitted in arrival order while their arrival step has come, a slot is
 // free and free pages - owed pages >= the sequence's target.
-__global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) {
+__device__ void serve_admit(const ServeDev& d) {
     __shared__ u64 s_ctl[kCtlWords];
     __shared__ long long s_free;
     __shared__ uint32_t s_arr[kAdmitWindow];
@@ -149,12 +149,14 @@ __global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) {
     }
 }
 
+__global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) { serve_admit(d); }
+
 // Grow: every admitted slot gets one page.  A sequence that reaches its
 // target is recorded as finished and its slot listed for release.  The
 // step's tallies are summed over the workgroup in LDS and added to the
 // control block with one atomic per word per workgroup.
 template <int S>
-__global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
+__device__ void serve_grow(const ServeDev& d, uint32_t slot) {
     using namespace pbm;
     // [0] finished, [1] allocations, [2] allocations still owed, [3..10]
     // the pool counters (Stats order)
@@ -162,17 +164,20 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
     __shared__ u64 s_base[2];  // first done-list and latency entries of the workgroup
     if (threadIdx.x < 11) s_sum[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
     const u64 step = ld_ctl(&d.ctl[kStep]);
     Stats st;
-    uint32_t seq = kEmpty;
-    if (slot < d.max_slots) seq = d.slot_seq[slot];
+    // the slot's sequence, length and target are loaded together (one
+    // round trip; an empty slot's length and target go unused)
+    uint32_t seq = kEmpty, len = 0, tgt = 0;
+    if (slot < d.max_slots) {
+        seq = d.slot_seq[slot];
+        len = d.slot_len[slot];
+        tgt = d.slot_target[slot];
+    }
     const bool act = seq != kEmpty;
     bool fin = false, got = false;
     uint32_t ticks = 0;
     if (act) {
-        const uint32_t len = d.slot_len[slot];
-        const uint32_t tgt = d.slot_target[slot];
         const int32_t seed = request_seed(d.c.seed, (u64)seq * d.max_pages + len);
         const u64 t0 = wall_clock64();
         const int32_t id = get_page_of<S>(d.c, slot, seed, nullptr, &st);
@@ -253,19 +258,22 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
     }
 }
 
+template <int S>
+__global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
+    serve_grow<S>(d, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // Release: the page lists of the sequences that finished at this step's
 // grow, cut into 64-page chunks dealt round-robin to the waves.  A chunk's
 // payloads are read back, checked and summed into the sequence's checksum,
 // then its pages go back to the pool.
 template <int S>
-__global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
+__device__ void serve_release(const ServeDev& d, uint32_t wave, uint32_t n_waves) {
     using namespace pbm;
     __shared__ uint32_t s_sum[9];
     if (threadIdx.x < 9) s_sum[threadIdx.x] = 0;
     __syncthreads();
     const unsigned lane = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t done_n = (uint32_t)ld_ctl(&d.ctl[kDone]);
     const bool payload = (d.flags & PBM_SERVE_PAYLOAD) && d.c.pool;
     Stats st;
@@ -343,6 +351,96 @@ __global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
     }
 }
 
+template <int S>
+__global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
+    serve_release<S>(d, (blockIdx.x * blockDim.x + threadIdx.x) >> 6,
+                     (gridDim.x * blockDim.x) >> 6);
+}
+
+// ---- the device-resident loop: n_steps steps in one kernel
+//
+// One 1,024-thread workgroup per 1,024 slots.  Each step: workgroup 0
+// admits; every workgroup grows its slots; every workgroup frees its share
+// of the finished sequences' pages (sixteen waves each, dealt as the
+// release kernel deals them).  Fewer, larger workgroups mean fewer
+// arrivals on the phase counters and fewer cache write-backs per phase.  The phases follow one another through three
+// counters in `sync`: [0] steps admitted, [1] grow shares done, [2]
+// release shares done; [3] counts waits that gave up.  A workgroup makes
+// its writes visible before it counts itself done (every wave waits for its
+// stores, then the barrier, then one device-scope release and the count),
+// and acquires after it sees what it waited for (one acquire, then the
+// barrier).  Every workgroup waits for every other,
+// so all of them must be resident at once: the launch checks that the
+// device holds them, and every wait is bounded (kLoopWaitTicks) and
+// counted when it gives up, so a device shared with other work cannot hang
+// the loop, though its results are then not valid.
+constexpr u64 kLoopWaitTicks = 200000000ull;  // 2 s of the 100 MHz wall clock
+
+// Thread 0 waits for the word to reach `target` and acquires for the
+// workgroup: its acquire invalidates this CU's L1 and this XCD's L2, and the
+// barrier orders every wave's later reads after it.
+__device__ inline void loop_wait_ge(u64* sync, uint32_t word, u64 target) {
+    if (threadIdx.x == 0) {
+        const u64* p = sync + word;
+        if (ld_ctl(p) < target) {
+            const u64 t0 = wall_clock64();
+            for (uint32_t spins = 1;; ++spins) {
+                if (spins < 16u) __builtin_amdgcn_s_sleep(2);
+                else __builtin_amdgcn_s_sleep(16);
+                if (ld_ctl(p) >= target) break;
+                if ((spins & 15u) == 0u && wall_clock64() - t0 > kLoopWaitTicks) {
+                    pbm::atom_add_u64(sync + 3, 1ull);
+                    break;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
+// Every wave waits until its own memory operations have completed (so the
+// workgroup's writes are all in this XCD's L2), then thread 0 releases for
+// the workgroup (writing the L2's dirty lines back) and counts.
+__device__ inline void loop_signal(u64* sync, uint32_t word, bool store, u64 value) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (store) __hip_atomic_store(sync + word, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else pbm::atom_add_u64(sync + word, value);
+    }
+}
+
+constexpr uint32_t kLoopThreads = 1024;  // slots per workgroup of the loop
+
+template <int S>
+__global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_t n_steps,
+                                                             u64* sync) {
+    const u64 P = gridDim.x;
+    const uint32_t wave0 = blockIdx.x * (kLoopThreads / 64u) + (threadIdx.x >> 6);
+    for (uint32_t k = 0; k < n_steps; ++k) {
+        if (blockIdx.x == 0) {
+            // the last step's pages are all back before admission counts them
+            if (k > 0) loop_wait_ge(sync, 2, (u64)k * P);
+            serve_admit(d);
+            loop_signal(sync, 0, true, (u64)k + 1u);
+        }
+        loop_wait_ge(sync, 0, (u64)k + 1u);
+        serve_grow<S>(d, blockIdx.x * kLoopThreads + threadIdx.x);
+        loop_signal(sync, 1, false, 1ull);
+        loop_wait_ge(sync, 1, (u64)(k + 1u) * P);
+        serve_release<(S == PBM_LINKED_LIST || S == PBM_OUROBOROS) ? S : PBM_RW_BM>(
+            d, wave0, (uint32_t)P * (kLoopThreads / 64u));
+        loop_signal(sync, 2, false, 1ull);
+    }
+}
+
+// Before a loop launch: its three phase counters start at zero.
+__global__ void k_serve_loop_prep(u64* sync) {
+    if (threadIdx.x < 3) sync[threadIdx.x] = 0;
+}
+
 __global__ void k_serve_init(ServeDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d.max_slots) {
@@ -378,6 +476,7 @@ struct pbm_server {
     // sequence id is ever written twice, so a copy in flight is never
     // overwritten
     uint32_t* h_mirror = nullptr;  // [3 * max_seqs]
+    u64* d_sync = nullptr;         // device-resident loop: phase counters, give-ups
 };
 
 extern "C" int pbm_serve_destroy(pbm_server* s) {
@@ -448,6 +547,10 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
     PBM_SALLOC(res_wait, uint32_t, ns)
 #undef PBM_SALLOC
     if (!rc) {
+        rc = alloc(&p, 4u * sizeof(u64));
+        if (!rc) s->d_sync = (u64*)p;
+    }
+    if (!rc) {
         const hipError_t he = hipHostMalloc((void**)&s->h_mirror, 3u * ns * 4u, hipHostMallocDefault);
         if (he != hipSuccess) rc = pbmi::hip_fail(he, "hipHostMalloc(serve mirror)", __FILE__, __LINE__);
     }
@@ -461,7 +564,8 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
         (e = hipMemset((void*)d.res_len, 0, ns * 4u)) != hipSuccess ||
         (e = hipMemset((void*)d.res_sum, 0, ns * 8u)) != hipSuccess ||
         (e = hipMemset((void*)d.res_wait, 0xFF, ns * 4u)) != hipSuccess ||
-        (e = hipMemset((void*)d.step_tick, 0, (size_t)kStepRing * 8u)) != hipSuccess) {
+        (e = hipMemset((void*)d.step_tick, 0, (size_t)kStepRing * 8u)) != hipSuccess ||
+        (e = hipMemset((void*)s->d_sync, 0, 4u * sizeof(u64))) != hipSuccess) {
         pbm_serve_destroy(s);
         return pbmi::hip_fail(e, "hipMemset(serve)", __FILE__, __LINE__);
     }
@@ -589,6 +693,60 @@ extern "C" int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream) {
     return PBM_OK;
 }
 
+namespace {
+
+template <int S>
+int launch_loop_s(pbm_server* s, uint32_t n_steps, hipStream_t st) {
+    const uint32_t groups = (s->d.max_slots + kLoopThreads - 1u) / kLoopThreads;
+    int per_cu = 0;
+    PBM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve_loop<S>, kLoopThreads,
+                                                         0));
+    if ((uint64_t)per_cu * (uint64_t)s->h->n_cu < groups)
+        return pbmi::fail(PBM_ERR_UNSUPPORTED,
+                          "the device-resident loop needs every workgroup of the slot table "
+                          "resident at once; this device holds fewer");
+    hipLaunchKernelGGL(k_serve_loop_prep, dim3(1), dim3(64), 0, st, s->d_sync);
+    hipLaunchKernelGGL(k_serve_loop<S>, dim3(groups), dim3(kLoopThreads), 0, st, s->d, n_steps,
+                       s->d_sync);
+    PBM_HIP(hipGetLastError());
+    return PBM_OK;
+}
+
+}  // namespace
+
+extern "C" int pbm_serve_run_loop(pbm_server* s, uint32_t n_steps, void* stream) {
+    if (!s) return pbmi::fail(PBM_ERR_INVALID, "null server");
+    PBM_CHECK_HANDLE(s->h);
+    if (n_steps == 0) return PBM_OK;
+    hipStream_t st = pbmi::as_stream(stream);
+    switch (s->h->cfg.strategy) {
+#define PBM_LOOP_CASE(S) \
+    case S: return launch_loop_s<S>(s, n_steps, st);
+        PBM_LOOP_CASE(PBM_RW_BM)
+        PBM_LOOP_CASE(PBM_COLLAB_RW_BM)
+        PBM_LOOP_CASE(PBM_CRW_BM)
+        PBM_LOOP_CASE(PBM_RW_MALLOC)
+        PBM_LOOP_CASE(PBM_LINKED_LIST)
+        PBM_LOOP_CASE(PBM_SINGLE_CLOCK)
+        PBM_LOOP_CASE(PBM_DIST_CLOCK)
+        PBM_LOOP_CASE(PBM_RW_BASIC)
+        PBM_LOOP_CASE(PBM_CRW_BASIC)
+        PBM_LOOP_CASE(PBM_COLLAB_RW_BASIC)
+        PBM_LOOP_CASE(PBM_OUROBOROS)
+#undef PBM_LOOP_CASE
+        default: return pbmi::fail(PBM_ERR_UNSUPPORTED, "no serving loop for this strategy");
+    }
+}
+
+extern "C" int pbm_serve_loop_timeouts(pbm_server* s, unsigned long long* out, void* stream) {
+    if (!s || !out) return pbmi::fail(PBM_ERR_INVALID, "null server or output");
+    PBM_CHECK_HANDLE(s->h);
+    hipStream_t st = pbmi::as_stream(stream);
+    PBM_HIP(hipMemcpyAsync(out, s->d_sync + 3, 8u, hipMemcpyDeviceToHost, st));
+    PBM_HIP(hipStreamSynchronize(st));
+    return PBM_OK;
+}
+
 extern "C" int pbm_serve_run_timed(pbm_server* s, uint32_t n_steps, float* kernel_ms,
                                    void* stream) {
     if (!s || !kernel_ms) return pbmi::fail(PBM_ERR_INVALID, "null server or output");
@@ -626,7 +784,11 @@ extern "C" int pbm_serve_run_timed(pbm_server* s, uint32_t n_steps, float* kerne
     return PBM_OK;
 }
 
-extern "C" int pbm_serve_graph(pbm_server* s, uint32_t n_steps, pbm_graph** out) {
+namespace {
+
+// Captures n_steps steps, as step kernels or as one device-resident loop
+// launch, into a HIP graph.
+int capture_steps(pbm_server* s, uint32_t n_steps, bool loop, pbm_graph** out) {
     if (!s || !out) return pbmi::fail(PBM_ERR_INVALID, "null server or output graph");
     PBM_CHECK_HANDLE(s->h);
     *out = nullptr;
@@ -640,7 +802,8 @@ extern "C" int pbm_serve_graph(pbm_server* s, uint32_t n_steps, pbm_graph** out)
     if (e != hipSuccess) {
         rc = pbmi::hip_fail(e, "hipStreamBeginCapture", __FILE__, __LINE__);
     } else {
-        for (uint32_t k = 0; k < n_steps && !rc; ++k) rc = launch_step(s, st);
+        if (loop) rc = pbm_serve_run_loop(s, n_steps, st);
+        for (uint32_t k = 0; !loop && k < n_steps && !rc; ++k) rc = launch_step(s, st);
         hipGraph_t graph = nullptr;
         e = hipStreamEndCapture(st, &graph);
         g->graph = graph;
@@ -657,6 +820,16 @@ extern "C" int pbm_serve_graph(pbm_server* s, uint32_t n_steps, pbm_graph** out)
     }
     *out = g;
     return PBM_OK;
+}
+
+}  // namespace
+
+extern "C" int pbm_serve_graph(pbm_server* s, uint32_t n_steps, pbm_graph** out) {
+    return capture_steps(s, n_steps, false, out);
+}
+
+extern "C" int pbm_serve_graph_loop(pbm_server* s, uint32_t n_steps, pbm_graph** out) {
+    return capture_steps(s, n_steps, true, out);
 }
 
 extern "C" int pbm_serve_read_stats(pbm_server* s, pbm_serve_stats* out, void* stream) {

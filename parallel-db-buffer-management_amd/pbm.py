@@ -214,6 +214,9 @@ _SIGS = {
                                        C.c_void_p]),
     "pbm_serve_run": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "pbm_serve_run_timed": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "pbm_serve_run_loop": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pbm_serve_graph_loop": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "pbm_serve_loop_timeouts": (C.c_int, [C.c_void_p, C.POINTER(C.c_ulonglong), C.c_void_p]),
     "pbm_serve_graph": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "pbm_serve_read_stats": (C.c_int, [C.c_void_p, C.POINTER(ServeStats), C.c_void_p]),
     "pbm_serve_read_results": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
@@ -629,6 +632,16 @@ class Server:
     def run(self, n_steps: int, stream=None) -> None:
         check(lib().pbm_serve_run(self.s, n_steps, _stream(stream)), "pbm_serve_run")
 
+    def run_loop(self, n_steps: int, stream=None) -> None:
+        """n_steps steps in one kernel (the device-resident loop)."""
+        check(lib().pbm_serve_run_loop(self.s, n_steps, _stream(stream)), "pbm_serve_run_loop")
+
+    def loop_timeouts(self, stream=None) -> int:
+        v = C.c_ulonglong()
+        check(lib().pbm_serve_loop_timeouts(self.s, C.byref(v), _stream(stream)),
+              "pbm_serve_loop_timeouts")
+        return int(v.value)
+
     def run_timed(self, n_steps: int, stream=None) -> dict:
         """Runs n_steps steps; the mean kernel-own ms of admit, grow, release."""
         out = (C.c_float * 3)()
@@ -639,6 +652,12 @@ class Server:
     def graph(self, n_steps: int) -> Graph:
         g = C.c_void_p()
         check(lib().pbm_serve_graph(self.s, n_steps, C.byref(g)), "pbm_serve_graph")
+        return _StepGraph(g, self)
+
+    def graph_loop(self, n_steps: int) -> Graph:
+        """One device-resident loop launch of n_steps steps as a HIP graph."""
+        g = C.c_void_p()
+        check(lib().pbm_serve_graph_loop(self.s, n_steps, C.byref(g)), "pbm_serve_graph_loop")
         return _StepGraph(g, self)
 
     def stats(self, stream=None) -> dict:

@@ -67,6 +67,91 @@ def test_serve_schedule_matches_oracle(gpu, strategy):
         assert np.array_equal(pool.snapshot(), orc.prefill(T, 0.5, 3))
 
 
+def _run_loop_to_end(server, n_steps, chunk=64):
+    done = 0
+    while done < n_steps:
+        k = min(chunk, n_steps - done)
+        server.run_loop(k)
+        done += k
+
+
+@pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.COLLAB_RW_BM, pbm.CRW_BM, pbm.RW_MALLOC,
+                                      pbm.SINGLE_CLOCK, pbm.DIST_CLOCK, pbm.LINKED_LIST,
+                                      pbm.RW_BASIC, pbm.CRW_BASIC, pbm.COLLAB_RW_BASIC,
+                                      pbm.OUROBOROS])
+def test_serve_loop_schedule_matches_oracle(gpu, strategy):
+    # the device-resident loop (many steps per kernel, phases ordered by
+    # device-side counters) gives the oracle's schedule too
+    T, n = 1 << 16, 2500
+    log_case(f"serve loop schedule strategy={strategy}")
+    arr, tgt = _trace(n, 8000, 60, 0.8, 2048)
+    with pbm.Pool(T, 64, strategy=strategy, backing=True, seed=5) as pool:
+        pool.prefill_bernoulli(0.5, 3)
+        A = pool.free_count()
+        adm, fin, tot = orc.serve_sim(A, 512, arr, tgt, 100000)
+        steps = int(fin.max()) + 2
+        with pbm.Server(pool, 512, 2048, n, pbm.SERVE_PAYLOAD, seed=SEED) as srv:
+            srv.submit(arr, tgt)
+            _run_loop_to_end(srv, steps, chunk=97)
+            res = srv.results()
+            st = srv.stats()
+            assert srv.loop_timeouts() == 0
+        assert np.array_equal(res["admit_step"], adm)
+        assert np.array_equal(res["finish_step"], fin)
+        _check_results(res, np.arange(n), tgt)
+        assert st["alloc_failures"] == 0 and st["payload_errors"] == 0
+        assert st["allocs"] == tot[0] and st["frees"] == tot[1]
+        assert st["admitted"] == st["finished"] == n and st["active"] == 0
+        assert pool.free_count() == A
+        assert np.array_equal(pool.snapshot(), orc.prefill(T, 0.5, 3))
+
+
+@pytest.mark.parametrize("slots", [4096, 32768])
+def test_serve_loop_equals_eager_steps(gpu, slots):
+    # many workgroups (16 and 128): loop runs, eager steps and a graph of
+    # loop runs give the same schedule, lengths, checksums and counters,
+    # with live submission block by block
+    T, n, B = 1 << 20, 12000, 50
+    log_case(f"serve loop vs eager slots={slots}")
+    arr, tgt = _trace(n, 20000, 200, 0.8, 4096, seed=17)
+    keys = (np.arange(n) * 13 + 1).astype(np.uint32)
+    out = []
+    for mode in ("eager", "loop", "loop_graph"):
+        with pbm.Pool(T, 64, strategy=pbm.RW_BM, backing=True, seed=9) as pool:
+            pool.prefill_bernoulli(0.2, 2)
+            with pbm.Server(pool, slots, 4096, n, pbm.SERVE_PAYLOAD | pbm.SERVE_LATENCY,
+                            seed=SEED, lat_capacity=1 << 22) as srv:
+                sub = 0
+                g = None
+                for blk in range(60):
+                    j = int(np.searchsorted(arr, (blk + 1) * B, side="right"))
+                    if j > sub:
+                        srv.submit(arr[sub:j], tgt[sub:j], keys[sub:j])
+                        sub = j
+                    if mode == "eager":
+                        srv.run(B)
+                    elif mode == "loop":
+                        srv.run_loop(B)
+                    else:
+                        if g is None:
+                            g = srv.graph_loop(B)
+                        g.launch()
+                if g is not None:
+                    g.close()
+                gpu.cuda.synchronize()
+                res = srv.results()
+                st = srv.stats()
+                assert srv.loop_timeouts() == 0
+                assert st["payload_errors"] == 0 and st["alloc_failures"] == 0
+                out.append((res, st))
+    (a, sa) = out[0]
+    for b, sb in out[1:]:
+        for k in ("admit_step", "finish_step", "length", "checksum"):
+            assert np.array_equal(a[k], b[k]), k
+        assert sa == sb
+    assert (a["finish_step"] > 0).sum() > 1000
+
+
 def test_serve_alone_equals_in_mix(gpu):
     T, n = 1 << 16, 3000
     log_case("serve alone vs in mix")
