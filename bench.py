@@ -579,6 +579,25 @@ def cache_leg(torch, pbm, stream, n_refs: int, batch: int) -> dict:
                                 "load": "k_cache_load<2>"}[dom],
                      "bytes_per_launch": bytes_k[dom], "launch_ms": kms[dom]},
     }
+    # larger batches: fewer launches, judged against an older cache state
+    out["batch_sweep"] = {}
+    for b2 in (1 << 18, 1 << 20):
+        pool = pbm.Pool(T, 4096, strategy=pbm.CRW_BM, backing=False, seed=7)
+        cache = pbm.Cache(pool, L, C, b2)
+        torch.cuda.synchronize()
+        t0.record(stream)
+        cache.run(refs, stream=stream)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        ms2 = t0.elapsed_time(t1)
+        st3 = cache.stats(stream)
+        cache.close()
+        pool.close()
+        out["batch_sweep"][str(b2)] = {"ms": ms2, "refs_per_s": n_refs / (ms2 * 1e-3),
+                                       "hits": st3["hits"], "evictions": st3["evictions"],
+                                       "hit_rate": st3["hits"] / n_refs,
+                                       "scan_timeouts": st3["scan_timeouts"]}
+        progress(f"cache: batches of {b2}: {ms2:.1f} ms")
     # the references on the host for the serial CLOCK baseline
     out["_cpu_refs"] = refs.cpu().numpy()
     del refs
@@ -876,6 +895,8 @@ def cpu_legs(orc, c1, bursts, cache, serving, seconds: float) -> None:
             "hits": int(cc.stats[0]), "misses": int(cc.stats[1]), "evictions": int(cc.stats[2]),
             "hit_rate": float(cc.stats[0]) / refs.size,
             "batched_minus_serial_hits": cache["hits"] - int(cc.stats[0])}
+        for row in cache.get("batch_sweep", {}).values():
+            row["batched_minus_serial_hits"] = row["hits"] - int(cc.stats[0])
         progress("cpu baseline: cache")
     if serving and "_cpu_sample_map" in serving:
         bm, g = serving["_cpu_sample_map"], serving["_cpu_sample_grants"]

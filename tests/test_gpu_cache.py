@@ -123,12 +123,14 @@ def test_cache_frame_policies(gpu, strategy):
             _check(gpu, pool, cache, cc, T)
 
 
-def test_cache_c3_sizes(gpu):
+@pytest.mark.parametrize("B", [1 << 16, 1 << 18, 1 << 20])
+def test_cache_c3_sizes(gpu, B):
     # the buffer-cache workload whole: 16,777,216 frames (page map only),
-    # 4,194,304 logical pages, 1,048,576 slots, Zipf(1.1) seed 7, batches
-    # of 65,536, all 67,108,864 references
-    T, L, C, B, n = 1 << 24, 1 << 22, 1 << 20, 1 << 16, 1 << 26
-    log_case("cache C3 sizes, 64M references")
+    # 4,194,304 logical pages, 1,048,576 slots, Zipf(1.1) seed 7, all
+    # 67,108,864 references, in batches of 65,536 (the benchmark's) and of
+    # 262,144 and 1,048,576 (as many as the slots)
+    T, L, C, n = 1 << 24, 1 << 22, 1 << 20, 1 << 26
+    log_case(f"cache C3 sizes, 64M references, batch {B}")
     refs = _trace(gpu, L, 1.1, 7, n)
     h_refs = refs.cpu().numpy()
     cc = orc.ClockCache(L, C, B, log_cap=n)
@@ -139,11 +141,13 @@ def test_cache_c3_sizes(gpu):
             gpu.cuda.synchronize()
             _check(gpu, pool, cache, cc, T)
     # the textbook CLOCK (one reference at a time) on the same trace: the
-    # batch rule's hit count lies within 0.01% of the references of it
-    tb = orc.ClockCache(L, C, 1)
-    tb.run_textbook(h_refs)
-    assert int(tb.stats[0] + tb.stats[1]) == n
-    assert abs(int(cc.stats[0]) - int(tb.stats[0])) < n // 10000
+    # batch rule's hit count lies within 0.01% of the references of it at
+    # the benchmark's batch
+    if B == 1 << 16:
+        tb = orc.ClockCache(L, C, 1)
+        tb.run_textbook(h_refs)
+        assert int(tb.stats[0] + tb.stats[1]) == n
+        assert abs(int(cc.stats[0]) - int(tb.stats[0])) < n // 10000
 
 
 def test_cache_rejects_bad_configs(gpu):
