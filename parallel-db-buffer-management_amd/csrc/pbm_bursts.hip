@@ -169,7 +169,11 @@ __global__ __launch_bounds__(64) void k_bursts_persistent(pbm_ctx c, BurstDev t,
     u64 tk = 0;
     if (lane == 0) tk = atom_add_u64(&t.ctl[0], 1ull);
     long long room = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t seen_done = b0;  // bursts b0 .. seen_done-1 are known to be finished
+    // burst seen_done-1 is known to have finished (the window's burst only
+    // moves forward for a wave: its tickets, and so its bursts, increase,
+    // and behind[] does not decrease); a burst before b0 finished with the
+    // launches before this one
+    uint32_t seen_done = b0;
     u64 it = (u64)i0 + readlane64(tk, 0);
     while (it < (u64)i1) {
         // the next ticket is taken once this item has nothing left to wait
