@@ -343,6 +343,18 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             __syncthreads();
         }
         r = base + mine;
+    } else if (WPT == 1 && ballot(mask[0] == kFull) == kFull) {
+        // every lane of the wave was granted its whole word (an empty map):
+        // the wave's 4,096 ranks are its 64 words' pages in order, so it
+        // writes them as 64 stores of 64 consecutive ids each (lane l's
+        // ranks start 64 l after lane 0's, and its word is lane 0's + l)
+        const uint32_t r0 = readlane(base, 0);
+        const uint32_t p0 = readlane(w0, 0) * 64u;
+        for (uint32_t j = 0; j < 64u; ++j) {
+            ids[r0 + j * 64u + lane] = (int32_t)(p0 + j * 64u + lane);
+            if (steps_out) steps_out[r0 + j * 64u + lane] = 1u;
+        }
+        r = base + 64u;
     } else {
         // each thread its own
 #pragma unroll
