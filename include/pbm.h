@@ -507,7 +507,9 @@ int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream);
  * and a wait that gives up (2 s, another kernel holding the device) is
  * counted by pbm_serve_loop_timeouts and leaves the results invalid.
  * Capturable in a HIP graph.  Sequences submitted before the call are seen
- * by its steps; later ones by the next call. */
+ * by its steps; later ones by the next call.  Like every call on a server,
+ * it must be ordered with the server's other calls (one stream, or
+ * synchronised): two runs of one server at once race on its slot table. */
 int pbm_serve_run_loop(pbm_server* s, uint32_t n_steps, void* stream);
 
 /* Captures one pbm_serve_run_loop call of n_steps steps as a HIP graph
