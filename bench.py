@@ -1142,7 +1142,7 @@ def main() -> int:
 
     serving = (serving_leg(torch, pbm, stream, args.serve_warm, args.serve_steps)
                if args.serve_steps > 0 else None)
-    bursts = (bursts_leg(torch, pbm, stream, args.bursts, 256, bool(args.bursts_persistent))
+    bursts = (bursts_leg(torch, pbm, stream, args.bursts, 1024, bool(args.bursts_persistent))
               if args.bursts > 0 else None)
     cache = (cache_leg(torch, pbm, stream, args.cache_refs, args.cache_batch)
              if args.cache_refs > 0 else None)
