@@ -116,6 +116,7 @@ constexpr u64 kScanIncl = 1ull << 62;
 constexpr uint32_t kScanSat = 0x7FFFFFFFu;
 constexpr uint32_t kScanSpin = 1u << 20;  // look-back polls before giving up
 constexpr uint32_t kRankStage = 4096;     // ids staged in LDS per pass (large tiles)
+constexpr uint32_t kGroupTiles = 64;      // tiles per group of the two-level look-back
 constexpr int kLookRounds = 4;            // look-back reads 64 * kLookRounds words at once
 
 __device__ __forceinline__ u64 scan_word(uint32_t parity, bool incl, uint32_t grant, u64 v) {
@@ -209,6 +210,65 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             if (lane == 0) {
                 st_scan(&c.scan[0], scan_word(parity, true, grant, agg));
                 if (grant) atom_add_ll(c.free_count, -(long long)grant);
+            }
+        } else if (c.n_tiles > kGroupTiles * 4u) {
+            // two-level look-back, for maps of more than 256 tiles: tile
+            // counts are never replaced by prefixes; the last tile of each
+            // group of 64 publishes the group's total (group 0's with the
+            // grant) once its group's counts are out, and a tile sums its
+            // group's tiles before it and the groups before its own.  No
+            // prefix is passed from tile to tile, so no chain forms.
+            if (lane == 0) st_scan(&c.scan[tile], scan_word(parity, false, 0, agg));
+            u64* gagg = c.scan + c.n_tiles;
+            const uint32_t grp = tile / kGroupTiles, first = grp * kGroupTiles;
+            const uint32_t npre = tile - first;
+            const uint32_t last = first + kGroupTiles - 1u < c.n_tiles - 1u
+                                      ? first + kGroupTiles - 1u : c.n_tiles - 1u;
+            uint32_t spins = 0;
+            bool gave_up = false;
+            u64 v = 0;
+            for (;;) {  // this group's tiles before this one
+                bool ready = true;
+                if (lane < npre) {
+                    v = ld_word(&c.scan[first + lane]);
+                    ready = (uint32_t)(v >> 63) == parity;
+                }
+                if (!ballot(!ready)) break;
+                if (++spins > kScanSpin) { gave_up = true; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            u64 part = lane < npre ? (v & kScanSat) : 0ull;
+#pragma unroll
+            for (int d = 32; d >= 1; d >>= 1) part += shfl64(part, lane ^ (unsigned)d);
+            excl = part;
+            if (grp == 0) grant = (uint32_t)((readlane64(v, 0) >> 31) & 0x7FFFFFFFull);
+            if (tile == last && lane == 0)
+                st_scan(&gagg[grp], scan_word(parity, true, grp == 0 ? grant : 0u, part + agg));
+            for (uint32_t r0 = 0; r0 < grp && !gave_up; r0 += 64u) {  // the groups before
+                const uint32_t gi = r0 + lane;
+                for (;;) {
+                    bool ready = true;
+                    v = 0;
+                    if (gi < grp) {
+                        v = ld_word(&gagg[gi]);
+                        ready = (uint32_t)(v >> 63) == parity;
+                    }
+                    if (!ballot(!ready)) break;
+                    if (++spins > kScanSpin) { gave_up = true; break; }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                part = gi < grp ? (v & kScanSat) : 0ull;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) part += shfl64(part, lane ^ (unsigned)d);
+                excl += part;
+                if (r0 == 0) grant = (uint32_t)((readlane64(v, 0) >> 31) & 0x7FFFFFFFull);
+            }
+            if (gave_up) {
+                // cannot happen while every started tile runs on; recorded,
+                // and this tile grants nothing
+                if (lane == 0) atom_add_u64(&c.scan_ctl[1], 1ull);
+                st.errors++;
+                grant = 0;
             }
         } else {
             if (lane == 0) st_scan(&c.scan[tile], scan_word(parity, false, 0, agg));

@@ -212,7 +212,10 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     const uint32_t scan_threads = n_words <= PBM_SMALL_SCAN_WORDS ? 64u : pbm::kScanThreads;
     const uint32_t tile_words = scan_threads * scan_wpt;
     const uint32_t n_tiles = (n_words + tile_words - 1u) / tile_words;
-    PBM_ALLOC(h->d_scan, (size_t)n_tiles * 8u);
+    // one look-back word per tile, then one per group of 64 tiles (the
+    // two-level look-back of maps with more than 256 tiles)
+    const size_t scan_words = (size_t)n_tiles + (n_tiles + 63u) / 64u;
+    PBM_ALLOC(h->d_scan, scan_words * 8u);
     PBM_ALLOC(h->d_scan_ctl, 32u);
     if (cfg->backing) PBM_ALLOC(h->d_pool, (size_t)cfg->n_pages * cfg->page_size);
     if (cfg->strategy == PBM_CRW_BM || cfg->strategy == PBM_CRW_BASIC)
@@ -266,7 +269,7 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
     // Look-back words start with call parity 0: the first call has parity 1
     // and no zeroed word reads as published by it.
-    e = hipMemset(h->d_scan, 0, (size_t)n_tiles * 8u);
+    e = hipMemset(h->d_scan, 0, scan_words * 8u);
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
     {
         const unsigned long long ctl[4] = {0, 0, 0, 0};

@@ -123,6 +123,9 @@ def test_one_wave_batched_budget_sweep_matches_oracle(gpu, T, fill, n, budget, o
                                       (65536, 0.0, 32768), (1000, 0.4, 700), (1 << 16, 0.999, 100),
                                       (1 << 22, 0.9, 300000), (1 << 24, 0.97, 200000),
                                       (64 * 256 * 65 + 17, 0.6, 1 << 20),
+                                      # more than 256 tiles: two-level look-back,
+                                      # a last group of 45 tiles
+                                      (301 * 256 * 64 - 4000, 0.7, 1 << 20),
                                       # 2^20 map words and more: four words per thread
                                       (1 << 26, 0.9, 300000), (1 << 26, 0.999, 100000)])
 @pytest.mark.parametrize("one_wave,excl", [(False, False), (True, False), (False, True),
