@@ -115,7 +115,7 @@ __global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t 
 constexpr u64 kScanIncl = 1ull << 62;
 constexpr uint32_t kScanSat = 0x7FFFFFFFu;
 constexpr uint32_t kScanSpin = 1u << 20;  // look-back polls before giving up
-constexpr uint32_t kRankStage = 4096;     // ids staged in LDS per pass (large tiles)
+constexpr uint32_t kRankStage = 8192;     // ids staged in LDS per pass (large tiles)
 constexpr uint32_t kGroupTiles = 64;      // tiles per group of the two-level look-back
 constexpr int kLookRounds = 4;            // look-back reads 64 * kLookRounds words at once
 
