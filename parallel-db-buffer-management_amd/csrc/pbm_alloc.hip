@@ -117,6 +117,9 @@ constexpr uint32_t kScanSat = 0x7FFFFFFFu;
 constexpr uint32_t kScanSpin = 1u << 20;  // look-back polls before giving up
 constexpr uint32_t kRankStage = 8192;     // ids staged in LDS per pass (large tiles)
 constexpr uint32_t kGroupTiles = 64;      // tiles per group of the two-level look-back
+constexpr uint32_t kGroupStride = 32;     // words between group totals (256 B: each in
+                                          // a line and channel of its own, since every
+                                          // later tile polls them)
 constexpr int kLookRounds = 4;            // look-back reads 64 * kLookRounds words at once
 
 __device__ __forceinline__ u64 scan_word(uint32_t parity, bool incl, uint32_t grant, u64 v) {
@@ -235,7 +238,7 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
                 }
                 if (!ballot(!ready)) break;
                 if (++spins > kScanSpin) { gave_up = true; break; }
-                __builtin_amdgcn_s_sleep(1);
+                if (spins < 8u) __builtin_amdgcn_s_sleep(1); else __builtin_amdgcn_s_sleep(2);
             }
             u64 part = lane < npre ? (v & kScanSat) : 0ull;
 #pragma unroll
@@ -243,19 +246,20 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             excl = part;
             if (grp == 0) grant = (uint32_t)((readlane64(v, 0) >> 31) & 0x7FFFFFFFull);
             if (tile == last && lane == 0)
-                st_scan(&gagg[grp], scan_word(parity, true, grp == 0 ? grant : 0u, part + agg));
+                st_scan(&gagg[grp * kGroupStride],
+                        scan_word(parity, true, grp == 0 ? grant : 0u, part + agg));
             for (uint32_t r0 = 0; r0 < grp && !gave_up; r0 += 64u) {  // the groups before
                 const uint32_t gi = r0 + lane;
                 for (;;) {
                     bool ready = true;
                     v = 0;
                     if (gi < grp) {
-                        v = ld_word(&gagg[gi]);
+                        v = ld_word(&gagg[gi * kGroupStride]);
                         ready = (uint32_t)(v >> 63) == parity;
                     }
                     if (!ballot(!ready)) break;
                     if (++spins > kScanSpin) { gave_up = true; break; }
-                    __builtin_amdgcn_s_sleep(1);
+                    if (spins < 8u) __builtin_amdgcn_s_sleep(1); else __builtin_amdgcn_s_sleep(2);
                 }
                 part = gi < grp ? (v & kScanSat) : 0ull;
 #pragma unroll

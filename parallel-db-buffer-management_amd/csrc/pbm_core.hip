@@ -212,9 +212,9 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     const uint32_t scan_threads = n_words <= PBM_SMALL_SCAN_WORDS ? 64u : pbm::kScanThreads;
     const uint32_t tile_words = scan_threads * scan_wpt;
     const uint32_t n_tiles = (n_words + tile_words - 1u) / tile_words;
-    // one look-back word per tile, then one per group of 64 tiles (the
-    // two-level look-back of maps with more than 256 tiles)
-    const size_t scan_words = (size_t)n_tiles + (n_tiles + 63u) / 64u;
+    // one look-back word per tile, then one per group of 64 tiles, 32 words
+    // apart (the two-level look-back of maps with more than 256 tiles)
+    const size_t scan_words = (size_t)n_tiles + 32u * ((n_tiles + 63u) / 64u);
     PBM_ALLOC(h->d_scan, scan_words * 8u);
     PBM_ALLOC(h->d_scan_ctl, 32u);
     if (cfg->backing) PBM_ALLOC(h->d_pool, (size_t)cfg->n_pages * cfg->page_size);
