@@ -157,22 +157,26 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
     constexpr uint32_t kWaves = THREADS / 64;
     __shared__ uint32_t s_parity, s_excl, s_grant;
     __shared__ long long s_free;
-    __shared__ uint32_t s_wsum[kWaves];
+    __shared__ uint32_t s_wsum[WPT][kWaves];
     const unsigned lane = lane_id();
     const unsigned wv = threadIdx.x >> 6;
-    // thread t holds WPT consecutive words of the tile
-    const uint32_t w0 = tile * kTileWords + threadIdx.x * WPT;
+    // thread t holds word k * THREADS + t of the tile, k < WPT: each load
+    // and each claim instruction of a wave covers 64 consecutive words
+    // (eight words a line, so the memory side takes a claim per line, not
+    // per two words), and the tile's map order is k-major
+    const uint32_t w0 = tile * kTileWords + threadIdx.x;
     Stats st;
     u64 fm[WPT];
-    uint32_t cnt = 0;
+    uint32_t cntk[WPT];
 #pragma unroll
     for (int k = 0; k < WPT; ++k) {
+        const uint32_t wi = w0 + (uint32_t)k * THREADS;
         fm[k] = 0;
-        if (w0 + k < c.n_words) {
-            fm[k] = ~ld_word(&c.bitmap[w0 + k]);  // padding bits past n_pages read as used
+        if (wi < c.n_words) {
+            fm[k] = ~ld_word(&c.bitmap[wi]);  // padding bits past n_pages read as used
             st.probes++;
         }
-        cnt += (uint32_t)__popcll(fm[k]);
+        cntk[k] = (uint32_t)__popcll(fm[k]);
     }
     // this tile's word from the last call, and the free count for tile 0's
     // grant, read while the map words are in flight
@@ -183,25 +187,40 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
         if (tile == 0)
             fr = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    uint32_t inc = cnt;  // inclusive scan of the free counts over the wave
+    // inclusive scans of each word row's free counts over the wave
+    uint32_t pre[WPT];
 #pragma unroll
-    for (unsigned d = 1; d < 64u; d <<= 1) {
-        const uint32_t v = shfl32(inc, lane >= d ? lane - d : lane);
-        if (lane >= d) inc += v;
+    for (int k = 0; k < WPT; ++k) {
+        uint32_t inc = cntk[k];
+#pragma unroll
+        for (unsigned d = 1; d < 64u; d <<= 1) {
+            const uint32_t v = shfl32(inc, lane >= d ? lane - d : lane);
+            if (lane >= d) inc += v;
+        }
+        pre[k] = inc - cntk[k];
+        if (lane == 63) s_wsum[k][wv] = inc;
     }
-    const uint32_t pre = inc - cnt;
-    if (lane == 63) s_wsum[wv] = inc;
     if (threadIdx.x == 0) {
         s_parity = (uint32_t)(own >> 63) ^ 1u;
         s_free = fr;
     }
     __syncthreads();
     const uint32_t parity = s_parity;
-    uint32_t agg = 0, woff = 0;
+    // wrank[k]: tile-local rank of the first free page of this thread's
+    // word k (the rows before, then the waves and lanes before in the row);
+    // agg: the tile's free pages
+    uint32_t wrank[WPT];
+    uint32_t agg = 0;
 #pragma unroll
-    for (unsigned i = 0; i < kWaves; ++i) {
-        if (i < wv) woff += s_wsum[i];
-        agg += s_wsum[i];
+    for (int k = 0; k < WPT; ++k) {
+        uint32_t woff = 0, row = 0;
+#pragma unroll
+        for (unsigned i = 0; i < kWaves; ++i) {
+            if (i < wv) woff += s_wsum[k][i];
+            row += s_wsum[k][i];
+        }
+        wrank[k] = agg + woff + pre[k];
+        agg += row;
     }
 
     if (wv == 0) {
@@ -351,52 +370,49 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
     // claim the bits of rank < G in this thread's words; the results are
     // checked once the ids are out
     const uint32_t grant = s_grant;
-    const uint32_t base = s_excl + woff + pre;  // rank of this thread's first free page
+    const uint32_t base = s_excl + wrank[0];  // rank of this thread's first free page
     u64 mask[WPT], old[WPT];
-    uint32_t rk = base;
+    uint32_t mine = 0;  // pages granted to this thread
 #pragma unroll
     for (int k = 0; k < WPT; ++k) {
-        const uint32_t pc = (uint32_t)__popcll(fm[k]);
+        const uint32_t rk = s_excl + wrank[k];
+        const uint32_t pc = cntk[k];
         uint32_t take = 0;
         if (rk < grant) take = grant - rk < pc ? grant - rk : pc;
         mask[k] = lowest_bits(fm[k], take);
+        mine += take;
         old[k] = 0;
         if (mask[k]) {
+            const uint32_t wi = w0 + (uint32_t)k * THREADS;
             if constexpr (EXCL) {
-                c.bitmap[w0 + k] = ~(fm[k] & ~mask[k]);
+                c.bitmap[wi] = ~(fm[k] & ~mask[k]);
             } else {
-                old[k] = atom_or(&c.bitmap[w0 + k], mask[k]);
+                old[k] = atom_or(&c.bitmap[wi], mask[k]);
                 st.atomics++;
             }
         }
-        rk += pc;
     }
     // ids: the granted bits in rank order
-    uint32_t r = base;
     if constexpr (WPT >= 4) {
         // large tiles (thousands of grants): the tile's ids go through LDS
         // a stage at a time and out in consecutive 4-byte stores, so every
         // wave writes whole lines instead of one scattered id per lane
         __shared__ int32_t s_ids[kRankStage];
-        uint32_t mine = 0;
-#pragma unroll
-        for (int k = 0; k < WPT; ++k) mine += (uint32_t)__popcll(mask[k]);
         const uint32_t t_lo = s_excl;
         const uint32_t t_hi = grant > s_excl ? (grant - s_excl < agg ? grant : s_excl + agg) : s_excl;
         for (uint32_t lo = t_lo; lo < t_hi; lo += kRankStage) {
             const uint32_t hi = t_hi - lo < kRankStage ? t_hi : lo + kRankStage;
-            if (base < hi && base + mine > lo) {
-                uint32_t q = base;
 #pragma unroll
-                for (int k = 0; k < WPT; ++k) {
-                    u64 m = mask[k];
-                    const uint32_t wb = (w0 + k) * 64u;
-                    while (m && q < hi) {
-                        const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
-                        m &= m - 1ull;
-                        if (q >= lo) s_ids[q - lo] = (int32_t)(wb + bit);
-                        ++q;
-                    }
+            for (int k = 0; k < WPT; ++k) {
+                uint32_t q = s_excl + wrank[k];
+                u64 m = mask[k];
+                if (!m || q >= hi || q + (uint32_t)__popcll(m) <= lo) continue;
+                const uint32_t wb = (w0 + (uint32_t)k * THREADS) * 64u;
+                while (m && q < hi) {
+                    const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
+                    m &= m - 1ull;
+                    if (q >= lo) s_ids[q - lo] = (int32_t)(wb + bit);
+                    ++q;
                 }
             }
             __syncthreads();
@@ -406,7 +422,6 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             }
             __syncthreads();
         }
-        r = base + mine;
     } else if (WPT == 1 && ballot(mask[0] == kFull) == kFull) {
         // every lane of the wave was granted its whole word (an empty map):
         // the wave's 4,096 ranks are its 64 words' pages in order, so it
@@ -418,13 +433,13 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             ids[r0 + j * 64u + lane] = (int32_t)(p0 + j * 64u + lane);
             if (steps_out) steps_out[r0 + j * 64u + lane] = 1u;
         }
-        r = base + 64u;
     } else {
         // each thread its own
 #pragma unroll
         for (int k = 0; k < WPT; ++k) {
             u64 m = mask[k];
-            const uint32_t wb = (w0 + k) * 64u;
+            uint32_t r = s_excl + wrank[k];
+            const uint32_t wb = (w0 + (uint32_t)k * THREADS) * 64u;
             while (m) {
                 const unsigned bit = (unsigned)__ffsll((long long)m) - 1u;
                 m &= m - 1ull;
@@ -434,18 +449,18 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             }
         }
     }
-    st.allocs += r - base;
-    st.steps += r - base;
+    st.allocs += mine;
+    st.steps += mine;
     // a concurrent writer took granted bits: their requests walk and
     // overwrite what was written above
-    uint32_t rb = base;
 #pragma unroll
     for (int k = 0; k < WPT; ++k) {
         u64 lost = mask[k] & old[k];
         while (lost) {
             const unsigned bit = (unsigned)__ffsll((long long)lost) - 1u;
             lost &= lost - 1ull;
-            const uint32_t q = rb + (uint32_t)__popcll(fm[k] & ((1ull << bit) - 1ull));
+            const uint32_t q =
+                s_excl + wrank[k] + (uint32_t)__popcll(fm[k] & ((1ull << bit) - 1ull));
             uint32_t steps = 1;
             st.fallbacks++;
             st.steps--;
@@ -458,7 +473,6 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
                 st.failures++;
             }
         }
-        rb += (uint32_t)__popcll(fm[k]);
     }
 
     // requests reserved beyond the free pages the scan saw (a concurrent
