@@ -417,8 +417,11 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
             }
             __syncthreads();
             for (uint32_t i = threadIdx.x; i < hi - lo; i += THREADS) {
-                ids[lo + i] = s_ids[i];
-                if (steps_out) steps_out[lo + i] = 1u;
+                // streamed past the L2: a large call's ids would otherwise
+                // sit dirty in the L2s until the end of the kernel writes
+                // them back, after the last wave
+                __builtin_nontemporal_store(s_ids[i], &ids[lo + i]);
+                if (steps_out) __builtin_nontemporal_store(1u, &steps_out[lo + i]);
             }
             __syncthreads();
         }
