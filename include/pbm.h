@@ -249,7 +249,10 @@ int pbm_get_pages(pbm_handle* h, uint32_t n, uint64_t req_base,
                   int32_t* d_ids, uint32_t* d_steps,
                   const pbm_launch* launch, void* stream);
 
-/* Returns the n pages in d_ids (entries < 0 are skipped).
+/* Returns the n pages in d_ids (entries < 0 are skipped).  In batched
+ * mode the workgroups' reservation credits meet in per-pool partial
+ * counters, so two batched frees of one pool must not run at once (as for
+ * every batched call).
  * Replaces freePage_kernel (unitTests/unitTest1.5.cu:38-44) calling
  * freePage (RandomWalkBasic_BM.cuh:98-102, parallelPage.cu:399,511,594). */
 int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,

@@ -733,8 +733,19 @@ __global__ __launch_bounds__(1024) void k_free_plain(pbm_ctx c, uint32_t n,
 // one atomic.
 constexpr uint32_t kFreeUnroll = 8;
 
+// Reservation credit of the batched free: workgroup w adds its freed pages
+// to partial counter w % kCreditSlots, packed with an arrival count
+// (arrivals << 40 | pages); the last workgroup to arrive at a counter moves
+// its sum to the reservation counter and clears it for the next call.  At
+// most a sixteenth of the workgroups meet at one address, where one
+// reservation update per workgroup would queue them all at one (about
+// 12 ns each: 6 us for the 512 workgroups of a 4,194,304-id call).
+constexpr uint32_t kCreditSlots = 16;
+constexpr uint32_t kCreditStride = 32;  // words between partial counters (256 B)
+
 __global__ __launch_bounds__(1024) void k_free_batched(pbm_ctx c, uint32_t n,
-                                                       const int32_t* __restrict__ ids) {
+                                                       const int32_t* __restrict__ ids,
+                                                       u64* __restrict__ credit) {
     using namespace pbm;
     const unsigned lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -795,7 +806,19 @@ __global__ __launch_bounds__(1024) void k_free_batched(pbm_ctx c, uint32_t n,
     if (lane == 0 && fr) atomicAdd(&s_freed, fr);
     flush_stats(c, st);
     __syncthreads();
-    if (threadIdx.x == 0 && s_freed) atom_add_ll(c.free_count, (long long)s_freed);
+    if (threadIdx.x == 0) {
+        const uint32_t slot = blockIdx.x % kCreditSlots;
+        const u64 arrivals = (gridDim.x - slot + kCreditSlots - 1u) / kCreditSlots;
+        u64* w = credit + (size_t)slot * kCreditStride;
+        const u64 old = atom_add_u64(w, (1ull << 40) | (u64)s_freed);
+        if ((old >> 40) + 1u == arrivals) {
+            // every workgroup of this counter has added: none adds again
+            // in this call, and the next call starts after this kernel
+            __hip_atomic_store(w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const u64 total = (old & ((1ull << 40) - 1ull)) + s_freed;
+            if (total) atom_add_ll(c.free_count, (long long)total);
+        }
+    }
 }
 
 // ---------------------------------------------------------- launches
@@ -1019,7 +1042,7 @@ extern "C" int pbm_free_pages(pbm_handle* h, uint32_t n, const int32_t* d_ids,
     }
     if (L.mode == PBM_MODE_BATCHED) {
         const uint32_t nb = (batched_blocks(h, L, n) + 3u) / 4u;
-        pbmi::launch_timed(h, k_free_batched, dim3(nb), dim3(1024), s, c, n, d_ids);
+        pbmi::launch_timed(h, k_free_batched, dim3(nb), dim3(1024), s, c, n, d_ids, h->d_credit);
     } else if (L.mode == PBM_MODE_SERIAL) {
         pbmi::launch_timed(h, k_free_plain, dim3(1), dim3(1), s, c, n, d_ids);
     } else {

@@ -217,6 +217,7 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     const size_t scan_words = (size_t)n_tiles + 32u * ((n_tiles + 63u) / 64u);
     PBM_ALLOC(h->d_scan, scan_words * 8u);
     PBM_ALLOC(h->d_scan_ctl, 32u);
+    PBM_ALLOC(h->d_credit, 16u * 32u * 8u);
     if (cfg->backing) PBM_ALLOC(h->d_pool, (size_t)cfg->n_pages * cfg->page_size);
     if (cfg->strategy == PBM_CRW_BM || cfg->strategy == PBM_CRW_BASIC)
         PBM_ALLOC(h->d_hint, (size_t)h->cfg.max_threads * 4u);
@@ -270,6 +271,7 @@ extern "C" int pbm_create(const pbm_config* cfg, pbm_handle** out) {
     // Look-back words start with call parity 0: the first call has parity 1
     // and no zeroed word reads as published by it.
     e = hipMemset(h->d_scan, 0, scan_words * 8u);
+    if (e == hipSuccess) e = hipMemset(h->d_credit, 0, 16u * 32u * 8u);
     if (e != hipSuccess) return cleanup(pbmi::hip_fail(e, "hipMemset", __FILE__, __LINE__));
     {
         const unsigned long long ctl[4] = {0, 0, 0, 0};
@@ -290,7 +292,7 @@ extern "C" int pbm_destroy(pbm_handle* h) {
     (void)hipDeviceSynchronize();
     void* ptrs[] = {h->d_bitmap, h->d_free, h->d_counters, h->d_pool, h->d_hint, h->d_arms,
                     h->d_clock, h->d_ring, h->d_ring_ht, h->d_phase, h->d_scratch,
-                    h->d_scan, h->d_scan_ctl, h->d_fresh, h->d_carve, h->d_qavail,
+                    h->d_scan, h->d_scan_ctl, h->d_credit, h->d_fresh, h->d_carve, h->d_qavail,
                     h->d_shadow, h->d_shadow_rep};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);

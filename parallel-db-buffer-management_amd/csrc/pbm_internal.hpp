@@ -32,6 +32,7 @@ struct pbm_handle {
     unsigned long long* d_scratch = nullptr;  // 8 words for reductions
     unsigned long long* d_scan = nullptr;     // rank scan: one look-back word per tile
     unsigned long long* d_scan_ctl = nullptr; // rank scan: [1] look-back give-ups
+    unsigned long long* d_credit = nullptr;   // batched free: 16 partial reservation credits
     unsigned* d_fresh = nullptr;              // Ouroboros: free pages at the last reset
     unsigned long long* d_carve = nullptr;    // Ouroboros: [0] carved, [1] fresh pages
     long long* d_qavail = nullptr;            // Ouroboros: unpromised queued pages
