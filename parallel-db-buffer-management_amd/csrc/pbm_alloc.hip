@@ -740,8 +740,12 @@ constexpr uint32_t kFreeUnroll = 8;
 // most a sixteenth of the workgroups meet at one address, where one
 // reservation update per workgroup would queue them all at one (about
 // 12 ns each: 6 us for the 512 workgroups of a 4,194,304-id call).
+// Grids of fewer than kCreditMinGroups workgroups add to the reservation
+// counter directly: their short queue costs less than the partial
+// counter's extra round trip.
 constexpr uint32_t kCreditSlots = 16;
 constexpr uint32_t kCreditStride = 32;  // words between partial counters (256 B)
+constexpr uint32_t kCreditMinGroups = 128;
 
 __global__ __launch_bounds__(1024) void k_free_batched(pbm_ctx c, uint32_t n,
                                                        const int32_t* __restrict__ ids,
@@ -806,7 +810,9 @@ __global__ __launch_bounds__(1024) void k_free_batched(pbm_ctx c, uint32_t n,
     if (lane == 0 && fr) atomicAdd(&s_freed, fr);
     flush_stats(c, st);
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (gridDim.x < kCreditMinGroups) {
+        if (threadIdx.x == 0 && s_freed) atom_add_ll(c.free_count, (long long)s_freed);
+    } else if (threadIdx.x == 0) {
         const uint32_t slot = blockIdx.x % kCreditSlots;
         const u64 arrivals = (gridDim.x - slot + kCreditSlots - 1u) / kCreditSlots;
         u64* w = credit + (size_t)slot * kCreditStride;
