@@ -20,7 +20,7 @@ enum : uint32_t {
     kHead,        // next sequence to admit
     kTail,        // sequences submitted
     kActive,      // slots admitted at the last grow (finished ones included)
-    kReserved,    // pages owed to admitted sequences after the last admission
+    kPromised,    // pages promised at admission, summed over every admitted sequence
     kTop,         // free-slot stack height
     kDone,        // sequences that finished at the last grow
     kAdmitted,
@@ -65,16 +65,27 @@ __device__ __forceinline__ u64 ld_ctl(const u64* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Admission (one workgroup).  First the bookkeeping of the last step: its
-// grow granted one page to each of kActive slots, and kDone of them
-// finished and go back on the free-slot stack.  Then waiting sequences are
-// admitted in arrival order while their arrival step has come, a slot is
-// free and free pages - owed pages >= the sequence's target.
+// Admission (one workgroup).  First the bookkeeping of the last step: kDone
+// of its kActive slots finished and go back on the free-slot stack.  Then
+// waiting sequences are admitted in arrival order while their arrival step
+// has come, a slot is free and the pool's unreserved pages cover the
+// sequence's target.  Admission reserves every admitted sequence's whole
+// target in the pool's reservation counter at once, so the counter holds
+// free pages less the pages still owed to admitted sequences (the oracle's
+// free - owed), and the grow takes pages already reserved.
+//
+// Two round trips: the control block and the pool counter, then everything
+// the admissions read (the finished slots, the stack below them, the
+// waiting sequences' arrival steps, targets and submission stamps).  The
+// slots a step pops are the last finished ones first, then the stack from
+// its old top down, so both come in with the same loads.
 __device__ void serve_admit(const ServeDev& d) {
     __shared__ u64 s_ctl[kCtlWords];
     __shared__ long long s_free;
     __shared__ uint32_t s_arr[kAdmitWindow];
     __shared__ uint32_t s_tgt[kAdmitWindow];
+    __shared__ uint32_t s_slot[kAdmitWindow];  // the slot the i-th admission takes
+    __shared__ u64 s_vis[kAdmitWindow];
     __shared__ uint32_t s_m;
     __shared__ long long s_add;
     __shared__ u64 s_now;
@@ -90,18 +101,25 @@ __device__ void serve_admit(const ServeDev& d) {
     const u64 head = s_ctl[kHead];
     const u64 tail = s_ctl[kTail];
     const u64 done_n = s_ctl[kDone];
-    const u64 top = s_ctl[kTop] + done_n;
-    for (u64 i = t; i < done_n; i += blockDim.x) d.stack[s_ctl[kTop] + i] = d.done[2u * i];
+    const u64 top_old = s_ctl[kTop];
+    const u64 top = top_old + done_n;
     const u64 active = s_ctl[kActive] - done_n;
-    const long long owed = (long long)s_ctl[kReserved] - (long long)s_ctl[kActive];
     const uint32_t win = (uint32_t)(tail - head < kAdmitWindow ? tail - head : kAdmitWindow);
+    for (u64 i = t; i < done_n; i += blockDim.x) {
+        const uint32_t sl = d.done[2u * i];
+        d.stack[top_old + i] = sl;
+        const u64 j = done_n - 1u - i;  // popped j-th
+        if (j < win) s_slot[j] = sl;
+    }
     for (uint32_t i = t; i < win; i += blockDim.x) {
+        if (i >= done_n && i < top) s_slot[i] = d.stack[top_old - 1u - (i - done_n)];
         s_arr[i] = d.arr_step[head + i];
         s_tgt[i] = d.target[head + i];
+        s_vis[i] = d.seq_visible[head + i];
     }
     __syncthreads();
     if (t == 0) {
-        long long budget = s_free - owed;
+        long long budget = s_free;
         const u64 slots = (u64)d.max_slots - active;
         uint32_t m = 0;
         long long add = 0;
@@ -110,13 +128,14 @@ __device__ void serve_admit(const ServeDev& d) {
             add += s_tgt[m];
             ++m;
         }
+        if (add) pbm::atom_add_ll(d.c.free_count, -add);
         s_m = m;
         s_add = add;
     }
     __syncthreads();
     const uint32_t m = s_m;
     for (uint32_t i = t; i < m; i += blockDim.x) {
-        const uint32_t slot = d.stack[top - 1u - i];
+        const uint32_t slot = s_slot[i];
         const uint32_t seq = (uint32_t)(head + i);
         d.slot_seq[slot] = seq;
         d.slot_len[slot] = 0;
@@ -132,7 +151,7 @@ __device__ void serve_admit(const ServeDev& d) {
         if (as == step) a = now;
         else if (as > 0u && step - as < kStepRing) a = d.step_tick[as % kStepRing];
         else if (as > 0u) known = false;
-        const u64 vis = d.seq_visible[seq];
+        const u64 vis = s_vis[i];
         const u64 arrive = a > vis ? a : vis;
         d.res_wait[seq] = !known ? kWaitUnknown
                           : (now - arrive > 0xFFFFFFF0ull ? 0xFFFFFFF0u : (uint32_t)(now - arrive));
@@ -142,7 +161,7 @@ __device__ void serve_admit(const ServeDev& d) {
         d.ctl[kStep] = step;
         d.ctl[kHead] = head + m;
         d.ctl[kActive] = active + m;
-        d.ctl[kReserved] = (u64)(owed + s_add);
+        d.ctl[kPromised] = s_ctl[kPromised] + (u64)s_add;
         d.ctl[kTop] = top - m;
         d.ctl[kDone] = 0;
         d.ctl[kAdmitted] = s_ctl[kAdmitted] + m;
@@ -155,15 +174,136 @@ __global__ __launch_bounds__(1024) void k_serve_admit(ServeDev d) { serve_admit(
 // target is recorded as finished and its slot listed for release.  The
 // step's tallies are summed over the workgroup in LDS and added to the
 // control block with one atomic per word per workgroup.
+//
+// The pages were reserved when the sequences were admitted, so the
+// getPage calls reserve from a count in LDS that grants every request (it
+// starts far above any workgroup's requests) instead of the pool's
+// reservation counter; what a failed call gives back there stays reserved
+// for the sequence's next step.
+constexpr long long kGrowUnits = 1ll << 40;
+
+struct GrowLds {
+    uint32_t sum[12];  // [0] finished, [1] allocations, [2] failed, [3..10] pool counters
+                       // (Stats order), [11] payload errors
+    u64 base[2];       // first done-list and latency entries of the workgroup
+    long long units;   // the getPage calls' reservation count
+};
+
+// One slot's page for step `step`: written to its page list (and its
+// payload), the length stored; a slot that reaches its target is emptied
+// (seq becomes kEmpty) and its sequence's finish recorded.
+template <int S>
+__device__ __forceinline__ void grow_slot(const ServeDev& d, const pbm_ctx& cl, uint32_t slot,
+                                          u64 step, uint32_t& seq, uint32_t& len, uint32_t tgt,
+                                          pbm::Stats& st, bool& fin, bool& got, uint32_t& ticks) {
+    using namespace pbm;
+    const int32_t seed = request_seed(d.c.seed, (u64)seq * d.max_pages + len);
+    const u64 t0 = wall_clock64();
+    const int32_t id = get_page_of<S>(cl, slot, seed, nullptr, &st);
+    const u64 t1 = wall_clock64();
+    ticks = t1 - t0 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(t1 - t0);
+    if (id < 0) return;
+    got = true;
+    d.pages[(u64)slot * d.max_pages + len] = id;
+    if ((d.flags & PBM_SERVE_PAYLOAD) && d.c.pool)
+        *(u64*)page_address(d.c, id) = serve_payload(d.seed, d.key[seq], len);
+    ++len;
+    d.slot_len[slot] = len;
+    if (len == tgt) {
+        fin = true;
+        d.res_finish[seq] = (uint32_t)step;
+        d.res_len[seq] = len;
+        d.slot_seq[slot] = kEmpty;
+        seq = kEmpty;
+    }
+}
+
+// The workgroup's tallies of one grow (every thread calls it): finished
+// sequences go on the done list, latency samples on the latency list, and
+// the counts into the control block, one atomic per word.  st and bad may
+// also hold the frees of the same step (the loop releases its own
+// finished sequences).  g.sum must be zero and visible to the workgroup.
+__device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, u64 bad,
+                           bool act, bool got, bool fin, uint32_t ticks, uint32_t slot,
+                           uint32_t seq) {
+    using namespace pbm;
+    const unsigned lane = lane_id();
+    const u64 F = ballot(fin);
+    const u64 G = ballot(got);
+    const u64 X = ballot(act && !got);
+    // this wave's place among the workgroup's finished and granted lanes
+    uint32_t wf = 0, wg = 0;
+    if (lane == 0) {
+        if (F) wf = atomicAdd(&g.sum[0], (uint32_t)__popcll(F));
+        if (G) wg = atomicAdd(&g.sum[1], (uint32_t)__popcll(G));
+        if (X) atomicAdd(&g.sum[2], (uint32_t)__popcll(X));
+    }
+    wf = readlane(wf, 0);
+    wg = readlane(wg, 0);
+    uint32_t v[8] = {st.allocs, st.frees, st.failures, st.steps,
+                     st.probes, st.atomics, st.fallbacks, st.errors};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        uint32_t x = v[i];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+        if (lane == 0 && x) atomicAdd(&g.sum[3 + i], x);
+    }
+    {
+        uint32_t x = (uint32_t)bad;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+        if (lane == 0 && x) atomicAdd(&g.sum[11], x);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // one lane per control word: finished (done-list base), finished
+        // total, allocations, latency base, failures, frees, payload
+        // errors, then the pool counters
+        const uint32_t nf = g.sum[0], ng = g.sum[1], nx = g.sum[2];
+        const bool lat = (d.flags & PBM_SERVE_LATENCY) != 0;
+        u64* dst = nullptr;
+        u64 add = 0;
+        switch (lane) {
+            case 0: dst = &d.ctl[kDone]; add = nf; break;
+            case 1: dst = &d.ctl[kFinished]; add = nf; break;
+            case 2: dst = &d.ctl[kAllocs]; add = ng; break;
+            case 3: dst = lat ? &d.ctl[kLatCount] : nullptr; add = ng; break;
+            case 4: dst = &d.ctl[kAllocFail]; add = nx; break;
+            case 5: dst = &d.ctl[kFrees]; add = g.sum[4]; break;
+            case 6: dst = &d.ctl[kPayloadErr]; add = g.sum[11]; break;
+            default:
+                if (lane >= 8 && lane < 16 && d.c.counters) {
+                    dst = (u64*)d.c.counters + (lane - 8u);
+                    add = g.sum[3 + (lane - 8u)];
+                }
+        }
+        u64 old = 0;
+        if (dst && add) old = atom_add_u64(dst, add);
+        if (lane == 0) g.base[0] = old;
+        if (lane == 3) g.base[1] = old;
+    }
+    __syncthreads();
+    if (fin) {
+        const u64 k = g.base[0] + wf + rank_in(F);
+        d.done[2u * k] = slot;
+        d.done[2u * k + 1u] = seq;
+    }
+    if ((d.flags & PBM_SERVE_LATENCY) && got) {
+        const u64 k = g.base[1] + wg + rank_in(G);
+        if (k < d.lat_cap) d.lat[k] = ticks;
+    }
+}
+
 template <int S>
 __device__ void serve_grow(const ServeDev& d, uint32_t slot) {
     using namespace pbm;
-    // [0] finished, [1] allocations, [2] allocations still owed, [3..10]
-    // the pool counters (Stats order)
-    __shared__ uint32_t s_sum[11];
-    __shared__ u64 s_base[2];  // first done-list and latency entries of the workgroup
-    if (threadIdx.x < 11) s_sum[threadIdx.x] = 0;
+    __shared__ GrowLds g;
+    if (threadIdx.x < 12) g.sum[threadIdx.x] = 0;
+    if (threadIdx.x == 0) g.units = kGrowUnits;
     __syncthreads();
+    pbm_ctx cl = d.c;
+    cl.free_count = &g.units;
     const u64 step = ld_ctl(&d.ctl[kStep]);
     Stats st;
     // the slot's sequence, length and target are loaded together (one
@@ -175,87 +315,11 @@ __device__ void serve_grow(const ServeDev& d, uint32_t slot) {
         tgt = d.slot_target[slot];
     }
     const bool act = seq != kEmpty;
+    const uint32_t fseq = seq;
     bool fin = false, got = false;
     uint32_t ticks = 0;
-    if (act) {
-        const int32_t seed = request_seed(d.c.seed, (u64)seq * d.max_pages + len);
-        const u64 t0 = wall_clock64();
-        const int32_t id = get_page_of<S>(d.c, slot, seed, nullptr, &st);
-        const u64 t1 = wall_clock64();
-        ticks = t1 - t0 > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)(t1 - t0);
-        if (id >= 0) {
-            got = true;
-            d.pages[(u64)slot * d.max_pages + len] = id;
-            if ((d.flags & PBM_SERVE_PAYLOAD) && d.c.pool)
-                *(u64*)page_address(d.c, id) = serve_payload(d.seed, d.key[seq], len);
-            d.slot_len[slot] = len + 1u;
-            if (len + 1u == tgt) {
-                fin = true;
-                d.res_finish[seq] = (uint32_t)step;
-                d.res_len[seq] = len + 1u;
-                d.slot_seq[slot] = kEmpty;
-            }
-        }
-    }
-    const unsigned lane = lane_id();
-    const u64 F = ballot(fin);
-    const u64 G = ballot(got);
-    const u64 X = ballot(act && !got);
-    // this wave's place among the workgroup's finished and granted lanes
-    uint32_t wf = 0, wg = 0;
-    if (lane == 0) {
-        if (F) wf = atomicAdd(&s_sum[0], (uint32_t)__popcll(F));
-        if (G) wg = atomicAdd(&s_sum[1], (uint32_t)__popcll(G));
-        if (X) atomicAdd(&s_sum[2], (uint32_t)__popcll(X));
-    }
-    wf = readlane(wf, 0);
-    wg = readlane(wg, 0);
-    uint32_t v[8] = {st.allocs, st.frees, st.failures, st.steps,
-                     st.probes, st.atomics, st.fallbacks, st.errors};
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        uint32_t x = v[i];
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
-        if (lane == 0 && x) atomicAdd(&s_sum[3 + i], x);
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        // one lane per control word: finished (done-list base), finished
-        // total, allocations, latency base, failures, owed pages, then the
-        // pool counters
-        const uint32_t nf = s_sum[0], ng = s_sum[1], nx = s_sum[2];
-        const bool lat = (d.flags & PBM_SERVE_LATENCY) != 0;
-        u64* dst = nullptr;
-        u64 add = 0;
-        switch (lane) {
-            case 0: dst = &d.ctl[kDone]; add = nf; break;
-            case 1: dst = &d.ctl[kFinished]; add = nf; break;
-            case 2: dst = &d.ctl[kAllocs]; add = ng; break;
-            case 3: dst = lat ? &d.ctl[kLatCount] : nullptr; add = ng; break;
-            case 4: dst = &d.ctl[kAllocFail]; add = nx; break;
-            case 5: dst = &d.ctl[kReserved]; add = nx; break;
-            default:
-                if (lane >= 8 && lane < 16 && d.c.counters) {
-                    dst = (u64*)d.c.counters + (lane - 8u);
-                    add = s_sum[3 + (lane - 8u)];
-                }
-        }
-        u64 old = 0;
-        if (dst && add) old = atom_add_u64(dst, add);
-        if (lane == 0) s_base[0] = old;
-        if (lane == 3) s_base[1] = old;
-    }
-    __syncthreads();
-    if (fin) {
-        const u64 k = s_base[0] + wf + rank_in(F);
-        d.done[2u * k] = slot;
-        d.done[2u * k + 1u] = seq;
-    }
-    if ((d.flags & PBM_SERVE_LATENCY) && got) {
-        const u64 k = s_base[1] + wg + rank_in(G);
-        if (k < d.lat_cap) d.lat[k] = ticks;
-    }
+    if (act) grow_slot<S>(d, cl, slot, step, seq, len, tgt, st, fin, got, ticks);
+    grow_tally(d, g, st, 0ull, act, got, fin, ticks, slot, fseq);
 }
 
 template <int S>
@@ -263,29 +327,22 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
     serve_grow<S>(d, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Release: the page lists of the sequences that finished at this step's
-// grow, cut into 64-page chunks dealt round-robin to the waves.  A chunk's
-// payloads are read back, checked and summed into the sequence's checksum,
-// then its pages go back to the pool.
-template <int S>
-__device__ void serve_release(const ServeDev& d, uint32_t wave, uint32_t n_waves) {
+// Frees the page lists of n finished sequences, cut into 64-page chunks
+// dealt round-robin to waves wave, wave + n_waves, ...; entry(e, slot, seq,
+// len) gives the e-th sequence (a wave's lanes ask for 64 at once).  A
+// chunk's payloads are read back, checked and summed into the sequence's
+// checksum, then its pages go back to the pool through cl, whose
+// reservation counter takes the credit.
+template <int S, typename Entry>
+__device__ void free_finished(const ServeDev& d, const pbm_ctx& cl, uint32_t n, Entry entry,
+                              uint32_t wave, uint32_t n_waves, pbm::Stats& st, u64& bad) {
     using namespace pbm;
-    __shared__ uint32_t s_sum[9];
-    if (threadIdx.x < 9) s_sum[threadIdx.x] = 0;
-    __syncthreads();
     const unsigned lane = lane_id();
-    const uint32_t done_n = (uint32_t)ld_ctl(&d.ctl[kDone]);
     const bool payload = (d.flags & PBM_SERVE_PAYLOAD) && d.c.pool;
-    Stats st;
-    u64 bad = 0;
-    for (uint32_t g0 = 0; g0 < done_n; g0 += 64u) {
+    for (uint32_t g0 = 0; g0 < n; g0 += 64u) {
         const uint32_t e = g0 + lane;
         uint32_t slot = 0, seq = 0, len = 0;
-        if (e < done_n) {
-            slot = d.done[2u * e];
-            seq = d.done[2u * e + 1u];
-            len = d.slot_len[slot];
-        }
+        if (e < n) entry(e, slot, seq, len);
         const uint32_t chunks = (len + 63u) / 64u;
         uint32_t incl = chunks;
 #pragma unroll
@@ -317,10 +374,45 @@ __device__ void serve_release(const ServeDev& d, uint32_t wave, uint32_t n_waves
 #pragma unroll
                 for (int o = 32; o >= 1; o >>= 1) sum += shfl64(sum, lane ^ (unsigned)o);
                 if (lane == 0) atom_add_u64(&d.res_sum[c_seq], sum);
+                // the read-back completes before the pages go back: a page
+                // freed with its load still in flight could be taken and
+                // written by a concurrent grow (the loop's workgroups free
+                // while others grow) before the load is served
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             }
-            if (valid) free_page_of<S>(d.c, id, &st);
+            if (valid) free_page_of<S>(cl, id, &st);
         }
     }
+}
+
+// Release: the page lists of the sequences that finished at this step's
+// grow (the done list), dealt over the waves of the grid.
+//
+// The freed pages are credited to the pool's reservation counter once per
+// workgroup, at the end: the frees count them into LDS instead of one
+// atomic per wave at the pool counter's one address.
+template <int S>
+__device__ void serve_release(const ServeDev& d, uint32_t wave, uint32_t n_waves) {
+    using namespace pbm;
+    __shared__ uint32_t s_sum[9];
+    __shared__ long long s_credit;
+    if (threadIdx.x < 9) s_sum[threadIdx.x] = 0;
+    if (threadIdx.x == 0) s_credit = 0;
+    __syncthreads();
+    pbm_ctx cl = d.c;
+    cl.free_count = &s_credit;
+    const unsigned lane = lane_id();
+    const uint32_t done_n = (uint32_t)ld_ctl(&d.ctl[kDone]);
+    Stats st;
+    u64 bad = 0;
+    free_finished<S>(
+        d, cl, done_n,
+        [&](uint32_t e, uint32_t& slot, uint32_t& seq, uint32_t& len) {
+            slot = d.done[2u * e];
+            seq = d.done[2u * e + 1u];
+            len = d.slot_len[slot];
+        },
+        wave, n_waves, st, bad);
     // tallies summed over the workgroup, then one atomic per word: [0]
     // payload errors, [1..8] the pool counters (Stats order; frees is also
     // the control block's count)
@@ -348,6 +440,7 @@ __device__ void serve_release(const ServeDev& d, uint32_t wave, uint32_t n_waves
             add = s_sum[1 + (lane - 8u)];
         }
         if (dst && add) atom_add_u64(dst, add);
+        if (lane == 2 && s_credit) atom_add_ll(d.c.free_count, s_credit);
     }
 }
 
@@ -359,21 +452,31 @@ __global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
 
 // ---- the device-resident loop: n_steps steps in one kernel
 //
-// One 1,024-thread workgroup per 1,024 slots.  Each step: workgroup 0
-// admits; every workgroup grows its slots; every workgroup frees its share
-// of the finished sequences' pages (sixteen waves each, dealt as the
-// release kernel deals them).  Fewer, larger workgroups mean fewer
-// arrivals on the phase counters and fewer cache write-backs per phase.  The phases follow one another through three
-// counters in `sync`: [0] steps admitted, [1] grow shares done, [2]
-// release shares done; [3] counts waits that gave up.  A workgroup makes
-// its writes visible before it counts itself done (every wave waits for its
-// stores, then the barrier, then one device-scope release and the count),
-// and acquires after it sees what it waited for (one acquire, then the
-// barrier).  Every workgroup waits for every other,
-// so all of them must be resident at once: the launch checks that the
-// device holds them, and every wait is bounded (kLoopWaitTicks) and
-// counted when it gives up, so a device shared with other work cannot hang
-// the loop, though its results are then not valid.
+// One 1,024-thread workgroup per 1,024 slots, and one more that admits.
+// Each step the admitting workgroup admits, and every slot workgroup grows
+// its slots and frees the pages of its own slots' finished sequences
+// (their 64-page chunks dealt over its sixteen waves).
+//
+// A slot workgroup keeps its slots' state in registers from step to step.
+// It grows the slots admitted before this step, and frees the sequences
+// that this finishes, while admission runs: admission writes only empty
+// slots, and the pages were reserved at admission.  The credit of those
+// frees waits in LDS until admission is visible, so admission counts
+// exactly the pages the last step's releases gave back.  Then the
+// workgroup grows the slots admitted now, frees what that finishes, counts
+// the step and credits the pool.  No slot workgroup waits for another's
+// grow; admission waits for every workgroup's step.
+//
+// The phases follow one another through counters in `sync`: [0] steps
+// admitted, [2] workgroup steps done; [3] counts waits that gave up.  A
+// workgroup makes its writes visible before it counts itself done (every
+// wave waits for its stores, then the barrier, then one device-scope
+// release and the count), and acquires after it sees what it waited for
+// (one acquire, then the barrier).  Admission waits for every slot
+// workgroup, so all of them must be resident at once: the launch checks
+// that the device holds them, and every wait is bounded (kLoopWaitTicks)
+// and counted when it gives up, so a device shared with other work cannot
+// hang the loop, though its results are then not valid.
 constexpr u64 kLoopWaitTicks = 200000000ull;  // 2 s of the 100 MHz wall clock
 
 // Thread 0 waits for the word to reach `target` and acquires for the
@@ -414,29 +517,129 @@ __device__ inline void loop_signal(u64* sync, uint32_t word, bool store, u64 val
 
 constexpr uint32_t kLoopThreads = 1024;  // slots per workgroup of the loop
 
+// A slot workgroup's list of its sequences that finished this step, for
+// its own release.
+struct LoopLds {
+    uint32_t n;
+    uint32_t slot[kLoopThreads], seq[kLoopThreads], len[kLoopThreads];
+    long long credit;  // pages freed this step, not yet credited to the pool
+};
+
+// Lists the lanes' finished sequences (every thread calls it).
+__device__ __forceinline__ void loop_list(LoopLds& L, bool fin, uint32_t slot, uint32_t seq,
+                                          uint32_t len) {
+    const u64 F = pbm::ballot(fin);
+    uint32_t b = 0;
+    if (pbm::lane_id() == 0 && F) b = atomicAdd(&L.n, (uint32_t)__popcll(F));
+    b = pbm::readlane(b, 0);
+    if (fin) {
+        const uint32_t k = b + pbm::rank_in(F);
+        L.slot[k] = slot;
+        L.seq[k] = seq;
+        L.len[k] = len;
+    }
+}
+
 template <int S>
 __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_t n_steps,
                                                              u64* sync) {
-    const u64 P = gridDim.x;
-    const uint32_t wave0 = blockIdx.x * (kLoopThreads / 64u) + (threadIdx.x >> 6);
-    for (uint32_t k = 0; k < n_steps; ++k) {
-        if (blockIdx.x == 0) {
+    using namespace pbm;
+    constexpr int R = (S == PBM_LINKED_LIST || S == PBM_OUROBOROS) ? S : PBM_RW_BM;
+    constexpr uint32_t kWaves = kLoopThreads / 64u;
+    const u64 P = gridDim.x - 1u;  // slot workgroups; the last workgroup admits
+    if (blockIdx.x == P) {
+        for (uint32_t k = 0; k < n_steps; ++k) {
             // the last step's pages are all back before admission counts them
             if (k > 0) loop_wait_ge(sync, 2, (u64)k * P);
             serve_admit(d);
             loop_signal(sync, 0, true, (u64)k + 1u);
         }
+        return;
+    }
+    __shared__ GrowLds g;
+    __shared__ LoopLds L;
+    if (threadIdx.x == 0) {
+        g.units = kGrowUnits;
+        L.n = 0;
+        L.credit = 0;
+    }
+    pbm_ctx cl = d.c;  // the grow's: pages reserved at admission
+    cl.free_count = &g.units;
+    pbm_ctx cr = d.c;  // the release's: credit held in LDS
+    cr.free_count = &L.credit;
+    const uint32_t slot = blockIdx.x * kLoopThreads + threadIdx.x;
+    const uint32_t wave = threadIdx.x >> 6;
+    // the slot as this thread left it at its last grow (kEmpty: load it
+    // once admission is visible)
+    uint32_t seq = kEmpty, len = 0, tgt = 0;
+    u64 step = 0;
+    for (uint32_t k = 0; k < n_steps; ++k) {
+        if (threadIdx.x < 12) g.sum[threadIdx.x] = 0;
+        Stats st;
+        u64 bad = 0;
+        bool act = false, fin = false, got = false;
+        uint32_t ticks = 0, fseq = kEmpty;
+        // slots admitted before this step, while admission runs
+        const bool held = seq != kEmpty;  // never at k = 0
+        if (held) {
+            act = true;
+            fseq = seq;
+            grow_slot<S>(d, cl, slot, step + 1u, seq, len, tgt, st, fin, got, ticks);
+        }
+        loop_list(L, fin, slot, fseq, len);
+        __syncthreads();
+        const uint32_t n_held = L.n;
+        free_finished<R>(
+            d, cr, n_held,
+            [](uint32_t e, uint32_t& sl, uint32_t& sq, uint32_t& ln) {
+                sl = L.slot[e];
+                sq = L.seq[e];
+                ln = L.len[e];
+            },
+            wave, kWaves, st, bad);
+        // slots admitted now
         loop_wait_ge(sync, 0, (u64)k + 1u);
-        serve_grow<S>(d, blockIdx.x * kLoopThreads + threadIdx.x);
-        loop_signal(sync, 1, false, 1ull);
-        loop_wait_ge(sync, 1, (u64)(k + 1u) * P);
-        serve_release<(S == PBM_LINKED_LIST || S == PBM_OUROBOROS) ? S : PBM_RW_BM>(
-            d, wave0, (uint32_t)P * (kLoopThreads / 64u));
+        uint32_t nseq = kEmpty, nlen = 0, ntgt = 0;
+        u64 cur = step + 1u;
+        if (k == 0) cur = ld_ctl(&d.ctl[kStep]);
+        if (!held && slot < d.max_slots) {
+            nseq = d.slot_seq[slot];
+            nlen = d.slot_len[slot];
+            ntgt = d.slot_target[slot];
+        }
+        step = cur;
+        bool fin_new = false;
+        if (nseq != kEmpty) {
+            seq = nseq;
+            len = nlen;
+            tgt = ntgt;
+            act = true;
+            fseq = seq;
+            grow_slot<S>(d, cl, slot, step, seq, len, tgt, st, fin_new, got, ticks);
+            fin = fin_new;
+        }
+        loop_list(L, fin_new, slot, fseq, len);
+        __syncthreads();
+        free_finished<R>(
+            d, cr, L.n - n_held,
+            [n_held](uint32_t e, uint32_t& sl, uint32_t& sq, uint32_t& ln) {
+                sl = L.slot[n_held + e];
+                sq = L.seq[n_held + e];
+                ln = L.len[n_held + e];
+            },
+            wave, kWaves, st, bad);
+        grow_tally(d, g, st, bad, act, got, fin, ticks, slot, fseq);
+        // the step's freed pages back to the pool, after admission read it
+        if (threadIdx.x == 0) {
+            if (L.credit) atom_add_ll(d.c.free_count, L.credit);
+            L.credit = 0;
+            L.n = 0;
+        }
         loop_signal(sync, 2, false, 1ull);
     }
 }
 
-// Before a loop launch: its three phase counters start at zero.
+// Before a loop launch: its phase counters start at zero.
 __global__ void k_serve_loop_prep(u64* sync) {
     if (threadIdx.x < 3) sync[threadIdx.x] = 0;
 }
@@ -697,7 +900,8 @@ namespace {
 
 template <int S>
 int launch_loop_s(pbm_server* s, uint32_t n_steps, hipStream_t st) {
-    const uint32_t groups = (s->d.max_slots + kLoopThreads - 1u) / kLoopThreads;
+    // the slot workgroups and the admitting one
+    const uint32_t groups = (s->d.max_slots + kLoopThreads - 1u) / kLoopThreads + 1u;
     int per_cu = 0;
     PBM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_serve_loop<S>, kLoopThreads,
                                                          0));
@@ -844,7 +1048,7 @@ extern "C" int pbm_serve_read_stats(pbm_server* s, pbm_serve_stats* out, void* s
     out->admitted = ctl[kAdmitted];
     out->finished = ctl[kFinished];
     out->active = ctl[kActive] - ctl[kDone];
-    out->reserved_pages = ctl[kReserved] - ctl[kActive];
+    out->reserved_pages = ctl[kPromised] - ctl[kAllocs];
     out->allocs = ctl[kAllocs];
     out->frees = ctl[kFrees];
     out->alloc_failures = ctl[kAllocFail];

@@ -142,7 +142,7 @@ def test_serve_loop_equals_eager_steps(gpu, slots):
                 res = srv.results()
                 st = srv.stats()
                 assert srv.loop_timeouts() == 0
-                assert st["payload_errors"] == 0 and st["alloc_failures"] == 0
+                assert (st["payload_errors"], st["alloc_failures"]) == (0, 0), mode
                 out.append((res, st))
     (a, sa) = out[0]
     for b, sb in out[1:]:
