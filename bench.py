@@ -243,6 +243,10 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
                   "ops_per_s": ops_graph / (ms_graph * 1e-3)},
         "loop": {"steps_per_launch": block, "ms_per_step": ms_loop / timed_steps,
                  "ops_per_s": ops_loop / (ms_loop * 1e-3), "wait_timeouts": loop_timeouts},
+        "fastest": min(({"mode": "eager", "ms_per_step": ms / timed_steps},
+                        {"mode": "graph", "ms_per_step": ms_graph / timed_steps},
+                        {"mode": "loop", "ms_per_step": ms_loop / timed_steps}),
+                       key=lambda r: r["ms_per_step"]),
         "kernel_ms": kms,
         "active_at_end": s4["active"], "pages_in_use_at_end": pool.count_used(stream),
         "p50_alloc_us": float(np.percentile(lat, 50)) * 1e6 / hz if lat.size else None,
