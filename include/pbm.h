@@ -497,16 +497,21 @@ int pbm_serve_submit(pbm_server* s, uint32_t n, const uint32_t* arrival_step,
 int pbm_serve_read_waits(pbm_server* s, uint32_t first, uint32_t n, uint32_t* ticks,
                          void* stream);
 
-/* Runs n_steps steps (three kernels each) on the stream. */
+/* Runs n_steps steps (three kernels each) on the stream.  Admission
+ * reserves an admitted sequence's whole target in the pool's reservation
+ * counter, which then reads free pages less the pages owed to running
+ * sequences; the grows take pages already reserved. */
 int pbm_serve_run(pbm_server* s, uint32_t n_steps, void* stream);
 
 /* Runs n_steps steps in one kernel: the device-resident loop.  One
- * 1,024-thread workgroup per 1,024 slots; each step workgroup 0 admits, every
- * workgroup grows its slots, then every workgroup frees its share of the
- * finished sequences' pages, the phases ordered by device-side counters
- * (no kernel boundary between steps).  The same schedule and results as
- * pbm_serve_run.  Every workgroup waits for the others, so all must be
- * resident at once: PBM_ERR_UNSUPPORTED when the device cannot hold them,
+ * 1,024-thread workgroup per 1,024 slots and one that admits; each step
+ * the admitting workgroup admits while every slot workgroup grows the
+ * slots it already holds and frees the pages of the sequences that
+ * finishes, then, once the admission is visible, grows the slots admitted
+ * now and frees what that finishes; the phases are ordered by device-side
+ * counters (no kernel boundary between steps).  The same schedule and
+ * results as pbm_serve_run.  Admission waits for every slot workgroup, so
+ * all must be resident at once: PBM_ERR_UNSUPPORTED when the device cannot hold them,
  * and a wait that gives up (2 s, another kernel holding the device) is
  * counted by pbm_serve_loop_timeouts and leaves the results invalid.
  * Capturable in a HIP graph.  Sequences submitted before the call are seen
