@@ -474,6 +474,10 @@ typedef struct pbm_server pbm_server;
 
 /* Creates a server on pool h (any strategy).  The pool must outlive it. */
 int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_server** out);
+/* Waits for the device, then gives the pool back what the server holds:
+ * the pages of its running sequences and the pages promised to them at
+ * admission and not yet granted.  The pool's map and reservation counter
+ * are then as if the server had never run. */
 int pbm_serve_destroy(pbm_server* s);
 
 /* Appends n sequences (host arrays): arrival_step[i] (non-decreasing, also

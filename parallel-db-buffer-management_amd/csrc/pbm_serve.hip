@@ -664,6 +664,36 @@ __global__ void k_serve_set_tail(ServeDev d, u64 first, u64 tail) {
     if (blockIdx.x == 0 && threadIdx.x == 0) d.ctl[kTail] = tail;
 }
 
+// Gives the pool back what a server holds when it is destroyed: the pages
+// of every running sequence (its page list, through the strategy's
+// freePage) and the pages promised at admission and not yet granted.  One
+// wave per 64 slots; the sequences of a wave's slots are freed one after
+// another, 64 pages at a time.
+template <int S>
+__global__ __launch_bounds__(256) void k_serve_drop(ServeDev d) {
+    using namespace pbm;
+    Stats st;
+    const unsigned lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (u64 g0 = (u64)wave * 64u; g0 < d.max_slots; g0 += (u64)n_waves * 64u) {
+        const uint32_t slot = (uint32_t)g0 + lane;
+        uint32_t len = 0;
+        if (slot < d.max_slots && d.slot_seq[slot] != kEmpty) len = d.slot_len[slot];
+        for (u64 act = ballot(len > 0u); act; act &= act - 1ull) {
+            const unsigned q = (unsigned)__ffsll((long long)act) - 1u;
+            const uint32_t qs = readlane(slot, q), ql = readlane(len, q);
+            for (uint32_t j = lane; j < ql; j += 64u)
+                free_page_of<S>(d.c, d.pages[(u64)qs * d.max_pages + j], &st);
+        }
+    }
+    flush_stats(d.c, st);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const u64 owed = d.ctl[kPromised] - d.ctl[kAllocs];
+        if (owed) atom_add_ll(d.c.free_count, (long long)owed);
+    }
+}
+
 }  // namespace
 
 struct pbm_server {
@@ -680,12 +710,23 @@ struct pbm_server {
     // overwritten
     uint32_t* h_mirror = nullptr;  // [3 * max_seqs]
     u64* d_sync = nullptr;         // device-resident loop: phase counters, give-ups
+    bool ready = false;            // created and initialised
 };
 
 extern "C" int pbm_serve_destroy(pbm_server* s) {
     if (!s) return PBM_OK;
     if (s->h) (void)pbmi::use_device(s->h);
     (void)hipDeviceSynchronize();
+    if (s->ready) {
+        const dim3 g((s->d.max_slots + 255u) / 256u < 1024u ? (s->d.max_slots + 255u) / 256u : 1024u);
+        if (s->h->cfg.strategy == PBM_LINKED_LIST)
+            hipLaunchKernelGGL(k_serve_drop<PBM_LINKED_LIST>, g, dim3(256), 0, nullptr, s->d);
+        else if (s->h->cfg.strategy == PBM_OUROBOROS)
+            hipLaunchKernelGGL(k_serve_drop<PBM_OUROBOROS>, g, dim3(256), 0, nullptr, s->d);
+        else
+            hipLaunchKernelGGL(k_serve_drop<PBM_RW_BM>, g, dim3(256), 0, nullptr, s->d);
+        (void)hipDeviceSynchronize();
+    }
     for (void* p : s->mem)
         if (p) (void)hipFree(p);
     if (s->h_mirror) (void)hipHostFree(s->h_mirror);
@@ -779,6 +820,7 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
         return pbmi::hip_fail(e, "serve init", __FILE__, __LINE__);
     }
     s->release_blocks = (uint32_t)h->n_cu;  // four waves per CU
+    s->ready = true;
     *out = s;
     return PBM_OK;
 }

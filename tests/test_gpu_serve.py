@@ -299,6 +299,32 @@ def test_serve_c5_pool_metadata_only(gpu):
         assert pool.free_count() == A and pool.count_used() == 0
 
 
+@pytest.mark.parametrize("strategy", [pbm.RW_BM, pbm.COLLAB_RW_BM, pbm.LINKED_LIST, pbm.OUROBOROS])
+@pytest.mark.parametrize("loop", [False, True])
+def test_serve_destroy_gives_back_what_it_holds(gpu, strategy, loop):
+    # a server destroyed with sequences running: their pages and the pages
+    # promised to them go back, and the pool is as it was before
+    T, n = 1 << 16, 3000
+    log_case(f"serve destroy mid-run strategy={strategy} loop={loop}")
+    arr, tgt = _trace(n, 8000, 60, 0.8, 2048, seed=23)
+    with pbm.Pool(T, 64, strategy=strategy, seed=5) as pool:
+        pool.prefill_bernoulli(0.5, 3)
+        A = pool.free_count()
+        before = pool.snapshot()
+        srv = pbm.Server(pool, 512, 2048, n, 0, seed=SEED)
+        srv.submit(arr, tgt)
+        if loop:
+            srv.run_loop(40)
+        else:
+            srv.run(40)
+        st = srv.stats()
+        assert st["active"] > 100 and st["reserved_pages"] > 0
+        assert pool.free_count() == A - st["reserved_pages"] - (st["allocs"] - st["frees"])
+        srv.close()
+        assert pool.free_count() == A
+        assert np.array_equal(pool.snapshot(), before)
+
+
 def test_serve_rejects_bad_submissions(gpu):
     log_case("serve bad submissions")
     with pbm.Pool(1024, strategy=pbm.RW_BM) as pool:
