@@ -96,8 +96,9 @@ __global__ __launch_bounds__(1024) void k_collect_warp_data(pbm_ctx c, uint32_t 
 // pages, publishes the count, and finds the free pages before it by
 // decoupled look-back: wave 0 reads the look-back words of the 256
 // preceding tiles at once and sums back to the nearest one that published
-// its inclusive prefix.  Tile 0 fixes the call's grant G = min(n, free
-// count) and passes it down the chain inside the same words.  Then the
+// its inclusive prefix.  Tile 0 reserves the call's n requests at the
+// reservation counter, keeps G = min(n, free count) as the call's grant
+// (the rest goes back) and passes it down the chain inside the same words.  Then the
 // free page of rank q < G belongs to request q: each word claims its
 // granted bits with one atomicOr and each wave writes ids[q] for its ranks
 // in rank order (its own bits, lowest first).  A granted bit that a
@@ -178,14 +179,14 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
         }
         cntk[k] = (uint32_t)__popcll(fm[k]);
     }
-    // this tile's word from the last call, and the free count for tile 0's
-    // grant, read while the map words are in flight
+    // this tile's word from the last call, and tile 0's reservation of all
+    // n requests (the counter's value before it fixes the grant), in flight
+    // with the map words
     u64 own = 0;
     long long fr = 0;
     if (threadIdx.x == 0) {
         own = ld_word(&c.scan[tile]);
-        if (tile == 0)
-            fr = __hip_atomic_load(c.free_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tile == 0) fr = atom_add_ll(c.free_count, -(long long)n);
     }
     // inclusive scans of each word row's free counts over the wave
     uint32_t pre[WPT];
@@ -227,11 +228,13 @@ __device__ __forceinline__ void rank_tile(const pbm_ctx& c, uint32_t tile, uint3
         uint32_t grant = 0;
         u64 excl = 0;
         if (tile == 0) {
+            // the grant is what the reservation got, min(n, free count),
+            // exact even while other kernels reserve; the rest goes back
             const long long f = s_free;
             grant = f <= 0 ? 0u : (f > (long long)n ? n : (uint32_t)f);
             if (lane == 0) {
                 st_scan(&c.scan[0], scan_word(parity, true, grant, agg));
-                if (grant) atom_add_ll(c.free_count, -(long long)grant);
+                if (grant < n) atom_add_ll(c.free_count, (long long)(n - grant));
             }
         } else if (c.n_tiles > kGroupTiles * 4u) {
             // two-level look-back, for maps of more than 256 tiles: tile

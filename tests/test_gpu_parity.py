@@ -174,6 +174,64 @@ def test_batched_rank_calls_in_a_row(gpu, T):
 
 # ------------------------------------------------------------------ concurrent invariants
 
+
+def _with_bits(words, ids):
+    out = words.copy()
+    np.bitwise_or.at(out, ids >> 6, np.left_shift(np.uint64(1), (ids & 63).astype(np.uint64)))
+    return out
+
+
+@pytest.mark.parametrize("fill,n,drained", [(0.5, 1 << 17, False), (0.9, 1 << 18, True)])
+def test_rank_beside_plain_getpage_on_another_stream(gpu, fill, n, drained):
+    # the rank allocation while plain RandomWalkBasic_BM getPage kernels
+    # claim pages of the same map from another stream: every page granted
+    # once, the map is the pre-fill plus every grant, and the reservation
+    # counter still counts the map's free pages.  Below capacity every
+    # request is granted and no reservation is lost; a drained pool leaves
+    # no page free that a request went without
+    torch = gpu
+    T = 1 << 20
+    log_case(f"rank beside plain getPage fill={fill} n={n}")
+    with pbm.Pool(T, 4096, strategy=pbm.RW_BM, seed=11) as pool:
+        pool.prefill_bernoulli(fill, 5)
+        before = pool.snapshot()
+        A = pool.free_count()
+        sa, sb = torch.cuda.Stream(), torch.cuda.Stream()
+        rank = pbm.launch(pbm.MODE_BATCHED, flags=pbm.LAUNCH_RANK)
+        plain = pbm.launch(pbm.MODE_PLAIN)
+        grants = []
+        fallbacks = 0
+        for rep in range(3):
+            ids_a = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+            ids_b = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+            pool.clear_counters()
+            torch.cuda.synchronize()
+            if rep % 2:
+                pool.get_pages(n, ids_a, None, 0, rank, sa)
+                pool.get_pages(n, ids_b, None, 1 << 28, plain, sb)
+            else:
+                pool.get_pages(n, ids_b, None, 1 << 28, plain, sb)
+                pool.get_pages(n, ids_a, None, 0, rank, sa)
+            torch.cuda.synchronize()
+            a, b = ids_a.cpu().numpy(), ids_b.cpu().numpy()
+            assert (a >= -1).all() and (b >= -1).all()  # every request answered
+            g = np.concatenate([a[a >= 0], b[b >= 0]]).astype(np.int64)
+            cnt = pool.counters()
+            fallbacks += cnt["fallbacks"]
+            assert len(np.unique(g)) == len(g) and (g < T).all()
+            snap = pool.snapshot()
+            assert np.array_equal(snap, _with_bits(before, g))
+            used = int(np.unpackbits(snap.view(np.uint8)).sum())
+            assert pool.free_count() == T - used
+            if drained:
+                assert used == T
+            else:
+                assert len(g) == 2 * n and cnt["errors"] == 0
+            grants.append(len(g))
+            _free(torch, pool, g.astype(np.int32), pbm.MODE_BATCHED)
+            assert np.array_equal(pool.snapshot(), before) and pool.free_count() == A
+        print(f"grants per run {grants}, rank fallbacks over the runs {fallbacks}", flush=True)
+
 def _check_cycle(torch, pool, T, before, n, get_mode, free_mode, cw=0, expect_prefix=False,
                  budget=0):
     A = orc.count_free(before, T)
