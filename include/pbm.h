@@ -241,6 +241,14 @@ int pbm_clear_counters(pbm_handle* h, void* stream);
 /* Serves n one-page requests with indices req_base .. req_base+n-1.
  * d_ids[i] = page id or -1; d_steps[i] (optional, may be NULL) = walk
  * steps of request i.  The request index, not the thread, seeds the walk.
+ * Batched mode: a call's workgroups share the pool's completion and
+ * look-back words, so two batched calls of one pool must not run at once.
+ * The batched walk reads the reservation counter when it starts and takes
+ * its grant when it ends, so a plain call that reserves from the pool
+ * while it runs can leave both short (the excess requests fail after
+ * their walks and count as errors).  The rank launch (PBM_LAUNCH_RANK)
+ * takes its grant with one reserving atomic and may run beside plain
+ * calls.
  * Replaces the drivers' get1page_kernel / getPage_kernel
  * (unitTests/unitTest1.cu:22-30, unitTest1.5.cu:32-36) calling
  * getPage (RandomWalkBasic_BM.cuh:60-89, CollabRW_BM.cuh:114-160,
