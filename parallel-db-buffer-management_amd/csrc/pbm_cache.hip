@@ -61,6 +61,7 @@ enum : uint32_t {
     cPlan,          // (batch tag << 32) | V, published by the batch's last tile
     cInvalid,       // references to ids >= n_logical, skipped
     cInvalidBatch,  // the same in the current batch (cleared by its plan)
+    cPrevTiles,     // ring tiles the last batch's first sweep used (0: not known)
     cCtlWords = 32
 };
 
@@ -216,6 +217,7 @@ __device__ u64 lb_prefix(u64* words, uint32_t tile, u64 agg, const u64* done, u6
 struct TileShared {
     uint32_t tile, wsum[4];
     u64 excl, v;
+    uint32_t freed;  // frames a first-sweep tile gave back, credited once per tile
 };
 
 // tag: the batch's number since the cache was made, from 1.  A reference
@@ -229,6 +231,8 @@ __global__ __launch_bounds__(256) void k_cache_hits(CacheDev d, const int32_t* r
     for (uint32_t w = i; w < 2u * d.sweep_tiles; w += threads) d.lb_sweep[w] = 0;
     if (i == 0) {
         close_batch(d);
+        const u64 pd = d.ctl[cSweepDone0];
+        d.ctl[cPrevTiles] = pd == kNoStop ? 0u : pd + 1u;
         d.ctl[cTicket] = 0;
         d.ctl[cSweepTicket1] = 0;
         d.ctl[cSweepDone0] = kNoStop;
@@ -264,6 +268,16 @@ __device__ void miss_tile(const CacheDev& d, const int32_t* refs, uint32_t nb, u
                           uint32_t tile, TileShared& sh) {
     const u64 tag_hi = tag << 32;
     const uint32_t i0 = tile * kTile + threadIdx.x * kPer;
+    // the last tile's plan reads the counters the previous kernels left:
+    // loaded now, in flight with the tile's references
+    u64 r0 = 0, hits = 0, misses = 0, bad = 0;
+    const bool plans = tile == d.miss_tiles - 1u && threadIdx.x == 0;
+    if (plans) {
+        r0 = d.ctl[cResident];
+        hits = d.ctl[cHits];
+        misses = d.ctl[cMisses];
+        bad = d.ctl[cInvalidBatch];
+    }
     int32_t lv[kPer];
     uint32_t fm = 0, cnt = 0;
 #pragma unroll
@@ -293,10 +307,8 @@ __device__ void miss_tile(const CacheDev& d, const int32_t* refs, uint32_t nb, u
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k)
         if (fm >> k & 1u) d.miss[r++] = lv[k];
-    if (tile == d.miss_tiles - 1u && threadIdx.x == 0) {
+    if (plans) {
         const u64 K = sh.excl + agg;
-        const u64 r0 = d.ctl[cResident], hits = d.ctl[cHits], misses = d.ctl[cMisses];
-        const u64 bad = d.ctl[cInvalidBatch];
         const u64 empty = d.cap - r0;
         const u64 V = K > empty ? K - empty : 0u;
         d.ctl[cK] = K;
@@ -375,6 +387,7 @@ __device__ void sweep_tile(const CacheDev& d, uint32_t pass, uint32_t tile, u64 
             vv = w & kLow;
         }
         sh.v = vv;
+        sh.freed = 0;
         const u64 e = sh.excl;
         // a tile that holds the last victim, or lies past it, stops the
         // taking of further tiles (with no victims, every tile does)
@@ -398,6 +411,14 @@ __device__ void sweep_tile(const CacheDev& d, uint32_t pass, uint32_t tile, u64 
         for (u64 k = excl + agg + threadIdx.x; k < v; k += blockDim.x) d.victim[k] = -1;
     }
     u64 r = excl + off;  // candidates before this thread's first position
+    // first sweep: the victims' frames go back together after the loop, all
+    // their atomics in flight at once and one credit for the tile (a free
+    // per victim would wait for each atomic in turn).  The second sweep
+    // frees each frame before it publishes the victim, for the load
+    // threads that reserve a frame once they see it.
+    int32_t give[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) give[k] = -1;
 #pragma unroll
     for (uint32_t k = 0; k < kPer; ++k) {
         if (excl >= v || !(occ >> k & 1u) || r >= v) continue;
@@ -407,7 +428,8 @@ __device__ void sweep_tile(const CacheDev& d, uint32_t pass, uint32_t tile, u64 
             d.refbit[s] = kVictimBit;
             const int32_t old = own[k];
             d.slot_of[old] = -1;
-            if (frm[k] >= 0) pbm::free_page(d.c, frm[k], &st);
+            if (pass == 0) give[k] = frm[k];
+            else if (frm[k] >= 0) pbm::free_page(d.c, frm[k], &st);
             if (ev0 + vi < d.log_cap) d.log[ev0 + vi] = old;
             if (pass)
                 __hip_atomic_store(&d.victim[vi], (int32_t)s, __ATOMIC_RELEASE,
@@ -420,14 +442,53 @@ __device__ void sweep_tile(const CacheDev& d, uint32_t pass, uint32_t tile, u64 
             d.refbit[s] = 0u;  // the sweep passes a set bit
         }
     }
+    if (pass == 0) {
+        u64 mb[kPer], ob[kPer];
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            mb[k] = 0;
+            ob[k] = 0;
+            const int32_t f = give[k];
+            if (f >= 0 && (uint32_t)f < d.c.n_pages) {
+                mb[k] = 1ull << ((uint32_t)f & 63u);
+                ob[k] = pbm::atom_and(&d.c.bitmap[(uint32_t)f >> 6], ~mb[k]);
+                st.atomics++;
+            } else if (f >= 0) {
+                st.errors++;
+            }
+        }
+        uint32_t freed = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kPer; ++k) {
+            if (!mb[k]) continue;
+            if (ob[k] & mb[k]) ++freed;
+            else st.errors++;
+        }
+        st.frees += freed;
+        // the tile's credit: one atomic, after every wave's frees
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) freed += (uint32_t)__shfl_xor((int)freed, o, 64);
+        if (pbm::lane_id() == 0 && freed) atomicAdd(&sh.freed, freed);
+        __syncthreads();
+        if (threadIdx.x == 0 && sh.freed) pbm::atom_add_ll(d.c.free_count, (long long)sh.freed);
+    }
 }
 
-// Batch tiles, then ring tiles of the first sweep, taken by ticket.
+// Batch tiles, then ring tiles of the first sweep, taken by ticket.  Ring
+// tiles start before the batch's victims are known.  A ring tile at or
+// beyond `hold` (twice what the last batch's sweep used, and a margin) is
+// held, nothing loaded, until the victims are published and the tiles
+// before `hold` have counted their candidates; it is dropped when those
+// already cover the victims, or the tile that holds the last victim lies
+// before it.  (A dropped ticket is safe only past that tile: every tile
+// before it must run for the look-back.)
 __global__ __launch_bounds__(256) void k_cache_plan(CacheDev d, const int32_t* refs, uint32_t nb,
                                                     u64 tag) {
     __shared__ TileShared sh;
     const u64 arm = ldc(&d.ctl[cArm]), r0 = ldc(&d.ctl[cResident]);
     const u64 ev0 = ldc(&d.ctl[cEvictions]);
+    const u64 prev = ldc(&d.ctl[cPrevTiles]);
+    const u64 hold = prev ? 2u * prev + 8u : (u64)d.sweep_tiles;
     pbm::Stats st;
     for (;;) {
         if (threadIdx.x == 0) {
@@ -437,7 +498,29 @@ __global__ __launch_bounds__(256) void k_cache_plan(CacheDev d, const int32_t* r
                 tile = (uint32_t)t;
             } else {
                 const u64 rt = t - d.miss_tiles;
-                if (rt < d.sweep_tiles && !(ldc(&d.ctl[cSweepDone0]) < rt))
+                bool drop = false;
+                if (rt < d.sweep_tiles && rt >= hold) {
+                    // the batch tile that publishes the victims, and the
+                    // ring tiles before `hold`, were taken before this one
+                    // and wait on no held tile
+                    const u64 want = tag & kLow;
+                    uint32_t spins = 0;
+                    for (;;) {
+                        if (ldc(&d.ctl[cSweepDone0]) < rt) break;
+                        const u64 w = ldc(&d.ctl[cPlan]);
+                        const u64 c = ldc(&d.lb_sweep[hold - 1u]);
+                        if ((w >> 32) == want && (c >> 62) == 2u) {
+                            drop = (c & kLbCount) >= (w & kLow);
+                            break;
+                        }
+                        if (++spins > kLbSpin) {
+                            pbm::atom_add_u64(&d.ctl[cScanTimeouts], 1ull);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+                if (rt < d.sweep_tiles && !drop && !(ldc(&d.ctl[cSweepDone0]) < rt))
                     tile = d.miss_tiles + (uint32_t)rt;
             }
             sh.tile = tile;
