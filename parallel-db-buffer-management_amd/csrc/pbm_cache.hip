@@ -164,10 +164,12 @@ __device__ __forceinline__ uint32_t tile_scan(uint32_t cnt, uint32_t* s_wsum, ui
 // Decoupled look-back, run by wave 0: publishes the tile's own count, sums
 // the counts of the preceding tiles back to the nearest one that published
 // its inclusive prefix (64 at a time), publishes this tile's inclusive
-// prefix and returns the exclusive one.  Every preceding tile was taken
-// earlier by a running workgroup, so the wait ends, unless that workgroup
-// stopped because the sweep's stop (*done) lies before it, and then so
-// does this tile's: kNoStop is returned.  A bounded number of polls is
+// prefix and returns the exclusive one.  Every preceding tile is held by a
+// workgroup that has started (a workgroup's first tile is its own index,
+// and the dispatcher starts each XCD's workgroups in index order; later
+// tiles go by ticket, after all of those), so the wait ends, unless that
+// workgroup stopped because the sweep's stop (*done) lies before it, and
+// then so does this tile's: kNoStop is returned.  A bounded number of polls is
 // kept all the same, and giving up is counted.
 __device__ u64 lb_prefix(u64* words, uint32_t tile, u64 agg, const u64* done, u64* timeouts) {
     const unsigned lane = pbm::lane_id();
@@ -474,7 +476,9 @@ __device__ void sweep_tile(const CacheDev& d, uint32_t pass, uint32_t tile, u64 
     }
 }
 
-// Batch tiles, then ring tiles of the first sweep, taken by ticket.  Ring
+// Batch tiles, then ring tiles of the first sweep: workgroup w's first tile
+// is tile w (one ticket per workgroup at the start would queue them all at
+// the ticket's address), its later ones go by ticket from gridDim.x.  Ring
 // tiles start before the batch's victims are known.  A ring tile at or
 // beyond `hold` (twice what the last batch's sweep used, and a margin) is
 // held, nothing loaded, until the victims are published and the tiles
@@ -485,20 +489,26 @@ __device__ void sweep_tile(const CacheDev& d, uint32_t pass, uint32_t tile, u64 
 __global__ __launch_bounds__(256) void k_cache_plan(CacheDev d, const int32_t* refs, uint32_t nb,
                                                     u64 tag) {
     __shared__ TileShared sh;
-    const u64 arm = ldc(&d.ctl[cArm]), r0 = ldc(&d.ctl[cResident]);
-    const u64 ev0 = ldc(&d.ctl[cEvictions]);
-    const u64 prev = ldc(&d.ctl[cPrevTiles]);
-    const u64 hold = prev ? 2u * prev + 8u : (u64)d.sweep_tiles;
+    // the sweep's counters are loaded at the workgroup's first ring tile:
+    // a batch tile needs none of them and starts without waiting for them
+    u64 arm = 0, r0 = 0, ev0 = 0;
+    bool ring_ctl = false;
+    u64 hold = ~0ull;  // thread 0's
     pbm::Stats st;
-    for (;;) {
+    for (bool first = true;; first = false) {
         if (threadIdx.x == 0) {
-            const u64 t = pbm::atom_add_u64(&d.ctl[cTicket], 1ull);
+            const u64 t = first ? (u64)blockIdx.x
+                                : (u64)gridDim.x + pbm::atom_add_u64(&d.ctl[cTicket], 1ull);
             uint32_t tile = kNoTile;
             if (t < d.miss_tiles) {
                 tile = (uint32_t)t;
             } else {
                 const u64 rt = t - d.miss_tiles;
                 bool drop = false;
+                if (rt < d.sweep_tiles && hold == ~0ull) {
+                    const u64 prev = ldc(&d.ctl[cPrevTiles]);
+                    hold = prev ? 2u * prev + 8u : (u64)d.sweep_tiles;
+                }
                 if (rt < d.sweep_tiles && rt >= hold) {
                     // the batch tile that publishes the victims, and the
                     // ring tiles before `hold`, were taken before this one
@@ -528,10 +538,17 @@ __global__ __launch_bounds__(256) void k_cache_plan(CacheDev d, const int32_t* r
         __syncthreads();
         const uint32_t tile = sh.tile;
         if (tile == kNoTile) break;
-        if (tile < d.miss_tiles)
+        if (tile < d.miss_tiles) {
             miss_tile(d, refs, nb, tag, tile, sh);
-        else
+        } else {
+            if (!ring_ctl) {
+                arm = ldc(&d.ctl[cArm]);
+                r0 = ldc(&d.ctl[cResident]);
+                ev0 = ldc(&d.ctl[cEvictions]);
+                ring_ctl = true;
+            }
             sweep_tile(d, 0, tile - d.miss_tiles, kNoStop, tag, arm, r0, 0, ev0, sh, st);
+        }
         __syncthreads();  // the shared words are taken again
     }
     pbm::flush_stats(d.c, st);
