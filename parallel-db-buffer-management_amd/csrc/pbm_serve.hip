@@ -118,19 +118,37 @@ __device__ void serve_admit(const ServeDev& d) {
         s_vis[i] = d.seq_visible[head + i];
     }
     __syncthreads();
-    if (t == 0) {
-        long long budget = s_free;
+    if (t < 64u) {
+        // the first 64 waiting sequences are decided at once: sequence i is
+        // admitted while its arrival step has come, a slot is free and the
+        // targets up to i fit the unreserved pages.  Arrival steps do not
+        // decrease and targets are positive, so the admitted ones are a
+        // prefix; past 64, one at a time
+        const long long budget = s_free;
         const u64 slots = (u64)d.max_slots - active;
-        uint32_t m = 0;
-        long long add = 0;
-        while (m < win && (u64)s_arr[m] <= step && m < slots && budget >= (long long)s_tgt[m]) {
-            budget -= s_tgt[m];
-            add += s_tgt[m];
-            ++m;
+        long long pre = t < win ? (long long)s_tgt[t] : 0;
+#pragma unroll
+        for (unsigned o = 1; o < 64u; o <<= 1) {
+            const long long v = (long long)pbm::shfl64((u64)pre, t >= o ? t - o : t);
+            if (t >= o) pre += v;
         }
-        if (add) pbm::atom_add_ll(d.c.free_count, -add);
-        s_m = m;
-        s_add = add;
+        const bool ok = t < win && (u64)s_arr[t] <= step && t < slots && pre <= budget;
+        const u64 okm = pbm::ballot(ok);
+        uint32_t m = okm == pbm::kFull ? 64u : (uint32_t)__ffsll((long long)~okm) - 1u;
+        long long add = m ? (long long)pbm::readlane64((u64)pre, m - 1u) : 0;
+        if (t == 0) {
+            if (m == 64u) {
+                long long left = budget - add;
+                while (m < win && (u64)s_arr[m] <= step && m < slots && left >= (long long)s_tgt[m]) {
+                    left -= s_tgt[m];
+                    add += s_tgt[m];
+                    ++m;
+                }
+            }
+            if (add) pbm::atom_add_ll(d.c.free_count, -add);
+            s_m = m;
+            s_add = add;
+        }
     }
     __syncthreads();
     const uint32_t m = s_m;
@@ -218,14 +236,41 @@ __device__ __forceinline__ void grow_slot(const ServeDev& d, const pbm_ctx& cl, 
     }
 }
 
+// Control word of tally lane `lane` (nullptr: none): [0] finished at this
+// grow (the done list's base), [1] finished in all, [2] allocations, [3]
+// latency samples (the latency list's base), [4] failed allocations, [5]
+// frees, [6] payload errors, [8..15] the pool counters (Stats order).
+__device__ __forceinline__ u64* tally_word(const ServeDev& d, unsigned lane) {
+    switch (lane) {
+        case 0: return &d.ctl[kDone];
+        case 1: return &d.ctl[kFinished];
+        case 2: return &d.ctl[kAllocs];
+        case 3: return (d.flags & PBM_SERVE_LATENCY) ? &d.ctl[kLatCount] : nullptr;
+        case 4: return &d.ctl[kAllocFail];
+        case 5: return &d.ctl[kFrees];
+        case 6: return &d.ctl[kPayloadErr];
+        default:
+            return (lane >= 8u && lane < 16u && d.c.counters) ? (u64*)d.c.counters + (lane - 8u)
+                                                              : nullptr;
+    }
+}
+
+// Counts a device-resident loop launch adds up and writes once, at its
+// end (per tally lane, as tally_word): no phase of the loop reads them.
+struct LoopAcc {
+    u64 v[16];
+};
+
 // The workgroup's tallies of one grow (every thread calls it): finished
 // sequences go on the done list, latency samples on the latency list, and
 // the counts into the control block, one atomic per word.  st and bad may
 // also hold the frees of the same step (the loop releases its own
-// finished sequences).  g.sum must be zero and visible to the workgroup.
+// finished sequences).  With acc, every count but the two list bases is
+// added to acc instead (lane-owned words, written at the launch's end).
+// g.sum must be zero and visible to the workgroup.
 __device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, u64 bad,
                            bool act, bool got, bool fin, uint32_t ticks, uint32_t slot,
-                           uint32_t seq) {
+                           uint32_t seq, LoopAcc* acc = nullptr) {
     using namespace pbm;
     const unsigned lane = lane_id();
     const u64 F = ballot(fin);
@@ -257,29 +302,24 @@ __device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, 
     }
     __syncthreads();
     if (threadIdx.x < 64) {
-        // one lane per control word: finished (done-list base), finished
-        // total, allocations, latency base, failures, frees, payload
-        // errors, then the pool counters
+        // one lane per control word (tally_word)
         const uint32_t nf = g.sum[0], ng = g.sum[1], nx = g.sum[2];
-        const bool lat = (d.flags & PBM_SERVE_LATENCY) != 0;
-        u64* dst = nullptr;
+        u64* dst = tally_word(d, lane);
         u64 add = 0;
         switch (lane) {
-            case 0: dst = &d.ctl[kDone]; add = nf; break;
-            case 1: dst = &d.ctl[kFinished]; add = nf; break;
-            case 2: dst = &d.ctl[kAllocs]; add = ng; break;
-            case 3: dst = lat ? &d.ctl[kLatCount] : nullptr; add = ng; break;
-            case 4: dst = &d.ctl[kAllocFail]; add = nx; break;
-            case 5: dst = &d.ctl[kFrees]; add = g.sum[4]; break;
-            case 6: dst = &d.ctl[kPayloadErr]; add = g.sum[11]; break;
+            case 0: case 1: add = nf; break;
+            case 2: case 3: add = ng; break;
+            case 4: add = nx; break;
+            case 5: add = g.sum[4]; break;
+            case 6: add = g.sum[11]; break;
             default:
-                if (lane >= 8 && lane < 16 && d.c.counters) {
-                    dst = (u64*)d.c.counters + (lane - 8u);
-                    add = g.sum[3 + (lane - 8u)];
-                }
+                if (lane >= 8 && lane < 16) add = g.sum[3 + (lane - 8u)];
         }
         u64 old = 0;
-        if (dst && add) old = atom_add_u64(dst, add);
+        if (dst && add) {
+            if (acc && lane != 0 && lane != 3) acc->v[lane] += add;
+            else old = atom_add_u64(dst, add);
+        }
         if (lane == 0) g.base[0] = old;
         if (lane == 3) g.base[1] = old;
     }
@@ -333,10 +373,19 @@ __global__ __launch_bounds__(256) void k_serve_grow(ServeDev d) {
 // chunk's payloads are read back, checked and summed into the sequence's
 // checksum, then its pages go back to the pool through cl, whose
 // reservation counter takes the credit.
+//
+// With a bitmap strategy a wave takes kFreeChunks of its chunks at a time:
+// their page ids load together and their atomicAnds go out together, so a
+// wave with many chunks (a workgroup of the loop freeing its own long
+// sequences) waits two round trips per kFreeChunks chunks, not per chunk.
+constexpr uint32_t kFreeChunks = 4;
+
 template <int S, typename Entry>
 __device__ void free_finished(const ServeDev& d, const pbm_ctx& cl, uint32_t n, Entry entry,
                               uint32_t wave, uint32_t n_waves, pbm::Stats& st, u64& bad) {
     using namespace pbm;
+    constexpr bool kBits = S != PBM_LINKED_LIST && S != PBM_OUROBOROS;
+    constexpr uint32_t U = kBits ? kFreeChunks : 1u;
     const unsigned lane = lane_id();
     const bool payload = (d.flags & PBM_SERVE_PAYLOAD) && d.c.pool;
     for (uint32_t g0 = 0; g0 < n; g0 += 64u) {
@@ -351,36 +400,75 @@ __device__ void free_finished(const ServeDev& d, const pbm_ctx& cl, uint32_t n, 
             if (lane >= s) incl += v;
         }
         const uint32_t total = readlane(incl, 63u);
-        for (uint32_t ci = wave; ci < total; ci += n_waves) {
-            const unsigned src = (unsigned)__ffsll((long long)ballot(incl > ci)) - 1u;
-            const uint32_t c_slot = readlane(slot, src);
-            const uint32_t c_seq = readlane(seq, src);
-            const uint32_t c_key = d.key[c_seq];
-            const uint32_t c_len = readlane(len, src);
-            const uint32_t c_first = readlane(incl - chunks, src);
-            const uint32_t j = (ci - c_first) * 64u + lane;
-            const bool valid = j < c_len;
-            int32_t id = -1;
-            u64 sum = 0;
-            if (valid) {
-                id = d.pages[(u64)c_slot * d.max_pages + j];
-                if (payload) {
-                    const u64 v = *(const u64*)page_address(d.c, id);
-                    sum = v;
-                    bad += v != serve_payload(d.seed, c_key, j);
+        for (uint32_t c0 = wave; c0 < total; c0 += n_waves * U) {
+            int32_t id[U];
+            uint32_t cj[U], cseq[U];
+            bool valid[U];
+#pragma unroll
+            for (uint32_t u = 0; u < U; ++u) {
+                const uint32_t ci = c0 + u * n_waves;  // the same for the whole wave
+                id[u] = -1;
+                cj[u] = 0;
+                cseq[u] = 0;
+                valid[u] = false;
+                if (ci < total) {
+                    const unsigned src = (unsigned)__ffsll((long long)ballot(incl > ci)) - 1u;
+                    const uint32_t c_slot = readlane(slot, src);
+                    const uint32_t c_len = readlane(len, src);
+                    const uint32_t c_first = readlane(incl - chunks, src);
+                    cseq[u] = readlane(seq, src);
+                    cj[u] = (ci - c_first) * 64u + lane;
+                    valid[u] = cj[u] < c_len;
+                    if (valid[u]) id[u] = d.pages[(u64)c_slot * d.max_pages + cj[u]];
                 }
             }
             if (payload) {
 #pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) sum += shfl64(sum, lane ^ (unsigned)o);
-                if (lane == 0) atom_add_u64(&d.res_sum[c_seq], sum);
+                for (uint32_t u = 0; u < U; ++u) {
+                    if (c0 + u * n_waves >= total) continue;
+                    u64 sum = 0;
+                    if (valid[u]) {
+                        const u64 v = *(const u64*)page_address(d.c, id[u]);
+                        sum = v;
+                        bad += v != serve_payload(d.seed, d.key[cseq[u]], cj[u]);
+                    }
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) sum += shfl64(sum, lane ^ (unsigned)o);
+                    if (lane == 0) atom_add_u64(&d.res_sum[cseq[u]], sum);
+                }
                 // the read-back completes before the pages go back: a page
                 // freed with its load still in flight could be taken and
                 // written by a concurrent grow (the loop's workgroups free
                 // while others grow) before the load is served
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             }
-            if (valid) free_page_of<S>(cl, id, &st);
+            if constexpr (kBits) {
+                // free_page, the chunks' atomics in flight together
+                u64 m[U], o[U];
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    m[u] = 0;
+                    o[u] = 0;
+                    if (valid[u] && (uint32_t)id[u] < d.c.n_pages) {
+                        m[u] = 1ull << ((uint32_t)id[u] & 63u);
+                        o[u] = atom_and(&d.c.bitmap[(uint32_t)id[u] >> 6], ~m[u]);
+                        st.atomics++;
+                    }
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u) {
+                    const bool ok = (o[u] & m[u]) != 0ull;
+                    if (valid[u]) {
+                        if (ok) st.frees++;
+                        else st.errors++;
+                    }
+                    release_one(cl, ok);
+                }
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < U; ++u)
+                    if (valid[u]) free_page_of<S>(cl, id[u], &st);
+            }
         }
     }
 }
@@ -558,11 +646,13 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
     }
     __shared__ GrowLds g;
     __shared__ LoopLds L;
+    __shared__ LoopAcc acc;
     if (threadIdx.x == 0) {
         g.units = kGrowUnits;
         L.n = 0;
         L.credit = 0;
     }
+    if (threadIdx.x < 16) acc.v[threadIdx.x] = 0;
     pbm_ctx cl = d.c;  // the grow's: pages reserved at admission
     cl.free_count = &g.units;
     pbm_ctx cr = d.c;  // the release's: credit held in LDS
@@ -628,7 +718,7 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
                 ln = L.len[n_held + e];
             },
             wave, kWaves, st, bad);
-        grow_tally(d, g, st, bad, act, got, fin, ticks, slot, fseq);
+        grow_tally(d, g, st, bad, act, got, fin, ticks, slot, fseq, &acc);
         // the step's freed pages back to the pool, after admission read it
         if (threadIdx.x == 0) {
             if (L.credit) atom_add_ll(d.c.free_count, L.credit);
@@ -637,6 +727,11 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
         }
         loop_signal(sync, 2, false, 1ull);
     }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        u64* dst = tally_word(d, threadIdx.x);
+        if (dst && acc.v[threadIdx.x]) atom_add_u64(dst, acc.v[threadIdx.x]);
+    }
 }
 
 // Before a loop launch: its phase counters start at zero.
@@ -644,12 +739,30 @@ __global__ void k_serve_loop_prep(u64* sync) {
     if (threadIdx.x < 3) sync[threadIdx.x] = 0;
 }
 
+// The j-th slot handed out: slots go round the 1,024-slot groups (0, 1024,
+// 2048, ..., 1, 1025, ...), so the running sequences spread evenly over
+// the device-resident loop's workgroups instead of filling the first ones.
+__device__ __forceinline__ uint32_t slot_order(uint32_t j, uint32_t max_slots) {
+    const uint32_t G = max_slots / 1024u, rem = max_slots % 1024u;
+    uint32_t r, q;
+    if (j < rem * (G + 1u)) {
+        r = j / (G + 1u);
+        q = j % (G + 1u);
+    } else {
+        const uint32_t jj = j - rem * (G + 1u);
+        r = rem + jj / G;
+        q = jj % G;
+    }
+    return q * 1024u + r;
+}
+
 __global__ void k_serve_init(ServeDev d) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < d.max_slots) {
         d.slot_seq[i] = kEmpty;
         d.slot_len[i] = 0;
-        d.stack[i] = d.max_slots - 1u - i;  // slot 0 is popped first
+        // popped from the top: stack[max_slots - 1] goes first
+        d.stack[i] = slot_order(d.max_slots - 1u - i, d.max_slots);
     }
     if (i < kCtlWords) d.ctl[i] = i == kTop ? (u64)d.max_slots : 0ull;
 }
