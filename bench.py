@@ -353,7 +353,8 @@ def bursts_leg(torch, pbm, stream, n_bursts: int, tune_bursts: int, persistent: 
         "malloc_failures": int((ptrs == 0).sum()),
         "launch": {k: tune[k] for k in ("waves_per_cu", "walk_budget", "cluster_words",
                                         "persistent")},
-        "tune_ms_first_bursts": tune["ms"], "tune_candidates": tune["candidates"],
+        "tune_ms_first_bursts": tune["ms"], "tune_ms_longer_run": tune["ms_longer_run"],
+        "tune_candidates": tune["candidates"],
         "wait_timeouts": timeouts,
         "live_runs_at_end": end["live_runs"], "pages_live_at_end": end["live_pages"],
         # the serial oracle serves every request of this trace

@@ -753,10 +753,13 @@ int pbm_bursts_wait_timeouts(pbm_bursts* b, unsigned long long* out, void* strea
 /* Times the first `count` bursts from the pool's current state over the
  * batched launch space (waves per CU 1/2/4/8/16, walk budget 4/16/64,
  * getNPage window 64/128/256 words, phase kernels or one persistent
- * kernel; phase kernels only with PBM_TUNE_NO_PERSISTENT) and returns the
- * fastest in get_launch (ms_alloc = its time).  The pool's map and the
- * pointer table are put back after every candidate and at the end, so the
- * pool must hold only what the trace's first burst expects. */
+ * kernel; phase kernels only with PBM_TUNE_NO_PERSISTENT), times the three
+ * fastest again over the first min(n_bursts, 4 * count) bursts when that
+ * is more, and returns the fastest of those in get_launch (ms_alloc = its
+ * time over the first count bursts, ms_free = its time over the longer run,
+ * 0 when there was none).  The pool's map and the pointer table are put
+ * back after every candidate and at the end, so the pool must hold only
+ * what the trace's first burst expects. */
 int pbm_bursts_autotune(pbm_bursts* b, uint32_t count, uint32_t flags, pbm_tune_result* out,
                         void* stream);
 

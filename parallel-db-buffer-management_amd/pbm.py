@@ -855,7 +855,7 @@ class Bursts:
         return {"launch": g, "waves_per_cu": int(g.waves_per_cu), "walk_budget": int(g.walk_budget),
                 "cluster_words": int(g.cluster_words) or 64,
                 "persistent": bool(g.flags & LAUNCH_PERSISTENT), "ms": float(r.ms_alloc),
-                "candidates": int(r.candidates)}
+                "ms_longer_run": float(r.ms_free), "candidates": int(r.candidates)}
 
     def close(self) -> None:
         if self.b:
