@@ -106,11 +106,12 @@ def test_serve_loop_schedule_matches_oracle(gpu, strategy):
         assert np.array_equal(pool.snapshot(), orc.prefill(T, 0.5, 3))
 
 
-@pytest.mark.parametrize("slots", [4096, 32768])
+@pytest.mark.parametrize("slots", [3000, 4096, 32768])
 def test_serve_loop_equals_eager_steps(gpu, slots):
-    # many workgroups (16 and 128): loop runs, eager steps and a graph of
-    # loop runs give the same schedule, lengths, checksums and counters,
-    # with live submission block by block
+    # 3, 4 and 32 loop workgroups (the first with a partial last group of
+    # slots, which the slot order deals round them too): loop runs, eager
+    # steps and a graph of loop runs give the same schedule, lengths,
+    # checksums and counters, with live submission block by block
     T, n, B = 1 << 20, 12000, 50
     log_case(f"serve loop vs eager slots={slots}")
     arr, tgt = _trace(n, 20000, 200, 0.8, 4096, seed=17)
