@@ -142,8 +142,8 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
     over as many steps replayed from a captured `block`-step graph and as
     many run by the device-resident loop (`block` steps per launch), the
     per-allocation latency of the device getPage calls and the
-    arrival-to-admission wait of the sequences admitted in the eager
-    region, and each step kernel's own duration."""
+    arrival-to-admission wait of the sequences admitted in the eager region
+    and in the loop region, and each step kernel's own duration."""
     T, n_steps = 1 << 26, warm_steps + 3 * timed_steps + 200 + 2 * block
     n = int(20000 * n_steps * 1e-3 * 1.2) + 64
     arr, tgt = pbm.serve_trace(n, 20000.0, 1e-3, 512.0, 0.8, 7, 16384)
@@ -213,12 +213,16 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
     state["done"] += 200
     s4 = srv.stats(stream)
     c4 = pool.counters(stream)
-    lat = srv.latency_ticks(1 << 26, stream)[s0["lat_samples"]:s1["lat_samples"]]
+    lat_all = srv.latency_ticks(1 << 26, stream)
+    lat = lat_all[s0["lat_samples"]:s1["lat_samples"]]
+    lat_loop = lat_all[s2["lat_samples"]:s2l["lat_samples"]]
     res = srv.results(0, state["sub"], stream)
     waits = srv.wait_ticks(0, state["sub"], stream)
     hz = srv.clock_hz()
     in_eager = (res["admit_step"] > s0["step"]) & (res["admit_step"] <= s1["step"])
     w = waits[in_eager].astype(np.float64)
+    in_loop = (res["admit_step"] > s2["step"]) & (res["admit_step"] <= s2l["step"])
+    w_loop = waits[in_loop].astype(np.float64)
     w_steps = (res["admit_step"][in_eager].astype(np.int64) - arr[:state["sub"]][in_eager])
     ops = (s1["allocs"] - s0["allocs"]) + (s1["frees"] - s0["frees"])
     ops_graph = (s2["allocs"] - s1["allocs"]) + (s2["frees"] - s1["frees"])
@@ -242,7 +246,12 @@ def serving_leg(torch, pbm, stream, warm_steps: int, timed_steps: int, block: in
         "graph": {"steps_per_graph": block, "ms_per_step": ms_graph / timed_steps,
                   "ops_per_s": ops_graph / (ms_graph * 1e-3)},
         "loop": {"steps_per_launch": block, "ms_per_step": ms_loop / timed_steps,
-                 "ops_per_s": ops_loop / (ms_loop * 1e-3), "wait_timeouts": loop_timeouts},
+                 "ops_per_s": ops_loop / (ms_loop * 1e-3), "wait_timeouts": loop_timeouts,
+                 "p50_alloc_us": float(np.percentile(lat_loop, 50)) * 1e6 / hz if lat_loop.size else None,
+                 "p99_alloc_us": float(np.percentile(lat_loop, 99)) * 1e6 / hz if lat_loop.size else None,
+                 "latency_samples": int(lat_loop.size), "admitted": int(in_loop.sum()),
+                 "p50_wait_us": float(np.percentile(w_loop, 50)) * 1e6 / hz if w_loop.size else None,
+                 "p99_wait_us": float(np.percentile(w_loop, 99)) * 1e6 / hz if w_loop.size else None},
         "fastest": min(({"mode": "eager", "ms_per_step": ms / timed_steps},
                         {"mode": "graph", "ms_per_step": ms_graph / timed_steps},
                         {"mode": "loop", "ms_per_step": ms_loop / timed_steps}),
