@@ -492,9 +492,10 @@ int pbm_serve_destroy(pbm_server* s);
  * across calls), target[i] in [1, max_pages] and key[i], the request's
  * own id that its payloads derive from (NULL: the sequence id).  Sequence
  * ids count up from 0 in submission order.  Stream-ordered and without
- * host synchronisation: the arrays are copied (through the server's
- * pinned mirror) and made visible to admission after the work already on
- * the stream, so sequences can be submitted while steps run; a sequence
+ * host synchronisation: the arrays are copied into the server's pinned
+ * mirror, which one kernel on the stream reads from the device and makes
+ * visible to admission after the work already on the stream, so sequences
+ * can be submitted while steps run; a sequence
  * is admissible from its arrival step or from the first step after its
  * submission, whichever is later.  The host arrays may be reused when the
  * call returns. */

@@ -35,9 +35,9 @@ enum : uint32_t {
 
 struct ServeDev {
     pbm_ctx c;
-    const uint32_t* arr_step;  // [max_seqs] arrival step
-    const uint32_t* target;    // [max_seqs] target length
-    const uint32_t* key;       // [max_seqs] payload key
+    uint32_t* arr_step;        // [max_seqs] arrival step
+    uint32_t* target;          // [max_seqs] target length
+    uint32_t* key;             // [max_seqs] payload key
     uint32_t* res_admit;       // [max_seqs]
     uint32_t* res_finish;      // [max_seqs]
     uint32_t* res_len;         // [max_seqs]
@@ -767,13 +767,20 @@ __global__ void k_serve_init(ServeDev d) {
     if (i < kCtlWords) d.ctl[i] = i == kTop ? (u64)d.max_slots : 0ull;
 }
 
-// Makes sequences first .. tail-1 (copied in before this kernel on the
-// same stream) visible to admission, stamping when that happened.
-__global__ void k_serve_set_tail(ServeDev d, u64 first, u64 tail) {
+// Takes sequences first .. tail-1 from the server's pinned host mirror
+// (read by the device directly, so a submission is this one kernel on the
+// stream, no copies), stamps when they became visible, and makes them
+// visible to admission: the steps after this kernel on the stream see them.
+__global__ void k_serve_take(ServeDev d, const uint32_t* m_arr, const uint32_t* m_tgt,
+                             const uint32_t* m_key, u64 first, u64 tail) {
     const u64 now = wall_clock64();
     for (u64 i = first + blockIdx.x * blockDim.x + threadIdx.x; i < tail;
-         i += (u64)gridDim.x * blockDim.x)
+         i += (u64)gridDim.x * blockDim.x) {
+        d.arr_step[i] = m_arr[i];
+        d.target[i] = m_tgt[i];
+        d.key[i] = m_key[i];
         d.seq_visible[i] = now;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) d.ctl[kTail] = tail;
 }
 
@@ -817,11 +824,12 @@ struct pbm_server {
     uint32_t last_arrival = 0;
     uint32_t release_blocks = 1;
     std::vector<void*> mem;
-    // pinned host mirror of arr_step, target and key: a submit copies into
-    // it and from it to the device without waiting for the stream; no
-    // sequence id is ever written twice, so a copy in flight is never
-    // overwritten
+    // pinned host mirror of arr_step, target and key: a submit writes into
+    // it and launches one kernel that reads it from the device, without
+    // waiting for the stream; no sequence id is ever written twice, so an
+    // entry a launch still has to read is never overwritten
     uint32_t* h_mirror = nullptr;  // [3 * max_seqs]
+    uint32_t* d_mirror = nullptr;  // the same memory as the device addresses it
     u64* d_sync = nullptr;         // device-resident loop: phase counters, give-ups
     bool ready = false;            // created and initialised
 };
@@ -908,8 +916,13 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
         if (!rc) s->d_sync = (u64*)p;
     }
     if (!rc) {
-        const hipError_t he = hipHostMalloc((void**)&s->h_mirror, 3u * ns * 4u, hipHostMallocDefault);
+        hipError_t he = hipHostMalloc((void**)&s->h_mirror, 3u * ns * 4u, hipHostMallocDefault);
         if (he != hipSuccess) rc = pbmi::hip_fail(he, "hipHostMalloc(serve mirror)", __FILE__, __LINE__);
+        if (!rc) {
+            he = hipHostGetDevicePointer((void**)&s->d_mirror, s->h_mirror, 0);
+            if (he != hipSuccess)
+                rc = pbmi::hip_fail(he, "hipHostGetDevicePointer(serve mirror)", __FILE__, __LINE__);
+        }
     }
     if (rc) {
         pbm_serve_destroy(s);
@@ -961,15 +974,11 @@ extern "C" int pbm_serve_submit(pbm_server* s, uint32_t n, const uint32_t* arriv
     std::memcpy(m_arr, arrival_step, (size_t)n * 4u);
     std::memcpy(m_tgt, target, (size_t)n * 4u);
     for (uint32_t i = 0; i < n; ++i) m_key[i] = key ? key[i] : s->submitted + i;
-    PBM_HIP(hipMemcpyAsync((void*)(s->d.arr_step + at), m_arr, (size_t)n * 4u,
-                           hipMemcpyHostToDevice, st));
-    PBM_HIP(hipMemcpyAsync((void*)(s->d.target + at), m_tgt, (size_t)n * 4u,
-                           hipMemcpyHostToDevice, st));
-    PBM_HIP(hipMemcpyAsync((void*)(s->d.key + at), m_key, (size_t)n * 4u,
-                           hipMemcpyHostToDevice, st));
+    // the device reads the mirror directly; the host's writes above come
+    // before the launch that reads them
     const uint32_t blocks = n > 65536u ? 256u : (n + 255u) / 256u;
-    hipLaunchKernelGGL(k_serve_set_tail, dim3(blocks), dim3(256), 0, st, s->d, (u64)at,
-                       (u64)at + n);
+    hipLaunchKernelGGL(k_serve_take, dim3(blocks), dim3(256), 0, st, s->d, s->d_mirror,
+                       s->d_mirror + ns, s->d_mirror + 2u * ns, (u64)at, (u64)at + n);
     PBM_HIP(hipGetLastError());
     s->submitted += n;
     s->last_arrival = last;
