@@ -22,7 +22,7 @@ enum : uint32_t {
     kActive,      // slots admitted at the last grow (finished ones included)
     kPromised,    // pages promised at admission, summed over every admitted sequence
     kTop,         // free-slot stack height
-    kDone,        // sequences that finished at the last grow
+    kDone,        // sequences that finished at a grow of an even step (done list 0)
     kAdmitted,
     kFinished,
     kAllocs,
@@ -30,6 +30,7 @@ enum : uint32_t {
     kAllocFail,
     kPayloadErr,
     kLatCount,
+    kDoneOdd,     // the same for odd steps (done list 1)
     kCtlWords = 16
 };
 
@@ -47,7 +48,8 @@ struct ServeDev {
     uint32_t* slot_target;     // [max_slots]
     int32_t* pages;            // [max_slots * max_pages]
     uint32_t* stack;           // [max_slots] free slots
-    uint32_t* done;            // [2 * max_slots] (slot, sequence) finished at the last grow
+    uint32_t* done;            // [2][2 * max_slots] (slot, sequence) finished at the grow of a
+                               // step, one list per step parity
     u64* ctl;                  // [kCtlWords]
     uint32_t* lat;             // [lat_cap] wall-clock ticks per allocation
     u64* seq_visible;          // [max_seqs] wall clock when the sequence was submitted
@@ -65,8 +67,17 @@ __device__ __forceinline__ u64 ld_ctl(const u64* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Admission (one workgroup).  First the bookkeeping of the last step: kDone
-// of its kActive slots finished and go back on the free-slot stack.  Then
+// The done list of step s and its length word: two lists by parity, so the
+// grow of step s can list its finished sequences while the admission of
+// step s reads those of step s - 1.
+__device__ __forceinline__ uint32_t done_word(u64 s) { return (s & 1u) ? kDoneOdd : kDone; }
+__device__ __forceinline__ uint32_t* done_list(const ServeDev& d, u64 s) {
+    return d.done + (size_t)(s & 1u) * 2u * d.max_slots;
+}
+
+// Admission (one workgroup).  First the bookkeeping of the last step: the
+// sequences on its done list finished, and their slots go back on the
+// free-slot stack.  Then
 // waiting sequences are admitted in arrival order while their arrival step
 // has come, a slot is free and the pool's unreserved pages cover the
 // sequence's target.  Admission reserves every admitted sequence's whole
@@ -100,13 +111,14 @@ __device__ void serve_admit(const ServeDev& d) {
     const u64 now = s_now;
     const u64 head = s_ctl[kHead];
     const u64 tail = s_ctl[kTail];
-    const u64 done_n = s_ctl[kDone];
+    const u64 done_n = s_ctl[done_word(step - 1u)];
+    const uint32_t* done = done_list(d, step - 1u);
     const u64 top_old = s_ctl[kTop];
     const u64 top = top_old + done_n;
     const u64 active = s_ctl[kActive] - done_n;
     const uint32_t win = (uint32_t)(tail - head < kAdmitWindow ? tail - head : kAdmitWindow);
     for (u64 i = t; i < done_n; i += blockDim.x) {
-        const uint32_t sl = d.done[2u * i];
+        const uint32_t sl = done[2u * i];
         d.stack[top_old + i] = sl;
         const u64 j = done_n - 1u - i;  // popped j-th
         if (j < win) s_slot[j] = sl;
@@ -181,7 +193,7 @@ __device__ void serve_admit(const ServeDev& d) {
         d.ctl[kActive] = active + m;
         d.ctl[kPromised] = s_ctl[kPromised] + (u64)s_add;
         d.ctl[kTop] = top - m;
-        d.ctl[kDone] = 0;
+        d.ctl[done_word(step - 1u)] = 0;  // the list of step + 1
         d.ctl[kAdmitted] = s_ctl[kAdmitted] + m;
     }
 }
@@ -236,13 +248,14 @@ __device__ __forceinline__ void grow_slot(const ServeDev& d, const pbm_ctx& cl, 
     }
 }
 
-// Control word of tally lane `lane` (nullptr: none): [0] finished at this
-// grow (the done list's base), [1] finished in all, [2] allocations, [3]
-// latency samples (the latency list's base), [4] failed allocations, [5]
-// frees, [6] payload errors, [8..15] the pool counters (Stats order).
-__device__ __forceinline__ u64* tally_word(const ServeDev& d, unsigned lane) {
+// Control word of tally lane `lane` (nullptr: none): [0] finished at the
+// grow of step `step` (the done list's base), [1] finished in all, [2]
+// allocations, [3] latency samples (the latency list's base), [4] failed
+// allocations, [5] frees, [6] payload errors, [8..15] the pool counters
+// (Stats order).
+__device__ __forceinline__ u64* tally_word(const ServeDev& d, unsigned lane, u64 step) {
     switch (lane) {
-        case 0: return &d.ctl[kDone];
+        case 0: return &d.ctl[done_word(step)];
         case 1: return &d.ctl[kFinished];
         case 2: return &d.ctl[kAllocs];
         case 3: return (d.flags & PBM_SERVE_LATENCY) ? &d.ctl[kLatCount] : nullptr;
@@ -255,22 +268,13 @@ __device__ __forceinline__ u64* tally_word(const ServeDev& d, unsigned lane) {
     }
 }
 
-// Counts a device-resident loop launch adds up and writes once, at its
-// end (per tally lane, as tally_word): no phase of the loop reads them.
-struct LoopAcc {
-    u64 v[16];
-};
-
 // The workgroup's tallies of one grow (every thread calls it): finished
 // sequences go on the done list, latency samples on the latency list, and
-// the counts into the control block, one atomic per word.  st and bad may
-// also hold the frees of the same step (the loop releases its own
-// finished sequences).  With acc, every count but the two list bases is
-// added to acc instead (lane-owned words, written at the launch's end).
-// g.sum must be zero and visible to the workgroup.
+// the counts into the control block, one atomic per word.  g.sum must be
+// zero and visible to the workgroup.
 __device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, u64 bad,
                            bool act, bool got, bool fin, uint32_t ticks, uint32_t slot,
-                           uint32_t seq, LoopAcc* acc = nullptr) {
+                           uint32_t seq, u64 step) {
     using namespace pbm;
     const unsigned lane = lane_id();
     const u64 F = ballot(fin);
@@ -304,7 +308,7 @@ __device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, 
     if (threadIdx.x < 64) {
         // one lane per control word (tally_word)
         const uint32_t nf = g.sum[0], ng = g.sum[1], nx = g.sum[2];
-        u64* dst = tally_word(d, lane);
+        u64* dst = tally_word(d, lane, step);
         u64 add = 0;
         switch (lane) {
             case 0: case 1: add = nf; break;
@@ -316,18 +320,16 @@ __device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, 
                 if (lane >= 8 && lane < 16) add = g.sum[3 + (lane - 8u)];
         }
         u64 old = 0;
-        if (dst && add) {
-            if (acc && lane != 0 && lane != 3) acc->v[lane] += add;
-            else old = atom_add_u64(dst, add);
-        }
+        if (dst && add) old = atom_add_u64(dst, add);
         if (lane == 0) g.base[0] = old;
         if (lane == 3) g.base[1] = old;
     }
     __syncthreads();
     if (fin) {
         const u64 k = g.base[0] + wf + rank_in(F);
-        d.done[2u * k] = slot;
-        d.done[2u * k + 1u] = seq;
+        uint32_t* done = done_list(d, step);
+        done[2u * k] = slot;
+        done[2u * k + 1u] = seq;
     }
     if ((d.flags & PBM_SERVE_LATENCY) && got) {
         const u64 k = g.base[1] + wg + rank_in(G);
@@ -359,7 +361,7 @@ __device__ void serve_grow(const ServeDev& d, uint32_t slot) {
     bool fin = false, got = false;
     uint32_t ticks = 0;
     if (act) grow_slot<S>(d, cl, slot, step, seq, len, tgt, st, fin, got, ticks);
-    grow_tally(d, g, st, 0ull, act, got, fin, ticks, slot, fseq);
+    grow_tally(d, g, st, 0ull, act, got, fin, ticks, slot, fseq, step);
 }
 
 template <int S>
@@ -490,14 +492,17 @@ __device__ void serve_release(const ServeDev& d, uint32_t wave, uint32_t n_waves
     pbm_ctx cl = d.c;
     cl.free_count = &s_credit;
     const unsigned lane = lane_id();
-    const uint32_t done_n = (uint32_t)ld_ctl(&d.ctl[kDone]);
+    const u64 step = ld_ctl(&d.ctl[kStep]);
+    const u64 dn[2] = {ld_ctl(&d.ctl[kDone]), ld_ctl(&d.ctl[kDoneOdd])};
+    const uint32_t done_n = (uint32_t)dn[step & 1u];
+    const uint32_t* done = done_list(d, step);
     Stats st;
     u64 bad = 0;
     free_finished<S>(
         d, cl, done_n,
         [&](uint32_t e, uint32_t& slot, uint32_t& seq, uint32_t& len) {
-            slot = d.done[2u * e];
-            seq = d.done[2u * e + 1u];
+            slot = done[2u * e];
+            seq = done[2u * e + 1u];
             len = d.slot_len[slot];
         },
         wave, n_waves, st, bad);
@@ -628,6 +633,34 @@ __device__ __forceinline__ void loop_list(LoopLds& L, bool fin, uint32_t slot, u
     }
 }
 
+// Positions in a device list for the workgroup's items: one atomic on the
+// list's length word `len` for the whole workgroup, none when it has no
+// item (every thread calls it, with k items of its own).  Returns the
+// position of this thread's first item.  s.n is zero on entry.
+struct TakeLds {
+    uint32_t n;
+    u64 base;
+};
+
+__device__ u64 take_positions(u64* len, uint32_t k, TakeLds& s) {
+    using namespace pbm;
+    const unsigned lane = lane_id();
+    uint32_t inc = k;
+#pragma unroll
+    for (unsigned o = 1; o < 64u; o <<= 1) {
+        const uint32_t v = shfl32(inc, lane >= o ? lane - o : lane);
+        if (lane >= o) inc += v;
+    }
+    const uint32_t wtot = readlane(inc, 63u);
+    uint32_t wbase = 0;
+    if (lane == 0 && wtot) wbase = atomicAdd(&s.n, wtot);
+    wbase = readlane(wbase, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) s.base = s.n ? atom_add_u64(len, (u64)s.n) : 0ull;
+    __syncthreads();
+    return s.base + wbase + (inc - k);
+}
+
 template <int S>
 __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_t n_steps,
                                                              u64* sync) {
@@ -646,39 +679,69 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
     }
     __shared__ GrowLds g;
     __shared__ LoopLds L;
-    __shared__ LoopAcc acc;
+    // [0] finished held sequences, [1] latency samples, [2] sequences that
+    // finished at their first page
+    __shared__ TakeLds tk[3];
     if (threadIdx.x == 0) {
         g.units = kGrowUnits;
         L.n = 0;
         L.credit = 0;
     }
-    if (threadIdx.x < 16) acc.v[threadIdx.x] = 0;
     pbm_ctx cl = d.c;  // the grow's: pages reserved at admission
     cl.free_count = &g.units;
     pbm_ctx cr = d.c;  // the release's: credit held in LDS
     cr.free_count = &L.credit;
     const uint32_t slot = blockIdx.x * kLoopThreads + threadIdx.x;
     const uint32_t wave = threadIdx.x >> 6;
+    const bool lat = (d.flags & PBM_SERVE_LATENCY) != 0;
+    // the launch's counts, per thread, written once at its end
+    Stats st;
+    u64 bad = 0;
+    uint32_t c_alloc = 0, c_fin = 0, c_fail = 0;
+    // a new slot's first latency sample, listed with the next step's
+    bool pend = false;
+    uint32_t pend_ticks = 0;
     // the slot as this thread left it at its last grow (kEmpty: load it
     // once admission is visible)
     uint32_t seq = kEmpty, len = 0, tgt = 0;
     u64 step = 0;
     for (uint32_t k = 0; k < n_steps; ++k) {
-        if (threadIdx.x < 12) g.sum[threadIdx.x] = 0;
-        Stats st;
-        u64 bad = 0;
-        bool act = false, fin = false, got = false;
+        if (threadIdx.x < 3u) tk[threadIdx.x].n = 0;
+        bool fin = false, got = false;
         uint32_t ticks = 0, fseq = kEmpty;
         // slots admitted before this step, while admission runs
         const bool held = seq != kEmpty;  // never at k = 0
+        const u64 now_step = step + 1u;   // this step, for k > 0
         if (held) {
-            act = true;
             fseq = seq;
-            grow_slot<S>(d, cl, slot, step + 1u, seq, len, tgt, st, fin, got, ticks);
+            grow_slot<S>(d, cl, slot, now_step, seq, len, tgt, st, fin, got, ticks);
+            c_alloc += got ? 1u : 0u;
+            c_fail += got ? 0u : 1u;
+            c_fin += fin ? 1u : 0u;
         }
         loop_list(L, fin, slot, fseq, len);
         __syncthreads();
         const uint32_t n_held = L.n;
+        if (k > 0) {
+            // also while admission runs: they go on this step's done list
+            // (admission reads the last step's) and the latency list
+            const u64 dp = take_positions(&d.ctl[done_word(now_step)], fin ? 1u : 0u, tk[0]);
+            if (fin) {
+                uint32_t* dl = done_list(d, now_step);
+                dl[2u * dp] = slot;
+                dl[2u * dp + 1u] = fseq;
+            }
+            if (lat) {
+                u64 lp = take_positions(&d.ctl[kLatCount], (pend ? 1u : 0u) + (got ? 1u : 0u),
+                                        tk[1]);
+                if (pend) {
+                    if (lp < d.lat_cap) d.lat[lp] = pend_ticks;
+                    ++lp;
+                }
+                if (got && lp < d.lat_cap) d.lat[lp] = ticks;
+                pend = false;
+            }
+        }
         free_finished<R>(
             d, cr, n_held,
             [](uint32_t e, uint32_t& sl, uint32_t& sq, uint32_t& ln) {
@@ -690,7 +753,7 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
         // slots admitted now
         loop_wait_ge(sync, 0, (u64)k + 1u);
         uint32_t nseq = kEmpty, nlen = 0, ntgt = 0;
-        u64 cur = step + 1u;
+        u64 cur = now_step;
         if (k == 0) cur = ld_ctl(&d.ctl[kStep]);
         if (!held && slot < d.max_slots) {
             nseq = d.slot_seq[slot];
@@ -700,13 +763,20 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
         step = cur;
         bool fin_new = false;
         if (nseq != kEmpty) {
+            bool got_new = false;
+            uint32_t ticks_new = 0;
             seq = nseq;
             len = nlen;
             tgt = ntgt;
-            act = true;
             fseq = seq;
-            grow_slot<S>(d, cl, slot, step, seq, len, tgt, st, fin_new, got, ticks);
-            fin = fin_new;
+            grow_slot<S>(d, cl, slot, step, seq, len, tgt, st, fin_new, got_new, ticks_new);
+            c_alloc += got_new ? 1u : 0u;
+            c_fail += got_new ? 0u : 1u;
+            c_fin += fin_new ? 1u : 0u;
+            if (got_new) {
+                pend = true;
+                pend_ticks = ticks_new;
+            }
         }
         loop_list(L, fin_new, slot, fseq, len);
         __syncthreads();
@@ -718,7 +788,14 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
                 ln = L.len[n_held + e];
             },
             wave, kWaves, st, bad);
-        grow_tally(d, g, st, bad, act, got, fin, ticks, slot, fseq, &acc);
+        // sequences that finished now (at their first page; at k = 0 every
+        // sequence grows here): no atomic when there is none
+        const u64 dp2 = take_positions(&d.ctl[done_word(step)], fin_new ? 1u : 0u, tk[2]);
+        if (fin_new) {
+            uint32_t* dl = done_list(d, step);
+            dl[2u * dp2] = slot;
+            dl[2u * dp2 + 1u] = fseq;
+        }
         // the step's freed pages back to the pool, after admission read it
         if (threadIdx.x == 0) {
             if (L.credit) atom_add_ll(d.c.free_count, L.credit);
@@ -727,10 +804,43 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
         }
         loop_signal(sync, 2, false, 1ull);
     }
+    // the launch's last latency samples, then its counts: summed over the
+    // workgroup, one atomic per control word
     __syncthreads();
-    if (threadIdx.x < 16) {
-        u64* dst = tally_word(d, threadIdx.x);
-        if (dst && acc.v[threadIdx.x]) atom_add_u64(dst, acc.v[threadIdx.x]);
+    if (threadIdx.x < 12u) g.sum[threadIdx.x] = 0;
+    if (threadIdx.x == 0) tk[1].n = 0;
+    __syncthreads();
+    if (lat) {
+        const u64 lp = take_positions(&d.ctl[kLatCount], pend ? 1u : 0u, tk[1]);
+        if (pend && lp < d.lat_cap) d.lat[lp] = pend_ticks;
+    }
+    const unsigned lane = lane_id();
+    uint32_t v[12] = {c_fin, c_alloc, c_fail, st.allocs, st.frees, st.failures, st.steps,
+                      st.probes, st.atomics, st.fallbacks, st.errors, (uint32_t)bad};
+#pragma unroll
+    for (int i = 0; i < 12; ++i) {
+        uint32_t x = v[i];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) x += (uint32_t)__shfl_xor((int)x, o, 64);
+        if (lane == 0 && x) atomicAdd(&g.sum[i], x);
+    }
+    __syncthreads();
+    if (threadIdx.x < 16u) {
+        u64* dst = nullptr;
+        u64 add = 0;
+        switch (threadIdx.x) {
+            case 0: dst = &d.ctl[kFinished]; add = g.sum[0]; break;
+            case 1: dst = &d.ctl[kAllocs]; add = g.sum[1]; break;
+            case 2: dst = &d.ctl[kAllocFail]; add = g.sum[2]; break;
+            case 3: dst = &d.ctl[kFrees]; add = g.sum[4]; break;
+            case 4: dst = &d.ctl[kPayloadErr]; add = g.sum[11]; break;
+            default:
+                if (threadIdx.x >= 8u && d.c.counters) {
+                    dst = (u64*)d.c.counters + (threadIdx.x - 8u);
+                    add = g.sum[3 + (threadIdx.x - 8u)];
+                }
+        }
+        if (dst && add) atom_add_u64(dst, add);
     }
 }
 
@@ -904,7 +1014,7 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
     PBM_SALLOC(slot_target, uint32_t, nl)
     PBM_SALLOC(pages, int32_t, nl * cfg->max_pages)
     PBM_SALLOC(stack, uint32_t, nl)
-    PBM_SALLOC(done, uint32_t, 2u * nl)
+    PBM_SALLOC(done, uint32_t, 4u * nl)
     PBM_SALLOC(ctl, u64, kCtlWords)
     PBM_SALLOC(lat, uint32_t, d.lat_cap)
     PBM_SALLOC(seq_visible, u64, ns)
@@ -1211,7 +1321,7 @@ extern "C" int pbm_serve_read_stats(pbm_server* s, pbm_serve_stats* out, void* s
     out->submitted = ctl[kTail];
     out->admitted = ctl[kAdmitted];
     out->finished = ctl[kFinished];
-    out->active = ctl[kActive] - ctl[kDone];
+    out->active = ctl[kActive] - ctl[(ctl[kStep] & 1u) ? kDoneOdd : kDone];
     out->reserved_pages = ctl[kPromised] - ctl[kAllocs];
     out->allocs = ctl[kAllocs];
     out->frees = ctl[kFrees];
