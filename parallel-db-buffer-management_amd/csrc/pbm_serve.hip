@@ -551,14 +551,18 @@ __global__ __launch_bounds__(256) void k_serve_release(ServeDev d) {
 // (their 64-page chunks dealt over its sixteen waves).
 //
 // A slot workgroup keeps its slots' state in registers from step to step.
-// It grows the slots admitted before this step, and frees the sequences
-// that this finishes, while admission runs: admission writes only empty
-// slots, and the pages were reserved at admission.  The credit of those
-// frees waits in LDS until admission is visible, so admission counts
-// exactly the pages the last step's releases gave back.  Then the
-// workgroup grows the slots admitted now, frees what that finishes, counts
-// the step and credits the pool.  No slot workgroup waits for another's
-// grow; admission waits for every workgroup's step.
+// It grows the slots admitted before this step, lists the sequences that
+// this finishes on the step's done list (admission reads the last step's:
+// one list per step parity) and their latency samples, and frees their
+// pages, all while admission runs: admission writes only empty slots, and
+// the pages were reserved at admission.  The credit of those frees waits
+// in LDS until admission is visible, so admission counts exactly the pages
+// the last step's releases gave back.  Then the workgroup grows the slots
+// admitted now, lists and frees what that finishes (a sequence of one page
+// only), and credits the pool.  A new slot's first latency sample is
+// listed with the next step's; the counts are summed per thread and
+// written once, at the end of the launch.  No slot workgroup waits for
+// another's grow; admission waits for every workgroup's step.
 //
 // The phases follow one another through counters in `sync`: [0] steps
 // admitted, [2] workgroup steps done; [3] counts waits that gave up.  A
