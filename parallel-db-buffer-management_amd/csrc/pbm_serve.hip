@@ -71,8 +71,11 @@ __device__ __forceinline__ u64 ld_ctl(const u64* p) {
 // grow of step s can list its finished sequences while the admission of
 // step s reads those of step s - 1.
 __device__ __forceinline__ uint32_t done_word(u64 s) { return (s & 1u) ? kDoneOdd : kDone; }
+// A done-list entry: slot, sequence, pages held (the release frees them
+// without reading the slot's length).
+constexpr uint32_t kDoneWords = 3;
 __device__ __forceinline__ uint32_t* done_list(const ServeDev& d, u64 s) {
-    return d.done + (size_t)(s & 1u) * 2u * d.max_slots;
+    return d.done + (size_t)(s & 1u) * kDoneWords * d.max_slots;
 }
 
 // Admission (one workgroup).  First the bookkeeping of the last step: the
@@ -118,7 +121,7 @@ __device__ void serve_admit(const ServeDev& d) {
     const u64 active = s_ctl[kActive] - done_n;
     const uint32_t win = (uint32_t)(tail - head < kAdmitWindow ? tail - head : kAdmitWindow);
     for (u64 i = t; i < done_n; i += blockDim.x) {
-        const uint32_t sl = done[2u * i];
+        const uint32_t sl = done[kDoneWords * i];
         d.stack[top_old + i] = sl;
         const u64 j = done_n - 1u - i;  // popped j-th
         if (j < win) s_slot[j] = sl;
@@ -274,7 +277,7 @@ __device__ __forceinline__ u64* tally_word(const ServeDev& d, unsigned lane, u64
 // zero and visible to the workgroup.
 __device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, u64 bad,
                            bool act, bool got, bool fin, uint32_t ticks, uint32_t slot,
-                           uint32_t seq, u64 step) {
+                           uint32_t seq, uint32_t len, u64 step) {
     using namespace pbm;
     const unsigned lane = lane_id();
     const u64 F = ballot(fin);
@@ -328,8 +331,9 @@ __device__ void grow_tally(const ServeDev& d, GrowLds& g, const pbm::Stats& st, 
     if (fin) {
         const u64 k = g.base[0] + wf + rank_in(F);
         uint32_t* done = done_list(d, step);
-        done[2u * k] = slot;
-        done[2u * k + 1u] = seq;
+        done[kDoneWords * k] = slot;
+        done[kDoneWords * k + 1u] = seq;
+        done[kDoneWords * k + 2u] = len;
     }
     if ((d.flags & PBM_SERVE_LATENCY) && got) {
         const u64 k = g.base[1] + wg + rank_in(G);
@@ -361,7 +365,7 @@ __device__ void serve_grow(const ServeDev& d, uint32_t slot) {
     bool fin = false, got = false;
     uint32_t ticks = 0;
     if (act) grow_slot<S>(d, cl, slot, step, seq, len, tgt, st, fin, got, ticks);
-    grow_tally(d, g, st, 0ull, act, got, fin, ticks, slot, fseq, step);
+    grow_tally(d, g, st, 0ull, act, got, fin, ticks, slot, fseq, len, step);
 }
 
 template <int S>
@@ -492,18 +496,38 @@ __device__ void serve_release(const ServeDev& d, uint32_t wave, uint32_t n_waves
     pbm_ctx cl = d.c;
     cl.free_count = &s_credit;
     const unsigned lane = lane_id();
+    // the step, both parities' counts and both done lists' first 64 entries
+    // load together (one round trip); the step's parity picks the list
     const u64 step = ld_ctl(&d.ctl[kStep]);
     const u64 dn[2] = {ld_ctl(&d.ctl[kDone]), ld_ctl(&d.ctl[kDoneOdd])};
+    uint32_t pre[2][kDoneWords] = {};
+    if (lane < d.max_slots) {
+#pragma unroll
+        for (uint32_t p = 0; p < 2; ++p)
+#pragma unroll
+            for (uint32_t w = 0; w < kDoneWords; ++w)
+                pre[p][w] = done_list(d, p)[kDoneWords * lane + w];
+    }
     const uint32_t done_n = (uint32_t)dn[step & 1u];
     const uint32_t* done = done_list(d, step);
+    const uint32_t par = (uint32_t)(step & 1u);
+    const uint32_t p_slot = par ? pre[1][0] : pre[0][0];
+    const uint32_t p_seq = par ? pre[1][1] : pre[0][1];
+    const uint32_t p_len = par ? pre[1][2] : pre[0][2];
     Stats st;
     u64 bad = 0;
     free_finished<S>(
         d, cl, done_n,
         [&](uint32_t e, uint32_t& slot, uint32_t& seq, uint32_t& len) {
-            slot = done[2u * e];
-            seq = done[2u * e + 1u];
-            len = d.slot_len[slot];
+            if (e < 64u) {  // e is the lane's own: its entry is loaded
+                slot = p_slot;
+                seq = p_seq;
+                len = p_len;
+            } else {
+                slot = done[kDoneWords * e];
+                seq = done[kDoneWords * e + 1u];
+                len = done[kDoneWords * e + 2u];
+            }
         },
         wave, n_waves, st, bad);
     // tallies summed over the workgroup, then one atomic per word: [0]
@@ -732,8 +756,9 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
             const u64 dp = take_positions(&d.ctl[done_word(now_step)], fin ? 1u : 0u, tk[0]);
             if (fin) {
                 uint32_t* dl = done_list(d, now_step);
-                dl[2u * dp] = slot;
-                dl[2u * dp + 1u] = fseq;
+                dl[kDoneWords * dp] = slot;
+                dl[kDoneWords * dp + 1u] = fseq;
+                dl[kDoneWords * dp + 2u] = len;
             }
             if (lat) {
                 u64 lp = take_positions(&d.ctl[kLatCount], (pend ? 1u : 0u) + (got ? 1u : 0u),
@@ -797,8 +822,9 @@ __global__ __launch_bounds__(kLoopThreads) void k_serve_loop(ServeDev d, uint32_
         const u64 dp2 = take_positions(&d.ctl[done_word(step)], fin_new ? 1u : 0u, tk[2]);
         if (fin_new) {
             uint32_t* dl = done_list(d, step);
-            dl[2u * dp2] = slot;
-            dl[2u * dp2 + 1u] = fseq;
+            dl[kDoneWords * dp2] = slot;
+            dl[kDoneWords * dp2 + 1u] = fseq;
+            dl[kDoneWords * dp2 + 2u] = len;
         }
         // the step's freed pages back to the pool, after admission read it
         if (threadIdx.x == 0) {
@@ -1018,7 +1044,7 @@ extern "C" int pbm_serve_create(pbm_handle* h, const pbm_serve_config* cfg, pbm_
     PBM_SALLOC(slot_target, uint32_t, nl)
     PBM_SALLOC(pages, int32_t, nl * cfg->max_pages)
     PBM_SALLOC(stack, uint32_t, nl)
-    PBM_SALLOC(done, uint32_t, 4u * nl)
+    PBM_SALLOC(done, uint32_t, 2u * kDoneWords * nl)
     PBM_SALLOC(ctl, u64, kCtlWords)
     PBM_SALLOC(lat, uint32_t, d.lat_cap)
     PBM_SALLOC(seq_visible, u64, ns)
