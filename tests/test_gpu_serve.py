@@ -106,6 +106,43 @@ def test_serve_loop_schedule_matches_oracle(gpu, strategy):
         assert np.array_equal(pool.snapshot(), orc.prefill(T, 0.5, 3))
 
 
+@pytest.mark.parametrize("loop", [False, True])
+def test_serve_many_sequences_finishing_in_one_step(gpu, loop):
+    # 500 one-page sequences finish at the grow of step 1 and about 250
+    # more at step 2: the release walks a done list far past the first 64
+    # entries that each wave loads with the step word
+    T = 1 << 16
+    log_case(f"serve many finishing at once loop={loop}")
+    rng = np.random.default_rng(11)
+    arr = np.repeat(np.array([1, 2, 3], dtype=np.uint32), [500, 500, 500])
+    tgt = np.concatenate([np.ones(500), rng.integers(1, 3, 500),
+                          rng.integers(1, 9, 500)]).astype(np.uint32)
+    n = arr.size
+    with pbm.Pool(T, 64, strategy=pbm.RW_BM, backing=True, seed=5) as pool:
+        pool.prefill_bernoulli(0.5, 3)
+        A = pool.free_count()
+        adm, fin, tot = orc.serve_sim(A, 1024, arr, tgt, 1000)
+        assert (fin > 0).all()
+        assert np.bincount(fin)[1] == 500 and np.bincount(fin)[2] > 64
+        steps = int(fin.max()) + 2
+        with pbm.Server(pool, 1024, 16, n, pbm.SERVE_PAYLOAD, seed=SEED) as srv:
+            srv.submit(arr, tgt)
+            if loop:
+                srv.run_loop(steps)
+            else:
+                srv.run(steps)
+            res = srv.results()
+            st = srv.stats()
+        assert np.array_equal(res["admit_step"], adm)
+        assert np.array_equal(res["finish_step"], fin)
+        _check_results(res, np.arange(n), tgt)
+        assert st["alloc_failures"] == 0 and st["payload_errors"] == 0
+        assert st["allocs"] == tot[0] and st["frees"] == tot[1]
+        assert st["admitted"] == st["finished"] == n and st["active"] == 0
+        assert pool.free_count() == A
+        assert np.array_equal(pool.snapshot(), orc.prefill(T, 0.5, 3))
+
+
 @pytest.mark.parametrize("slots", [3000, 4096, 32768])
 def test_serve_loop_equals_eager_steps(gpu, slots):
     # 3, 4 and 32 loop workgroups (the first with a partial last group of
